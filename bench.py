@@ -1,0 +1,186 @@
+"""Throughput benchmark of the gfx950 parallel-tempering engine (one JSON line on stdout).
+
+Workload (default): 3-D Edwards-Anderson +/-J glass, L = 32 periodic (N = 32768), 64 replicas on a
+geometric beta ladder 0.2 .. 3.0, one sweep then one exchange attempt per step.  A step is one
+sweep of all 64 replicas (64 x 32768 spin-flip attempts) plus the device-side exchange.
+
+Three device runs: a warm-up (--warmup steps), the timed run (--steps steps; device events around
+the sweep + exchange loop, inputs already in HBM), and a short run that times every sweep-kernel
+launch with device events for the roofline figure.  The CPU baseline is the oracle
+(oracle/pt_oracle.c, the same chain) on a bounded sample of the same workload on this host's cores.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def workload(name: str):
+    from drift_amd.builders.lattice import cubic_pmj_glass, er_sparse_weighted, square_ferromagnet
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    if name == "c2":
+        model = cubic_pmj_glass(32, 0)
+        R, beta = 64, ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))
+        desc = ("3-D Edwards-Anderson +/-J glass, L=32 periodic (N=32768), R=64, beta 0.2..3.0 "
+                "geometric, exchange every sweep")
+    elif name == "c1":
+        model = square_ferromagnet(16)
+        R, beta = 16, ladder_betas(geometric_ladder(16, 1.0, 10.0))
+        desc = "2-D ferromagnet, L=16 periodic (N=256), R=16, beta 0.1..1.0, exchange every sweep"
+    elif name == "c3":
+        model = er_sparse_weighted(100000, 6.0, 0)
+        R, beta = 32, ladder_betas(geometric_ladder(32, 0.2, 20.0))
+        desc = ("Erdos-Renyi N=100000 mean degree 6, weights in {+-1,+-2,+-3}, R=32, "
+                "beta 0.05..5.0, exchange every sweep")
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return model, R, beta, desc
+
+
+def algorithmic_bytes_per_attempt(model, enc) -> float:
+    """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse:
+    own spin read + write, d neighbour spins, d (weight + column index), the row pointers and the
+    field.  Spins are int8; positions are relabelled by colour, so no colour list is read."""
+    d = model.nnz / model.n
+    b_s = 1.0
+    b_w = enc.weight.dtype.itemsize
+    b_h = 0.0 if enc.h is None else enc.h.dtype.itemsize
+    return b_s * (d + 1) + b_s + d * (b_w + 4.0) + 8.0 + b_h
+
+
+def cpu_baseline(model, enc, R, beta, budget_s: float):
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    probe = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=2, sweeps_per_round=1,
+                                  beta=beta, n_threads=cores)
+    per_sweep = max(probe["seconds"] / 2, 1e-6)
+    sweeps = int(max(2, min(10000, budget_s / per_sweep)))
+    run = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=sweeps, sweeps_per_round=1,
+                                beta=beta, n_threads=cores)
+    attempts = R * model.n * sweeps
+    return {
+        "value": attempts / run["seconds"] / 1e9,
+        "unit": "attempts/ns",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"same workload, {sweeps} sweeps+exchanges of all {R} replicas "
+                  f"({attempts:.3g} attempts, {run['seconds']:.1f} s, OpenMP over replicas)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--engine", default="auto")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--kernel-timing-steps", type=int, default=200)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this engine has no "
+              "multi-GPU path", file=sys.stderr)
+        return 2
+
+    from drift_amd import gpu
+
+    if not gpu.gpu_available():
+        print("bench.py: GPU engine not available (no gfx950 device or library not built)",
+              file=sys.stderr)
+        return 1
+    model, R, beta, desc = workload(args.workload)
+    enc = gpu.encode_problem(model)
+    kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,
+              block_threads=args.block, record_history=False)
+
+    print(f"[bench] warm-up {args.warmup} steps", file=sys.stderr, flush=True)
+    gpu.parallel_tempering_gpu(model, n_rounds=args.warmup, **kw)
+    print(f"[bench] timed {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    res = gpu.parallel_tempering_gpu(model, n_rounds=args.steps, **kw)
+    wall = time.perf_counter() - t0
+    attempts = R * model.n * args.steps
+    value = attempts / res.seconds / 1e9
+    print(f"[bench] {value:.2f} attempts/ns over {res.seconds:.3f} s (wall {wall:.3f} s), "
+          f"engine {res.engine}", file=sys.stderr, flush=True)
+
+    print("[bench] kernel timing run", file=sys.stderr, flush=True)
+    kt = gpu.parallel_tempering_gpu(model, n_rounds=args.kernel_timing_steps, time_kernels=True, **kw)
+    B = algorithmic_bytes_per_attempt(model, enc)
+    per_launch_s = kt.kernel_seconds / max(kt.kernel_launches, 1)
+    per_launch_attempts = kt.kernel_attempts / max(kt.kernel_launches, 1)
+    achieved = B * per_launch_attempts / per_launch_s / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "g_color_step" if kt.engine == "global" else "pt_lds_kernel",
+        "bytes_per_attempt": round(B, 3),
+        "attempts_per_launch": per_launch_attempts,
+        "avg_launch_us": round(per_launch_s * 1e6, 3),
+    }
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(model, enc, R, beta, args.cpu_budget)
+
+    line = {
+        "metric": "spin-flips/ns on 1 GPU",
+        "value": round(value, 3),
+        "unit": "attempts/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res.seconds * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32" if enc.is_integer else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "format": f"int8 spins [position][replica], CSR int32 columns, "
+                      f"{enc.weight.dtype.name} weights",
+            "engine": res.engine,
+            "n": model.n,
+            "replicas": R,
+            "sweeps_per_step": 1,
+            "attempts_per_step": R * model.n,
+            "wall_s": round(wall, 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

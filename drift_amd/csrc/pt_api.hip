@@ -1,0 +1,690 @@
+// pt_api.hip -- the C ABI of libdriftpt.so (include/drift_pt.h): validation, colouring,
+// relabelling by colour position, threshold tables, device buffers, launches and result copies.
+//
+// Reference counterpart: the host half of cuda/ising_pt.cu (main, :159-243) plus the Python glue
+// that fed it through a file and a subprocess (drift/gpu.py:59-114, CSR :66-77, colouring :122-145).
+#include "../../include/drift_pt.h"
+#include "pt_engine.h"
+#include "pt_rules.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PT_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? DRIFT_PT_ENOMEM : DRIFT_PT_EHIP,          \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Device allocations owned by one call; freed on every exit path.
+struct DevArena {
+    std::vector<void*> ptrs;
+    ~DevArena() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T** out, size_t count) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(p);
+        *out = (T*)p;
+        return e;
+    }
+    template <class T>
+    hipError_t upload(T** out, const std::vector<T>& v) {
+        hipError_t e = alloc(out, v.size());
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+// Device events for per-launch timing, destroyed with the call.
+struct EventPool {
+    std::vector<hipEvent_t> ev;
+    ~EventPool() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t pair(hipEvent_t* a, hipEvent_t* b) {
+        hipError_t e = hipEventCreate(a);
+        if (e != hipSuccess) return e;
+        ev.push_back(*a);
+        e = hipEventCreate(b);
+        if (e != hipSuccess) return e;
+        ev.push_back(*b);
+        return hipSuccess;
+    }
+};
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~StreamGuard() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// One instance prepared on the host: relabelled by colour position, thresholds built.
+struct HostInst {
+    int n = 0, nnz = 0, k = 0, R = 0, Rp = 0, wm = 0, TQ = 1, S = 0, n_rounds = 0, flags = 0;
+    uint64_t seed = 0;
+    double unit = 1.0;
+    std::vector<int32_t> row_ptr, col, color_ptr, orig;
+    std::vector<int8_t> w8;
+    std::vector<int32_t> w32, h32;
+    std::vector<float> wf, hf;
+    bool has_h = false;
+    std::vector<double> beta;
+    std::vector<uint32_t> thr;
+};
+
+int32_t greedy_color(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* cptr,
+                     int32_t* cspins) {
+    std::vector<int32_t> color((size_t)std::max(n, 0), -1);
+    int32_t maxdeg = 0;
+    for (int32_t i = 0; i < n; ++i) maxdeg = std::max(maxdeg, rp[i + 1] - rp[i]);
+    std::vector<int32_t> stamp((size_t)maxdeg + 2, -1);
+    int32_t k = 1;
+    for (int32_t i = 0; i < n; ++i) {
+        for (int32_t t = rp[i]; t < rp[i + 1]; ++t) {
+            const int32_t c = color[ci[t]];
+            if (c >= 0 && c <= maxdeg) stamp[c] = i;
+        }
+        int32_t c = 0;
+        while (stamp[c] == i) ++c;
+        color[i] = c;
+        k = std::max(k, c + 1);
+    }
+    std::fill(cptr, cptr + k + 1, 0);
+    for (int32_t i = 0; i < n; ++i) cptr[color[i] + 1] += 1;
+    for (int32_t c = 0; c < k; ++c) cptr[c + 1] += cptr[c];
+    std::vector<int32_t> fill(cptr, cptr + k);
+    for (int32_t i = 0; i < n; ++i) cspins[fill[color[i]]++] = i;
+    return k;
+}
+
+int padded_slots(int R) {
+    if (R > 64) return ((R + 63) / 64) * 64;
+    int p = 1;
+    while (p < R) p <<= 1;
+    return p;
+}
+
+int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) {
+    if (!P || !prm) return fail(DRIFT_PT_EINVAL, "null problem or params");
+    const int n = P->n, nnz = P->nnz;
+    if (n < 1) return fail(DRIFT_PT_EINVAL, "n must be >= 1");
+    if (nnz < 0 || (nnz > 0 && (!P->col_idx || !P->weight)) || !P->row_ptr)
+        return fail(DRIFT_PT_EINVAL, "missing CSR arrays");
+    if (P->weight_type != DRIFT_PT_W_I8 && P->weight_type != DRIFT_PT_W_I32 &&
+        P->weight_type != DRIFT_PT_W_F32)
+        return fail(DRIFT_PT_EINVAL, "unknown weight_type");
+    const int R = prm->n_replicas;
+    if (R < 1 || R > 1024) return fail(DRIFT_PT_EINVAL, "n_replicas must be in [1, 1024]");
+    if (prm->n_rounds < 0 || prm->sweeps_per_round < 0)
+        return fail(DRIFT_PT_EINVAL, "n_rounds and sweeps_per_round must be >= 0");
+    if ((int64_t)prm->n_rounds * std::max(prm->sweeps_per_round, 1) >= ((int64_t)1 << 32))
+        return fail(DRIFT_PT_ERANGE, "more than 2^32 sweeps");
+    const int32_t* rp = P->row_ptr;
+    const int32_t* ci = P->col_idx;
+    if (rp[0] != 0 || rp[n] != nnz) return fail(DRIFT_PT_EINVAL, "row_ptr must run 0..nnz");
+    for (int i = 0; i < n; ++i) {
+        if (rp[i + 1] < rp[i]) return fail(DRIFT_PT_EINVAL, "row_ptr not monotone");
+        for (int t = rp[i]; t < rp[i + 1]; ++t) {
+            if (ci[t] < 0 || ci[t] >= n) return fail(DRIFT_PT_EINVAL, "col_idx out of range");
+            if (ci[t] == i) return fail(DRIFT_PT_EINVAL, "self coupling on the diagonal");
+        }
+    }
+    const bool isf = P->weight_type == DRIFT_PT_W_F32;
+    auto wval = [&](int t) -> double {
+        if (P->weight_type == DRIFT_PT_W_I8) return (double)((const int8_t*)P->weight)[t];
+        if (P->weight_type == DRIFT_PT_W_I32) return (double)((const int32_t*)P->weight)[t];
+        return (double)((const float*)P->weight)[t];
+    };
+    auto hval = [&](int i) -> double {
+        if (!P->h) return 0.0;
+        return isf ? (double)((const float*)P->h)[i] : (double)((const int32_t*)P->h)[i];
+    };
+    if (!(prm->flags & DRIFT_PT_SKIP_VALIDATION)) {
+        // symmetry: every (i -> j, w) has its (j -> i, w); no duplicate columns within a row
+        std::vector<std::pair<int32_t, double>> row;
+        std::vector<int32_t> sorted_cols(ci, ci + nnz);
+        for (int i = 0; i < n; ++i) std::sort(sorted_cols.begin() + rp[i], sorted_cols.begin() + rp[i + 1]);
+        for (int i = 0; i < n; ++i)
+            for (int t = rp[i]; t + 1 < rp[i + 1]; ++t)
+                if (sorted_cols[t] == sorted_cols[t + 1])
+                    return fail(DRIFT_PT_EINVAL, "duplicate column in a CSR row");
+        for (int i = 0; i < n; ++i)
+            for (int t = rp[i]; t < rp[i + 1]; ++t) {
+                const int j = ci[t];
+                bool found = false;
+                for (int u = rp[j]; u < rp[j + 1]; ++u)
+                    if (ci[u] == i) {
+                        found = wval(u) == wval(t);
+                        break;
+                    }
+                if (!found) return fail(DRIFT_PT_EINVAL, "coupling matrix is not symmetric");
+            }
+    }
+
+    // colouring: given or greedy
+    std::vector<int32_t> cptr, cspins((size_t)n);
+    int k = P->n_colors;
+    if (k <= 0) {
+        cptr.assign((size_t)n + 1, 0);
+        k = greedy_color(n, rp, ci, cptr.data(), cspins.data());
+        cptr.resize((size_t)k + 1);
+    } else {
+        if (!P->color_ptr || !P->color_spins) return fail(DRIFT_PT_ECOLOR, "missing colour arrays");
+        cptr.assign(P->color_ptr, P->color_ptr + k + 1);
+        std::copy(P->color_spins, P->color_spins + n, cspins.begin());
+        if (cptr[0] != 0 || cptr[k] != n) return fail(DRIFT_PT_ECOLOR, "color_ptr must run 0..n");
+        for (int c = 0; c < k; ++c)
+            if (cptr[c + 1] < cptr[c]) return fail(DRIFT_PT_ECOLOR, "color_ptr not monotone");
+    }
+    std::vector<int32_t> pos((size_t)n, -1), colour_of((size_t)n, -1);
+    for (int c = 0; c < k; ++c)
+        for (int p = cptr[c]; p < cptr[c + 1]; ++p) {
+            const int i = cspins[p];
+            if (i < 0 || i >= n || pos[i] >= 0)
+                return fail(DRIFT_PT_ECOLOR, "color_spins is not a permutation of 0..n-1");
+            pos[i] = p;
+            colour_of[i] = c;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int t = rp[i]; t < rp[i + 1]; ++t)
+            if (colour_of[ci[t]] == colour_of[i])
+                return fail(DRIFT_PT_ECOLOR, "two adjacent spins share a colour");
+
+    H = HostInst();
+    H.n = n; H.nnz = nnz; H.k = k; H.R = R; H.Rp = padded_slots(R);
+    H.wm = P->weight_type; H.S = prm->sweeps_per_round; H.n_rounds = prm->n_rounds;
+    H.flags = prm->flags; H.seed = prm->seed; H.has_h = P->h != nullptr;
+    H.color_ptr = cptr;
+    H.orig = cspins;
+    H.row_ptr.assign((size_t)n + 1, 0);
+    H.col.resize((size_t)nnz);
+    for (int p = 0; p < n; ++p) H.row_ptr[p + 1] = H.row_ptr[p] + (rp[cspins[p] + 1] - rp[cspins[p]]);
+    if (P->weight_type == DRIFT_PT_W_I8) H.w8.resize((size_t)nnz);
+    if (P->weight_type == DRIFT_PT_W_I32) H.w32.resize((size_t)nnz);
+    if (isf) H.wf.resize((size_t)nnz);
+    if (H.has_h) {
+        if (isf) H.hf.resize((size_t)n);
+        else H.h32.resize((size_t)n);
+    }
+    double maxfield = 0.0, total_abs = 0.0;
+    for (int p = 0; p < n; ++p) {
+        const int i = cspins[p];
+        int dst = H.row_ptr[p];
+        double field = std::fabs(hval(i));
+        for (int t = rp[i]; t < rp[i + 1]; ++t, ++dst) {   // row order kept: fixed summation order
+            H.col[dst] = pos[ci[t]];
+            if (P->weight_type == DRIFT_PT_W_I8) H.w8[dst] = ((const int8_t*)P->weight)[t];
+            else if (P->weight_type == DRIFT_PT_W_I32) H.w32[dst] = ((const int32_t*)P->weight)[t];
+            else H.wf[dst] = ((const float*)P->weight)[t];
+            field += std::fabs(wval(t));
+        }
+        if (H.has_h) {
+            if (isf) H.hf[p] = ((const float*)P->h)[i];
+            else H.h32[p] = ((const int32_t*)P->h)[i];
+        }
+        maxfield = std::max(maxfield, field);
+        total_abs += field;
+    }
+    if (!std::isfinite(maxfield)) return fail(DRIFT_PT_ERANGE, "non-finite coupling or field");
+    if (isf) {
+        if (maxfield >= 32768.0 || total_abs >= 1.0e8)
+            return fail(DRIFT_PT_ERANGE, "float couplings too large for the 2^-36 fixed-point energy");
+        H.unit = PT_FIX36_UNIT;
+    } else {
+        if (maxfield >= (double)(1 << 30)) return fail(DRIFT_PT_ERANGE, "local field exceeds 2^30");
+        if (!(P->energy_scale > 0.0) || !std::isfinite(P->energy_scale))
+            return fail(DRIFT_PT_EINVAL, "energy_scale must be positive");
+        H.unit = P->energy_scale;
+    }
+
+    // ladder (drift/solvers/parallel_tempering.py:29-37, cuda/ising_pt.cu:174-178)
+    H.beta.resize((size_t)R);
+    if (prm->beta) {
+        for (int r = 0; r < R; ++r) {
+            if (!(prm->beta[r] > 0.0) || !std::isfinite(prm->beta[r]))
+                return fail(DRIFT_PT_EINVAL, "beta entries must be positive and finite");
+            H.beta[r] = prm->beta[r];
+        }
+    } else {
+        if (!(prm->T_min > 0.0) || !(prm->T_max > 0.0))
+            return fail(DRIFT_PT_EINVAL, "T_min and T_max must be positive");
+        for (int r = 0; r < R; ++r) {
+            const double T = R == 1 ? prm->T_min
+                                    : prm->T_min * std::pow(prm->T_max / prm->T_min,
+                                                            (double)r / (double)(R - 1));
+            H.beta[r] = 1.0 / std::max(T, 1e-12);
+        }
+    }
+
+    // integer thresholds [rung][q-1]; moves with q > TQ are rejected at every rung
+    if (!isf) {
+        const int qmax = std::max(1, (int)maxfield);
+        double bmin = H.beta[0];
+        for (double b : H.beta) bmin = std::min(bmin, b);
+        int tq = 1;
+        while (tq < qmax && pt_threshold_int(bmin * H.unit, tq + 1) != 0u) ++tq;
+        if ((int64_t)tq * R > ((int64_t)1 << 24))
+            return fail(DRIFT_PT_ERANGE, "threshold table too large (tiny beta with huge couplings)");
+        H.TQ = tq;
+        H.thr.resize((size_t)R * tq);
+        for (int r = 0; r < R; ++r)
+            for (int q = 1; q <= tq; ++q) H.thr[(size_t)r * tq + q - 1] = pt_threshold_int(H.beta[r] * H.unit, q);
+    }
+    return 0;
+}
+
+int device_lds_limit() {
+    static int cached = -1;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cached < 0) {
+        hipDeviceProp_t prop;
+        int dev = 0;
+        cached = 65536;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cached = (int)prop.sharedMemPerBlock;
+    }
+    return cached;
+}
+
+int check_device() {
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) {
+        (void)hipGetLastError();
+        return fail(DRIFT_PT_ENODEV, "no HIP device visible (GPU engine not built for this host)");
+    }
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(DRIFT_PT_ENODEV, "cannot query the HIP device");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(DRIFT_PT_ENODEV, std::string("device is ") + prop.gcnArchName + ", engine built for gfx950");
+    return 0;
+}
+
+// Device copies of one instance's arrays, and the PtInstance pointing at them.
+int upload_instance(DevArena& A, const HostInst& H, PtInstance& I, bool want_final, bool want_hist) {
+    int32_t *d_rp, *d_col, *d_cptr, *d_orig;
+    PT_HIP(A.upload(&d_rp, H.row_ptr));
+    PT_HIP(A.upload(&d_col, H.col));
+    PT_HIP(A.upload(&d_cptr, H.color_ptr));
+    PT_HIP(A.upload(&d_orig, H.orig));
+    const void* d_w = nullptr;
+    const void* d_h = nullptr;
+    if (H.wm == PT_WM_I8) { int8_t* x; PT_HIP(A.upload(&x, H.w8)); d_w = x; }
+    if (H.wm == PT_WM_I32) { int32_t* x; PT_HIP(A.upload(&x, H.w32)); d_w = x; }
+    if (H.wm == PT_WM_F32) { float* x; PT_HIP(A.upload(&x, H.wf)); d_w = x; }
+    if (H.has_h) {
+        if (H.wm == PT_WM_F32) { float* x; PT_HIP(A.upload(&x, H.hf)); d_h = x; }
+        else { int32_t* x; PT_HIP(A.upload(&x, H.h32)); d_h = x; }
+    }
+    double* d_beta;
+    PT_HIP(A.upload(&d_beta, H.beta));
+    uint32_t* d_thr = nullptr;
+    if (H.wm != PT_WM_F32) PT_HIP(A.upload(&d_thr, H.thr));
+    int8_t* d_best;
+    PT_HIP(A.alloc(&d_best, (size_t)H.n));
+    int8_t* d_final = nullptr;
+    if (want_final) PT_HIP(A.alloc(&d_final, (size_t)H.n * H.R));
+    double* d_hist = nullptr;
+    if (want_hist && H.n_rounds > 0) PT_HIP(A.alloc(&d_hist, (size_t)H.n_rounds));
+    double* d_trace = nullptr;
+    if ((H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
+        PT_HIP(A.alloc(&d_trace, (size_t)H.n_rounds * H.R));
+    int64_t* d_out;
+    PT_HIP(A.alloc(&d_out, (size_t)pt_out_len(H.R)));
+    PT_HIP(hipMemset(d_out, 0, sizeof(int64_t) * (size_t)pt_out_len(H.R)));
+    I = PtInstance{};
+    I.row_ptr = d_rp; I.col = d_col; I.w = d_w; I.h = d_h; I.color_ptr = d_cptr; I.orig = d_orig;
+    I.beta = d_beta; I.thr = d_thr; I.best_s = d_best; I.final_s = d_final; I.hist = d_hist; I.trace = d_trace;
+    I.out = d_out; I.unit = H.unit;
+    I.k0 = (uint32_t)H.seed; I.k1 = (uint32_t)(H.seed >> 32);
+    I.n = H.n; I.k = H.k; I.R = H.R; I.Rp = H.Rp; I.n_rounds = H.n_rounds; I.S = H.S;
+    I.TQ = H.TQ; I.wm = H.wm;
+    return 0;
+}
+
+void clear_result(drift_pt_result* r) {
+    r->best_E = 0.0; r->best_E_units = 0; r->best_round = 0; r->best_rung = 0; r->best_slot = 0;
+    r->engine_used = 0; r->seconds = 0.0; r->attempts = 0.0;
+    r->kernel_seconds = 0.0; r->kernel_attempts = 0.0; r->kernel_launches = 0;
+}
+
+// Copy results of one instance back and undo the colour relabelling.
+int fetch_results(const HostInst& H, const PtInstance& I, const std::vector<int64_t>& out,
+                  const std::vector<int8_t>& final_pos_major, drift_pt_result* res) {
+    const int n = H.n, R = H.R;
+    std::vector<int8_t> best((size_t)n);
+    PT_HIP(hipMemcpy(best.data(), I.best_s, (size_t)n, hipMemcpyDeviceToHost));
+    for (int p = 0; p < n; ++p) res->best_s[H.orig[p]] = best[p];
+    res->best_E_units = out[0];
+    res->best_E = (double)out[0] * H.unit;
+    res->best_round = (int32_t)out[1];
+    res->best_rung = (int32_t)out[2];
+    res->best_slot = (int32_t)out[3];
+    int64_t* o = const_cast<int64_t*>(out.data());
+    for (int r = 0; r < R; ++r) {
+        if (r + 1 < R) {
+            if (res->swap_tried) res->swap_tried[r] = pt_out_arr(o, R, PT_OUT_TRIED)[r];
+            if (res->swap_accepted) res->swap_accepted[r] = pt_out_arr(o, R, PT_OUT_ACC)[r];
+        }
+        if (res->round_trips) res->round_trips[r] = (int32_t)pt_out_arr(o, R, PT_OUT_TRIPS)[r];
+        if (res->rung_up) res->rung_up[r] = pt_out_arr(o, R, PT_OUT_UP)[r];
+        if (res->rung_down) res->rung_down[r] = pt_out_arr(o, R, PT_OUT_DOWN)[r];
+        if (res->rung_of) res->rung_of[r] = (int32_t)pt_out_arr(o, R, PT_OUT_RUNG)[r];
+        if (res->final_E_units) res->final_E_units[r] = pt_out_arr(o, R, PT_OUT_E)[r];
+    }
+    if (res->final_spins && (H.flags & DRIFT_PT_RECORD_FINAL)) {
+        if (!final_pos_major.empty()) {
+            for (int a = 0; a < R; ++a)
+                for (int p = 0; p < n; ++p)
+                    res->final_spins[(size_t)a * n + H.orig[p]] = final_pos_major[(size_t)p * H.Rp + a];
+        } else {
+            std::vector<int8_t> fs((size_t)n * R);
+            PT_HIP(hipMemcpy(fs.data(), I.final_s, fs.size(), hipMemcpyDeviceToHost));
+            for (int a = 0; a < R; ++a)
+                for (int p = 0; p < n; ++p)
+                    res->final_spins[(size_t)a * n + H.orig[p]] = fs[(size_t)a * n + p];
+        }
+    }
+    if (res->best_history && (H.flags & DRIFT_PT_RECORD_HISTORY) && H.n_rounds > 0)
+        PT_HIP(hipMemcpy(res->best_history, I.hist, sizeof(double) * (size_t)H.n_rounds,
+                         hipMemcpyDeviceToHost));
+    if (res->rung_energy_trace && (H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
+        PT_HIP(hipMemcpy(res->rung_energy_trace, I.trace,
+                         sizeof(double) * (size_t)H.n_rounds * R, hipMemcpyDeviceToHost));
+    res->attempts = (double)R * (double)H.n_rounds * (double)H.S * (double)n;
+    return 0;
+}
+
+int choose_lds_block(int Rp, int requested) {
+    int b = requested > 0 ? requested : 256;
+    b = std::max(b, Rp);
+    b = ((b + Rp - 1) / Rp) * Rp;
+    b = ((b + 63) / 64) * 64;
+    if (b % Rp) b = Rp;   // Rp > 64 is a multiple of 64 already
+    return std::min(b, 1024);
+}
+
+int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& results,
+            int block_req) {
+    DevArena A;
+    const int n_inst = (int)hs.size();
+    std::vector<PtInstance> insts((size_t)n_inst);
+    size_t lds = 0;
+    int maxRp = 1;
+    for (int i = 0; i < n_inst; ++i) {
+        const HostInst& H = hs[i];
+        int rc = upload_instance(A, H, insts[i], H.flags & DRIFT_PT_RECORD_FINAL,
+                                 H.flags & DRIFT_PT_RECORD_HISTORY);
+        if (rc) return rc;
+        lds = std::max(lds, pt_lds_bytes(H.n, H.Rp));
+        maxRp = std::max(maxRp, H.Rp);
+    }
+    if (lds + 1024 > (size_t)device_lds_limit())
+        return fail(DRIFT_PT_EINVAL, "instance does not fit one workgroup's LDS");
+    // largest work first (longest-processing-time order for the persistent queue)
+    std::vector<int32_t> order((size_t)n_inst);
+    std::iota(order.begin(), order.end(), 0);
+    auto work = [&](int i) {
+        const HostInst& H = hs[i];
+        return (double)H.R * H.n_rounds * std::max(H.S, 1) * (H.n + H.nnz);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    PtInstance* d_insts;
+    int32_t *d_order, *d_queue;
+    PT_HIP(A.upload(&d_insts, insts));
+    PT_HIP(A.upload(&d_order, order));
+    PT_HIP(A.alloc(&d_queue, 1));
+    const int block = choose_lds_block(maxRp, block_req);
+    for (const HostInst& H : hs)
+        if (block % H.Rp) return fail(DRIFT_PT_EINVAL, "replica counts in one batch need a common block size");
+    const int grid = std::min(n_inst, 256 * 8);
+    StreamGuard sg;
+    PT_HIP(hipStreamCreate(&sg.s));
+    PT_HIP(hipEventCreate(&sg.e0));
+    PT_HIP(hipEventCreate(&sg.e1));
+    PT_HIP(hipMemsetAsync(d_queue, 0, sizeof(int32_t), sg.s));
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, hs[0].wm, block, grid, lds, sg.s));
+    PT_HIP(hipEventRecord(sg.e1, sg.s));
+    PT_HIP(hipStreamSynchronize(sg.s));
+    float ms = 0.0f;
+    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    for (int i = 0; i < n_inst; ++i) {
+        const HostInst& H = hs[i];
+        std::vector<int64_t> out((size_t)pt_out_len(H.R));
+        PT_HIP(hipMemcpy(out.data(), insts[i].out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        int rc = fetch_results(H, insts[i], out, {}, results[i]);
+        if (rc) return rc;
+        results[i]->seconds = ms * 1e-3;
+        results[i]->engine_used = DRIFT_PT_ENGINE_LDS;
+        results[i]->kernel_seconds = ms * 1e-3;      // one persistent launch does everything
+        results[i]->kernel_launches = 1;
+        double total = 0.0;
+        for (const HostInst& Hx : hs) total += (double)Hx.R * Hx.n_rounds * Hx.S * Hx.n;
+        results[i]->kernel_attempts = total;
+    }
+    return 0;
+}
+
+int run_global(const HostInst& H, drift_pt_result* res, int block_req) {
+    DevArena A;
+    PtInstance I;
+    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
+    if (rc) return rc;
+    const int Rp = H.Rp;
+    GState G;
+    PT_HIP(A.alloc(&G.sp, (size_t)H.n * Rp));
+    PT_HIP(A.alloc(&G.Eb, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.dacc, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.rung, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.rep, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.label, (size_t)Rp));
+    PT_HIP(A.alloc(&G.best, 10));
+    StreamGuard sg;
+    PT_HIP(hipStreamCreate(&sg.s));
+    PT_HIP(hipEventCreate(&sg.e0));
+    PT_HIP(hipEventCreate(&sg.e1));
+    PT_HIP(hipMemsetAsync(G.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.dacc, 0, 2 * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.label, 0, sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.best, 0, 10 * sizeof(int64_t), sg.s));
+
+    int block = block_req > 0 ? block_req : 256;
+    block = std::max(block, Rp);
+    block = std::min(((block + Rp - 1) / Rp) * Rp, 1024);
+    int maxc = 0;
+    for (int c = 0; c < H.k; ++c) maxc = std::max(maxc, H.color_ptr[c + 1] - H.color_ptr[c]);
+    const int64_t threads = (int64_t)((maxc + 3) / 4 + 1) * Rp;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((threads + block - 1) / block, 2048));
+    const int reg = std::max(1, std::min(64, (H.n + 4095) / 4096));
+    PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));
+    // optional per-launch timing of the colour-step kernel (the dominant kernel)
+    const bool time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;
+    EventPool pool;
+    const int64_t max_timed = 4096;
+    int64_t timed = 0;
+    double timed_attempts = 0.0;
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    for (int rnd = 0; rnd < H.n_rounds; ++rnd) {
+        for (int j = 0; j < H.S; ++j) {
+            const uint32_t t = (uint32_t)(rnd * H.S + j);
+            for (int c = 0; c < H.k; ++c) {
+                const bool tk = time_k && timed < max_timed;
+                hipEvent_t a = nullptr, b = nullptr;
+                if (tk) {
+                    PT_HIP(pool.pair(&a, &b));
+                    PT_HIP(hipEventRecord(a, sg.s));
+                }
+                PT_HIP(pt_launch_g_color(I, G, c, t, rnd & 1, block, grid, sg.s));
+                if (tk) {
+                    PT_HIP(hipEventRecord(b, sg.s));
+                    ++timed;
+                    timed_attempts += (double)H.R * (H.color_ptr[c + 1] - H.color_ptr[c]);
+                }
+            }
+        }
+        PT_HIP(pt_launch_g_round_end(I, G, rnd, reg, sg.s));
+    }
+    PT_HIP(hipEventRecord(sg.e1, sg.s));
+    PT_HIP(hipStreamSynchronize(sg.s));
+    float ms = 0.0f;
+    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    double kernel_s = 0.0;
+    for (size_t x = 0; x < pool.ev.size(); x += 2) {
+        float km = 0.0f;
+        PT_HIP(hipEventElapsedTime(&km, pool.ev[x], pool.ev[x + 1]));
+        kernel_s += km * 1e-3;
+    }
+
+    // final state lives in the buffers of parity n_rounds & 1
+    const int fin = H.n_rounds & 1;
+    std::vector<int64_t> out((size_t)pt_out_len(H.R));
+    PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t best[5];
+    PT_HIP(hipMemcpy(best, G.best + 5 * fin, sizeof(best), hipMemcpyDeviceToHost));
+    std::vector<int64_t> E((size_t)Rp);
+    std::vector<int32_t> rung((size_t)Rp);
+    PT_HIP(hipMemcpy(E.data(), G.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(rung.data(), G.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
+    for (int x = 0; x < 5; ++x) out[x] = best[x];
+    for (int r = 0; r < H.R; ++r) {
+        pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
+        pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
+    }
+    std::vector<int8_t> fs;
+    if (H.flags & DRIFT_PT_RECORD_FINAL) {
+        fs.resize((size_t)H.n * Rp);
+        PT_HIP(hipMemcpy(fs.data(), G.sp, fs.size(), hipMemcpyDeviceToHost));
+    }
+    rc = fetch_results(H, I, out, fs, res);
+    if (rc) return rc;
+    res->seconds = ms * 1e-3;
+    res->engine_used = DRIFT_PT_ENGINE_GLOBAL;
+    if (time_k) {
+        res->kernel_seconds = kernel_s;
+        res->kernel_attempts = timed_attempts;
+        res->kernel_launches = timed;
+    } else {
+        res->kernel_seconds = res->seconds;
+        res->kernel_attempts = res->attempts;
+        res->kernel_launches = 1;
+    }
+    return 0;
+}
+
+// LDS engine fit: the dynamic area plus ~1 KB of the kernel's static shared variables.
+bool fits_lds(const HostInst& H) {
+    return pt_lds_bytes(H.n, H.Rp) + 1024 <= (size_t)device_lds_limit();
+}
+
+}  // namespace
+
+extern "C" {
+
+int drift_pt_abi_version(void) { return DRIFT_PT_ABI_VERSION; }
+
+const char* drift_pt_last_error(void) { return g_err.c_str(); }
+
+int drift_pt_device_available(void) {
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int drift_pt_greedy_coloring(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                             int32_t* color_ptr, int32_t* color_spins) {
+    if (n < 0 || !row_ptr || !color_ptr || (n > 0 && (!color_spins || !col_idx)))
+        return fail(DRIFT_PT_EINVAL, "bad arguments to drift_pt_greedy_coloring");
+    if (n == 0) {
+        color_ptr[0] = 0;
+        color_ptr[1] = 0;
+        return 1;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int t = row_ptr[i]; t < row_ptr[i + 1]; ++t)
+            if (col_idx[t] < 0 || col_idx[t] >= n) return fail(DRIFT_PT_EINVAL, "col_idx out of range");
+    return greedy_color(n, row_ptr, col_idx, color_ptr, color_spins);
+}
+
+int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
+                 drift_pt_result* result) {
+    g_err.clear();
+    if (!result || !result->best_s) return fail(DRIFT_PT_EINVAL, "result.best_s is required");
+    clear_result(result);
+    HostInst H;
+    int rc = prepare(problem, params, H);
+    if (rc) return rc;
+    rc = check_device();
+    if (rc) return rc;
+    int engine = params->engine;
+    if (engine == DRIFT_PT_ENGINE_AUTO)
+        engine = (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) ? DRIFT_PT_ENGINE_LDS
+                                                                  : DRIFT_PT_ENGINE_GLOBAL;
+    if (engine == DRIFT_PT_ENGINE_LDS) {
+        if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "instance does not fit one workgroup's LDS");
+        std::vector<HostInst> hs{std::move(H)};
+        return run_lds(hs, {result}, params->block_threads);
+    }
+    if (engine != DRIFT_PT_ENGINE_GLOBAL) return fail(DRIFT_PT_EINVAL, "unknown engine");
+    return run_global(H, result, params->block_threads);
+}
+
+int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
+                       int32_t n_instances, drift_pt_result* results) {
+    g_err.clear();
+    if (n_instances < 1 || !problems || !params || !results)
+        return fail(DRIFT_PT_EINVAL, "bad batch arguments");
+    std::vector<HostInst> hs((size_t)n_instances);
+    std::vector<drift_pt_result*> rs((size_t)n_instances);
+    for (int i = 0; i < n_instances; ++i) {
+        if (!results[i].best_s) return fail(DRIFT_PT_EINVAL, "result.best_s is required");
+        clear_result(&results[i]);
+        int rc = prepare(&problems[i], &params[i], hs[i]);
+        if (rc) return fail(rc, "instance " + std::to_string(i) + ": " + g_err);
+        if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "all instances of a batch need one weight_type");
+        rs[i] = &results[i];
+    }
+    int rc = check_device();
+    if (rc) return rc;
+    for (const HostInst& H : hs)
+        if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "batched instances must fit one workgroup's LDS");
+    return run_lds(hs, rs, 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// pt_engine.h -- device-side instance descriptors and the launchers of the gfx950 kernels.
+// Internal to libdriftpt.so; the public surface is include/drift_pt.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Coupling modes (match drift_pt_weight_type).
+enum : int { PT_WM_I8 = 1, PT_WM_I32 = 2, PT_WM_F32 = 3 };
+
+// Per-instance statistics block in device memory (int64 entries):
+//   [0] best units  [1] best round  [2] best rung  [3] best slot  [4] have-best flag
+//   then R-long arrays: E by slot, rung by slot, tried by rung, accepted by rung,
+//   round trips by slot, up-visits by rung, down-visits by rung.
+#define PT_OUT_HDR 5
+enum : int { PT_OUT_E = 0, PT_OUT_RUNG, PT_OUT_TRIED, PT_OUT_ACC, PT_OUT_TRIPS, PT_OUT_UP, PT_OUT_DOWN, PT_OUT_NARR };
+__host__ __device__ inline int64_t* pt_out_arr(int64_t* out, int R, int which) {
+    return out + PT_OUT_HDR + (int64_t)which * R;
+}
+inline int64_t pt_out_len(int R) { return PT_OUT_HDR + (int64_t)PT_OUT_NARR * R; }
+
+// One instance as the kernels see it.  Spins are relabelled by colour-list position p (colour c
+// owns positions color_ptr[c] .. color_ptr[c+1]-1); CSR rows and columns use positions; orig[p]
+// is the caller's spin index.  Device pointers throughout.
+struct PtInstance {
+    const int32_t* row_ptr;   // n + 1
+    const int32_t* col;       // nnz (positions)
+    const void* w;            // nnz: int8 / int32 / float
+    const void* h;            // n (position order) int32 / float, or null
+    const int32_t* color_ptr; // k + 1
+    const int32_t* orig;      // n
+    const double* beta;       // R, rung 0 coldest
+    const uint32_t* thr;      // R * TQ integer-mode thresholds
+    int8_t* best_s;           // n, position order
+    int8_t* final_s;          // R * n slot-major, position order, or null
+    double* hist;             // n_rounds or null
+    double* trace;            // n_rounds * R energies by rung, or null
+    int64_t* out;             // statistics block
+    double unit;              // energy per unit
+    uint32_t k0, k1;          // Philox key (seed)
+    int32_t n, k, R, Rp;      // Rp: replica slots padded (power of two <= 64, else multiple of 64)
+    int32_t n_rounds, S, TQ, wm;
+};
+
+// Global-engine state (replica-interleaved spins in HBM).  Two-deep buffers are indexed by
+// round parity so the many workgroups of one launch only ever read what another launch wrote.
+struct GState {
+    int8_t* sp;      // n * Rp, [position][slot]
+    int64_t* Eb;     // 2 * Rp: energy at the start of the round
+    int64_t* dacc;   // 2 * Rp: energy change accumulated during the round
+    int32_t* rung;   // 2 * Rp: slot -> rung
+    int32_t* rep;    // 2 * Rp: rung -> slot
+    int32_t* label;  // Rp: round-trip direction label (+1 last at cold end, -1 at hot end)
+    int64_t* best;   // 2 * 5: E, round, rung, slot, have
+};
+
+size_t pt_lds_bytes(int n, int Rp);
+
+hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int n_inst,
+                         int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
+                         hipStream_t s);
+
+hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
+                            hipStream_t s);
+hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, uint32_t t, int cur,
+                             int block, int grid, hipStream_t s);
+hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rnd,
+                                 int round_end_grid, hipStream_t s);

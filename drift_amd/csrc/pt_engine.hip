@@ -1,0 +1,549 @@
+// pt_engine.hip -- gfx950 kernels of the parallel-tempering engine.
+//
+// Two engines share one Markov chain (so their results are bit-identical):
+//
+//  * LDS engine (pt_lds_kernel): one workgroup owns a whole instance -- every replica's int8 spins
+//    live in LDS as [position][slot], so neighbour gathers for the slots of one position are
+//    contiguous bytes.  The workgroup runs all rounds itself: colour steps are separated by
+//    s_barrier, and the exchange, best tracking and round-trip bookkeeping happen in LDS with no
+//    host round trip.  A global work counter hands instances (largest first) to persistent
+//    workgroups, which is how thousands of uneven instances run in one launch.
+//
+//  * Global engine (g_* kernels): for instances whose replicas do not fit one workgroup's LDS.
+//    Spins live in HBM as [position][slot] (a wave reads the 64 slots of a neighbour as one
+//    64-byte row); each launch updates one colour class of all replicas across the whole chip.
+//    A small round-end launch reduces the energy changes, tracks the best state, copies it, and
+//    makes the exchange decisions on the device.
+//
+// Reference counterpart: cuda/ising_pt.cu:70-146 (init_kernel, sweep_kernel, swap_kernel), which
+// used one warp per replica, float spins in global memory, fp64 local fields and cuRAND.
+#include "pt_engine.h"
+#include "pt_rules.h"
+
+#include <algorithm>
+#include <climits>
+
+#define PT_WAVE 64
+
+namespace {
+
+__device__ __forceinline__ void lex_min(long long& e, int& r, long long e2, int r2) {
+    if (e2 < e || (e2 == e && r2 < r)) {
+        e = e2;
+        r = r2;
+    }
+}
+
+// Block-wide lexicographic minimum of (E[rep[r]], r) over rungs r < R.  Every thread calls it and
+// receives the result; this is the "scan rungs in order, strict <" rule of the reference's best
+// tracking (drift/solvers/parallel_tempering.py:99-102) evaluated in parallel.
+__device__ void block_best_rung(const int64_t* E, const int32_t* rep, int R, long long* scrE,
+                                int* scrR, int64_t& outE, int32_t& outR) {
+    long long e = LLONG_MAX;
+    int r = INT_MAX;
+    for (int x = threadIdx.x; x < R; x += blockDim.x) lex_min(e, r, (long long)E[rep[x]], x);
+#pragma unroll
+    for (int m = PT_WAVE / 2; m > 0; m >>= 1) {
+        const long long e2 = __shfl_xor(e, m);
+        const int r2 = __shfl_xor(r, m);
+        lex_min(e, r, e2, r2);
+    }
+    const int wid = threadIdx.x / PT_WAVE, nw = (blockDim.x + PT_WAVE - 1) / PT_WAVE;
+    if ((threadIdx.x & (PT_WAVE - 1)) == 0) {
+        scrE[wid] = e;
+        scrR[wid] = r;
+    }
+    __syncthreads();
+    e = scrE[0];
+    r = scrR[0];
+    for (int w = 1; w < nw; ++w) lex_min(e, r, scrE[w], scrR[w]);
+    outE = e;
+    outR = r;
+    __syncthreads();
+}
+
+// One Metropolis attempt at position p of slot a; returns the accepted energy change in units
+// (0 if rejected).  sp is [position][slot] with Rp slots per position (LDS or HBM).
+template <int WM>
+__device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
+                                               const int32_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col, const void* w,
+                                               const void* h, uint32_t u,
+                                               const uint32_t* thr_row, int TQ, float beta_f) {
+    const int t0 = rp[p], t1 = rp[p + 1];
+    const int8_t si = sp[(size_t)p * Rp + a];
+    if constexpr (WM == PT_WM_F32) {
+        const float* wf = (const float*)w;
+        float l = h ? ((const float*)h)[p] : 0.0f;
+        for (int t = t0; t < t1; ++t) {
+            const float wt = wf[t];
+            l = sp[(size_t)col[t] * Rp + a] > 0 ? l + wt : l - wt;
+        }
+        const float q = si > 0 ? l : -l;
+        if (pt_accept_float(beta_f, q, u)) {
+            sp[(size_t)p * Rp + a] = (int8_t)-si;
+            return pt_fix36(2.0f * q);
+        }
+        return 0;
+    } else {
+        int32_t l = h ? ((const int32_t*)h)[p] : 0;
+        for (int t = t0; t < t1; ++t) {
+            const int32_t wt = WM == PT_WM_I8 ? (int32_t)((const int8_t*)w)[t]
+                                              : ((const int32_t*)w)[t];
+            l += sp[(size_t)col[t] * Rp + a] > 0 ? wt : -wt;
+        }
+        const int32_t q = si > 0 ? l : -l;
+        const bool acc = q <= 0 || (q <= TQ && u < thr_row[q - 1]);
+        if (acc) {
+            sp[(size_t)p * Rp + a] = (int8_t)-si;
+            return 2 * (int64_t)q;
+        }
+        return 0;
+    }
+}
+
+// Energy contribution of position p in slot a at initialisation.  Integer modes return
+// s_p (sum_j J s_j + 2 h_p) (E = -total/2); float returns fix36(-s_p (l_J/2 + h_p)).
+template <int WM>
+__device__ __forceinline__ int64_t site_energy(int p, int a, int Rp, const int8_t* sp,
+                                               const int32_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col, const void* w,
+                                               const void* h) {
+    const int t0 = rp[p], t1 = rp[p + 1];
+    const int8_t si = sp[(size_t)p * Rp + a];
+    if constexpr (WM == PT_WM_F32) {
+        const float* wf = (const float*)w;
+        float lj = 0.0f;
+        for (int t = t0; t < t1; ++t) {
+            const float wt = wf[t];
+            lj = sp[(size_t)col[t] * Rp + a] > 0 ? lj + wt : lj - wt;
+        }
+        const float v = 0.5f * lj + (h ? ((const float*)h)[p] : 0.0f);
+        return pt_fix36(si > 0 ? -v : v);
+    } else {
+        int64_t lj = 0;
+        for (int t = t0; t < t1; ++t) {
+            const int32_t wt = WM == PT_WM_I8 ? (int32_t)((const int8_t*)w)[t]
+                                              : ((const int32_t*)w)[t];
+            lj += sp[(size_t)col[t] * Rp + a] > 0 ? wt : -wt;
+        }
+        const int64_t hh = h ? (int64_t)((const int32_t*)h)[p] : 0;
+        const int64_t v = lj + 2 * hh;
+        return si > 0 ? v : -v;
+    }
+}
+
+__device__ __forceinline__ int8_t init_spin(const PtInstance& I, int p, int a) {
+    const int i = I.orig[p];
+    const pt_u4 r = pt_philox((uint32_t)(i >> 2), (uint32_t)a, 0u, PT_TAG_INIT, I.k0, I.k1);
+    return (pt_word(r, i & 3) & 1u) ? (int8_t)1 : (int8_t)-1;
+}
+
+// Exchange of one round for the pairs (r, r+1), r = rnd%2 + 2m (parallel_tempering.py:104-113),
+// with the permutation rung<->slot instead of moving configurations.  Every pair is disjoint, so
+// each thread owns one pair.  Caller syncs before and after.
+__device__ __forceinline__ void exchange_pairs(const PtInstance& I, int rnd, const int64_t* E,
+                                               int32_t* rep, int32_t* rung, int64_t* tried,
+                                               int64_t* accd) {
+    const int R = I.R;
+    for (int m = threadIdx.x;; m += blockDim.x) {
+        const int r = (rnd & 1) + 2 * m;
+        if (r + 1 >= R) break;
+        const int a = rep[r], b = rep[r + 1];
+        const pt_u4 rv = pt_philox((uint32_t)(r >> 2), 0u, (uint32_t)rnd, PT_TAG_EXCH, I.k0, I.k1);
+        const bool ok = pt_accept_exchange(I.beta[r], I.beta[r + 1], (double)E[a] * I.unit,
+                                           (double)E[b] * I.unit, pt_word(rv, r & 3));
+        tried[r] += 1;
+        if (ok) {
+            accd[r] += 1;
+            rep[r] = b;
+            rep[r + 1] = a;
+            rung[a] = r + 1;
+            rung[b] = r;
+        }
+    }
+}
+
+// Round-trip labels after an exchange: a slot reaching the coldest rung with a "hot" label
+// completes a round trip.  rung is a permutation, so per-rung counters have one writer each.
+__device__ __forceinline__ void update_labels(int R, const int32_t* rung, int32_t* label,
+                                              int32_t* trips32, int64_t* trips64, int64_t* up,
+                                              int64_t* down) {
+    if (R < 2) return;
+    for (int a = threadIdx.x; a < R; a += blockDim.x) {
+        const int g = rung[a];
+        int lb = label[a];
+        if (g == 0) {
+            if (lb == -1) {
+                if (trips32) trips32[a] += 1;
+                if (trips64) trips64[a] += 1;
+            }
+            lb = 1;
+        } else if (g == R - 1) {
+            lb = -1;
+        }
+        label[a] = lb;
+        if (lb == 1) up[g] += 1;
+        if (lb == -1) down[g] += 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// LDS engine
+// ------------------------------------------------------------------------------------------
+struct LShared {
+    long long bestE;
+    int have, bround, brung, bslot, improved, inst;
+};
+
+template <int WM>
+__device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long long* scrE,
+                             int* scrR) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int n = I.n, R = I.R, Rp = I.Rp, k = I.k;
+    size_t off = ((size_t)n * Rp + 15) & ~(size_t)15;
+    int8_t* sp = (int8_t*)smem;
+    int64_t* E = (int64_t*)(smem + off);      off += 8 * (size_t)Rp;
+    int64_t* tried = (int64_t*)(smem + off);  off += 8 * (size_t)Rp;
+    int64_t* accd = (int64_t*)(smem + off);   off += 8 * (size_t)Rp;
+    int64_t* up = (int64_t*)(smem + off);     off += 8 * (size_t)Rp;
+    int64_t* down = (int64_t*)(smem + off);   off += 8 * (size_t)Rp;
+    int32_t* rung = (int32_t*)(smem + off);   off += 4 * (size_t)Rp;
+    int32_t* rep = (int32_t*)(smem + off);    off += 4 * (size_t)Rp;
+    int32_t* label = (int32_t*)(smem + off);  off += 4 * (size_t)Rp;
+    int32_t* trips = (int32_t*)(smem + off);  off += 4 * (size_t)Rp;
+    float* beta_f = (float*)(smem + off);
+
+    // --- init: spins from the stream, bookkeeping, energies -------------------------------
+    for (int idx = tid; idx < n * Rp; idx += nt) {
+        const int p = idx / Rp, a = idx - p * Rp;
+        sp[idx] = a < R ? init_spin(I, p, a) : (int8_t)0;
+    }
+    for (int a = tid; a < Rp; a += nt) {
+        E[a] = 0; tried[a] = 0; accd[a] = 0; up[a] = 0; down[a] = 0; trips[a] = 0;
+        rung[a] = a;
+        rep[a] = a;
+        label[a] = (R > 1 && a == 0) ? 1 : ((R > 1 && a == R - 1) ? -1 : 0);
+        beta_f[a] = a < R ? (float)I.beta[a] : 0.0f;
+    }
+    if (tid == 0) { sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; }
+    __syncthreads();
+
+    const int a = tid % Rp, qo = tid / Rp, qs = nt / Rp;
+    {
+        int64_t acc = 0;
+        if (a < R)
+            for (int p = qo; p < n; p += qs)
+                acc += site_energy<WM>(p, a, Rp, sp, I.row_ptr, I.col, I.w, I.h);
+        if (acc != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)acc);
+    }
+    __syncthreads();
+    if (WM != PT_WM_F32)
+        for (int x = tid; x < R; x += nt) E[x] = -(E[x] / 2);
+    __syncthreads();
+
+    for (int rnd = -1; rnd < I.n_rounds; ++rnd) {
+        if (rnd >= 0) {
+            // --- sweeps of this round --------------------------------------------------------
+            // Every thread runs the same loop nest so all waves meet the same barriers; padded
+            // slots (a >= R) only skip the work.
+            int64_t dsum = 0;
+            const bool live = a < R;
+            const int g = live ? rung[a] : 0;
+            const float bf = beta_f[g];
+            const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : I.thr + (size_t)g * I.TQ;
+            for (int j = 0; j < I.S; ++j) {
+                const uint32_t t = (uint32_t)(rnd * I.S + j);
+                for (int c = 0; c < k; ++c) {
+                    const int cs = I.color_ptr[c], ce = I.color_ptr[c + 1];
+                    if (live && ce > cs) {
+                        const int q1 = (ce - 1) >> 2;
+                        for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
+                            const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t, PT_TAG_METRO,
+                                                       I.k0, I.k1);
+#pragma unroll
+                            for (int kk = 0; kk < 4; ++kk) {
+                                const int p = 4 * qq + kk;
+                                if (p < cs || p >= ce) continue;
+                                dsum += site_update<WM>(p, a, Rp, sp, I.row_ptr, I.col, I.w, I.h,
+                                                        pt_word(rv, kk), thr_row, I.TQ, bf);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            if (dsum != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)dsum);
+            __syncthreads();
+        }
+
+        // --- best tracking at round granularity -------------------------------------------
+        int64_t mE;
+        int32_t mR;
+        block_best_rung(E, rep, R, scrE, scrR, mE, mR);
+        const bool improved = !sc->have || mE < sc->bestE;
+        const int slot = rep[mR];
+        __syncthreads();
+        if (tid == 0) {
+            if (improved) {
+                sc->bestE = mE; sc->bround = rnd + 1; sc->brung = mR; sc->bslot = slot; sc->have = 1;
+            }
+            if (rnd >= 0 && I.hist) I.hist[rnd] = (double)sc->bestE * I.unit;
+        }
+        if (improved)
+            for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
+        if (rnd >= 0 && I.trace)
+            for (int r = tid; r < R; r += nt)
+                I.trace[(size_t)rnd * R + r] = (double)E[rep[r]] * I.unit;
+        __syncthreads();
+        if (rnd < 0) continue;
+
+        // --- replica exchange and round-trip labels ---------------------------------------
+        exchange_pairs(I, rnd, E, rep, rung, tried, accd);
+        __syncthreads();
+        update_labels(R, rung, label, trips, nullptr, up, down);
+        __syncthreads();
+    }
+
+    // --- results ---------------------------------------------------------------------------
+    int64_t* out = I.out;
+    if (tid == 0) {
+        out[0] = sc->bestE; out[1] = sc->bround; out[2] = sc->brung; out[3] = sc->bslot;
+        out[4] = sc->have;
+    }
+    for (int x = tid; x < R; x += nt) {
+        pt_out_arr(out, R, PT_OUT_E)[x] = E[x];
+        pt_out_arr(out, R, PT_OUT_RUNG)[x] = rung[x];
+        pt_out_arr(out, R, PT_OUT_TRIED)[x] = tried[x];
+        pt_out_arr(out, R, PT_OUT_ACC)[x] = accd[x];
+        pt_out_arr(out, R, PT_OUT_TRIPS)[x] = trips[x];
+        pt_out_arr(out, R, PT_OUT_UP)[x] = up[x];
+        pt_out_arr(out, R, PT_OUT_DOWN)[x] = down[x];
+    }
+    if (I.final_s)
+        for (int idx = tid; idx < n * R; idx += nt) {
+            const int s = idx / n, p = idx - s * n;
+            I.final_s[idx] = sp[(size_t)p * Rp + s];
+        }
+}
+
+template <int WM>
+__global__ __launch_bounds__(1024) void pt_lds_kernel(const PtInstance* __restrict__ insts,
+                                                      const int32_t* __restrict__ order,
+                                                      int n_inst, int32_t* queue) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ LShared sc;
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    for (;;) {
+        if (threadIdx.x == 0) sc.inst = atomicAdd(queue, 1);
+        __syncthreads();
+        const int qi = sc.inst;
+        if (qi >= n_inst) return;
+        const PtInstance I = insts[order[qi]];
+        lds_instance<WM>(I, smem, &sc, scrE, scrR);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Global engine
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void g_init_spins(PtInstance I, GState G) {
+    const int64_t total = (int64_t)I.n * I.Rp;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(idx / I.Rp), a = (int)(idx - (int64_t)p * I.Rp);
+        G.sp[idx] = a < I.R ? init_spin(I, p, a) : (int8_t)0;
+    }
+}
+
+// Reduce a per-thread energy change over the slots of the block, then one atomic per slot.
+__device__ __forceinline__ void block_slot_reduce(int64_t v, int a, int R, long long* red,
+                                                  int64_t* dst) {
+    if (v != 0) atomicAdd((unsigned long long*)&red[a], (unsigned long long)v);
+    __syncthreads();
+    for (int x = threadIdx.x; x < R; x += blockDim.x)
+        if (red[x] != 0) atomicAdd((unsigned long long*)&dst[x], (unsigned long long)red[x]);
+}
+
+template <int WM>
+__global__ __launch_bounds__(1024) void g_init_energy(PtInstance I, GState G) {
+    __shared__ long long red[1024];
+    const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
+    for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
+    __syncthreads();
+    const int a = tid % Rp, qo = tid / Rp, qpb = blockDim.x / Rp;
+    int64_t acc = 0;
+    if (a < R)
+        for (int p = blockIdx.x * qpb + qo; p < I.n; p += gridDim.x * qpb)
+            acc += site_energy<WM>(p, a, Rp, G.sp, I.row_ptr, I.col, I.w, I.h);
+    block_slot_reduce(acc, a, R, red, G.dacc + Rp);   // dacc[1] holds the init totals
+}
+
+template <int WM>
+__global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int c, uint32_t t,
+                                                     int cur) {
+    __shared__ long long red[1024];
+    const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
+    for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
+    __syncthreads();
+    const int a = tid % Rp, qo = tid / Rp, qpb = blockDim.x / Rp;
+    const int cs = I.color_ptr[c], ce = I.color_ptr[c + 1];
+    int64_t dsum = 0;
+    if (a < R && ce > cs) {
+        const int g = G.rung[cur * Rp + a];
+        const float bf = (float)I.beta[g];
+        const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : I.thr + (size_t)g * I.TQ;
+        const int q1 = (ce - 1) >> 2;
+        for (int qq = (cs >> 2) + blockIdx.x * qpb + qo; qq <= q1; qq += gridDim.x * qpb) {
+            const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int p = 4 * qq + kk;
+                if (p < cs || p >= ce) continue;
+                dsum += site_update<WM>(p, a, Rp, G.sp, I.row_ptr, I.col, I.w, I.h,
+                                        pt_word(rv, kk), thr_row, I.TQ, bf);
+            }
+        }
+    }
+    block_slot_reduce(dsum, a, R, red, G.dacc + (size_t)cur * Rp);
+}
+
+// Round end (rnd >= 0) or initialisation end (rnd == -1).  Every workgroup evaluates the energies
+// and the best-rung scan (a few hundred bytes) and copies its share of the best configuration if
+// it improved; workgroup 0 alone writes the next round's state, so no workgroup reads what another
+// one of the same launch writes.
+template <int WM>
+__global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int rnd) {
+    __shared__ int64_t Es[1024];
+    __shared__ int32_t reps[1024];
+    __shared__ int32_t rungs[1024];
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    const int R = I.R, Rp = I.Rp, tid = threadIdx.x;
+    const bool init = rnd < 0;
+    const int cur = rnd & 1, nxt = cur ^ 1;
+    for (int x = tid; x < R; x += blockDim.x) {
+        if (init) {
+            const int64_t T = G.dacc[(size_t)cur * Rp + x];
+            Es[x] = WM == PT_WM_F32 ? T : -(T / 2);
+            reps[x] = x;
+            rungs[x] = x;
+        } else {
+            Es[x] = G.Eb[(size_t)cur * Rp + x] + G.dacc[(size_t)cur * Rp + x];
+            reps[x] = G.rep[(size_t)cur * Rp + x];
+            rungs[x] = G.rung[(size_t)cur * Rp + x];
+        }
+    }
+    __syncthreads();
+    int64_t mE;
+    int32_t mR;
+    block_best_rung(Es, reps, R, scrE, scrR, mE, mR);
+    const int64_t* bc = G.best + 5 * cur;
+    const bool have = !init && bc[4] != 0;
+    const bool improved = !have || mE < bc[0];
+    const int slot = reps[mR];
+    if (improved) {
+        const int chunk = (I.n + gridDim.x - 1) / gridDim.x;
+        const int p0 = blockIdx.x * chunk, p1 = min(I.n, p0 + chunk);
+        for (int p = p0 + tid; p < p1; p += blockDim.x)
+            I.best_s[p] = G.sp[(size_t)p * Rp + slot];
+    }
+    if (blockIdx.x != 0) return;
+
+    int64_t* out = I.out;
+    if (tid == 0) {
+        int64_t* bn = G.best + 5 * nxt;
+        if (improved) {
+            bn[0] = mE; bn[1] = rnd + 1; bn[2] = mR; bn[3] = slot; bn[4] = 1;
+        } else {
+            for (int x = 0; x < 5; ++x) bn[x] = bc[x];
+        }
+        if (!init && I.hist) I.hist[rnd] = (double)(improved ? mE : bc[0]) * I.unit;
+    }
+    if (!init && I.trace)
+        for (int r = tid; r < R; r += blockDim.x)
+            I.trace[(size_t)rnd * R + r] = (double)Es[reps[r]] * I.unit;
+    if (init) {
+        for (int x = tid; x < R; x += blockDim.x)
+            G.label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
+    } else {
+        __syncthreads();   // the trace above reads reps before the exchange rewrites them
+        exchange_pairs(I, rnd, Es, reps, rungs, pt_out_arr(out, R, PT_OUT_TRIED),
+                       pt_out_arr(out, R, PT_OUT_ACC));
+        __syncthreads();
+        update_labels(R, rungs, G.label, nullptr, pt_out_arr(out, R, PT_OUT_TRIPS),
+                      pt_out_arr(out, R, PT_OUT_UP), pt_out_arr(out, R, PT_OUT_DOWN));
+        __syncthreads();
+    }
+    for (int x = tid; x < R; x += blockDim.x) {
+        G.Eb[(size_t)nxt * Rp + x] = Es[x];
+        G.dacc[(size_t)nxt * Rp + x] = 0;
+        G.rung[(size_t)nxt * Rp + x] = rungs[x];
+        G.rep[(size_t)nxt * Rp + x] = reps[x];
+    }
+}
+
+inline int block_for_slots(int Rp) { return Rp <= 64 ? 64 : ((Rp + 63) / 64) * 64; }
+
+}  // namespace
+
+size_t pt_lds_bytes(int n, int Rp) {
+    return (((size_t)n * Rp + 15) & ~(size_t)15) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+}
+
+hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int n_inst,
+                         int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
+                         hipStream_t s) {
+    const void* fn = wm == PT_WM_I8    ? (const void*)pt_lds_kernel<PT_WM_I8>
+                     : wm == PT_WM_I32 ? (const void*)pt_lds_kernel<PT_WM_I32>
+                                       : (const void*)pt_lds_kernel<PT_WM_F32>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    if (wm == PT_WM_I8)
+        pt_lds_kernel<PT_WM_I8><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
+    else if (wm == PT_WM_I32)
+        pt_lds_kernel<PT_WM_I32><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
+    else
+        pt_lds_kernel<PT_WM_F32><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
+    return hipGetLastError();
+}
+
+template <int WM>
+static hipError_t g_init_impl(const PtInstance& I, const GState& G, int block, int reg,
+                              hipStream_t s) {
+    const int64_t total = (int64_t)I.n * I.Rp;
+    const int gs = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    g_init_spins<<<gs, 256, 0, s>>>(I, G);
+    const int qpb = block / I.Rp;
+    const int ge = (int)std::min<int64_t>((I.n + qpb - 1) / qpb, 2048);
+    g_init_energy<WM><<<ge, block, 0, s>>>(I, G);
+    g_round_end<WM><<<reg, block_for_slots(I.Rp), 0, s>>>(I, G, -1);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int reg,
+                            hipStream_t s) {
+    if (I.wm == PT_WM_I8) return g_init_impl<PT_WM_I8>(I, G, block, reg, s);
+    if (I.wm == PT_WM_I32) return g_init_impl<PT_WM_I32>(I, G, block, reg, s);
+    return g_init_impl<PT_WM_F32>(I, G, block, reg, s);
+}
+
+hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, uint32_t t, int cur,
+                             int block, int grid, hipStream_t s) {
+    if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8><<<grid, block, 0, s>>>(I, G, c, t, cur);
+    else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32><<<grid, block, 0, s>>>(I, G, c, t, cur);
+    else g_color_step<PT_WM_F32><<<grid, block, 0, s>>>(I, G, c, t, cur);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rnd,
+                                 int round_end_grid, hipStream_t s) {
+    const int b = block_for_slots(I.Rp);
+    if (I.wm == PT_WM_I8) g_round_end<PT_WM_I8><<<round_end_grid, b, 0, s>>>(I, G, rnd);
+    else if (I.wm == PT_WM_I32) g_round_end<PT_WM_I32><<<round_end_grid, b, 0, s>>>(I, G, rnd);
+    else g_round_end<PT_WM_F32><<<round_end_grid, b, 0, s>>>(I, G, rnd);
+    return hipGetLastError();
+}

@@ -1,0 +1,145 @@
+/*
+ * drift_pt.h -- C ABI of libdriftpt.so, the gfx950 parallel-tempering engine.
+ *
+ * It replaces the reference's process boundary: drift/gpu.py:83-106 wrote problem.bin, spawned
+ * cuda/ising_pt and parsed result.bin plus a stderr line; cuda/ising_pt.cu:159-243 was that
+ * executable.  Here the caller hands plain host arrays to one synchronous call and receives the
+ * results in caller-allocated host arrays.  The library copies inputs to device memory, never keeps
+ * a caller pointer after returning, and never calls exit().
+ *
+ * Conventions (drift/ising.py:25-68): spins are +1/-1, E(s) = -1/2 s^T J s - h^T s, the coupling
+ * matrix is symmetric with a zero diagonal and is passed as CSR holding both directions of every
+ * bond.  Rung 0 is the coldest rung (largest beta), as in the reference's ascending T ladder
+ * (drift/solvers/parallel_tempering.py:29-37).
+ *
+ * All entry points return 0 on success and a negative drift_pt_status otherwise; the message is
+ * then available from drift_pt_last_error() (thread-local).
+ */
+#ifndef DRIFT_PT_H
+#define DRIFT_PT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRIFT_PT_ABI_VERSION 1
+
+enum drift_pt_status {
+    DRIFT_PT_OK = 0,
+    DRIFT_PT_EINVAL = -1,   /* malformed argument or CSR */
+    DRIFT_PT_ECOLOR = -2,   /* colouring is not a partition or two same-colour spins are adjacent */
+    DRIFT_PT_ENOMEM = -3,   /* device or host allocation failed */
+    DRIFT_PT_EHIP = -4,     /* a HIP runtime call failed */
+    DRIFT_PT_ENODEV = -5,   /* no gfx950 device visible */
+    DRIFT_PT_ERANGE = -6    /* values outside what the exact integer path can represent */
+};
+
+/* coupling storage (cuda/ising_pt.cu:44 used float weights only) */
+enum drift_pt_weight_type {
+    DRIFT_PT_W_I8 = 1,      /* int8 weights, int32 field: exact integer path */
+    DRIFT_PT_W_I32 = 2,     /* int32 weights, int32 field: exact integer path */
+    DRIFT_PT_W_F32 = 3      /* float weights, float field */
+};
+
+enum drift_pt_engine {
+    DRIFT_PT_ENGINE_AUTO = 0,
+    DRIFT_PT_ENGINE_LDS = 1,    /* whole instance (all replicas) in one workgroup's LDS, persistent */
+    DRIFT_PT_ENGINE_GLOBAL = 2  /* replica-interleaved spins in HBM, one launch per colour step */
+};
+
+enum drift_pt_flags {
+    DRIFT_PT_RECORD_HISTORY = 1, /* fill result.best_history */
+    DRIFT_PT_RECORD_FINAL = 2,   /* fill result.final_spins */
+    DRIFT_PT_SKIP_VALIDATION = 4, /* trust CSR symmetry/colouring (the colouring is still used) */
+    DRIFT_PT_RECORD_TRACE = 8,    /* fill result.rung_energy_trace */
+    DRIFT_PT_TIME_KERNELS = 16    /* time each sweep-kernel launch (up to 4096) with device events */
+};
+
+/* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and
+ * drift/gpu.py:83-95. */
+typedef struct drift_pt_problem {
+    int32_t n;                  /* spins */
+    int32_t nnz;                /* CSR entries, both directions of each bond */
+    const int32_t* row_ptr;     /* n + 1 */
+    const int32_t* col_idx;     /* nnz, no self loops */
+    int32_t weight_type;        /* enum drift_pt_weight_type */
+    int32_t n_colors;           /* k; 0 = colour greedily inside the library */
+    const void* weight;         /* nnz entries of weight_type */
+    const void* h;              /* n entries: int32 for integer types, float for F32; NULL = 0 */
+    double energy_scale;        /* energy = scale * integer energy (integer types; 1.0 if unsure) */
+    const int32_t* color_ptr;   /* k + 1 (ignored when n_colors == 0) */
+    const int32_t* color_spins; /* n, spins grouped by colour */
+} drift_pt_problem;
+
+/* Run parameters (drift/gpu.py:43-52 keyword arguments). */
+typedef struct drift_pt_params {
+    uint64_t seed;
+    int32_t n_replicas;         /* R */
+    int32_t n_rounds;
+    int32_t sweeps_per_round;
+    int32_t engine;             /* enum drift_pt_engine */
+    double T_min, T_max;        /* geometric ladder when beta == NULL */
+    const double* beta;         /* optional R inverse temperatures, rung 0 coldest */
+    int32_t flags;              /* enum drift_pt_flags */
+    int32_t block_threads;      /* 0 = chosen by the tuner */
+} drift_pt_params;
+
+/* Results, all arrays caller-allocated (NULL = not wanted, except best_s). */
+typedef struct drift_pt_result {
+    int8_t* best_s;             /* n: lowest-energy configuration seen (original spin order) */
+    double best_E;              /* its energy */
+    int64_t best_E_units;       /* exact integer units (integer types) or 2^-36 units (F32) */
+    int32_t best_round;         /* 0 = initial states, r + 1 = after round r */
+    int32_t best_rung;
+    int32_t best_slot;
+    int32_t engine_used;
+    int64_t* swap_tried;        /* R - 1, per lower rung */
+    int64_t* swap_accepted;     /* R - 1 */
+    int32_t* round_trips;       /* R, per replica slot: coldest -> hottest -> coldest trips */
+    int64_t* rung_up;           /* R, per rung: rounds spent there by replicas last at the cold end */
+    int64_t* rung_down;         /* R, per rung: rounds spent there by replicas last at the hot end */
+    int32_t* rung_of;           /* R, final rung of each slot */
+    int64_t* final_E_units;     /* R, final energy of each slot in units */
+    int8_t* final_spins;        /* R * n slot-major (needs DRIFT_PT_RECORD_FINAL) */
+    double* best_history;       /* n_rounds (needs DRIFT_PT_RECORD_HISTORY) */
+    double* rung_energy_trace;  /* n_rounds * R: energy at each rung after each round's sweeps
+                                   (needs DRIFT_PT_RECORD_TRACE) */
+    double seconds;             /* device time of the sweep + exchange loop (cuda/ising_pt.cu:213-222) */
+    double attempts;            /* R * n_rounds * sweeps_per_round * n (cuda/ising_pt.cu:222) */
+    double kernel_seconds;      /* summed device time of the timed sweep-kernel launches */
+    double kernel_attempts;     /* spin-flip attempts those launches performed */
+    int64_t kernel_launches;    /* launches timed (DRIFT_PT_TIME_KERNELS), else 1 for the whole loop */
+} drift_pt_result;
+
+/* ABI version of the loaded library (DRIFT_PT_ABI_VERSION). */
+int drift_pt_abi_version(void);
+
+/* 1 if a gfx950 device is visible, else 0.  Never fails.  Replaces the binary-exists probe of
+ * drift/gpu.py:38-40. */
+int drift_pt_device_available(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* drift_pt_last_error(void);
+
+/* Greedy colouring of a CSR graph (drift/gpu.py:122-145): vertices in index order take the smallest
+ * colour unused by earlier neighbours; color_spins is the stable sort by colour.  color_ptr must hold
+ * n + 1 entries.  Returns k (>= 1) or a negative status. */
+int drift_pt_greedy_coloring(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                             int32_t* color_ptr, int32_t* color_spins);
+
+/* One parallel-tempering run on the GPU.  Replaces cuda/ising_pt.cu:159-243 (main) and the
+ * subprocess round trip of drift/gpu.py:83-106. */
+int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
+                 drift_pt_result* result);
+
+/* Many independent instances in one persistent launch; each result equals what drift_pt_run gives
+ * for that instance and its params alone.  No reference counterpart: the reference solves one
+ * instance per process (drift/gpu.py:97). */
+int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
+                       int32_t n_instances, drift_pt_result* results);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,185 @@
+"""GPU parity: the gfx950 engines through the C ABI against the CPU oracle, bit for bit.
+
+Every case runs the same instance, colouring, ladder and seed on the GPU (drift_amd.gpu ->
+libdriftpt.so) and on the oracle (oracle/pt_oracle.c) and compares spins, energies, swap
+decisions, the rung permutation and the best state exactly.  Covered: integer couplings (int8,
+int32, half-integer QUBO scale, fields), float couplings, both engines, several tile sizes, replica
+counts that are not powers of two, R = 1 and 2, the 2-D ferromagnet of config C1 (L=4 and L=16),
+the first 10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), and a batch of uneven float instances
+in one persistent launch.  Reference GPU tests (tests/test_gpu.py:45-67) are reproduced last.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from drift_amd import gpu
+from drift_amd.builders.families import pmj_spin_glass
+from drift_amd.builders.lattice import cubic_pmj_glass, random_regular_gaussian, square_ferromagnet
+from drift_amd.builders.qubo import maxcut_ising, qubo_to_ising, random_graph
+from drift_amd.ising import IsingModel
+from drift_amd.solve import solve
+from drift_amd.solvers.exact import exact_ground_state
+from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skipif(not gpu.gpu_available(), reason="no gfx950 device / engine not built"),
+]
+
+THREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+def _announce(name):
+    print(f"[gpu-parity] {name}", flush=True)
+
+
+def compare(model, *, R, n_rounds, S, seed, T_min=0.1, T_max=5.0, beta=None, engine="auto",
+            block=0, force_float=False):
+    enc = gpu.encode_problem(model, force_float=force_float)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    if beta is None:
+        beta = ladder_betas(geometric_ladder(R, T_min, T_max))
+    g = gpu.parallel_tempering_gpu(
+        model, n_replicas=R, n_rounds=n_rounds, sweeps_per_round=S, seed=seed, betas=beta,
+        engine=engine, block_threads=block, coloring=(cptr, cspins), record_history=True,
+        return_final=True, record_trace=True, force_float=force_float)
+    o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=seed, R=R, n_rounds=n_rounds,
+                              sweeps_per_round=S, beta=beta, n_threads=THREADS, trace=True)
+    assert g.best_E_units == o["best_units"]
+    assert (g.best_round, g.best_rung) == (o["best_round"], o["best_rung"])
+    assert np.array_equal(g.best_s.astype(np.int8), o["best_s"])
+    assert np.array_equal(g.final_spins, o["final_spins"])
+    assert np.array_equal(g.final_E, o["final_units"])
+    assert np.array_equal(g.rung_of, o["rung_of"])
+    assert np.array_equal(g.swap_tried, o["swap_tried"])
+    assert np.array_equal(g.swap_accepted, o["swap_acc"])
+    assert np.array_equal(g.round_trips, o["round_trips"])
+    assert np.array_equal(g.rung_up, o["rung_up"]) and np.array_equal(g.rung_down, o["rung_down"])
+    assert np.array_equal(g.best_history, o["best_history"])
+    assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
+    # the reported energy is the energy of the reported configuration
+    e = model.energy(g.best_s)
+    assert g.best_E == pytest.approx(e, rel=1e-5, abs=1e-6)
+    return g, o, enc
+
+
+@pytest.mark.parametrize("engine", ["lds", "global"])
+def test_maxcut_int8(engine):
+    _announce(f"maxcut int8 {engine}")
+    g, _, enc = compare(maxcut_ising(random_graph(24, p=0.5, seed=3)), R=16, n_rounds=60, S=4,
+                        seed=1, T_min=0.05, T_max=4.0, engine=engine)
+    assert enc.weight_type == 1 and g.engine == engine
+
+
+@pytest.mark.parametrize("engine", ["lds", "global"])
+def test_float_couplings_and_field(engine):
+    _announce(f"float field {engine}")
+    model = pmj_spin_glass(20, 7, field=0.3)
+    g, _, enc = compare(model, R=20, n_rounds=80, S=3, seed=2, T_min=0.05, T_max=5.0,
+                        engine=engine)
+    assert not enc.is_integer
+
+
+@pytest.mark.parametrize("engine", ["lds", "global"])
+def test_qubo_half_integer_scale_and_int32(engine):
+    _announce(f"qubo/int32 {engine}")
+    rng = np.random.default_rng(4)
+    Q = rng.integers(-300, 300, size=(18, 18)).astype(np.float64)
+    model, _ = qubo_to_ising(Q)
+    g, _, enc = compare(model, R=12, n_rounds=50, S=2, seed=9, T_min=5.0, T_max=400.0,
+                        engine=engine)
+    assert enc.weight_type == 2 and enc.scale in (0.5, 0.25)
+
+
+@pytest.mark.parametrize("R", [1, 2, 5, 12, 33, 64, 100])
+def test_replica_counts(R):
+    _announce(f"R={R}")
+    model = maxcut_ising(random_graph(30, p=0.3, seed=R))
+    compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="lds")
+    compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="global")
+
+
+@pytest.mark.parametrize("block", [64, 128, 512, 1024])
+def test_tile_sizes_do_not_change_results(block):
+    _announce(f"block={block}")
+    model = pmj_spin_glass(26, 9)
+    a = compare(model, R=16, n_rounds=25, S=2, seed=3, engine="lds", block=block)[0]
+    b = compare(model, R=16, n_rounds=25, S=2, seed=3, engine="global", block=block)[0]
+    assert np.array_equal(a.final_spins, b.final_spins)
+
+
+@pytest.mark.parametrize("L", [4, 16])
+def test_config_c1_square_ferromagnet(L):
+    _announce(f"C1 L={L}")
+    beta = ladder_betas(geometric_ladder(16, 1.0, 10.0))
+    compare(square_ferromagnet(L), R=16, n_rounds=400 if L == 4 else 150, S=1, seed=11,
+            beta=beta, engine="lds")
+    compare(square_ferromagnet(L), R=16, n_rounds=60, S=1, seed=11, beta=beta, engine="global")
+
+
+def test_config_c2_first_thousand_sweeps():
+    _announce("C2 L=32 R=64 1000 sweeps")
+    model = cubic_pmj_glass(32, 0)
+    beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))     # beta 3.0 .. 0.2
+    g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="global")
+    assert enc.weight_type == 1 and g.engine == "global"
+
+
+def test_batch_equals_oracle_per_instance():
+    _announce("batch of 48 uneven float instances")
+    rng = np.random.default_rng(0)
+    sizes = (2 * rng.integers(32, 129, size=48)).tolist()
+    models = [random_regular_gaussian(n, seed=1000 + i) for i, n in enumerate(sizes)]
+    seeds = [77 + 3 * i for i in range(len(models))]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=16, T_min=0.1, T_max=3.0,
+                                           n_rounds=40, sweeps_per_round=2, seeds=seeds,
+                                           return_final=True)
+    beta = ladder_betas(geometric_ladder(16, 0.1, 3.0))
+    for model, seed, r in zip(models, seeds, res):
+        enc = gpu.encode_problem(model)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=16, n_rounds=40,
+                                  sweeps_per_round=2, beta=beta, n_threads=THREADS)
+        assert r.best_E_units == o["best_units"]
+        assert np.array_equal(r.final_spins, o["final_spins"])
+        assert np.array_equal(r.rung_of, o["rung_of"])
+        assert r.best_E == pytest.approx(model.energy(r.best_s), rel=1e-5)
+    # one instance alone gives the same result as inside the batch
+    alone = gpu.parallel_tempering_gpu(models[5], n_replicas=16, T_min=0.1, T_max=3.0,
+                                       n_rounds=40, sweeps_per_round=2, seed=seeds[5],
+                                       engine="lds", return_final=True)
+    assert np.array_equal(alone.final_spins, res[5].final_spins)
+
+
+def test_reference_gpu_reproduces_exact_on_small_maxcut():
+    _announce("reference test_gpu exact")
+    model = maxcut_ising(random_graph(12, p=0.5, seed=3))
+    _, e_exact, _ = exact_ground_state(model)
+    res = gpu.parallel_tempering_gpu(model, n_replicas=16, T_min=0.05, T_max=4.0, n_rounds=200,
+                                     sweeps_per_round=4, seed=1)
+    assert res.best_E == pytest.approx(e_exact)
+    assert res.throughput > 0 and res.seconds > 0
+
+
+def test_reference_solve_gpu_path_is_not_certified():
+    _announce("reference solve gpu-pt")
+    model = maxcut_ising(random_graph(20, p=0.5, seed=2))
+    sol = solve(model, exact_max=16, use_gpu=True, n_replicas=16, n_rounds=80, sweeps_per_round=4,
+                seed=1)
+    assert sol.method == "gpu-pt" and sol.certified is False
+    assert np.isclose(model.energy(sol.s), sol.energy)
+
+
+def test_engine_errors_are_reported():
+    _announce("error paths")
+    J = np.zeros((4, 4))
+    J[0, 1] = J[1, 0] = 1.0
+    model = IsingModel(J, np.zeros(4))
+    with pytest.raises(RuntimeError, match="colour"):
+        gpu.parallel_tempering_gpu(model, n_replicas=2, n_rounds=1,
+                                   coloring=(np.array([0, 4]), np.arange(4)))
+    with pytest.raises(RuntimeError, match="n_replicas"):
+        gpu.parallel_tempering_gpu(model, n_replicas=2000, n_rounds=1)
