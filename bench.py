@@ -51,15 +51,34 @@ def workload(name: str):
     return model, R, beta, desc
 
 
-def algorithmic_bytes_per_attempt(model, enc) -> float:
-    """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse:
-    own spin read + write, d neighbour spins, d (weight + column index), the row pointers and the
-    field.  Spins are int8; positions are relabelled by colour, so no colour list is read."""
+def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
+    """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse.
+    Spins are int8, so own spin read + write and d neighbour reads are d + 2 bytes.  The CSR
+    kernels add d (weight + int32 column) and two int32 row pointers (plus the field if any);
+    positions are relabelled by colour, so no colour list is read.  The lattice kernel computes
+    neighbours from coordinates and reads one byte holding the site's bond signs."""
     d = model.nnz / model.n
     b_s = 1.0
+    if engine == "lattice":
+        return b_s * (d + 1) + b_s + 1.0
     b_w = enc.weight.dtype.itemsize
     b_h = 0.0 if enc.h is None else enc.h.dtype.itemsize
     return b_s * (d + 1) + b_s + d * (b_w + 4.0) + 8.0 + b_h
+
+
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device copy of nbytes (torch's copy kernel) for scaling rocprof's
+    FETCH_SIZE / WRITE_SIZE counters against a transfer of known size."""
+    import torch
+
+    n = nbytes // 4
+    src = torch.arange(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4}), flush=True)
+    return 0
 
 
 def cpu_baseline(model, enc, R, beta, budget_s: float):
@@ -95,7 +114,11 @@ def main() -> int:
     ap.add_argument("--kernel-timing-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a device copy of this many bytes (rocprof counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this engine has no "
               "multi-GPU path", file=sys.stderr)
@@ -125,7 +148,7 @@ def main() -> int:
 
     print("[bench] kernel timing run", file=sys.stderr, flush=True)
     kt = gpu.parallel_tempering_gpu(model, n_rounds=args.kernel_timing_steps, time_kernels=True, **kw)
-    B = algorithmic_bytes_per_attempt(model, enc)
+    B = algorithmic_bytes_per_attempt(model, enc, kt.engine)
     per_launch_s = kt.kernel_seconds / max(kt.kernel_launches, 1)
     per_launch_attempts = kt.kernel_attempts / max(kt.kernel_launches, 1)
     achieved = B * per_launch_attempts / per_launch_s / 1e9
@@ -136,7 +159,8 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "kernel": "g_color_step" if kt.engine == "global" else "pt_lds_kernel",
+        "kernel": {"global": "g_color_step", "lattice": "g_lattice_step"}.get(kt.engine,
+                                                                               "pt_lds_kernel"),
         "bytes_per_attempt": round(B, 3),
         "attempts_per_launch": per_launch_attempts,
         "avg_launch_us": round(per_launch_s * 1e6, 3),
@@ -166,8 +190,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "format": f"int8 spins [position][replica], CSR int32 columns, "
-                      f"{enc.weight.dtype.name} weights",
+            "format": ("int8 spins [position][replica], implicit periodic neighbours, one "
+                       "byte of bond signs per site" if res.engine == "lattice" else
+                       f"int8 spins [position][replica], CSR int32 columns, "
+                       f"{enc.weight.dtype.name} weights"),
             "engine": res.engine,
             "n": model.n,
             "replicas": R,

@@ -74,6 +74,15 @@ struct EventPool {
     }
 };
 
+struct GraphGuard {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphGuard() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 struct StreamGuard {
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -96,6 +105,8 @@ struct HostInst {
     bool has_h = false;
     std::vector<double> beta;
     std::vector<uint32_t> thr;
+    int ldim = 0, lL = 0;           // periodic hypercubic lattice detected (see detect_lattice)
+    std::vector<uint8_t> jbits;     // per site: bit d set when the bond in direction d is -1
 };
 
 int32_t greedy_color(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* cptr,
@@ -299,6 +310,57 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     return 0;
 }
 
+// Recognise a periodic L^dim lattice (dim 3 or 2, L even >= 4) with +/-1 couplings, no field and
+// the parity colouring in index order (what greedy colouring gives such a lattice).  Fills the
+// per-site sign byte used by the lattice kernel.  Directions: 0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z.
+void detect_lattice(const drift_pt_problem* P, HostInst& H) {
+    H.ldim = 0;
+    if (H.wm != PT_WM_I8 || H.has_h || H.k != 2 || (H.n & 1)) return;
+    const int n = H.n;
+    for (int dim = 3; dim >= 2; --dim) {
+        int L = (int)std::lround(std::pow((double)n, 1.0 / dim));
+        int64_t nn = 1;
+        for (int d = 0; d < dim; ++d) nn *= L;
+        if (nn != n || L < 4 || (L & 1)) continue;
+        if (P->nnz != 2 * dim * n) return;
+        std::vector<uint8_t> jb((size_t)n, 0);
+        const int32_t* rp = P->row_ptr;
+        const int32_t* ci = P->col_idx;
+        const int8_t* w = (const int8_t*)P->weight;
+        bool ok = true;
+        for (int i = 0; i < n && ok; ++i) {
+            const int x = i % L, y = (i / L) % L, z = dim == 3 ? i / (L * L) : 0;
+            int nb[6];
+            auto idx = [&](int xx, int yy, int zz) { return (zz * L + yy) * L + xx; };
+            nb[0] = idx((x + 1) % L, y, z); nb[1] = idx((x + L - 1) % L, y, z);
+            nb[2] = idx(x, (y + 1) % L, z); nb[3] = idx(x, (y + L - 1) % L, z);
+            if (dim == 3) { nb[4] = idx(x, y, (z + 1) % L); nb[5] = idx(x, y, (z + L - 1) % L); }
+            if (rp[i + 1] - rp[i] != 2 * dim) { ok = false; break; }
+            unsigned seen = 0;
+            for (int t = rp[i]; t < rp[i + 1]; ++t) {
+                int d = 0;
+                while (d < 2 * dim && nb[d] != ci[t]) ++d;
+                if (d == 2 * dim || (seen >> d & 1u) || (w[t] != 1 && w[t] != -1)) { ok = false; break; }
+                seen |= 1u << d;
+                if (w[t] == -1) jb[i] |= (uint8_t)(1u << d);
+            }
+        }
+        if (!ok) return;
+        // parity colouring in index order
+        if (H.color_ptr[1] != n / 2) return;
+        int p0 = 0, p1 = n / 2;
+        for (int i = 0; i < n; ++i) {
+            const int x = i % L, y = (i / L) % L, z = dim == 3 ? i / (L * L) : 0;
+            const int par = (x + y + z) & 1;
+            if (H.orig[par ? p1++ : p0++] != i) return;
+        }
+        H.ldim = dim;
+        H.lL = L;
+        H.jbits.swap(jb);
+        return;
+    }
+}
+
 int device_lds_limit() {
     static int cached = -1;
     static std::mutex mu;
@@ -367,6 +429,12 @@ int upload_instance(DevArena& A, const HostInst& H, PtInstance& I, bool want_fin
     I.k0 = (uint32_t)H.seed; I.k1 = (uint32_t)(H.seed >> 32);
     I.n = H.n; I.k = H.k; I.R = H.R; I.Rp = H.Rp; I.n_rounds = H.n_rounds; I.S = H.S;
     I.TQ = H.TQ; I.wm = H.wm;
+    I.ldim = H.ldim; I.lL = H.lL; I.jbits = nullptr;
+    if (H.ldim) {
+        uint8_t* d_jb;
+        PT_HIP(A.upload(&d_jb, H.jbits));
+        I.jbits = d_jb;
+    }
     return 0;
 }
 
@@ -494,30 +562,33 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     return 0;
 }
 
-int run_global(const HostInst& H, drift_pt_result* res, int block_req) {
+int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_lattice) {
     DevArena A;
     PtInstance I;
     int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
     if (rc) return rc;
+    if (!use_lattice) I.jbits = nullptr;
     const int Rp = H.Rp;
     GState G;
     PT_HIP(A.alloc(&G.sp, (size_t)H.n * Rp));
     PT_HIP(A.alloc(&G.Eb, 2 * (size_t)Rp));
-    PT_HIP(A.alloc(&G.dacc, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.dacc, 2 * (size_t)PT_DACC_SPLIT * Rp));
     PT_HIP(A.alloc(&G.rung, 2 * (size_t)Rp));
     PT_HIP(A.alloc(&G.rep, 2 * (size_t)Rp));
     PT_HIP(A.alloc(&G.label, (size_t)Rp));
     PT_HIP(A.alloc(&G.best, 10));
+    PT_HIP(A.alloc(&G.round_base, 1));
     StreamGuard sg;
     PT_HIP(hipStreamCreate(&sg.s));
     PT_HIP(hipEventCreate(&sg.e0));
     PT_HIP(hipEventCreate(&sg.e1));
     PT_HIP(hipMemsetAsync(G.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(G.dacc, 0, 2 * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.dacc, 0, 2 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
     PT_HIP(hipMemsetAsync(G.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
     PT_HIP(hipMemsetAsync(G.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
     PT_HIP(hipMemsetAsync(G.label, 0, sizeof(int32_t) * Rp, sg.s));
     PT_HIP(hipMemsetAsync(G.best, 0, 10 * sizeof(int64_t), sg.s));
+    PT_HIP(hipMemsetAsync(G.round_base, 0, sizeof(int32_t), sg.s));
 
     int block = block_req > 0 ? block_req : 256;
     block = std::max(block, Rp);
@@ -534,26 +605,55 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req) {
     const int64_t max_timed = 4096;
     int64_t timed = 0;
     double timed_attempts = 0.0;
-    PT_HIP(hipEventRecord(sg.e0, sg.s));
-    for (int rnd = 0; rnd < H.n_rounds; ++rnd) {
-        for (int j = 0; j < H.S; ++j) {
-            const uint32_t t = (uint32_t)(rnd * H.S + j);
+    // one round: S sweeps of k colour steps, then the round-end kernel; rl is relative to the
+    // device-side round counter
+    auto launch_round = [&](int rl, bool timing) -> int {
+        for (int j = 0; j < H.S; ++j)
             for (int c = 0; c < H.k; ++c) {
-                const bool tk = time_k && timed < max_timed;
+                const bool tk = timing && timed < max_timed;
                 hipEvent_t a = nullptr, b = nullptr;
                 if (tk) {
                     PT_HIP(pool.pair(&a, &b));
                     PT_HIP(hipEventRecord(a, sg.s));
                 }
-                PT_HIP(pt_launch_g_color(I, G, c, t, rnd & 1, block, grid, sg.s));
+                if (use_lattice) PT_HIP(pt_launch_g_lattice(I, G, c, j, rl, block, grid, sg.s));
+                else PT_HIP(pt_launch_g_color(I, G, c, j, rl, block, grid, sg.s));
                 if (tk) {
                     PT_HIP(hipEventRecord(b, sg.s));
                     ++timed;
                     timed_attempts += (double)H.R * (H.color_ptr[c + 1] - H.color_ptr[c]);
                 }
             }
+        PT_HIP(pt_launch_g_round_end(I, G, rl, reg, sg.s));
+        return 0;
+    };
+    // Chunks of rounds are captured once into a hipGraph (rounds read their index from the
+    // device counter, which a tick kernel advances) and replayed; leftovers launch directly.
+    GraphGuard gg;
+    int chunk = 0;
+    const int per_round = H.S * H.k + 1;
+    if (!time_k && !(H.flags & DRIFT_PT_NO_GRAPH) && H.n_rounds >= 4) {
+        chunk = std::max(1, std::min(H.n_rounds, 1024 / per_round));
+        PT_HIP(hipStreamBeginCapture(sg.s, hipStreamCaptureModeThreadLocal));
+        for (int rl = 0; rl < chunk; ++rl) {
+            rc = launch_round(rl, false);
+            if (rc) {
+                hipGraph_t junk = nullptr;
+                (void)hipStreamEndCapture(sg.s, &junk);
+                if (junk) (void)hipGraphDestroy(junk);
+                return rc;
+            }
         }
-        PT_HIP(pt_launch_g_round_end(I, G, rnd, reg, sg.s));
+        PT_HIP(pt_launch_g_tick(G, chunk, sg.s));
+        PT_HIP(hipStreamEndCapture(sg.s, &gg.graph));
+        PT_HIP(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0));
+    }
+    const int replays = chunk ? H.n_rounds / chunk : 0;
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    for (int g = 0; g < replays; ++g) PT_HIP(hipGraphLaunch(gg.exec, sg.s));
+    for (int rl = 0; rl < H.n_rounds - replays * chunk; ++rl) {
+        rc = launch_round(rl, time_k);
+        if (rc) return rc;
     }
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));
@@ -589,7 +689,7 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req) {
     rc = fetch_results(H, I, out, fs, res);
     if (rc) return rc;
     res->seconds = ms * 1e-3;
-    res->engine_used = DRIFT_PT_ENGINE_GLOBAL;
+    res->engine_used = use_lattice ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
     if (time_k) {
         res->kernel_seconds = kernel_s;
         res->kernel_attempts = timed_attempts;
@@ -652,17 +752,25 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     if (rc) return rc;
     rc = check_device();
     if (rc) return rc;
+    detect_lattice(problem, H);
     int engine = params->engine;
-    if (engine == DRIFT_PT_ENGINE_AUTO)
-        engine = (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) ? DRIFT_PT_ENGINE_LDS
-                                                                  : DRIFT_PT_ENGINE_GLOBAL;
+    if (engine == DRIFT_PT_ENGINE_AUTO) {
+        if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) engine = DRIFT_PT_ENGINE_LDS;
+        else engine = H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+    }
+    if (engine == DRIFT_PT_ENGINE_LATTICE) {
+        if (!H.ldim)
+            return fail(DRIFT_PT_EINVAL, "lattice engine needs a periodic L^2/L^3 lattice (L even >= 4) "
+                                         "with +/-1 couplings, no field and the parity colouring");
+        return run_global(H, result, params->block_threads, true);
+    }
     if (engine == DRIFT_PT_ENGINE_LDS) {
         if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "instance does not fit one workgroup's LDS");
         std::vector<HostInst> hs{std::move(H)};
         return run_lds(hs, {result}, params->block_threads);
     }
     if (engine != DRIFT_PT_ENGINE_GLOBAL) return fail(DRIFT_PT_EINVAL, "unknown engine");
-    return run_global(H, result, params->block_threads);
+    return run_global(H, result, params->block_threads, false);
 }
 
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,

@@ -35,22 +35,30 @@ struct PtInstance {
     double* hist;             // n_rounds or null
     double* trace;            // n_rounds * R energies by rung, or null
     int64_t* out;             // statistics block
+    const uint8_t* jbits;     // lattice engine: per site, bit d set when the bond in direction
+                              // d (+x,-x,+y,-y,+z,-z) is -1; null otherwise
     double unit;              // energy per unit
     uint32_t k0, k1;          // Philox key (seed)
     int32_t n, k, R, Rp;      // Rp: replica slots padded (power of two <= 64, else multiple of 64)
     int32_t n_rounds, S, TQ, wm;
+    int32_t ldim, lL;         // lattice dimension and side (0 when not a lattice)
 };
+
+#define PT_DACC_SPLIT 8
 
 // Global-engine state (replica-interleaved spins in HBM).  Two-deep buffers are indexed by
 // round parity so the many workgroups of one launch only ever read what another launch wrote.
 struct GState {
     int8_t* sp;      // n * Rp, [position][slot]
     int64_t* Eb;     // 2 * Rp: energy at the start of the round
-    int64_t* dacc;   // 2 * Rp: energy change accumulated during the round
+    int64_t* dacc;   // 2 * PT_DACC_SPLIT * Rp: energy change accumulated during the round, split
+                     // over PT_DACC_SPLIT counters (by block index) to spread atomic traffic
     int32_t* rung;   // 2 * Rp: slot -> rung
     int32_t* rep;    // 2 * Rp: rung -> slot
     int32_t* label;  // Rp: round-trip direction label (+1 last at cold end, -1 at hot end)
     int64_t* best;   // 2 * 5: E, round, rung, slot, have
+    int32_t* round_base;  // round index that launch argument 'rl' is relative to; advanced on the
+                          // device by pt_launch_g_tick so a captured chunk of rounds can replay
 };
 
 size_t pt_lds_bytes(int n, int Rp);
@@ -61,7 +69,13 @@ hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int 
 
 hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
                             hipStream_t s);
-hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, uint32_t t, int cur,
+// Colour step c of sweep j of round (*round_base + rl).
+hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j, int rl,
                              int block, int grid, hipStream_t s);
-hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rnd,
+hipError_t pt_launch_g_lattice(const PtInstance& I, const GState& G, int c, int j, int rl,
+                               int block, int grid, hipStream_t s);
+// End of round (*round_base + rl): energies, best state, exchange.
+hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rl,
                                  int round_end_grid, hipStream_t s);
+// *round_base += rounds.
+hipError_t pt_launch_g_tick(const GState& G, int rounds, hipStream_t s);

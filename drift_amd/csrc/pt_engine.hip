@@ -358,13 +358,22 @@ __global__ __launch_bounds__(256) void g_init_spins(PtInstance I, GState G) {
     }
 }
 
-// Reduce a per-thread energy change over the slots of the block, then one atomic per slot.
-__device__ __forceinline__ void block_slot_reduce(int64_t v, int a, int R, long long* red,
-                                                  int64_t* dst) {
+// Reduce a per-thread energy change over the slots of the block, then one atomic per slot into
+// the block's sub-counter (integer sums: the order of arrival does not matter).
+__device__ __forceinline__ void block_slot_reduce(int64_t v, int a, int R, int Rp, long long* red,
+                                                  int64_t* dacc, int par) {
     if (v != 0) atomicAdd((unsigned long long*)&red[a], (unsigned long long)v);
     __syncthreads();
+    int64_t* dst = dacc + ((size_t)par * PT_DACC_SPLIT + blockIdx.x % PT_DACC_SPLIT) * Rp;
     for (int x = threadIdx.x; x < R; x += blockDim.x)
         if (red[x] != 0) atomicAdd((unsigned long long*)&dst[x], (unsigned long long)red[x]);
+}
+
+__device__ __forceinline__ int64_t dacc_total(const GState& G, int par, int Rp, int x) {
+    int64_t v = 0;
+#pragma unroll
+    for (int k = 0; k < PT_DACC_SPLIT; ++k) v += G.dacc[((size_t)par * PT_DACC_SPLIT + k) * Rp + x];
+    return v;
 }
 
 template <int WM>
@@ -378,17 +387,22 @@ __global__ __launch_bounds__(1024) void g_init_energy(PtInstance I, GState G) {
     if (a < R)
         for (int p = blockIdx.x * qpb + qo; p < I.n; p += gridDim.x * qpb)
             acc += site_energy<WM>(p, a, Rp, G.sp, I.row_ptr, I.col, I.w, I.h);
-    block_slot_reduce(acc, a, R, red, G.dacc + Rp);   // dacc[1] holds the init totals
+    block_slot_reduce(acc, a, R, Rp, red, G.dacc, 1);   // parity 1 holds the init totals
 }
 
-template <int WM>
-__global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int c, uint32_t t,
-                                                     int cur) {
+template <int WM, bool UNI>
+__global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int c, int j,
+                                                     int rl) {
     __shared__ long long red[1024];
+    const int rnd = *G.round_base + rl, cur = rnd & 1;
+    const uint32_t t = (uint32_t)(rnd * I.S + j);
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
     for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
     __syncthreads();
-    const int a = tid % Rp, qo = tid / Rp, qpb = blockDim.x / Rp;
+    // with Rp a multiple of 64 every wave holds one position: make that visible to the compiler
+    // so row pointers, columns and weights go through the scalar unit
+    const int a = tid % Rp, qpb = blockDim.x / Rp;
+    const int qo = UNI ? __builtin_amdgcn_readfirstlane(tid / Rp) : tid / Rp;
     const int cs = I.color_ptr[c], ce = I.color_ptr[c + 1];
     int64_t dsum = 0;
     if (a < R && ce > cs) {
@@ -407,7 +421,81 @@ __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int
             }
         }
     }
-    block_slot_reduce(dsum, a, R, red, G.dacc + (size_t)cur * Rp);
+    block_slot_reduce(dsum, a, R, Rp, red, G.dacc, cur);
+}
+
+// Colour step on a periodic hypercubic lattice (L even) with +/-1 couplings and no field.
+// Neighbour positions follow from coordinates, so no column indices are read; a site's six bond
+// signs are one byte.  Positions of colour c are the parity-c sites in index order, so rank r of
+// colour c sits in row r / (L/2) at x = 2 (r mod L/2) + parity.  With 64 slots per position all
+// coordinate arithmetic is wave-uniform and runs on the scalar unit.
+template <int DIM, bool UNI>
+__global__ __launch_bounds__(1024) void g_lattice_step(PtInstance I, GState G, int c, int j,
+                                                       int rl) {
+    __shared__ long long red[1024];
+    const int rnd = *G.round_base + rl, cur = rnd & 1;
+    const uint32_t t = (uint32_t)(rnd * I.S + j);
+    const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
+    for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
+    __syncthreads();
+    const int a = tid % Rp, qpb = blockDim.x / Rp;
+    const int qo = UNI ? __builtin_amdgcn_readfirstlane(tid / Rp) : tid / Rp;
+    const int L = I.lL, half = L >> 1, nhalf = I.n >> 1;
+    const int cs = c * nhalf, ce = cs + nhalf, ob = (1 - c) * nhalf;
+    int64_t dsum = 0;
+    if (a < R) {
+        const int g = G.rung[cur * Rp + a];
+        const uint32_t* tr = I.thr + (size_t)g * I.TQ;
+        const uint32_t t2 = I.TQ >= 2 ? tr[1] : 0u;
+        const uint32_t t4 = I.TQ >= 4 ? tr[3] : 0u;
+        const uint32_t t6 = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
+        int8_t* base = G.sp + a;
+        const int q1 = (ce - 1) >> 2;
+        for (int qq = (cs >> 2) + blockIdx.x * qpb + qo; qq <= q1; qq += gridDim.x * qpb) {
+            const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int p = 4 * qq + kk;
+                if (p < cs || p >= ce) continue;
+                const int r = p - cs;
+                const int row = r / half, jr = r - row * half;
+                int x, y, z = 0;
+                if (DIM == 3) {
+                    y = row % L;
+                    z = row / L;
+                    x = 2 * jr + ((c + y + z) & 1);
+                } else {
+                    y = row;
+                    x = 2 * jr + ((c + y) & 1);
+                }
+                const int plane = DIM == 3 ? z * L : 0;
+                const uint32_t jb = I.jbits[(plane + y) * L + x];
+                const int xp = x + 1 == L ? 0 : x + 1, xm = x == 0 ? L - 1 : x - 1;
+                const int yp = y + 1 == L ? 0 : y + 1, ym = y == 0 ? L - 1 : y - 1;
+                const int rowpos = ob + (plane + y) * half;
+                int n0 = base[(size_t)(rowpos + (xp >> 1)) * Rp];
+                int n1 = base[(size_t)(rowpos + (xm >> 1)) * Rp];
+                int n2 = base[(size_t)(ob + (plane + yp) * half + (x >> 1)) * Rp];
+                int n3 = base[(size_t)(ob + (plane + ym) * half + (x >> 1)) * Rp];
+                int l = ((jb & 1u) ? -n0 : n0) + ((jb & 2u) ? -n1 : n1) +
+                        ((jb & 4u) ? -n2 : n2) + ((jb & 8u) ? -n3 : n3);
+                if (DIM == 3) {
+                    const int zp = z + 1 == L ? 0 : z + 1, zm = z == 0 ? L - 1 : z - 1;
+                    const int n4 = base[(size_t)(ob + (zp * L + y) * half + (x >> 1)) * Rp];
+                    const int n5 = base[(size_t)(ob + (zm * L + y) * half + (x >> 1)) * Rp];
+                    l += ((jb & 16u) ? -n4 : n4) + ((jb & 32u) ? -n5 : n5);
+                }
+                const int8_t si = base[(size_t)p * Rp];
+                const int q = si > 0 ? l : -l;
+                const uint32_t th = q == 2 ? t2 : (q == 4 ? t4 : t6);
+                if (q <= 0 || pt_word(rv, kk) < th) {
+                    base[(size_t)p * Rp] = (int8_t)-si;
+                    dsum += 2 * q;
+                }
+            }
+        }
+    }
+    block_slot_reduce(dsum, a, R, Rp, red, G.dacc, cur);
 }
 
 // Round end (rnd >= 0) or initialisation end (rnd == -1).  Every workgroup evaluates the energies
@@ -415,23 +503,24 @@ __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int
 // it improved; workgroup 0 alone writes the next round's state, so no workgroup reads what another
 // one of the same launch writes.
 template <int WM>
-__global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int rnd) {
+__global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int rl, int init_end) {
     __shared__ int64_t Es[1024];
     __shared__ int32_t reps[1024];
     __shared__ int32_t rungs[1024];
     __shared__ long long scrE[16];
     __shared__ int scrR[16];
     const int R = I.R, Rp = I.Rp, tid = threadIdx.x;
-    const bool init = rnd < 0;
+    const bool init = init_end != 0;
+    const int rnd = init ? -1 : *G.round_base + rl;
     const int cur = rnd & 1, nxt = cur ^ 1;
     for (int x = tid; x < R; x += blockDim.x) {
         if (init) {
-            const int64_t T = G.dacc[(size_t)cur * Rp + x];
+            const int64_t T = dacc_total(G, cur, Rp, x);
             Es[x] = WM == PT_WM_F32 ? T : -(T / 2);
             reps[x] = x;
             rungs[x] = x;
         } else {
-            Es[x] = G.Eb[(size_t)cur * Rp + x] + G.dacc[(size_t)cur * Rp + x];
+            Es[x] = G.Eb[(size_t)cur * Rp + x] + dacc_total(G, cur, Rp, x);
             reps[x] = G.rep[(size_t)cur * Rp + x];
             rungs[x] = G.rung[(size_t)cur * Rp + x];
         }
@@ -479,10 +568,14 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
     }
     for (int x = tid; x < R; x += blockDim.x) {
         G.Eb[(size_t)nxt * Rp + x] = Es[x];
-        G.dacc[(size_t)nxt * Rp + x] = 0;
+        for (int k = 0; k < PT_DACC_SPLIT; ++k) G.dacc[((size_t)nxt * PT_DACC_SPLIT + k) * Rp + x] = 0;
         G.rung[(size_t)nxt * Rp + x] = rungs[x];
         G.rep[(size_t)nxt * Rp + x] = reps[x];
     }
+}
+
+__global__ void g_tick(int32_t* round_base, int rounds) {
+    if (threadIdx.x == 0) *round_base += rounds;
 }
 
 inline int block_for_slots(int Rp) { return Rp <= 64 ? 64 : ((Rp + 63) / 64) * 64; }
@@ -520,7 +613,7 @@ static hipError_t g_init_impl(const PtInstance& I, const GState& G, int block, i
     const int qpb = block / I.Rp;
     const int ge = (int)std::min<int64_t>((I.n + qpb - 1) / qpb, 2048);
     g_init_energy<WM><<<ge, block, 0, s>>>(I, G);
-    g_round_end<WM><<<reg, block_for_slots(I.Rp), 0, s>>>(I, G, -1);
+    g_round_end<WM><<<reg, block_for_slots(I.Rp), 0, s>>>(I, G, 0, 1);
     return hipGetLastError();
 }
 
@@ -531,19 +624,44 @@ hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int
     return g_init_impl<PT_WM_F32>(I, G, block, reg, s);
 }
 
-hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, uint32_t t, int cur,
+template <bool UNI>
+static void g_color_dispatch(const PtInstance& I, const GState& G, int c, int j, int rl, int block,
+                             int grid, hipStream_t s) {
+    if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    else g_color_step<PT_WM_F32, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
+}
+
+hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j, int rl,
                              int block, int grid, hipStream_t s) {
-    if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8><<<grid, block, 0, s>>>(I, G, c, t, cur);
-    else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32><<<grid, block, 0, s>>>(I, G, c, t, cur);
-    else g_color_step<PT_WM_F32><<<grid, block, 0, s>>>(I, G, c, t, cur);
+    if (I.Rp % 64 == 0) g_color_dispatch<true>(I, G, c, j, rl, block, grid, s);
+    else g_color_dispatch<false>(I, G, c, j, rl, block, grid, s);
     return hipGetLastError();
 }
 
-hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rnd,
+hipError_t pt_launch_g_lattice(const PtInstance& I, const GState& G, int c, int j, int rl,
+                               int block, int grid, hipStream_t s) {
+    const bool uni = I.Rp % 64 == 0;
+    if (I.ldim == 3) {
+        if (uni) g_lattice_step<3, true><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else g_lattice_step<3, false><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    } else {
+        if (uni) g_lattice_step<2, true><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else g_lattice_step<2, false><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    }
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rl,
                                  int round_end_grid, hipStream_t s) {
     const int b = block_for_slots(I.Rp);
-    if (I.wm == PT_WM_I8) g_round_end<PT_WM_I8><<<round_end_grid, b, 0, s>>>(I, G, rnd);
-    else if (I.wm == PT_WM_I32) g_round_end<PT_WM_I32><<<round_end_grid, b, 0, s>>>(I, G, rnd);
-    else g_round_end<PT_WM_F32><<<round_end_grid, b, 0, s>>>(I, G, rnd);
+    if (I.wm == PT_WM_I8) g_round_end<PT_WM_I8><<<round_end_grid, b, 0, s>>>(I, G, rl, 0);
+    else if (I.wm == PT_WM_I32) g_round_end<PT_WM_I32><<<round_end_grid, b, 0, s>>>(I, G, rl, 0);
+    else g_round_end<PT_WM_F32><<<round_end_grid, b, 0, s>>>(I, G, rl, 0);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_g_tick(const GState& G, int rounds, hipStream_t s) {
+    g_tick<<<1, 64, 0, s>>>(G.round_base, rounds);
     return hipGetLastError();
 }

@@ -161,9 +161,10 @@ class _Buffers:
 
 
 def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, engine,
-            block_threads, final, history, trace=False, time_kernels=False):
+            block_threads, final, history, trace=False, time_kernels=False, graph=True):
     flags = ((_lib.RECORD_FINAL if final else 0) | (_lib.RECORD_HISTORY if history else 0)
-             | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0))
+             | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0)
+             | (0 if graph else _lib.NO_GRAPH))
     if engine not in _lib.ENGINES:
         raise ValueError(f"engine must be one of {sorted(_lib.ENGINES)}")
     return _lib.Params(
@@ -241,6 +242,7 @@ def parallel_tempering_gpu(
     return_final: bool = False,
     record_trace: bool = False,
     time_kernels: bool = False,
+    graph: bool = True,
     force_float: bool = False,
 ) -> PtResult:
     """Solve ``model`` by parallel tempering on the GPU; same signature and result as the CPU
@@ -250,7 +252,8 @@ def parallel_tempering_gpu(
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
     prm = _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, b, engine,
-                  block_threads, return_final, record_history, record_trace, time_kernels)
+                  block_threads, return_final, record_history, record_trace, time_kernels,
+                  graph)
     buf = _Buffers(prob.n, n_replicas, n_rounds, return_final, record_history, record_trace)
     res = buf.struct
     rc = lib.drift_pt_run(prob.struct, prm, res)

@@ -45,7 +45,9 @@ enum drift_pt_weight_type {
 enum drift_pt_engine {
     DRIFT_PT_ENGINE_AUTO = 0,
     DRIFT_PT_ENGINE_LDS = 1,    /* whole instance (all replicas) in one workgroup's LDS, persistent */
-    DRIFT_PT_ENGINE_GLOBAL = 2  /* replica-interleaved spins in HBM, one launch per colour step */
+    DRIFT_PT_ENGINE_GLOBAL = 2, /* replica-interleaved spins in HBM, one launch per colour step */
+    DRIFT_PT_ENGINE_LATTICE = 3 /* the global engine's periodic-lattice kernel: implicit neighbours,
+                                   one byte of bond signs per site (+/-1 couplings, no field) */
 };
 
 enum drift_pt_flags {
@@ -53,7 +55,8 @@ enum drift_pt_flags {
     DRIFT_PT_RECORD_FINAL = 2,   /* fill result.final_spins */
     DRIFT_PT_SKIP_VALIDATION = 4, /* trust CSR symmetry/colouring (the colouring is still used) */
     DRIFT_PT_RECORD_TRACE = 8,    /* fill result.rung_energy_trace */
-    DRIFT_PT_TIME_KERNELS = 16    /* time each sweep-kernel launch (up to 4096) with device events */
+    DRIFT_PT_TIME_KERNELS = 16,   /* time each sweep-kernel launch (up to 4096) with device events */
+    DRIFT_PT_NO_GRAPH = 32        /* launch every round directly instead of replaying a hipGraph */
 };
 
 /* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and

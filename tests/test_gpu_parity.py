@@ -37,7 +37,7 @@ def _announce(name):
 
 
 def compare(model, *, R, n_rounds, S, seed, T_min=0.1, T_max=5.0, beta=None, engine="auto",
-            block=0, force_float=False):
+            block=0, force_float=False, graph=True):
     enc = gpu.encode_problem(model, force_float=force_float)
     k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
     if beta is None:
@@ -45,7 +45,7 @@ def compare(model, *, R, n_rounds, S, seed, T_min=0.1, T_max=5.0, beta=None, eng
     g = gpu.parallel_tempering_gpu(
         model, n_replicas=R, n_rounds=n_rounds, sweeps_per_round=S, seed=seed, betas=beta,
         engine=engine, block_threads=block, coloring=(cptr, cspins), record_history=True,
-        return_final=True, record_trace=True, force_float=force_float)
+        return_final=True, record_trace=True, force_float=force_float, graph=graph)
     o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=seed, R=R, n_rounds=n_rounds,
                               sweeps_per_round=S, beta=beta, n_threads=THREADS, trace=True)
     assert g.best_E_units == o["best_units"]
@@ -118,14 +118,47 @@ def test_config_c1_square_ferromagnet(L):
     compare(square_ferromagnet(L), R=16, n_rounds=400 if L == 4 else 150, S=1, seed=11,
             beta=beta, engine="lds")
     compare(square_ferromagnet(L), R=16, n_rounds=60, S=1, seed=11, beta=beta, engine="global")
+    g = compare(square_ferromagnet(L), R=16, n_rounds=60, S=1, seed=11, beta=beta,
+                engine="lattice")[0]
+    assert g.engine == "lattice"
+
+
+@pytest.mark.parametrize("dims", [(2, 6), (3, 4), (3, 6), (2, 32)])
+def test_lattice_kernel_shapes(dims):
+    dim, L = dims
+    _announce(f"lattice dim={dim} L={L}")
+    model = square_ferromagnet(L) if dim == 2 else cubic_pmj_glass(L, 3)
+    for R in (16, 64, 80):
+        g = compare(model, R=R, n_rounds=40, S=2, seed=R, T_min=0.3, T_max=4.0,
+                    engine="lattice")[0]
+        assert g.engine == "lattice"
+
+
+@pytest.mark.parametrize("engine", ["global", "lattice"])
+def test_direct_launches_equal_graph_replay(engine):
+    _announce(f"direct launches vs graph, {engine}")
+    model = cubic_pmj_glass(8, 5)
+    a = compare(model, R=32, n_rounds=37, S=2, seed=4, engine=engine, graph=False)[0]
+    b = compare(model, R=32, n_rounds=37, S=2, seed=4, engine=engine, graph=True)[0]
+    assert np.array_equal(a.final_spins, b.final_spins)
+
+
+def test_lattice_engine_rejects_non_lattices():
+    _announce("lattice rejects a random graph")
+    with pytest.raises(RuntimeError, match="lattice"):
+        gpu.parallel_tempering_gpu(maxcut_ising(random_graph(16, 0.5, seed=1)), n_replicas=4,
+                                   n_rounds=1, engine="lattice")
 
 
 def test_config_c2_first_thousand_sweeps():
-    _announce("C2 L=32 R=64 1000 sweeps")
+    _announce("C2 L=32 R=64 1000 sweeps, lattice kernel")
     model = cubic_pmj_glass(32, 0)
     beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))     # beta 3.0 .. 0.2
-    g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="global")
-    assert enc.weight_type == 1 and g.engine == "global"
+    g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="auto")
+    assert enc.weight_type == 1 and g.engine == "lattice"
+    _announce("C2 L=32 R=64 100 sweeps, CSR kernel")
+    g2 = compare(model, R=64, n_rounds=100, S=1, seed=1, beta=beta, engine="global")[0]
+    assert g2.engine == "global"
 
 
 def test_batch_equals_oracle_per_instance():
