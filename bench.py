@@ -59,7 +59,7 @@ def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
     neighbours from coordinates and reads one byte holding the site's bond signs."""
     d = model.nnz / model.n
     b_s = 1.0
-    if engine == "lattice":
+    if engine in ("lattice", "slab"):
         return b_s * (d + 1) + b_s + 1.0
     b_w = enc.weight.dtype.itemsize
     b_h = 0.0 if enc.h is None else enc.h.dtype.itemsize
@@ -159,13 +159,13 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "kernel": {"global": "g_color_step", "lattice": "g_lattice_step"}.get(kt.engine,
-                                                                               "pt_lds_kernel"),
+        "kernel": {"global": "g_color_step", "lattice": "g_lattice_step",
+                   "slab": "slab_sweep"}.get(kt.engine, "pt_lds_kernel"),
         "bytes_per_attempt": round(B, 3),
         "attempts_per_launch": per_launch_attempts,
         "avg_launch_us": round(per_launch_s * 1e6, 3),
     }
-    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{kt.engine}.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
@@ -192,6 +192,9 @@ def main() -> int:
             "workload": desc,
             "format": ("int8 spins [position][replica], implicit periodic neighbours, one "
                        "byte of bond signs per site" if res.engine == "lattice" else
+                       "int8 spins [replica][site], slabs of planes staged in LDS, implicit "
+                       "periodic neighbours, one byte of bond signs per site"
+                       if res.engine == "slab" else
                        f"int8 spins [position][replica], CSR int32 columns, "
                        f"{enc.weight.dtype.name} weights"),
             "engine": res.engine,

@@ -318,11 +318,10 @@ void detect_lattice(const drift_pt_problem* P, HostInst& H) {
     if (H.wm != PT_WM_I8 || H.has_h || H.k != 2 || (H.n & 1)) return;
     const int n = H.n;
     for (int dim = 3; dim >= 2; --dim) {
-        int L = (int)std::lround(std::pow((double)n, 1.0 / dim));
+        const int L = (int)std::lround(std::pow((double)n, 1.0 / dim));
         int64_t nn = 1;
         for (int d = 0; d < dim; ++d) nn *= L;
-        if (nn != n || L < 4 || (L & 1)) continue;
-        if (P->nnz != 2 * dim * n) return;
+        if (nn != n || L < 4 || (L & 1) || P->nnz != 2 * dim * n) continue;
         std::vector<uint8_t> jb((size_t)n, 0);
         const int32_t* rp = P->row_ptr;
         const int32_t* ci = P->col_idx;
@@ -345,15 +344,15 @@ void detect_lattice(const drift_pt_problem* P, HostInst& H) {
                 if (w[t] == -1) jb[i] |= (uint8_t)(1u << d);
             }
         }
-        if (!ok) return;
+        if (!ok || H.color_ptr[1] != n / 2) continue;
         // parity colouring in index order
-        if (H.color_ptr[1] != n / 2) return;
         int p0 = 0, p1 = n / 2;
-        for (int i = 0; i < n; ++i) {
+        for (int i = 0; i < n && ok; ++i) {
             const int x = i % L, y = (i / L) % L, z = dim == 3 ? i / (L * L) : 0;
             const int par = (x + y + z) & 1;
-            if (H.orig[par ? p1++ : p0++] != i) return;
+            if (H.orig[par ? p1++ : p0++] != i) ok = false;
         }
+        if (!ok) continue;
         H.ldim = dim;
         H.lL = L;
         H.jbits.swap(jb);
@@ -702,6 +701,130 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
     return 0;
 }
 
+// Slabs per replica for the slab engine: about 256 workgroups in all, each slab thick enough
+// that its two halo planes on either side stay distinct planes (T + 4 <= L).
+int slab_count(const HostInst& H) {
+    int NB = std::max(1, std::min(H.lL, (256 + H.R - 1) / H.R));
+    while (NB > 1 && (H.lL + NB - 1) / NB + 4 > H.lL) --NB;
+    return NB;
+}
+
+bool slab_fits(const HostInst& H) {
+    // power-of-two side (the kernel indexes by shifts) with at least one quad per plane and colour
+    if (!H.ldim || (H.lL & (H.lL - 1)) || (H.ldim == 2 ? H.lL < 8 : H.lL < 4)) return false;
+    const int NB = slab_count(H);
+    return pt_slab_lds_bytes(H.n, H.lL, H.ldim, NB) + 20 * 1024 <= (size_t)device_lds_limit();
+}
+
+int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
+    DevArena A;
+    PtInstance I;
+    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
+    if (rc) return rc;
+    const int Rp = H.Rp, n = H.n;
+    SState S;
+    S.NB = slab_count(H);
+    const size_t lds = pt_slab_lds_bytes(n, H.lL, H.ldim, S.NB);
+    PT_HIP(A.alloc(&S.sp, 2 * (size_t)Rp * n));
+    PT_HIP(A.alloc(&S.Eb, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&S.dacc, 3 * (size_t)Rp));
+    PT_HIP(A.alloc(&S.einit, (size_t)Rp));
+    PT_HIP(A.alloc(&S.rung, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&S.rep, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&S.label, (size_t)Rp));
+    PT_HIP(A.alloc(&S.best, 10));
+    PT_HIP(A.alloc(&S.round_base, 1));
+    StreamGuard sg;
+    PT_HIP(hipStreamCreate(&sg.s));
+    PT_HIP(hipEventCreate(&sg.e0));
+    PT_HIP(hipEventCreate(&sg.e1));
+    PT_HIP(hipMemsetAsync(S.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(S.dacc, 0, 3 * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(S.einit, 0, sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(S.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(S.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(S.label, 0, sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(S.best, 0, 10 * sizeof(int64_t), sg.s));
+    PT_HIP(hipMemsetAsync(S.round_base, 0, sizeof(int32_t), sg.s));
+    PT_HIP(pt_slab_set_lds(H.ldim, S.NB, lds));
+    int block = block_req > 0 ? block_req : 512;
+    block = std::max(64, std::min(512, (block / 64) * 64));
+    PT_HIP(pt_launch_slab_init(I, S, sg.s));
+    const int per_round = std::max(H.S, 1);
+    auto launch_round = [&](int rl) -> int {
+        if (H.S == 0) {
+            PT_HIP(pt_launch_slab_sweep(I, S, 0, rl, 0, block, lds, sg.s));
+            return 0;
+        }
+        for (int j = 0; j < H.S; ++j) PT_HIP(pt_launch_slab_sweep(I, S, j, rl, 1, block, lds, sg.s));
+        return 0;
+    };
+    GraphGuard gg;
+    int chunk = 0;
+    if (!(H.flags & DRIFT_PT_NO_GRAPH) && H.n_rounds >= 4) {
+        chunk = std::max(1, std::min(H.n_rounds, 1024 / per_round));
+        PT_HIP(hipStreamBeginCapture(sg.s, hipStreamCaptureModeThreadLocal));
+        for (int rl = 0; rl < chunk; ++rl) {
+            rc = launch_round(rl);
+            if (rc) {
+                hipGraph_t junk = nullptr;
+                (void)hipStreamEndCapture(sg.s, &junk);
+                if (junk) (void)hipGraphDestroy(junk);
+                return rc;
+            }
+        }
+        PT_HIP(pt_launch_slab_tick(S, chunk, sg.s));
+        PT_HIP(hipStreamEndCapture(sg.s, &gg.graph));
+        PT_HIP(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0));
+    }
+    const int replays = chunk ? H.n_rounds / chunk : 0;
+    const int rest = H.n_rounds - replays * chunk;
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    for (int g = 0; g < replays; ++g) PT_HIP(hipGraphLaunch(gg.exec, sg.s));
+    for (int rl = 0; rl < rest; ++rl) {
+        rc = launch_round(rl);
+        if (rc) return rc;
+    }
+    PT_HIP(pt_launch_slab_sweep(I, S, 0, rest, 0, block, lds, sg.s));   // end of the last round
+    PT_HIP(hipEventRecord(sg.e1, sg.s));
+    PT_HIP(hipStreamSynchronize(sg.s));
+    float ms = 0.0f;
+    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+
+    const int fin = H.n_rounds & 1;
+    std::vector<int64_t> out((size_t)pt_out_len(H.R));
+    PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t best[5];
+    PT_HIP(hipMemcpy(best, S.best + 5 * fin, sizeof(best), hipMemcpyDeviceToHost));
+    std::vector<int64_t> E((size_t)Rp);
+    std::vector<int32_t> rung((size_t)Rp);
+    PT_HIP(hipMemcpy(E.data(), S.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(rung.data(), S.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
+    for (int x = 0; x < 5; ++x) out[x] = best[x];
+    for (int r = 0; r < H.R; ++r) {
+        pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
+        pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
+    }
+    std::vector<int8_t> fs;
+    if (H.flags & DRIFT_PT_RECORD_FINAL) {
+        const int buf = (int)(((int64_t)H.n_rounds * H.S) & 1);
+        std::vector<int8_t> slot_major((size_t)n * Rp);
+        PT_HIP(hipMemcpy(slot_major.data(), S.sp + (size_t)buf * Rp * n, slot_major.size(),
+                         hipMemcpyDeviceToHost));
+        fs.resize((size_t)n * Rp);
+        for (int p = 0; p < n; ++p)
+            for (int a = 0; a < H.R; ++a) fs[(size_t)p * Rp + a] = slot_major[(size_t)a * n + H.orig[p]];
+    }
+    rc = fetch_results(H, I, out, fs, res);
+    if (rc) return rc;
+    res->seconds = ms * 1e-3;
+    res->engine_used = DRIFT_PT_ENGINE_SLAB;
+    res->kernel_seconds = res->seconds;   // one sweep kernel per sweep carries all the work
+    res->kernel_attempts = res->attempts;
+    res->kernel_launches = (int64_t)H.n_rounds * per_round + 1;
+    return 0;
+}
+
 // LDS engine fit: the dynamic area plus ~1 KB of the kernel's static shared variables.
 bool fits_lds(const HostInst& H) {
     return pt_lds_bytes(H.n, H.Rp) + 1024 <= (size_t)device_lds_limit();
@@ -756,7 +879,15 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     int engine = params->engine;
     if (engine == DRIFT_PT_ENGINE_AUTO) {
         if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) engine = DRIFT_PT_ENGINE_LDS;
-        else engine = H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+        else if (H.ldim) engine = slab_fits(H) ? DRIFT_PT_ENGINE_SLAB : DRIFT_PT_ENGINE_LATTICE;
+        else engine = DRIFT_PT_ENGINE_GLOBAL;
+    }
+    if (engine == DRIFT_PT_ENGINE_SLAB) {
+        if (!slab_fits(H))
+            return fail(DRIFT_PT_EINVAL, "slab engine needs a periodic L^2/L^3 lattice (L even >= 4) "
+                                         "with +/-1 couplings, no field, the parity colouring, and "
+                                         "slabs that fit LDS");
+        return run_slab(H, result, params->block_threads);
     }
     if (engine == DRIFT_PT_ENGINE_LATTICE) {
         if (!H.ldim)

@@ -61,6 +61,32 @@ struct GState {
                           // device by pt_launch_g_tick so a captured chunk of rounds can replay
 };
 
+// Slab-engine state: a periodic lattice split into slabs of planes along its slowest axis; one
+// workgroup owns one slab of one replica per sweep.  Spins are double-buffered by sweep parity,
+// energy changes rotate through three accumulators by round, and the round-end bookkeeping of
+// round r-1 runs at the start of round r's first sweep kernel.
+struct SState {
+    int8_t* sp;        // 2 * Rp * n: [buffer][slot][site]
+    int64_t* Eb;       // 2 * Rp: energy at the start of round r, index r & 1
+    int64_t* dacc;     // 3 * Rp: energy change of round r, index r % 3
+    int64_t* einit;    // Rp: initial -2E (sum of s_i * local_i) per slot
+    int32_t* rung;     // 2 * Rp: slot -> rung during round r, index r & 1
+    int32_t* rep;      // 2 * Rp: rung -> slot
+    int32_t* label;    // Rp
+    int64_t* best;     // 2 * 5: best after the round-end scan of round r-1, index r & 1
+    int32_t* round_base;
+    int32_t NB;        // slabs per replica (1: the whole lattice wraps inside one workgroup)
+};
+
+size_t pt_slab_lds_bytes(int n, int L, int dim, int NB);
+hipError_t pt_launch_slab_init(const PtInstance& I, const SState& S, hipStream_t s);
+// Sweep j of round (*round_base + rl); j == 0 first finishes round rl-1.  sweep == 0 runs only
+// that round end (the final one).
+hipError_t pt_launch_slab_sweep(const PtInstance& I, const SState& S, int j, int rl, int sweep,
+                                int block, size_t lds, hipStream_t s);
+hipError_t pt_launch_slab_tick(const SState& S, int rounds, hipStream_t s);
+hipError_t pt_slab_set_lds(int dim, int NB, size_t lds);
+
 size_t pt_lds_bytes(int n, int Rp);
 
 hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int n_inst,

@@ -1,0 +1,360 @@
+// pt_slab.hip -- slab engine: one kernel per sweep for periodic lattices with +/-1 bonds.
+//
+// The lattice is cut into NB slabs of planes along its slowest axis (z in 3-D, y in 2-D) and one
+// workgroup owns one slab of one replica.  It stages its planes plus two halo planes on each side
+// in LDS, updates colour 0 on its planes and on one halo plane per side, then colour 1 on its own
+// planes, and writes its planes to the other spin buffer.  The halo updates repeat exactly what the
+// neighbouring workgroup computes for those sites -- the random stream is addressed by
+// (position, replica, sweep), not by who computes it -- so no workgroup ever waits for another
+// inside a launch, and the sweep needs one launch instead of one per colour.
+//
+// The first sweep kernel of round r also finishes round r-1: every workgroup recomputes the
+// energies, the best-rung scan and the exchange decisions from a few hundred bytes and takes its
+// own replica's new rung; workgroup 0 alone writes the bookkeeping.  Workgroups of one replica are
+// placed on one XCD (blockIdx mod 8) so their halo reads hit the same L2.
+//
+// Positions follow the parity colouring in index order: position p of colour c is the site 2r or
+// 2r+1 (r = p - c N/2) whose coordinate parity is c.  Reference counterpart: the sweep and swap
+// kernels of cuda/ising_pt.cu:86-146.
+#include "pt_device.h"
+#include "pt_engine.h"
+#include "pt_rules.h"
+
+#include <algorithm>
+
+namespace {
+
+using namespace pt_dev;
+
+struct Geo {
+    int L, P, N, half, hp;   // side, sites per plane, sites, N/2, P/2
+};
+
+template <int DIM>
+__device__ __forceinline__ Geo make_geo(const PtInstance& I) {
+    Geo g;
+    g.L = I.lL;
+    g.N = I.n;
+    g.P = DIM == 3 ? g.L * g.L : g.L;
+    g.half = g.N >> 1;
+    g.hp = g.P >> 1;
+    return g;
+}
+
+// site holding position p of colour c
+template <int DIM>
+__device__ __forceinline__ int site_of(int p, int c, const Geo& g) {
+    const int s0 = 2 * (p - c * g.half);
+    const int yz = DIM == 3 ? (s0 / g.L) % g.L + s0 / g.P : s0 / g.L;
+    return s0 + ((yz + c) & 1);
+}
+
+template <int DIM>
+__device__ __forceinline__ int pos_of(int site, const Geo& g) {
+    const int par = DIM == 3 ? (site % g.L + (site / g.L) % g.L + site / g.P) & 1
+                             : (site % g.L + site / g.L) & 1;
+    return par * g.half + (site >> 1);
+}
+
+// workgroup -> (replica slot, slab); the NB slabs of one slot share blockIdx mod 8
+__device__ __forceinline__ void wg_slot_slab(int w, int NB, int& a, int& b) {
+    const int k = w >> 3;
+    a = (k / NB) * 8 + (w & 7);
+    b = k % NB;
+}
+
+// Round end of round rnd-1 (or the initial state when rnd == 0), evaluated by every workgroup.
+// Returns the rung of slot a for round rnd.
+template <int DIM>
+__device__ int slab_round_end(const PtInstance& I, const SState& S, const Geo& g, int rnd, int a,
+                              const int8_t* spins, int z0, int z1, int64_t* Es, int32_t* reps,
+                              int32_t* rungs, long long* scrE, int* scrR) {
+    const int R = I.R, Rp = I.Rp, tid = threadIdx.x, nt = blockDim.x;
+    const bool init = rnd == 0, owner = blockIdx.x == 0;
+    const int prev = rnd - 1, pin = prev & 1, din = (prev % 3 + 3) % 3;
+    for (int x = tid; x < R; x += nt) {
+        if (init) {
+            Es[x] = -(S.einit[x] / 2);
+            reps[x] = x;
+            rungs[x] = x;
+        } else {
+            Es[x] = S.Eb[(size_t)pin * Rp + x] + S.dacc[(size_t)din * Rp + x];
+            reps[x] = S.rep[(size_t)pin * Rp + x];
+            rungs[x] = S.rung[(size_t)pin * Rp + x];
+        }
+    }
+    __syncthreads();
+    int64_t mE;
+    int32_t mR;
+    block_best_rung(Es, reps, R, scrE, scrR, mE, mR);
+    const int64_t* bc = S.best + 5 * pin;
+    const bool have = !init && bc[4] != 0;
+    const bool improved = !have || mE < bc[0];
+    const int slot = reps[mR];
+    if (improved && slot == a)
+        for (int site = z0 * g.P + tid; site < z1 * g.P; site += nt)
+            I.best_s[pos_of<DIM>(site, g)] = spins[site];
+    int64_t* out = I.out;
+    if (owner) {
+        if (tid == 0) {
+            int64_t* bn = S.best + 5 * (rnd & 1);
+            if (improved) {
+                bn[0] = mE; bn[1] = rnd; bn[2] = mR; bn[3] = slot; bn[4] = 1;
+            } else {
+                for (int x = 0; x < 5; ++x) bn[x] = bc[x];
+            }
+            if (!init && I.hist) I.hist[prev] = (double)(improved ? mE : bc[0]) * I.unit;
+        }
+        if (!init && I.trace)
+            for (int r = tid; r < R; r += nt) I.trace[(size_t)prev * R + r] = (double)Es[reps[r]] * I.unit;
+    }
+    if (!init) {
+        __syncthreads();
+        exchange_pairs(I, prev, Es, reps, rungs, owner ? pt_out_arr(out, R, PT_OUT_TRIED) : nullptr,
+                       owner ? pt_out_arr(out, R, PT_OUT_ACC) : nullptr);
+        __syncthreads();
+    }
+    if (owner) {
+        if (init) {
+            for (int x = tid; x < R; x += nt)
+                S.label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
+        } else {
+            update_labels(R, rungs, S.label, nullptr, pt_out_arr(out, R, PT_OUT_TRIPS),
+                          pt_out_arr(out, R, PT_OUT_UP), pt_out_arr(out, R, PT_OUT_DOWN));
+        }
+        const int cur = rnd & 1, zero = (rnd + 1) % 3;
+        for (int x = tid; x < R; x += nt) {
+            S.Eb[(size_t)cur * Rp + x] = Es[x];
+            S.rung[(size_t)cur * Rp + x] = rungs[x];
+            S.rep[(size_t)cur * Rp + x] = reps[x];
+            S.dacc[(size_t)zero * Rp + x] = 0;
+        }
+    }
+    const int mine = rungs[a];
+    __syncthreads();
+    return mine;
+}
+
+template <int DIM, bool WRAP>
+__global__ __launch_bounds__(512) void slab_sweep(PtInstance I, SState S, int j, int rl,
+                                                  int do_sweep) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lsp[];
+    __shared__ int64_t Es[1024];
+    __shared__ int32_t reps[1024];
+    __shared__ int32_t rungs[1024];
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    int a, b;
+    wg_slot_slab(blockIdx.x, S.NB, a, b);
+    if (a >= I.R) return;   // whole workgroup: padding slots of the grid
+    const Geo g = make_geo<DIM>(I);
+    const int L = g.L, P = g.P, tid = threadIdx.x, nt = blockDim.x;
+    const int Lb = 31 - __builtin_clz(L);                // L is a power of two
+    const int Pb = DIM == 3 ? 2 * Lb : Lb;
+    const int z0 = b * L / S.NB, z1 = (b + 1) * L / S.NB, T = z1 - z0;
+    const int rnd = *S.round_base + rl;
+    const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
+    const int8_t* in = S.sp + ((size_t)(t & 1) * I.Rp + a) * g.N;
+    int8_t* outb = S.sp + ((size_t)((t + 1) & 1) * I.Rp + a) * g.N;
+    const int g_rung = j == 0 ? slab_round_end<DIM>(I, S, g, rnd, a, in, z0, z1, Es, reps, rungs,
+                                                     scrE, scrR)
+                              : S.rung[(size_t)(rnd & 1) * I.Rp + a];
+    if (!do_sweep) return;
+
+    // stage spins of slots 0..nslots-1 and bond signs of the updated slots 1..T+2 (all slots when
+    // the lattice wraps inside the workgroup); slot s holds plane (WRAP ? s : z0 - 2 + s) mod L
+    const int nslots = WRAP ? L : T + 4;
+    const int jlo = WRAP ? 0 : 1, njs = WRAP ? L : T + 2;
+    int8_t* ljb = lsp + (size_t)nslots * P;
+    const int wpp = P >> 2;
+    for (int idx = tid; idx < nslots * wpp; idx += nt) {
+        const int s = idx >> (Pb - 2), w = idx & (wpp - 1);
+        const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
+        ((uint32_t*)lsp)[idx] = ((const uint32_t*)(in + ((size_t)plane << Pb)))[w];
+    }
+    for (int idx = tid; idx < njs * wpp; idx += nt) {
+        const int s = jlo + (idx >> (Pb - 2)), w = idx & (wpp - 1);
+        const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
+        ((uint32_t*)ljb)[idx] = ((const uint32_t*)(I.jbits + ((size_t)plane << Pb)))[w];
+    }
+    const uint32_t* tr = I.thr + (size_t)g_rung * I.TQ;
+    const uint32_t t2 = I.TQ >= 2 ? tr[1] : 0u;
+    const uint32_t t4 = I.TQ >= 4 ? tr[3] : 0u;
+    const uint32_t t6 = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
+    __syncthreads();
+
+    // Quad k of plane `plane`, colour c holds positions c N/2 + plane P/2 + 4k .. +3, i.e. the
+    // colour-c site of each of the site pairs (plane P + 8k + 2kk, +1), kk = 0..3.
+    int64_t dsum = 0;
+    const int Qb = Pb - 3;                   // log2 of quads per plane and colour (P/8)
+    for (int c = 0; c < 2; ++c) {
+        const int s_lo = WRAP ? 0 : (c == 0 ? 1 : 2);
+        const int ns = WRAP ? L : (c == 0 ? T + 2 : T);
+        for (int w = tid; w < (ns << Qb); w += nt) {
+            const int s = s_lo + (w >> Qb), k = w & ((1 << Qb) - 1);
+            const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
+            const bool owned = WRAP || (s >= 2 && s < T + 2);
+            const int su = WRAP ? ((s + 1) & (L - 1)) : s + 1;
+            const int sd = WRAP ? ((s - 1) & (L - 1)) : s - 1;
+            const uint32_t q = (uint32_t)((c * g.half + (plane << (Pb - 1))) >> 2) + (uint32_t)k;
+            const pt_u4 rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+            const int8_t* ls = lsp + ((size_t)s << Pb);
+            const int8_t* lu = lsp + ((size_t)su << Pb);
+            const int8_t* ld = lsp + ((size_t)sd << Pb);
+            const int8_t* lj = ljb + ((size_t)(s - jlo) << Pb);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int loc0 = 8 * k + 2 * kk;              // even site of the pair, in plane
+                const int y = DIM == 3 ? loc0 >> Lb : plane;
+                const int par = DIM == 3 ? ((y + plane + c) & 1) : ((plane + c) & 1);
+                const int loc = loc0 + par;
+                const int x = loc & (L - 1);
+                const int rowb = loc - x;
+                const uint32_t jb = (uint8_t)lj[loc];
+                const int n0 = ls[rowb + ((x + 1) & (L - 1))];
+                const int n1 = ls[rowb + ((x - 1) & (L - 1))];
+                int l = ((jb & 1u) ? -n0 : n0) + ((jb & 2u) ? -n1 : n1);
+                if (DIM == 3) {
+                    const int n2 = ls[(((y + 1) & (L - 1)) << Lb) + x];
+                    const int n3 = ls[(((y - 1) & (L - 1)) << Lb) + x];
+                    const int n4 = lu[loc];
+                    const int n5 = ld[loc];
+                    l += ((jb & 4u) ? -n2 : n2) + ((jb & 8u) ? -n3 : n3) +
+                         ((jb & 16u) ? -n4 : n4) + ((jb & 32u) ? -n5 : n5);
+                } else {
+                    const int n2 = lu[loc];
+                    const int n3 = ld[loc];
+                    l += ((jb & 4u) ? -n2 : n2) + ((jb & 8u) ? -n3 : n3);
+                }
+                int8_t* me = (int8_t*)ls + loc;
+                const int8_t si = *me;
+                const int qv = si > 0 ? l : -l;
+                const uint32_t th = qv == 2 ? t2 : (qv == 4 ? t4 : t6);
+                if (qv <= 0 || pt_word(rv, kk) < th) {
+                    *me = (int8_t)-si;
+                    if (owned) dsum += 2 * qv;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // own planes to the other buffer
+    const int so = WRAP ? 0 : 2;
+    for (int idx = tid; idx < T * wpp; idx += nt) {
+        const int s = idx >> (Pb - 2), w = idx & (wpp - 1);
+        ((uint32_t*)(outb + ((size_t)(z0 + s) << Pb)))[w] = ((const uint32_t*)lsp)[((s + so) << (Pb - 2)) + w];
+    }
+    // energy change of the owned sites: one atomic per workgroup
+    long long v = (long long)dsum;
+#pragma unroll
+    for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    if ((tid & (PT_WAVE - 1)) == 0) scrE[tid / PT_WAVE] = v;
+    __syncthreads();
+    if (tid == 0) {
+        long long tot = 0;
+        for (int w = 0; w < (nt + PT_WAVE - 1) / PT_WAVE; ++w) tot += scrE[w];
+        if (tot != 0)
+            atomicAdd((unsigned long long*)&S.dacc[(size_t)(rnd % 3) * I.Rp + a], (unsigned long long)tot);
+    }
+}
+
+// initial spins of every slot (buffer 0, site order) from the stream contract
+__global__ __launch_bounds__(256) void slab_init_spins(PtInstance I, SState S) {
+    const int64_t total = (int64_t)I.R * I.n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(idx / I.n), site = (int)(idx - (int64_t)a * I.n);
+        const pt_u4 r = pt_philox((uint32_t)(site >> 2), (uint32_t)a, 0u, PT_TAG_INIT, I.k0, I.k1);
+        S.sp[(size_t)a * I.n + site] = (pt_word(r, site & 3) & 1u) ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
+// einit[a] = sum_i s_i sum_j J_ij s_j over all sites (E = -einit/2); one slot per block row
+template <int DIM>
+__global__ __launch_bounds__(256) void slab_init_energy(PtInstance I, SState S) {
+    __shared__ long long scr[8];
+    const int a = blockIdx.y;
+    const int L = I.lL, P = DIM == 3 ? L * L : L;
+    const int8_t* sp = S.sp + (size_t)a * I.n;
+    long long acc = 0;
+    for (int site = blockIdx.x * blockDim.x + threadIdx.x; site < I.n; site += gridDim.x * blockDim.x) {
+        const int x = site % L, y = (site / L) % L, z = DIM == 3 ? site / P : 0;
+        const int base = DIM == 3 ? z * P : 0;
+        const uint32_t jb = I.jbits[site];
+        int nb[6];
+        nb[0] = base + y * L + (x + 1) % L;
+        nb[1] = base + y * L + (x + L - 1) % L;
+        nb[2] = base + ((y + 1) % L) * L + x;
+        nb[3] = base + ((y + L - 1) % L) * L + x;
+        if (DIM == 3) {
+            nb[4] = ((z + 1) % L) * P + y * L + x;
+            nb[5] = ((z + L - 1) % L) * P + y * L + x;
+        }
+        int l = 0;
+        for (int d = 0; d < 2 * DIM; ++d) {
+            const int v = sp[nb[d]];
+            l += ((jb >> d) & 1u) ? -v : v;
+        }
+        acc += sp[site] > 0 ? l : -l;
+    }
+#pragma unroll
+    for (int m = PT_WAVE / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
+    if ((threadIdx.x & (PT_WAVE - 1)) == 0) scr[threadIdx.x / PT_WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tot = 0;
+        for (int w = 0; w < (int)(blockDim.x / PT_WAVE); ++w) tot += scr[w];
+        atomicAdd((unsigned long long*)&S.einit[a], (unsigned long long)tot);
+    }
+}
+
+__global__ void slab_tick(int32_t* round_base, int rounds) {
+    if (threadIdx.x == 0) *round_base += rounds;
+}
+
+}  // namespace
+
+size_t pt_slab_lds_bytes(int n, int L, int dim, int NB) {
+    (void)n;
+    const size_t P = dim == 3 ? (size_t)L * L : (size_t)L;
+    const size_t T = (size_t)((L + NB - 1) / NB);
+    const size_t spins = NB == 1 ? (size_t)L : T + 4;     // staged spin planes
+    const size_t signs = NB == 1 ? (size_t)L : T + 2;     // bond-sign planes of updated slots
+    return ((spins + signs) * P + 15) & ~(size_t)15;
+}
+
+hipError_t pt_launch_slab_init(const PtInstance& I, const SState& S, hipStream_t s) {
+    const int64_t total = (int64_t)I.R * I.n;
+    slab_init_spins<<<(int)std::min<int64_t>((total + 255) / 256, 4096), 256, 0, s>>>(I, S);
+    const dim3 grid((unsigned)std::min(64, (I.n + 255) / 256), (unsigned)I.R);
+    if (I.ldim == 3) slab_init_energy<3><<<grid, 256, 0, s>>>(I, S);
+    else slab_init_energy<2><<<grid, 256, 0, s>>>(I, S);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_slab_sweep(const PtInstance& I, const SState& S, int j, int rl, int sweep,
+                                int block, size_t lds, hipStream_t s) {
+    const int grid = ((I.R + 7) / 8) * 8 * S.NB;
+    const bool wrap = S.NB == 1;
+    if (I.ldim == 3) {
+        if (wrap) slab_sweep<3, true><<<grid, block, lds, s>>>(I, S, j, rl, sweep);
+        else slab_sweep<3, false><<<grid, block, lds, s>>>(I, S, j, rl, sweep);
+    } else {
+        if (wrap) slab_sweep<2, true><<<grid, block, lds, s>>>(I, S, j, rl, sweep);
+        else slab_sweep<2, false><<<grid, block, lds, s>>>(I, S, j, rl, sweep);
+    }
+    return hipGetLastError();
+}
+
+hipError_t pt_slab_set_lds(int dim, int NB, size_t lds) {
+    const bool wrap = NB == 1;
+    const void* fn = dim == 3 ? (wrap ? (const void*)slab_sweep<3, true> : (const void*)slab_sweep<3, false>)
+                              : (wrap ? (const void*)slab_sweep<2, true> : (const void*)slab_sweep<2, false>);
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t pt_launch_slab_tick(const SState& S, int rounds, hipStream_t s) {
+    slab_tick<<<1, 64, 0, s>>>(S.round_base, rounds);
+    return hipGetLastError();
+}

@@ -46,8 +46,10 @@ enum drift_pt_engine {
     DRIFT_PT_ENGINE_AUTO = 0,
     DRIFT_PT_ENGINE_LDS = 1,    /* whole instance (all replicas) in one workgroup's LDS, persistent */
     DRIFT_PT_ENGINE_GLOBAL = 2, /* replica-interleaved spins in HBM, one launch per colour step */
-    DRIFT_PT_ENGINE_LATTICE = 3 /* the global engine's periodic-lattice kernel: implicit neighbours,
+    DRIFT_PT_ENGINE_LATTICE = 3,/* the global engine's periodic-lattice kernel: implicit neighbours,
                                    one byte of bond signs per site (+/-1 couplings, no field) */
+    DRIFT_PT_ENGINE_SLAB = 4    /* periodic lattices: one launch per sweep, a workgroup per slab of
+                                   planes of one replica, halo planes recomputed in LDS */
 };
 
 enum drift_pt_flags {

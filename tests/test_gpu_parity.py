@@ -123,24 +123,24 @@ def test_config_c1_square_ferromagnet(L):
     assert g.engine == "lattice"
 
 
-@pytest.mark.parametrize("dims", [(2, 6), (3, 4), (3, 6), (2, 32)])
-def test_lattice_kernel_shapes(dims):
-    dim, L = dims
-    _announce(f"lattice dim={dim} L={L}")
+@pytest.mark.parametrize("case", [("lattice", 2, 6), ("lattice", 3, 4), ("lattice", 3, 6),
+                                  ("lattice", 2, 32), ("slab", 2, 8), ("slab", 3, 4),
+                                  ("slab", 3, 8), ("slab", 3, 16), ("slab", 2, 32)])
+def test_lattice_kernel_shapes(case):
+    engine, dim, L = case
+    _announce(f"{engine} dim={dim} L={L}")
     model = square_ferromagnet(L) if dim == 2 else cubic_pmj_glass(L, 3)
-    for R in (16, 64, 80):
-        g = compare(model, R=R, n_rounds=40, S=2, seed=R, T_min=0.3, T_max=4.0,
-                    engine="lattice")[0]
-        assert g.engine == "lattice"
+    for R, S in ((16, 2), (64, 1), (80, 3), (5, 1)):
+        g = compare(model, R=R, n_rounds=40, S=S, seed=R, T_min=0.3, T_max=4.0, engine=engine)[0]
+        assert g.engine == engine
 
 
-@pytest.mark.parametrize("engine", ["global", "lattice"])
-def test_direct_launches_equal_graph_replay(engine):
-    _announce(f"direct launches vs graph, {engine}")
-    model = cubic_pmj_glass(8, 5)
-    a = compare(model, R=32, n_rounds=37, S=2, seed=4, engine=engine, graph=False)[0]
-    b = compare(model, R=32, n_rounds=37, S=2, seed=4, engine=engine, graph=True)[0]
-    assert np.array_equal(a.final_spins, b.final_spins)
+@pytest.mark.parametrize("engine", ["lds", "global", "lattice", "slab"])
+def test_zero_sweeps_per_round_only_exchanges(engine):
+    _announce(f"S=0 {engine}")
+    model = cubic_pmj_glass(8, 2)
+    g = compare(model, R=8, n_rounds=9, S=0, seed=3, T_min=0.5, T_max=2.0, engine=engine)[0]
+    assert g.best_round == 0
 
 
 def test_lattice_engine_rejects_non_lattices():
@@ -148,17 +148,20 @@ def test_lattice_engine_rejects_non_lattices():
     with pytest.raises(RuntimeError, match="lattice"):
         gpu.parallel_tempering_gpu(maxcut_ising(random_graph(16, 0.5, seed=1)), n_replicas=4,
                                    n_rounds=1, engine="lattice")
+    with pytest.raises(RuntimeError, match="slab"):
+        gpu.parallel_tempering_gpu(square_ferromagnet(6), n_replicas=4, n_rounds=1, engine="slab")
 
 
 def test_config_c2_first_thousand_sweeps():
-    _announce("C2 L=32 R=64 1000 sweeps, lattice kernel")
+    _announce("C2 L=32 R=64 1000 sweeps, slab engine")
     model = cubic_pmj_glass(32, 0)
     beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))     # beta 3.0 .. 0.2
     g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="auto")
-    assert enc.weight_type == 1 and g.engine == "lattice"
-    _announce("C2 L=32 R=64 100 sweeps, CSR kernel")
-    g2 = compare(model, R=64, n_rounds=100, S=1, seed=1, beta=beta, engine="global")[0]
-    assert g2.engine == "global"
+    assert enc.weight_type == 1 and g.engine == "slab"
+    for engine in ("lattice", "global"):
+        _announce(f"C2 L=32 R=64 100 sweeps, {engine}")
+        g2 = compare(model, R=64, n_rounds=100, S=1, seed=1, beta=beta, engine=engine)[0]
+        assert g2.engine == engine
 
 
 def test_batch_equals_oracle_per_instance():
