@@ -41,6 +41,10 @@ def workload(name: str):
         model = square_ferromagnet(16)
         R, beta = 16, ladder_betas(geometric_ladder(16, 1.0, 10.0))
         desc = "2-D ferromagnet, L=16 periodic (N=256), R=16, beta 0.1..1.0, exchange every sweep"
+    elif name == "c4":
+        return None, 16, None, ("4096 random 3-regular instances, N uniform over even values in "
+                                "[64, 2048], J ~ N(0,1) and h ~ N(0,0.1^2) in float32, R=16 each, "
+                                "beta 0.2..5.0 geometric, all in one persistent batched launch")
     elif name == "c3":
         model = er_sparse_weighted(100000, 6.0, 0)
         R, beta = 32, ladder_betas(geometric_ladder(32, 0.2, 20.0))
@@ -81,6 +85,73 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
+def c4_instances(count: int, seed: int = 0):
+    """The batched workload: `count` random 3-regular float instances of uneven size."""
+    from drift_amd.builders.lattice import random_regular_gaussian
+
+    rng = np.random.default_rng(seed)
+    sizes = 2 * rng.integers(32, 1025, size=count)
+    return [random_regular_gaussian(int(n), seed=10_000 + i) for i, n in enumerate(sizes)]
+
+
+def bench_c4(args) -> int:
+    """Batched uneven instances: whole-job attempts/ns of one persistent launch."""
+    from drift_amd import gpu
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    count = args.instances
+    print(f"[bench] building {count} instances", file=sys.stderr, flush=True)
+    models = c4_instances(count)
+    R = 16
+    kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, sweeps_per_round=1)
+    print(f"[bench] warm-up {args.warmup} steps", file=sys.stderr, flush=True)
+    gpu.parallel_tempering_gpu_batch(models, n_rounds=args.warmup, **kw)
+    print(f"[bench] timed {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    res = gpu.parallel_tempering_gpu_batch(models, n_rounds=args.steps, **kw)
+    wall = time.perf_counter() - t0
+    total_n = sum(m.n for m in models)
+    attempts = R * total_n * args.steps
+    secs = res[0].seconds
+    value = attempts / secs / 1e9
+    d = 3.0
+    B = (d + 1) + 1 + d * (4 + 4) + 8 + 4          # int8 spins, fp32 J + int32 column, row ptrs, fp32 h
+    roofline = {"bound": "hbm", "achieved": round(B * attempts / secs / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(B * attempts / secs / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel": "pt_lds_kernel", "bytes_per_attempt": B,
+                "attempts_per_launch": attempts, "avg_launch_us": round(secs * 1e6, 1)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        cores = len(os.sched_getaffinity(0))
+        sample = models[:64]
+        beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+        tot_s, tot_a = 0.0, 0
+        for m in sample:
+            enc = gpu.encode_problem(m)
+            k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+            o = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=50, sweeps_per_round=1,
+                                      beta=beta, n_threads=cores)
+            tot_s += o["seconds"]
+            tot_a += R * m.n * 50
+        cpu = {"value": tot_a / tot_s / 1e9, "unit": "attempts/ns", "cores": cores, "kind": "port",
+               "sample": f"first 64 instances of the batch, 50 sweeps each, one at a time "
+                         f"(OpenMP over replicas; {tot_a:.3g} attempts, {tot_s:.1f} s)"}
+    best = np.array([r.best_E for r in res])
+    line = {
+        "metric": "spin-flips/ns on 1 GPU", "value": round(value, 3), "unit": "attempts/ns",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3 / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload("c4")[3], "instances": count, "total_spins": int(total_n),
+                   "engine": res[0].engine, "wall_s": round(wall, 3),
+                   "mean_best_E_per_spin": round(float((best / [m.n for m in models]).mean()), 5)},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def cpu_baseline(model, enc, R, beta, budget_s: float):
     import oracle
 
@@ -114,6 +185,7 @@ def main() -> int:
     ap.add_argument("--kernel-timing-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (rocprof counter scaling)")
     args = ap.parse_args()
@@ -130,6 +202,8 @@ def main() -> int:
         print("bench.py: GPU engine not available (no gfx950 device or library not built)",
               file=sys.stderr)
         return 1
+    if args.workload == "c4":
+        return bench_c4(args)
     model, R, beta, desc = workload(args.workload)
     enc = gpu.encode_problem(model)
     kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,

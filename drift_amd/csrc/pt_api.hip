@@ -27,6 +27,13 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+int pt_fail(int code, const char* msg) { return fail(code, msg); }
+void pt_clear_error() { g_err.clear(); }
+
+namespace {
+
 #define PT_HIP(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
@@ -390,50 +397,105 @@ int check_device() {
 }
 
 // Device copies of one instance's arrays, and the PtInstance pointing at them.
+// Device copies of many instances' arrays in one allocation: every input array is packed into a
+// host staging blob at 256-byte aligned offsets and copied once; outputs get their own aligned
+// regions in the same allocation (statistics blocks cleared by one memset).
+struct Packer {
+    std::vector<uint8_t> blob;
+    size_t add(const void* src, size_t bytes) {
+        const size_t off = (blob.size() + 255) & ~(size_t)255;
+        blob.resize(off + std::max<size_t>(bytes, 1), 0);
+        if (bytes) std::memcpy(blob.data() + off, src, bytes);
+        return off;
+    }
+    template <class T>
+    size_t add(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
+};
+
+int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
+                bool force_final) {
+    const size_t m = hs.size();
+    struct Off { size_t rp, col, cptr, orig, w, h, beta, thr, jb; };
+    std::vector<Off> off(m);
+    Packer in;
+    for (size_t i = 0; i < m; ++i) {
+        const HostInst& H = hs[i];
+        Off& o = off[i];
+        o.rp = in.add(H.row_ptr);
+        o.col = in.add(H.col);
+        o.cptr = in.add(H.color_ptr);
+        o.orig = in.add(H.orig);
+        o.w = H.wm == PT_WM_I8 ? in.add(H.w8) : (H.wm == PT_WM_I32 ? in.add(H.w32) : in.add(H.wf));
+        o.h = !H.has_h ? SIZE_MAX : (H.wm == PT_WM_F32 ? in.add(H.hf) : in.add(H.h32));
+        o.beta = in.add(H.beta);
+        o.thr = H.wm != PT_WM_F32 ? in.add(H.thr) : SIZE_MAX;
+        o.jb = H.ldim ? in.add(H.jbits) : SIZE_MAX;
+    }
+    // outputs after the inputs
+    struct OutOff { size_t best, fin, hist, trace, out; };
+    std::vector<OutOff> oo(m);
+    size_t cursor = (in.blob.size() + 255) & ~(size_t)255;
+    const size_t out_begin = cursor;
+    auto take = [&](size_t bytes) {
+        const size_t at = cursor;
+        cursor = (cursor + std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+        return at;
+    };
+    for (size_t i = 0; i < m; ++i) {
+        const HostInst& H = hs[i];
+        oo[i].out = take(sizeof(int64_t) * (size_t)pt_out_len(H.R));
+    }
+    const size_t out_end = cursor;   // statistics blocks are the region that must start at zero
+    for (size_t i = 0; i < m; ++i) {
+        const HostInst& H = hs[i];
+        oo[i].best = take((size_t)H.n);
+        const bool fin = force_final || (H.flags & DRIFT_PT_RECORD_FINAL);
+        oo[i].fin = fin ? take((size_t)H.n * H.R) : SIZE_MAX;
+        oo[i].hist = ((H.flags & DRIFT_PT_RECORD_HISTORY) && H.n_rounds > 0)
+                         ? take(sizeof(double) * (size_t)H.n_rounds) : SIZE_MAX;
+        oo[i].trace = ((H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
+                          ? take(sizeof(double) * (size_t)H.n_rounds * H.R) : SIZE_MAX;
+    }
+    uint8_t* base;
+    PT_HIP(A.alloc(&base, cursor));
+    PT_HIP(hipMemcpy(base, in.blob.data(), in.blob.size(), hipMemcpyHostToDevice));
+    PT_HIP(hipMemset(base + out_begin, 0, out_end - out_begin));
+    insts.assign(m, PtInstance{});
+    for (size_t i = 0; i < m; ++i) {
+        const HostInst& H = hs[i];
+        const Off& o = off[i];
+        PtInstance& I = insts[i];
+        I.row_ptr = (const int32_t*)(base + o.rp);
+        I.col = (const int32_t*)(base + o.col);
+        I.color_ptr = (const int32_t*)(base + o.cptr);
+        I.orig = (const int32_t*)(base + o.orig);
+        I.w = base + o.w;
+        I.h = o.h == SIZE_MAX ? nullptr : base + o.h;
+        I.beta = (const double*)(base + o.beta);
+        I.thr = o.thr == SIZE_MAX ? nullptr : (const uint32_t*)(base + o.thr);
+        I.jbits = o.jb == SIZE_MAX ? nullptr : (const uint8_t*)(base + o.jb);
+        I.best_s = (int8_t*)(base + oo[i].best);
+        I.final_s = oo[i].fin == SIZE_MAX ? nullptr : (int8_t*)(base + oo[i].fin);
+        I.hist = oo[i].hist == SIZE_MAX ? nullptr : (double*)(base + oo[i].hist);
+        I.trace = oo[i].trace == SIZE_MAX ? nullptr : (double*)(base + oo[i].trace);
+        I.out = (int64_t*)(base + oo[i].out);
+        I.unit = H.unit;
+        I.k0 = (uint32_t)H.seed;
+        I.k1 = (uint32_t)(H.seed >> 32);
+        I.n = H.n; I.k = H.k; I.R = H.R; I.Rp = H.Rp; I.n_rounds = H.n_rounds; I.S = H.S;
+        I.TQ = H.TQ; I.wm = H.wm;
+        I.ldim = H.ldim; I.lL = H.lL;
+    }
+    return 0;
+}
+
 int upload_instance(DevArena& A, const HostInst& H, PtInstance& I, bool want_final, bool want_hist) {
-    int32_t *d_rp, *d_col, *d_cptr, *d_orig;
-    PT_HIP(A.upload(&d_rp, H.row_ptr));
-    PT_HIP(A.upload(&d_col, H.col));
-    PT_HIP(A.upload(&d_cptr, H.color_ptr));
-    PT_HIP(A.upload(&d_orig, H.orig));
-    const void* d_w = nullptr;
-    const void* d_h = nullptr;
-    if (H.wm == PT_WM_I8) { int8_t* x; PT_HIP(A.upload(&x, H.w8)); d_w = x; }
-    if (H.wm == PT_WM_I32) { int32_t* x; PT_HIP(A.upload(&x, H.w32)); d_w = x; }
-    if (H.wm == PT_WM_F32) { float* x; PT_HIP(A.upload(&x, H.wf)); d_w = x; }
-    if (H.has_h) {
-        if (H.wm == PT_WM_F32) { float* x; PT_HIP(A.upload(&x, H.hf)); d_h = x; }
-        else { int32_t* x; PT_HIP(A.upload(&x, H.h32)); d_h = x; }
-    }
-    double* d_beta;
-    PT_HIP(A.upload(&d_beta, H.beta));
-    uint32_t* d_thr = nullptr;
-    if (H.wm != PT_WM_F32) PT_HIP(A.upload(&d_thr, H.thr));
-    int8_t* d_best;
-    PT_HIP(A.alloc(&d_best, (size_t)H.n));
-    int8_t* d_final = nullptr;
-    if (want_final) PT_HIP(A.alloc(&d_final, (size_t)H.n * H.R));
-    double* d_hist = nullptr;
-    if (want_hist && H.n_rounds > 0) PT_HIP(A.alloc(&d_hist, (size_t)H.n_rounds));
-    double* d_trace = nullptr;
-    if ((H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
-        PT_HIP(A.alloc(&d_trace, (size_t)H.n_rounds * H.R));
-    int64_t* d_out;
-    PT_HIP(A.alloc(&d_out, (size_t)pt_out_len(H.R)));
-    PT_HIP(hipMemset(d_out, 0, sizeof(int64_t) * (size_t)pt_out_len(H.R)));
-    I = PtInstance{};
-    I.row_ptr = d_rp; I.col = d_col; I.w = d_w; I.h = d_h; I.color_ptr = d_cptr; I.orig = d_orig;
-    I.beta = d_beta; I.thr = d_thr; I.best_s = d_best; I.final_s = d_final; I.hist = d_hist; I.trace = d_trace;
-    I.out = d_out; I.unit = H.unit;
-    I.k0 = (uint32_t)H.seed; I.k1 = (uint32_t)(H.seed >> 32);
-    I.n = H.n; I.k = H.k; I.R = H.R; I.Rp = H.Rp; I.n_rounds = H.n_rounds; I.S = H.S;
-    I.TQ = H.TQ; I.wm = H.wm;
-    I.ldim = H.ldim; I.lL = H.lL; I.jbits = nullptr;
-    if (H.ldim) {
-        uint8_t* d_jb;
-        PT_HIP(A.upload(&d_jb, H.jbits));
-        I.jbits = d_jb;
-    }
+    (void)want_hist;   // history follows H.flags
+    std::vector<HostInst> one{H};
+    std::vector<PtInstance> out;
+    int rc = upload_many(A, one, out, want_final);
+    if (rc) return rc;
+    I = out[0];
     return 0;
 }
 
@@ -506,13 +568,11 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     std::vector<PtInstance> insts((size_t)n_inst);
     size_t lds = 0;
     int maxRp = 1;
+    int rc = upload_many(A, hs, insts, false);
+    if (rc) return rc;
     for (int i = 0; i < n_inst; ++i) {
-        const HostInst& H = hs[i];
-        int rc = upload_instance(A, H, insts[i], H.flags & DRIFT_PT_RECORD_FINAL,
-                                 H.flags & DRIFT_PT_RECORD_HISTORY);
-        if (rc) return rc;
-        lds = std::max(lds, pt_lds_bytes(H.n, H.Rp));
-        maxRp = std::max(maxRp, H.Rp);
+        lds = std::max(lds, pt_lds_bytes(hs[i].n, hs[i].Rp));
+        maxRp = std::max(maxRp, hs[i].Rp);
     }
     if (lds + 1024 > (size_t)device_lds_limit())
         return fail(DRIFT_PT_EINVAL, "instance does not fit one workgroup's LDS");

@@ -89,6 +89,10 @@ hipError_t pt_slab_set_lds(int dim, int NB, size_t lds);
 
 size_t pt_lds_bytes(int n, int Rp);
 
+// Error reporting shared by every entry point of the library (drift_pt_last_error).
+int pt_fail(int code, const char* msg);
+void pt_clear_error();
+
 hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int n_inst,
                          int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
                          hipStream_t s);

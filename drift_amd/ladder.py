@@ -1,0 +1,309 @@
+"""Temperature ladders: geometric, feedback-optimised, and learned.
+
+The reference runs every instance on the geometric ladder T_k = T_min (T_max/T_min)^(k/(R-1))
+(drift/solvers/parallel_tempering.py:29-37).  This module adds two ways to place the same number of
+rungs between the same end points; neither touches the Metropolis or exchange rules, so detailed
+balance is unchanged:
+
+* ``feedback_optimise`` -- the feedback method of Katzgraber, Trebst, Huse and Troyer
+  (J. Stat. Mech. P03018, 2006).  Each replica carries a direction label (last end of the ladder it
+  visited); the engine counts, per rung, the rounds spent there by replicas last seen at the cold
+  end (n_up) and at the hot end (n_down).  With f = n_up / (n_up + n_down), rungs are moved so that
+  every interval carries the same share of sqrt(df): in log T, the new density is
+  sqrt(eta |df/du|), which makes f linear in the rung index and maximises round trips.
+* ``LadderNet`` -- a 16 -> 3x128 -> 32 network (BF16 weights, FP32 accumulation, trained and run by
+  the library's HIP kernels, include/drift_mlp.h) mapping 16 instance statistics to rung positions.
+
+Ladders are carried as positions x in [0, 1] on the log-temperature axis: x_0 = 0 (T_min, coldest,
+rung 0), x_{R-1} = 1 (T_max); the geometric ladder is x_j = j / (R - 1).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_float, c_int32, c_uint64, c_void_p
+
+import numpy as np
+
+from . import _lib
+
+N_FEATURES = 16
+N_RUNGS = 32
+
+
+# ------------------------------------------------------------------------------------------------
+# ladder positions <-> temperatures
+# ------------------------------------------------------------------------------------------------
+def geometric_positions(R: int) -> np.ndarray:
+    return np.linspace(0.0, 1.0, R) if R > 1 else np.zeros(1)
+
+
+def temperatures_from_positions(x, T_min: float, T_max: float) -> np.ndarray:
+    x = np.asarray(x, dtype=np.float64)
+    return np.exp(np.log(T_min) + x * (np.log(T_max) - np.log(T_min)))
+
+
+def betas_from_positions(x, T_min: float, T_max: float) -> np.ndarray:
+    return 1.0 / temperatures_from_positions(x, T_min, T_max)
+
+
+def positions_from_temperatures(T) -> np.ndarray:
+    u = np.log(np.asarray(T, dtype=np.float64))
+    return (u - u[0]) / (u[-1] - u[0])
+
+
+# ------------------------------------------------------------------------------------------------
+# feedback optimisation
+# ------------------------------------------------------------------------------------------------
+def up_fraction(rung_up, rung_down) -> np.ndarray:
+    """f_k = n_up / (n_up + n_down), pinned to 1 at the cold end and 0 at the hot end, made
+    non-increasing (running minimum from the cold end), unvisited rungs interpolated."""
+    up = np.asarray(rung_up, dtype=np.float64)
+    down = np.asarray(rung_down, dtype=np.float64)
+    tot = up + down
+    R = up.size
+    f = np.full(R, np.nan)
+    seen = tot > 0
+    f[seen] = up[seen] / tot[seen]
+    f[0], f[-1] = 1.0, 0.0
+    k = np.arange(R)
+    ok = ~np.isnan(f)
+    f = np.interp(k, k[ok], f[ok])
+    return np.minimum.accumulate(np.clip(f, 0.0, 1.0))
+
+
+def feedback_update(x, f, *, damping: float = 0.5, floor: float = 1e-4) -> np.ndarray:
+    """One feedback step: interval k of the new ladder carries an equal share of sqrt(df_k)."""
+    x = np.asarray(x, dtype=np.float64)
+    df = np.maximum(-np.diff(np.asarray(f, dtype=np.float64)), floor)
+    w = np.sqrt(df)
+    W = np.concatenate([[0.0], np.cumsum(w)])
+    target = np.linspace(0.0, W[-1], x.size)
+    new = np.interp(target, W, x)
+    new[0], new[-1] = 0.0, 1.0
+    out = (1.0 - damping) * new + damping * x
+    return np.maximum.accumulate(out)
+
+
+def feedback_optimise(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
+                      iterations: int = 4, sweeps: int = 500, seeds=None, damping: float = 0.5):
+    """Feedback-optimised ladders for a list of instances, all iterations batched on the GPU.
+
+    Iteration i runs ``sweeps * 2**i`` sweeps (one exchange attempt per sweep) on every instance
+    with its current ladder, then moves the rungs.  Returns (positions (m, R), last results)."""
+    from . import gpu
+
+    m = len(models)
+    x = np.tile(geometric_positions(R), (m, 1))
+    seeds = list(range(m)) if seeds is None else list(seeds)
+    res = None
+    for it in range(iterations):
+        betas = [betas_from_positions(x[i], T_min, T_max) for i in range(m)]
+        res = gpu.parallel_tempering_gpu_batch(
+            models, n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=sweeps * (1 << it),
+            sweeps_per_round=1, seeds=[s + 7919 * it for s in seeds], betas=betas)
+        for i, r in enumerate(res):
+            x[i] = feedback_update(x[i], up_fraction(r.rung_up, r.rung_down), damping=damping)
+    return x, res
+
+
+def round_trip_rate(res, sweeps: int) -> float:
+    """Round trips per replica per sweep of a result."""
+    return float(np.sum(res.round_trips)) / (len(res.round_trips) * max(sweeps, 1))
+
+
+# ------------------------------------------------------------------------------------------------
+# instance features
+# ------------------------------------------------------------------------------------------------
+def structural_features(model, T_min: float, T_max: float) -> np.ndarray:
+    """Features 0-11: size, degree and coupling statistics, colour count and ladder span."""
+    rp, ci, w = model.to_csr()
+    rp = np.asarray(rp, dtype=np.int64)
+    h = np.asarray(model.h, dtype=np.float64)
+    deg = np.diff(rp).astype(np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    aw = np.abs(w) if w.size else np.zeros(1)
+    lib = _lib.load()
+    n = model.n
+    cptr = np.zeros(n + 1, dtype=np.int32)
+    cs = np.zeros(n, dtype=np.int32)
+    rp32 = np.ascontiguousarray(rp, dtype=np.int32)
+    ci32 = np.ascontiguousarray(ci, dtype=np.int32)
+    k = lib.drift_pt_greedy_coloring(n, _lib.ptr(rp32, _lib.c_int32), _lib.ptr(ci32, _lib.c_int32),
+                                     _lib.ptr(cptr, _lib.c_int32), _lib.ptr(cs, _lib.c_int32))
+    if k < 1:
+        raise RuntimeError(f"colouring failed: {_lib.last_error(lib)}")
+    return np.array([
+        np.log(n), deg.mean(), deg.var(), aw.mean(), w.std() if w.size else 0.0,
+        w.mean() if w.size else 0.0, aw.max() / max(aw.mean(), 1e-12),
+        float((w < 0).mean()) if w.size else 0.0, np.abs(h).mean(), h.std(), float(k),
+        np.log(T_max / T_min)])
+
+
+def dynamic_features(res, n: int, T: np.ndarray) -> np.ndarray:
+    """Features 12-15 from a short geometric-ladder run with an energy trace: specific-heat proxies
+    var(E) beta^2 / n at a quarter, half and three quarters of the ladder, and the mean exchange
+    acceptance."""
+    tr = res.rung_energy_trace
+    tail = tr[tr.shape[0] // 2:]
+    R = tr.shape[1]
+    out = []
+    for r in (R // 4, R // 2, (3 * R) // 4):
+        out.append(tail[:, r].var() / (T[r] ** 2) / n)
+    acc = res.swap_accepted.sum() / max(res.swap_tried.sum(), 1)
+    return np.array(out + [acc])
+
+
+def instance_features(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
+                      short_sweeps: int = 200, seeds=None) -> np.ndarray:
+    """The 16 features of each instance (one batched short run for the dynamic part)."""
+    from . import gpu
+
+    T = temperatures_from_positions(geometric_positions(R), T_min, T_max)
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=T_min, T_max=T_max,
+                                           n_rounds=short_sweeps, sweeps_per_round=1, seeds=seeds,
+                                           record_trace=True)
+    feats = [np.concatenate([structural_features(m, T_min, T_max), dynamic_features(r, m.n, T)])
+             for m, r in zip(models, res)]
+    return np.array(feats)
+
+
+# ------------------------------------------------------------------------------------------------
+# the network
+# ------------------------------------------------------------------------------------------------
+def _mlp_lib():
+    lib = _lib.load()
+    if not getattr(lib, "_mlp_bound", False):
+        P = POINTER(c_float)
+        sig = {
+            "drift_mlp_create": ([c_uint64, POINTER(c_void_p)], c_int32),
+            "drift_mlp_destroy": ([c_void_p], c_int32),
+            "drift_mlp_get_params": ([c_void_p, P], c_int32),
+            "drift_mlp_set_params": ([c_void_p, P], c_int32),
+            "drift_mlp_forward": ([c_void_p, P, c_int32, P], c_int32),
+            "drift_mlp_gradients": ([c_void_p, P, P, c_int32, P, P], c_int32),
+            "drift_mlp_train_step": ([c_void_p, P, P, c_int32, c_float, P], c_int32),
+        }
+        for name, (args, ret) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ret
+        lib._mlp_bound = True
+    return lib
+
+
+N_PARAMS = 39328
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class LadderNet:
+    """Handle on the device-resident network; inputs are standardised with (mu, sd) before use."""
+
+    def __init__(self, seed: int = 0):
+        self._lib = _mlp_lib()
+        h = c_void_p()
+        rc = self._lib.drift_mlp_create(int(seed) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"ladder network: {_lib.last_error(self._lib)}")
+        self._h = h
+        self.mu = np.zeros(N_FEATURES)
+        self.sd = np.ones(N_FEATURES)
+
+    def close(self):
+        if self._h:
+            self._lib.drift_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"ladder network: {_lib.last_error(self._lib)}")
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(N_PARAMS, dtype=np.float32)
+        self._check(self._lib.drift_mlp_get_params(self._h, p.ctypes.data_as(POINTER(c_float))))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p)
+        assert p.size == N_PARAMS
+        self._check(self._lib.drift_mlp_set_params(self._h, p.ctypes.data_as(POINTER(c_float))))
+
+    def fit_scaler(self, F):
+        F = np.asarray(F, dtype=np.float64)
+        self.mu = F.mean(axis=0)
+        self.sd = np.where(F.std(axis=0) > 1e-12, F.std(axis=0), 1.0)
+
+    def _scale(self, F):
+        return _f32((np.asarray(F, dtype=np.float64) - self.mu) / self.sd)
+
+    def raw_forward(self, X) -> np.ndarray:
+        X = _f32(np.atleast_2d(X))
+        out = np.zeros((X.shape[0], N_RUNGS), dtype=np.float32)
+        self._check(self._lib.drift_mlp_forward(self._h, X.ctypes.data_as(POINTER(c_float)),
+                                                X.shape[0], out.ctypes.data_as(POINTER(c_float))))
+        return out
+
+    def raw_gradients(self, X, T):
+        X, T = _f32(np.atleast_2d(X)), _f32(np.atleast_2d(T))
+        g = np.zeros(N_PARAMS, dtype=np.float32)
+        loss = c_float()
+        self._check(self._lib.drift_mlp_gradients(
+            self._h, X.ctypes.data_as(POINTER(c_float)), T.ctypes.data_as(POINTER(c_float)),
+            X.shape[0], g.ctypes.data_as(POINTER(c_float)), ctypes.byref(loss)))
+        return g, float(loss.value)
+
+    def train_step(self, X, T, lr: float) -> float:
+        X, T = _f32(np.atleast_2d(X)), _f32(np.atleast_2d(T))
+        loss = c_float()
+        self._check(self._lib.drift_mlp_train_step(
+            self._h, X.ctypes.data_as(POINTER(c_float)), T.ctypes.data_as(POINTER(c_float)),
+            X.shape[0], float(lr), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def fit(self, F, targets, *, epochs: int = 2000, lr: float = 3e-3, batch: int = 0,
+            seed: int = 0):
+        """Adam on standardised features; full batch unless ``batch`` is given.  Returns losses."""
+        self.fit_scaler(F)
+        X = self._scale(F)
+        T = _f32(targets)
+        rng = np.random.default_rng(seed)
+        losses = []
+        for ep in range(epochs):
+            if batch and batch < X.shape[0]:
+                idx = rng.choice(X.shape[0], batch, replace=False)
+                losses.append(self.train_step(X[idx], T[idx], lr))
+            else:
+                losses.append(self.train_step(X, T, lr))
+        return np.array(losses)
+
+    def predict(self, F) -> np.ndarray:
+        """Rung positions (m, 32) for raw features F (m, 16)."""
+        return self.raw_forward(self._scale(F)).astype(np.float64)
+
+    def save(self, path: str):
+        np.savez(path, params=self.params, mu=self.mu, sd=self.sd)
+
+    def load(self, path: str):
+        d = np.load(path)
+        self.params, self.mu, self.sd = d["params"], d["mu"], d["sd"]
+
+
+# ------------------------------------------------------------------------------------------------
+# time to target
+# ------------------------------------------------------------------------------------------------
+def sweeps_to_target(best_history, target: float, tol: float = 1e-9):
+    """First round (1-based count of sweeps when sweeps_per_round = 1) whose best energy reaches the
+    target, or None."""
+    hit = np.nonzero(np.asarray(best_history) <= target + tol * max(1.0, abs(target)))[0]
+    return int(hit[0]) + 1 if hit.size else None

@@ -1,0 +1,98 @@
+"""The ladder network's HIP kernels against the float64 oracle, training, and the feedback ladder
+on the GPU engine."""
+
+import numpy as np
+import pytest
+
+from drift_amd import gpu, ladder
+from drift_amd.builders.lattice import square_ferromagnet
+from oracle import mlp_oracle as M
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skipif(not gpu.gpu_available(), reason="no gfx950 device / engine not built"),
+]
+
+
+def _announce(name):
+    print(f"[ladder-gpu] {name}", flush=True)
+
+
+def _targets(F):
+    """A smooth family of monotone ladders controlled by the first two features."""
+    j = np.linspace(0.0, 1.0, 32)
+    a = 1.0 + 0.6 * np.tanh(F[:, :1])
+    b = 0.15 * np.tanh(F[:, 1:2])
+    x = j[None, :] ** a + b * np.sin(np.pi * j)[None, :]
+    x = np.maximum.accumulate(np.clip(x, 0, 1), axis=1)
+    x[:, 0], x[:, -1] = 0.0, 1.0
+    return x
+
+
+def test_forward_matches_oracle():
+    _announce("forward vs oracle")
+    net = ladder.LadderNet(seed=3)
+    X = np.random.default_rng(0).normal(size=(37, 16)).astype(np.float32)
+    got = net.raw_forward(X)
+    ref = M.forward(net.params.astype(np.float64), X.astype(np.float64), bf16=True)
+    np.testing.assert_allclose(got, ref, atol=2e-5)
+    assert np.all(np.diff(got, axis=1) > 0) and np.all(got[:, 0] == 0)
+
+
+def test_gradients_match_oracle():
+    _announce("gradients vs oracle")
+    net = ladder.LadderNet(seed=4)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(64, 16)).astype(np.float32)
+    T = _targets(X).astype(np.float32)
+    g, loss = net.raw_gradients(X, T)
+    p = net.params.astype(np.float64)
+    ref = M.gradients(p, X.astype(np.float64), T.astype(np.float64), bf16=True)
+    assert loss == pytest.approx(M.loss(p, X.astype(np.float64), T.astype(np.float64)), rel=1e-4)
+    err = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    assert err < 1e-4, err
+    assert np.max(np.abs(g - ref)) < 1e-4 * np.max(np.abs(ref)) + 1e-9
+
+
+def test_training_fits_a_smooth_ladder_family():
+    _announce("training")
+    rng = np.random.default_rng(2)
+    F = rng.normal(size=(1024, 16))
+    T = _targets(F)
+    net = ladder.LadderNet(seed=5)
+    losses = net.fit(F, T, epochs=1500, lr=3e-3)
+    assert losses[-1] < losses[0] / 20, (losses[0], losses[-1])
+    Fh = rng.normal(size=(256, 16))
+    Th = _targets(Fh)
+    held = np.mean((net.predict(Fh) - Th)[:, 1:-1] ** 2)
+    mean_ladder = np.mean((T.mean(axis=0)[None, :] - Th)[:, 1:-1] ** 2)
+    assert held < 0.15 * mean_ladder, (held, mean_ladder)
+
+
+def test_params_round_trip_and_seed_determinism():
+    _announce("params")
+    a, b, c = ladder.LadderNet(seed=9), ladder.LadderNet(seed=9), ladder.LadderNet(seed=10)
+    assert np.array_equal(a.params, b.params) and not np.array_equal(a.params, c.params)
+    p = np.random.default_rng(3).normal(size=ladder.N_PARAMS).astype(np.float32)
+    a.params = p
+    assert np.array_equal(a.params, p)
+    with pytest.raises(RuntimeError):
+        a.raw_forward(np.zeros((0, 16), dtype=np.float32))
+
+
+def test_feedback_ladder_increases_round_trips_on_ferromagnets():
+    _announce("feedback ladder")
+    models = [square_ferromagnet(L) for L in (16, 24, 32, 16, 24, 32)]
+    T_min, T_max = 0.8, 8.0
+    x, _ = ladder.feedback_optimise(models, T_min=T_min, T_max=T_max, iterations=4, sweeps=500)
+    assert np.all(x[:, 0] == 0) and np.allclose(x[:, -1], 1) and np.all(np.diff(x, axis=1) >= 0)
+    sweeps = 6000
+    geo = gpu.parallel_tempering_gpu_batch(models, n_replicas=32, T_min=T_min, T_max=T_max,
+                                           n_rounds=sweeps, sweeps_per_round=1, seeds=range(50, 56))
+    opt = gpu.parallel_tempering_gpu_batch(
+        models, n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=sweeps, sweeps_per_round=1,
+        seeds=range(50, 56), betas=[ladder.betas_from_positions(xi, T_min, T_max) for xi in x])
+    rg = sum(ladder.round_trip_rate(r, sweeps) for r in geo)
+    ro = sum(ladder.round_trip_rate(r, sweeps) for r in opt)
+    print(f"[ladder-gpu] round trips per replica-sweep: geometric {rg:.3e}, feedback {ro:.3e}")
+    assert ro > 1.2 * rg
