@@ -152,6 +152,165 @@ def bench_c4(args) -> int:
     return 0
 
 
+LADDER_FAMILIES = {          # family: (T_min, T_max)
+    "ferro2d": (0.8, 8.0),
+    "ea3d": (0.3, 3.0),
+    "rrg": (0.1, 3.0),
+    "er": (0.2, 10.0),
+}
+
+
+def ladder_instance(family: str, seed: int):
+    """One instance of the learned-ladder mix (N between 216 and 2048)."""
+    from drift_amd.builders.lattice import (cubic_pmj_glass, er_sparse_weighted,
+                                            random_regular_gaussian, square_ferromagnet)
+
+    rng = np.random.default_rng(seed)
+    if family == "ferro2d":
+        return square_ferromagnet(int(2 * rng.integers(8, 23)))
+    if family == "ea3d":
+        return cubic_pmj_glass(int(rng.choice([6, 8, 10, 12])), seed)
+    if family == "rrg":
+        return random_regular_gaussian(int(2 * rng.integers(128, 1025)), seed)
+    return er_sparse_weighted(int(rng.integers(256, 2049)), float(rng.uniform(3.0, 12.0)), seed)
+
+
+def ladder_set(per_family: int, base_seed: int):
+    items = []
+    for fi, fam in enumerate(LADDER_FAMILIES):
+        for k in range(per_family):
+            items.append((fam, ladder_instance(fam, base_seed + 1000 * fi + k)))
+    return items
+
+
+def bench_ladder(args) -> int:
+    """Learned vs geometric ladder: feedback labels -> features -> network trained by the HIP
+    kernels -> held-out time-to-target, with batched vs one-instance-per-launch wall time."""
+    from drift_amd import gpu, ladder
+
+    R, budget = 32, args.steps
+    t_all = time.perf_counter()
+    train = ladder_set(args.train_per_family, 1)
+    test = ladder_set(args.test_per_family, 777_777)
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)   # noqa: E731
+
+    def by_family(items):
+        groups = {}
+        for i, (fam, m) in enumerate(items):
+            groups.setdefault(fam, []).append(i)
+        return groups
+
+    # 1. feedback-optimised labels and features of the training set, one batch per family
+    X, Y = np.zeros((len(train), 16)), np.zeros((len(train), R))
+    t0 = time.perf_counter()
+    for fam, idx in by_family(train).items():
+        T_min, T_max = LADDER_FAMILIES[fam]
+        models = [train[i][1] for i in idx]
+        x, _ = ladder.feedback_optimise(models, T_min=T_min, T_max=T_max, R=R,
+                                        iterations=5, sweeps=args.label_sweeps, damping=0.5)
+        Y[idx] = x
+        X[idx] = ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R)
+        log(f"labels {fam}: {len(idx)} instances")
+    t_labels = time.perf_counter() - t0
+    # 2. train the network with its own kernels
+    net = ladder.LadderNet(seed=11)
+    t0 = time.perf_counter()
+    losses = net.fit(X, Y, epochs=args.epochs, lr=3e-3)
+    t_train = time.perf_counter() - t0
+    log(f"trained: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    # 3. held-out: geometric vs learned ladders, same seeds, one batched launch per family
+    rows, wall_learn_batch, t_geo_batch, t_learn_batch = [], 0.0, 0.0, 0.0
+    for fam, idx in by_family(test).items():
+        T_min, T_max = LADDER_FAMILIES[fam]
+        models = [test[i][1] for i in idx]
+        xs = net.predict(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R))
+        seeds = [9000 + i for i in idx]
+        kw = dict(n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=budget, sweeps_per_round=1,
+                  seeds=seeds, record_history=True)
+        t0 = time.perf_counter()
+        geo = gpu.parallel_tempering_gpu_batch(models, **kw)
+        t1 = time.perf_counter()
+        lrn = gpu.parallel_tempering_gpu_batch(
+            models, betas=[ladder.betas_from_positions(x, T_min, T_max) for x in xs], **kw)
+        t2 = time.perf_counter()
+        t_geo_batch += t1 - t0
+        t_learn_batch += t2 - t1
+        fam_rows = []
+        for m, g, l in zip(models, geo, lrn):
+            target = min(g.best_E, l.best_E)
+            fam_rows.append(dict(family=fam, n=m.n, target=target, seed=seeds[len(fam_rows)],
+                                 tt_geo=ladder.sweeps_to_target(g.best_history, target),
+                                 tt_learn=ladder.sweeps_to_target(l.best_history, target),
+                                 rt_geo=ladder.round_trip_rate(g, budget),
+                                 rt_learn=ladder.round_trip_rate(l, budget), best_geo=g.best_E))
+        # the batched learned launch stopped once its slowest instance is at the target
+        slowest = max(r["tt_learn"] or budget for r in fam_rows)
+        wall_learn_batch += (t2 - t1) * min(slowest, budget) / budget
+        rows += fam_rows
+        log(f"held-out {fam}: {len(idx)} instances")
+    # 4. the reference's way: each instance alone in its own launch with the geometric ladder,
+    # stopped at its own time to target (instances that never reach it run the whole budget)
+    wall_geo_single, mismatch = 0.0, 0
+    for r, (fam, m) in zip(rows, test):
+        T_min, T_max = LADDER_FAMILIES[fam]
+        t0 = time.perf_counter()
+        one = gpu.parallel_tempering_gpu(m, n_replicas=R, T_min=T_min, T_max=T_max,
+                                         n_rounds=budget, sweeps_per_round=1, seed=r["seed"],
+                                         record_history=False)
+        wall = time.perf_counter() - t0
+        mismatch += int(one.best_E != r["best_geo"])       # same stream -> same answer
+        wall_geo_single += wall * min(r["tt_geo"] or budget, budget) / budget
+    log(f"single launches done, {mismatch} best-energy mismatches against the batched run")
+
+    def med(v):
+        return float(np.median(v)) if len(v) else None
+
+    tt_g = np.array([r["tt_geo"] or 2 * budget for r in rows], dtype=float)
+    tt_l = np.array([r["tt_learn"] or 2 * budget for r in rows], dtype=float)
+    hit_g = int(sum(r["tt_geo"] is not None for r in rows))
+    hit_l = int(sum(r["tt_learn"] is not None for r in rows))
+    per_family = {}
+    for fam in LADDER_FAMILIES:
+        sel = [i for i, r in enumerate(rows) if r["family"] == fam]
+        per_family[fam] = {
+            "hit_geo": int(sum(rows[i]["tt_geo"] is not None for i in sel)),
+            "hit_learned": int(sum(rows[i]["tt_learn"] is not None for i in sel)),
+            "median_sweeps_geo": med(tt_g[sel]), "median_sweeps_learned": med(tt_l[sel]),
+            "round_trip_ratio": med([rows[i]["rt_learn"] / max(rows[i]["rt_geo"], 1e-12) for i in sel]),
+        }
+    line = {
+        "metric": "time-to-target energy, learned vs geometric ladder",
+        "value": round(wall_geo_single / max(wall_learn_batch, 1e-9), 3),
+        "unit": "x less wall time (batched+learned vs one-launch-per-instance+geometric)",
+        "n_gpus": 1, "steps": budget, "warmup": 0, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32/f32", "data": "synthetic",
+        "config": {
+            "workload": "learned 32-rung ladders on held-out instances of four families "
+                        "(2-D ferromagnet, 3-D +/-J glass, 3-regular N(0,1), sparse ER integer), "
+                        "N 216..2048",
+            "train_instances": len(train), "test_instances": len(test),
+            "budget_sweeps": budget, "label_sweeps": args.label_sweeps, "epochs": args.epochs,
+            "target": "best energy found by either ladder within the budget",
+        },
+        "held_out": {
+            "reached_target_geo": hit_g, "reached_target_learned": hit_l,
+            "median_sweeps_to_target_geo": med(tt_g), "median_sweeps_to_target_learned": med(tt_l),
+            "sweeps_ratio_median": med(tt_g / tt_l),
+            "wall_s_geo_one_per_launch": round(wall_geo_single, 3),
+            "wall_s_learned_batched": round(wall_learn_batch, 3),
+            "batched_s_geo_full_budget": round(t_geo_batch, 3),
+            "batched_s_learned_full_budget": round(t_learn_batch, 3),
+            "single_vs_batched_best_mismatches": mismatch,
+            "per_family": per_family,
+        },
+        "timing_s": {"labels": round(t_labels, 1), "train": round(t_train, 1),
+                     "total": round(time.perf_counter() - t_all, 1)},
+        "train_loss": [float(losses[0]), float(losses[-1])],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def cpu_baseline(model, enc, R, beta, budget_s: float):
     import oracle
 
@@ -186,6 +345,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
+    ap.add_argument("--train-per-family", type=int, default=128)
+    ap.add_argument("--test-per-family", type=int, default=16)
+    ap.add_argument("--label-sweeps", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (rocprof counter scaling)")
     args = ap.parse_args()
@@ -204,6 +367,8 @@ def main() -> int:
         return 1
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "ladder":
+        return bench_ladder(args)
     model, R, beta, desc = workload(args.workload)
     enc = gpu.encode_problem(model)
     kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,
