@@ -58,16 +58,17 @@ def workload(name: str):
 def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
     """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse.
     Spins are int8, so own spin read + write and d neighbour reads are d + 2 bytes.  The CSR
-    kernels add d (weight + int32 column) and two int32 row pointers (plus the field if any);
-    positions are relabelled by colour, so no colour list is read.  The lattice kernel computes
-    neighbours from coordinates and reads one byte holding the site's bond signs."""
+    kernels add d edge records (int8 weight and 24-bit column packed in 4 bytes; int32 / float
+    weight and int32 column in 8) and two int32 row pointers (plus the field if any); positions
+    are relabelled by colour, so no colour list is read.  The lattice kernel computes neighbours
+    from coordinates and reads one byte holding the site's bond signs."""
     d = model.nnz / model.n
     b_s = 1.0
     if engine in ("lattice", "slab"):
         return b_s * (d + 1) + b_s + 1.0
-    b_w = enc.weight.dtype.itemsize
+    b_e = 4.0 if enc.weight.dtype.itemsize == 1 and model.n < (1 << 24) else 8.0
     b_h = 0.0 if enc.h is None else enc.h.dtype.itemsize
-    return b_s * (d + 1) + b_s + d * (b_w + 4.0) + 8.0 + b_h
+    return b_s * (d + 1) + b_s + d * b_e + 8.0 + b_h
 
 
 def copy_calibration(nbytes: int) -> int:
@@ -399,7 +400,7 @@ def main() -> int:
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
         "kernel": {"global": "g_color_step", "lattice": "g_lattice_step",
-                   "slab": "slab_sweep"}.get(kt.engine, "pt_lds_kernel"),
+                   "slab": "slab_sweep", "coop": "coop_run"}.get(kt.engine, "pt_lds_kernel"),
         "bytes_per_attempt": round(B, 3),
         "attempts_per_launch": per_launch_attempts,
         "avg_launch_us": round(per_launch_s * 1e6, 3),

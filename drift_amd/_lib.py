@@ -21,9 +21,9 @@ OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
 # enum drift_pt_weight_type
 W_I8, W_I32, W_F32 = 1, 2, 3
 # enum drift_pt_engine
-ENGINE_AUTO, ENGINE_LDS, ENGINE_GLOBAL, ENGINE_LATTICE, ENGINE_SLAB = 0, 1, 2, 3, 4
+ENGINE_AUTO, ENGINE_LDS, ENGINE_GLOBAL, ENGINE_LATTICE, ENGINE_SLAB, ENGINE_COOP = 0, 1, 2, 3, 4, 5
 ENGINES = {"auto": ENGINE_AUTO, "lds": ENGINE_LDS, "global": ENGINE_GLOBAL,
-           "lattice": ENGINE_LATTICE, "slab": ENGINE_SLAB}
+           "lattice": ENGINE_LATTICE, "slab": ENGINE_SLAB, "coop": ENGINE_COOP}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 # enum drift_pt_flags
 RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GRAPH = 1, 2, 4, 8, 16, 32

@@ -105,10 +105,10 @@ struct HostInst {
     int n = 0, nnz = 0, k = 0, R = 0, Rp = 0, wm = 0, TQ = 1, S = 0, n_rounds = 0, flags = 0;
     uint64_t seed = 0;
     double unit = 1.0;
-    std::vector<int32_t> row_ptr, col, color_ptr, orig;
-    std::vector<int8_t> w8;
-    std::vector<int32_t> w32, h32;
-    std::vector<float> wf, hf;
+    std::vector<int32_t> row_ptr, color_ptr, orig;
+    std::vector<uint32_t> edge;     // device edge records (see PtInstance::edge)
+    std::vector<int32_t> h32;
+    std::vector<float> hf;
     bool has_h = false;
     std::vector<double> beta;
     std::vector<uint32_t> thr;
@@ -241,11 +241,11 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     H.color_ptr = cptr;
     H.orig = cspins;
     H.row_ptr.assign((size_t)n + 1, 0);
-    H.col.resize((size_t)nnz);
     for (int p = 0; p < n; ++p) H.row_ptr[p + 1] = H.row_ptr[p] + (rp[cspins[p] + 1] - rp[cspins[p]]);
-    if (P->weight_type == DRIFT_PT_W_I8) H.w8.resize((size_t)nnz);
-    if (P->weight_type == DRIFT_PT_W_I32) H.w32.resize((size_t)nnz);
-    if (isf) H.wf.resize((size_t)nnz);
+    // int8 weights pack with a 24-bit position into one word; larger instances use two words
+    if (H.wm == PT_WM_I8 && n >= (1 << 24)) H.wm = PT_WM_I32;
+    const int ew = H.wm == PT_WM_I8 ? 1 : 2;
+    H.edge.resize((size_t)nnz * ew);
     if (H.has_h) {
         if (isf) H.hf.resize((size_t)n);
         else H.h32.resize((size_t)n);
@@ -256,10 +256,17 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
         int dst = H.row_ptr[p];
         double field = std::fabs(hval(i));
         for (int t = rp[i]; t < rp[i + 1]; ++t, ++dst) {   // row order kept: fixed summation order
-            H.col[dst] = pos[ci[t]];
-            if (P->weight_type == DRIFT_PT_W_I8) H.w8[dst] = ((const int8_t*)P->weight)[t];
-            else if (P->weight_type == DRIFT_PT_W_I32) H.w32[dst] = ((const int32_t*)P->weight)[t];
-            else H.wf[dst] = ((const float*)P->weight)[t];
+            const uint32_t c = (uint32_t)pos[ci[t]];
+            uint32_t wb;
+            if (P->weight_type == DRIFT_PT_W_I8) wb = (uint32_t)(int32_t)((const int8_t*)P->weight)[t];
+            else if (P->weight_type == DRIFT_PT_W_I32) wb = (uint32_t)((const int32_t*)P->weight)[t];
+            else std::memcpy(&wb, (const float*)P->weight + t, 4);
+            if (ew == 1) {
+                H.edge[dst] = (c << 8) | (wb & 0xffu);
+            } else {
+                H.edge[2 * (size_t)dst] = c;
+                H.edge[2 * (size_t)dst + 1] = wb;
+            }
             field += std::fabs(wval(t));
         }
         if (H.has_h) {
@@ -322,7 +329,7 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
 // per-site sign byte used by the lattice kernel.  Directions: 0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z.
 void detect_lattice(const drift_pt_problem* P, HostInst& H) {
     H.ldim = 0;
-    if (H.wm != PT_WM_I8 || H.has_h || H.k != 2 || (H.n & 1)) return;
+    if (P->weight_type != DRIFT_PT_W_I8 || H.has_h || H.k != 2 || (H.n & 1)) return;
     const int n = H.n;
     for (int dim = 3; dim >= 2; --dim) {
         const int L = (int)std::lround(std::pow((double)n, 1.0 / dim));
@@ -415,17 +422,16 @@ struct Packer {
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
                 bool force_final) {
     const size_t m = hs.size();
-    struct Off { size_t rp, col, cptr, orig, w, h, beta, thr, jb; };
+    struct Off { size_t rp, edge, cptr, orig, h, beta, thr, jb; };
     std::vector<Off> off(m);
     Packer in;
     for (size_t i = 0; i < m; ++i) {
         const HostInst& H = hs[i];
         Off& o = off[i];
         o.rp = in.add(H.row_ptr);
-        o.col = in.add(H.col);
+        o.edge = in.add(H.edge);
         o.cptr = in.add(H.color_ptr);
         o.orig = in.add(H.orig);
-        o.w = H.wm == PT_WM_I8 ? in.add(H.w8) : (H.wm == PT_WM_I32 ? in.add(H.w32) : in.add(H.wf));
         o.h = !H.has_h ? SIZE_MAX : (H.wm == PT_WM_F32 ? in.add(H.hf) : in.add(H.h32));
         o.beta = in.add(H.beta);
         o.thr = H.wm != PT_WM_F32 ? in.add(H.thr) : SIZE_MAX;
@@ -466,10 +472,9 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         const Off& o = off[i];
         PtInstance& I = insts[i];
         I.row_ptr = (const int32_t*)(base + o.rp);
-        I.col = (const int32_t*)(base + o.col);
+        I.edge = base + o.edge;
         I.color_ptr = (const int32_t*)(base + o.cptr);
         I.orig = (const int32_t*)(base + o.orig);
-        I.w = base + o.w;
         I.h = o.h == SIZE_MAX ? nullptr : base + o.h;
         I.beta = (const double*)(base + o.beta);
         I.thr = o.thr == SIZE_MAX ? nullptr : (const uint32_t*)(base + o.thr);
@@ -621,6 +626,32 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     return 0;
 }
 
+// Results of the engines that keep their state in a GState: statistics block, best record, final
+// energies and rungs from the buffers of parity n_rounds & 1.
+int finish_global(const HostInst& H, const PtInstance& I, const GState& G, drift_pt_result* res) {
+    const int Rp = H.Rp;
+    const int fin = H.n_rounds & 1;
+    std::vector<int64_t> out((size_t)pt_out_len(H.R));
+    PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t best[5];
+    PT_HIP(hipMemcpy(best, G.best + 5 * fin, sizeof(best), hipMemcpyDeviceToHost));
+    std::vector<int64_t> E((size_t)Rp);
+    std::vector<int32_t> rung((size_t)Rp);
+    PT_HIP(hipMemcpy(E.data(), G.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(rung.data(), G.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
+    for (int x = 0; x < 5; ++x) out[x] = best[x];
+    for (int r = 0; r < H.R; ++r) {
+        pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
+        pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
+    }
+    std::vector<int8_t> fs;
+    if (H.flags & DRIFT_PT_RECORD_FINAL) {
+        fs.resize((size_t)H.n * Rp);
+        PT_HIP(hipMemcpy(fs.data(), G.sp, fs.size(), hipMemcpyDeviceToHost));
+    }
+    return fetch_results(H, I, out, fs, res);
+}
+
 int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_lattice) {
     DevArena A;
     PtInstance I;
@@ -656,6 +687,10 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
     for (int c = 0; c < H.k; ++c) maxc = std::max(maxc, H.color_ptr[c + 1] - H.color_ptr[c]);
     const int64_t threads = (int64_t)((maxc + 3) / 4 + 1) * Rp;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((threads + block - 1) / block, 2048));
+    // CSR colour steps run one attempt per lane: (block / Rp) positions per workgroup
+    std::vector<int> cgrid((size_t)H.k);
+    for (int c = 0; c < H.k; ++c)
+        cgrid[c] = std::max(1, (H.color_ptr[c + 1] - H.color_ptr[c] + block / Rp - 1) / (block / Rp));
     const int reg = std::max(1, std::min(64, (H.n + 4095) / 4096));
     PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));
     // optional per-launch timing of the colour-step kernel (the dominant kernel)
@@ -676,7 +711,7 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
                     PT_HIP(hipEventRecord(a, sg.s));
                 }
                 if (use_lattice) PT_HIP(pt_launch_g_lattice(I, G, c, j, rl, block, grid, sg.s));
-                else PT_HIP(pt_launch_g_color(I, G, c, j, rl, block, grid, sg.s));
+                else PT_HIP(pt_launch_g_color(I, G, c, j, rl, block, cgrid[c], sg.s));
                 if (tk) {
                     PT_HIP(hipEventRecord(b, sg.s));
                     ++timed;
@@ -725,27 +760,7 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         kernel_s += km * 1e-3;
     }
 
-    // final state lives in the buffers of parity n_rounds & 1
-    const int fin = H.n_rounds & 1;
-    std::vector<int64_t> out((size_t)pt_out_len(H.R));
-    PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-    int64_t best[5];
-    PT_HIP(hipMemcpy(best, G.best + 5 * fin, sizeof(best), hipMemcpyDeviceToHost));
-    std::vector<int64_t> E((size_t)Rp);
-    std::vector<int32_t> rung((size_t)Rp);
-    PT_HIP(hipMemcpy(E.data(), G.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
-    PT_HIP(hipMemcpy(rung.data(), G.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
-    for (int x = 0; x < 5; ++x) out[x] = best[x];
-    for (int r = 0; r < H.R; ++r) {
-        pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
-        pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
-    }
-    std::vector<int8_t> fs;
-    if (H.flags & DRIFT_PT_RECORD_FINAL) {
-        fs.resize((size_t)H.n * Rp);
-        PT_HIP(hipMemcpy(fs.data(), G.sp, fs.size(), hipMemcpyDeviceToHost));
-    }
-    rc = fetch_results(H, I, out, fs, res);
+    rc = finish_global(H, I, G, res);
     if (rc) return rc;
     res->seconds = ms * 1e-3;
     res->engine_used = use_lattice ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
@@ -758,6 +773,156 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         res->kernel_attempts = res->attempts;
         res->kernel_launches = 1;
     }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// coop engine: one cooperative launch, each workgroup's share of the rows resident in its LDS
+// ------------------------------------------------------------------------------------------
+struct CoopPlan {
+    int block = 0, grid = 0;
+    size_t lds = 0;
+    CoopArgs C{};
+};
+
+// LDS layout of one coop workgroup when the colour classes are split W ways; returns its bytes.
+size_t coop_layout(const HostInst& H, int W, CoopArgs& C) {
+    int64_t max_own = 0, max_edges = 0;   // the largest slice over the workgroups
+    for (int w = 0; w < W; ++w) {
+        int64_t own = 0, edges = 0;
+        for (int c = 0; c < H.k; ++c) {
+            const int cs = H.color_ptr[c], cz = H.color_ptr[c + 1] - cs;
+            const int lo = cs + (int)((int64_t)cz * w / W), hi = cs + (int)((int64_t)cz * (w + 1) / W);
+            own += hi - lo;
+            edges += H.row_ptr[hi] - H.row_ptr[lo];
+        }
+        max_own = std::max(max_own, own);
+        max_edges = std::max(max_edges, edges);
+    }
+    const size_t Rp = (size_t)H.Rp, ew = H.wm == PT_WM_I8 ? 1 : 2;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 15) & ~(size_t)15;
+        return (uint32_t)at;
+    };
+    C.off_cptr = take(4 * ((size_t)H.k + 1));
+    C.off_lrp = take(4 * ((size_t)max_own + H.k));      // one end offset per colour
+    C.off_edge = take(4 * ew * (size_t)max_edges);
+    C.off_lsp = take((size_t)max_own * Rp);              // own spins, [own position][slot]
+    const size_t thr_bytes = H.wm != PT_WM_F32 ? 4 * (size_t)H.R * H.TQ : 0;
+    C.thr_lds = thr_bytes > 0 && thr_bytes <= 32768;
+    C.off_thr = take(C.thr_lds ? thr_bytes : 0);
+    C.off_beta = take(8 * Rp);
+    C.off_betaf = take(4 * Rp);
+    C.off_Eb = take(8 * Rp);
+    C.off_Es = take(8 * Rp);
+    C.off_red = take(8 * Rp);
+    C.off_rep = take(4 * Rp);
+    C.off_rung = take(4 * Rp);
+    C.off_label = take(4 * Rp);
+    C.off_stats = take(5 * 8 * Rp);
+    C.off_scr = take(16 * 8 + 16 * 4);
+    C.off_sh = take(64);
+    return off;
+}
+
+// Largest co-resident grid whose slices fit LDS: up to 8 workgroups per CU, never more workgroups
+// than one pass over the largest colour class needs.  False when no split fits.
+bool coop_plan(const HostInst& H, int block_req, CoopPlan& P) {
+    int dev = 0, coop = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop)
+        return false;
+    const int Rp = H.Rp;
+    int block = block_req > 0 ? block_req : 1024;
+    block = std::max(block, Rp);
+    block = std::min(((block + Rp - 1) / Rp) * Rp, 1024);
+    int maxc = 1;
+    for (int c = 0; c < H.k; ++c) maxc = std::max(maxc, H.color_ptr[c + 1] - H.color_ptr[c]);
+    const int ppb = block / Rp;
+    const int need = (maxc + ppb - 1) / ppb;
+    const bool uni = Rp % 64 == 0;
+    for (int bpc = 8; bpc >= 1; --bpc) {
+        const int W = std::max(1, std::min(prop.multiProcessorCount * bpc, need));
+        CoopArgs C{};
+        const size_t lds = coop_layout(H, W, C);
+        if (lds > (size_t)device_lds_limit()) continue;
+        int per_cu = 0;
+        if (pt_coop_occupancy(H.wm, uni, block, lds, &per_cu) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if ((int64_t)per_cu * prop.multiProcessorCount < W) continue;
+        P.block = block;
+        P.grid = W;
+        P.lds = lds;
+        P.C = C;
+        return true;
+    }
+    return false;
+}
+
+int run_coop(const HostInst& H, drift_pt_result* res, CoopPlan P) {
+    DevArena A;
+    PtInstance I;
+    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
+    if (rc) return rc;
+    I.jbits = nullptr;
+    const int Rp = H.Rp;
+    GState G;
+    PT_HIP(A.alloc(&G.sp, (size_t)H.n * Rp));
+    PT_HIP(A.alloc(&G.Eb, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.dacc, 3 * (size_t)PT_DACC_SPLIT * Rp));
+    PT_HIP(A.alloc(&G.rung, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.rep, 2 * (size_t)Rp));
+    PT_HIP(A.alloc(&G.label, (size_t)Rp));
+    PT_HIP(A.alloc(&G.best, 10));
+    PT_HIP(A.alloc(&G.round_base, 1));
+    PT_HIP(A.alloc(&P.C.flags, (size_t)P.grid));
+    PT_HIP(A.alloc(&P.C.gen, 64));
+    PT_HIP(A.alloc(&P.C.abort, 1));
+    StreamGuard sg;
+    PT_HIP(hipStreamCreate(&sg.s));
+    PT_HIP(hipEventCreate(&sg.e0));
+    PT_HIP(hipEventCreate(&sg.e1));
+    PT_HIP(hipMemsetAsync(G.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.dacc, 0, 3 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.label, 0, sizeof(int32_t) * Rp, sg.s));
+    PT_HIP(hipMemsetAsync(G.best, 0, 10 * sizeof(int64_t), sg.s));
+    PT_HIP(hipMemsetAsync(G.round_base, 0, sizeof(int32_t), sg.s));
+    PT_HIP(hipMemsetAsync(P.C.flags, 0, sizeof(uint32_t) * P.grid, sg.s));
+    PT_HIP(hipMemsetAsync(P.C.gen, 0, sizeof(uint32_t), sg.s));
+    PT_HIP(hipMemsetAsync(P.C.abort, 0, sizeof(int32_t), sg.s));
+    // initial spins, energies and round-0 state from the global engine's init kernels
+    const int iblock = std::min(((std::max(256, Rp) + Rp - 1) / Rp) * Rp, 1024);
+    const int reg = std::max(1, std::min(64, (H.n + 4095) / 4096));
+    PT_HIP(pt_launch_g_init(I, G, iblock, reg, sg.s));
+    PT_HIP(hipMemsetAsync(G.dacc, 0, 3 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
+    int dev = 0, khz = 0;
+    PT_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    P.C.timeout_ticks = (unsigned long long)khz * 1000ull * 20ull;   // 20 s per barrier
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    if (H.n_rounds > 0) PT_HIP(pt_launch_coop(I, G, P.C, P.block, P.grid, P.lds, sg.s));
+    PT_HIP(hipEventRecord(sg.e1, sg.s));
+    PT_HIP(hipStreamSynchronize(sg.s));
+    float ms = 0.0f;
+    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    int32_t aborted = 0;
+    PT_HIP(hipMemcpy(&aborted, P.C.abort, sizeof(aborted), hipMemcpyDeviceToHost));
+    if (aborted) return fail(DRIFT_PT_EHIP, "coop engine: a grid barrier wait exceeded its bound");
+    rc = finish_global(H, I, G, res);
+    if (rc) return rc;
+    res->seconds = ms * 1e-3;
+    res->engine_used = DRIFT_PT_ENGINE_COOP;
+    res->kernel_seconds = res->seconds;
+    res->kernel_attempts = res->attempts;
+    res->kernel_launches = 1;
     return 0;
 }
 
@@ -939,8 +1104,18 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     int engine = params->engine;
     if (engine == DRIFT_PT_ENGINE_AUTO) {
         if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) engine = DRIFT_PT_ENGINE_LDS;
-        else if (H.ldim) engine = slab_fits(H) ? DRIFT_PT_ENGINE_SLAB : DRIFT_PT_ENGINE_LATTICE;
-        else engine = DRIFT_PT_ENGINE_GLOBAL;
+        else if (H.ldim && slab_fits(H)) engine = DRIFT_PT_ENGINE_SLAB;
+        else {
+            CoopPlan P;
+            if (coop_plan(H, params->block_threads, P)) return run_coop(H, result, P);
+            engine = H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+        }
+    }
+    if (engine == DRIFT_PT_ENGINE_COOP) {
+        CoopPlan P;
+        if (!coop_plan(H, params->block_threads, P))
+            return fail(DRIFT_PT_EINVAL, "coop engine: no co-resident grid whose row slices fit LDS");
+        return run_coop(H, result, P);
     }
     if (engine == DRIFT_PT_ENGINE_SLAB) {
         if (!slab_fits(H))

@@ -1,0 +1,385 @@
+// pt_coop.hip -- persistent CSR engine for instances whose replicas do not fit one workgroup.
+//
+// One cooperative launch runs every round.  Workgroup w owns a fixed slice of every colour class
+// for the whole run.  Its LDS holds the row offsets and edge records of its slice, the spins of
+// its own positions (all slots), the Metropolis thresholds and the ladder, so a colour step reads
+// only neighbour spins from memory.  Spins also live in HBM as [position][slot] (Rp bytes per
+// position): the owner writes every flip through to it for the other workgroups.  Colour steps
+// are separated by a grid barrier; the barrier after the last colour of a round also publishes
+// the round's energy sums, and then every workgroup evaluates the same round end (best state,
+// exchange, labels) from those sums -- the decisions are deterministic, so the copies agree --
+// and workgroup 0 alone keeps the statistics.  Lane (i, a) updates rank i of its slice in slot a
+// with word (p & 3) of the Philox block (p >> 2, a, t), the uniform every engine draws for that
+// attempt, so results are bit-identical to the other engines.
+//
+// Reference counterpart: the per-launch sweep and swap kernels of cuda/ising_pt.cu:86-146.
+#include "pt_engine.h"
+#include "pt_rules.h"
+#include "pt_device.h"
+
+#include <type_traits>
+
+namespace {
+
+using namespace pt_dev;
+
+struct CoopShared {
+    long long best[5];   // E, round, rung, slot, have
+    int abort;
+};
+
+// Every access one workgroup makes to data another workgroup wrote in this launch (spins, energy
+// sums, barrier flags) is an agent-scope relaxed atomic: coherent across the chip without cache
+// maintenance.
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_u32(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int64_t ld_coh(const int64_t* p) {
+    return __hip_atomic_load(const_cast<int64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier with a bounded wait.  Each wave waits for its own stores to complete; then every
+// workgroup stores the phase number into its own arrival flag, workgroup 0 polls all flags (one
+// thread per flag) and publishes the phase in the generation word, which the others poll.  No
+// read-modify-write is involved, so arrivals do not serialise on one address, and only workgroup
+// 0 reads the W flags.  A wait longer than timeout_ticks (or another workgroup's abort) sets the
+// abort flag; the caller then leaves the kernel, so every wave reaches the end.
+__device__ __forceinline__ bool wait_phase(const CoopArgs& C, const uint32_t* p, uint32_t phase) {
+    const unsigned long long t0 = (unsigned long long)wall_clock64();
+    while (ld_u32(p) < phase) {
+        if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
+            __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool coop_sync(const CoopArgs& C, uint32_t& phase, int* sh_abort) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    ++phase;
+    if (blockIdx.x == 0) {
+        for (int x = 1 + threadIdx.x; x < (int)gridDim.x; x += blockDim.x)
+            if (!wait_phase(C, C.flags + x, phase)) *sh_abort = 1;
+        __syncthreads();
+        if (threadIdx.x == 0 && *sh_abort == 0) st_u32(C.gen, phase);
+    } else if (threadIdx.x == 0) {
+        st_u32(C.flags + blockIdx.x, phase);
+        if (!wait_phase(C, C.gen, phase)) *sh_abort = 1;
+    }
+    __syncthreads();
+    return *sh_abort == 0;
+}
+
+// First rank and size of workgroup w's slice of colour class [cs, cs + cz).
+__device__ __forceinline__ void slice(int cs, int cz, int w, int W, int& lo, int& cnt) {
+    lo = cs + (int)((int64_t)cz * w / W);
+    cnt = cs + (int)((int64_t)cz * (w + 1) / W) - lo;
+}
+
+template <int WM>
+using field_t = std::conditional_t<WM == PT_WM_F32, float, int32_t>;
+
+template <int WM>
+__device__ __forceinline__ int edge_at(const uint32_t* e, int t, field_t<WM>& w) {
+    if constexpr (WM == PT_WM_F32) return edge_float(e, t, w);
+    else return edge_int<WM>(e, t, w);
+}
+
+// Local fields of two rows [s0, e0) and [s1, e1) (an empty row gives its start value).  The first
+// PT_PAIR_CHUNK edges of both rows are gathered together -- edge records from LDS, then every
+// neighbour spin from memory -- so the two rows' loads are in flight at once; longer rows finish
+// chunk by chunk.  Terms are added in row order.
+#define PT_PAIR_CHUNK 8
+
+template <int WM>
+__device__ __forceinline__ void fields2(int a, int Rp, const int8_t* sp, const uint32_t* edge,
+                                        int s0, int e0, int s1, int e1, field_t<WM>& l0,
+                                        field_t<WM>& l1) {
+    constexpr int K = PT_PAIR_CHUNK;
+    int8_t v0[K], v1[K];
+    field_t<WM> w0[K], w1[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const int c0 = s0 + u < e0 ? edge_at<WM>(edge, s0 + u, w0[u]) : -1;
+        const int c1 = s1 + u < e1 ? edge_at<WM>(edge, s1 + u, w1[u]) : -1;
+        v0[u] = c0 >= 0 ? ld_spin<true>(sp + (size_t)c0 * Rp + a) : 0;
+        v1[u] = c1 >= 0 ? ld_spin<true>(sp + (size_t)c1 * Rp + a) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        if (s0 + u < e0) l0 = v0[u] > 0 ? l0 + w0[u] : l0 - w0[u];
+        if (s1 + u < e1) l1 = v1[u] > 0 ? l1 + w1[u] : l1 - w1[u];
+    }
+    for (int t = s0 + K; t < e0; ++t) {
+        field_t<WM> w;
+        const int c = edge_at<WM>(edge, t, w);
+        l0 = ld_spin<true>(sp + (size_t)c * Rp + a) > 0 ? l0 + w : l0 - w;
+    }
+    for (int t = s1 + K; t < e1; ++t) {
+        field_t<WM> w;
+        const int c = edge_at<WM>(edge, t, w);
+        l1 = ld_spin<true>(sp + (size_t)c * Rp + a) > 0 ? l1 + w : l1 - w;
+    }
+}
+
+// Metropolis decision for spin si with local field l: true when the flip is accepted (a flip
+// with zero energy change is accepted), with its energy change in units in d.
+template <int WM>
+__device__ __forceinline__ bool decide(int8_t si, field_t<WM> l, uint32_t u,
+                                       const uint32_t* thr_row, int TQ, float bf, int64_t& d) {
+    if constexpr (WM == PT_WM_F32) {
+        const float q = si > 0 ? l : -l;
+        d = pt_fix36(2.0f * q);
+        return pt_accept_float(bf, q, u);
+    } else {
+        const int32_t q = si > 0 ? l : -l;
+        d = 2 * (int64_t)q;
+        return q <= 0 || (q <= TQ && u < thr_row[q - 1]);
+    }
+}
+
+template <int WM>
+__device__ __forceinline__ field_t<WM> field_h(const void* h, int p) {
+    if constexpr (WM == PT_WM_F32) return h ? ((const float*)h)[p] : 0.0f;
+    else return h ? ((const int32_t*)h)[p] : 0;
+}
+
+template <int WM, bool UNI>
+__global__ __launch_bounds__(1024) void coop_run(PtInstance I, GState G, CoopArgs C) {
+    extern __shared__ __align__(16) char smem[];
+    int32_t* cptr = (int32_t*)(smem + C.off_cptr);
+    int32_t* lrp = (int32_t*)(smem + C.off_lrp);
+    uint32_t* ledge = (uint32_t*)(smem + C.off_edge);
+    int8_t* lsp = (int8_t*)(smem + C.off_lsp);
+    uint32_t* thrL = (uint32_t*)(smem + C.off_thr);
+    double* betaL = (double*)(smem + C.off_beta);
+    float* betafL = (float*)(smem + C.off_betaf);
+    int64_t* EbL = (int64_t*)(smem + C.off_Eb);
+    int64_t* Es = (int64_t*)(smem + C.off_Es);
+    long long* red = (long long*)(smem + C.off_red);
+    int32_t* repL = (int32_t*)(smem + C.off_rep);
+    int32_t* rungL = (int32_t*)(smem + C.off_rung);
+    int32_t* labelL = (int32_t*)(smem + C.off_label);
+    int64_t* stats = (int64_t*)(smem + C.off_stats);   // tried, acc, trips, up, down (Rp each)
+    long long* scrE = (long long*)(smem + C.off_scr);
+    int* scrR = (int*)(smem + C.off_scr + 16 * sizeof(long long));
+    CoopShared* sh = (CoopShared*)(smem + C.off_sh);
+
+    const int tid = threadIdx.x, nt = blockDim.x, w = blockIdx.x, W = gridDim.x;
+    const int k = I.k, R = I.R, Rp = I.Rp;
+    constexpr int ew = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
+    const bool owner = w == 0;
+
+    // ---- stage this workgroup's rows and spins, the thresholds, the ladder, round-0 state ----
+    for (int x = tid; x <= k; x += nt) cptr[x] = I.color_ptr[x];
+    __syncthreads();
+    {
+        int ob = 0, ebase = 0;
+        for (int c = 0; c < k; ++c) {
+            int lo, cnt;
+            slice(cptr[c], cptr[c + 1] - cptr[c], w, W, lo, cnt);
+            const int e0 = I.row_ptr[lo], e1 = I.row_ptr[lo + cnt];
+            for (int i = tid; i <= cnt; i += nt) lrp[ob + c + i] = ebase + (I.row_ptr[lo + i] - e0);
+            const uint32_t* src = (const uint32_t*)I.edge + (size_t)e0 * ew;
+            uint32_t* dst = ledge + (size_t)ebase * ew;
+            for (int i = tid; i < (e1 - e0) * ew; i += nt) dst[i] = src[i];
+            for (int i = tid; i < cnt * Rp; i += nt) lsp[(size_t)ob * Rp + i] = G.sp[(size_t)lo * Rp + i];
+            ob += cnt;
+            ebase += e1 - e0;
+        }
+    }
+    const uint32_t* thr = I.thr;
+    if (WM != PT_WM_F32 && C.thr_lds) {
+        for (int x = tid; x < R * I.TQ; x += nt) thrL[x] = I.thr[x];
+        thr = thrL;
+    }
+    for (int x = tid; x < Rp; x += nt) {
+        betaL[x] = x < R ? I.beta[x] : 0.0;
+        betafL[x] = x < R ? (float)I.beta[x] : 0.0f;
+        EbL[x] = G.Eb[x];
+        repL[x] = G.rep[x];
+        rungL[x] = G.rung[x];
+        labelL[x] = G.label[x];
+        red[x] = 0;
+    }
+    for (int x = tid; x < 5 * Rp; x += nt) stats[x] = 0;
+    if (tid < 5) sh->best[tid] = G.best[tid];
+    if (tid == 0) sh->abort = 0;
+    __syncthreads();
+
+    const int a = tid % Rp;
+    const int pl = UNI ? __builtin_amdgcn_readfirstlane(tid / Rp) : tid / Rp;
+    const int ppb = nt / Rp;
+    const bool live = a < R;
+    const int split = w % PT_DACC_SPLIT;
+    uint32_t phase = 0;
+    int64_t dsum = 0;
+    for (int rnd = 0; rnd < I.n_rounds; ++rnd) {
+        const int buf = rnd % 3;
+        const int g = live ? rungL[a] : 0;
+        const float bf = betafL[g];
+        const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : thr + (size_t)g * I.TQ;
+        for (int j = 0; j < I.S; ++j) {
+            const uint32_t t = (uint32_t)(rnd * I.S + j);
+            int ob = 0;
+            for (int c = 0; c < k; ++c) {
+                int lo, cnt;
+                slice(cptr[c], cptr[c + 1] - cptr[c], w, W, lo, cnt);
+                const int* rows = lrp + ob + c;
+                int8_t* own = lsp + (size_t)ob * Rp + a;
+                if (live)
+                    for (int i0 = pl; i0 < cnt; i0 += 2 * ppb) {
+                        const int i1 = i0 + ppb;
+                        const bool has1 = i1 < cnt;
+                        const int p0 = lo + i0, p1 = lo + i1;
+                        const int s0 = rows[i0], e0 = rows[i0 + 1];
+                        const int s1 = has1 ? rows[i1] : 0, e1 = has1 ? rows[i1 + 1] : 0;
+                        field_t<WM> l0 = field_h<WM>(I.h, p0);
+                        field_t<WM> l1 = has1 ? field_h<WM>(I.h, p1) : field_t<WM>(0);
+                        fields2<WM>(a, Rp, G.sp, ledge, s0, e0, s1, e1, l0, l1);
+                        const pt_u4 r0 = pt_philox((uint32_t)(p0 >> 2), (uint32_t)a, t,
+                                                   PT_TAG_METRO, I.k0, I.k1);
+                        const int8_t si0 = own[(size_t)i0 * Rp];
+                        int64_t d0;
+                        if (decide<WM>(si0, l0, pt_word(r0, p0 & 3), thr_row, I.TQ, bf, d0)) {
+                            own[(size_t)i0 * Rp] = (int8_t)-si0;
+                            st_spin<true>(G.sp + (size_t)p0 * Rp + a, (int8_t)-si0);
+                            dsum += d0;
+                        }
+                        if (has1) {
+                            const pt_u4 r1 = pt_philox((uint32_t)(p1 >> 2), (uint32_t)a, t,
+                                                       PT_TAG_METRO, I.k0, I.k1);
+                            const int8_t si1 = own[(size_t)i1 * Rp];
+                            int64_t d1;
+                            if (decide<WM>(si1, l1, pt_word(r1, p1 & 3), thr_row, I.TQ, bf, d1)) {
+                                own[(size_t)i1 * Rp] = (int8_t)-si1;
+                                st_spin<true>(G.sp + (size_t)p1 * Rp + a, (int8_t)-si1);
+                                dsum += d1;
+                            }
+                        }
+                    }
+                ob += cnt;
+                if (j == I.S - 1 && c == k - 1) {
+                    // the round's energy change of this workgroup, one atomic per slot
+                    if (dsum != 0) atomicAdd((unsigned long long*)&red[a], (unsigned long long)dsum);
+                    dsum = 0;
+                    __syncthreads();
+                    int64_t* dst = G.dacc + ((size_t)buf * PT_DACC_SPLIT + split) * Rp;
+                    for (int x = tid; x < R; x += nt) {
+                        if (red[x] != 0) atomicAdd((unsigned long long*)&dst[x], (unsigned long long)red[x]);
+                        red[x] = 0;
+                    }
+                }
+                if (!coop_sync(C, phase, &sh->abort)) return;
+            }
+        }
+
+        // ---- round end: identical in every workgroup ---------------------------------------
+        for (int x = tid; x < R; x += nt) {
+            int64_t v = EbL[x];
+#pragma unroll 8
+            for (int s = 0; s < PT_DACC_SPLIT; ++s)
+                v += ld_coh(G.dacc + ((size_t)buf * PT_DACC_SPLIT + s) * Rp + x);
+            Es[x] = v;
+        }
+        __syncthreads();
+        int64_t mE;
+        int32_t mR;
+        block_best_rung(Es, repL, R, scrE, scrR, mE, mR);
+        const bool improved = sh->best[4] == 0 || mE < sh->best[0];
+        const int slot = repL[mR];
+        if (improved) {
+            // this workgroup's positions of the best slot, from its own LDS copy
+            int ob = 0;
+            for (int c = 0; c < k; ++c) {
+                int lo, cnt;
+                slice(cptr[c], cptr[c + 1] - cptr[c], w, W, lo, cnt);
+                for (int i = tid; i < cnt; i += nt) I.best_s[lo + i] = lsp[(size_t)(ob + i) * Rp + slot];
+                ob += cnt;
+            }
+        }
+        if (owner) {
+            if (tid == 0 && I.hist) I.hist[rnd] = (double)(improved ? mE : sh->best[0]) * I.unit;
+            if (I.trace)
+                for (int r = tid; r < R; r += nt)
+                    I.trace[(size_t)rnd * R + r] = (double)Es[repL[r]] * I.unit;
+        }
+        __syncthreads();   // best and rep are read above before they change below
+        if (tid == 0 && improved) {
+            sh->best[0] = mE;
+            sh->best[1] = rnd + 1;
+            sh->best[2] = mR;
+            sh->best[3] = slot;
+            sh->best[4] = 1;
+        }
+        exchange_pairs(I, betaL, rnd, Es, repL, rungL, owner ? stats : nullptr,
+                       owner ? stats + Rp : nullptr);
+        __syncthreads();
+        update_labels(R, rungL, labelL, nullptr, owner ? stats + 2 * Rp : nullptr,
+                      owner ? stats + 3 * Rp : nullptr, owner ? stats + 4 * Rp : nullptr);
+        for (int x = tid; x < R; x += nt) EbL[x] = Es[x];
+        if (owner) {
+            // buffer of round rnd + 2: its last readers finished before this round's barriers
+            int64_t* z = G.dacc + (size_t)((rnd + 2) % 3) * PT_DACC_SPLIT * Rp;
+            for (int x = tid; x < PT_DACC_SPLIT * Rp; x += nt)
+                __hip_atomic_store(z + x, (int64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+
+    // final state in the buffers of parity n_rounds & 1, where the host reads it
+    if (owner) {
+        const int fin = I.n_rounds & 1;
+        for (int x = tid; x < R; x += nt) {
+            G.Eb[(size_t)fin * Rp + x] = EbL[x];
+            G.rung[(size_t)fin * Rp + x] = rungL[x];
+            G.rep[(size_t)fin * Rp + x] = repL[x];
+            G.label[x] = labelL[x];
+            pt_out_arr(I.out, R, PT_OUT_TRIED)[x] = stats[x];
+            pt_out_arr(I.out, R, PT_OUT_ACC)[x] = stats[Rp + x];
+            pt_out_arr(I.out, R, PT_OUT_TRIPS)[x] = stats[2 * Rp + x];
+            pt_out_arr(I.out, R, PT_OUT_UP)[x] = stats[3 * Rp + x];
+            pt_out_arr(I.out, R, PT_OUT_DOWN)[x] = stats[4 * Rp + x];
+        }
+        if (tid < 5) G.best[5 * fin + tid] = sh->best[tid];
+    }
+}
+
+template <int WM, bool UNI>
+const void* coop_fn() {
+    return reinterpret_cast<const void*>(&coop_run<WM, UNI>);
+}
+
+const void* coop_pick(int wm, bool uni) {
+    if (wm == PT_WM_I8) return uni ? coop_fn<PT_WM_I8, true>() : coop_fn<PT_WM_I8, false>();
+    if (wm == PT_WM_I32) return uni ? coop_fn<PT_WM_I32, true>() : coop_fn<PT_WM_I32, false>();
+    return uni ? coop_fn<PT_WM_F32, true>() : coop_fn<PT_WM_F32, false>();
+}
+
+}  // namespace
+
+hipError_t pt_coop_occupancy(int wm, bool uni, int block, size_t lds, int* per_cu) {
+    const void* f = coop_pick(wm, uni);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, block, lds);
+}
+
+hipError_t pt_launch_coop(const PtInstance& I, const GState& G, const CoopArgs& C, int block,
+                          int grid, size_t lds, hipStream_t s) {
+    PtInstance i = I;
+    GState g = G;
+    CoopArgs c = C;
+    void* args[] = {(void*)&i, (void*)&g, (void*)&c};
+    return hipLaunchCooperativeKernel(coop_pick(I.wm, I.Rp % 64 == 0), dim3(grid), dim3(block),
+                                      args, (unsigned)lds, s);
+}

@@ -1,5 +1,6 @@
 // pt_device.h -- device helpers shared by the engines: best-rung scan, exchange, round-trip
-// labels and the initial spin of the stream contract.  Internal to libdriftpt.so.
+// labels, the initial spin of the stream contract and the CSR site update.  Internal to
+// libdriftpt.so.
 #pragma once
 #include <climits>
 
@@ -54,16 +55,17 @@ __device__ __forceinline__ int8_t init_spin(const PtInstance& I, int p, int a) {
 // Exchange of one round for the pairs (r, r+1), r = rnd%2 + 2m (parallel_tempering.py:104-113),
 // with the permutation rung<->slot instead of moving configurations.  Every pair is disjoint, so
 // each thread owns one pair.  Caller syncs before and after; null counters are not updated.
-__device__ __forceinline__ void exchange_pairs(const PtInstance& I, int rnd, const int64_t* E,
-                                               int32_t* rep, int32_t* rung, int64_t* tried,
-                                               int64_t* accd) {
+// beta may be a copy of the instance's ladder in LDS.
+__device__ __forceinline__ void exchange_pairs(const PtInstance& I, const double* beta, int rnd,
+                                               const int64_t* E, int32_t* rep, int32_t* rung,
+                                               int64_t* tried, int64_t* accd) {
     const int R = I.R;
     for (int m = threadIdx.x;; m += blockDim.x) {
         const int r = (rnd & 1) + 2 * m;
         if (r + 1 >= R) break;
         const int a = rep[r], b = rep[r + 1];
         const pt_u4 rv = pt_philox((uint32_t)(r >> 2), 0u, (uint32_t)rnd, PT_TAG_EXCH, I.k0, I.k1);
-        const bool ok = pt_accept_exchange(I.beta[r], I.beta[r + 1], (double)E[a] * I.unit,
+        const bool ok = pt_accept_exchange(beta[r], beta[r + 1], (double)E[a] * I.unit,
                                            (double)E[b] * I.unit, pt_word(rv, r & 3));
         if (tried) tried[r] += 1;
         if (ok) {
@@ -76,8 +78,15 @@ __device__ __forceinline__ void exchange_pairs(const PtInstance& I, int rnd, con
     }
 }
 
+__device__ __forceinline__ void exchange_pairs(const PtInstance& I, int rnd, const int64_t* E,
+                                               int32_t* rep, int32_t* rung, int64_t* tried,
+                                               int64_t* accd) {
+    exchange_pairs(I, I.beta, rnd, E, rep, rung, tried, accd);
+}
+
 // Round-trip labels after an exchange: a slot reaching the coldest rung with a "hot" label
 // completes a round trip.  rung is a permutation, so per-rung counters have one writer each.
+// Null counters are not updated.
 __device__ __forceinline__ void update_labels(int R, const int32_t* rung, int32_t* label,
                                               int32_t* trips32, int64_t* trips64, int64_t* up,
                                               int64_t* down) {
@@ -95,8 +104,147 @@ __device__ __forceinline__ void update_labels(int R, const int32_t* rung, int32_
             lb = -1;
         }
         label[a] = lb;
-        if (lb == 1) up[g] += 1;
-        if (lb == -1) down[g] += 1;
+        if (up && lb == 1) up[g] += 1;
+        if (down && lb == -1) down[g] += 1;
+    }
+}
+
+// Edge t of the CSR (row order): neighbour position and weight.
+template <int WM>
+__device__ __forceinline__ int edge_int(const void* __restrict__ e, int t, int32_t& wt) {
+    if constexpr (WM == PT_WM_I8) {
+        const uint32_t v = ((const uint32_t*)e)[t];
+        wt = (int32_t)(int8_t)(v & 0xffu);
+        return (int)(v >> 8);
+    } else {
+        const uint2 v = ((const uint2*)e)[t];
+        wt = (int32_t)v.y;
+        return (int)v.x;
+    }
+}
+
+__device__ __forceinline__ int edge_float(const void* __restrict__ e, int t, float& wt) {
+    const uint2 v = ((const uint2*)e)[t];
+    wt = __uint_as_float(v.y);
+    return (int)v.x;
+}
+
+// Spin access.  COH accesses are agent-scope relaxed atomics (loads and stores coherent across
+// the chip without cache maintenance): the coop engine's workgroups read spins other workgroups
+// wrote in the same launch.  Other engines use plain accesses.
+template <bool COH>
+__device__ __forceinline__ int8_t ld_spin(const int8_t* p) {
+    if constexpr (COH)
+        return __hip_atomic_load(const_cast<int8_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+
+template <bool COH>
+__device__ __forceinline__ void st_spin(int8_t* p, int8_t v) {
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+// Local field of position p in slot a: the sum over its row of w_t * s_col(t), in row order.
+// Rows are gathered in chunks of G edges: every edge record of a chunk is loaded, then every
+// neighbour spin, then the terms are added in order, so the loads of a chunk overlap instead of
+// forming one dependent edge -> spin chain per neighbour.  Spins in HBM use G = 8; spins in LDS
+// use G = 1 (the plain loop), where the extra predicated lanes only cost issue slots.
+template <int WM, int G, bool COH = false>
+__device__ __forceinline__ int32_t field_int(int a, int Rp, const int8_t* sp, int t0, int t1,
+                                             const void* __restrict__ edge, int32_t l) {
+    for (int t = t0; t < t1; t += G) {
+        int c[G];
+        int32_t w[G];
+        int8_t v[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = t + u < t1 ? edge_int<WM>(edge, t + u, w[u]) : 0;
+#pragma unroll
+        for (int u = 0; u < G; ++u) v[u] = t + u < t1 ? ld_spin<COH>(sp + (size_t)c[u] * Rp + a) : 0;
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+            if (t + u < t1) l += v[u] > 0 ? w[u] : -w[u];
+    }
+    return l;
+}
+
+template <int G, bool COH = false>
+__device__ __forceinline__ float field_float(int a, int Rp, const int8_t* sp, int t0, int t1,
+                                             const void* __restrict__ edge, float l) {
+    for (int t = t0; t < t1; t += G) {
+        int c[G];
+        float w[G];
+        int8_t v[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = t + u < t1 ? edge_float(edge, t + u, w[u]) : 0;
+#pragma unroll
+        for (int u = 0; u < G; ++u) v[u] = t + u < t1 ? ld_spin<COH>(sp + (size_t)c[u] * Rp + a) : 0;
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+            if (t + u < t1) l = v[u] > 0 ? l + w[u] : l - w[u];
+    }
+    return l;
+}
+
+// One Metropolis attempt at position p of slot a; returns the accepted energy change in units
+// (0 if rejected).  sp is [position][slot] with Rp slots per position (LDS or HBM).
+// The row of p is edge[t0 .. t1).
+template <int WM, int G = 1, bool COH = false>
+__device__ __forceinline__ int64_t site_update_row(int p, int a, int Rp, int8_t* sp, int t0, int t1,
+                                                   const void* __restrict__ edge, const void* h,
+                                                   uint32_t u, const uint32_t* thr_row, int TQ,
+                                                   float beta_f) {
+    int8_t* own = sp + (size_t)p * Rp + a;
+    const int8_t si = ld_spin<COH>(own);
+    if constexpr (WM == PT_WM_F32) {
+        const float l = field_float<G, COH>(a, Rp, sp, t0, t1, edge, h ? ((const float*)h)[p] : 0.0f);
+        const float q = si > 0 ? l : -l;
+        if (pt_accept_float(beta_f, q, u)) {
+            st_spin<COH>(own, (int8_t)-si);
+            return pt_fix36(2.0f * q);
+        }
+        return 0;
+    } else {
+        const int32_t l = field_int<WM, G, COH>(a, Rp, sp, t0, t1, edge,
+                                                h ? ((const int32_t*)h)[p] : 0);
+        const int32_t q = si > 0 ? l : -l;
+        const bool acc = q <= 0 || (q <= TQ && u < thr_row[q - 1]);
+        if (acc) {
+            st_spin<COH>(own, (int8_t)-si);
+            return 2 * (int64_t)q;
+        }
+        return 0;
+    }
+}
+
+template <int WM, int G = 1>
+__device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
+                                               const int32_t* __restrict__ rp,
+                                               const void* __restrict__ edge, const void* h,
+                                               uint32_t u, const uint32_t* thr_row, int TQ,
+                                               float beta_f) {
+    return site_update_row<WM, G>(p, a, Rp, sp, rp[p], rp[p + 1], edge, h, u, thr_row, TQ, beta_f);
+}
+
+// Energy contribution of position p in slot a at initialisation.  Integer modes return
+// s_p (sum_j J s_j + 2 h_p) (E = -total/2); float returns fix36(-s_p (l_J/2 + h_p)).
+template <int WM, int G = 1>
+__device__ __forceinline__ int64_t site_energy(int p, int a, int Rp, const int8_t* sp,
+                                               const int32_t* __restrict__ rp,
+                                               const void* __restrict__ edge, const void* h) {
+    const int t0 = rp[p], t1 = rp[p + 1];
+    const int8_t si = sp[(size_t)p * Rp + a];
+    if constexpr (WM == PT_WM_F32) {
+        const float lj = field_float<G>(a, Rp, sp, t0, t1, edge, 0.0f);
+        const float v = 0.5f * lj + (h ? ((const float*)h)[p] : 0.0f);
+        return pt_fix36(si > 0 ? -v : v);
+    } else {
+        // |sum_j J s_j| < 2^31 is checked on the host (maximum field), so int32 is exact
+        const int64_t lj = field_int<WM, G>(a, Rp, sp, t0, t1, edge, 0);
+        const int64_t hh = h ? (int64_t)((const int32_t*)h)[p] : 0;
+        const int64_t v = lj + 2 * hh;
+        return si > 0 ? v : -v;
     }
 }
 

@@ -23,8 +23,9 @@ inline int64_t pt_out_len(int R) { return PT_OUT_HDR + (int64_t)PT_OUT_NARR * R;
 // is the caller's spin index.  Device pointers throughout.
 struct PtInstance {
     const int32_t* row_ptr;   // n + 1
-    const int32_t* col;       // nnz (positions)
-    const void* w;            // nnz: int8 / int32 / float
+    const void* edge;         // nnz edge records in row order: int8 weights -> uint32
+                              // (position << 8 | weight byte); int32 / float -> uint2
+                              // {position, weight bits}.  One load per neighbour.
     const void* h;            // n (position order) int32 / float, or null
     const int32_t* color_ptr; // k + 1
     const int32_t* orig;      // n
@@ -44,7 +45,7 @@ struct PtInstance {
     int32_t ldim, lL;         // lattice dimension and side (0 when not a lattice)
 };
 
-#define PT_DACC_SPLIT 8
+#define PT_DACC_SPLIT 32
 
 // Global-engine state (replica-interleaved spins in HBM).  Two-deep buffers are indexed by
 // round parity so the many workgroups of one launch only ever read what another launch wrote.
@@ -77,6 +78,26 @@ struct SState {
     int32_t* round_base;
     int32_t NB;        // slabs per replica (1: the whole lattice wraps inside one workgroup)
 };
+
+// Persistent CSR engine (one cooperative launch).  Workgroup w of W owns ranks
+// [cz w / W, cz (w+1) / W) of every colour class (cz = class size) for the whole run, with their
+// row offsets and edge records copied into its LDS.  GState.dacc has three buffers here (round
+// r uses r % 3).  Offsets are byte offsets into the dynamic LDS of one workgroup.
+struct CoopArgs {
+    uint32_t* flags;                // W arrival flags: the last barrier phase each workgroup
+                                    // reached (start at 0)
+    uint32_t* gen;                  // last phase every workgroup reached (starts at 0)
+    int32_t* abort;                 // set when a barrier wait exceeded timeout_ticks
+    unsigned long long timeout_ticks;  // wall_clock64 ticks one barrier may wait
+    int32_t thr_lds;                // thresholds staged in LDS
+    uint32_t off_cptr, off_lrp, off_edge, off_lsp, off_thr, off_beta, off_betaf, off_Eb, off_Es,
+        off_red, off_rep, off_rung, off_label, off_stats, off_scr, off_sh;
+};
+
+hipError_t pt_launch_coop(const PtInstance& I, const GState& G, const CoopArgs& C, int block,
+                          int grid, size_t lds, hipStream_t s);
+// Workgroups of the coop kernel one CU holds at this block size and LDS size.
+hipError_t pt_coop_occupancy(int wm, bool uni, int block, size_t lds, int* per_cu);
 
 size_t pt_slab_lds_bytes(int n, int L, int dim, int NB);
 hipError_t pt_launch_slab_init(const PtInstance& I, const SState& S, hipStream_t s);

@@ -29,77 +29,6 @@ namespace {
 
 using namespace pt_dev;
 
-// One Metropolis attempt at position p of slot a; returns the accepted energy change in units
-// (0 if rejected).  sp is [position][slot] with Rp slots per position (LDS or HBM).
-template <int WM>
-__device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
-                                               const int32_t* __restrict__ rp,
-                                               const int32_t* __restrict__ col, const void* w,
-                                               const void* h, uint32_t u,
-                                               const uint32_t* thr_row, int TQ, float beta_f) {
-    const int t0 = rp[p], t1 = rp[p + 1];
-    const int8_t si = sp[(size_t)p * Rp + a];
-    if constexpr (WM == PT_WM_F32) {
-        const float* wf = (const float*)w;
-        float l = h ? ((const float*)h)[p] : 0.0f;
-        for (int t = t0; t < t1; ++t) {
-            const float wt = wf[t];
-            l = sp[(size_t)col[t] * Rp + a] > 0 ? l + wt : l - wt;
-        }
-        const float q = si > 0 ? l : -l;
-        if (pt_accept_float(beta_f, q, u)) {
-            sp[(size_t)p * Rp + a] = (int8_t)-si;
-            return pt_fix36(2.0f * q);
-        }
-        return 0;
-    } else {
-        int32_t l = h ? ((const int32_t*)h)[p] : 0;
-        for (int t = t0; t < t1; ++t) {
-            const int32_t wt = WM == PT_WM_I8 ? (int32_t)((const int8_t*)w)[t]
-                                              : ((const int32_t*)w)[t];
-            l += sp[(size_t)col[t] * Rp + a] > 0 ? wt : -wt;
-        }
-        const int32_t q = si > 0 ? l : -l;
-        const bool acc = q <= 0 || (q <= TQ && u < thr_row[q - 1]);
-        if (acc) {
-            sp[(size_t)p * Rp + a] = (int8_t)-si;
-            return 2 * (int64_t)q;
-        }
-        return 0;
-    }
-}
-
-// Energy contribution of position p in slot a at initialisation.  Integer modes return
-// s_p (sum_j J s_j + 2 h_p) (E = -total/2); float returns fix36(-s_p (l_J/2 + h_p)).
-template <int WM>
-__device__ __forceinline__ int64_t site_energy(int p, int a, int Rp, const int8_t* sp,
-                                               const int32_t* __restrict__ rp,
-                                               const int32_t* __restrict__ col, const void* w,
-                                               const void* h) {
-    const int t0 = rp[p], t1 = rp[p + 1];
-    const int8_t si = sp[(size_t)p * Rp + a];
-    if constexpr (WM == PT_WM_F32) {
-        const float* wf = (const float*)w;
-        float lj = 0.0f;
-        for (int t = t0; t < t1; ++t) {
-            const float wt = wf[t];
-            lj = sp[(size_t)col[t] * Rp + a] > 0 ? lj + wt : lj - wt;
-        }
-        const float v = 0.5f * lj + (h ? ((const float*)h)[p] : 0.0f);
-        return pt_fix36(si > 0 ? -v : v);
-    } else {
-        int64_t lj = 0;
-        for (int t = t0; t < t1; ++t) {
-            const int32_t wt = WM == PT_WM_I8 ? (int32_t)((const int8_t*)w)[t]
-                                              : ((const int32_t*)w)[t];
-            lj += sp[(size_t)col[t] * Rp + a] > 0 ? wt : -wt;
-        }
-        const int64_t hh = h ? (int64_t)((const int32_t*)h)[p] : 0;
-        const int64_t v = lj + 2 * hh;
-        return si > 0 ? v : -v;
-    }
-}
-
 // ------------------------------------------------------------------------------------------
 // LDS engine
 // ------------------------------------------------------------------------------------------
@@ -146,7 +75,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
         int64_t acc = 0;
         if (a < R)
             for (int p = qo; p < n; p += qs)
-                acc += site_energy<WM>(p, a, Rp, sp, I.row_ptr, I.col, I.w, I.h);
+                acc += site_energy<WM>(p, a, Rp, sp, I.row_ptr, I.edge, I.h);
         if (acc != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)acc);
     }
     __syncthreads();
@@ -177,7 +106,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                             for (int kk = 0; kk < 4; ++kk) {
                                 const int p = 4 * qq + kk;
                                 if (p < cs || p >= ce) continue;
-                                dsum += site_update<WM>(p, a, Rp, sp, I.row_ptr, I.col, I.w, I.h,
+                                dsum += site_update<WM>(p, a, Rp, sp, I.row_ptr, I.edge, I.h,
                                                         pt_word(rv, kk), thr_row, I.TQ, bf);
                             }
                         }
@@ -298,10 +227,14 @@ __global__ __launch_bounds__(1024) void g_init_energy(PtInstance I, GState G) {
     int64_t acc = 0;
     if (a < R)
         for (int p = blockIdx.x * qpb + qo; p < I.n; p += gridDim.x * qpb)
-            acc += site_energy<WM>(p, a, Rp, G.sp, I.row_ptr, I.col, I.w, I.h);
+            acc += site_energy<WM, 8>(p, a, Rp, G.sp, I.row_ptr, I.edge, I.h);
     block_slot_reduce(acc, a, R, Rp, red, G.dacc, 1);   // parity 1 holds the init totals
 }
 
+// Colour step of the CSR engine: one Metropolis attempt per lane, so a launch has (colour size x
+// slots) independent lanes in flight to cover the dependent row-pointer -> edge -> neighbour-spin
+// loads.  Lane (r, a) updates rank r of colour c in slot a with word (p & 3) of the Philox block
+// (p >> 2, a, t): the same uniform the LDS engine draws for that attempt.
 template <int WM, bool UNI>
 __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int c, int j,
                                                      int rl) {
@@ -312,28 +245,20 @@ __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int
     for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
     __syncthreads();
     // with Rp a multiple of 64 every wave holds one position: make that visible to the compiler
-    // so row pointers, columns and weights go through the scalar unit
-    const int a = tid % Rp, qpb = blockDim.x / Rp;
-    const int qo = UNI ? __builtin_amdgcn_readfirstlane(tid / Rp) : tid / Rp;
-    const int cs = I.color_ptr[c], ce = I.color_ptr[c + 1];
-    int64_t dsum = 0;
-    if (a < R && ce > cs) {
+    // so row pointers and edges go through the scalar unit
+    const int a = tid % Rp;
+    const int ro = UNI ? __builtin_amdgcn_readfirstlane(tid / Rp) : tid / Rp;
+    const int p = I.color_ptr[c] + blockIdx.x * (blockDim.x / Rp) + ro;
+    int64_t d = 0;
+    if (a < R && p < I.color_ptr[c + 1]) {
         const int g = G.rung[cur * Rp + a];
         const float bf = (float)I.beta[g];
         const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : I.thr + (size_t)g * I.TQ;
-        const int q1 = (ce - 1) >> 2;
-        for (int qq = (cs >> 2) + blockIdx.x * qpb + qo; qq <= q1; qq += gridDim.x * qpb) {
-            const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const int p = 4 * qq + kk;
-                if (p < cs || p >= ce) continue;
-                dsum += site_update<WM>(p, a, Rp, G.sp, I.row_ptr, I.col, I.w, I.h,
-                                        pt_word(rv, kk), thr_row, I.TQ, bf);
-            }
-        }
+        const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+        d = site_update<WM, 8>(p, a, Rp, G.sp, I.row_ptr, I.edge, I.h, pt_word(rv, p & 3), thr_row,
+                            I.TQ, bf);
     }
-    block_slot_reduce(dsum, a, R, Rp, red, G.dacc, cur);
+    block_slot_reduce(d, a, R, Rp, red, G.dacc, cur);
 }
 
 // Colour step on a periodic hypercubic lattice (L even) with +/-1 couplings and no field.

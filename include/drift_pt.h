@@ -48,8 +48,11 @@ enum drift_pt_engine {
     DRIFT_PT_ENGINE_GLOBAL = 2, /* replica-interleaved spins in HBM, one launch per colour step */
     DRIFT_PT_ENGINE_LATTICE = 3,/* the global engine's periodic-lattice kernel: implicit neighbours,
                                    one byte of bond signs per site (+/-1 couplings, no field) */
-    DRIFT_PT_ENGINE_SLAB = 4    /* periodic lattices: one launch per sweep, a workgroup per slab of
+    DRIFT_PT_ENGINE_SLAB = 4,   /* periodic lattices: one launch per sweep, a workgroup per slab of
                                    planes of one replica, halo planes recomputed in LDS */
+    DRIFT_PT_ENGINE_COOP = 5    /* CSR instances too large for one workgroup: one cooperative launch
+                                   runs every round; each workgroup keeps its share of the rows in
+                                   LDS, colour steps are separated by grid barriers */
 };
 
 enum drift_pt_flags {

@@ -66,7 +66,7 @@ def compare(model, *, R, n_rounds, S, seed, T_min=0.1, T_max=5.0, beta=None, eng
     return g, o, enc
 
 
-@pytest.mark.parametrize("engine", ["lds", "global"])
+@pytest.mark.parametrize("engine", ["lds", "global", "coop"])
 def test_maxcut_int8(engine):
     _announce(f"maxcut int8 {engine}")
     g, _, enc = compare(maxcut_ising(random_graph(24, p=0.5, seed=3)), R=16, n_rounds=60, S=4,
@@ -74,7 +74,7 @@ def test_maxcut_int8(engine):
     assert enc.weight_type == 1 and g.engine == engine
 
 
-@pytest.mark.parametrize("engine", ["lds", "global"])
+@pytest.mark.parametrize("engine", ["lds", "global", "coop"])
 def test_float_couplings_and_field(engine):
     _announce(f"float field {engine}")
     model = pmj_spin_glass(20, 7, field=0.3)
@@ -83,7 +83,7 @@ def test_float_couplings_and_field(engine):
     assert not enc.is_integer
 
 
-@pytest.mark.parametrize("engine", ["lds", "global"])
+@pytest.mark.parametrize("engine", ["lds", "global", "coop"])
 def test_qubo_half_integer_scale_and_int32(engine):
     _announce(f"qubo/int32 {engine}")
     rng = np.random.default_rng(4)
@@ -100,6 +100,7 @@ def test_replica_counts(R):
     model = maxcut_ising(random_graph(30, p=0.3, seed=R))
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="lds")
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="global")
+    compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="coop")
 
 
 @pytest.mark.parametrize("block", [64, 128, 512, 1024])
@@ -108,7 +109,25 @@ def test_tile_sizes_do_not_change_results(block):
     model = pmj_spin_glass(26, 9)
     a = compare(model, R=16, n_rounds=25, S=2, seed=3, engine="lds", block=block)[0]
     b = compare(model, R=16, n_rounds=25, S=2, seed=3, engine="global", block=block)[0]
+    c = compare(model, R=16, n_rounds=25, S=2, seed=3, engine="coop", block=block)[0]
     assert np.array_equal(a.final_spins, b.final_spins)
+    assert np.array_equal(a.final_spins, c.final_spins)
+
+
+@pytest.mark.parametrize("case", [("int8", 20000, 32, 40, 1), ("int8", 6000, 64, 150, 3),
+                                  ("f32", 4096, 16, 60, 2), ("int8", 3000, 100, 80, 1)])
+def test_coop_engine_on_sparse_graphs(case):
+    """The persistent CSR engine on graphs of the C3/C4 kinds: many workgroups, uneven colour
+    classes, several sweeps per round and enough rounds to cycle its three energy buffers."""
+    from drift_amd.builders.lattice import er_sparse_weighted, random_regular_gaussian
+
+    kind, n, R, rounds, S = case
+    _announce(f"coop {kind} n={n} R={R} rounds={rounds} S={S}")
+    model = (er_sparse_weighted(n, 6.0, n) if kind == "int8"
+             else random_regular_gaussian(n, n))
+    g = compare(model, R=R, n_rounds=rounds, S=S, seed=n + R, T_min=0.05, T_max=5.0,
+                engine="coop")[0]
+    assert g.engine == "coop"
 
 
 @pytest.mark.parametrize("L", [4, 16])
@@ -135,7 +154,7 @@ def test_lattice_kernel_shapes(case):
         assert g.engine == engine
 
 
-@pytest.mark.parametrize("engine", ["lds", "global", "lattice", "slab"])
+@pytest.mark.parametrize("engine", ["lds", "global", "lattice", "slab", "coop"])
 def test_zero_sweeps_per_round_only_exchanges(engine):
     _announce(f"S=0 {engine}")
     model = cubic_pmj_glass(8, 2)
@@ -158,7 +177,7 @@ def test_config_c2_first_thousand_sweeps():
     beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))     # beta 3.0 .. 0.2
     g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="auto")
     assert enc.weight_type == 1 and g.engine == "slab"
-    for engine in ("lattice", "global"):
+    for engine in ("lattice", "global", "coop"):
         _announce(f"C2 L=32 R=64 100 sweeps, {engine}")
         g2 = compare(model, R=64, n_rounds=100, S=1, seed=1, beta=beta, engine=engine)[0]
         assert g2.engine == engine

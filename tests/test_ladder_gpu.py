@@ -96,3 +96,44 @@ def test_feedback_ladder_increases_round_trips_on_ferromagnets():
     ro = sum(ladder.round_trip_rate(r, sweeps) for r in opt)
     print(f"[ladder-gpu] round trips per replica-sweep: geometric {rg:.3e}, feedback {ro:.3e}")
     assert ro > 1.2 * rg
+
+
+def test_learned_ladder_reaches_target_more_often_on_held_out_glasses():
+    """The whole pipeline at test scale: feedback labels on 48 random 3-regular N(0,1) glasses,
+    the network trained by its HIP kernels, then 16 held-out instances run with the geometric and
+    the predicted ladders on the same streams.  The target of each instance is the lower of the
+    two best energies; the learned ladder must reach it far more often and usually first."""
+    from drift_amd.builders.lattice import random_regular_gaussian
+
+    _announce("learned vs geometric, held out")
+    T_min, T_max, budget = 0.1, 3.0, 10000
+
+    def instance(seed):
+        rng = np.random.default_rng(seed)
+        return random_regular_gaussian(int(2 * rng.integers(128, 385)), seed)
+
+    train = [instance(100 + i) for i in range(48)]
+    test = [instance(5000 + i) for i in range(16)]
+    x, _ = ladder.feedback_optimise(train, T_min=T_min, T_max=T_max, iterations=5, sweeps=1000)
+    net = ladder.LadderNet(seed=5)
+    losses = net.fit(ladder.instance_features(train, T_min=T_min, T_max=T_max), x, epochs=1500)
+    assert losses[-1] < 1e-2 * losses[0]
+    xs = net.predict(ladder.instance_features(test, T_min=T_min, T_max=T_max))
+    assert np.all(np.diff(xs, axis=1) > 0)
+    kw = dict(n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=budget, sweeps_per_round=1,
+              seeds=range(70, 86), record_history=True)
+    geo = gpu.parallel_tempering_gpu_batch(test, **kw)
+    lrn = gpu.parallel_tempering_gpu_batch(
+        test, betas=[ladder.betas_from_positions(v, T_min, T_max) for v in xs], **kw)
+    tg, tl = [], []
+    for g, l in zip(geo, lrn):
+        target = min(g.best_E, l.best_E)
+        tg.append(ladder.sweeps_to_target(g.best_history, target) or 2 * budget)
+        tl.append(ladder.sweeps_to_target(l.best_history, target) or 2 * budget)
+    tg, tl = np.array(tg), np.array(tl)
+    hit_g, hit_l = int((tg <= budget).sum()), int((tl <= budget).sum())
+    print(f"[ladder-gpu] reached target: geometric {hit_g}/16, learned {hit_l}/16; "
+          f"median sweeps {np.median(tg):.0f} vs {np.median(tl):.0f}")
+    assert hit_l >= hit_g + 6
+    assert int((tl < tg).sum()) >= 10
+    assert np.median(tl) < np.median(tg)
