@@ -691,7 +691,7 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
     std::vector<int> cgrid((size_t)H.k);
     for (int c = 0; c < H.k; ++c)
         cgrid[c] = std::max(1, (H.color_ptr[c + 1] - H.color_ptr[c] + block / Rp - 1) / (block / Rp));
-    const int reg = std::max(1, std::min(64, (H.n + 4095) / 4096));
+    const int reg = std::max(1, std::min(256, (H.n + 1023) / 1024));   // round-end grid
     PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));
     // optional per-launch timing of the colour-step kernel (the dominant kernel)
     const bool time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;
@@ -899,7 +899,7 @@ int run_coop(const HostInst& H, drift_pt_result* res, CoopPlan P) {
     PT_HIP(hipMemsetAsync(P.C.abort, 0, sizeof(int32_t), sg.s));
     // initial spins, energies and round-0 state from the global engine's init kernels
     const int iblock = std::min(((std::max(256, Rp) + Rp - 1) / Rp) * Rp, 1024);
-    const int reg = std::max(1, std::min(64, (H.n + 4095) / 4096));
+    const int reg = std::max(1, std::min(256, (H.n + 1023) / 1024));   // round-end grid
     PT_HIP(pt_launch_g_init(I, G, iblock, reg, sg.s));
     PT_HIP(hipMemsetAsync(G.dacc, 0, 3 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
     int dev = 0, khz = 0;
@@ -1105,11 +1105,7 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     if (engine == DRIFT_PT_ENGINE_AUTO) {
         if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) engine = DRIFT_PT_ENGINE_LDS;
         else if (H.ldim && slab_fits(H)) engine = DRIFT_PT_ENGINE_SLAB;
-        else {
-            CoopPlan P;
-            if (coop_plan(H, params->block_threads, P)) return run_coop(H, result, P);
-            engine = H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
-        }
+        else engine = H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
     }
     if (engine == DRIFT_PT_ENGINE_COOP) {
         CoopPlan P;

@@ -95,9 +95,10 @@ __device__ __forceinline__ int edge_at(const uint32_t* e, int t, field_t<WM>& w)
 }
 
 // Local fields of two rows [s0, e0) and [s1, e1) (an empty row gives its start value).  The first
-// PT_PAIR_CHUNK edges of both rows are gathered together -- edge records from LDS, then every
-// neighbour spin from memory -- so the two rows' loads are in flight at once; longer rows finish
-// chunk by chunk.  Terms are added in row order.
+// PT_PAIR_CHUNK edges of both rows are gathered together without branches -- edge records from
+// LDS, then every neighbour spin from memory; a slot past the end of its row reads edge 0 and
+// position 0 and is dropped by a select -- so all the loads issue back to back and the pair waits
+// on memory once.  Longer rows finish edge by edge.  Terms are added in row order.
 #define PT_PAIR_CHUNK 8
 
 template <int WM>
@@ -105,19 +106,28 @@ __device__ __forceinline__ void fields2(int a, int Rp, const int8_t* sp, const u
                                         int s0, int e0, int s1, int e1, field_t<WM>& l0,
                                         field_t<WM>& l1) {
     constexpr int K = PT_PAIR_CHUNK;
+    int c0[K], c1[K];
     int8_t v0[K], v1[K];
     field_t<WM> w0[K], w1[K];
 #pragma unroll
     for (int u = 0; u < K; ++u) {
-        const int c0 = s0 + u < e0 ? edge_at<WM>(edge, s0 + u, w0[u]) : -1;
-        const int c1 = s1 + u < e1 ? edge_at<WM>(edge, s1 + u, w1[u]) : -1;
-        v0[u] = c0 >= 0 ? ld_spin<true>(sp + (size_t)c0 * Rp + a) : 0;
-        v1[u] = c1 >= 0 ? ld_spin<true>(sp + (size_t)c1 * Rp + a) : 0;
+        const bool ok0 = s0 + u < e0, ok1 = s1 + u < e1;
+        c0[u] = edge_at<WM>(edge, ok0 ? s0 + u : 0, w0[u]);
+        c1[u] = edge_at<WM>(edge, ok1 ? s1 + u : 0, w1[u]);
+        c0[u] = ok0 ? c0[u] : 0;
+        c1[u] = ok1 ? c1[u] : 0;
     }
 #pragma unroll
     for (int u = 0; u < K; ++u) {
-        if (s0 + u < e0) l0 = v0[u] > 0 ? l0 + w0[u] : l0 - w0[u];
-        if (s1 + u < e1) l1 = v1[u] > 0 ? l1 + w1[u] : l1 - w1[u];
+        v0[u] = ld_spin<true>(sp + (size_t)c0[u] * Rp + a);
+        v1[u] = ld_spin<true>(sp + (size_t)c1[u] * Rp + a);
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const field_t<WM> n0 = v0[u] > 0 ? l0 + w0[u] : l0 - w0[u];
+        const field_t<WM> n1 = v1[u] > 0 ? l1 + w1[u] : l1 - w1[u];
+        l0 = s0 + u < e0 ? n0 : l0;
+        l1 = s1 + u < e1 ? n1 : l1;
     }
     for (int t = s0 + K; t < e0; ++t) {
         field_t<WM> w;
@@ -284,12 +294,13 @@ __global__ __launch_bounds__(1024) void coop_run(PtInstance I, GState G, CoopArg
         }
 
         // ---- round end: identical in every workgroup ---------------------------------------
-        for (int x = tid; x < R; x += nt) {
-            int64_t v = EbL[x];
-#pragma unroll 8
-            for (int s = 0; s < PT_DACC_SPLIT; ++s)
-                v += ld_coh(G.dacc + ((size_t)buf * PT_DACC_SPLIT + s) * Rp + x);
-            Es[x] = v;
+        for (int x = tid; x < R; x += nt) Es[x] = EbL[x];
+        __syncthreads();
+        // one coherent load per (counter, slot) pair, summed in LDS: a single memory round trip
+        for (int idx = tid; idx < PT_DACC_SPLIT * R; idx += nt) {
+            const int sp = idx / R, x = idx - sp * R;
+            const int64_t v = ld_coh(G.dacc + ((size_t)buf * PT_DACC_SPLIT + sp) * Rp + x);
+            if (v != 0) atomicAdd((unsigned long long*)&Es[x], (unsigned long long)v);
         }
         __syncthreads();
         int64_t mE;

@@ -147,10 +147,10 @@ __device__ __forceinline__ void st_spin(int8_t* p, int8_t v) {
 }
 
 // Local field of position p in slot a: the sum over its row of w_t * s_col(t), in row order.
-// Rows are gathered in chunks of G edges: every edge record of a chunk is loaded, then every
-// neighbour spin, then the terms are added in order, so the loads of a chunk overlap instead of
-// forming one dependent edge -> spin chain per neighbour.  Spins in HBM use G = 8; spins in LDS
-// use G = 1 (the plain loop), where the extra predicated lanes only cost issue slots.
+// Rows are gathered in chunks of G edges without branches: a slot past the end of the row reads
+// edge 0 and position 0 (valid addresses) and is dropped by a select, so every load of a chunk
+// issues back to back and the chunk waits on memory once.  Spins in HBM use G = 8; spins in LDS
+// use G = 1 (the plain loop), where the extra lanes only cost issue slots.
 template <int WM, int G, bool COH = false>
 __device__ __forceinline__ int32_t field_int(int a, int Rp, const int8_t* sp, int t0, int t1,
                                              const void* __restrict__ edge, int32_t l) {
@@ -159,12 +159,18 @@ __device__ __forceinline__ int32_t field_int(int a, int Rp, const int8_t* sp, in
         int32_t w[G];
         int8_t v[G];
 #pragma unroll
-        for (int u = 0; u < G; ++u) c[u] = t + u < t1 ? edge_int<WM>(edge, t + u, w[u]) : 0;
+        for (int u = 0; u < G; ++u) {
+            const bool ok = t + u < t1;
+            c[u] = edge_int<WM>(edge, ok ? t + u : 0, w[u]);
+            c[u] = ok ? c[u] : 0;
+        }
 #pragma unroll
-        for (int u = 0; u < G; ++u) v[u] = t + u < t1 ? ld_spin<COH>(sp + (size_t)c[u] * Rp + a) : 0;
+        for (int u = 0; u < G; ++u) v[u] = ld_spin<COH>(sp + (size_t)c[u] * Rp + a);
 #pragma unroll
-        for (int u = 0; u < G; ++u)
-            if (t + u < t1) l += v[u] > 0 ? w[u] : -w[u];
+        for (int u = 0; u < G; ++u) {
+            const int32_t term = v[u] > 0 ? w[u] : -w[u];
+            l += t + u < t1 ? term : 0;
+        }
     }
     return l;
 }
@@ -177,18 +183,22 @@ __device__ __forceinline__ float field_float(int a, int Rp, const int8_t* sp, in
         float w[G];
         int8_t v[G];
 #pragma unroll
-        for (int u = 0; u < G; ++u) c[u] = t + u < t1 ? edge_float(edge, t + u, w[u]) : 0;
+        for (int u = 0; u < G; ++u) {
+            const bool ok = t + u < t1;
+            c[u] = edge_float(edge, ok ? t + u : 0, w[u]);
+            c[u] = ok ? c[u] : 0;
+        }
 #pragma unroll
-        for (int u = 0; u < G; ++u) v[u] = t + u < t1 ? ld_spin<COH>(sp + (size_t)c[u] * Rp + a) : 0;
+        for (int u = 0; u < G; ++u) v[u] = ld_spin<COH>(sp + (size_t)c[u] * Rp + a);
 #pragma unroll
-        for (int u = 0; u < G; ++u)
-            if (t + u < t1) l = v[u] > 0 ? l + w[u] : l - w[u];
+        for (int u = 0; u < G; ++u) {
+            const float nl = v[u] > 0 ? l + w[u] : l - w[u];
+            l = t + u < t1 ? nl : l;
+        }
     }
     return l;
 }
 
-// One Metropolis attempt at position p of slot a; returns the accepted energy change in units
-// (0 if rejected).  sp is [position][slot] with Rp slots per position (LDS or HBM).
 // The row of p is edge[t0 .. t1).
 template <int WM, int G = 1, bool COH = false>
 __device__ __forceinline__ int64_t site_update_row(int p, int a, int Rp, int8_t* sp, int t0, int t1,

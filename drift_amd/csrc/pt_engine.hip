@@ -210,13 +210,6 @@ __device__ __forceinline__ void block_slot_reduce(int64_t v, int a, int R, int R
         if (red[x] != 0) atomicAdd((unsigned long long*)&dst[x], (unsigned long long)red[x]);
 }
 
-__device__ __forceinline__ int64_t dacc_total(const GState& G, int par, int Rp, int x) {
-    int64_t v = 0;
-#pragma unroll
-    for (int k = 0; k < PT_DACC_SPLIT; ++k) v += G.dacc[((size_t)par * PT_DACC_SPLIT + k) * Rp + x];
-    return v;
-}
-
 template <int WM>
 __global__ __launch_bounds__(1024) void g_init_energy(PtInstance I, GState G) {
     __shared__ long long red[1024];
@@ -351,18 +344,22 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
     const int rnd = init ? -1 : *G.round_base + rl;
     const int cur = rnd & 1, nxt = cur ^ 1;
     for (int x = tid; x < R; x += blockDim.x) {
-        if (init) {
-            const int64_t T = dacc_total(G, cur, Rp, x);
-            Es[x] = WM == PT_WM_F32 ? T : -(T / 2);
-            reps[x] = x;
-            rungs[x] = x;
-        } else {
-            Es[x] = G.Eb[(size_t)cur * Rp + x] + dacc_total(G, cur, Rp, x);
-            reps[x] = G.rep[(size_t)cur * Rp + x];
-            rungs[x] = G.rung[(size_t)cur * Rp + x];
-        }
+        Es[x] = init ? 0 : G.Eb[(size_t)cur * Rp + x];
+        reps[x] = init ? x : G.rep[(size_t)cur * Rp + x];
+        rungs[x] = init ? x : G.rung[(size_t)cur * Rp + x];
     }
     __syncthreads();
+    // the round's energy changes: one load per (counter, slot) pair, summed in LDS
+    for (int idx = tid; idx < PT_DACC_SPLIT * R; idx += blockDim.x) {
+        const int sp = idx / R, x = idx - sp * R;
+        const int64_t v = G.dacc[((size_t)cur * PT_DACC_SPLIT + sp) * Rp + x];
+        if (v != 0) atomicAdd((unsigned long long*)&Es[x], (unsigned long long)v);
+    }
+    __syncthreads();
+    if (init && WM != PT_WM_F32) {
+        for (int x = tid; x < R; x += blockDim.x) Es[x] = -(Es[x] / 2);
+        __syncthreads();
+    }
     int64_t mE;
     int32_t mR;
     block_best_rung(Es, reps, R, scrE, scrR, mE, mR);
@@ -415,7 +412,9 @@ __global__ void g_tick(int32_t* round_base, int rounds) {
     if (threadIdx.x == 0) *round_base += rounds;
 }
 
-inline int block_for_slots(int Rp) { return Rp <= 64 ? 64 : ((Rp + 63) / 64) * 64; }
+// Round-end workgroup: at least 256 threads (the best-state copy is spread over them), one per
+// slot beyond that.
+inline int block_for_slots(int Rp) { return Rp <= 256 ? 256 : ((Rp + 63) / 64) * 64; }
 
 }  // namespace
 
