@@ -373,7 +373,8 @@ def main() -> int:
     model, R, beta, desc = workload(args.workload)
     enc = gpu.encode_problem(model)
     kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,
-              block_threads=args.block, record_history=False)
+              block_threads=args.block, record_history=False,
+              autotune=args.engine == "auto" and args.block == 0)
 
     print(f"[bench] warm-up {args.warmup} steps", file=sys.stderr, flush=True)
     gpu.parallel_tempering_gpu(model, n_rounds=args.warmup, **kw)

@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64,
 LIB_NAME = "libdriftpt.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # enum drift_pt_status
 OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
@@ -27,6 +27,8 @@ ENGINES = {"auto": ENGINE_AUTO, "lds": ENGINE_LDS, "global": ENGINE_GLOBAL,
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 # enum drift_pt_flags
 RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GRAPH = 1, 2, 4, 8, 16, 32
+AUTOTUNE = 64
+TUNE_MAX = 16
 
 
 class Problem(ctypes.Structure):
@@ -88,6 +90,19 @@ class Result(ctypes.Structure):
 
 
 # every symbol include/drift_pt.h declares, with its signature
+class Tuning(ctypes.Structure):
+    """drift_pt_tuning: the timed candidates of one autotune call."""
+    _fields_ = [
+        ("engine", c_int32),
+        ("block_threads", c_int32),
+        ("trial_rounds", c_int32),
+        ("n_candidates", c_int32),
+        ("cand_engine", c_int32 * TUNE_MAX),
+        ("cand_block", c_int32 * TUNE_MAX),
+        ("cand_seconds", c_double * TUNE_MAX),
+    ]
+
+
 SIGNATURES = {
     "drift_pt_abi_version": ([], c_int),
     "drift_pt_device_available": ([], c_int),
@@ -96,6 +111,8 @@ SIGNATURES = {
         [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)], c_int),
     "drift_pt_run": ([POINTER(Problem), POINTER(Params), POINTER(Result)], c_int),
     "drift_pt_run_batch": ([POINTER(Problem), POINTER(Params), c_int32, POINTER(Result)], c_int),
+    "drift_pt_autotune": ([POINTER(Problem), POINTER(Params), c_int32, POINTER(Tuning)], c_int),
+    "drift_pt_autotune_clear": ([], None),
 }
 
 _lib = None

@@ -13,9 +13,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -1055,6 +1057,124 @@ bool fits_lds(const HostInst& H) {
     return pt_lds_bytes(H.n, H.Rp) + 1024 <= (size_t)device_lds_limit();
 }
 
+// Run one prepared instance on a resolved engine (not AUTO).
+int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
+    switch (engine) {
+    case DRIFT_PT_ENGINE_SLAB:
+        if (!slab_fits(H))
+            return fail(DRIFT_PT_EINVAL, "slab engine needs a periodic L^2/L^3 lattice (L even >= 4) "
+                                         "with +/-1 couplings, no field, the parity colouring, and "
+                                         "slabs that fit LDS");
+        return run_slab(H, res, block);
+    case DRIFT_PT_ENGINE_LATTICE:
+        if (!H.ldim)
+            return fail(DRIFT_PT_EINVAL, "lattice engine needs a periodic L^2/L^3 lattice (L even >= 4) "
+                                         "with +/-1 couplings, no field and the parity colouring");
+        return run_global(H, res, block, true);
+    case DRIFT_PT_ENGINE_LDS: {
+        if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "instance does not fit one workgroup's LDS");
+        std::vector<HostInst> hs{H};
+        return run_lds(hs, {res}, block);
+    }
+    case DRIFT_PT_ENGINE_COOP: {
+        CoopPlan P;
+        if (!coop_plan(H, block, P))
+            return fail(DRIFT_PT_EINVAL, "coop engine: no co-resident grid whose row slices fit LDS");
+        return run_coop(H, res, P);
+    }
+    case DRIFT_PT_ENGINE_GLOBAL:
+        return run_global(H, res, block, false);
+    default:
+        return fail(DRIFT_PT_EINVAL, "unknown engine");
+    }
+}
+
+// The engine AUTO takes without tuning.
+int default_engine(const HostInst& H) {
+    if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) return DRIFT_PT_ENGINE_LDS;
+    if (H.ldim && slab_fits(H)) return DRIFT_PT_ENGINE_SLAB;
+    return H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+}
+
+// ------------------------------------------------------------------------------------------
+// autotuner: timed trials per shape class, cached for the process
+// ------------------------------------------------------------------------------------------
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
+
+struct TuneEntry {
+    int engine = 0, block = 0;
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+
+TuneKey tune_key(const HostInst& H) {
+    int nb = 0;
+    while ((1 << nb) < H.n) ++nb;                               // ceil(log2 n)
+    const int half_degree = H.n ? (int)std::lround(2.0 * H.nnz / H.n) : 0;
+    return TuneKey(H.wm, H.Rp, H.ldim, H.lL, nb, std::min(half_degree, 256), H.S, H.has_h ? 1 : 0);
+}
+
+int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry& choice) {
+    std::vector<std::pair<int, int>> cands;
+    const int blocks[3] = {256, 512, 1024};
+    for (int b : blocks) {
+        if (fits_lds(H)) cands.emplace_back(DRIFT_PT_ENGINE_LDS, b);
+        if (H.ldim && slab_fits(H)) cands.emplace_back(DRIFT_PT_ENGINE_SLAB, b);
+        if (H.ldim) cands.emplace_back(DRIFT_PT_ENGINE_LATTICE, b);
+        cands.emplace_back(DRIFT_PT_ENGINE_GLOBAL, b);
+        CoopPlan P;
+        if (coop_plan(H, b, P)) cands.emplace_back(DRIFT_PT_ENGINE_COOP, b);
+    }
+    const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
+    const int rounds = trial_rounds > 0 ? trial_rounds
+                                        : (int)std::min(2000.0, std::max(4.0, 2.5e8 / per_round));
+    HostInst T = H;
+    T.flags &= ~(DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE | DRIFT_PT_RECORD_FINAL |
+                 DRIFT_PT_TIME_KERNELS);
+    std::vector<int8_t> bs((size_t)std::max(H.n, 1));
+    drift_pt_result r{};
+    r.best_s = bs.data();
+    double best = -1.0;
+    if (table) {
+        std::memset(table, 0, sizeof(*table));
+        table->trial_rounds = rounds;
+    }
+    int slot = 0;
+    for (const auto& c : cands) {
+        double secs = -1.0;
+        T.n_rounds = 2;                                      // warm-up: code objects, first touch
+        clear_result(&r);
+        if (run_engine(T, c.first, c.second, &r) == 0) {
+            T.n_rounds = rounds;
+            clear_result(&r);
+            if (run_engine(T, c.first, c.second, &r) == 0) secs = r.seconds;
+        }
+        (void)hipGetLastError();
+        if (secs > 0.0 && (best < 0.0 || secs < best)) {
+            best = secs;
+            choice.engine = c.first;
+            choice.block = c.second;
+        }
+        if (table && slot < DRIFT_PT_TUNE_MAX) {
+            table->cand_engine[slot] = c.first;
+            table->cand_block[slot] = c.second;
+            table->cand_seconds[slot] = secs;
+            ++slot;
+        }
+    }
+    g_err.clear();
+    if (table) {
+        table->n_candidates = slot;
+        table->engine = choice.engine;
+        table->block_threads = choice.block;
+    }
+    if (best < 0.0) return fail(DRIFT_PT_EHIP, "autotune: no candidate engine ran");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[tune_key(H)] = choice;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1101,38 +1221,48 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     rc = check_device();
     if (rc) return rc;
     detect_lattice(problem, H);
-    int engine = params->engine;
+    int engine = params->engine, block = params->block_threads;
     if (engine == DRIFT_PT_ENGINE_AUTO) {
-        if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) engine = DRIFT_PT_ENGINE_LDS;
-        else if (H.ldim && slab_fits(H)) engine = DRIFT_PT_ENGINE_SLAB;
-        else engine = H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+        engine = default_engine(H);
+        if (params->flags & DRIFT_PT_AUTOTUNE) {
+            TuneEntry choice;
+            bool hit = false;
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tune.find(tune_key(H));
+                if (it != g_tune.end()) {
+                    choice = it->second;
+                    hit = true;
+                }
+            }
+            if (!hit) {
+                rc = tune(H, 0, nullptr, choice);
+                if (rc) return rc;
+            }
+            engine = choice.engine;
+            block = choice.block;
+        }
     }
-    if (engine == DRIFT_PT_ENGINE_COOP) {
-        CoopPlan P;
-        if (!coop_plan(H, params->block_threads, P))
-            return fail(DRIFT_PT_EINVAL, "coop engine: no co-resident grid whose row slices fit LDS");
-        return run_coop(H, result, P);
-    }
-    if (engine == DRIFT_PT_ENGINE_SLAB) {
-        if (!slab_fits(H))
-            return fail(DRIFT_PT_EINVAL, "slab engine needs a periodic L^2/L^3 lattice (L even >= 4) "
-                                         "with +/-1 couplings, no field, the parity colouring, and "
-                                         "slabs that fit LDS");
-        return run_slab(H, result, params->block_threads);
-    }
-    if (engine == DRIFT_PT_ENGINE_LATTICE) {
-        if (!H.ldim)
-            return fail(DRIFT_PT_EINVAL, "lattice engine needs a periodic L^2/L^3 lattice (L even >= 4) "
-                                         "with +/-1 couplings, no field and the parity colouring");
-        return run_global(H, result, params->block_threads, true);
-    }
-    if (engine == DRIFT_PT_ENGINE_LDS) {
-        if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "instance does not fit one workgroup's LDS");
-        std::vector<HostInst> hs{std::move(H)};
-        return run_lds(hs, {result}, params->block_threads);
-    }
-    if (engine != DRIFT_PT_ENGINE_GLOBAL) return fail(DRIFT_PT_EINVAL, "unknown engine");
-    return run_global(H, result, params->block_threads, false);
+    return run_engine(H, engine, block, result);
+}
+
+int drift_pt_autotune(const drift_pt_problem* problem, const drift_pt_params* params,
+                      int32_t trial_rounds, drift_pt_tuning* out) {
+    g_err.clear();
+    if (!out || !params) return fail(DRIFT_PT_EINVAL, "autotune needs params and an output table");
+    HostInst H;
+    int rc = prepare(problem, params, H);
+    if (rc) return rc;
+    rc = check_device();
+    if (rc) return rc;
+    detect_lattice(problem, H);
+    TuneEntry choice;
+    return tune(H, trial_rounds, out, choice);
+}
+
+void drift_pt_autotune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
 }
 
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,

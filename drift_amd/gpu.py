@@ -161,10 +161,11 @@ class _Buffers:
 
 
 def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, engine,
-            block_threads, final, history, trace=False, time_kernels=False, graph=True):
+            block_threads, final, history, trace=False, time_kernels=False, graph=True,
+            autotune=False):
     flags = ((_lib.RECORD_FINAL if final else 0) | (_lib.RECORD_HISTORY if history else 0)
              | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0)
-             | (0 if graph else _lib.NO_GRAPH))
+             | (0 if graph else _lib.NO_GRAPH) | (_lib.AUTOTUNE if autotune else 0))
     if engine not in _lib.ENGINES:
         raise ValueError(f"engine must be one of {sorted(_lib.ENGINES)}")
     return _lib.Params(
@@ -244,16 +245,19 @@ def parallel_tempering_gpu(
     time_kernels: bool = False,
     graph: bool = True,
     force_float: bool = False,
+    autotune: bool = False,
 ) -> PtResult:
     """Solve ``model`` by parallel tempering on the GPU; same signature and result as the CPU
     ``parallel_tempering`` (drift/gpu.py:43-114).  ``throughput`` is spin-flip attempts per second
-    and ``seconds`` the device time of the sweep and exchange loop."""
+    and ``seconds`` the device time of the sweep and exchange loop.  ``autotune`` (with
+    ``engine="auto"``) takes the engine and block size from the library's autotune cache, timing
+    the candidates the first time this shape class is seen; results do not depend on it."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
     prm = _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, b, engine,
                   block_threads, return_final, record_history, record_trace, time_kernels,
-                  graph)
+                  graph, autotune)
     buf = _Buffers(prob.n, n_replicas, n_rounds, return_final, record_history, record_trace)
     res = buf.struct
     rc = lib.drift_pt_run(prob.struct, prm, res)
@@ -303,3 +307,30 @@ def parallel_tempering_gpu_batch(
         raise RuntimeError(f"GPU engine failed (status {rc}): {_lib.last_error(lib)}")
     return [_to_result(Rs[i], bufs[i], ladders[i][0], n_rounds, n_replicas, record_history)
             for i in range(m)]
+
+
+def autotune(model, *, n_replicas: int = 16, T_min: float = 0.1, T_max: float = 5.0,
+             sweeps_per_round: int = 5, seed: int = 0, betas=None, coloring=None,
+             trial_rounds: int = 0, force_float: bool = False) -> dict:
+    """Time every engine and block size that can run ``model`` (drift_pt_autotune), cache the
+    fastest for its shape class, and return {"engine", "block_threads", "trial_rounds",
+    "candidates": [(engine, block, seconds), ...]} (seconds < 0: the candidate could not run)."""
+    lib = _engine_lib()
+    temps, b = _ladder(n_replicas, T_min, T_max, betas)
+    prob = encode_problem(model, coloring, force_float)
+    prm = _params(n_replicas, T_min, T_max, max(int(trial_rounds), 1), sweeps_per_round, seed, b,
+                  "auto", 0, False, False)
+    tab = _lib.Tuning()
+    rc = lib.drift_pt_autotune(prob.struct, prm, int(trial_rounds), tab)
+    if rc != 0:
+        raise RuntimeError(f"GPU engine failed (status {rc}): {_lib.last_error(lib)}")
+    cands = [(_lib.ENGINE_NAMES.get(int(tab.cand_engine[i]), str(tab.cand_engine[i])),
+              int(tab.cand_block[i]), float(tab.cand_seconds[i])) for i in range(tab.n_candidates)]
+    return {"engine": _lib.ENGINE_NAMES.get(int(tab.engine), str(tab.engine)),
+            "block_threads": int(tab.block_threads), "trial_rounds": int(tab.trial_rounds),
+            "candidates": cands}
+
+
+def autotune_clear() -> None:
+    """Empty the library's autotune cache."""
+    _engine_lib().drift_pt_autotune_clear()

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_PT_ABI_VERSION 1
+#define DRIFT_PT_ABI_VERSION 2
 
 enum drift_pt_status {
     DRIFT_PT_OK = 0,
@@ -61,7 +61,9 @@ enum drift_pt_flags {
     DRIFT_PT_SKIP_VALIDATION = 4, /* trust CSR symmetry/colouring (the colouring is still used) */
     DRIFT_PT_RECORD_TRACE = 8,    /* fill result.rung_energy_trace */
     DRIFT_PT_TIME_KERNELS = 16,   /* time each sweep-kernel launch (up to 4096) with device events */
-    DRIFT_PT_NO_GRAPH = 32        /* launch every round directly instead of replaying a hipGraph */
+    DRIFT_PT_NO_GRAPH = 32,       /* launch every round directly instead of replaying a hipGraph */
+    DRIFT_PT_AUTOTUNE = 64        /* with DRIFT_PT_ENGINE_AUTO: take the engine and block size from
+                                     the autotune cache, timing the candidates on a miss */
 };
 
 /* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and
@@ -120,6 +122,19 @@ typedef struct drift_pt_result {
     int64_t kernel_launches;    /* launches timed (DRIFT_PT_TIME_KERNELS), else 1 for the whole loop */
 } drift_pt_result;
 
+/* Timed candidates of one autotune call (drift_pt_autotune). */
+#define DRIFT_PT_TUNE_MAX 16
+typedef struct drift_pt_tuning {
+    int32_t engine;                          /* the fastest candidate */
+    int32_t block_threads;
+    int32_t trial_rounds;                    /* rounds each timed trial ran */
+    int32_t n_candidates;
+    int32_t cand_engine[DRIFT_PT_TUNE_MAX];
+    int32_t cand_block[DRIFT_PT_TUNE_MAX];
+    double cand_seconds[DRIFT_PT_TUNE_MAX];  /* device time of the trial's round loop; < 0 if the
+                                                candidate could not run */
+} drift_pt_tuning;
+
 /* ABI version of the loaded library (DRIFT_PT_ABI_VERSION). */
 int drift_pt_abi_version(void);
 
@@ -146,6 +161,19 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
  * instance per process (drift/gpu.py:97). */
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
                        int32_t n_instances, drift_pt_result* results);
+
+/* Autotuner.  Times every engine and block size that can run this instance (LDS, slab, lattice,
+ * global, coop; 256 / 512 / 1024 threads) for trial_rounds rounds (0: about 2.5e8 attempts),
+ * stores the fastest in a process-wide cache keyed by the instance's shape class (weight type,
+ * replica slots, lattice shape, size and degree buckets, sweeps per round), and reports every
+ * candidate's time.  All candidates run the same Markov chain, so the choice changes speed only.
+ * No reference counterpart: the reference has one kernel with a fixed launch shape
+ * (cuda/ising_pt.cu:196-211). */
+int drift_pt_autotune(const drift_pt_problem* problem, const drift_pt_params* params,
+                      int32_t trial_rounds, drift_pt_tuning* out);
+
+/* Empties the autotune cache. */
+void drift_pt_autotune_clear(void);
 
 #ifdef __cplusplus
 }

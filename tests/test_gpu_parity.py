@@ -238,3 +238,30 @@ def test_engine_errors_are_reported():
                                    coloring=(np.array([0, 4]), np.arange(4)))
     with pytest.raises(RuntimeError, match="n_replicas"):
         gpu.parallel_tempering_gpu(model, n_replicas=2000, n_rounds=1)
+
+
+def test_autotune_times_candidates_and_keeps_results():
+    """The autotuner times every engine and block size that can run the instance, caches the
+    fastest, and a run that takes its choice still matches the oracle bit for bit."""
+    from drift_amd.builders.lattice import er_sparse_weighted
+
+    _announce("autotune")
+    gpu.autotune_clear()
+    model = er_sparse_weighted(6000, 6.0, 5)
+    tab = gpu.autotune(model, n_replicas=32, sweeps_per_round=1, trial_rounds=20)
+    ran = [c for c in tab["candidates"] if c[2] > 0]
+    engines = {c[0] for c in ran}
+    assert {"global", "coop"} <= engines and len(ran) >= 6
+    best = min(ran, key=lambda c: c[2])
+    assert (tab["engine"], tab["block_threads"]) == best[:2]
+    enc = gpu.encode_problem(model)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    beta = ladder_betas(geometric_ladder(32, 0.1, 5.0))
+    g = gpu.parallel_tempering_gpu(model, n_replicas=32, n_rounds=30, sweeps_per_round=1, seed=4,
+                                   betas=beta, coloring=(cptr, cspins), autotune=True,
+                                   return_final=True)
+    o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=4, R=32, n_rounds=30, sweeps_per_round=1,
+                              beta=beta, n_threads=THREADS)
+    assert g.engine == tab["engine"]
+    assert g.best_E_units == o["best_units"]
+    assert np.array_equal(g.final_spins, o["final_spins"])
