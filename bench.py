@@ -390,8 +390,18 @@ def main() -> int:
     print("[bench] kernel timing run", file=sys.stderr, flush=True)
     kt = gpu.parallel_tempering_gpu(model, n_rounds=args.kernel_timing_steps, time_kernels=True, **kw)
     B = algorithmic_bytes_per_attempt(model, enc, kt.engine)
-    per_launch_s = kt.kernel_seconds / max(kt.kernel_launches, 1)
-    per_launch_attempts = kt.kernel_attempts / max(kt.kernel_launches, 1)
+    if kt.kernel_launches > 1:
+        # The dominant kernel's average launch in the timed region: the region's time per step
+        # times that kernel's share of device time, over its launches per step.  Shares come from
+        # the per-launch event timing run (whose event pairs themselves add about 2 us a launch).
+        launches_per_step = kt.kernel_launches / args.kernel_timing_steps
+        share = min(1.0, kt.kernel_seconds / kt.seconds)
+        per_launch_s = res.seconds / args.steps * share / launches_per_step
+        per_launch_attempts = kt.kernel_attempts / kt.kernel_launches
+        event_us = kt.kernel_seconds / kt.kernel_launches * 1e6
+    else:
+        # one launch runs the whole timed region
+        per_launch_s, per_launch_attempts, event_us = res.seconds, attempts, res.seconds * 1e6
     achieved = B * per_launch_attempts / per_launch_s / 1e9
     roofline = {
         "bound": "hbm",
@@ -405,6 +415,7 @@ def main() -> int:
         "bytes_per_attempt": round(B, 3),
         "attempts_per_launch": per_launch_attempts,
         "avg_launch_us": round(per_launch_s * 1e6, 3),
+        "avg_launch_us_event_pairs": round(event_us, 3),
     }
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{kt.engine}.json")
     if os.path.exists(traffic_file):
