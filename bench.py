@@ -64,7 +64,7 @@ def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
     from coordinates and reads one byte holding the site's bond signs."""
     d = model.nnz / model.n
     b_s = 1.0
-    if engine in ("lattice", "slab"):
+    if engine in ("lattice", "slab", "pslab"):
         return b_s * (d + 1) + b_s + 1.0
     b_e = 4.0 if enc.weight.dtype.itemsize == 1 and model.n < (1 << 24) else 8.0
     b_h = 0.0 if enc.h is None else enc.h.dtype.itemsize
@@ -392,10 +392,11 @@ def main() -> int:
     B = algorithmic_bytes_per_attempt(model, enc, kt.engine)
     if kt.kernel_launches > 1:
         # The dominant kernel's average launch in the timed region: the region's time per step
-        # times that kernel's share of device time, over its launches per step.  Shares come from
-        # the per-launch event timing run (whose event pairs themselves add about 2 us a launch).
+        # times that kernel's share of device time, over its launches per step.  The share comes
+        # from the per-launch event timing run, where every launch (sweep kernels and round ends
+        # alike) is bracketed by an event pair that itself adds about 2 us.
         launches_per_step = kt.kernel_launches / args.kernel_timing_steps
-        share = min(1.0, kt.kernel_seconds / kt.seconds)
+        share = kt.kernel_seconds / (kt.kernel_seconds + (kt.other_kernel_seconds or 0.0))
         per_launch_s = res.seconds / args.steps * share / launches_per_step
         per_launch_attempts = kt.kernel_attempts / kt.kernel_launches
         event_us = kt.kernel_seconds / kt.kernel_launches * 1e6
@@ -411,7 +412,8 @@ def main() -> int:
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
         "kernel": {"global": "g_color_step", "lattice": "g_lattice_step",
-                   "slab": "slab_sweep", "coop": "coop_run"}.get(kt.engine, "pt_lds_kernel"),
+                   "slab": "slab_sweep", "coop": "coop_run",
+                   "pslab": "pslab_run"}.get(kt.engine, "pt_lds_kernel"),
         "bytes_per_attempt": round(B, 3),
         "attempts_per_launch": per_launch_attempts,
         "avg_launch_us": round(per_launch_s * 1e6, 3),

@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64,
 LIB_NAME = "libdriftpt.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # enum drift_pt_status
 OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
@@ -22,8 +22,10 @@ OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
 W_I8, W_I32, W_F32 = 1, 2, 3
 # enum drift_pt_engine
 ENGINE_AUTO, ENGINE_LDS, ENGINE_GLOBAL, ENGINE_LATTICE, ENGINE_SLAB, ENGINE_COOP = 0, 1, 2, 3, 4, 5
+ENGINE_PSLAB = 6
 ENGINES = {"auto": ENGINE_AUTO, "lds": ENGINE_LDS, "global": ENGINE_GLOBAL,
-           "lattice": ENGINE_LATTICE, "slab": ENGINE_SLAB, "coop": ENGINE_COOP}
+           "lattice": ENGINE_LATTICE, "slab": ENGINE_SLAB, "coop": ENGINE_COOP,
+           "pslab": ENGINE_PSLAB}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 # enum drift_pt_flags
 RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GRAPH = 1, 2, 4, 8, 16, 32
@@ -86,6 +88,8 @@ class Result(ctypes.Structure):
         ("kernel_seconds", c_double),
         ("kernel_attempts", c_double),
         ("kernel_launches", c_int64),
+        ("other_kernel_seconds", c_double),
+        ("other_kernel_launches", c_int64),
     ]
 
 

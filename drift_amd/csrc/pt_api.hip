@@ -510,6 +510,7 @@ void clear_result(drift_pt_result* r) {
     r->best_E = 0.0; r->best_E_units = 0; r->best_round = 0; r->best_rung = 0; r->best_slot = 0;
     r->engine_used = 0; r->seconds = 0.0; r->attempts = 0.0;
     r->kernel_seconds = 0.0; r->kernel_attempts = 0.0; r->kernel_launches = 0;
+    r->other_kernel_seconds = 0.0; r->other_kernel_launches = 0;
 }
 
 // Copy results of one instance back and undo the colour relabelling.
@@ -697,9 +698,9 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
     PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));
     // optional per-launch timing of the colour-step kernel (the dominant kernel)
     const bool time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;
-    EventPool pool;
+    EventPool pool, other;
     const int64_t max_timed = 4096;
-    int64_t timed = 0;
+    int64_t timed = 0, timed_other = 0;
     double timed_attempts = 0.0;
     // one round: S sweeps of k colour steps, then the round-end kernel; rl is relative to the
     // device-side round counter
@@ -720,7 +721,17 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
                     timed_attempts += (double)H.R * (H.color_ptr[c + 1] - H.color_ptr[c]);
                 }
             }
+        const bool tr = timing && timed_other < max_timed;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (tr) {
+            PT_HIP(other.pair(&a, &b));
+            PT_HIP(hipEventRecord(a, sg.s));
+        }
         PT_HIP(pt_launch_g_round_end(I, G, rl, reg, sg.s));
+        if (tr) {
+            PT_HIP(hipEventRecord(b, sg.s));
+            ++timed_other;
+        }
         return 0;
     };
     // Chunks of rounds are captured once into a hipGraph (rounds read their index from the
@@ -761,6 +772,12 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         PT_HIP(hipEventElapsedTime(&km, pool.ev[x], pool.ev[x + 1]));
         kernel_s += km * 1e-3;
     }
+    double other_s = 0.0;
+    for (size_t x = 0; x < other.ev.size(); x += 2) {
+        float km = 0.0f;
+        PT_HIP(hipEventElapsedTime(&km, other.ev[x], other.ev[x + 1]));
+        other_s += km * 1e-3;
+    }
 
     rc = finish_global(H, I, G, res);
     if (rc) return rc;
@@ -770,6 +787,8 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         res->kernel_seconds = kernel_s;
         res->kernel_attempts = timed_attempts;
         res->kernel_launches = timed;
+        res->other_kernel_seconds = other_s;
+        res->other_kernel_launches = timed_other;
     } else {
         res->kernel_seconds = res->seconds;
         res->kernel_attempts = res->attempts;
@@ -940,18 +959,44 @@ bool slab_fits(const HostInst& H) {
     // power-of-two side (the kernel indexes by shifts) with at least one quad per plane and colour
     if (!H.ldim || (H.lL & (H.lL - 1)) || (H.ldim == 2 ? H.lL < 8 : H.lL < 4)) return false;
     const int NB = slab_count(H);
-    return pt_slab_lds_bytes(H.n, H.lL, H.ldim, NB) + 20 * 1024 <= (size_t)device_lds_limit();
+    return pt_slab_lds_bytes(H.n, H.lL, H.ldim, NB) + 32 * 1024 <= (size_t)device_lds_limit();
 }
 
-int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
-    DevArena A;
-    PtInstance I;
-    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
-    if (rc) return rc;
+// Results of the slab engines: statistics, best record, final energies and rungs from the
+// buffers of parity n_rounds & 1, final spins from spin buffer (n_rounds * S) & 1.
+int finish_slab(const HostInst& H, const PtInstance& I, const SState& S, drift_pt_result* res) {
     const int Rp = H.Rp, n = H.n;
-    SState S;
+    const int fin = H.n_rounds & 1;
+    std::vector<int64_t> out((size_t)pt_out_len(H.R));
+    PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t best[5];
+    PT_HIP(hipMemcpy(best, S.best + 5 * fin, sizeof(best), hipMemcpyDeviceToHost));
+    std::vector<int64_t> E((size_t)Rp);
+    std::vector<int32_t> rung((size_t)Rp);
+    PT_HIP(hipMemcpy(E.data(), S.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(rung.data(), S.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
+    for (int x = 0; x < 5; ++x) out[x] = best[x];
+    for (int r = 0; r < H.R; ++r) {
+        pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
+        pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
+    }
+    std::vector<int8_t> fs;
+    if (H.flags & DRIFT_PT_RECORD_FINAL) {
+        const int buf = (int)(((int64_t)H.n_rounds * H.S) & 1);
+        std::vector<int8_t> slot_major((size_t)n * Rp);
+        PT_HIP(hipMemcpy(slot_major.data(), S.sp + (size_t)buf * Rp * n, slot_major.size(),
+                         hipMemcpyDeviceToHost));
+        fs.resize((size_t)n * Rp);
+        for (int p = 0; p < n; ++p)
+            for (int a = 0; a < H.R; ++a) fs[(size_t)p * Rp + a] = slot_major[(size_t)a * n + H.orig[p]];
+    }
+    return fetch_results(H, I, out, fs, res);
+}
+
+// Slab state of one instance: allocations, zeroed bookkeeping, initial spins and energies.
+int slab_setup(DevArena& A, const HostInst& H, const PtInstance& I, SState& S, hipStream_t st) {
+    const int Rp = H.Rp, n = H.n;
     S.NB = slab_count(H);
-    const size_t lds = pt_slab_lds_bytes(n, H.lL, H.ldim, S.NB);
     PT_HIP(A.alloc(&S.sp, 2 * (size_t)Rp * n));
     PT_HIP(A.alloc(&S.Eb, 2 * (size_t)Rp));
     PT_HIP(A.alloc(&S.dacc, 3 * (size_t)Rp));
@@ -961,22 +1006,35 @@ int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
     PT_HIP(A.alloc(&S.label, (size_t)Rp));
     PT_HIP(A.alloc(&S.best, 10));
     PT_HIP(A.alloc(&S.round_base, 1));
+    PT_HIP(hipMemsetAsync(S.Eb, 0, 2 * sizeof(int64_t) * Rp, st));
+    PT_HIP(hipMemsetAsync(S.dacc, 0, 3 * sizeof(int64_t) * Rp, st));
+    PT_HIP(hipMemsetAsync(S.einit, 0, sizeof(int64_t) * Rp, st));
+    PT_HIP(hipMemsetAsync(S.rung, 0, 2 * sizeof(int32_t) * Rp, st));
+    PT_HIP(hipMemsetAsync(S.rep, 0, 2 * sizeof(int32_t) * Rp, st));
+    PT_HIP(hipMemsetAsync(S.label, 0, sizeof(int32_t) * Rp, st));
+    PT_HIP(hipMemsetAsync(S.best, 0, 10 * sizeof(int64_t), st));
+    PT_HIP(hipMemsetAsync(S.round_base, 0, sizeof(int32_t), st));
+    PT_HIP(pt_launch_slab_init(I, S, st));
+    return 0;
+}
+
+int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
+    DevArena A;
+    PtInstance I;
+    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
+    if (rc) return rc;
+    const int n = H.n;
     StreamGuard sg;
     PT_HIP(hipStreamCreate(&sg.s));
     PT_HIP(hipEventCreate(&sg.e0));
     PT_HIP(hipEventCreate(&sg.e1));
-    PT_HIP(hipMemsetAsync(S.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(S.dacc, 0, 3 * sizeof(int64_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(S.einit, 0, sizeof(int64_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(S.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(S.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(S.label, 0, sizeof(int32_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(S.best, 0, 10 * sizeof(int64_t), sg.s));
-    PT_HIP(hipMemsetAsync(S.round_base, 0, sizeof(int32_t), sg.s));
+    SState S;
+    rc = slab_setup(A, H, I, S, sg.s);
+    if (rc) return rc;
+    const size_t lds = pt_slab_lds_bytes(n, H.lL, H.ldim, S.NB);
     PT_HIP(pt_slab_set_lds(H.ldim, S.NB, lds));
     int block = block_req > 0 ? block_req : 512;
-    block = std::max(64, std::min(512, (block / 64) * 64));
-    PT_HIP(pt_launch_slab_init(I, S, sg.s));
+    block = std::max(64, std::min(1024, (block / 64) * 64));
     const int per_round = std::max(H.S, 1);
     auto launch_round = [&](int rl) -> int {
         if (H.S == 0) {
@@ -1018,37 +1076,85 @@ int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
     float ms = 0.0f;
     PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
 
-    const int fin = H.n_rounds & 1;
-    std::vector<int64_t> out((size_t)pt_out_len(H.R));
-    PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-    int64_t best[5];
-    PT_HIP(hipMemcpy(best, S.best + 5 * fin, sizeof(best), hipMemcpyDeviceToHost));
-    std::vector<int64_t> E((size_t)Rp);
-    std::vector<int32_t> rung((size_t)Rp);
-    PT_HIP(hipMemcpy(E.data(), S.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
-    PT_HIP(hipMemcpy(rung.data(), S.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
-    for (int x = 0; x < 5; ++x) out[x] = best[x];
-    for (int r = 0; r < H.R; ++r) {
-        pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
-        pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
-    }
-    std::vector<int8_t> fs;
-    if (H.flags & DRIFT_PT_RECORD_FINAL) {
-        const int buf = (int)(((int64_t)H.n_rounds * H.S) & 1);
-        std::vector<int8_t> slot_major((size_t)n * Rp);
-        PT_HIP(hipMemcpy(slot_major.data(), S.sp + (size_t)buf * Rp * n, slot_major.size(),
-                         hipMemcpyDeviceToHost));
-        fs.resize((size_t)n * Rp);
-        for (int p = 0; p < n; ++p)
-            for (int a = 0; a < H.R; ++a) fs[(size_t)p * Rp + a] = slot_major[(size_t)a * n + H.orig[p]];
-    }
-    rc = fetch_results(H, I, out, fs, res);
+    rc = finish_slab(H, I, S, res);
     if (rc) return rc;
     res->seconds = ms * 1e-3;
     res->engine_used = DRIFT_PT_ENGINE_SLAB;
     res->kernel_seconds = res->seconds;   // one sweep kernel per sweep carries all the work
     res->kernel_attempts = res->attempts;
     res->kernel_launches = (int64_t)H.n_rounds * per_round + 1;
+    return 0;
+}
+
+// Block size of the persistent slab engine for this instance, or 0 when its grid cannot be
+// co-resident.
+int pslab_block(const HostInst& H, int block_req, size_t& lds) {
+    int dev = 0, coop = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const int NB = slab_count(H);
+    lds = pt_slab_lds_bytes(H.n, H.lL, H.ldim, NB);
+    const int64_t W = (int64_t)((H.R + 7) / 8) * 8 * NB;
+    // an explicit block size is taken or refused; otherwise the largest that fits
+    const int first = block_req > 0 ? std::max(64, std::min(1024, (block_req / 64) * 64)) : 1024;
+    for (int block = first; block >= 64; block /= 2) {
+        int per_cu = 0;
+        if (pt_pslab_occupancy(H.ldim, NB, block, lds, &per_cu) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        if ((int64_t)per_cu * cus >= W) return block;
+        if (block_req > 0) break;
+    }
+    return 0;
+}
+
+int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
+    size_t lds = 0;
+    const int block = pslab_block(H, block_req, lds);
+    if (!block) return fail(DRIFT_PT_EINVAL, "pslab engine: the slab grid cannot be co-resident");
+    DevArena A;
+    PtInstance I;
+    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
+    if (rc) return rc;
+    StreamGuard sg;
+    PT_HIP(hipStreamCreate(&sg.s));
+    PT_HIP(hipEventCreate(&sg.e0));
+    PT_HIP(hipEventCreate(&sg.e1));
+    SState S;
+    rc = slab_setup(A, H, I, S, sg.s);
+    if (rc) return rc;
+    CoopArgs C{};
+    const int W = pt_pslab_grid(I, S);
+    PT_HIP(A.alloc(&C.flags, (size_t)W));
+    PT_HIP(A.alloc(&C.gen, 64));
+    PT_HIP(A.alloc(&C.abort, 1));
+    PT_HIP(hipMemsetAsync(C.flags, 0, sizeof(uint32_t) * W, sg.s));
+    PT_HIP(hipMemsetAsync(C.gen, 0, sizeof(uint32_t), sg.s));
+    PT_HIP(hipMemsetAsync(C.abort, 0, sizeof(int32_t), sg.s));
+    int dev = 0, khz = 0;
+    PT_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    C.timeout_ticks = (unsigned long long)khz * 1000ull * 20ull;   // 20 s per barrier
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    PT_HIP(pt_launch_pslab(I, S, C, block, lds, sg.s));
+    PT_HIP(hipEventRecord(sg.e1, sg.s));
+    PT_HIP(hipStreamSynchronize(sg.s));
+    float ms = 0.0f;
+    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    int32_t aborted = 0;
+    PT_HIP(hipMemcpy(&aborted, C.abort, sizeof(aborted), hipMemcpyDeviceToHost));
+    if (aborted) return fail(DRIFT_PT_EHIP, "pslab engine: a grid barrier wait exceeded its bound");
+    rc = finish_slab(H, I, S, res);
+    if (rc) return rc;
+    res->seconds = ms * 1e-3;
+    res->engine_used = DRIFT_PT_ENGINE_PSLAB;
+    res->kernel_seconds = res->seconds;
+    res->kernel_attempts = res->attempts;
+    res->kernel_launches = 1;
     return 0;
 }
 
@@ -1076,6 +1182,10 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
         std::vector<HostInst> hs{H};
         return run_lds(hs, {res}, block);
     }
+    case DRIFT_PT_ENGINE_PSLAB:
+        if (!slab_fits(H))
+            return fail(DRIFT_PT_EINVAL, "pslab engine needs what the slab engine needs");
+        return run_pslab(H, res, block);
     case DRIFT_PT_ENGINE_COOP: {
         CoopPlan P;
         if (!coop_plan(H, block, P))
@@ -1121,6 +1231,9 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
     for (int b : blocks) {
         if (fits_lds(H)) cands.emplace_back(DRIFT_PT_ENGINE_LDS, b);
         if (H.ldim && slab_fits(H)) cands.emplace_back(DRIFT_PT_ENGINE_SLAB, b);
+        size_t plds = 0;
+        if (H.ldim && slab_fits(H) && pslab_block(H, b, plds) == b)
+            cands.emplace_back(DRIFT_PT_ENGINE_PSLAB, b);
         if (H.ldim) cands.emplace_back(DRIFT_PT_ENGINE_LATTICE, b);
         cands.emplace_back(DRIFT_PT_ENGINE_GLOBAL, b);
         CoopPlan P;

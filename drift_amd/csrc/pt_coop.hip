@@ -28,57 +28,6 @@ struct CoopShared {
     int abort;
 };
 
-// Every access one workgroup makes to data another workgroup wrote in this launch (spins, energy
-// sums, barrier flags) is an agent-scope relaxed atomic: coherent across the chip without cache
-// maintenance.
-__device__ __forceinline__ uint32_t ld_u32(const uint32_t* p) {
-    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void st_u32(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ int64_t ld_coh(const int64_t* p) {
-    return __hip_atomic_load(const_cast<int64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier with a bounded wait.  Each wave waits for its own stores to complete; then every
-// workgroup stores the phase number into its own arrival flag, workgroup 0 polls all flags (one
-// thread per flag) and publishes the phase in the generation word, which the others poll.  No
-// read-modify-write is involved, so arrivals do not serialise on one address, and only workgroup
-// 0 reads the W flags.  A wait longer than timeout_ticks (or another workgroup's abort) sets the
-// abort flag; the caller then leaves the kernel, so every wave reaches the end.
-__device__ __forceinline__ bool wait_phase(const CoopArgs& C, const uint32_t* p, uint32_t phase) {
-    const unsigned long long t0 = (unsigned long long)wall_clock64();
-    while (ld_u32(p) < phase) {
-        if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-            (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
-            __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return true;
-}
-
-__device__ __forceinline__ bool coop_sync(const CoopArgs& C, uint32_t& phase, int* sh_abort) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    ++phase;
-    if (blockIdx.x == 0) {
-        for (int x = 1 + threadIdx.x; x < (int)gridDim.x; x += blockDim.x)
-            if (!wait_phase(C, C.flags + x, phase)) *sh_abort = 1;
-        __syncthreads();
-        if (threadIdx.x == 0 && *sh_abort == 0) st_u32(C.gen, phase);
-    } else if (threadIdx.x == 0) {
-        st_u32(C.flags + blockIdx.x, phase);
-        if (!wait_phase(C, C.gen, phase)) *sh_abort = 1;
-    }
-    __syncthreads();
-    return *sh_abort == 0;
-}
-
 // First rank and size of workgroup w's slice of colour class [cs, cs + cz).
 __device__ __forceinline__ void slice(int cs, int cz, int w, int W, int& lo, int& cnt) {
     lo = cs + (int)((int64_t)cz * w / W);

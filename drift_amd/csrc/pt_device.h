@@ -52,6 +52,10 @@ __device__ __forceinline__ int8_t init_spin(const PtInstance& I, int p, int a) {
     return (pt_word(r, i & 3) & 1u) ? (int8_t)1 : (int8_t)-1;
 }
 
+// Counter increment without waiting for the old value (each counter has one writer per round;
+// the atomic only spares the dependent load-add-store round trip to memory).
+__device__ __forceinline__ void bump(int64_t* p) { atomicAdd((unsigned long long*)p, 1ull); }
+
 // Exchange of one round for the pairs (r, r+1), r = rnd%2 + 2m (parallel_tempering.py:104-113),
 // with the permutation rung<->slot instead of moving configurations.  Every pair is disjoint, so
 // each thread owns one pair.  Caller syncs before and after; null counters are not updated.
@@ -67,9 +71,9 @@ __device__ __forceinline__ void exchange_pairs(const PtInstance& I, const double
         const pt_u4 rv = pt_philox((uint32_t)(r >> 2), 0u, (uint32_t)rnd, PT_TAG_EXCH, I.k0, I.k1);
         const bool ok = pt_accept_exchange(beta[r], beta[r + 1], (double)E[a] * I.unit,
                                            (double)E[b] * I.unit, pt_word(rv, r & 3));
-        if (tried) tried[r] += 1;
+        if (tried) bump(tried + r);
         if (ok) {
-            if (accd) accd[r] += 1;
+            if (accd) bump(accd + r);
             rep[r] = b;
             rep[r + 1] = a;
             rung[a] = r + 1;
@@ -96,16 +100,16 @@ __device__ __forceinline__ void update_labels(int R, const int32_t* rung, int32_
         int lb = label[a];
         if (g == 0) {
             if (lb == -1) {
-                if (trips32) trips32[a] += 1;
-                if (trips64) trips64[a] += 1;
+                if (trips32) atomicAdd(trips32 + a, 1);
+                if (trips64) bump(trips64 + a);
             }
             lb = 1;
         } else if (g == R - 1) {
             lb = -1;
         }
         label[a] = lb;
-        if (up && lb == 1) up[g] += 1;
-        if (down && lb == -1) down[g] += 1;
+        if (up && lb == 1) bump(up + g);
+        if (down && lb == -1) bump(down + g);
     }
 }
 
@@ -256,6 +260,58 @@ __device__ __forceinline__ int64_t site_energy(int p, int a, int Rp, const int8_
         const int64_t v = lj + 2 * hh;
         return si > 0 ? v : -v;
     }
+}
+
+// Persistent engines (coop, pslab): every access one workgroup makes to data another workgroup
+// wrote in the same launch (spins, energy sums, barrier flags) is an agent-scope relaxed atomic,
+// which gfx950 compiles to sc1 loads and stores: coherent across the chip without cache
+// maintenance.
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_u32(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int64_t ld_coh(const int64_t* p) {
+    return __hip_atomic_load(const_cast<int64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier with a bounded wait.  Each wave waits for its own stores to complete; then every
+// workgroup stores the phase number into its own arrival flag, workgroup 0 polls all flags (one
+// thread per flag) and publishes the phase in the generation word, which the others poll.  No
+// read-modify-write is involved, so arrivals do not serialise on one address, and only workgroup
+// 0 reads the W flags.  A wait longer than timeout_ticks (or another workgroup's abort) sets the
+// abort flag; the caller then leaves the kernel, so every wave reaches the end.
+__device__ __forceinline__ bool wait_phase(const CoopArgs& C, const uint32_t* p, uint32_t phase) {
+    const unsigned long long t0 = (unsigned long long)wall_clock64();
+    while (ld_u32(p) < phase) {
+        if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
+            __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool coop_sync(const CoopArgs& C, uint32_t& phase, int* sh_abort) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    ++phase;
+    if (blockIdx.x == 0) {
+        for (int x = 1 + threadIdx.x; x < (int)gridDim.x; x += blockDim.x)
+            if (!wait_phase(C, C.flags + x, phase)) *sh_abort = 1;
+        __syncthreads();
+        if (threadIdx.x == 0 && *sh_abort == 0) st_u32(C.gen, phase);
+    } else if (threadIdx.x == 0) {
+        st_u32(C.flags + blockIdx.x, phase);
+        if (!wait_phase(C, C.gen, phase)) *sh_abort = 1;
+    }
+    __syncthreads();
+    return *sh_abort == 0;
 }
 
 }  // namespace pt_dev

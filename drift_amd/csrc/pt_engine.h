@@ -107,6 +107,12 @@ hipError_t pt_launch_slab_sweep(const PtInstance& I, const SState& S, int j, int
                                 int block, size_t lds, hipStream_t s);
 hipError_t pt_launch_slab_tick(const SState& S, int rounds, hipStream_t s);
 hipError_t pt_slab_set_lds(int dim, int NB, size_t lds);
+// Persistent slab engine: one cooperative launch of pt_pslab_grid workgroups runs every round
+// (C.flags needs one entry per workgroup).  SState.dacc rotates through three buffers by round.
+int pt_pslab_grid(const PtInstance& I, const SState& S);
+hipError_t pt_pslab_occupancy(int dim, int NB, int block, size_t lds, int* per_cu);
+hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs& C, int block,
+                           size_t lds, hipStream_t s);
 
 size_t pt_lds_bytes(int n, int Rp);
 

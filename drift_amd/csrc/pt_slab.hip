@@ -63,37 +63,68 @@ __device__ __forceinline__ void wg_slot_slab(int w, int NB, int& a, int& b) {
     b = k % NB;
 }
 
+// Round-end inputs of slot x = threadIdx.x, loaded before the sweep's staging so both sets of
+// loads are in flight together.
+struct ReIn {
+    int64_t E;
+    int32_t rep, rung;
+    int64_t best[5];   // best record after the previous round end (tid 0 only)
+};
+
+__device__ __forceinline__ ReIn re_load(const PtInstance& I, const SState& S, int rnd, int x) {
+    ReIn v{0, x, x, {0, 0, 0, 0, 0}};
+    if (x == 0 && rnd > 0)
+        for (int k = 0; k < 5; ++k) v.best[k] = S.best[5 * ((rnd - 1) & 1) + k];
+    if (x >= I.R) return v;
+    if (rnd == 0) {
+        v.E = -(S.einit[x] / 2);
+    } else {
+        const int prev = rnd - 1, pin = prev & 1, din = (prev % 3 + 3) % 3;
+        v.E = S.Eb[(size_t)pin * I.Rp + x] + S.dacc[(size_t)din * I.Rp + x];
+        v.rep = S.rep[(size_t)pin * I.Rp + x];
+        v.rung = S.rung[(size_t)pin * I.Rp + x];
+    }
+    return v;
+}
+
 // Round end of round rnd-1 (or the initial state when rnd == 0), evaluated by every workgroup.
-// Returns the rung of slot a for round rnd.
+// Returns the rung of slot a for round rnd.  mine holds re_load(tid); the best state of the
+// workgroup's own planes is read as src[site - src_off] (the staged planes in LDS, or the spin
+// buffer itself).
 template <int DIM>
 __device__ int slab_round_end(const PtInstance& I, const SState& S, const Geo& g, int rnd, int a,
-                              const int8_t* spins, int z0, int z1, int64_t* Es, int32_t* reps,
-                              int32_t* rungs, long long* scrE, int* scrR) {
+                              const ReIn& mine_in, const int8_t* src, int src_off, int z0, int z1,
+                              int64_t* Es, int32_t* reps, int32_t* rungs, long long* scrE,
+                              int* scrR) {
     const int R = I.R, Rp = I.Rp, tid = threadIdx.x, nt = blockDim.x;
     const bool init = rnd == 0, owner = blockIdx.x == 0;
-    const int prev = rnd - 1, pin = prev & 1, din = (prev % 3 + 3) % 3;
-    for (int x = tid; x < R; x += nt) {
-        if (init) {
-            Es[x] = -(S.einit[x] / 2);
-            reps[x] = x;
-            rungs[x] = x;
-        } else {
-            Es[x] = S.Eb[(size_t)pin * Rp + x] + S.dacc[(size_t)din * Rp + x];
-            reps[x] = S.rep[(size_t)pin * Rp + x];
-            rungs[x] = S.rung[(size_t)pin * Rp + x];
-        }
+    const int prev = rnd - 1;
+    if (tid < R) {
+        Es[tid] = mine_in.E;
+        reps[tid] = mine_in.rep;
+        rungs[tid] = mine_in.rung;
+    }
+    for (int x = tid + nt; x < R; x += nt) {
+        const ReIn v = re_load(I, S, rnd, x);
+        Es[x] = v.E;
+        reps[x] = v.rep;
+        rungs[x] = v.rung;
     }
     __syncthreads();
     int64_t mE;
     int32_t mR;
     block_best_rung(Es, reps, R, scrE, scrR, mE, mR);
-    const int64_t* bc = S.best + 5 * pin;
+    __shared__ long long bcs[5];
+    if (tid == 0)
+        for (int k = 0; k < 5; ++k) bcs[k] = mine_in.best[k];
+    __syncthreads();
+    const long long* bc = bcs;
     const bool have = !init && bc[4] != 0;
     const bool improved = !have || mE < bc[0];
     const int slot = reps[mR];
     if (improved && slot == a)
         for (int site = z0 * g.P + tid; site < z1 * g.P; site += nt)
-            I.best_s[pos_of<DIM>(site, g)] = spins[site];
+            I.best_s[pos_of<DIM>(site, g)] = src[site - src_off];
     int64_t* out = I.out;
     if (owner) {
         if (tid == 0) {
@@ -135,57 +166,21 @@ __device__ int slab_round_end(const PtInstance& I, const SState& S, const Geo& g
     return mine;
 }
 
+// Both colour steps of sweep t on a staged slab: colour 0 on the owned planes and one halo plane
+// per side, then colour 1 on the owned planes.  Returns the energy change of the owned sites.
+// lsp: spin slots (slot s holds plane (WRAP ? s : z0 - 2 + s) mod L); ljb: bond signs of slots
+// jlo.. ; t2, t4, t6: thresholds of q = 2, 4, 6 at this replica's rung.
 template <int DIM, bool WRAP>
-__global__ __launch_bounds__(512) void slab_sweep(PtInstance I, SState S, int j, int rl,
-                                                  int do_sweep) {
-    extern __shared__ __attribute__((aligned(16))) int8_t lsp[];
-    __shared__ int64_t Es[1024];
-    __shared__ int32_t reps[1024];
-    __shared__ int32_t rungs[1024];
-    __shared__ long long scrE[16];
-    __shared__ int scrR[16];
-    int a, b;
-    wg_slot_slab(blockIdx.x, S.NB, a, b);
-    if (a >= I.R) return;   // whole workgroup: padding slots of the grid
-    const Geo g = make_geo<DIM>(I);
-    const int L = g.L, P = g.P, tid = threadIdx.x, nt = blockDim.x;
-    const int Lb = 31 - __builtin_clz(L);                // L is a power of two
+__device__ __forceinline__ int64_t slab_colours(const PtInstance& I, const Geo& g, int8_t* lsp,
+                                                const int8_t* ljb, int a, uint32_t t, int z0,
+                                                int T, int jlo, uint32_t t2, uint32_t t4,
+                                                uint32_t t6) {
+    const int L = g.L, tid = threadIdx.x, nt = blockDim.x;
+    const int Lb = 31 - __builtin_clz(L);
     const int Pb = DIM == 3 ? 2 * Lb : Lb;
-    const int z0 = b * L / S.NB, z1 = (b + 1) * L / S.NB, T = z1 - z0;
-    const int rnd = *S.round_base + rl;
-    const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
-    const int8_t* in = S.sp + ((size_t)(t & 1) * I.Rp + a) * g.N;
-    int8_t* outb = S.sp + ((size_t)((t + 1) & 1) * I.Rp + a) * g.N;
-    const int g_rung = j == 0 ? slab_round_end<DIM>(I, S, g, rnd, a, in, z0, z1, Es, reps, rungs,
-                                                     scrE, scrR)
-                              : S.rung[(size_t)(rnd & 1) * I.Rp + a];
-    if (!do_sweep) return;
-
-    // stage spins of slots 0..nslots-1 and bond signs of the updated slots 1..T+2 (all slots when
-    // the lattice wraps inside the workgroup); slot s holds plane (WRAP ? s : z0 - 2 + s) mod L
-    const int nslots = WRAP ? L : T + 4;
-    const int jlo = WRAP ? 0 : 1, njs = WRAP ? L : T + 2;
-    int8_t* ljb = lsp + (size_t)nslots * P;
-    const int wpp = P >> 2;
-    for (int idx = tid; idx < nslots * wpp; idx += nt) {
-        const int s = idx >> (Pb - 2), w = idx & (wpp - 1);
-        const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
-        ((uint32_t*)lsp)[idx] = ((const uint32_t*)(in + ((size_t)plane << Pb)))[w];
-    }
-    for (int idx = tid; idx < njs * wpp; idx += nt) {
-        const int s = jlo + (idx >> (Pb - 2)), w = idx & (wpp - 1);
-        const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
-        ((uint32_t*)ljb)[idx] = ((const uint32_t*)(I.jbits + ((size_t)plane << Pb)))[w];
-    }
-    const uint32_t* tr = I.thr + (size_t)g_rung * I.TQ;
-    const uint32_t t2 = I.TQ >= 2 ? tr[1] : 0u;
-    const uint32_t t4 = I.TQ >= 4 ? tr[3] : 0u;
-    const uint32_t t6 = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
-    __syncthreads();
-
+    int64_t dsum = 0;
     // Quad k of plane `plane`, colour c holds positions c N/2 + plane P/2 + 4k .. +3, i.e. the
     // colour-c site of each of the site pairs (plane P + 8k + 2kk, +1), kk = 0..3.
-    int64_t dsum = 0;
     const int Qb = Pb - 3;                   // log2 of quads per plane and colour (P/8)
     for (int c = 0; c < 2; ++c) {
         const int s_lo = WRAP ? 0 : (c == 0 ? 1 : 2);
@@ -239,8 +234,83 @@ __global__ __launch_bounds__(512) void slab_sweep(PtInstance I, SState S, int j,
         __syncthreads();
     }
 
-    // own planes to the other buffer
+    return dsum;
+}
+
+template <int DIM, bool WRAP>
+__global__ __launch_bounds__(1024) void slab_sweep(PtInstance I, SState S, int j, int rl,
+                                                   int do_sweep) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lsp[];
+    __shared__ int64_t Es[1024];
+    __shared__ int32_t reps[1024];
+    __shared__ int32_t rungs[1024];
+    __shared__ uint32_t thr3[3 * 1024];      // thresholds of q = 2, 4, 6 for every rung
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    int a, b;
+    wg_slot_slab(blockIdx.x, S.NB, a, b);
+    if (a >= I.R) return;   // whole workgroup: padding slots of the grid
+    const Geo g = make_geo<DIM>(I);
+    const int L = g.L, P = g.P, tid = threadIdx.x, nt = blockDim.x;
+    const int Lb = 31 - __builtin_clz(L);                // L is a power of two
+    const int Pb = DIM == 3 ? 2 * Lb : Lb;
+    const int z0 = b * L / S.NB, z1 = (b + 1) * L / S.NB, T = z1 - z0;
+    const int rnd = *S.round_base + rl;
+    const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
+    const int8_t* in = S.sp + ((size_t)(t & 1) * I.Rp + a) * g.N;
+    int8_t* outb = S.sp + ((size_t)((t + 1) & 1) * I.Rp + a) * g.N;
+    const bool re = j == 0;
+    const ReIn mine_in = re ? re_load(I, S, rnd, tid) : ReIn{0, 0, 0, {0, 0, 0, 0, 0}};
+    if (!do_sweep) {
+        if (re) slab_round_end<DIM>(I, S, g, rnd, a, mine_in, in, 0, z0, z1, Es, reps, rungs, scrE, scrR);
+        return;
+    }
+    const int keep_rung = re ? 0 : S.rung[(size_t)(rnd & 1) * I.Rp + a];
+
+    // Stage spins of slots 0..nslots-1 and bond signs of the updated slots 1..T+2 (all slots when
+    // the lattice wraps inside the workgroup); slot s holds plane (WRAP ? s : z0 - 2 + s) mod L.
+    // Each thread issues eight independent word loads per pass (an index past the end reads word
+    // 0 and is not stored), then the thresholds of every rung.
+    const int nslots = WRAP ? L : T + 4;
+    const int jlo = WRAP ? 0 : 1, njs = WRAP ? L : T + 2;
+    int8_t* ljb = lsp + (size_t)nslots * P;
+    const int wpp = P >> 2;
+    const int nsw = nslots * wpp, total = nsw + njs * wpp;
+    for (int base = 0; base < total; base += 8 * nt) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int idx = base + u * nt + tid;
+            idx = idx < total ? idx : 0;
+            const bool spin = idx < nsw;
+            const int r = spin ? idx : idx - nsw;
+            const int s = (spin ? 0 : jlo) + (r >> (Pb - 2)), w = r & (wpp - 1);
+            const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
+            const uint32_t* src = (const uint32_t*)((spin ? in : (const int8_t*)I.jbits) + ((size_t)plane << Pb));
+            v[u] = src[w];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * nt + tid;
+            if (idx < total) ((uint32_t*)lsp)[idx < nsw ? idx : idx - nsw + ((nslots * P) >> 2)] = v[u];
+        }
+    }
+    for (int x = tid; x < I.R; x += nt) {
+        const uint32_t* tr = I.thr + (size_t)x * I.TQ;
+        thr3[3 * x] = I.TQ >= 2 ? tr[1] : 0u;
+        thr3[3 * x + 1] = I.TQ >= 4 ? tr[3] : 0u;
+        thr3[3 * x + 2] = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
+    }
+    __syncthreads();
     const int so = WRAP ? 0 : 2;
+    const int g_rung = re ? slab_round_end<DIM>(I, S, g, rnd, a, mine_in, lsp + (size_t)so * P,
+                                                z0 * P, z0, z1, Es, reps, rungs, scrE, scrR)
+                          : keep_rung;
+    const uint32_t t2 = thr3[3 * g_rung], t4 = thr3[3 * g_rung + 1], t6 = thr3[3 * g_rung + 2];
+
+    const int64_t dsum = slab_colours<DIM, WRAP>(I, g, lsp, ljb, a, t, z0, T, jlo, t2, t4, t6);
+
+    // own planes to the other buffer
     for (int idx = tid; idx < T * wpp; idx += nt) {
         const int s = idx >> (Pb - 2), w = idx & (wpp - 1);
         ((uint32_t*)(outb + ((size_t)(z0 + s) << Pb)))[w] = ((const uint32_t*)lsp)[((s + so) << (Pb - 2)) + w];
@@ -313,6 +383,198 @@ __global__ void slab_tick(int32_t* round_base, int rounds) {
     if (threadIdx.x == 0) *round_base += rounds;
 }
 
+// Persistent slab engine: one cooperative launch runs every round.  Each workgroup keeps its
+// slab (owned planes, two halo planes per side, bond signs) in LDS for the whole run.  After a
+// sweep it publishes its two first and two last owned planes into the spin buffer of the next
+// sweep parity (agent-scope stores); after the grid barrier it reads its four halo planes from
+// there (agent-scope loads).  Round ends are evaluated identically by every workgroup from the
+// energy sums; workgroup 0 alone writes the statistics.  Padding workgroups (slots >= R) only
+// take part in the barriers.
+template <int DIM, bool WRAP>
+__global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopArgs C) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lsp[];
+    __shared__ int64_t Es[1024];
+    __shared__ int64_t EbL[1024];
+    __shared__ int32_t reps[1024];
+    __shared__ int32_t rungs[1024];
+    __shared__ uint32_t thr3[3 * 1024];
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    __shared__ long long bestL[5];
+    __shared__ int sh_abort;
+    int a, b;
+    wg_slot_slab(blockIdx.x, S.NB, a, b);
+    const bool live = a < I.R, owner = blockIdx.x == 0;
+    const Geo g = make_geo<DIM>(I);
+    const int L = g.L, P = g.P, tid = threadIdx.x, nt = blockDim.x;
+    const int Lb = 31 - __builtin_clz(L);
+    const int Pb = DIM == 3 ? 2 * Lb : Lb;
+    const int R = I.R, Rp = I.Rp;
+    const int z0 = b * L / S.NB, z1 = (b + 1) * L / S.NB, T = z1 - z0;
+    const int nslots = WRAP ? L : T + 4;
+    const int jlo = WRAP ? 0 : 1, njs = WRAP ? L : T + 2;
+    const int so = WRAP ? 0 : 2;
+    int8_t* ljb = lsp + (size_t)nslots * P;
+    const int wpp = P >> 2;
+
+    // ---- stage the slab from spin buffer 0 (written by the init kernels) ----------------------
+    if (live) {
+        const int8_t* in = S.sp + (size_t)a * g.N;
+        for (int idx = tid; idx < nslots * wpp; idx += nt) {
+            const int sl = idx >> (Pb - 2), w = idx & (wpp - 1);
+            const int plane = WRAP ? sl : (z0 - 2 + sl) & (L - 1);
+            ((uint32_t*)lsp)[idx] = ((const uint32_t*)(in + ((size_t)plane << Pb)))[w];
+        }
+        for (int idx = tid; idx < njs * wpp; idx += nt) {
+            const int sl = jlo + (idx >> (Pb - 2)), w = idx & (wpp - 1);
+            const int plane = WRAP ? sl : (z0 - 2 + sl) & (L - 1);
+            ((uint32_t*)ljb)[idx] = ((const uint32_t*)(I.jbits + ((size_t)plane << Pb)))[w];
+        }
+    }
+    for (int x = tid; x < R; x += nt) {
+        const uint32_t* tr = I.thr + (size_t)x * I.TQ;
+        thr3[3 * x] = I.TQ >= 2 ? tr[1] : 0u;
+        thr3[3 * x + 1] = I.TQ >= 4 ? tr[3] : 0u;
+        thr3[3 * x + 2] = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
+    }
+    if (tid == 0) {
+        sh_abort = 0;
+        for (int k = 0; k < 5; ++k) bestL[k] = 0;
+    }
+    __syncthreads();
+
+    uint32_t phase = 0;
+    for (int rnd = 0;; ++rnd) {
+        // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
+        for (int x = tid; x < R; x += nt) {
+            if (rnd == 0) {
+                Es[x] = -(S.einit[x] / 2);
+                reps[x] = x;
+                rungs[x] = x;
+                if (owner) S.label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
+            } else {
+                Es[x] = EbL[x] + ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + x);
+            }
+        }
+        __syncthreads();
+        int64_t mE;
+        int32_t mR;
+        block_best_rung(Es, reps, R, scrE, scrR, mE, mR);
+        const bool improved = bestL[4] == 0 || mE < bestL[0];
+        const int slot = reps[mR];
+        // best state of the owned planes.  Different rounds' best slots live on different XCDs,
+        // so these bytes are written through (agent scope): plain stores would leave dirty copies
+        // in several L2s, and the last write-back at kernel end -- not the last round -- would win.
+        if (improved && live && slot == a)
+            for (int site = z0 * P + tid; site < z1 * P; site += nt)
+                st_spin<true>(I.best_s + pos_of<DIM>(site, g), lsp[(size_t)so * P + site - z0 * P]);
+        if (owner && rnd > 0) {
+            if (tid == 0 && I.hist) I.hist[rnd - 1] = (double)(improved ? mE : bestL[0]) * I.unit;
+            if (I.trace)
+                for (int r = tid; r < R; r += nt)
+                    I.trace[(size_t)(rnd - 1) * R + r] = (double)Es[reps[r]] * I.unit;
+        }
+        __syncthreads();   // best and reps are read above before they change below
+        if (tid == 0 && improved) {
+            bestL[0] = mE;
+            bestL[1] = rnd;
+            bestL[2] = mR;
+            bestL[3] = slot;
+            bestL[4] = 1;
+        }
+        if (rnd > 0) {
+            exchange_pairs(I, rnd - 1, Es, reps, rungs,
+                           owner ? pt_out_arr(I.out, R, PT_OUT_TRIED) : nullptr,
+                           owner ? pt_out_arr(I.out, R, PT_OUT_ACC) : nullptr);
+            __syncthreads();
+            if (owner)
+                update_labels(R, rungs, S.label, nullptr, pt_out_arr(I.out, R, PT_OUT_TRIPS),
+                              pt_out_arr(I.out, R, PT_OUT_UP), pt_out_arr(I.out, R, PT_OUT_DOWN));
+        }
+        for (int x = tid; x < R; x += nt) EbL[x] = Es[x];
+        if (owner)   // buffer of round rnd + 1: its last readers finished before this round began
+            for (int x = tid; x < R; x += nt)
+                __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + x, (int64_t)0,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (rnd == I.n_rounds) break;
+
+        // ---- the sweeps of round rnd --------------------------------------------------------
+        const int gr = live ? rungs[a] : 0;
+        const uint32_t t2 = thr3[3 * gr], t4 = thr3[3 * gr + 1], t6 = thr3[3 * gr + 2];
+        int64_t dsum = 0;
+        for (int j = 0; j < I.S; ++j) {
+            const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
+            if (live) dsum += slab_colours<DIM, WRAP>(I, g, lsp, ljb, a, t, z0, T, jlo, t2, t4, t6);
+            int8_t* nxt = S.sp + ((size_t)((t + 1) & 1) * Rp + a) * g.N;
+            if (live && !WRAP) {
+                // the first and last min(2, T) owned planes: the neighbours' halos
+                const int nb = T < 2 ? T : 2;
+                for (int idx = tid; idx < 2 * nb * wpp; idx += nt) {
+                    const int k = idx >> (Pb - 2), w = idx & (wpp - 1);
+                    const int o = k < nb ? k : T - 2 * nb + k;
+                    st_u32((uint32_t*)(nxt + ((size_t)(z0 + o) << Pb)) + w,
+                           ((const uint32_t*)lsp)[((so + o) << (Pb - 2)) + w]);
+                }
+            }
+            if (j == I.S - 1) {
+                long long v = (long long)dsum;
+#pragma unroll
+                for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
+                if ((tid & (PT_WAVE - 1)) == 0) scrE[tid / PT_WAVE] = v;
+                __syncthreads();
+                if (tid == 0 && live) {
+                    long long tot = 0;
+                    for (int wv = 0; wv < (nt + PT_WAVE - 1) / PT_WAVE; ++wv) tot += scrE[wv];
+                    if (tot != 0)
+                        atomicAdd((unsigned long long*)&S.dacc[(size_t)(rnd % 3) * Rp + a],
+                                  (unsigned long long)tot);
+                }
+            }
+            if (!coop_sync(C, phase, &sh_abort)) return;
+            if (live && !WRAP) {
+                // halo slots 0, 1, T + 2, T + 3 as the neighbours left them after this sweep
+                for (int idx = tid; idx < 4 * wpp; idx += nt) {
+                    const int k = idx >> (Pb - 2), w = idx & (wpp - 1);
+                    const int sl = k < 2 ? k : T + k;
+                    const int plane = (z0 - 2 + sl) & (L - 1);
+                    ((uint32_t*)lsp)[(sl << (Pb - 2)) + w] =
+                        ld_u32((const uint32_t*)(nxt + ((size_t)plane << Pb)) + w);
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- final state where the host reads it ---------------------------------------------------
+    if (live) {
+        int8_t* fb = S.sp + ((size_t)(((int64_t)I.n_rounds * I.S) & 1) * Rp + a) * g.N;
+        for (int idx = tid; idx < T * wpp; idx += nt) {
+            const int o = idx >> (Pb - 2), w = idx & (wpp - 1);
+            ((uint32_t*)(fb + ((size_t)(z0 + o) << Pb)))[w] = ((const uint32_t*)lsp)[((so + o) << (Pb - 2)) + w];
+        }
+    }
+    if (owner) {
+        const int fin = I.n_rounds & 1;
+        for (int x = tid; x < R; x += nt) {
+            S.Eb[(size_t)fin * Rp + x] = EbL[x];
+            S.rung[(size_t)fin * Rp + x] = rungs[x];
+            S.rep[(size_t)fin * Rp + x] = reps[x];
+        }
+        if (tid < 5) S.best[5 * fin + tid] = bestL[tid];
+    }
+}
+
+template <int DIM, bool WRAP>
+const void* pslab_fn() {
+    return reinterpret_cast<const void*>(&pslab_run<DIM, WRAP>);
+}
+
+const void* pslab_pick(int dim, bool wrap) {
+    if (dim == 3) return wrap ? pslab_fn<3, true>() : pslab_fn<3, false>();
+    return wrap ? pslab_fn<2, true>() : pslab_fn<2, false>();
+}
+
 }  // namespace
 
 size_t pt_slab_lds_bytes(int n, int L, int dim, int NB) {
@@ -357,4 +619,23 @@ hipError_t pt_slab_set_lds(int dim, int NB, size_t lds) {
 hipError_t pt_launch_slab_tick(const SState& S, int rounds, hipStream_t s) {
     slab_tick<<<1, 64, 0, s>>>(S.round_base, rounds);
     return hipGetLastError();
+}
+
+hipError_t pt_pslab_occupancy(int dim, int NB, int block, size_t lds, int* per_cu) {
+    const void* f = pslab_pick(dim, NB == 1);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, block, lds);
+}
+
+int pt_pslab_grid(const PtInstance& I, const SState& S) { return ((I.R + 7) / 8) * 8 * S.NB; }
+
+hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs& C, int block,
+                           size_t lds, hipStream_t s) {
+    PtInstance i = I;
+    SState st = S;
+    CoopArgs c = C;
+    void* args[] = {(void*)&i, (void*)&st, (void*)&c};
+    return hipLaunchCooperativeKernel(pslab_pick(I.ldim, S.NB == 1), dim3(pt_pslab_grid(I, S)),
+                                      dim3(block), args, (unsigned)lds, s);
 }

@@ -223,6 +223,8 @@ def _to_result(res: _lib.Result, buf: _Buffers, temps, n_rounds, R, history) -> 
         kernel_seconds=float(res.kernel_seconds),
         kernel_attempts=float(res.kernel_attempts),
         kernel_launches=int(res.kernel_launches),
+        other_kernel_seconds=float(res.other_kernel_seconds),
+        other_kernel_launches=int(res.other_kernel_launches),
     )
 
 

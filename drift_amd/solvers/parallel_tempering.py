@@ -59,6 +59,8 @@ class PtResult:
     kernel_seconds: float = None         # device time of the timed sweep-kernel launches
     kernel_attempts: float = None        # attempts those launches performed
     kernel_launches: int = None
+    other_kernel_seconds: float = None
+    other_kernel_launches: int = None
 
 
 def _neighbour_access(model):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_PT_ABI_VERSION 2
+#define DRIFT_PT_ABI_VERSION 3
 
 enum drift_pt_status {
     DRIFT_PT_OK = 0,
@@ -50,9 +50,12 @@ enum drift_pt_engine {
                                    one byte of bond signs per site (+/-1 couplings, no field) */
     DRIFT_PT_ENGINE_SLAB = 4,   /* periodic lattices: one launch per sweep, a workgroup per slab of
                                    planes of one replica, halo planes recomputed in LDS */
-    DRIFT_PT_ENGINE_COOP = 5    /* CSR instances too large for one workgroup: one cooperative launch
+    DRIFT_PT_ENGINE_COOP = 5,   /* CSR instances too large for one workgroup: one cooperative launch
                                    runs every round; each workgroup keeps its share of the rows in
                                    LDS, colour steps are separated by grid barriers */
+    DRIFT_PT_ENGINE_PSLAB = 6   /* the slab engine's lattices in one cooperative launch: slabs stay
+                                   in LDS for the whole run, halo planes move through HBM between
+                                   sweeps, one grid barrier per sweep */
 };
 
 enum drift_pt_flags {
@@ -120,6 +123,9 @@ typedef struct drift_pt_result {
     double kernel_seconds;      /* summed device time of the timed sweep-kernel launches */
     double kernel_attempts;     /* spin-flip attempts those launches performed */
     int64_t kernel_launches;    /* launches timed (DRIFT_PT_TIME_KERNELS), else 1 for the whole loop */
+    double other_kernel_seconds;    /* DRIFT_PT_TIME_KERNELS: summed device time of the timed
+                                       non-sweep launches (round ends of the per-launch engines) */
+    int64_t other_kernel_launches;  /* and their number (0 when the sweep kernel does everything) */
 } drift_pt_result;
 
 /* Timed candidates of one autotune call (drift_pt_autotune). */

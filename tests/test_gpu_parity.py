@@ -144,7 +144,9 @@ def test_config_c1_square_ferromagnet(L):
 
 @pytest.mark.parametrize("case", [("lattice", 2, 6), ("lattice", 3, 4), ("lattice", 3, 6),
                                   ("lattice", 2, 32), ("slab", 2, 8), ("slab", 3, 4),
-                                  ("slab", 3, 8), ("slab", 3, 16), ("slab", 2, 32)])
+                                  ("slab", 3, 8), ("slab", 3, 16), ("slab", 2, 32),
+                                  ("pslab", 2, 8), ("pslab", 3, 4), ("pslab", 3, 8),
+                                  ("pslab", 3, 16), ("pslab", 2, 32)])
 def test_lattice_kernel_shapes(case):
     engine, dim, L = case
     _announce(f"{engine} dim={dim} L={L}")
@@ -154,7 +156,7 @@ def test_lattice_kernel_shapes(case):
         assert g.engine == engine
 
 
-@pytest.mark.parametrize("engine", ["lds", "global", "lattice", "slab", "coop"])
+@pytest.mark.parametrize("engine", ["lds", "global", "lattice", "slab", "coop", "pslab"])
 def test_zero_sweeps_per_round_only_exchanges(engine):
     _announce(f"S=0 {engine}")
     model = cubic_pmj_glass(8, 2)
@@ -177,6 +179,9 @@ def test_config_c2_first_thousand_sweeps():
     beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))     # beta 3.0 .. 0.2
     g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="auto")
     assert enc.weight_type == 1 and g.engine == "slab"
+    _announce("C2 L=32 R=64 1000 sweeps, pslab engine")
+    g3 = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="pslab")[0]
+    assert g3.engine == "pslab"
     for engine in ("lattice", "global", "coop"):
         _announce(f"C2 L=32 R=64 100 sweeps, {engine}")
         g2 = compare(model, R=64, n_rounds=100, S=1, seed=1, beta=beta, engine=engine)[0]
