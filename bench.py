@@ -71,6 +71,29 @@ def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
     return b_s * (d + 1) + b_s + d * b_e + 8.0 + b_h
 
 
+def engine_format(engine: str, model, enc) -> str:
+    """Storage format the engine ran the workload in (what the algorithmic bytes count)."""
+    if engine == "lattice":
+        return ("int8 spins [position][replica], implicit periodic neighbours, one byte of bond "
+                "signs per site")
+    if engine == "slab":
+        return ("int8 spins [replica][site], slabs of planes staged in LDS each sweep, implicit "
+                "periodic neighbours, one byte of bond signs per site")
+    if engine == "pslab":
+        side = round(model.n ** (1 / 3))
+        side = side if side ** 3 == model.n else round(model.n ** 0.5)
+        bits = side >= 32 and side % 32 == 0
+        return ("one launch for the whole run (occupancy-sized grid, bounded grid barriers); "
+                "each workgroup keeps its slab resident in LDS "
+                + ("as 32-site spin words and bond bit-planes (bit-sliced field), halo planes "
+                   "exchanged as bits" if bits else
+                   "as int8 spins and bond-sign bytes, halo planes exchanged as bytes")
+                + "; int8 spins [replica][site] in HBM at the ends of the run")
+    return (f"int8 spins [position][replica], packed CSR edge records "
+            f"({'4' if enc.weight.dtype.itemsize == 1 else '8'} bytes: column and "
+            f"{enc.weight.dtype.name} weight)")
+
+
 def copy_calibration(nbytes: int) -> int:
     """One device-to-device copy of nbytes (torch's copy kernel) for scaling rocprof's
     FETCH_SIZE / WRITE_SIZE counters against a transfer of known size."""
@@ -374,7 +397,10 @@ def main() -> int:
     enc = gpu.encode_problem(model)
     kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,
               block_threads=args.block, record_history=False,
-              autotune=args.engine == "auto" and args.block == 0)
+              autotune=args.engine == "auto" and args.block == 0,
+              # the bench owns the device, so the persistent engines start with an ordinary
+              # dispatch (rocprofv3 cannot trace cooperative launches)
+              plain_launch=True)
 
     print(f"[bench] warm-up {args.warmup} steps", file=sys.stderr, flush=True)
     gpu.parallel_tempering_gpu(model, n_rounds=args.warmup, **kw)
@@ -422,7 +448,12 @@ def main() -> int:
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{kt.engine}.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        if "hbm_bytes_per_launch" in tr:
+            roofline["traffic"] = tr["hbm_bytes_per_launch"]
+        elif "hbm_bytes_per_sweep" in tr and kt.kernel_launches <= 1:
+            # one launch runs the whole timed region
+            roofline["traffic"] = round(tr["hbm_bytes_per_sweep"] * args.steps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -444,13 +475,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "format": ("int8 spins [position][replica], implicit periodic neighbours, one "
-                       "byte of bond signs per site" if res.engine == "lattice" else
-                       "int8 spins [replica][site], slabs of planes staged in LDS, implicit "
-                       "periodic neighbours, one byte of bond signs per site"
-                       if res.engine == "slab" else
-                       f"int8 spins [position][replica], CSR int32 columns, "
-                       f"{enc.weight.dtype.name} weights"),
+            "format": engine_format(res.engine, model, enc),
             "engine": res.engine,
             "n": model.n,
             "replicas": R,

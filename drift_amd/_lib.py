@@ -30,6 +30,7 @@ ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 # enum drift_pt_flags
 RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GRAPH = 1, 2, 4, 8, 16, 32
 AUTOTUNE = 64
+PLAIN_LAUNCH = 128
 TUNE_MAX = 16
 
 

@@ -929,7 +929,9 @@ int run_coop(const HostInst& H, drift_pt_result* res, CoopPlan P) {
         khz = 100000;
     P.C.timeout_ticks = (unsigned long long)khz * 1000ull * 20ull;   // 20 s per barrier
     PT_HIP(hipEventRecord(sg.e0, sg.s));
-    if (H.n_rounds > 0) PT_HIP(pt_launch_coop(I, G, P.C, P.block, P.grid, P.lds, sg.s));
+    if (H.n_rounds > 0)
+        PT_HIP(pt_launch_coop(I, G, P.C, P.block, P.grid, P.lds,
+                              !(H.flags & DRIFT_PT_PLAIN_LAUNCH), sg.s));
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
@@ -1095,13 +1097,13 @@ int pslab_block(const HostInst& H, int block_req, size_t& lds) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
     const int NB = slab_count(H);
-    lds = pt_slab_lds_bytes(H.n, H.lL, H.ldim, NB);
+    lds = pt_pslab_lds_bytes(H.lL, H.ldim, NB);
     const int64_t W = (int64_t)((H.R + 7) / 8) * 8 * NB;
     // an explicit block size is taken or refused; otherwise the largest that fits
-    const int first = block_req > 0 ? std::max(64, std::min(1024, (block_req / 64) * 64)) : 1024;
+    const int first = block_req > 0 ? std::max(64, std::min(1024, (block_req / 64) * 64)) : 512;
     for (int block = first; block >= 64; block /= 2) {
         int per_cu = 0;
-        if (pt_pslab_occupancy(H.ldim, NB, block, lds, &per_cu) != hipSuccess) {
+        if (pt_pslab_occupancy(H.ldim, NB, H.lL, block, lds, &per_cu) != hipSuccess) {
             (void)hipGetLastError();
             return 0;
         }
@@ -1126,6 +1128,12 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     SState S;
     rc = slab_setup(A, H, I, S, sg.s);
     if (rc) return rc;
+    S.hbits = nullptr;
+    if (pt_pslab_bits(H.lL)) {
+        const size_t words = 2 * (size_t)H.Rp * (size_t)H.n / 32;
+        PT_HIP(A.alloc(&S.hbits, words));
+        PT_HIP(hipMemsetAsync(S.hbits, 0, words * sizeof(uint32_t), sg.s));
+    }
     CoopArgs C{};
     const int W = pt_pslab_grid(I, S);
     PT_HIP(A.alloc(&C.flags, (size_t)W));
@@ -1140,7 +1148,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
         khz = 100000;
     C.timeout_ticks = (unsigned long long)khz * 1000ull * 20ull;   // 20 s per barrier
     PT_HIP(hipEventRecord(sg.e0, sg.s));
-    PT_HIP(pt_launch_pslab(I, S, C, block, lds, sg.s));
+    PT_HIP(pt_launch_pslab(I, S, C, block, lds, !(H.flags & DRIFT_PT_PLAIN_LAUNCH), sg.s));
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
@@ -1202,7 +1210,10 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
 // The engine AUTO takes without tuning.
 int default_engine(const HostInst& H) {
     if (fits_lds(H) && (int64_t)H.R * H.n <= (1 << 17)) return DRIFT_PT_ENGINE_LDS;
-    if (H.ldim && slab_fits(H)) return DRIFT_PT_ENGINE_SLAB;
+    if (H.ldim && slab_fits(H)) {
+        size_t lds = 0;
+        return pslab_block(H, 0, lds) ? DRIFT_PT_ENGINE_PSLAB : DRIFT_PT_ENGINE_SLAB;
+    }
     return H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
 }
 

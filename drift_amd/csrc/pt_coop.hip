@@ -335,11 +335,12 @@ hipError_t pt_coop_occupancy(int wm, bool uni, int block, size_t lds, int* per_c
 }
 
 hipError_t pt_launch_coop(const PtInstance& I, const GState& G, const CoopArgs& C, int block,
-                          int grid, size_t lds, hipStream_t s) {
+                          int grid, size_t lds, bool cooperative, hipStream_t s) {
     PtInstance i = I;
     GState g = G;
     CoopArgs c = C;
     void* args[] = {(void*)&i, (void*)&g, (void*)&c};
-    return hipLaunchCooperativeKernel(coop_pick(I.wm, I.Rp % 64 == 0), dim3(grid), dim3(block),
-                                      args, (unsigned)lds, s);
+    const void* f = coop_pick(I.wm, I.Rp % 64 == 0);
+    return cooperative ? hipLaunchCooperativeKernel(f, dim3(grid), dim3(block), args, (unsigned)lds, s)
+                       : hipLaunchKernel(f, dim3(grid), dim3(block), args, lds, s);
 }

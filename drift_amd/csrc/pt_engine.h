@@ -76,6 +76,8 @@ struct SState {
     int32_t* label;    // Rp
     int64_t* best;     // 2 * 5: best after the round-end scan of round r-1, index r & 1
     int32_t* round_base;
+    uint32_t* hbits;   // bit-sliced persistent engine: [sweep parity][slot][plane][word] spin bits
+                       // of the planes workgroups hand to their neighbours (1 = spin -1)
     int32_t NB;        // slabs per replica (1: the whole lattice wraps inside one workgroup)
 };
 
@@ -95,7 +97,7 @@ struct CoopArgs {
 };
 
 hipError_t pt_launch_coop(const PtInstance& I, const GState& G, const CoopArgs& C, int block,
-                          int grid, size_t lds, hipStream_t s);
+                          int grid, size_t lds, bool cooperative, hipStream_t s);
 // Workgroups of the coop kernel one CU holds at this block size and LDS size.
 hipError_t pt_coop_occupancy(int wm, bool uni, int block, size_t lds, int* per_cu);
 
@@ -110,9 +112,12 @@ hipError_t pt_slab_set_lds(int dim, int NB, size_t lds);
 // Persistent slab engine: one cooperative launch of pt_pslab_grid workgroups runs every round
 // (C.flags needs one entry per workgroup).  SState.dacc rotates through three buffers by round.
 int pt_pslab_grid(const PtInstance& I, const SState& S);
-hipError_t pt_pslab_occupancy(int dim, int NB, int block, size_t lds, int* per_cu);
+// The bit-sliced variant runs when the side is a multiple of 32 (rows of 32-bit words).
+bool pt_pslab_bits(int L);
+size_t pt_pslab_lds_bytes(int L, int dim, int NB);
+hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu);
 hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs& C, int block,
-                           size_t lds, hipStream_t s);
+                           size_t lds, bool cooperative, hipStream_t s);
 
 size_t pt_lds_bytes(int n, int Rp);
 

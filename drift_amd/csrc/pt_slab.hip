@@ -401,6 +401,8 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
     __shared__ long long scrE[16];
     __shared__ int scrR[16];
     __shared__ long long bestL[5];
+    __shared__ double betaL[1024];
+    __shared__ int32_t labelL[1024];   // round-trip labels (workgroup 0's copy is the one used)
     __shared__ int sh_abort;
     int a, b;
     wg_slot_slab(blockIdx.x, S.NB, a, b);
@@ -432,6 +434,8 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
         }
     }
     for (int x = tid; x < R; x += nt) {
+        betaL[x] = I.beta[x];
+        labelL[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
         const uint32_t* tr = I.thr + (size_t)x * I.TQ;
         thr3[3 * x] = I.TQ >= 2 ? tr[1] : 0u;
         thr3[3 * x + 1] = I.TQ >= 4 ? tr[3] : 0u;
@@ -451,7 +455,6 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
                 Es[x] = -(S.einit[x] / 2);
                 reps[x] = x;
                 rungs[x] = x;
-                if (owner) S.label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
             } else {
                 Es[x] = EbL[x] + ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + x);
             }
@@ -483,12 +486,12 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
             bestL[4] = 1;
         }
         if (rnd > 0) {
-            exchange_pairs(I, rnd - 1, Es, reps, rungs,
+            exchange_pairs(I, betaL, rnd - 1, Es, reps, rungs,
                            owner ? pt_out_arr(I.out, R, PT_OUT_TRIED) : nullptr,
                            owner ? pt_out_arr(I.out, R, PT_OUT_ACC) : nullptr);
             __syncthreads();
             if (owner)
-                update_labels(R, rungs, S.label, nullptr, pt_out_arr(I.out, R, PT_OUT_TRIPS),
+                update_labels(R, rungs, labelL, nullptr, pt_out_arr(I.out, R, PT_OUT_TRIPS),
                               pt_out_arr(I.out, R, PT_OUT_UP), pt_out_arr(I.out, R, PT_OUT_DOWN));
         }
         for (int x = tid; x < R; x += nt) EbL[x] = Es[x];
@@ -560,6 +563,317 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
             S.Eb[(size_t)fin * Rp + x] = EbL[x];
             S.rung[(size_t)fin * Rp + x] = rungs[x];
             S.rep[(size_t)fin * Rp + x] = reps[x];
+            S.label[x] = labelL[x];
+        }
+        if (tid < 5) S.best[5 * fin + tid] = bestL[tid];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bit-sliced persistent slab engine (side L a multiple of 32).  Same workgroup layout, rounds,
+// barriers and round ends as pslab_run, but the slab lives in LDS as bits: row words of 32 sites
+// (1 = spin -1) and one bit-plane per bond direction (1 = bond -1).  A lane takes an 8-site
+// segment of a row: it XORs the six neighbour words (the row shifted by one site either way, the
+// rows above and below, the same row in the planes above and below) with the bond planes,
+// counts the unsatisfied terms of all 32 sites at once with full adders, and then decides the
+// segment's four sites of the colour being updated against its Philox quad -- the uniforms every
+// engine draws -- and flips them with one LDS atomic XOR.  Halo planes move between workgroups
+// as bit planes.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void fadd(uint32_t a, uint32_t b, uint32_t c, uint32_t& s, uint32_t& cy) {
+    const uint32_t t = a ^ b;
+    s = t ^ c;
+    cy = (a & b) | (c & t);
+}
+
+// int8 spins of one row word (32 sites) -> bits, 1 for spin -1
+__device__ __forceinline__ uint32_t row_bits(const int8_t* src) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint32_t v = ((const uint32_t*)src)[q];
+        const uint32_t nib = ((v >> 7) & 1u) | ((v >> 14) & 2u) | ((v >> 21) & 4u) | ((v >> 28) & 8u);
+        bits |= nib << (4 * q);
+    }
+    return bits;
+}
+
+// bit d of each of 32 bond-sign bytes -> one word
+__device__ __forceinline__ uint32_t bond_bits(const uint8_t* src, int d) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint32_t v = ((const uint32_t*)src)[q] >> d;
+        const uint32_t nib = (v & 1u) | ((v >> 7) & 2u) | ((v >> 14) & 4u) | ((v >> 21) & 8u);
+        bits |= nib << (4 * q);
+    }
+    return bits;
+}
+
+// bits -> 32 int8 spins
+__device__ __forceinline__ void bits_row(uint32_t bits, int8_t* dst) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v |= (((bits >> (4 * q + e)) & 1u) ? 0xFFu : 0x01u) << (8 * e);
+        ((uint32_t*)dst)[q] = v;
+    }
+}
+
+template <int DIM, bool WRAP>
+__global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, CoopArgs C) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lw[];
+    __shared__ int64_t Es[1024];
+    __shared__ int64_t EbL[1024];
+    __shared__ int32_t reps[1024];
+    __shared__ int32_t rungs[1024];
+    __shared__ uint32_t thr3[3 * 1024];
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    __shared__ long long bestL[5];
+    __shared__ double betaL[1024];
+    __shared__ int32_t labelL[1024];
+    __shared__ int sh_abort;
+    constexpr int D = 2 * DIM;
+    int a, b;
+    wg_slot_slab(blockIdx.x, S.NB, a, b);
+    const bool live = a < I.R, owner = blockIdx.x == 0;
+    const Geo g = make_geo<DIM>(I);
+    const int L = g.L, P = g.P, tid = threadIdx.x, nt = blockDim.x;
+    const int Lb = 31 - __builtin_clz(L);
+    const int WPR = L >> 5;                               // words per row
+    const int rows = DIM == 3 ? L : 1;                    // rows per plane
+    const int WPP = rows * WPR;                           // words per plane
+    const int wppb = (DIM == 3 ? Lb : 0) + Lb - 5;        // log2 WPP
+    const int R = I.R, Rp = I.Rp;
+    const int z0 = b * L / S.NB, z1 = (b + 1) * L / S.NB, T = z1 - z0;
+    const int nslots = WRAP ? L : T + 4;
+    const int jlo = WRAP ? 0 : 1, njs = WRAP ? L : T + 2;
+    const int so = WRAP ? 0 : 2;
+    uint32_t* sb = lw;                                    // [slot][row][word]
+    uint32_t* jb = lw + (size_t)nslots * WPP;             // [dir][jslot][row][word]
+    uint32_t* hb = S.hbits;                               // [parity][slot][plane][word]
+    const size_t hrep = (size_t)L * WPP;                  // words per replica and parity
+
+    // ---- stage the slab (bits) and the bond planes ----------------------------------------------
+    if (live) {
+        const int8_t* in = S.sp + (size_t)a * g.N;        // spin buffer 0, written by the init kernels
+        for (int idx = tid; idx < nslots * WPP; idx += nt) {
+            const int sl = idx >> wppb, rw = idx & (WPP - 1);
+            const int plane = WRAP ? sl : (z0 - 2 + sl) & (L - 1);
+            sb[idx] = row_bits(in + ((size_t)plane * P) + ((size_t)rw << 5));
+        }
+        for (int idx = tid; idx < D * njs * WPP; idx += nt) {
+            const int d = idx / (njs * WPP), r2 = idx - d * njs * WPP;
+            const int sl = jlo + (r2 >> wppb), rw = r2 & (WPP - 1);
+            const int plane = WRAP ? sl : (z0 - 2 + sl) & (L - 1);
+            jb[idx] = bond_bits(I.jbits + ((size_t)plane * P) + ((size_t)rw << 5), d);
+        }
+    }
+    for (int x = tid; x < R; x += nt) {
+        betaL[x] = I.beta[x];
+        labelL[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
+        const uint32_t* tr = I.thr + (size_t)x * I.TQ;
+        thr3[3 * x] = I.TQ >= 2 ? tr[1] : 0u;
+        thr3[3 * x + 1] = I.TQ >= 4 ? tr[3] : 0u;
+        thr3[3 * x + 2] = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
+    }
+    if (tid == 0) {
+        sh_abort = 0;
+        for (int k = 0; k < 5; ++k) bestL[k] = 0;
+    }
+    __syncthreads();
+
+    uint32_t phase = 0;
+    const int segb = Lb - 3;                              // log2 segments of 8 sites per row
+    for (int rnd = 0;; ++rnd) {
+        // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
+        for (int x = tid; x < R; x += nt) {
+            if (rnd == 0) {
+                Es[x] = -(S.einit[x] / 2);
+                reps[x] = x;
+                rungs[x] = x;
+            } else {
+                Es[x] = EbL[x] + ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + x);
+            }
+        }
+        __syncthreads();
+        int64_t mE;
+        int32_t mR;
+        block_best_rung(Es, reps, R, scrE, scrR, mE, mR);
+        const bool improved = bestL[4] == 0 || mE < bestL[0];
+        const int slot = reps[mR];
+        // best state of the owned planes, written through (see pslab_run)
+        if (improved && live && slot == a)
+            for (int site = z0 * P + tid; site < z1 * P; site += nt) {
+                const int ls = site - z0 * P;
+                const uint32_t wd = sb[(size_t)so * WPP + (ls >> 5)];
+                st_spin<true>(I.best_s + pos_of<DIM>(site, g), ((wd >> (ls & 31)) & 1u) ? (int8_t)-1 : (int8_t)1);
+            }
+        if (owner && rnd > 0) {
+            if (tid == 0 && I.hist) I.hist[rnd - 1] = (double)(improved ? mE : bestL[0]) * I.unit;
+            if (I.trace)
+                for (int r = tid; r < R; r += nt)
+                    I.trace[(size_t)(rnd - 1) * R + r] = (double)Es[reps[r]] * I.unit;
+        }
+        __syncthreads();
+        if (tid == 0 && improved) {
+            bestL[0] = mE;
+            bestL[1] = rnd;
+            bestL[2] = mR;
+            bestL[3] = slot;
+            bestL[4] = 1;
+        }
+        if (rnd > 0) {
+            exchange_pairs(I, betaL, rnd - 1, Es, reps, rungs,
+                           owner ? pt_out_arr(I.out, R, PT_OUT_TRIED) : nullptr,
+                           owner ? pt_out_arr(I.out, R, PT_OUT_ACC) : nullptr);
+            __syncthreads();
+            if (owner)
+                update_labels(R, rungs, labelL, nullptr, pt_out_arr(I.out, R, PT_OUT_TRIPS),
+                              pt_out_arr(I.out, R, PT_OUT_UP), pt_out_arr(I.out, R, PT_OUT_DOWN));
+        }
+        for (int x = tid; x < R; x += nt) EbL[x] = Es[x];
+        if (owner)
+            for (int x = tid; x < R; x += nt)
+                __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + x, (int64_t)0,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (rnd == I.n_rounds) break;
+
+        // ---- the sweeps of round rnd --------------------------------------------------------
+        const int gr = live ? rungs[a] : 0;
+        const uint32_t t2 = thr3[3 * gr], t4 = thr3[3 * gr + 1], t6 = thr3[3 * gr + 2];
+        int64_t dsum = 0;
+        for (int j = 0; j < I.S; ++j) {
+            const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
+            if (live) {
+                for (int c = 0; c < 2; ++c) {
+                    const int s_lo = WRAP ? 0 : (c == 0 ? 1 : 2);
+                    const int ns = WRAP ? L : (c == 0 ? T + 2 : T);
+                    const int psb = (DIM == 3 ? Lb : 0) + segb;   // log2 segments per slot
+                    for (int w = tid; w < (ns << psb); w += nt) {
+                        const int s = s_lo + (w >> psb), rem = w & ((1 << psb) - 1);
+                        const int r = rem >> segb, k = rem & ((1 << segb) - 1);
+                        const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
+                        const bool owned = WRAP || (s >= 2 && s < T + 2);
+                        const int su = WRAP ? ((s + 1) & (L - 1)) : s + 1;
+                        const int sd = WRAP ? ((s - 1) & (L - 1)) : s - 1;
+                        const int wi = k >> 2;                      // word of the row
+                        uint32_t* rowp = sb + ((size_t)s * rows + r) * WPR;
+                        const uint32_t W = rowp[wi];
+                        const uint32_t Wn = rowp[(wi + 1) & (WPR - 1)];
+                        const uint32_t Wp = rowp[(wi - 1) & (WPR - 1)];
+                        uint32_t A[6];
+                        A[0] = (W >> 1) | (Wn << 31);               // +x
+                        A[1] = (W << 1) | (Wp >> 31);               // -x
+                        if (DIM == 3) {
+                            A[2] = sb[((size_t)s * rows + ((r + 1) & (L - 1))) * WPR + wi];
+                            A[3] = sb[((size_t)s * rows + ((r - 1) & (L - 1))) * WPR + wi];
+                            A[4] = sb[((size_t)su * rows + r) * WPR + wi];
+                            A[5] = sb[((size_t)sd * rows + r) * WPR + wi];
+                        } else {
+                            A[2] = sb[(size_t)su * WPR + wi];
+                            A[3] = sb[(size_t)sd * WPR + wi];
+                        }
+                        const size_t jrow = ((size_t)(s - jlo) * rows + r) * WPR + wi;
+                        uint32_t Y[6];
+#pragma unroll
+                        for (int d = 0; d < D; ++d) Y[d] = A[d] ^ jb[(size_t)d * njs * WPP + jrow];
+                        // unsatisfied-term count of every site: bits c0, c1, c2
+                        uint32_t c0, c1, c2;
+                        if (DIM == 3) {
+                            uint32_t s1, k1, s2, k2;
+                            fadd(Y[0], Y[1], Y[2], s1, k1);
+                            fadd(Y[3], Y[4], Y[5], s2, k2);
+                            c0 = s1 ^ s2;
+                            fadd(k1, k2, s1 & s2, c1, c2);
+                        } else {
+                            uint32_t s1, k1;
+                            fadd(Y[0], Y[1], Y[2], s1, k1);
+                            c0 = s1 ^ Y[3];
+                            const uint32_t k0 = s1 & Y[3];
+                            c1 = k1 ^ k0;
+                            c2 = k1 & k0;
+                        }
+                        const int x0 = k << 3;                      // first site of the segment
+                        const int par = DIM == 3 ? ((c + r + plane) & 1) : ((c + plane) & 1);
+                        const uint32_t q = (uint32_t)((c * g.half) >> 2) +
+                                           (uint32_t)(((size_t)plane * P + (size_t)r * L + x0) >> 3);
+                        const pt_u4 rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                        uint32_t flip = 0;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int bx = (x0 + 2 * i + par) & 31;
+                            const int cnt = (int)(((c0 >> bx) & 1u) | (((c1 >> bx) & 1u) << 1) |
+                                                  (((c2 >> bx) & 1u) << 2));
+                            const int m = ((W >> bx) & 1u) ? D - cnt : cnt;
+                            const int qv = D - 2 * m;
+                            const uint32_t th = qv == 2 ? t2 : (qv == 4 ? t4 : t6);
+                            if (qv <= 0 || pt_word(rv, i) < th) {
+                                flip |= 1u << bx;
+                                if (owned) dsum += 2 * qv;
+                            }
+                        }
+                        if (flip) atomicXor(rowp + wi, flip);
+                    }
+                    __syncthreads();
+                }
+            }
+            uint32_t* nxt = hb + ((size_t)((t + 1) & 1) * Rp + a) * hrep;
+            if (live && !WRAP) {
+                // the first and last min(2, T) owned planes: the neighbours' halos
+                const int nb = T < 2 ? T : 2;
+                for (int idx = tid; idx < 2 * nb * WPP; idx += nt) {
+                    const int kk = idx >> wppb, w = idx & (WPP - 1);
+                    const int o = kk < nb ? kk : T - 2 * nb + kk;
+                    st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
+                }
+            }
+            if (j == I.S - 1) {
+                long long v = (long long)dsum;
+#pragma unroll
+                for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
+                if ((tid & (PT_WAVE - 1)) == 0) scrE[tid / PT_WAVE] = v;
+                __syncthreads();
+                if (tid == 0 && live) {
+                    long long tot = 0;
+                    for (int wv = 0; wv < (nt + PT_WAVE - 1) / PT_WAVE; ++wv) tot += scrE[wv];
+                    if (tot != 0)
+                        atomicAdd((unsigned long long*)&S.dacc[(size_t)(rnd % 3) * Rp + a],
+                                  (unsigned long long)tot);
+                }
+                dsum = 0;
+            }
+            if (!coop_sync(C, phase, &sh_abort)) return;
+            if (live && !WRAP) {
+                for (int idx = tid; idx < 4 * WPP; idx += nt) {
+                    const int kk = idx >> wppb, w = idx & (WPP - 1);
+                    const int sl = kk < 2 ? kk : T + kk;
+                    const int plane = (z0 - 2 + sl) & (L - 1);
+                    sb[(size_t)sl * WPP + w] = ld_u32(nxt + (size_t)plane * WPP + w);
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- final state where the host reads it (int8 spins) ---------------------------------------
+    if (live) {
+        int8_t* fb = S.sp + ((size_t)(((int64_t)I.n_rounds * I.S) & 1) * Rp + a) * g.N;
+        for (int idx = tid; idx < T * WPP; idx += nt) {
+            const int o = idx >> wppb, rw = idx & (WPP - 1);
+            bits_row(sb[(size_t)(so + o) * WPP + rw], fb + (size_t)(z0 + o) * P + ((size_t)rw << 5));
+        }
+    }
+    if (owner) {
+        const int fin = I.n_rounds & 1;
+        for (int x = tid; x < R; x += nt) {
+            S.Eb[(size_t)fin * Rp + x] = EbL[x];
+            S.rung[(size_t)fin * Rp + x] = rungs[x];
+            S.rep[(size_t)fin * Rp + x] = reps[x];
+            S.label[x] = labelL[x];
         }
         if (tid < 5) S.best[5 * fin + tid] = bestL[tid];
     }
@@ -570,7 +884,16 @@ const void* pslab_fn() {
     return reinterpret_cast<const void*>(&pslab_run<DIM, WRAP>);
 }
 
-const void* pslab_pick(int dim, bool wrap) {
+template <int DIM, bool WRAP>
+const void* pslab_bits_fn() {
+    return reinterpret_cast<const void*>(&pslab_bits_run<DIM, WRAP>);
+}
+
+const void* pslab_pick(int dim, bool wrap, bool bits) {
+    if (bits) {
+        if (dim == 3) return wrap ? pslab_bits_fn<3, true>() : pslab_bits_fn<3, false>();
+        return wrap ? pslab_bits_fn<2, true>() : pslab_bits_fn<2, false>();
+    }
     if (dim == 3) return wrap ? pslab_fn<3, true>() : pslab_fn<3, false>();
     return wrap ? pslab_fn<2, true>() : pslab_fn<2, false>();
 }
@@ -621,8 +944,19 @@ hipError_t pt_launch_slab_tick(const SState& S, int rounds, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t pt_pslab_occupancy(int dim, int NB, int block, size_t lds, int* per_cu) {
-    const void* f = pslab_pick(dim, NB == 1);
+bool pt_pslab_bits(int L) { return L >= 32 && (L & 31) == 0; }
+
+size_t pt_pslab_lds_bytes(int L, int dim, int NB) {
+    if (!pt_pslab_bits(L)) return pt_slab_lds_bytes(0, L, dim, NB);
+    const size_t WPP = (dim == 3 ? (size_t)L : 1) * (size_t)(L / 32);
+    const size_t T = (size_t)((L + NB - 1) / NB);
+    const size_t spins = NB == 1 ? (size_t)L : T + 4;
+    const size_t signs = NB == 1 ? (size_t)L : T + 2;
+    return (spins + 2 * (size_t)dim * signs) * WPP * 4;
+}
+
+hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu) {
+    const void* f = pslab_pick(dim, NB == 1, pt_pslab_bits(L));
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, block, lds);
@@ -631,11 +965,13 @@ hipError_t pt_pslab_occupancy(int dim, int NB, int block, size_t lds, int* per_c
 int pt_pslab_grid(const PtInstance& I, const SState& S) { return ((I.R + 7) / 8) * 8 * S.NB; }
 
 hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs& C, int block,
-                           size_t lds, hipStream_t s) {
+                           size_t lds, bool cooperative, hipStream_t s) {
     PtInstance i = I;
     SState st = S;
     CoopArgs c = C;
     void* args[] = {(void*)&i, (void*)&st, (void*)&c};
-    return hipLaunchCooperativeKernel(pslab_pick(I.ldim, S.NB == 1), dim3(pt_pslab_grid(I, S)),
-                                      dim3(block), args, (unsigned)lds, s);
+    const void* f = pslab_pick(I.ldim, S.NB == 1, pt_pslab_bits(I.lL));
+    const dim3 grid(pt_pslab_grid(I, S));
+    return cooperative ? hipLaunchCooperativeKernel(f, grid, dim3(block), args, (unsigned)lds, s)
+                       : hipLaunchKernel(f, grid, dim3(block), args, lds, s);
 }

@@ -162,10 +162,11 @@ class _Buffers:
 
 def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, engine,
             block_threads, final, history, trace=False, time_kernels=False, graph=True,
-            autotune=False):
+            autotune=False, plain_launch=False):
     flags = ((_lib.RECORD_FINAL if final else 0) | (_lib.RECORD_HISTORY if history else 0)
              | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0)
-             | (0 if graph else _lib.NO_GRAPH) | (_lib.AUTOTUNE if autotune else 0))
+             | (0 if graph else _lib.NO_GRAPH) | (_lib.AUTOTUNE if autotune else 0)
+             | (_lib.PLAIN_LAUNCH if plain_launch else 0))
     if engine not in _lib.ENGINES:
         raise ValueError(f"engine must be one of {sorted(_lib.ENGINES)}")
     return _lib.Params(
@@ -248,18 +249,21 @@ def parallel_tempering_gpu(
     graph: bool = True,
     force_float: bool = False,
     autotune: bool = False,
+    plain_launch: bool = False,
 ) -> PtResult:
     """Solve ``model`` by parallel tempering on the GPU; same signature and result as the CPU
     ``parallel_tempering`` (drift/gpu.py:43-114).  ``throughput`` is spin-flip attempts per second
     and ``seconds`` the device time of the sweep and exchange loop.  ``autotune`` (with
     ``engine="auto"``) takes the engine and block size from the library's autotune cache, timing
-    the candidates the first time this shape class is seen; results do not depend on it."""
+    the candidates the first time this shape class is seen; results do not depend on it.
+    ``plain_launch`` starts the persistent engines with an ordinary dispatch (for profilers that
+    cannot follow cooperative launches; needs the device to itself)."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
     prm = _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, b, engine,
                   block_threads, return_final, record_history, record_trace, time_kernels,
-                  graph, autotune)
+                  graph, autotune, plain_launch)
     buf = _Buffers(prob.n, n_replicas, n_rounds, return_final, record_history, record_trace)
     res = buf.struct
     rc = lib.drift_pt_run(prob.struct, prm, res)

@@ -65,8 +65,14 @@ enum drift_pt_flags {
     DRIFT_PT_RECORD_TRACE = 8,    /* fill result.rung_energy_trace */
     DRIFT_PT_TIME_KERNELS = 16,   /* time each sweep-kernel launch (up to 4096) with device events */
     DRIFT_PT_NO_GRAPH = 32,       /* launch every round directly instead of replaying a hipGraph */
-    DRIFT_PT_AUTOTUNE = 64        /* with DRIFT_PT_ENGINE_AUTO: take the engine and block size from
+    DRIFT_PT_AUTOTUNE = 64,       /* with DRIFT_PT_ENGINE_AUTO: take the engine and block size from
                                      the autotune cache, timing the candidates on a miss */
+    DRIFT_PT_PLAIN_LAUNCH = 128   /* persistent engines (coop, pslab): an ordinary dispatch of the
+                                     occupancy-sized grid instead of a cooperative launch.  All
+                                     workgroups are resident when nothing else runs on the device;
+                                     otherwise the bounded grid barrier fails the call after its
+                                     time bound.  For profilers that cannot follow cooperative
+                                     launches. */
 };
 
 /* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and

@@ -177,10 +177,10 @@ def test_config_c2_first_thousand_sweeps():
     _announce("C2 L=32 R=64 1000 sweeps, slab engine")
     model = cubic_pmj_glass(32, 0)
     beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))     # beta 3.0 .. 0.2
-    g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="auto")
+    g, o, enc = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="slab")
     assert enc.weight_type == 1 and g.engine == "slab"
-    _announce("C2 L=32 R=64 1000 sweeps, pslab engine")
-    g3 = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="pslab")[0]
+    _announce("C2 L=32 R=64 1000 sweeps, automatic engine (persistent bit-sliced slabs)")
+    g3 = compare(model, R=64, n_rounds=1000, S=1, seed=1, beta=beta, engine="auto")[0]
     assert g3.engine == "pslab"
     for engine in ("lattice", "global", "coop"):
         _announce(f"C2 L=32 R=64 100 sweeps, {engine}")
@@ -269,4 +269,37 @@ def test_autotune_times_candidates_and_keeps_results():
                               beta=beta, n_threads=THREADS)
     assert g.engine == tab["engine"]
     assert g.best_E_units == o["best_units"]
+    assert np.array_equal(g.final_spins, o["final_spins"])
+
+
+@pytest.mark.parametrize("case", [(2, 64, 12, 2), (3, 64, 8, 2), (3, 32, 3, 3)])
+def test_pslab_bit_sliced_rows(case):
+    """The bit-sliced persistent slab engine on rows of one and two 32-site words (2-D and 3-D),
+    with a replica count that is not a multiple of 8 and several sweeps per round."""
+    dim, L, R, S = case
+    _announce(f"pslab bits dim={dim} L={L} R={R} S={S}")
+    model = square_ferromagnet(L) if dim == 2 else cubic_pmj_glass(L, 7)
+    g = compare(model, R=R, n_rounds=25, S=S, seed=L + R, T_min=0.3, T_max=4.0, engine="pslab")[0]
+    assert g.engine == "pslab"
+
+
+@pytest.mark.parametrize("engine", ["pslab", "coop"])
+def test_persistent_engines_with_an_ordinary_dispatch(engine):
+    """DRIFT_PT_PLAIN_LAUNCH starts the persistent engines without the cooperative-launch API;
+    the chain is the same."""
+    from drift_amd.builders.lattice import er_sparse_weighted
+
+    _announce(f"plain launch {engine}")
+    model = cubic_pmj_glass(32, 5) if engine == "pslab" else er_sparse_weighted(5000, 6.0, 9)
+    enc = gpu.encode_problem(model)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    beta = ladder_betas(geometric_ladder(16, 0.3, 4.0))
+    g = gpu.parallel_tempering_gpu(model, n_replicas=16, n_rounds=20, sweeps_per_round=2, seed=6,
+                                   betas=beta, engine=engine, coloring=(cptr, cspins),
+                                   return_final=True, plain_launch=True)
+    o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=6, R=16, n_rounds=20, sweeps_per_round=2,
+                              beta=beta, n_threads=THREADS)
+    assert g.engine == engine
+    assert g.best_E_units == o["best_units"]
+    assert np.array_equal(g.best_s.astype(np.int8), o["best_s"])
     assert np.array_equal(g.final_spins, o["final_spins"])
