@@ -1265,6 +1265,9 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
         table->trial_rounds = rounds;
     }
     int slot = 0;
+    // Each candidate runs `rounds` and then 2 * `rounds` rounds; the difference is its time for
+    // `rounds` rounds with the fixed costs (staging, ramp, launch) cancelled, which is what a long
+    // run pays.
     for (const auto& c : cands) {
         double secs = -1.0;
         T.n_rounds = 2;                                      // warm-up: code objects, first touch
@@ -1272,7 +1275,13 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
         if (run_engine(T, c.first, c.second, &r) == 0) {
             T.n_rounds = rounds;
             clear_result(&r);
-            if (run_engine(T, c.first, c.second, &r) == 0) secs = r.seconds;
+            if (run_engine(T, c.first, c.second, &r) == 0) {
+                const double t1 = r.seconds;
+                T.n_rounds = 2 * rounds;
+                clear_result(&r);
+                if (run_engine(T, c.first, c.second, &r) == 0)
+                    secs = r.seconds - t1 > 0.0 ? r.seconds - t1 : 0.5 * r.seconds;
+            }
         }
         (void)hipGetLastError();
         if (secs > 0.0 && (best < 0.0 || secs < best)) {

@@ -46,15 +46,15 @@ __device__ inline void block_best_rung(const int64_t* E, const int32_t* rep, int
     __syncthreads();
 }
 
+// Counter increment without waiting for the old value (each counter has one writer per round;
+// the atomic only spares the dependent load-add-store round trip to memory).
+__device__ __forceinline__ void bump(int64_t* p) { atomicAdd((unsigned long long*)p, 1ull); }
+
 __device__ __forceinline__ int8_t init_spin(const PtInstance& I, int p, int a) {
     const int i = I.orig[p];
     const pt_u4 r = pt_philox((uint32_t)(i >> 2), (uint32_t)a, 0u, PT_TAG_INIT, I.k0, I.k1);
     return (pt_word(r, i & 3) & 1u) ? (int8_t)1 : (int8_t)-1;
 }
-
-// Counter increment without waiting for the old value (each counter has one writer per round;
-// the atomic only spares the dependent load-add-store round trip to memory).
-__device__ __forceinline__ void bump(int64_t* p) { atomicAdd((unsigned long long*)p, 1ull); }
 
 // Exchange of one round for the pairs (r, r+1), r = rnd%2 + 2m (parallel_tempering.py:104-113),
 // with the permutation rung<->slot instead of moving configurations.  Every pair is disjoint, so

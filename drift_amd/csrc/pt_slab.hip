@@ -49,10 +49,13 @@ __device__ __forceinline__ int site_of(int p, int c, const Geo& g) {
     return s0 + ((yz + c) & 1);
 }
 
+// colour-list position of a site (the slab engines need a power-of-two side, so coordinates
+// come from shifts)
 template <int DIM>
 __device__ __forceinline__ int pos_of(int site, const Geo& g) {
-    const int par = DIM == 3 ? (site % g.L + (site / g.L) % g.L + site / g.P) & 1
-                             : (site % g.L + site / g.L) & 1;
+    const int Lb = __builtin_ctz(g.L);
+    const int par = DIM == 3 ? ((site & (g.L - 1)) + ((site >> Lb) & (g.L - 1)) + (site >> (2 * Lb))) & 1
+                             : ((site & (g.L - 1)) + (site >> Lb)) & 1;
     return par * g.half + (site >> 1);
 }
 

@@ -143,8 +143,9 @@ typedef struct drift_pt_tuning {
     int32_t n_candidates;
     int32_t cand_engine[DRIFT_PT_TUNE_MAX];
     int32_t cand_block[DRIFT_PT_TUNE_MAX];
-    double cand_seconds[DRIFT_PT_TUNE_MAX];  /* device time of the trial's round loop; < 0 if the
-                                                candidate could not run */
+    double cand_seconds[DRIFT_PT_TUNE_MAX];  /* device time of trial_rounds rounds (a run of
+                                                2 * trial_rounds minus a run of trial_rounds, so
+                                                fixed costs cancel); < 0 if it could not run */
 } drift_pt_tuning;
 
 /* ABI version of the loaded library (DRIFT_PT_ABI_VERSION). */
