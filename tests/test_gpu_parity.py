@@ -303,3 +303,22 @@ def test_persistent_engines_with_an_ordinary_dispatch(engine):
     assert g.best_E_units == o["best_units"]
     assert np.array_equal(g.best_s.astype(np.int8), o["best_s"])
     assert np.array_equal(g.final_spins, o["final_spins"])
+
+
+def test_reference_problem_file_runs_on_the_engine(tmp_path):
+    """A problem.bin as drift/gpu.py writes it, solved by run_problem_bin: the result.bin reads back
+    as the reference parses it, and equals a direct call with the file's parameters."""
+    from drift_amd.problem_io import read_result_bin, run_problem_bin, write_problem_bin
+
+    _announce("problem.bin")
+    model = maxcut_ising(random_graph(60, p=0.2, seed=3))
+    prob, res = str(tmp_path / "problem.bin"), str(tmp_path / "result.bin")
+    write_problem_bin(prob, model, n_replicas=24, n_rounds=80, sweeps_per_round=3, seed=9,
+                      T_min=0.1, T_max=4.0)
+    g = run_problem_bin(prob, res)
+    best_E, best_s = read_result_bin(res, model.n)
+    assert best_E == g.best_E and np.array_equal(best_s, g.best_s)
+    assert model.energy(best_s) == pytest.approx(best_E, abs=1e-9)
+    d = gpu.parallel_tempering_gpu(model, n_replicas=24, T_min=np.float32(0.1), T_max=4.0,
+                                   n_rounds=80, sweeps_per_round=3, seed=9)
+    assert d.best_E == g.best_E
