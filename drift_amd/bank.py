@@ -22,7 +22,8 @@ from .builders.qubo import cut_value, maxcut_ising
 from .solve import solve
 
 __all__ = ["BankRow", "bank_model", "sweep_row", "planted_bipartite", "bipartite_check",
-           "crystal_check"]
+           "crystal_check", "V2_FAMILIES", "V2_GRID", "V2_RUN", "v2_id", "v2_model", "v2_specs",
+           "v2_rounds", "v2_best_known_run"]
 
 
 @dataclass
@@ -104,3 +105,75 @@ def crystal_check(L: int, *, n_replicas: int = 256, n_rounds: int = 600,
     grid = np.asarray(sol.s).reshape(L, L)
     return {"L": L, "method": sol.method, "energy_per_spin": float(sol.energy) / (L * L),
             "period": int(column_period(grid)), "striped": bool(is_striped(grid))}
+
+
+# ------------------------------------------------------------------------------------------------
+# bank v2: the families of the parallel-tempering workloads, fingerprinted over their CSR
+# ------------------------------------------------------------------------------------------------
+V2_FAMILIES = ("ea3d", "er-sparse", "rrg3")
+
+# (family, size parameter, seeds): 3-D +/-J glasses by side L, sparse ER integer graphs (mean
+# degree 6, weights +-1..3) and 3-regular N(0,1) graphs (field N(0, 0.1^2)) by spin count
+V2_GRID = (
+    ("ea3d", (8, 12, 16, 32), (1, 2)),
+    ("er-sparse", (1000, 10000, 100000), (1, 2)),
+    ("rrg3", (512, 2048, 8192), (1, 2)),
+)
+
+
+def v2_id(family: str, size: int, seed: int) -> str:
+    key = "L" if family == "ea3d" else "n"
+    return f"v2.{family}.{key}{size:06d}.s{seed:03d}"
+
+
+def v2_model(family: str, size: int, seed: int):
+    """The instance of a v2 row (SparseIsingModel)."""
+    from .builders.lattice import cubic_pmj_glass, er_sparse_weighted, random_regular_gaussian
+
+    if family == "ea3d":
+        return cubic_pmj_glass(size, seed)
+    if family == "er-sparse":
+        return er_sparse_weighted(size, 6.0, seed)
+    if family == "rrg3":
+        return random_regular_gaussian(size, seed)
+    raise ValueError(f"unknown v2 family {family!r}; expected one of {V2_FAMILIES}")
+
+
+def v2_specs() -> list[dict]:
+    """Every v2 row: {id, family, size, seed, n, nnz, fingerprint} (CSR sha256[:16], the bank's
+    fingerprint convention of drift/benchmarks/bank.py:113-117 applied to the sparse arrays)."""
+    from .ising import csr_fingerprint
+
+    out = []
+    for family, sizes, seeds in V2_GRID:
+        for size in sizes:
+            for seed in seeds:
+                m = v2_model(family, size, seed)
+                out.append({"id": v2_id(family, size, seed), "family": family, "size": size,
+                            "seed": seed, "n": m.n, "nnz": m.nnz,
+                            "fingerprint": csr_fingerprint(m.row_ptr, m.col_idx, m.weight, m.h)})
+    return out
+
+
+# The recorded best-known run of every v2 row: PT with R replicas over the geometric ladder
+# T_MIN..T_MAX, one sweep and one exchange per round; the runs are deterministic (the Philox
+# stream contract), so a replay reproduces the recorded energy exactly.
+V2_RUN = {"ea3d": dict(R=64, T_min=1 / 3.0, T_max=5.0),
+          "er-sparse": dict(R=32, T_min=0.2, T_max=20.0),
+          "rrg3": dict(R=32, T_min=0.1, T_max=3.0)}
+
+
+def v2_rounds(spec: dict) -> int:
+    """Rounds of the recorded best-known run: about 2e11 attempts, at most 200000 rounds."""
+    return int(min(200_000, 2e11 // (V2_RUN[spec["family"]]["R"] * spec["n"])))
+
+
+def v2_best_known_run(spec: dict, n_rounds: int, seed: int = 1):
+    """The recorded best-known run of a v2 row; returns the PtResult."""
+    from . import gpu
+
+    cfg = V2_RUN[spec["family"]]
+    model = v2_model(spec["family"], spec["size"], spec["seed"])
+    return gpu.parallel_tempering_gpu(model, n_replicas=cfg["R"], T_min=cfg["T_min"],
+                                      T_max=cfg["T_max"], n_rounds=n_rounds, sweeps_per_round=1,
+                                      seed=seed, record_history=False)

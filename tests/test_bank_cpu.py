@@ -66,3 +66,14 @@ def test_enumeration_certifies_published_best_known_rows():
         h = np.asarray(model.h, dtype=np.float64)
         e_min, _, _, _ = oracle.exact(rp, ci, w, h)
         assert e_min == pytest.approx(r["published_energy"], abs=1e-9), r["id"]
+
+
+def test_bank_v2_generators_reproduce_the_recorded_fingerprints():
+    from drift_amd.bank import v2_specs
+
+    rec = {r["id"]: r for r in json.load(open(os.path.join(HERE, "golden/bank_v2.json")))["rows"]}
+    specs = v2_specs()
+    assert len(specs) == len(rec) == 20
+    for s in specs:
+        r = rec[s["id"]]
+        assert (s["n"], s["nnz"], s["fingerprint"]) == (r["n"], r["nnz"], r["fingerprint"])

@@ -53,3 +53,18 @@ def test_planted_bipartite_max_cut_is_optimal(n):
     res = bipartite_check(n)
     print(f"[bank-gpu] bipartite: {res}", flush=True)
     assert res["optimal"], res
+
+
+V2 = json.load(open(os.path.join(HERE, "golden/bank_v2.json")))["rows"]
+
+
+@pytest.mark.parametrize("row", [r for r in V2 if r["seed"] == 1], ids=lambda r: r["id"])
+def test_bank_v2_best_known_energy_replays_exactly(row):
+    """The recorded best-known run of a v2 row replays to the recorded energy and round: the runs
+    are deterministic under the stream contract, whichever engine the automatic choice takes."""
+    from drift_amd.bank import v2_best_known_run
+
+    res = v2_best_known_run(row, row["run"]["n_rounds"], seed=row["run"]["seed"])
+    print(f"[bank-gpu] {row['id']}: {res.best_E} on {res.engine} in {res.seconds:.2f} s", flush=True)
+    assert res.best_E == row["best_known_energy"]
+    assert res.best_round == row["best_round"]
