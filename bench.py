@@ -335,6 +335,128 @@ def bench_ladder(args) -> int:
     return 0
 
 
+C5_FAMILIES = {            # family: (T_min, T_max)
+    "ea3d": (0.3, 3.0),
+    "rrg": (0.1, 3.0),
+    "er": (0.2, 10.0),
+}
+
+
+def c5_instance(family: str, seed: int, n_target: int = 0):
+    """One instance of the C5 mix: 3-D +/-J glasses, 3-regular N(0,1) graphs and sparse ER graphs
+    (mean degree 3..12); N uniform on a log scale over 256..16384, or about n_target when given
+    (the glass then takes the nearest cube)."""
+    from drift_amd.builders.lattice import (cubic_pmj_glass, er_sparse_weighted,
+                                            random_regular_gaussian)
+
+    rng = np.random.default_rng(seed)
+    n = n_target or int(round(np.exp(rng.uniform(np.log(256), np.log(16384)))))
+    if family == "ea3d":
+        return cubic_pmj_glass(max(6, int(round(n ** (1 / 3)))), seed)
+    if family == "rrg":
+        return random_regular_gaussian(2 * (n // 2), seed)
+    return er_sparse_weighted(n, float(rng.uniform(3.0, 12.0)), seed)
+
+
+def bench_ladder_c5(args) -> int:
+    """C5: learned 32-rung ladders at scale, in two stages (each fits one call of a few minutes).
+    --stage train: feedback-optimised labels for --train-per-family instances of each family (N 256..
+    16384), features, network trained by the HIP kernels, saved to --net.
+    --stage eval: the saved network's ladders against geometric ones on --test-per-family held-out
+    instances per family at N ~ --test-n, time to target within --steps sweeps."""
+    from drift_amd import ladder
+
+    R = 32
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)   # noqa: E731
+    t_all = time.perf_counter()
+    if args.stage == "train":
+        feats, labels, sizes = [], [], []
+        for fi, fam in enumerate(C5_FAMILIES):
+            T_min, T_max = C5_FAMILIES[fam]
+            models = [c5_instance(fam, 10_000 * fi + k) for k in range(args.train_per_family)]
+            t0 = time.perf_counter()
+            for c0 in range(0, len(models), 128):        # chunks, so progress shows as it goes
+                chunk = models[c0:c0 + 128]
+                x, _ = ladder.feedback_optimise(chunk, T_min=T_min, T_max=T_max, R=R, iterations=5,
+                                                sweeps=args.label_sweeps, damping=0.5,
+                                                seeds=range(c0, c0 + len(chunk)))
+                labels.append(x)
+                feats.append(ladder.instance_features(chunk, T_min=T_min, T_max=T_max, R=R))
+                log(f"labels {fam}: {c0 + len(chunk)}/{len(models)}, "
+                    f"{time.perf_counter() - t0:.1f} s")
+            sizes += [m.n for m in models]
+            log(f"labels {fam}: {len(models)} instances, N {min(m.n for m in models)}.."
+                f"{max(m.n for m in models)}, {time.perf_counter() - t0:.1f} s")
+        X, Y = np.concatenate(feats), np.concatenate(labels)
+        net = ladder.LadderNet(seed=11)
+        t0 = time.perf_counter()
+        losses = net.fit(X, Y, epochs=args.epochs, lr=3e-3)
+        net.save(args.net)
+        line = {"stage": "train", "instances": int(X.shape[0]), "n_min": int(min(sizes)),
+                "n_max": int(max(sizes)), "label_sweeps": args.label_sweeps,
+                "train_loss": [float(losses[0]), float(losses[-1])],
+                "train_s": round(time.perf_counter() - t0, 1),
+                "total_s": round(time.perf_counter() - t_all, 1), "net": args.net}
+        print(json.dumps(line), flush=True)
+        return 0
+
+    net = ladder.LadderNet(seed=11)
+    net.load(args.net)
+    budget, rows = args.steps, []
+    for fi, fam in enumerate(C5_FAMILIES):
+        T_min, T_max = C5_FAMILIES[fam]
+        models = [c5_instance(fam, 900_000 + 1000 * fi + k, n_target=args.test_n)
+                  for k in range(args.test_per_family)]
+        xs = net.predict(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R))
+        seeds = [4242 + k for k in range(len(models))]
+        kw = dict(n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=budget, sweeps_per_round=1,
+                  seeds=seeds, record_history=True)
+        t0 = time.perf_counter()
+        geo = ladder.run_instances(models, **kw)
+        t1 = time.perf_counter()
+        lrn = ladder.run_instances(
+            models, betas=[ladder.betas_from_positions(x, T_min, T_max) for x in xs], **kw)
+        t2 = time.perf_counter()
+        for m, g, l in zip(models, geo, lrn):
+            target = min(g.best_E, l.best_E)
+            rows.append(dict(family=fam, n=m.n,
+                             tt_geo=ladder.sweeps_to_target(g.best_history, target),
+                             tt_learn=ladder.sweeps_to_target(l.best_history, target),
+                             rt_geo=ladder.round_trip_rate(g, budget),
+                             rt_learn=ladder.round_trip_rate(l, budget),
+                             gap=(g.best_E - l.best_E) / abs(target), engine=g.engine))
+        log(f"held-out {fam}: {len(models)} instances at N {models[0].n}, geometric {t1 - t0:.1f} s, "
+            f"learned {t2 - t1:.1f} s")
+
+    def summary(sel):
+        tg = np.array([r["tt_geo"] or 2 * budget for r in sel], dtype=float)
+        tl = np.array([r["tt_learn"] or 2 * budget for r in sel], dtype=float)
+        return {"instances": len(sel), "reached_geo": int((tg <= budget).sum()),
+                "reached_learned": int((tl <= budget).sum()),
+                "median_sweeps_geo": float(np.median(tg)), "median_sweeps_learned": float(np.median(tl)),
+                "learned_first": int((tl < tg).sum()), "geometric_first": int((tg < tl).sum()),
+                "median_rel_gap_geo_minus_learned": float(np.median([r["gap"] for r in sel])),
+                "round_trip_ratio_median": float(np.median(
+                    [r["rt_learn"] / max(r["rt_geo"], 1e-12) for r in sel]))}
+
+    line = {
+        "metric": "time-to-target energy, learned vs geometric ladder (held-out, large N)",
+        "value": summary(rows)["reached_learned"], "unit": "held-out instances at target",
+        "n_gpus": 1, "steps": budget, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32/f32", "data": "synthetic",
+        "config": {"workload": f"C5: learned 32-rung ladders, held-out N ~ {args.test_n} "
+                               "(3-D +/-J glass at the nearest cube, 3-regular N(0,1), sparse ER)",
+                   "test_per_family": args.test_per_family, "budget_sweeps": budget,
+                   "target": "best energy found by either ladder within the budget",
+                   "net": args.net},
+        "held_out": summary(rows),
+        "per_family": {f: summary([r for r in rows if r["family"] == f]) for f in C5_FAMILIES},
+        "total_s": round(time.perf_counter() - t_all, 1),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def cpu_baseline(model, enc, R, beta, budget_s: float):
     import oracle
 
@@ -370,6 +492,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
     ap.add_argument("--train-per-family", type=int, default=128)
+    ap.add_argument("--stage", default="train", choices=("train", "eval"),
+                    help="ladder-c5 workload: train the network or evaluate a saved one")
+    ap.add_argument("--net", default="ladder_c5_net.npz", help="ladder-c5 network file")
+    ap.add_argument("--test-n", type=int, default=65536, help="ladder-c5 held-out size")
     ap.add_argument("--test-per-family", type=int, default=16)
     ap.add_argument("--label-sweeps", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=3000)
@@ -393,6 +519,8 @@ def main() -> int:
         return bench_c4(args)
     if args.workload == "ladder":
         return bench_ladder(args)
+    if args.workload == "ladder-c5":
+        return bench_ladder_c5(args)
     model, R, beta, desc = workload(args.workload)
     enc = gpu.encode_problem(model)
     kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,

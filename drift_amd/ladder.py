@@ -85,13 +85,45 @@ def feedback_update(x, f, *, damping: float = 0.5, floor: float = 1e-4) -> np.nd
     return np.maximum.accumulate(out)
 
 
+# instances up to this many spin-replica bytes run in the batched launch (one workgroup's LDS
+# each); larger ones run one per launch on the automatic multi-CU engine
+BATCH_LIMIT = 120_000
+
+
+def run_instances(models, *, n_replicas: int, T_min: float, T_max: float, n_rounds: int,
+                  sweeps_per_round: int = 1, seeds=None, betas=None, record_history: bool = False,
+                  record_trace: bool = False):
+    """PtResults of many instances in input order: those small enough in one batched launch, the
+    rest one launch each.  betas: None or one ladder per instance."""
+    from . import gpu
+
+    m = len(models)
+    seeds = list(range(m)) if seeds is None else list(seeds)
+    small = [i for i, x in enumerate(models) if x.n * n_replicas <= BATCH_LIMIT]
+    out = [None] * m
+    kw = dict(n_replicas=n_replicas, T_min=T_min, T_max=T_max, n_rounds=n_rounds,
+              sweeps_per_round=sweeps_per_round, record_history=record_history,
+              record_trace=record_trace)
+    if small:
+        res = gpu.parallel_tempering_gpu_batch(
+            [models[i] for i in small], seeds=[seeds[i] for i in small],
+            betas=None if betas is None else [betas[i] for i in small], **kw)
+        for i, r in zip(small, res):
+            out[i] = r
+    for i in range(m):
+        if out[i] is None:
+            out[i] = gpu.parallel_tempering_gpu(models[i], seed=seeds[i],
+                                                betas=None if betas is None else betas[i], **kw)
+    return out
+
+
 def feedback_optimise(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
                       iterations: int = 4, sweeps: int = 500, seeds=None, damping: float = 0.5):
-    """Feedback-optimised ladders for a list of instances, all iterations batched on the GPU.
+    """Feedback-optimised ladders for a list of instances, each iteration one batched launch for
+    the instances that fit one workgroup (the rest one launch each).
 
     Iteration i runs ``sweeps * 2**i`` sweeps (one exchange attempt per sweep) on every instance
     with its current ladder, then moves the rungs.  Returns (positions (m, R), last results)."""
-    from . import gpu
 
     m = len(models)
     x = np.tile(geometric_positions(R), (m, 1))
@@ -99,9 +131,9 @@ def feedback_optimise(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
     res = None
     for it in range(iterations):
         betas = [betas_from_positions(x[i], T_min, T_max) for i in range(m)]
-        res = gpu.parallel_tempering_gpu_batch(
-            models, n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=sweeps * (1 << it),
-            sweeps_per_round=1, seeds=[s + 7919 * it for s in seeds], betas=betas)
+        res = run_instances(models, n_replicas=R, T_min=T_min, T_max=T_max,
+                            n_rounds=sweeps * (1 << it), sweeps_per_round=1,
+                            seeds=[s + 7919 * it for s in seeds], betas=betas)
         for i, r in enumerate(res):
             x[i] = feedback_update(x[i], up_fraction(r.rung_up, r.rung_down), damping=damping)
     return x, res
@@ -156,13 +188,10 @@ def dynamic_features(res, n: int, T: np.ndarray) -> np.ndarray:
 
 def instance_features(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
                       short_sweeps: int = 200, seeds=None) -> np.ndarray:
-    """The 16 features of each instance (one batched short run for the dynamic part)."""
-    from . import gpu
-
+    """The 16 features of each instance (one short run for the dynamic part)."""
     T = temperatures_from_positions(geometric_positions(R), T_min, T_max)
-    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=T_min, T_max=T_max,
-                                           n_rounds=short_sweeps, sweeps_per_round=1, seeds=seeds,
-                                           record_trace=True)
+    res = run_instances(models, n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=short_sweeps,
+                        sweeps_per_round=1, seeds=seeds, record_trace=True)
     feats = [np.concatenate([structural_features(m, T_min, T_max), dynamic_features(r, m.n, T)])
              for m, r in zip(models, res)]
     return np.array(feats)
