@@ -370,16 +370,22 @@ def bench_ladder_c5(args) -> int:
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)   # noqa: E731
     t_all = time.perf_counter()
     if args.stage == "train":
-        feats, labels, sizes = [], [], []
+        feats, labels, sizes, picks = [], [], [], {}
         for fi, fam in enumerate(C5_FAMILIES):
             T_min, T_max = C5_FAMILIES[fam]
             models = [c5_instance(fam, 10_000 * fi + k) for k in range(args.train_per_family)]
             t0 = time.perf_counter()
             for c0 in range(0, len(models), 128):        # chunks, so progress shows as it goes
                 chunk = models[c0:c0 + 128]
-                x, _ = ladder.feedback_optimise(chunk, T_min=T_min, T_max=T_max, R=R, iterations=5,
-                                                sweeps=args.label_sweeps, damping=0.5,
-                                                seeds=range(c0, c0 + len(chunk)))
+                if args.labels == "budget":
+                    x, pick = ladder.budget_labels(chunk, T_min=T_min, T_max=T_max, R=R,
+                                                   budget=args.steps,
+                                                   seeds=range(c0, c0 + len(chunk)))
+                    picks.setdefault(fam, []).extend(int(k) for k in pick)
+                else:
+                    x, _ = ladder.feedback_optimise(chunk, T_min=T_min, T_max=T_max, R=R,
+                                                    iterations=5, sweeps=args.label_sweeps,
+                                                    damping=0.5, seeds=range(c0, c0 + len(chunk)))
                 labels.append(x)
                 feats.append(ladder.instance_features(chunk, T_min=T_min, T_max=T_max, R=R))
                 log(f"labels {fam}: {c0 + len(chunk)}/{len(models)}, "
@@ -392,9 +398,12 @@ def bench_ladder_c5(args) -> int:
         t0 = time.perf_counter()
         losses = net.fit(X, Y, epochs=args.epochs, lr=3e-3)
         net.save(args.net)
-        line = {"stage": "train", "instances": int(X.shape[0]), "n_min": int(min(sizes)),
-                "n_max": int(max(sizes)), "label_sweeps": args.label_sweeps,
+        line = {"stage": "train", "labels": args.labels, "instances": int(X.shape[0]),
+                "n_min": int(min(sizes)), "n_max": int(max(sizes)),
+                "label_sweeps": args.steps if args.labels == "budget" else args.label_sweeps,
                 "train_loss": [float(losses[0]), float(losses[-1])],
+                "chosen_gamma": {f: {str(g): p.count(k) for k, g in enumerate(ladder.BUDGET_GAMMAS)}
+                                 for f, p in picks.items()},
                 "train_s": round(time.perf_counter() - t0, 1),
                 "total_s": round(time.perf_counter() - t_all, 1), "net": args.net}
         print(json.dumps(line), flush=True)
@@ -408,6 +417,8 @@ def bench_ladder_c5(args) -> int:
         models = [c5_instance(fam, 900_000 + 1000 * fi + k, n_target=args.test_n)
                   for k in range(args.test_per_family)]
         xs = net.predict(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R))
+        log(f"{fam}: predicted rung 8 position {np.mean(xs[:, 8]):.3f} (geometric 0.258), "
+            f"rung 16 {np.mean(xs[:, 16]):.3f} (0.516)")
         seeds = [4242 + k for k in range(len(models))]
         kw = dict(n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=budget, sweeps_per_round=1,
                   seeds=seeds, record_history=True)
@@ -495,6 +506,9 @@ def main() -> int:
     ap.add_argument("--stage", default="train", choices=("train", "eval"),
                     help="ladder-c5 workload: train the network or evaluate a saved one")
     ap.add_argument("--net", default="ladder_c5_net.npz", help="ladder-c5 network file")
+    ap.add_argument("--labels", default="feedback", choices=("feedback", "budget"),
+                    help="ladder-c5 training labels: feedback-optimised ladders, or the best of "
+                         "the ladder shapes at a budget of --steps sweeps")
     ap.add_argument("--test-n", type=int, default=65536, help="ladder-c5 held-out size")
     ap.add_argument("--test-per-family", type=int, default=16)
     ap.add_argument("--label-sweeps", type=int, default=2000)

@@ -297,10 +297,21 @@ __device__ __forceinline__ bool wait_phase(const CoopArgs& C, const uint32_t* p,
     return true;
 }
 
-__device__ __forceinline__ bool coop_sync(const CoopArgs& C, uint32_t& phase, int* sh_abort) {
+// gsum/lsum (optional): an LDS partial sum the waves have added into; thread 0 adds it to the
+// global accumulator and waits for that atomic before the workgroup arrives.
+__device__ __forceinline__ bool coop_sync(const CoopArgs& C, uint32_t& phase, int* sh_abort,
+                                          int64_t* gsum = nullptr, long long* lsum = nullptr) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     ++phase;
+    if (lsum && threadIdx.x == 0) {
+        const long long v = *lsum;
+        *lsum = 0;
+        if (v != 0) {
+            atomicAdd((unsigned long long*)gsum, (unsigned long long)v);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+    }
     if (blockIdx.x == 0) {
         for (int x = 1 + threadIdx.x; x < (int)gridDim.x; x += blockDim.x)
             if (!wait_phase(C, C.flags + x, phase)) *sh_abort = 1;
@@ -312,6 +323,134 @@ __device__ __forceinline__ bool coop_sync(const CoopArgs& C, uint32_t& phase, in
     }
     __syncthreads();
     return *sh_abort == 0;
+}
+
+// Round end of R <= 64 replicas by one wavefront, the ladder held in rung order: lane r keeps
+// the slot at rung r with that slot's energy, round-trip label and trip count, rung r's inverse
+// temperature and rung r's counters.  Exchange partners trade their slot, energy, label and trip
+// count through shuffles, so no lane reads another's data through LDS and the wave needs no
+// workgroup barrier.  The rules are those of block_best_rung, exchange_pairs and update_labels:
+// the same lexicographic best rung, the same pairs, uniforms and acceptance arithmetic, the same
+// label transitions; only where each quantity is kept differs.
+struct WaveLadder {
+    int32_t rep;        // slot at this lane's rung
+    int64_t Eb;         // its energy at the last round end
+    int32_t lab;        // its label (1: last at the cold end, -1: last at the hot end)
+    int32_t trips;      // its completed round trips
+    double beta;        // inverse temperature of this rung
+    int32_t tried, acc, up, down;   // counters of this rung (tried/acc: pair (r, r+1))
+    int64_t bestE;      // best record (uniform over the wave)
+    int32_t best_round, best_rung, best_slot, have;
+};
+
+__device__ __forceinline__ void wave_ladder_init(WaveLadder& W, const PtInstance& I,
+                                                 const int64_t* einit, int lane) {
+    const int R = I.R;
+    W.rep = lane;
+    W.Eb = (einit && lane < R) ? -(einit[lane] / 2) : 0;
+    W.lab = (R > 1 && lane == 0) ? 1 : ((R > 1 && lane == R - 1) ? -1 : 0);
+    W.trips = 0;
+    W.beta = lane < R ? I.beta[lane] : 0.0;
+    W.tried = W.acc = W.up = W.down = 0;
+    W.bestE = 0;
+    W.best_round = W.best_rung = W.best_slot = W.have = 0;
+}
+
+// (einit may be null: the energies then come with the first round end.)
+
+// One round end: E is the energy of the slot at this lane's rung after round rnd - 1 (rnd == 0:
+// the initial energies).  Updates the best record (returns true if it improved), exchanges the
+// pairs of round rnd - 1 and advances the labels; the owner records history and trace.
+__device__ __forceinline__ bool wave_ladder_round(WaveLadder& W, const PtInstance& I, int rnd,
+                                                  int64_t E, int lane, bool owner) {
+    const int R = I.R;
+    long long e = lane < R ? (long long)E : LLONG_MAX;
+    int r = lane < R ? lane : INT_MAX;
+#pragma unroll
+    for (int m = PT_WAVE / 2; m > 0; m >>= 1) {
+        const long long e2 = __shfl_xor(e, m);
+        const int r2 = __shfl_xor(r, m);
+        lex_min(e, r, e2, r2);
+    }
+    const int slot = __shfl(W.rep, r);
+    const bool improved = W.have == 0 || e < W.bestE;
+    if (owner && rnd > 0) {
+        if (lane == 0 && I.hist) I.hist[rnd - 1] = (double)(improved ? e : W.bestE) * I.unit;
+        if (I.trace && lane < R) I.trace[(size_t)(rnd - 1) * R + lane] = (double)E * I.unit;
+    }
+    if (improved) {
+        W.bestE = e;
+        W.best_round = rnd;
+        W.best_rung = r;
+        W.best_slot = slot;
+        W.have = 1;
+    }
+    if (rnd > 0 && R > 1) {
+        const int k = rnd - 1, par = k & 1;
+        const bool lo = lane >= par && ((lane - par) & 1) == 0 && lane + 1 < R;
+        const bool hi = lane >= par + 1 && ((lane - 1 - par) & 1) == 0 && lane < R;
+        const int pr = lo ? lane + 1 : (hi ? lane - 1 : lane);
+        const int rlo = hi ? lane - 1 : lane;
+        const pt_u4 rv = pt_philox((uint32_t)(rlo >> 2), 0u, (uint32_t)k, PT_TAG_EXCH, I.k0, I.k1);
+        const int64_t Ep = __shfl(E, pr);
+        const double bp = __shfl(W.beta, pr);
+        const int32_t repP = __shfl(W.rep, pr), labP = __shfl(W.lab, pr), tripP = __shfl(W.trips, pr);
+        bool ok = false;
+        if (lo || hi) {
+            const int64_t Elo = lo ? E : Ep, Ehi = lo ? Ep : E;
+            const double blo = lo ? W.beta : bp, bhi = lo ? bp : W.beta;
+            ok = pt_accept_exchange(blo, bhi, (double)Elo * I.unit, (double)Ehi * I.unit,
+                                    pt_word(rv, rlo & 3));
+        }
+        if (lo) {
+            ++W.tried;
+            W.acc += ok ? 1 : 0;
+        }
+        if (ok) {
+            W.rep = repP;
+            E = Ep;
+            W.lab = labP;
+            W.trips = tripP;
+        }
+        if (lane == 0) {
+            if (W.lab == -1) ++W.trips;
+            W.lab = 1;
+        } else if (lane == R - 1) {
+            W.lab = -1;
+        }
+        W.up += W.lab == 1 ? 1 : 0;
+        W.down += W.lab == -1 ? 1 : 0;
+    }
+    W.Eb = E;
+    return improved;
+}
+
+// Final bookkeeping of the owner: energies, rungs and labels by slot, the permutation, the best
+// record and the counters, in the layouts the per-block round ends leave.
+__device__ __forceinline__ void wave_ladder_store(const WaveLadder& W, const PtInstance& I,
+                                                  int64_t* Eb, int32_t* rung, int32_t* rep,
+                                                  int32_t* label, int64_t* best, int lane) {
+    const int R = I.R;
+    if (lane < R) {
+        Eb[W.rep] = W.Eb;
+        rung[W.rep] = lane;
+        rep[lane] = W.rep;
+        label[W.rep] = W.lab;
+        pt_out_arr(I.out, R, PT_OUT_TRIPS)[W.rep] = W.trips;
+        pt_out_arr(I.out, R, PT_OUT_UP)[lane] = W.up;
+        pt_out_arr(I.out, R, PT_OUT_DOWN)[lane] = W.down;
+        if (lane + 1 < R) {
+            pt_out_arr(I.out, R, PT_OUT_TRIED)[lane] = W.tried;
+            pt_out_arr(I.out, R, PT_OUT_ACC)[lane] = W.acc;
+        }
+    }
+    if (lane == 0) {
+        best[0] = W.bestE;
+        best[1] = W.best_round;
+        best[2] = W.best_rung;
+        best[3] = W.best_slot;
+        best[4] = W.have;
+    }
 }
 
 }  // namespace pt_dev

@@ -83,6 +83,10 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
         for (int x = tid; x < R; x += nt) E[x] = -(E[x] / 2);
     __syncthreads();
 
+    // R <= 64: the round end is wave 0's (WaveLadder), with one workgroup barrier per round.  The
+    // ladder stays in the LDS arrays between round ends, so no registers are held over the sweeps.
+    const bool fast = R <= PT_WAVE;
+
     for (int rnd = -1; rnd < I.n_rounds; ++rnd) {
         if (rnd >= 0) {
             // --- sweeps of this round --------------------------------------------------------
@@ -118,6 +122,53 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
             __syncthreads();
         }
 
+        if (fast) {
+            if (tid < PT_WAVE) {
+                const int r = tid;
+                WaveLadder W;
+                W.rep = r < R ? rep[r] : r;
+                W.Eb = 0;
+                W.lab = r < R ? label[W.rep] : 0;
+                W.trips = r < R ? trips[W.rep] : 0;
+                W.beta = r < R ? I.beta[r] : 0.0;
+                W.tried = r < R ? (int32_t)tried[r] : 0;
+                W.acc = r < R ? (int32_t)accd[r] : 0;
+                W.up = r < R ? (int32_t)up[r] : 0;
+                W.down = r < R ? (int32_t)down[r] : 0;
+                W.bestE = sc->bestE;
+                W.have = sc->have;
+                W.best_round = sc->bround;
+                W.best_rung = sc->brung;
+                W.best_slot = sc->bslot;
+                const int64_t Ecur = r < R ? E[W.rep] : 0;
+                const bool imp = wave_ladder_round(W, I, rnd + 1, Ecur, r, true);
+                if (r < R) {
+                    rep[r] = W.rep;
+                    rung[W.rep] = r;
+                    label[W.rep] = W.lab;
+                    trips[W.rep] = W.trips;
+                    tried[r] = W.tried;
+                    accd[r] = W.acc;
+                    up[r] = W.up;
+                    down[r] = W.down;
+                }
+                if (r == 0) {
+                    sc->improved = imp;
+                    sc->bestE = W.bestE;
+                    sc->have = W.have;
+                    sc->bround = W.best_round;
+                    sc->brung = W.best_rung;
+                    sc->bslot = W.best_slot;
+                }
+            }
+            __syncthreads();
+            if (sc->improved) {
+                const int slot = sc->bslot;
+                for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
+                __syncthreads();   // read before the next round's sweeps change the spins
+            }
+            continue;
+        }
         // --- best tracking at round granularity -------------------------------------------
         int64_t mE;
         int32_t mR;

@@ -406,7 +406,8 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
     __shared__ long long bestL[5];
     __shared__ double betaL[1024];
     __shared__ int32_t labelL[1024];   // round-trip labels (workgroup 0's copy is the one used)
-    __shared__ int sh_abort;
+    __shared__ int sh_abort, sh_imp, sh_slot;
+    __shared__ long long sh_dsum;
     int a, b;
     wg_slot_slab(blockIdx.x, S.NB, a, b);
     const bool live = a < I.R, owner = blockIdx.x == 0;
@@ -446,13 +447,53 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
     }
     if (tid == 0) {
         sh_abort = 0;
+        sh_dsum = 0;
         for (int k = 0; k < 5; ++k) bestL[k] = 0;
     }
+    // R <= 64: the round end is wave 0's (WaveLadder); the other waves meanwhile bring in the
+    // halo planes of the round's last sweep
+    const bool fast = R <= PT_WAVE;
+    const bool defer = fast && !WRAP && nt > PT_WAVE && I.S > 0;
+    WaveLadder WL;
+    if (fast && tid < PT_WAVE) wave_ladder_init(WL, I, S.einit, tid);
     __syncthreads();
 
     uint32_t phase = 0;
     for (int rnd = 0;; ++rnd) {
         // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
+        if (fast) {
+            if (tid < PT_WAVE) {
+                int64_t E = WL.Eb;
+                if (rnd > 0 && tid < R)
+                    E += ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + WL.rep);
+                const bool imp = wave_ladder_round(WL, I, rnd, E, tid, owner);
+                if (tid < R) rungs[WL.rep] = tid;
+                if (tid == 0) {
+                    sh_imp = imp;
+                    sh_slot = WL.best_slot;
+                }
+                if (owner && tid < R)   // buffer of round rnd + 1 (see below)
+                    __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + tid, (int64_t)0,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (defer && live && rnd > 0 && rnd < I.n_rounds) {
+                const int8_t* prv = S.sp + ((size_t)(((uint32_t)rnd * (uint32_t)I.S) & 1) * Rp + a) * g.N;
+                for (int idx = tid - PT_WAVE; idx < 4 * wpp; idx += nt - PT_WAVE) {
+                    const int k = idx >> (Pb - 2), w = idx & (wpp - 1);
+                    const int sl = k < 2 ? k : T + k;
+                    const int plane = (z0 - 2 + sl) & (L - 1);
+                    ((uint32_t*)lsp)[(sl << (Pb - 2)) + w] =
+                        ld_u32((const uint32_t*)(prv + ((size_t)plane << Pb)) + w);
+                }
+            }
+            __syncthreads();
+            if (sh_imp) {
+                if (live && sh_slot == a)
+                    for (int site = z0 * P + tid; site < z1 * P; site += nt)
+                        st_spin<true>(I.best_s + pos_of<DIM>(site, g), lsp[(size_t)so * P + site - z0 * P]);
+                __syncthreads();   // the spins are read above before the sweep changes them
+            }
+            if (rnd == I.n_rounds) break;
+        } else {
         for (int x = tid; x < R; x += nt) {
             if (rnd == 0) {
                 Es[x] = -(S.einit[x] / 2);
@@ -504,6 +545,7 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (rnd == I.n_rounds) break;
+        }
 
         // ---- the sweeps of round rnd --------------------------------------------------------
         const int gr = live ? rungs[a] : 0;
@@ -523,21 +565,18 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
                            ((const uint32_t*)lsp)[((so + o) << (Pb - 2)) + w]);
                 }
             }
-            if (j == I.S - 1) {
+            const bool last = j == I.S - 1;
+            if (last) {   // the slot's energy change: waves into LDS, thread 0 to HBM on arrival
                 long long v = (long long)dsum;
 #pragma unroll
                 for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
-                if ((tid & (PT_WAVE - 1)) == 0) scrE[tid / PT_WAVE] = v;
-                __syncthreads();
-                if (tid == 0 && live) {
-                    long long tot = 0;
-                    for (int wv = 0; wv < (nt + PT_WAVE - 1) / PT_WAVE; ++wv) tot += scrE[wv];
-                    if (tot != 0)
-                        atomicAdd((unsigned long long*)&S.dacc[(size_t)(rnd % 3) * Rp + a],
-                                  (unsigned long long)tot);
-                }
+                if ((tid & (PT_WAVE - 1)) == 0 && v != 0 && live)
+                    atomicAdd((unsigned long long*)&sh_dsum, (unsigned long long)v);
             }
-            if (!coop_sync(C, phase, &sh_abort)) return;
+            if (!coop_sync(C, phase, &sh_abort, S.dacc + (size_t)(rnd % 3) * Rp + a,
+                           last && live ? &sh_dsum : nullptr))
+                return;
+            if (last && defer) continue;   // halo planes come in during the round end
             if (live && !WRAP) {
                 // halo slots 0, 1, T + 2, T + 3 as the neighbours left them after this sweep
                 for (int idx = tid; idx < 4 * wpp; idx += nt) {
@@ -560,7 +599,12 @@ __global__ __launch_bounds__(1024) void pslab_run(PtInstance I, SState S, CoopAr
             ((uint32_t*)(fb + ((size_t)(z0 + o) << Pb)))[w] = ((const uint32_t*)lsp)[((so + o) << (Pb - 2)) + w];
         }
     }
-    if (owner) {
+    if (owner && fast) {
+        const int fin = I.n_rounds & 1;
+        if (tid < PT_WAVE)
+            wave_ladder_store(WL, I, S.Eb + (size_t)fin * Rp, S.rung + (size_t)fin * Rp,
+                              S.rep + (size_t)fin * Rp, S.label, S.best + 5 * fin, tid);
+    } else if (owner) {
         const int fin = I.n_rounds & 1;
         for (int x = tid; x < R; x += nt) {
             S.Eb[(size_t)fin * Rp + x] = EbL[x];
@@ -637,7 +681,8 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
     __shared__ long long bestL[5];
     __shared__ double betaL[1024];
     __shared__ int32_t labelL[1024];
-    __shared__ int sh_abort;
+    __shared__ int sh_abort, sh_imp, sh_slot;
+    __shared__ long long sh_dsum;
     constexpr int D = 2 * DIM;
     int a, b;
     wg_slot_slab(blockIdx.x, S.NB, a, b);
@@ -684,14 +729,58 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
     }
     if (tid == 0) {
         sh_abort = 0;
+        sh_dsum = 0;
         for (int k = 0; k < 5; ++k) bestL[k] = 0;
     }
+    // R <= 64: the round end is wave 0's (WaveLadder); the other waves meanwhile bring in the
+    // halo planes of the round's last sweep
+    const bool fast = R <= PT_WAVE;
+    const bool defer = fast && !WRAP && nt > PT_WAVE && I.S > 0;
+    WaveLadder WL;
+    if (fast && tid < PT_WAVE) wave_ladder_init(WL, I, S.einit, tid);
     __syncthreads();
 
     uint32_t phase = 0;
     const int segb = Lb - 3;                              // log2 segments of 8 sites per row
     for (int rnd = 0;; ++rnd) {
         // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
+        if (fast) {
+            if (tid < PT_WAVE) {
+                int64_t E = WL.Eb;
+                if (rnd > 0 && tid < R)
+                    E += ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + WL.rep);
+                const bool imp = wave_ladder_round(WL, I, rnd, E, tid, owner);
+                if (tid < R) rungs[WL.rep] = tid;
+                if (tid == 0) {
+                    sh_imp = imp;
+                    sh_slot = WL.best_slot;
+                }
+                if (owner && tid < R)   // buffer of round rnd + 1 (see below)
+                    __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + tid, (int64_t)0,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (defer && live && rnd > 0 && rnd < I.n_rounds) {
+                const uint32_t* prv = hb + ((size_t)(((uint32_t)rnd * (uint32_t)I.S) & 1) * Rp + a) * hrep;
+                for (int idx = tid - PT_WAVE; idx < 4 * WPP; idx += nt - PT_WAVE) {
+                    const int kk = idx >> wppb, w = idx & (WPP - 1);
+                    const int sl = kk < 2 ? kk : T + kk;
+                    const int plane = (z0 - 2 + sl) & (L - 1);
+                    sb[(size_t)sl * WPP + w] = ld_u32(prv + (size_t)plane * WPP + w);
+                }
+            }
+            __syncthreads();
+            if (sh_imp) {
+                const int slot = sh_slot;
+                if (live && slot == a)
+                    for (int site = z0 * P + tid; site < z1 * P; site += nt) {
+                        const int ls = site - z0 * P;
+                        const uint32_t wd = sb[(size_t)so * WPP + (ls >> 5)];
+                        st_spin<true>(I.best_s + pos_of<DIM>(site, g),
+                                      ((wd >> (ls & 31)) & 1u) ? (int8_t)-1 : (int8_t)1);
+                    }
+                __syncthreads();   // the spins are read above before the sweep changes them
+            }
+            if (rnd == I.n_rounds) break;
+        } else {
         for (int x = tid; x < R; x += nt) {
             if (rnd == 0) {
                 Es[x] = -(S.einit[x] / 2);
@@ -744,6 +833,7 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (rnd == I.n_rounds) break;
+        }
 
         // ---- the sweeps of round rnd --------------------------------------------------------
         const int gr = live ? rungs[a] : 0;
@@ -834,22 +924,19 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                     st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
                 }
             }
-            if (j == I.S - 1) {
+            const bool last = j == I.S - 1;
+            if (last) {   // the slot's energy change: waves into LDS, thread 0 to HBM on arrival
                 long long v = (long long)dsum;
 #pragma unroll
                 for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
-                if ((tid & (PT_WAVE - 1)) == 0) scrE[tid / PT_WAVE] = v;
-                __syncthreads();
-                if (tid == 0 && live) {
-                    long long tot = 0;
-                    for (int wv = 0; wv < (nt + PT_WAVE - 1) / PT_WAVE; ++wv) tot += scrE[wv];
-                    if (tot != 0)
-                        atomicAdd((unsigned long long*)&S.dacc[(size_t)(rnd % 3) * Rp + a],
-                                  (unsigned long long)tot);
-                }
+                if ((tid & (PT_WAVE - 1)) == 0 && v != 0 && live)
+                    atomicAdd((unsigned long long*)&sh_dsum, (unsigned long long)v);
                 dsum = 0;
             }
-            if (!coop_sync(C, phase, &sh_abort)) return;
+            if (!coop_sync(C, phase, &sh_abort, S.dacc + (size_t)(rnd % 3) * Rp + a,
+                           last && live ? &sh_dsum : nullptr))
+                return;
+            if (last && defer) continue;   // halo planes come in during the round end
             if (live && !WRAP) {
                 for (int idx = tid; idx < 4 * WPP; idx += nt) {
                     const int kk = idx >> wppb, w = idx & (WPP - 1);
@@ -870,7 +957,12 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
             bits_row(sb[(size_t)(so + o) * WPP + rw], fb + (size_t)(z0 + o) * P + ((size_t)rw << 5));
         }
     }
-    if (owner) {
+    if (owner && fast) {
+        const int fin = I.n_rounds & 1;
+        if (tid < PT_WAVE)
+            wave_ladder_store(WL, I, S.Eb + (size_t)fin * Rp, S.rung + (size_t)fin * Rp,
+                              S.rep + (size_t)fin * Rp, S.label, S.best + 5 * fin, tid);
+    } else if (owner) {
         const int fin = I.n_rounds & 1;
         for (int x = tid; x < R; x += nt) {
             S.Eb[(size_t)fin * Rp + x] = EbL[x];

@@ -139,6 +139,37 @@ def feedback_optimise(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
     return x, res
 
 
+def power_positions(gamma: float, R: int = N_RUNGS) -> np.ndarray:
+    """Rung positions x_j = (j / (R - 1))^gamma on the log-temperature axis: gamma = 1 is the
+    geometric ladder, gamma > 1 packs rungs towards the cold end."""
+    return (np.arange(R) / (R - 1)) ** float(gamma)
+
+
+BUDGET_GAMMAS = (0.5, 0.7, 1.0, 1.4, 2.0)
+
+
+def budget_labels(models, *, T_min: float, T_max: float, R: int = N_RUNGS, budget: int = 20000,
+                  gammas=BUDGET_GAMMAS, seeds=None):
+    """Time-to-target labels: every instance runs each ladder shape power_positions(gamma) for
+    ``budget`` sweeps on the same streams; its label is the shape with the lowest best energy,
+    the one reaching that energy first on ties.  Returns (positions (m, R), chosen gamma index)."""
+    m = len(models)
+    seeds = list(range(m)) if seeds is None else list(seeds)
+    E = np.zeros((len(gammas), m))
+    first = np.zeros((len(gammas), m))
+    for g, gamma in enumerate(gammas):
+        b = betas_from_positions(power_positions(gamma, R), T_min, T_max)
+        res = run_instances(models, n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=budget,
+                            sweeps_per_round=1, seeds=seeds, betas=[b] * m, record_history=True)
+        for i, r in enumerate(res):
+            E[g, i] = r.best_E
+            first[g, i] = r.best_round
+    # lexicographic: lowest energy, then earliest round
+    order = np.lexsort((first, E), axis=0)[0]
+    x = np.array([power_positions(gammas[k], R) for k in order])
+    return x, order
+
+
 def round_trip_rate(res, sweeps: int) -> float:
     """Round trips per replica per sweep of a result."""
     return float(np.sum(res.round_trips)) / (len(res.round_trips) * max(sweeps, 1))

@@ -176,7 +176,7 @@ int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* 
                        int32_t n_instances, drift_pt_result* results);
 
 /* Autotuner.  Times every engine and block size that can run this instance (LDS, slab, lattice,
- * global, coop; 256 / 512 / 1024 threads) for trial_rounds rounds (0: about 2.5e8 attempts),
+ * global, coop, pslab; 256 / 512 / 1024 threads) for trial_rounds rounds (0: about 2.5e8 attempts),
  * stores the fastest in a process-wide cache keyed by the instance's shape class (weight type,
  * replica slots, lattice shape, size and degree buckets, sweeps per round), and reports every
  * candidate's time.  All candidates run the same Markov chain, so the choice changes speed only.
