@@ -957,12 +957,20 @@ int slab_count(const HostInst& H) {
     return NB;
 }
 
+// power-of-two side (the kernels index by shifts) with at least one quad per plane and colour
+bool slab_shape(const HostInst& H) {
+    return H.ldim && !(H.lL & (H.lL - 1)) && (H.ldim == 2 ? H.lL >= 8 : H.lL >= 4);
+}
+
 bool slab_fits(const HostInst& H) {
-    // power-of-two side (the kernel indexes by shifts) with at least one quad per plane and colour
-    if (!H.ldim || (H.lL & (H.lL - 1)) || (H.ldim == 2 ? H.lL < 8 : H.lL < 4)) return false;
+    if (!slab_shape(H)) return false;
     const int NB = slab_count(H);
     return pt_slab_lds_bytes(H.n, H.lL, H.ldim, NB) + 32 * 1024 <= (size_t)device_lds_limit();
 }
+
+// The persistent slab engine keeps slabs as bits when L is a multiple of 32 (an eighth of the
+// byte slab's LDS), else as bytes; whether the grid is co-resident is pslab_block's question.
+bool pslab_fits(const HostInst& H) { return slab_shape(H) && (pt_pslab_bits(H.lL) || slab_fits(H)); }
 
 // Results of the slab engines: statistics, best record, final energies and rungs from the
 // buffers of parity n_rounds & 1, final spins from spin buffer (n_rounds * S) & 1.
@@ -1191,8 +1199,10 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
         return run_lds(hs, {res}, block);
     }
     case DRIFT_PT_ENGINE_PSLAB:
-        if (!slab_fits(H))
-            return fail(DRIFT_PT_EINVAL, "pslab engine needs what the slab engine needs");
+        if (!pslab_fits(H))
+            return fail(DRIFT_PT_EINVAL, "pslab engine needs a periodic L^2/L^3 lattice (L a power of "
+                                         "two) with +/-1 couplings, no field, the parity colouring, "
+                                         "and slabs that fit LDS");
         return run_pslab(H, res, block);
     case DRIFT_PT_ENGINE_COOP: {
         CoopPlan P;
@@ -1216,10 +1226,11 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
 int default_engine(const HostInst& H) {
     const int64_t per_sweep = (int64_t)H.R * H.n;
     if (fits_lds(H) && per_sweep <= 8192) return DRIFT_PT_ENGINE_LDS;
-    if (H.ldim && slab_fits(H)) {
+    if (H.ldim && pslab_fits(H)) {
         size_t lds = 0;
-        return pslab_block(H, 0, lds) ? DRIFT_PT_ENGINE_PSLAB : DRIFT_PT_ENGINE_SLAB;
+        if (pslab_block(H, 0, lds)) return DRIFT_PT_ENGINE_PSLAB;
     }
+    if (H.ldim && slab_fits(H)) return DRIFT_PT_ENGINE_SLAB;
     CoopPlan P;
     if (per_sweep <= (1 << 17) && coop_plan(H, 0, P)) return DRIFT_PT_ENGINE_COOP;
     return H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
@@ -1251,7 +1262,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
         if (fits_lds(H)) cands.emplace_back(DRIFT_PT_ENGINE_LDS, b);
         if (H.ldim && slab_fits(H)) cands.emplace_back(DRIFT_PT_ENGINE_SLAB, b);
         size_t plds = 0;
-        if (H.ldim && slab_fits(H) && pslab_block(H, b, plds) == b)
+        if (H.ldim && pslab_fits(H) && pslab_block(H, b, plds) == b)
             cands.emplace_back(DRIFT_PT_ENGINE_PSLAB, b);
         if (H.ldim) cands.emplace_back(DRIFT_PT_ENGINE_LATTICE, b);
         cands.emplace_back(DRIFT_PT_ENGINE_GLOBAL, b);

@@ -742,6 +742,10 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
 
     uint32_t phase = 0;
     const int segb = Lb - 3;                              // log2 segments of 8 sites per row
+    const int rb = DIM == 3 ? Lb : 0;                     // log2 rows per plane
+    const int wb = Lb - 5;                                // log2 words per row
+    const int pb = DIM == 3 ? 2 * Lb : Lb;                // log2 sites per plane
+    const int jstride = njs << wppb;                      // words per bond direction
     for (int rnd = 0;; ++rnd) {
         // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
         if (fast) {
@@ -841,40 +845,61 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
         int64_t dsum = 0;
         for (int j = 0; j < I.S; ++j) {
             const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
+            uint32_t* nxt = hb + ((size_t)((t + 1) & 1) * Rp + a) * hrep;
+            // the first and last min(2, T) owned planes: the neighbours' halos
+            auto publish = [&]() {
+                const int nb = T < 2 ? T : 2;
+                for (int idx = tid; idx < 2 * nb * WPP; idx += nt) {
+                    const int kk = idx >> wppb, w = idx & (WPP - 1);
+                    const int o = kk < nb ? kk : T - 2 * nb + kk;
+                    st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
+                }
+            };
+            // With more than four owned planes, colour 1 runs in two passes: the two first and
+            // two last owned planes, which are then published while pass 2 sweeps the interior.
+            const bool split = !WRAP && T > 4;
             if (live) {
-                for (int c = 0; c < 2; ++c) {
-                    const int s_lo = WRAP ? 0 : (c == 0 ? 1 : 2);
-                    const int ns = WRAP ? L : (c == 0 ? T + 2 : T);
+                for (int c = 0; c < (split ? 3 : 2); ++c) {
+                    const int col = c < 2 ? c : 1;
+                    const int ns = WRAP ? L : (c == 0 ? T + 2 : (!split ? T : (c == 1 ? 4 : T - 4)));
                     const int psb = (DIM == 3 ? Lb : 0) + segb;   // log2 segments per slot
                     for (int w = tid; w < (ns << psb); w += nt) {
-                        const int s = s_lo + (w >> psb), rem = w & ((1 << psb) - 1);
+                        const int i = w >> psb, rem = w & ((1 << psb) - 1);
+                        const int s = WRAP ? i
+                                           : (c == 0 ? 1 + i
+                                                     : (!split ? 2 + i
+                                                               : (c == 1 ? (i < 2 ? 2 + i : T + i - 2)
+                                                                         : 4 + i)));
                         const int r = rem >> segb, k = rem & ((1 << segb) - 1);
                         const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
                         const bool owned = WRAP || (s >= 2 && s < T + 2);
                         const int su = WRAP ? ((s + 1) & (L - 1)) : s + 1;
                         const int sd = WRAP ? ((s - 1) & (L - 1)) : s - 1;
                         const int wi = k >> 2;                      // word of the row
-                        uint32_t* rowp = sb + ((size_t)s * rows + r) * WPR;
-                        const uint32_t W = rowp[wi];
-                        const uint32_t Wn = rowp[(wi + 1) & (WPR - 1)];
-                        const uint32_t Wp = rowp[(wi - 1) & (WPR - 1)];
+                        // word index of (slot, row) + wi, by shifts: rows = 2^rb, words/row = 2^wb
+                        const int wrow = (((s << rb) + r) << wb) + wi;
+                        const uint32_t W = sb[wrow];
+                        const uint32_t Wn = sb[wrow - wi + ((wi + 1) & (WPR - 1))];
+                        const uint32_t Wp = sb[wrow - wi + ((wi - 1) & (WPR - 1))];
                         uint32_t A[6];
                         A[0] = (W >> 1) | (Wn << 31);               // +x
                         A[1] = (W << 1) | (Wp >> 31);               // -x
                         if (DIM == 3) {
-                            A[2] = sb[((size_t)s * rows + ((r + 1) & (L - 1))) * WPR + wi];
-                            A[3] = sb[((size_t)s * rows + ((r - 1) & (L - 1))) * WPR + wi];
-                            A[4] = sb[((size_t)su * rows + r) * WPR + wi];
-                            A[5] = sb[((size_t)sd * rows + r) * WPR + wi];
+                            A[2] = sb[(((s << rb) + ((r + 1) & (L - 1))) << wb) + wi];
+                            A[3] = sb[(((s << rb) + ((r - 1) & (L - 1))) << wb) + wi];
+                            A[4] = sb[(((su << rb) + r) << wb) + wi];
+                            A[5] = sb[(((sd << rb) + r) << wb) + wi];
                         } else {
-                            A[2] = sb[(size_t)su * WPR + wi];
-                            A[3] = sb[(size_t)sd * WPR + wi];
+                            A[2] = sb[(su << wb) + wi];
+                            A[3] = sb[(sd << wb) + wi];
                         }
-                        const size_t jrow = ((size_t)(s - jlo) * rows + r) * WPR + wi;
+                        const int jrow = ((((s - jlo) << rb) + r) << wb) + wi;
+                        // Z[d] = 1 where the bond to neighbour d is unsatisfied (s_i J s_j = -1):
+                        // neighbour bit ^ bond bit ^ own bit
                         uint32_t Y[6];
 #pragma unroll
-                        for (int d = 0; d < D; ++d) Y[d] = A[d] ^ jb[(size_t)d * njs * WPP + jrow];
-                        // unsatisfied-term count of every site: bits c0, c1, c2
+                        for (int d = 0; d < D; ++d) Y[d] = A[d] ^ jb[d * jstride + jrow] ^ W;
+                        // unsatisfied-bond count m of every site: bits c0, c1, c2
                         uint32_t c0, c1, c2;
                         if (DIM == 3) {
                             uint32_t s1, k1, s2, k2;
@@ -891,39 +916,37 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                             c2 = k1 & k0;
                         }
                         const int x0 = k << 3;                      // first site of the segment
-                        const int par = DIM == 3 ? ((c + r + plane) & 1) : ((c + plane) & 1);
-                        const uint32_t q = (uint32_t)((c * g.half) >> 2) +
-                                           (uint32_t)(((size_t)plane * P + (size_t)r * L + x0) >> 3);
+                        const int par = DIM == 3 ? ((col + r + plane) & 1) : ((col + plane) & 1);
+                        const uint32_t q = ((uint32_t)(col * g.half) >> 2) +
+                                           ((uint32_t)((plane << pb) + (r << Lb) + x0) >> 3);
                         const pt_u4 rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                        // q = D - 2m <= 0 always flips: m >= 3 in 3-D, m >= 2 in 2-D.  Otherwise
+                        // the threshold of q: 3-D m = 2, 1, 0 -> t2, t4, t6; 2-D m = 1, 0 -> t2, t4.
+                        const uint32_t always = DIM == 3 ? (c2 | (c1 & c0)) : (c2 | c1);
+                        const int b0 = (x0 & 31) + par;
                         uint32_t flip = 0;
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
-                            const int bx = (x0 + 2 * i + par) & 31;
-                            const int cnt = (int)(((c0 >> bx) & 1u) | (((c1 >> bx) & 1u) << 1) |
-                                                  (((c2 >> bx) & 1u) << 2));
-                            const int m = ((W >> bx) & 1u) ? D - cnt : cnt;
-                            const int qv = D - 2 * m;
-                            const uint32_t th = qv == 2 ? t2 : (qv == 4 ? t4 : t6);
-                            if (qv <= 0 || pt_word(rv, i) < th) {
-                                flip |= 1u << bx;
-                                if (owned) dsum += 2 * qv;
-                            }
+                            const uint32_t bit = 1u << (b0 + 2 * i);
+                            const uint32_t th = DIM == 3 ? ((c1 & bit) ? t2 : ((c0 & bit) ? t4 : t6))
+                                                         : ((c0 & bit) ? t2 : t4);
+                            flip |= (always & bit) | (pt_word(rv, i) < th ? bit : 0u);
                         }
-                        if (flip) atomicXor(rowp + wi, flip);
+                        atomicXor(sb + wrow, flip);
+                        // energy change of the flips: sum of 2q = 2(D - 2m) over the flipped sites
+                        // (in signed arithmetic: __popc returns unsigned)
+                        if (owned) {
+                            const int nf = (int)__popc(flip);
+                            const int sm = (int)__popc(flip & c0) + 2 * (int)__popc(flip & c1) +
+                                           4 * (int)__popc(flip & c2);
+                            dsum += 2 * (D * nf - 2 * sm);
+                        }
                     }
                     __syncthreads();
+                    if (split && c == 1) publish();
                 }
             }
-            uint32_t* nxt = hb + ((size_t)((t + 1) & 1) * Rp + a) * hrep;
-            if (live && !WRAP) {
-                // the first and last min(2, T) owned planes: the neighbours' halos
-                const int nb = T < 2 ? T : 2;
-                for (int idx = tid; idx < 2 * nb * WPP; idx += nt) {
-                    const int kk = idx >> wppb, w = idx & (WPP - 1);
-                    const int o = kk < nb ? kk : T - 2 * nb + kk;
-                    st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
-                }
-            }
+            if (live && !WRAP && !split) publish();
             const bool last = j == I.S - 1;
             if (last) {   // the slot's energy change: waves into LDS, thread 0 to HBM on arrival
                 long long v = (long long)dsum;

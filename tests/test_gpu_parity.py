@@ -272,10 +272,13 @@ def test_autotune_times_candidates_and_keeps_results():
     assert np.array_equal(g.final_spins, o["final_spins"])
 
 
-@pytest.mark.parametrize("case", [(2, 64, 12, 2), (3, 64, 8, 2), (3, 32, 3, 3)])
+@pytest.mark.parametrize("case", [(2, 64, 12, 2), (3, 64, 8, 2), (3, 32, 3, 3), (3, 32, 48, 1),
+                                  (3, 32, 80, 1)])
 def test_pslab_bit_sliced_rows(case):
     """The bit-sliced persistent slab engine on rows of one and two 32-site words (2-D and 3-D),
-    with a replica count that is not a multiple of 8 and several sweeps per round."""
+    with a replica count that is not a multiple of 8 and several sweeps per round; slabs of five
+    or six planes (R = 48) and of eight (R = 80, round end by the whole workgroup), where the
+    boundary planes are swept and published before the interior."""
     dim, L, R, S = case
     _announce(f"pslab bits dim={dim} L={L} R={R} S={S}")
     model = square_ferromagnet(L) if dim == 2 else cubic_pmj_glass(L, 7)
