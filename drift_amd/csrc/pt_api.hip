@@ -1097,25 +1097,45 @@ int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
 }
 
 // Block size of the persistent slab engine for this instance, or 0 when its grid cannot be
-// co-resident.
-int pslab_block(const HostInst& H, int block_req, size_t& lds) {
+// co-resident.  upre: the grid also fits with the bit-sliced kernel's buffer of next-sweep
+// uniforms (taken when it does; it needs more than one wave per workgroup).
+int pslab_block(const HostInst& H, int block_req, size_t& lds, bool* upre = nullptr) {
     int dev = 0, coop = 0, cus = 0;
+    if (upre) *upre = false;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
     const int NB = slab_count(H);
-    lds = pt_pslab_lds_bytes(H.lL, H.ldim, NB);
+    const size_t lds0 = pt_pslab_lds_bytes(H.lL, H.ldim, NB);
+    const size_t lds1 = pt_pslab_bits(H.lL) ? pt_pslab_lds_bytes(H.lL, H.ldim, NB, true) : 0;
     const int64_t W = (int64_t)((H.R + 7) / 8) * 8 * NB;
+    auto fits = [&](int block, size_t bytes) -> int {   // 1 fits, 0 does not, -1 query failed
+        if (bytes > (size_t)device_lds_limit()) return 0;
+        int per_cu = 0;
+        if (pt_pslab_occupancy(H.ldim, NB, H.lL, block, bytes, &per_cu) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1;
+        }
+        return (int64_t)per_cu * cus >= W ? 1 : 0;
+    };
     // an explicit block size is taken or refused; otherwise the largest that fits
     const int first = block_req > 0 ? std::max(64, std::min(1024, (block_req / 64) * 64)) : 512;
     for (int block = first; block >= 64; block /= 2) {
-        int per_cu = 0;
-        if (pt_pslab_occupancy(H.ldim, NB, H.lL, block, lds, &per_cu) != hipSuccess) {
-            (void)hipGetLastError();
-            return 0;
+        if (lds1 && block > 64 && upre) {
+            const int f = fits(block, lds1);
+            if (f == 1) {
+                lds = lds1;
+                *upre = true;
+                return block;
+            }
         }
-        if ((int64_t)per_cu * cus >= W) return block;
+        const int f = fits(block, lds0);
+        if (f < 0) return 0;
+        if (f == 1) {
+            lds = lds0;
+            return block;
+        }
         if (block_req > 0) break;
     }
     return 0;
@@ -1123,7 +1143,8 @@ int pslab_block(const HostInst& H, int block_req, size_t& lds) {
 
 int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     size_t lds = 0;
-    const int block = pslab_block(H, block_req, lds);
+    bool upre = false;
+    const int block = pslab_block(H, block_req, lds, &upre);
     if (!block) return fail(DRIFT_PT_EINVAL, "pslab engine: the slab grid cannot be co-resident");
     DevArena A;
     PtInstance I;
@@ -1137,6 +1158,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     rc = slab_setup(A, H, I, S, sg.s);
     if (rc) return rc;
     S.hbits = nullptr;
+    S.upre = upre ? 1 : 0;
     if (pt_pslab_bits(H.lL)) {
         const size_t words = 2 * (size_t)H.Rp * (size_t)H.n / 32;
         PT_HIP(A.alloc(&S.hbits, words));

@@ -361,16 +361,20 @@ __device__ __forceinline__ void wave_ladder_init(WaveLadder& W, const PtInstance
 // One round end: E is the energy of the slot at this lane's rung after round rnd - 1 (rnd == 0:
 // the initial energies).  Updates the best record (returns true if it improved), exchanges the
 // pairs of round rnd - 1 and advances the labels; the owner records history and trace.
+// uex (optional): the exchange uniforms of round rnd - 1 by lower rung, drawn ahead of time.
 __device__ __forceinline__ bool wave_ladder_round(WaveLadder& W, const PtInstance& I, int rnd,
-                                                  int64_t E, int lane, bool owner) {
+                                                  int64_t E, int lane, bool owner,
+                                                  const uint32_t* uex = nullptr) {
     const int R = I.R;
     long long e = lane < R ? (long long)E : LLONG_MAX;
     int r = lane < R ? lane : INT_MAX;
 #pragma unroll
-    for (int m = PT_WAVE / 2; m > 0; m >>= 1) {
+    for (int m = PT_WAVE / 2; m > 0; m >>= 1) {   // branch-free lexicographic minimum
         const long long e2 = __shfl_xor(e, m);
         const int r2 = __shfl_xor(r, m);
-        lex_min(e, r, e2, r2);
+        const bool take = (e2 < e) | ((e2 == e) & (r2 < r));
+        e = take ? e2 : e;
+        r = take ? r2 : r;
     }
     const int slot = __shfl(W.rep, r);
     const bool improved = W.have == 0 || e < W.bestE;
@@ -391,7 +395,13 @@ __device__ __forceinline__ bool wave_ladder_round(WaveLadder& W, const PtInstanc
         const bool hi = lane >= par + 1 && ((lane - 1 - par) & 1) == 0 && lane < R;
         const int pr = lo ? lane + 1 : (hi ? lane - 1 : lane);
         const int rlo = hi ? lane - 1 : lane;
-        const pt_u4 rv = pt_philox((uint32_t)(rlo >> 2), 0u, (uint32_t)k, PT_TAG_EXCH, I.k0, I.k1);
+        uint32_t u;
+        if (uex) {
+            u = uex[rlo < R ? rlo : 0];
+        } else {
+            const pt_u4 rv = pt_philox((uint32_t)(rlo >> 2), 0u, (uint32_t)k, PT_TAG_EXCH, I.k0, I.k1);
+            u = pt_word(rv, rlo & 3);
+        }
         const int64_t Ep = __shfl(E, pr);
         const double bp = __shfl(W.beta, pr);
         const int32_t repP = __shfl(W.rep, pr), labP = __shfl(W.lab, pr), tripP = __shfl(W.trips, pr);
@@ -399,8 +409,7 @@ __device__ __forceinline__ bool wave_ladder_round(WaveLadder& W, const PtInstanc
         if (lo || hi) {
             const int64_t Elo = lo ? E : Ep, Ehi = lo ? Ep : E;
             const double blo = lo ? W.beta : bp, bhi = lo ? bp : W.beta;
-            ok = pt_accept_exchange(blo, bhi, (double)Elo * I.unit, (double)Ehi * I.unit,
-                                    pt_word(rv, rlo & 3));
+            ok = pt_accept_exchange(blo, bhi, (double)Elo * I.unit, (double)Ehi * I.unit, u);
         }
         if (lo) {
             ++W.tried;

@@ -79,6 +79,8 @@ struct SState {
     uint32_t* hbits;   // bit-sliced persistent engine: [sweep parity][slot][plane][word] spin bits
                        // of the planes workgroups hand to their neighbours (1 = spin -1)
     int32_t NB;        // slabs per replica (1: the whole lattice wraps inside one workgroup)
+    int32_t upre;      // bit-sliced persistent engine: the next sweep's Metropolis uniforms are
+                       // drawn into LDS while the grid barrier and the round end are in progress
 };
 
 // Persistent CSR engine (one cooperative launch).  Workgroup w of W owns ranks
@@ -114,7 +116,7 @@ hipError_t pt_slab_set_lds(int dim, int NB, size_t lds);
 int pt_pslab_grid(const PtInstance& I, const SState& S);
 // The bit-sliced variant runs when the side is a multiple of 32 (rows of 32-bit words).
 bool pt_pslab_bits(int L);
-size_t pt_pslab_lds_bytes(int L, int dim, int NB);
+size_t pt_pslab_lds_bytes(int L, int dim, int NB, bool upre = false);
 hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu);
 hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs& C, int block,
                            size_t lds, bool cooperative, hipStream_t s);

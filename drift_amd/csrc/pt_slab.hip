@@ -668,8 +668,9 @@ __device__ __forceinline__ void bits_row(uint32_t bits, int8_t* dst) {
     }
 }
 
+// At most 512 threads: the next-sweep uniform path needs more than 128 registers per lane.
 template <int DIM, bool WRAP>
-__global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, CoopArgs C) {
+__global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, CoopArgs C) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lw[];
     __shared__ int64_t Es[1024];
     __shared__ int64_t EbL[1024];
@@ -683,6 +684,8 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
     __shared__ int32_t labelL[1024];
     __shared__ int sh_abort, sh_imp, sh_slot;
     __shared__ long long sh_dsum;
+    __shared__ uint32_t sh_gen, sh_uready;
+    __shared__ uint32_t sh_uex[PT_WAVE];
     constexpr int D = 2 * DIM;
     int a, b;
     wg_slot_slab(blockIdx.x, S.NB, a, b);
@@ -730,6 +733,8 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
     if (tid == 0) {
         sh_abort = 0;
         sh_dsum = 0;
+        sh_gen = 0;
+        sh_uready = 0;
         for (int k = 0; k < 5; ++k) bestL[k] = 0;
     }
     // R <= 64: the round end is wave 0's (WaveLadder); the other waves meanwhile bring in the
@@ -738,14 +743,47 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
     const bool defer = fast && !WRAP && nt > PT_WAVE && I.S > 0;
     WaveLadder WL;
     if (fast && tid < PT_WAVE) wave_ladder_init(WL, I, S.einit, tid);
-    __syncthreads();
 
     uint32_t phase = 0;
     const int segb = Lb - 3;                              // log2 segments of 8 sites per row
     const int rb = DIM == 3 ? Lb : 0;                     // log2 rows per plane
     const int wb = Lb - 5;                                // log2 words per row
     const int pb = DIM == 3 ? 2 * Lb : Lb;                // log2 sites per plane
+    const int psb = rb + segb;                            // log2 segments per slot
     const int jstride = njs << wppb;                      // words per bond direction
+    // A sweep is three passes when a slab has more than four planes: colour 0 over the slab and its
+    // two halo planes, then colour 1 over the two first and two last owned planes (published while
+    // the third pass sweeps the interior ones); otherwise two.  Pass c, slot i -> slab slot.
+    const bool split = !WRAP && T > 4;
+    auto pass_slot = [&](int c, int i) -> int {
+        return WRAP ? i : (c == 0 ? 1 + i : (!split ? 2 + i : (c == 1 ? (i < 2 ? 2 + i : T + i - 2) : 4 + i)));
+    };
+    const int ns0 = WRAP ? L : T + 2, ns1 = WRAP ? L : (!split ? T : 4);
+    const int uoff1 = ns0 << psb, uoff2 = uoff1 + (ns1 << psb);
+    const int nU = WRAP ? (2 * L) << psb : (2 * T + 2) << psb;   // segments a sweep updates
+    // Next-sweep uniforms: with S.upre, waves 1.. draw the Philox blocks of sweep t + 1 (they depend
+    // on position, slot and sweep only) into LDS while wave 0 alone completes the grid barrier and
+    // the round end; the colour passes then read them.
+    const bool pre = S.upre != 0 && fast && nt > PT_WAVE;
+    uint4* U = (uint4*)(lw + ((((size_t)nslots + (size_t)D * njs) * WPP + 3) & ~(size_t)3));
+    const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
+    auto draw = [&](uint32_t tt, int first, int stride) {
+        for (int idx = first; idx < nU; idx += stride) {
+            const int c = idx < uoff1 ? 0 : (idx < uoff2 ? 1 : 2);
+            const int w = idx - (c == 0 ? 0 : (c == 1 ? uoff1 : uoff2));
+            const int i = w >> psb, rem = w & ((1 << psb) - 1);
+            const int s = pass_slot(c, i);
+            const int r = rem >> segb, k = rem & ((1 << segb) - 1);
+            const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
+            const int col = c < 2 ? c : 1;
+            const uint32_t q = ((uint32_t)(col * g.half) >> 2) +
+                               ((uint32_t)((plane << pb) + (r << Lb) + (k << 3)) >> 3);
+            const pt_u4 v = pt_philox(q, (uint32_t)a, tt, PT_TAG_METRO, I.k0, I.k1);
+            U[idx] = make_uint4(v.x, v.y, v.z, v.w);
+        }
+    };
+    if (pre && live && nsweeps > 0) draw(0u, tid, nt);
+    __syncthreads();
     for (int rnd = 0;; ++rnd) {
         // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
         if (fast) {
@@ -753,7 +791,12 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                 int64_t E = WL.Eb;
                 if (rnd > 0 && tid < R)
                     E += ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + WL.rep);
-                const bool imp = wave_ladder_round(WL, I, rnd, E, tid, owner);
+                // pre: wave 1 drew this exchange's uniforms during the barrier (a short wait)
+                if (pre && rnd > 0)
+                    while (__hip_atomic_load(&sh_uready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < phase)
+                        __builtin_amdgcn_s_sleep(1);
+                const bool imp = wave_ladder_round(WL, I, rnd, E, tid, owner,
+                                                   pre && rnd > 0 ? sh_uex : nullptr);
                 if (tid < R) rungs[WL.rep] = tid;
                 if (tid == 0) {
                     sh_imp = imp;
@@ -772,6 +815,7 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                 }
             }
             __syncthreads();
+            if (sh_abort) return;
             if (sh_imp) {
                 const int slot = sh_slot;
                 if (live && slot == a)
@@ -855,21 +899,14 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                     st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
                 }
             };
-            // With more than four owned planes, colour 1 runs in two passes: the two first and
-            // two last owned planes, which are then published while pass 2 sweeps the interior.
-            const bool split = !WRAP && T > 4;
             if (live) {
                 for (int c = 0; c < (split ? 3 : 2); ++c) {
                     const int col = c < 2 ? c : 1;
-                    const int ns = WRAP ? L : (c == 0 ? T + 2 : (!split ? T : (c == 1 ? 4 : T - 4)));
-                    const int psb = (DIM == 3 ? Lb : 0) + segb;   // log2 segments per slot
+                    const int ns = c == 0 ? ns0 : (c == 1 ? ns1 : T - 4);
+                    const int uo = c == 0 ? 0 : (c == 1 ? uoff1 : uoff2);
                     for (int w = tid; w < (ns << psb); w += nt) {
                         const int i = w >> psb, rem = w & ((1 << psb) - 1);
-                        const int s = WRAP ? i
-                                           : (c == 0 ? 1 + i
-                                                     : (!split ? 2 + i
-                                                               : (c == 1 ? (i < 2 ? 2 + i : T + i - 2)
-                                                                         : 4 + i)));
+                        const int s = pass_slot(c, i);
                         const int r = rem >> segb, k = rem & ((1 << segb) - 1);
                         const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
                         const bool owned = WRAP || (s >= 2 && s < T + 2);
@@ -919,7 +956,13 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                         const int par = DIM == 3 ? ((col + r + plane) & 1) : ((col + plane) & 1);
                         const uint32_t q = ((uint32_t)(col * g.half) >> 2) +
                                            ((uint32_t)((plane << pb) + (r << Lb) + x0) >> 3);
-                        const pt_u4 rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                        pt_u4 rv;
+                        if (pre) {
+                            const uint4 v = U[uo + w];
+                            rv = pt_u4{v.x, v.y, v.z, v.w};
+                        } else {
+                            rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                        }
                         // q = D - 2m <= 0 always flips: m >= 3 in 3-D, m >= 2 in 2-D.  Otherwise
                         // the threshold of q: 3-D m = 2, 1, 0 -> t2, t4, t6; 2-D m = 1, 0 -> t2, t4.
                         const uint32_t always = DIM == 3 ? (c2 | (c1 & c0)) : (c2 | c1);
@@ -955,6 +998,60 @@ __global__ __launch_bounds__(1024) void pslab_bits_run(PtInstance I, SState S, C
                 if ((tid & (PT_WAVE - 1)) == 0 && v != 0 && live)
                     atomicAdd((unsigned long long*)&sh_dsum, (unsigned long long)v);
                 dsum = 0;
+            }
+            if (pre) {
+                // arrival: every wave's stores complete; thread 0 adds the energy sum and flags
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
+                ++phase;
+                if (tid == 0) {
+                    const long long v = sh_dsum;
+                    sh_dsum = 0;
+                    if (last && live && v != 0) {
+                        atomicAdd((unsigned long long*)(S.dacc + (size_t)(rnd % 3) * Rp + a),
+                                  (unsigned long long)v);
+                        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                    }
+                    if (blockIdx.x != 0) st_u32(C.flags + blockIdx.x, phase);
+                }
+                if (tid < PT_WAVE) {
+                    // wave 0 completes the barrier (the hub polls every flag, one lane per flag)
+                    if (blockIdx.x == 0) {
+                        for (int x = 1 + tid; x < (int)gridDim.x; x += PT_WAVE)
+                            if (!wait_phase(C, C.flags + x, phase)) sh_abort = 1;
+                        if (tid == 0 && sh_abort == 0) st_u32(C.gen, phase);
+                    } else if (tid == 0) {
+                        if (!wait_phase(C, C.gen, phase)) sh_abort = 1;
+                    }
+                    if (tid == 0)
+                        __hip_atomic_store(&sh_gen, phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    if (last && tid < 2 * PT_WAVE) {
+                        // wave 1: the uniforms of this round's exchange, for wave 0's round end
+                        const int r = tid - PT_WAVE;
+                        if (r < R) {
+                            const pt_u4 ev = pt_philox((uint32_t)(r >> 2), 0u, (uint32_t)rnd,
+                                                       PT_TAG_EXCH, I.k0, I.k1);
+                            sh_uex[r] = pt_word(ev, r & 3);
+                        }
+                        if (tid == PT_WAVE)
+                            __hip_atomic_store(&sh_uready, phase, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    if (live && t + 1 < nsweeps) draw(t + 1, tid - PT_WAVE, nt - PT_WAVE);
+                    while (__hip_atomic_load(&sh_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < phase)
+                        __builtin_amdgcn_s_sleep(1);
+                }
+                if (last) continue;   // the round end brings in the halo planes
+                if (tid >= PT_WAVE && live && !WRAP)
+                    for (int idx = tid - PT_WAVE; idx < 4 * WPP; idx += nt - PT_WAVE) {
+                        const int kk = idx >> wppb, w = idx & (WPP - 1);
+                        const int sl = kk < 2 ? kk : T + kk;
+                        const int plane = (z0 - 2 + sl) & (L - 1);
+                        sb[(size_t)sl * WPP + w] = ld_u32(nxt + (size_t)plane * WPP + w);
+                    }
+                __syncthreads();
+                if (sh_abort) return;
+                continue;
             }
             if (!coop_sync(C, phase, &sh_abort, S.dacc + (size_t)(rnd % 3) * Rp + a,
                            last && live ? &sh_dsum : nullptr))
@@ -1064,13 +1161,18 @@ hipError_t pt_launch_slab_tick(const SState& S, int rounds, hipStream_t s) {
 
 bool pt_pslab_bits(int L) { return L >= 32 && (L & 31) == 0; }
 
-size_t pt_pslab_lds_bytes(int L, int dim, int NB) {
+size_t pt_pslab_lds_bytes(int L, int dim, int NB, bool upre) {
     if (!pt_pslab_bits(L)) return pt_slab_lds_bytes(0, L, dim, NB);
     const size_t WPP = (dim == 3 ? (size_t)L : 1) * (size_t)(L / 32);
     const size_t T = (size_t)((L + NB - 1) / NB);
     const size_t spins = NB == 1 ? (size_t)L : T + 4;
     const size_t signs = NB == 1 ? (size_t)L : T + 2;
-    return (spins + 2 * (size_t)dim * signs) * WPP * 4;
+    const size_t words = (spins + 2 * (size_t)dim * signs) * WPP;
+    if (!upre) return words * 4;
+    // one Philox block (16 bytes) per 8-site segment of every slot a sweep updates
+    const size_t segs = (dim == 3 ? (size_t)L : 1) * (size_t)(L / 8);
+    const size_t slots = NB == 1 ? 2 * (size_t)L : 2 * T + 2;
+    return ((words * 4 + 15) & ~(size_t)15) + slots * segs * 16;
 }
 
 hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu) {
