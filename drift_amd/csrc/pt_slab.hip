@@ -751,16 +751,12 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     const int pb = DIM == 3 ? 2 * Lb : Lb;                // log2 sites per plane
     const int psb = rb + segb;                            // log2 segments per slot
     const int jstride = njs << wppb;                      // words per bond direction
-    // A sweep is three passes when a slab has more than four planes: colour 0 over the slab and its
-    // two halo planes, then colour 1 over the two first and two last owned planes (published while
-    // the third pass sweeps the interior ones); otherwise two.  Pass c, slot i -> slab slot.
-    const bool split = !WRAP && T > 4;
-    auto pass_slot = [&](int c, int i) -> int {
-        return WRAP ? i : (c == 0 ? 1 + i : (!split ? 2 + i : (c == 1 ? (i < 2 ? 2 + i : T + i - 2) : 4 + i)));
-    };
-    const int ns0 = WRAP ? L : T + 2, ns1 = WRAP ? L : (!split ? T : 4);
-    const int uoff1 = ns0 << psb, uoff2 = uoff1 + (ns1 << psb);
-    const int nU = WRAP ? (2 * L) << psb : (2 * T + 2) << psb;   // segments a sweep updates
+    // A sweep is two passes: colour 0 over the slab and its two halo planes (slots 1 .. T + 2), then
+    // colour 1 over the owned planes (slots 2 .. T + 1).  Pass c, slot i -> slab slot.
+    auto pass_slot = [&](int c, int i) -> int { return WRAP ? i : (c == 0 ? 1 + i : 2 + i); };
+    const int ns0 = WRAP ? L : T + 2, ns1 = WRAP ? L : T;
+    const int uoff1 = ns0 << psb;
+    const int nU = uoff1 + (ns1 << psb);                  // 8-site segments a sweep updates
     // Next-sweep uniforms: with S.upre, waves 1.. draw the Philox blocks of sweep t + 1 (they depend
     // on position, slot and sweep only) into LDS while wave 0 alone completes the grid barrier and
     // the round end; the colour passes then read them.
@@ -769,13 +765,13 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
     auto draw = [&](uint32_t tt, int first, int stride) {
         for (int idx = first; idx < nU; idx += stride) {
-            const int c = idx < uoff1 ? 0 : (idx < uoff2 ? 1 : 2);
-            const int w = idx - (c == 0 ? 0 : (c == 1 ? uoff1 : uoff2));
+            const int c = idx < uoff1 ? 0 : 1;
+            const int w = idx - (c == 0 ? 0 : uoff1);
             const int i = w >> psb, rem = w & ((1 << psb) - 1);
             const int s = pass_slot(c, i);
             const int r = rem >> segb, k = rem & ((1 << segb) - 1);
             const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
-            const int col = c < 2 ? c : 1;
+            const int col = c;
             const uint32_t q = ((uint32_t)(col * g.half) >> 2) +
                                ((uint32_t)((plane << pb) + (r << Lb) + (k << 3)) >> 3);
             const pt_u4 v = pt_philox(q, (uint32_t)a, tt, PT_TAG_METRO, I.k0, I.k1);
@@ -899,20 +895,24 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                     st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
                 }
             };
+            // One lane per 32-site row word: the neighbour words, bond planes and unsatisfied-bond
+            // counts of the word serve its 16 sites of the colour, decided against four Philox
+            // blocks (one per 8-site segment, the blocks every engine draws).  The lane owns the
+            // word in this pass; lanes reading it as a neighbour use only the other colour's bits,
+            // which do not change, so a plain store suffices.
             if (live) {
-                for (int c = 0; c < (split ? 3 : 2); ++c) {
-                    const int col = c < 2 ? c : 1;
-                    const int ns = c == 0 ? ns0 : (c == 1 ? ns1 : T - 4);
-                    const int uo = c == 0 ? 0 : (c == 1 ? uoff1 : uoff2);
-                    for (int w = tid; w < (ns << psb); w += nt) {
-                        const int i = w >> psb, rem = w & ((1 << psb) - 1);
+                const int wpb = rb + wb;                      // log2 words per slot
+                for (int c = 0; c < 2; ++c) {
+                    const int ns = c == 0 ? ns0 : ns1;
+                    const int uo = c == 0 ? 0 : uoff1;
+                    for (int w = tid; w < (ns << wpb); w += nt) {
+                        const int i = w >> wpb, rem = w & ((1 << wpb) - 1);
                         const int s = pass_slot(c, i);
-                        const int r = rem >> segb, k = rem & ((1 << segb) - 1);
+                        const int r = rem >> wb, wi = rem & (WPR - 1);
                         const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
                         const bool owned = WRAP || (s >= 2 && s < T + 2);
                         const int su = WRAP ? ((s + 1) & (L - 1)) : s + 1;
                         const int sd = WRAP ? ((s - 1) & (L - 1)) : s - 1;
-                        const int wi = k >> 2;                      // word of the row
                         // word index of (slot, row) + wi, by shifts: rows = 2^rb, words/row = 2^wb
                         const int wrow = (((s << rb) + r) << wb) + wi;
                         const uint32_t W = sb[wrow];
@@ -952,30 +952,32 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                             c1 = k1 ^ k0;
                             c2 = k1 & k0;
                         }
-                        const int x0 = k << 3;                      // first site of the segment
-                        const int par = DIM == 3 ? ((col + r + plane) & 1) : ((col + plane) & 1);
-                        const uint32_t q = ((uint32_t)(col * g.half) >> 2) +
-                                           ((uint32_t)((plane << pb) + (r << Lb) + x0) >> 3);
-                        pt_u4 rv;
-                        if (pre) {
-                            const uint4 v = U[uo + w];
-                            rv = pt_u4{v.x, v.y, v.z, v.w};
-                        } else {
-                            rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
-                        }
+                        const int par = DIM == 3 ? ((c + r + plane) & 1) : ((c + plane) & 1);
                         // q = D - 2m <= 0 always flips: m >= 3 in 3-D, m >= 2 in 2-D.  Otherwise
                         // the threshold of q: 3-D m = 2, 1, 0 -> t2, t4, t6; 2-D m = 1, 0 -> t2, t4.
                         const uint32_t always = DIM == 3 ? (c2 | (c1 & c0)) : (c2 | c1);
-                        const int b0 = (x0 & 31) + par;
                         uint32_t flip = 0;
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const uint32_t bit = 1u << (b0 + 2 * i);
-                            const uint32_t th = DIM == 3 ? ((c1 & bit) ? t2 : ((c0 & bit) ? t4 : t6))
-                                                         : ((c0 & bit) ? t2 : t4);
-                            flip |= (always & bit) | (pt_word(rv, i) < th ? bit : 0u);
+                        for (int kk = 0; kk < 4; ++kk) {          // the word's four segments
+                            pt_u4 rv;
+                            if (pre) {
+                                const uint4 v = U[uo + (w << 2) + kk];
+                                rv = pt_u4{v.x, v.y, v.z, v.w};
+                            } else {
+                                const uint32_t q = ((uint32_t)(c * g.half) >> 2) +
+                                                   ((uint32_t)((plane << pb) + (r << Lb) + (wi << 5) +
+                                                               (kk << 3)) >> 3);
+                                rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                            }
+#pragma unroll
+                            for (int ii = 0; ii < 4; ++ii) {
+                                const uint32_t bit = 1u << (8 * kk + par + 2 * ii);
+                                const uint32_t th = DIM == 3 ? ((c1 & bit) ? t2 : ((c0 & bit) ? t4 : t6))
+                                                             : ((c0 & bit) ? t2 : t4);
+                                flip |= (always & bit) | (pt_word(rv, ii) < th ? bit : 0u);
+                            }
                         }
-                        atomicXor(sb + wrow, flip);
+                        sb[wrow] = W ^ flip;
                         // energy change of the flips: sum of 2q = 2(D - 2m) over the flipped sites
                         // (in signed arithmetic: __popc returns unsigned)
                         if (owned) {
@@ -986,10 +988,9 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                         }
                     }
                     __syncthreads();
-                    if (split && c == 1) publish();
                 }
             }
-            if (live && !WRAP && !split) publish();
+            if (live && !WRAP) publish();
             const bool last = j == I.S - 1;
             if (last) {   // the slot's energy change: waves into LDS, thread 0 to HBM on arrival
                 long long v = (long long)dsum;
