@@ -375,8 +375,9 @@ def bench_ladder_c5(args) -> int:
             T_min, T_max = C5_FAMILIES[fam]
             models = [c5_instance(fam, 10_000 * fi + k) for k in range(args.train_per_family)]
             t0 = time.perf_counter()
-            for c0 in range(0, len(models), 128):        # chunks, so progress shows as it goes
-                chunk = models[c0:c0 + 128]
+            step = 32 if args.labels == "budget" else 128   # chunks, so progress shows as it goes
+            for c0 in range(0, len(models), step):
+                chunk = models[c0:c0 + step]
                 if args.labels == "budget":
                     x, pick = ladder.budget_labels(chunk, T_min=T_min, T_max=T_max, R=R,
                                                    budget=args.steps,

@@ -26,6 +26,7 @@ using namespace pt_dev;
 struct CoopShared {
     long long best[5];   // E, round, rung, slot, have
     int abort;
+    int imp, slot;       // the round end's best-rung result, for the whole workgroup
 };
 
 // First rank and size of workgroup w's slice of colour class [cs, cs + cz).
@@ -243,6 +244,79 @@ __global__ __launch_bounds__(1024) void coop_run(PtInstance I, GState G, CoopArg
         }
 
         // ---- round end: identical in every workgroup ---------------------------------------
+        if (R <= PT_WAVE) {
+            // by wave 0 (WaveLadder), the ladder kept in the LDS arrays between round ends
+            if (tid < PT_WAVE) {
+                const int r = tid;
+                const bool in = r < R;
+                WaveLadder Wl;
+                Wl.rep = in ? repL[r] : r;
+                int64_t E = 0;
+                if (in) {
+                    E = EbL[Wl.rep];
+                    int64_t d = 0;
+#pragma unroll 8
+                    for (int sp = 0; sp < PT_DACC_SPLIT; ++sp)
+                        d += ld_coh(G.dacc + ((size_t)buf * PT_DACC_SPLIT + sp) * Rp + Wl.rep);
+                    E += d;
+                }
+                Wl.Eb = E;
+                Wl.beta = in ? betaL[r] : 0.0;
+                Wl.lab = in ? labelL[Wl.rep] : 0;
+                Wl.trips = in ? (int32_t)stats[2 * Rp + Wl.rep] : 0;
+                Wl.tried = in ? (int32_t)stats[r] : 0;
+                Wl.acc = in ? (int32_t)stats[Rp + r] : 0;
+                Wl.up = in ? (int32_t)stats[3 * Rp + r] : 0;
+                Wl.down = in ? (int32_t)stats[4 * Rp + r] : 0;
+                Wl.bestE = sh->best[0];
+                Wl.best_round = (int32_t)sh->best[1];
+                Wl.best_rung = (int32_t)sh->best[2];
+                Wl.best_slot = (int32_t)sh->best[3];
+                Wl.have = (int32_t)sh->best[4];
+                // round rnd's end is round end rnd + 1 of the ladder (0 = the initial state)
+                const bool imp = wave_ladder_round(Wl, I, rnd + 1, E, r, owner);
+                if (in) {
+                    repL[r] = Wl.rep;
+                    rungL[Wl.rep] = r;
+                    EbL[Wl.rep] = Wl.Eb;
+                    labelL[Wl.rep] = Wl.lab;
+                    stats[2 * Rp + Wl.rep] = Wl.trips;
+                    stats[r] = Wl.tried;
+                    stats[Rp + r] = Wl.acc;
+                    stats[3 * Rp + r] = Wl.up;
+                    stats[4 * Rp + r] = Wl.down;
+                }
+                if (r == 0) {
+                    sh->best[0] = Wl.bestE;
+                    sh->best[1] = Wl.best_round;
+                    sh->best[2] = Wl.best_rung;
+                    sh->best[3] = Wl.best_slot;
+                    sh->best[4] = Wl.have;
+                    sh->imp = imp;
+                    sh->slot = Wl.best_slot;
+                }
+            }
+            if (owner) {
+                // buffer of round rnd + 2: its last readers finished before this round's barriers
+                int64_t* z = G.dacc + (size_t)((rnd + 2) % 3) * PT_DACC_SPLIT * Rp;
+                for (int x = tid; x < PT_DACC_SPLIT * Rp; x += nt)
+                    __hip_atomic_store(z + x, (int64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            if (sh->imp) {
+                // this workgroup's positions of the best slot, from its own LDS copy
+                const int slot = sh->slot;
+                int ob = 0;
+                for (int c = 0; c < k; ++c) {
+                    int lo, cnt;
+                    slice(cptr[c], cptr[c + 1] - cptr[c], w, W, lo, cnt);
+                    for (int i = tid; i < cnt; i += nt) I.best_s[lo + i] = lsp[(size_t)(ob + i) * Rp + slot];
+                    ob += cnt;
+                }
+                __syncthreads();
+            }
+            continue;
+        }
         for (int x = tid; x < R; x += nt) Es[x] = EbL[x];
         __syncthreads();
         // one coherent load per (counter, slot) pair, summed in LDS: a single memory round trip

@@ -394,6 +394,76 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
     const bool init = init_end != 0;
     const int rnd = init ? -1 : *G.round_base + rl;
     const int cur = rnd & 1, nxt = cur ^ 1;
+    if (R <= PT_WAVE) {
+        // wave 0 evaluates the round end (WaveLadder) with the ladder loaded from this round's
+        // buffers; workgroup 0 writes the next round's; every workgroup copies its share of an
+        // improved best configuration
+        __shared__ int sh_imp, sh_slot;
+        int64_t* out = I.out;
+        if (tid < PT_WAVE) {
+            const int r = tid;
+            const bool in = r < R;
+            WaveLadder W;
+            W.rep = init || !in ? r : G.rep[(size_t)cur * Rp + r];
+            int64_t E = 0;
+            if (in) {
+                E = init ? 0 : G.Eb[(size_t)cur * Rp + W.rep];
+                int64_t d = 0;
+#pragma unroll 8
+                for (int k = 0; k < PT_DACC_SPLIT; ++k) d += G.dacc[((size_t)cur * PT_DACC_SPLIT + k) * Rp + W.rep];
+                E += d;
+                if (init && WM != PT_WM_F32) E = -(E / 2);
+            }
+            W.Eb = E;
+            W.beta = in ? I.beta[r] : 0.0;
+            W.lab = init ? ((R > 1 && r == 0) ? 1 : ((R > 1 && r == R - 1) ? -1 : 0))
+                         : (in ? G.label[W.rep] : 0);
+            W.trips = in ? (int32_t)pt_out_arr(out, R, PT_OUT_TRIPS)[W.rep] : 0;
+            W.tried = in ? (int32_t)pt_out_arr(out, R, PT_OUT_TRIED)[r] : 0;
+            W.acc = in ? (int32_t)pt_out_arr(out, R, PT_OUT_ACC)[r] : 0;
+            W.up = in ? (int32_t)pt_out_arr(out, R, PT_OUT_UP)[r] : 0;
+            W.down = in ? (int32_t)pt_out_arr(out, R, PT_OUT_DOWN)[r] : 0;
+            const int64_t* bc = G.best + 5 * cur;
+            W.have = init ? 0 : (int32_t)bc[4];
+            W.bestE = bc[0];
+            W.best_round = (int32_t)bc[1];
+            W.best_rung = (int32_t)bc[2];
+            W.best_slot = (int32_t)bc[3];
+            const bool imp = wave_ladder_round(W, I, rnd + 1, E, r, blockIdx.x == 0);
+            if (r == 0) {
+                sh_imp = imp;
+                sh_slot = W.best_slot;
+            }
+            if (blockIdx.x == 0) {
+                if (in) {
+                    G.Eb[(size_t)nxt * Rp + W.rep] = W.Eb;
+                    G.rung[(size_t)nxt * Rp + W.rep] = r;
+                    G.rep[(size_t)nxt * Rp + r] = W.rep;
+                    G.label[W.rep] = W.lab;
+                    pt_out_arr(out, R, PT_OUT_TRIPS)[W.rep] = W.trips;
+                    pt_out_arr(out, R, PT_OUT_TRIED)[r] = W.tried;
+                    pt_out_arr(out, R, PT_OUT_ACC)[r] = W.acc;
+                    pt_out_arr(out, R, PT_OUT_UP)[r] = W.up;
+                    pt_out_arr(out, R, PT_OUT_DOWN)[r] = W.down;
+                    for (int k = 0; k < PT_DACC_SPLIT; ++k)
+                        G.dacc[((size_t)nxt * PT_DACC_SPLIT + k) * Rp + r] = 0;
+                }
+                if (r == 0) {
+                    int64_t* bn = G.best + 5 * nxt;
+                    bn[0] = W.bestE; bn[1] = W.best_round; bn[2] = W.best_rung; bn[3] = W.best_slot;
+                    bn[4] = W.have;
+                }
+            }
+        }
+        __syncthreads();
+        if (sh_imp) {
+            const int slot = sh_slot;
+            const int chunk = (I.n + gridDim.x - 1) / gridDim.x;
+            const int p0 = blockIdx.x * chunk, p1 = min(I.n, p0 + chunk);
+            for (int p = p0 + tid; p < p1; p += blockDim.x) I.best_s[p] = G.sp[(size_t)p * Rp + slot];
+        }
+        return;
+    }
     for (int x = tid; x < R; x += blockDim.x) {
         Es[x] = init ? 0 : G.Eb[(size_t)cur * Rp + x];
         reps[x] = init ? x : G.rep[(size_t)cur * Rp + x];
