@@ -375,7 +375,7 @@ def bench_ladder_c5(args) -> int:
             T_min, T_max = C5_FAMILIES[fam]
             models = [c5_instance(fam, 10_000 * fi + k) for k in range(args.train_per_family)]
             t0 = time.perf_counter()
-            step = 32 if args.labels == "budget" else 128   # chunks, so progress shows as it goes
+            step = 64 if args.labels == "budget" else 128   # chunks, so progress shows as it goes
             for c0 in range(0, len(models), step):
                 chunk = models[c0:c0 + step]
                 if args.labels == "budget":
