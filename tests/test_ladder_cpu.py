@@ -83,3 +83,36 @@ def test_bf16_rounding_values():
 
 def test_feature_vector_has_sixteen_entries():
     assert ladder.N_FEATURES == 16 and M.IN == 16 and M.NPARAMS == ladder.N_PARAMS == 39328
+
+
+def test_power_shapes_and_budget_label_choice(monkeypatch):
+    """gamma = 1 is the geometric ladder, gamma > 1 moves rungs towards the cold end; the label of
+    an instance is the shape with the lowest best energy, the earliest best round on ties."""
+    T = geometric_ladder(32, 0.1, 5.0)
+    assert np.allclose(ladder.power_positions(1.0), ladder.positions_from_temperatures(T))
+    assert np.allclose(1.0 / ladder.betas_from_positions(ladder.power_positions(1.0), 0.1, 5.0), T)
+    assert ladder.power_positions(2.0)[8] < ladder.power_positions(1.0)[8] < ladder.power_positions(0.5)[8]
+
+    class Res:
+        def __init__(self, E, rnd):
+            self.best_E, self.best_round = E, rnd
+
+    # per shape (gammas 0.5, 0.7, 1, 1.4, 2) and instance: (best energy, round it was first seen)
+    table = {0.5: [(-10, 50), (-7, 9), (-3, 1)], 0.7: [(-10, 40), (-8, 9), (-3, 1)],
+             1.0: [(-9, 1), (-8, 5), (-3, 1)], 1.4: [(-10, 45), (-8, 7), (-3, 1)],
+             2.0: [(-10, 60), (-6, 1), (-3, 1)]}
+    calls = []
+
+    def fake_run(models, *, betas, **kw):
+        x = ladder.positions_from_temperatures(1.0 / np.asarray(betas[0]))
+        g = min(table, key=lambda gam: np.abs(ladder.power_positions(gam) - x).max())
+        calls.append(g)
+        return [Res(*table[g][i]) for i in range(len(models))]
+
+    monkeypatch.setattr(ladder, "run_instances", fake_run)
+    x, pick = ladder.budget_labels([object()] * 3, T_min=0.1, T_max=5.0, budget=100)
+    assert calls == list(ladder.BUDGET_GAMMAS)
+    # instance 0: -10 reached first by gamma 0.7 (round 40); instance 1: -8 first by gamma 1.0
+    # (round 5); instance 2: all equal, the first shape wins
+    assert [ladder.BUDGET_GAMMAS[k] for k in pick] == [0.7, 1.0, 0.5]
+    assert np.allclose(x[1], ladder.power_positions(1.0))
