@@ -55,6 +55,15 @@ def workload(name: str):
     return model, R, beta, desc
 
 
+def lattice_side(n: int) -> int:
+    """Side L of an L^3 or L^2 lattice of n sites (0 if n is neither)."""
+    for d in (3, 2):
+        L = int(round(n ** (1.0 / d)))
+        if L ** d == n:
+            return L
+    return 0
+
+
 def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
     """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse.
     Spins are int8, so own spin read + write and d neighbour reads are d + 2 bytes.  The CSR
@@ -582,7 +591,9 @@ def main() -> int:
         "traffic": None,
         "kernel": {"global": "g_color_step", "lattice": "g_lattice_step",
                    "slab": "slab_sweep", "coop": "coop_run",
-                   "pslab": "pslab_run"}.get(kt.engine, "pt_lds_kernel"),
+                   # the persistent slab engine keeps slabs as bits when L is a multiple of 32
+                   "pslab": "pslab_bits_run" if lattice_side(model.n) % 32 == 0 else "pslab_run",
+                   }.get(kt.engine, "pt_lds_kernel"),
         "bytes_per_attempt": round(B, 3),
         "attempts_per_launch": per_launch_attempts,
         "avg_launch_us": round(per_launch_s * 1e6, 3),
