@@ -286,6 +286,33 @@ def test_pslab_bit_sliced_rows(case):
     assert g.engine == "pslab"
 
 
+@pytest.mark.parametrize("case", [("int8", 32, 256), ("int8", 32, 512), ("f32", 16, 256)])
+def test_global_engine_on_c3_sized_graphs(case):
+    """The per-launch engine on the C3 instance itself (Erdos-Renyi N = 10^5, mean degree 6, int8
+    weights, R = 32; colour classes up to 32 400 positions) and on a 3-regular N(0,1) graph of the
+    same size in float32: the chain is the oracle's."""
+    from drift_amd.builders.lattice import er_sparse_weighted, random_regular_gaussian
+
+    kind, R, block = case
+    _announce(f"global two-per-lane {kind} R={R} block={block}")
+    model = (er_sparse_weighted(100000, 6.0, 0) if kind == "int8"
+             else random_regular_gaussian(100000, 3))
+    g = compare(model, R=R, n_rounds=6, S=1, seed=17, T_min=0.2, T_max=5.0, engine="global",
+                block=block)[0]
+    assert g.engine == "global"
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_pslab_workgroup_sizes(block):
+    """The bit-sliced persistent slab engine with one wave per workgroup (the round end and the
+    barrier in the only wave, nothing drawn ahead) and with two and four waves (the other waves
+    draw the next sweep's uniforms during the barrier)."""
+    _announce(f"pslab bits block={block}")
+    g = compare(cubic_pmj_glass(32, 11), R=16, n_rounds=30, S=2, seed=block, T_min=0.3, T_max=4.0,
+                engine="pslab", block=block)[0]
+    assert g.engine == "pslab"
+
+
 @pytest.mark.parametrize("engine", ["pslab", "coop"])
 def test_persistent_engines_with_an_ordinary_dispatch(engine):
     """DRIFT_PT_PLAIN_LAUNCH starts the persistent engines without the cooperative-launch API;
