@@ -64,15 +64,16 @@ def lattice_side(n: int) -> int:
     return 0
 
 
-def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
+def algorithmic_bytes_per_attempt(model, enc, engine: str, replicas: int = 0) -> float:
     """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse.
-    Spins are int8, so own spin read + write and d neighbour reads are d + 2 bytes.  The CSR
-    kernels add d edge records (int8 weight and 24-bit column packed in 4 bytes; int32 / float
-    weight and int32 column in 8) and two int32 row pointers (plus the field if any); positions
-    are relabelled by colour, so no colour list is read.  The lattice kernel computes neighbours
-    from coordinates and reads one byte holding the site's bond signs."""
+    Spins are int8 (b_s = 1), so own spin read + write and d neighbour reads are d + 2 bytes;
+    the CSR engine keeps up to 32 replicas as one bit each of a word per position (b_s = 1/8).
+    The CSR kernels add d edge records (int8 weight and 24-bit column packed in 4 bytes; int32 /
+    float weight and int32 column in 8) and two int32 row pointers (plus the field if any);
+    positions are relabelled by colour, so no colour list is read.  The lattice kernel computes
+    neighbours from coordinates and reads one byte holding the site's bond signs."""
     d = model.nnz / model.n
-    b_s = 1.0
+    b_s = 0.125 if engine == "global" and 0 < replicas <= 32 else 1.0
     if engine in ("lattice", "slab", "pslab"):
         return b_s * (d + 1) + b_s + 1.0
     b_e = 4.0 if enc.weight.dtype.itemsize == 1 and model.n < (1 << 24) else 8.0
@@ -80,7 +81,7 @@ def algorithmic_bytes_per_attempt(model, enc, engine: str) -> float:
     return b_s * (d + 1) + b_s + d * b_e + 8.0 + b_h
 
 
-def engine_format(engine: str, model, enc) -> str:
+def engine_format(engine: str, model, enc, replicas: int = 0) -> str:
     """Storage format the engine ran the workload in (what the algorithmic bytes count)."""
     if engine == "lattice":
         return ("int8 spins [position][replica], implicit periodic neighbours, one byte of bond "
@@ -98,7 +99,9 @@ def engine_format(engine: str, model, enc) -> str:
                    "exchanged as bits" if bits else
                    "as int8 spins and bond-sign bytes, halo planes exchanged as bytes")
                 + "; int8 spins [replica][site] in HBM at the ends of the run")
-    return (f"int8 spins [position][replica], packed CSR edge records "
+    spins = ("spins as one 32-bit word per position (bit per replica)"
+             if engine == "global" and 0 < replicas <= 32 else "int8 spins [position][replica]")
+    return (f"{spins}, packed CSR edge records "
             f"({'4' if enc.weight.dtype.itemsize == 1 else '8'} bytes: column and "
             f"{enc.weight.dtype.name} weight)")
 
@@ -567,7 +570,7 @@ def main() -> int:
 
     print("[bench] kernel timing run", file=sys.stderr, flush=True)
     kt = gpu.parallel_tempering_gpu(model, n_rounds=args.kernel_timing_steps, time_kernels=True, **kw)
-    B = algorithmic_bytes_per_attempt(model, enc, kt.engine)
+    B = algorithmic_bytes_per_attempt(model, enc, kt.engine, R)
     if kt.kernel_launches > 1:
         # The dominant kernel's average launch in the timed region: the region's time per step
         # times that kernel's share of device time, over its launches per step.  The share comes
@@ -629,7 +632,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "format": engine_format(res.engine, model, enc),
+            "format": engine_format(res.engine, model, enc, R),
             "engine": res.engine,
             "n": model.n,
             "replicas": R,

@@ -662,7 +662,7 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
     if (rc) return rc;
     if (!use_lattice) I.jbits = nullptr;
     const int Rp = H.Rp;
-    GState G;
+    GState G{};
     PT_HIP(A.alloc(&G.sp, (size_t)H.n * Rp));
     PT_HIP(A.alloc(&G.Eb, 2 * (size_t)Rp));
     PT_HIP(A.alloc(&G.dacc, 2 * (size_t)PT_DACC_SPLIT * Rp));
@@ -691,11 +691,20 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
     const int64_t threads = (int64_t)((maxc + 3) / 4 + 1) * Rp;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((threads + block - 1) / block, 2048));
     // CSR colour steps run one attempt per lane: (block / Rp) positions per workgroup
+    // R <= 32 on the CSR engine: spins as one word per position (32 lanes per position)
+    const bool bits = !use_lattice && H.R <= 32 && block % 64 == 0;
+    const int ppb = bits ? block / 32 : block / Rp;
     std::vector<int> cgrid((size_t)H.k);
     for (int c = 0; c < H.k; ++c)
-        cgrid[c] = std::max(1, (H.color_ptr[c + 1] - H.color_ptr[c] + block / Rp - 1) / (block / Rp));
+        cgrid[c] = std::max(1, (H.color_ptr[c + 1] - H.color_ptr[c] + ppb - 1) / ppb);
     const int reg = std::max(1, std::min(256, (H.n + 1023) / 1024));   // round-end grid
-    PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));
+    PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));   // byte spins, initial energies and state
+    uint32_t* spw = nullptr;
+    if (bits) {
+        PT_HIP(A.alloc(&spw, (size_t)H.n));
+        G.spw = spw;
+        PT_HIP(pt_launch_g_pack(I, G, false, sg.s));
+    }
     // optional per-launch timing of the colour-step kernel (the dominant kernel)
     const bool time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;
     EventPool pool, other;
@@ -779,6 +788,10 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         other_s += km * 1e-3;
     }
 
+    if (bits) {   // final spins back to bytes where finish_global reads them
+        PT_HIP(pt_launch_g_pack(I, G, true, sg.s));
+        PT_HIP(hipStreamSynchronize(sg.s));
+    }
     rc = finish_global(H, I, G, res);
     if (rc) return rc;
     res->seconds = ms * 1e-3;

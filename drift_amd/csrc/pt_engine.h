@@ -60,6 +60,8 @@ struct GState {
     int64_t* best;   // 2 * 5: E, round, rung, slot, have
     int32_t* round_base;  // round index that launch argument 'rl' is relative to; advanced on the
                           // device by pt_launch_g_tick so a captured chunk of rounds can replay
+    uint32_t* spw;        // R <= 32 on the CSR engine: one word per position, bit a = slot a's spin
+                          // (1 = -1); the colour steps and round ends use it instead of sp
 };
 
 // Slab-engine state: a periodic lattice split into slabs of planes along its slowest axis; one
@@ -138,6 +140,8 @@ hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j,
                              int block, int grid, hipStream_t s);
 hipError_t pt_launch_g_lattice(const PtInstance& I, const GState& G, int c, int j, int rl,
                                int block, int grid, hipStream_t s);
+// Word-per-position spins (R <= 32): pack sp into spw, and back.
+hipError_t pt_launch_g_pack(const PtInstance& I, const GState& G, bool unpack, hipStream_t s);
 // End of round (*round_base + rl): energies, best state, exchange.
 hipError_t pt_launch_g_round_end(const PtInstance& I, const GState& G, int rl,
                                  int round_end_grid, hipStream_t s);

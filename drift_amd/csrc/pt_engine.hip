@@ -275,6 +275,116 @@ __global__ __launch_bounds__(1024) void g_init_energy(PtInstance I, GState G) {
     block_slot_reduce(acc, a, R, Rp, red, G.dacc, 1);   // parity 1 holds the init totals
 }
 
+// Spin of slot a at position p in either layout.
+__device__ __forceinline__ int8_t g_spin(const GState& G, int p, int Rp, int a) {
+    return G.spw ? (((G.spw[p] >> a) & 1u) ? (int8_t)-1 : (int8_t)1) : G.sp[(size_t)p * Rp + a];
+}
+
+// Colour step of the CSR engine with one 32-bit word per position (R <= 32, Rp = 32): the 32 lanes
+// of a position read each neighbour's word once (a broadcast), take their own bit, decide as
+// g_color_step does, and combine their flips with a ballot; lane 0 of the position stores the
+// word.  The whole spin array is n words, so the instance stays in each XCD's L2.
+template <int WM>
+__global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int c, int j, int rl) {
+    __shared__ long long red[1024];
+    const int rnd = *G.round_base + rl, cur = rnd & 1;
+    const uint32_t t = (uint32_t)(rnd * I.S + j);
+    const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
+    for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
+    __syncthreads();
+    const int a = tid & 31;
+    const int p = I.color_ptr[c] + blockIdx.x * (int)(blockDim.x >> 5) + (tid >> 5);
+    const bool live = p < I.color_ptr[c + 1];
+    uint32_t own = 0;
+    bool flip = false;
+    int64_t d = 0;
+    if (live) {
+        own = G.spw[p];
+        const int t0 = I.row_ptr[p], n0 = I.row_ptr[p + 1] - t0;
+        const int8_t si = ((own >> a) & 1u) ? (int8_t)-1 : (int8_t)1;
+        if constexpr (WM == PT_WM_F32) {
+            float l = I.h ? ((const float*)I.h)[p] : 0.0f;
+            for (int k = 0; k < n0; k += 8) {
+                int cc[8];
+                float w[8];
+                uint32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = k + u < n0;
+                    cc[u] = edge_float(I.edge, ok ? t0 + k + u : 0, w[u]);
+                    cc[u] = ok ? cc[u] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = G.spw[cc[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float nl = ((v[u] >> a) & 1u) ? l - w[u] : l + w[u];
+                    l = k + u < n0 ? nl : l;
+                }
+            }
+            if (a < R) {
+                const int g = G.rung[cur * Rp + a];
+                const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                const float q = si > 0 ? l : -l;
+                if (pt_accept_float((float)I.beta[g], q, pt_word(rv, p & 3))) {
+                    flip = true;
+                    d = pt_fix36(2.0f * q);
+                }
+            }
+        } else {
+            int32_t l = I.h ? ((const int32_t*)I.h)[p] : 0;
+            for (int k = 0; k < n0; k += 8) {
+                int cc[8];
+                int32_t w[8];
+                uint32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = k + u < n0;
+                    cc[u] = edge_int<WM>(I.edge, ok ? t0 + k + u : 0, w[u]);
+                    cc[u] = ok ? cc[u] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = G.spw[cc[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    l += k + u < n0 ? (((v[u] >> a) & 1u) ? -w[u] : w[u]) : 0;
+            }
+            if (a < R) {
+                const int g = G.rung[cur * Rp + a];
+                const uint32_t* thr_row = I.thr + (size_t)g * I.TQ;
+                const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                const int32_t q = si > 0 ? l : -l;
+                if (q <= 0 || (q <= I.TQ && pt_word(rv, p & 3) < thr_row[q - 1])) {
+                    flip = true;
+                    d = 2 * (int64_t)q;
+                }
+            }
+        }
+    }
+    // the position's 32 decisions as one mask; lane 0 of the position writes the word
+    const uint64_t m = __ballot(flip);
+    if (live && a == 0) {
+        const uint32_t fm = (uint32_t)(m >> (tid & 32));
+        if (fm) G.spw[p] = own ^ fm;
+    }
+    block_slot_reduce(d, a, R, Rp, red, G.dacc, cur);
+}
+
+__global__ __launch_bounds__(256) void g_pack_bits(PtInstance I, GState G, int unpack) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < I.n; p += gridDim.x * blockDim.x) {
+        int8_t* row = G.sp + (size_t)p * I.Rp;
+        if (unpack) {
+            const uint32_t w = G.spw[p];
+            for (int a = 0; a < I.Rp; ++a)
+                row[a] = a < I.R ? (((w >> a) & 1u) ? (int8_t)-1 : (int8_t)1) : (int8_t)0;
+        } else {
+            uint32_t w = 0;
+            for (int a = 0; a < I.R; ++a) w |= (row[a] < 0 ? 1u : 0u) << a;
+            G.spw[p] = w;
+        }
+    }
+}
+
 // Colour step of the CSR engine: one Metropolis attempt per lane, so a launch has (colour size x
 // slots) independent lanes in flight to cover the dependent row-pointer -> edge -> neighbour-spin
 // loads.  Lane (r, a) updates rank r of colour c in slot a with word (p & 3) of the Philox block
@@ -460,7 +570,7 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
             const int slot = sh_slot;
             const int chunk = (I.n + gridDim.x - 1) / gridDim.x;
             const int p0 = blockIdx.x * chunk, p1 = min(I.n, p0 + chunk);
-            for (int p = p0 + tid; p < p1; p += blockDim.x) I.best_s[p] = G.sp[(size_t)p * Rp + slot];
+            for (int p = p0 + tid; p < p1; p += blockDim.x) I.best_s[p] = g_spin(G, p, Rp, slot);
         }
         return;
     }
@@ -492,7 +602,7 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
         const int chunk = (I.n + gridDim.x - 1) / gridDim.x;
         const int p0 = blockIdx.x * chunk, p1 = min(I.n, p0 + chunk);
         for (int p = p0 + tid; p < p1; p += blockDim.x)
-            I.best_s[p] = G.sp[(size_t)p * Rp + slot];
+            I.best_s[p] = g_spin(G, p, Rp, slot);
     }
     if (blockIdx.x != 0) return;
 
@@ -589,8 +699,20 @@ static void g_color_dispatch(const PtInstance& I, const GState& G, int c, int j,
     else g_color_step<PT_WM_F32, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
 }
 
+hipError_t pt_launch_g_pack(const PtInstance& I, const GState& G, bool unpack, hipStream_t s) {
+    const int grid = std::max(1, std::min(1024, (I.n + 255) / 256));
+    g_pack_bits<<<grid, 256, 0, s>>>(I, G, unpack ? 1 : 0);
+    return hipGetLastError();
+}
+
 hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j, int rl,
                              int block, int grid, hipStream_t s) {
+    if (G.spw) {
+        if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else g_color_bits<PT_WM_F32><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        return hipGetLastError();
+    }
     if (I.Rp % 64 == 0) g_color_dispatch<true>(I, G, c, j, rl, block, grid, s);
     else g_color_dispatch<false>(I, G, c, j, rl, block, grid, s);
     return hipGetLastError();
