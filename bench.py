@@ -592,7 +592,8 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "kernel": {"global": "g_color_step", "lattice": "g_lattice_step",
+        "kernel": {"global": "g_color_bits" if R <= 32 else "g_color_step",
+                   "lattice": "g_lattice_step",
                    "slab": "slab_sweep", "coop": "coop_run",
                    # the persistent slab engine keeps slabs as bits when L is a multiple of 32
                    "pslab": "pslab_bits_run" if lattice_side(model.n) % 32 == 0 else "pslab_run",

@@ -286,10 +286,12 @@ def parallel_tempering_gpu_batch(
     return_final: bool = False,
     record_trace: bool = False,
     force_float: bool = False,
+    block_threads: int = 0,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
-    result equals ``parallel_tempering_gpu`` on that instance alone."""
+    result equals ``parallel_tempering_gpu`` on that instance alone.  ``block_threads`` sets the
+    workgroup size (0: the library's choice)."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -302,7 +304,7 @@ def parallel_tempering_gpu_batch(
         ladders.append((temps, b))
         probs.append(encode_problem(model, None, force_float))
         prms.append(_params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seeds[i], b,
-                            "lds", 0, return_final, record_history, record_trace))
+                            "lds", block_threads, return_final, record_history, record_trace))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])
