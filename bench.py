@@ -493,6 +493,10 @@ def cpu_baseline(model, enc, R, beta, budget_s: float):
     run = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=sweeps, sweeps_per_round=1,
                                 beta=beta, n_threads=cores)
     attempts = R * model.n * sweeps
+    # the same chain on one thread, over a fifth of the budget
+    sweeps1 = int(max(1, min(sweeps, budget_s / 5 / (per_sweep * cores))))
+    one = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=sweeps1, sweeps_per_round=1,
+                                beta=beta, n_threads=1)
     return {
         "value": attempts / run["seconds"] / 1e9,
         "unit": "attempts/ns",
@@ -500,7 +504,22 @@ def cpu_baseline(model, enc, R, beta, budget_s: float):
         "kind": "port",
         "sample": f"same workload, {sweeps} sweeps+exchanges of all {R} replicas "
                   f"({attempts:.3g} attempts, {run['seconds']:.1f} s, OpenMP over replicas)",
+        "value_1_thread": R * model.n * sweeps1 / one["seconds"] / 1e9,
+        "sample_1_thread": f"{sweeps1} sweeps+exchanges on one thread ({one['seconds']:.1f} s)",
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or '' where that is not readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def main() -> int:
