@@ -156,6 +156,10 @@ def bench_c4(args) -> int:
                 "unit": "GB/s", "frac": round(B * attempts / secs / 1e9 / HBM_PEAK_GBS, 4),
                 "traffic": None, "kernel": "pt_lds_kernel", "bytes_per_attempt": B,
                 "attempts_per_launch": attempts, "avg_launch_us": round(secs * 1e6, 1)}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_c4_lds.json")
+    if os.path.exists(traffic_file):   # one launch runs the whole timed region
+        with open(traffic_file) as f:
+            roofline["traffic"] = round(json.load(f)["hbm_bytes_per_sweep"] * args.steps)
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
