@@ -3,6 +3,7 @@
 // libdriftpt.so.
 #pragma once
 #include <climits>
+#include <type_traits>
 
 #include "pt_engine.h"
 #include "pt_rules.h"
@@ -239,6 +240,98 @@ __device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
                                                uint32_t u, const uint32_t* thr_row, int TQ,
                                                float beta_f) {
     return site_update_row<WM, G>(p, a, Rp, sp, rp[p], rp[p + 1], edge, h, u, thr_row, TQ, beta_f);
+}
+
+// The attempts of one Philox quad in one slot: positions 4q .. 4q+3 that lie in colour range
+// [cs, ce), with word k of rv for position 4q+k.  Same-colour positions share no edge, so the
+// first G edges of all four rows are gathered before any spin of the quad changes: the row
+// pointers, the edge records and the neighbour spins each wait on memory once per quad instead of
+// once per attempt and edge.  Rows longer than G finish in G-edge chunks.  Each field is summed
+// from h in row order with the same operations as site_update_row, so the decisions are the same.
+template <int WM, int G>
+__device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int a, int Rp,
+                                               int8_t* sp, const int32_t* __restrict__ rp,
+                                               const void* __restrict__ edge, const void* h,
+                                               const pt_u4& rv, const uint32_t* thr_row, int TQ,
+                                               float beta_f) {
+    using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    int rb[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) rb[k] = rp[min(4 * q + k, n)];
+    bool on[4];
+    int t1[4];
+    LT l[4];
+    int8_t si[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = 4 * q + k, pc = min(p, n - 1);
+        on[k] = p >= cs && p < ce;
+        t1[k] = on[k] ? rb[k + 1] : rb[k];
+        l[k] = h ? ((const LT*)h)[pc] : (LT)0;
+        si[k] = sp[(size_t)pc * Rp + a];
+    }
+    {
+        int c[4][G];
+        LT w[4][G];
+        int8_t v[4][G];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const bool ok = rb[k] + u < t1[k];
+                if constexpr (WM == PT_WM_F32)
+                    c[k][u] = edge_float(edge, ok ? rb[k] + u : 0, w[k][u]);
+                else
+                    c[k][u] = edge_int<WM>(edge, ok ? rb[k] + u : 0, w[k][u]);
+                c[k][u] = ok ? c[k][u] : 0;
+            }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int u = 0; u < G; ++u) v[k][u] = sp[(size_t)c[k][u] * Rp + a];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const bool ok = rb[k] + u < t1[k];
+                if constexpr (WM == PT_WM_F32) {
+                    const float nl = v[k][u] > 0 ? l[k] + w[k][u] : l[k] - w[k][u];
+                    l[k] = ok ? nl : l[k];
+                } else {
+                    const int32_t term = v[k][u] > 0 ? w[k][u] : -w[k][u];
+                    l[k] += ok ? term : 0;
+                }
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (rb[k] + G < t1[k]) {
+            if constexpr (WM == PT_WM_F32)
+                l[k] = field_float<G>(a, Rp, sp, rb[k] + G, t1[k], edge, l[k]);
+            else
+                l[k] = field_int<WM, G>(a, Rp, sp, rb[k] + G, t1[k], edge, l[k]);
+        }
+    }
+    int64_t d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!on[k]) continue;
+        int8_t* own = sp + (size_t)(4 * q + k) * Rp + a;
+        if constexpr (WM == PT_WM_F32) {
+            const float qf = si[k] > 0 ? l[k] : -l[k];
+            if (pt_accept_float(beta_f, qf, pt_word(rv, k))) {
+                *own = (int8_t)-si[k];
+                d += pt_fix36(2.0f * qf);
+            }
+        } else {
+            const int32_t qi = si[k] > 0 ? l[k] : -l[k];
+            if (qi <= 0 || (qi <= TQ && pt_word(rv, k) < thr_row[qi - 1])) {
+                *own = (int8_t)-si[k];
+                d += 2 * (int64_t)qi;
+            }
+        }
+    }
+    return d;
 }
 
 // Energy contribution of position p in slot a at initialisation.  Integer modes return

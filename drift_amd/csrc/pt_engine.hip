@@ -106,13 +106,8 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                         for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
                             const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t, PT_TAG_METRO,
                                                        I.k0, I.k1);
-#pragma unroll
-                            for (int kk = 0; kk < 4; ++kk) {
-                                const int p = 4 * qq + kk;
-                                if (p < cs || p >= ce) continue;
-                                dsum += site_update<WM>(p, a, Rp, sp, I.row_ptr, I.edge, I.h,
-                                                        pt_word(rv, kk), thr_row, I.TQ, bf);
-                            }
+                            dsum += quad_update<WM, 4>(qq, cs, ce, n, a, Rp, sp, I.row_ptr, I.edge,
+                                                       I.h, rv, thr_row, I.TQ, bf);
                         }
                     }
                     __syncthreads();
