@@ -116,7 +116,23 @@ struct HostInst {
     std::vector<uint32_t> thr;
     int ldim = 0, lL = 0;           // periodic hypercubic lattice detected (see detect_lattice)
     std::vector<uint8_t> jbits;     // per site: bit d set when the bond in direction d is -1
+    int gq = 4;                     // PtInstance::gq
 };
+
+// First-chunk width of the LDS engine's quad gather: the smallest of 2, 3, 4 that covers at least
+// three rows in four, else 4.  A wider chunk costs issue slots on every shorter row; a row longer
+// than the chunk costs one more wait on memory.  Wider chunks would hold more than 128 registers
+// per lane (four rows of edge records and spins) and spill.
+int quad_chunk(const std::vector<int32_t>& row_ptr, int n) {
+    int cnt[5] = {0};
+    for (int p = 0; p < n; ++p) cnt[std::min(row_ptr[p + 1] - row_ptr[p], 4)]++;
+    int covered = 0;
+    for (int d = 0; d <= 4; ++d) {
+        covered += cnt[d];
+        if (d >= 2 && 4 * (int64_t)covered >= 3 * (int64_t)n) return d;
+    }
+    return 4;
+}
 
 int32_t greedy_color(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* cptr,
                      int32_t* cspins) {
@@ -279,6 +295,7 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
         total_abs += field;
     }
     if (!std::isfinite(maxfield)) return fail(DRIFT_PT_ERANGE, "non-finite coupling or field");
+    H.gq = quad_chunk(H.row_ptr, n);
     if (isf) {
         if (maxfield >= 32768.0 || total_abs >= 1.0e8)
             return fail(DRIFT_PT_ERANGE, "float couplings too large for the 2^-36 fixed-point energy");
@@ -492,6 +509,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.n = H.n; I.k = H.k; I.R = H.R; I.Rp = H.Rp; I.n_rounds = H.n_rounds; I.S = H.S;
         I.TQ = H.TQ; I.wm = H.wm;
         I.ldim = H.ldim; I.lL = H.lL;
+        I.gq = H.gq;
     }
     return 0;
 }

@@ -248,6 +248,24 @@ __device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
 // pointers, the edge records and the neighbour spins each wait on memory once per quad instead of
 // once per attempt and edge.  Rows longer than G finish in G-edge chunks.  Each field is summed
 // from h in row order with the same operations as site_update_row, so the decisions are the same.
+// Edge t read through a global-address-space pointer: a global load waits on its own counter,
+// while a flat load (what a pointer read from memory compiles to) also holds up the wait for LDS
+// reads issued after it.
+#define PT_GLOBAL __attribute__((address_space(1)))
+template <int WM, typename LT>
+__device__ __forceinline__ int edge_g(const void* e, int t, LT& wt) {
+    if constexpr (WM == PT_WM_I8) {
+        const uint32_t v = ((const PT_GLOBAL uint32_t*)e)[t];
+        wt = (LT)(int8_t)(v & 0xffu);
+        return (int)(v >> 8);
+    } else {
+        const uint64_t v = ((const PT_GLOBAL uint64_t*)e)[t];   // {position, weight bits}
+        if constexpr (WM == PT_WM_F32) wt = __uint_as_float((uint32_t)(v >> 32));
+        else wt = (LT)(int32_t)(uint32_t)(v >> 32);
+        return (int)(uint32_t)v;
+    }
+}
+
 template <int WM, int G>
 __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int a, int Rp,
                                                int8_t* sp, const int32_t* __restrict__ rp,
@@ -257,7 +275,7 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
     using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
     int rb[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) rb[k] = rp[min(4 * q + k, n)];
+    for (int k = 0; k < 5; ++k) rb[k] = ((const PT_GLOBAL int32_t*)rp)[min(4 * q + k, n)];
     bool on[4];
     int t1[4];
     LT l[4];
@@ -267,7 +285,7 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
         const int p = 4 * q + k, pc = min(p, n - 1);
         on[k] = p >= cs && p < ce;
         t1[k] = on[k] ? rb[k + 1] : rb[k];
-        l[k] = h ? ((const LT*)h)[pc] : (LT)0;
+        l[k] = h ? ((const PT_GLOBAL LT*)h)[pc] : (LT)0;
         si[k] = sp[(size_t)pc * Rp + a];
     }
     {
@@ -279,10 +297,7 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 const bool ok = rb[k] + u < t1[k];
-                if constexpr (WM == PT_WM_F32)
-                    c[k][u] = edge_float(edge, ok ? rb[k] + u : 0, w[k][u]);
-                else
-                    c[k][u] = edge_int<WM>(edge, ok ? rb[k] + u : 0, w[k][u]);
+                c[k][u] = edge_g<WM>(edge, ok ? rb[k] + u : 0, w[k][u]);
                 c[k][u] = ok ? c[k][u] : 0;
             }
 #pragma unroll

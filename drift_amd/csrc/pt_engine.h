@@ -43,6 +43,8 @@ struct PtInstance {
     int32_t n, k, R, Rp;      // Rp: replica slots padded (power of two <= 64, else multiple of 64)
     int32_t n_rounds, S, TQ, wm;
     int32_t ldim, lL;         // lattice dimension and side (0 when not a lattice)
+    int32_t gq;               // LDS engine: edges of each row a quad gathers in its first chunk
+                              // (2, 3 or 4; longer rows finish in chunks of the same size)
 };
 
 #define PT_DACC_SPLIT 32

@@ -103,11 +103,19 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                     const int cs = I.color_ptr[c], ce = I.color_ptr[c + 1];
                     if (live && ce > cs) {
                         const int q1 = (ce - 1) >> 2;
-                        for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
-                            const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t, PT_TAG_METRO,
-                                                       I.k0, I.k1);
-                            dsum += quad_update<WM, 4>(qq, cs, ce, n, a, Rp, sp, I.row_ptr, I.edge,
-                                                       I.h, rv, thr_row, I.TQ, bf);
+                        auto quads = [&](auto gw) {
+                            for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
+                                const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t,
+                                                           PT_TAG_METRO, I.k0, I.k1);
+                                dsum += quad_update<WM, decltype(gw)::value>(
+                                    qq, cs, ce, n, a, Rp, sp, I.row_ptr, I.edge, I.h, rv, thr_row,
+                                    I.TQ, bf);
+                            }
+                        };
+                        switch (I.gq) {
+                            case 2: quads(std::integral_constant<int, 2>{}); break;
+                            case 3: quads(std::integral_constant<int, 3>{}); break;
+                            default: quads(std::integral_constant<int, 4>{}); break;
                         }
                     }
                     __syncthreads();

@@ -103,6 +103,44 @@ def test_replica_counts(R):
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="coop")
 
 
+def _degree_profile_model(kind):
+    """Graphs whose row lengths pick each first-chunk width of the LDS engine's quad gather
+    (2, 3, 4) and send long rows through the chunked remainder."""
+    rng = np.random.default_rng(11)
+    if kind == "ring_int":            # every row 2 long, two isolated spins
+        n = 50
+        J = np.zeros((n, n))
+        for i in range(n - 2):
+            j = (i + 1) % (n - 2)
+            J[i, j] = J[j, i] = rng.choice([-1.0, 1.0])
+        return IsingModel(J, np.zeros(n))
+    if kind == "cubic_float":         # 3-regular, float couplings and field
+        return random_regular_gaussian(60, seed=5)
+    if kind == "er_int_field":        # mean degree ~5: width 4 plus remainder chunks
+        n = 80
+        J = np.zeros((n, n))
+        iu = np.triu_indices(n, 1)
+        on = rng.random(len(iu[0])) < 0.065
+        w = rng.choice([-3.0, -2.0, -1.0, 1.0, 2.0, 3.0], size=len(iu[0]))
+        J[iu[0][on], iu[1][on]] = w[on]
+        J = J + J.T
+        return IsingModel(J, rng.integers(-2, 3, size=n).astype(float))
+    # hub_float: a chain of leaves (rows of 3) plus one hub row of n - 1
+    n = 40
+    J = np.zeros((n, n))
+    for i in range(1, n - 1):
+        J[i, i + 1] = J[i + 1, i] = rng.normal()
+    J[0, 1:] = J[1:, 0] = rng.normal(size=n - 1)
+    return IsingModel(J.astype(np.float32).astype(np.float64), np.zeros(n))
+
+
+@pytest.mark.parametrize("kind", ["ring_int", "cubic_float", "er_int_field", "hub_float"])
+def test_lds_quad_chunk_widths(kind):
+    _announce(f"lds quad chunk {kind}")
+    model = _degree_profile_model(kind)
+    compare(model, R=8, n_rounds=20, S=2, seed=21, engine="lds")
+
+
 @pytest.mark.parametrize("block", [64, 128, 512, 1024])
 def test_tile_sizes_do_not_change_results(block):
     _announce(f"block={block}")
