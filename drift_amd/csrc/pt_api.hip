@@ -438,10 +438,25 @@ struct Packer {
     size_t add(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
 };
 
+// The LDS engine's first-chunk records (PtInstance::ell): gq records per position, row-major,
+// zero-padded, positions padded to a multiple of 4.
+std::vector<uint32_t> build_ell(const HostInst& H) {
+    const int G = H.gq, ew = H.wm == PT_WM_I8 ? 1 : 2;
+    const size_t n4 = ((size_t)H.n + 3) & ~(size_t)3;
+    std::vector<uint32_t> ell(n4 * G * ew, 0u);
+    for (int p = 0; p < H.n; ++p) {
+        const int t0 = H.row_ptr[p], deg = H.row_ptr[p + 1] - t0;
+        for (int u = 0; u < std::min(deg, G); ++u)
+            for (int x = 0; x < ew; ++x)
+                ell[((size_t)p * G + u) * ew + x] = H.edge[(size_t)(t0 + u) * ew + x];
+    }
+    return ell;
+}
+
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
-                bool force_final) {
+                bool force_final, bool with_ell) {
     const size_t m = hs.size();
-    struct Off { size_t rp, edge, cptr, orig, h, beta, thr, jb; };
+    struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb; };
     std::vector<Off> off(m);
     Packer in;
     for (size_t i = 0; i < m; ++i) {
@@ -449,6 +464,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         Off& o = off[i];
         o.rp = in.add(H.row_ptr);
         o.edge = in.add(H.edge);
+        o.ell = with_ell ? in.add(build_ell(H)) : SIZE_MAX;
         o.cptr = in.add(H.color_ptr);
         o.orig = in.add(H.orig);
         o.h = !H.has_h ? SIZE_MAX : (H.wm == PT_WM_F32 ? in.add(H.hf) : in.add(H.h32));
@@ -492,6 +508,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         PtInstance& I = insts[i];
         I.row_ptr = (const int32_t*)(base + o.rp);
         I.edge = base + o.edge;
+        I.ell = o.ell == SIZE_MAX ? nullptr : base + o.ell;
         I.color_ptr = (const int32_t*)(base + o.cptr);
         I.orig = (const int32_t*)(base + o.orig);
         I.h = o.h == SIZE_MAX ? nullptr : base + o.h;
@@ -518,7 +535,7 @@ int upload_instance(DevArena& A, const HostInst& H, PtInstance& I, bool want_fin
     (void)want_hist;   // history follows H.flags
     std::vector<HostInst> one{H};
     std::vector<PtInstance> out;
-    int rc = upload_many(A, one, out, want_final);
+    int rc = upload_many(A, one, out, want_final, false);
     if (rc) return rc;
     I = out[0];
     return 0;
@@ -594,7 +611,7 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     std::vector<PtInstance> insts((size_t)n_inst);
     size_t lds = 0;
     int maxRp = 1;
-    int rc = upload_many(A, hs, insts, false);
+    int rc = upload_many(A, hs, insts, false, true);
     if (rc) return rc;
     for (int i = 0; i < n_inst; ++i) {
         lds = std::max(lds, pt_lds_bytes(hs[i].n, hs[i].Rp));

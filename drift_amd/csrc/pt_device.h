@@ -243,88 +243,85 @@ __device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
 }
 
 // The attempts of one Philox quad in one slot: positions 4q .. 4q+3 that lie in colour range
-// [cs, ce), with word k of rv for position 4q+k.  Same-colour positions share no edge, so the
-// first G edges of all four rows are gathered before any spin of the quad changes: the row
-// pointers, the edge records and the neighbour spins each wait on memory once per quad instead of
-// once per attempt and edge.  Rows longer than G finish in G-edge chunks.  Each field is summed
-// from h in row order with the same operations as site_update_row, so the decisions are the same.
-// Edge t read through a global-address-space pointer: a global load waits on its own counter,
-// while a flat load (what a pointer read from memory compiles to) also holds up the wait for LDS
-// reads issued after it.
+// [cs, ce), with word k of rv for position 4q+k.  The first G edge records of every row are also
+// stored row-major in ell (zero records pad short rows: position 0, weight 0), so the quad's first
+// chunks are 4G contiguous records read with wide loads and no per-edge bounds test.  Same-colour
+// positions share no edge, so all four rows are gathered before any spin of the quad changes: the
+// records and the neighbour spins each wait on memory once per quad.  Rows longer than G finish
+// from the CSR in G-edge chunks.  Each field is summed from h in row order with the operations of
+// site_update_row; a padding record adds +0 or -0, which changes at most the sign of a zero field,
+// and a zero field is accepted with zero energy change either way, so the decisions are the same.
+// Positions of the quad outside [cs, ce) are gathered too (valid addresses) and then left alone.
+// Loads through global-address-space pointers wait on their own counter, while flat loads (what a
+// pointer read from memory compiles to) also hold up the waits for LDS reads issued after them.
 #define PT_GLOBAL __attribute__((address_space(1)))
-template <int WM, typename LT>
-__device__ __forceinline__ int edge_g(const void* e, int t, LT& wt) {
-    if constexpr (WM == PT_WM_I8) {
-        const uint32_t v = ((const PT_GLOBAL uint32_t*)e)[t];
-        wt = (LT)(int8_t)(v & 0xffu);
-        return (int)(v >> 8);
-    } else {
-        const uint64_t v = ((const PT_GLOBAL uint64_t*)e)[t];   // {position, weight bits}
-        if constexpr (WM == PT_WM_F32) wt = __uint_as_float((uint32_t)(v >> 32));
-        else wt = (LT)(int32_t)(uint32_t)(v >> 32);
-        return (int)(uint32_t)v;
-    }
-}
+typedef uint32_t pt_v4u __attribute__((ext_vector_type(4)));
 
 template <int WM, int G>
 __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int a, int Rp,
                                                int8_t* sp, const int32_t* __restrict__ rp,
-                                               const void* __restrict__ edge, const void* h,
+                                               const void* __restrict__ edge,
+                                               const void* __restrict__ ell, const void* h,
                                                const pt_u4& rv, const uint32_t* thr_row, int TQ,
                                                float beta_f) {
     using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
+    constexpr int NW = 4 * G * EW;                // words of the quad's first chunks
+    uint32_t ev[NW];
+    {
+        const PT_GLOBAL pt_v4u* src = (const PT_GLOBAL pt_v4u*)ell + (size_t)q * (NW / 4);
+#pragma unroll
+        for (int i = 0; i < NW / 4; ++i) {
+            const pt_v4u x = src[i];
+            ev[4 * i] = x.x;
+            ev[4 * i + 1] = x.y;
+            ev[4 * i + 2] = x.z;
+            ev[4 * i + 3] = x.w;
+        }
+    }
     int rb[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) rb[k] = ((const PT_GLOBAL int32_t*)rp)[min(4 * q + k, n)];
-    bool on[4];
-    int t1[4];
     LT l[4];
     int8_t si[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int p = 4 * q + k, pc = min(p, n - 1);
-        on[k] = p >= cs && p < ce;
-        t1[k] = on[k] ? rb[k + 1] : rb[k];
+        const int pc = min(4 * q + k, n - 1);
         l[k] = h ? ((const PT_GLOBAL LT*)h)[pc] : (LT)0;
         si[k] = sp[(size_t)pc * Rp + a];
     }
     {
-        int c[4][G];
-        LT w[4][G];
-        int8_t v[4][G];
+        int8_t v[4 * G];
+#pragma unroll
+        for (int e = 0; e < 4 * G; ++e) {
+            const int c = WM == PT_WM_I8 ? (int)(ev[e] >> 8) : (int)ev[EW * e];
+            v[e] = sp[(size_t)c * Rp + a];
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int u = 0; u < G; ++u) {
-                const bool ok = rb[k] + u < t1[k];
-                c[k][u] = edge_g<WM>(edge, ok ? rb[k] + u : 0, w[k][u]);
-                c[k][u] = ok ? c[k][u] : 0;
-            }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int u = 0; u < G; ++u) v[k][u] = sp[(size_t)c[k][u] * Rp + a];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const bool ok = rb[k] + u < t1[k];
+                const int e = k * G + u;
                 if constexpr (WM == PT_WM_F32) {
-                    const float nl = v[k][u] > 0 ? l[k] + w[k][u] : l[k] - w[k][u];
-                    l[k] = ok ? nl : l[k];
+                    const float w = __uint_as_float(ev[EW * e + 1]);
+                    l[k] = v[e] > 0 ? l[k] + w : l[k] - w;
                 } else {
-                    const int32_t term = v[k][u] > 0 ? w[k][u] : -w[k][u];
-                    l[k] += ok ? term : 0;
+                    const int32_t w = WM == PT_WM_I8 ? (int32_t)(int8_t)(ev[e] & 0xffu)
+                                                     : (int32_t)ev[EW * e + 1];
+                    l[k] += v[e] > 0 ? w : -w;
                 }
             }
     }
+    bool on[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (rb[k] + G < t1[k]) {
+        const int p = 4 * q + k;
+        on[k] = p >= cs && p < ce;
+        if (on[k] && rb[k] + G < rb[k + 1]) {
             if constexpr (WM == PT_WM_F32)
-                l[k] = field_float<G>(a, Rp, sp, rb[k] + G, t1[k], edge, l[k]);
+                l[k] = field_float<G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
             else
-                l[k] = field_int<WM, G>(a, Rp, sp, rb[k] + G, t1[k], edge, l[k]);
+                l[k] = field_int<WM, G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
         }
     }
     int64_t d = 0;

@@ -26,6 +26,9 @@ struct PtInstance {
     const void* edge;         // nnz edge records in row order: int8 weights -> uint32
                               // (position << 8 | weight byte); int32 / float -> uint2
                               // {position, weight bits}.  One load per neighbour.
+    const void* ell;          // LDS engine: the first gq edge records of each row, row-major,
+                              // zero records padding short rows and the rows up to a multiple of
+                              // 4 positions (null on the other engines)
     const void* h;            // n (position order) int32 / float, or null
     const int32_t* color_ptr; // k + 1
     const int32_t* orig;      // n
@@ -43,8 +46,8 @@ struct PtInstance {
     int32_t n, k, R, Rp;      // Rp: replica slots padded (power of two <= 64, else multiple of 64)
     int32_t n_rounds, S, TQ, wm;
     int32_t ldim, lL;         // lattice dimension and side (0 when not a lattice)
-    int32_t gq;               // LDS engine: edges of each row a quad gathers in its first chunk
-                              // (2, 3 or 4; longer rows finish in chunks of the same size)
+    int32_t gq;               // LDS engine: edge records per row in ell (2, 3 or 4; longer rows
+                              // finish from the CSR in chunks of the same size)
 };
 
 #define PT_DACC_SPLIT 32
