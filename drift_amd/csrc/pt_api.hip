@@ -117,6 +117,7 @@ struct HostInst {
     int ldim = 0, lL = 0;           // periodic hypercubic lattice detected (see detect_lattice)
     std::vector<uint8_t> jbits;     // per site: bit d set when the bond in direction d is -1
     int gq = 4;                     // PtInstance::gq
+    int maxdeg = 0;                 // longest row
 };
 
 // First-chunk width of the LDS engine's quad gather: the smallest of 2, 3, 4 that covers at least
@@ -296,6 +297,7 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     }
     if (!std::isfinite(maxfield)) return fail(DRIFT_PT_ERANGE, "non-finite coupling or field");
     H.gq = quad_chunk(H.row_ptr, n);
+    for (int p = 0; p < n; ++p) H.maxdeg = std::max(H.maxdeg, H.row_ptr[p + 1] - H.row_ptr[p]);
     if (isf) {
         if (maxfield >= 32768.0 || total_abs >= 1.0e8)
             return fail(DRIFT_PT_ERANGE, "float couplings too large for the 2^-36 fixed-point energy");
@@ -439,18 +441,38 @@ struct Packer {
 };
 
 // The LDS engine's first-chunk records (PtInstance::ell): gq records per position, row-major,
-// zero-padded, positions padded to a multiple of 4.
+// zero-padded, positions padded to a multiple of 4, the neighbour stored as its LDS spin offset
+// position * Rp (below 2^24: the spins fit one workgroup's LDS).
 std::vector<uint32_t> build_ell(const HostInst& H) {
-    const int G = H.gq, ew = H.wm == PT_WM_I8 ? 1 : 2;
+    const int G = H.gq;
+    const bool i8 = H.wm == PT_WM_I8;
+    const int ew = i8 ? 1 : 2;
     const size_t n4 = ((size_t)H.n + 3) & ~(size_t)3;
     std::vector<uint32_t> ell(n4 * G * ew, 0u);
     for (int p = 0; p < H.n; ++p) {
         const int t0 = H.row_ptr[p], deg = H.row_ptr[p + 1] - t0;
-        for (int u = 0; u < std::min(deg, G); ++u)
-            for (int x = 0; x < ew; ++x)
-                ell[((size_t)p * G + u) * ew + x] = H.edge[(size_t)(t0 + u) * ew + x];
+        for (int u = 0; u < std::min(deg, G); ++u) {
+            uint32_t* dst = &ell[((size_t)p * G + u) * ew];
+            const uint32_t* src = &H.edge[(size_t)(t0 + u) * ew];
+            if (i8) {
+                const uint32_t off = (src[0] >> 8) * (uint32_t)H.Rp;
+                dst[0] = (off << 8) | (src[0] & 0xffu);
+            } else {
+                dst[0] = src[0] * (uint32_t)H.Rp;
+                dst[1] = src[1];
+            }
+        }
     }
     return ell;
+}
+
+// Field values zero-padded to a multiple of 4 positions (the LDS engine reads a quad's four at
+// once).
+template <class T>
+std::vector<T> pad4(const std::vector<T>& v) {
+    std::vector<T> out((v.size() + 3) & ~(size_t)3, (T)0);
+    std::copy(v.begin(), v.end(), out.begin());
+    return out;
 }
 
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
@@ -467,7 +489,9 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         o.ell = with_ell ? in.add(build_ell(H)) : SIZE_MAX;
         o.cptr = in.add(H.color_ptr);
         o.orig = in.add(H.orig);
-        o.h = !H.has_h ? SIZE_MAX : (H.wm == PT_WM_F32 ? in.add(H.hf) : in.add(H.h32));
+        o.h = !H.has_h ? SIZE_MAX
+              : with_ell ? (H.wm == PT_WM_F32 ? in.add(pad4(H.hf)) : in.add(pad4(H.h32)))
+                         : (H.wm == PT_WM_F32 ? in.add(H.hf) : in.add(H.h32));
         o.beta = in.add(H.beta);
         o.thr = H.wm != PT_WM_F32 ? in.add(H.thr) : SIZE_MAX;
         o.jb = H.ldim ? in.add(H.jbits) : SIZE_MAX;
@@ -527,6 +551,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.TQ = H.TQ; I.wm = H.wm;
         I.ldim = H.ldim; I.lL = H.lL;
         I.gq = H.gq;
+        I.ell_rest = H.maxdeg > H.gq ? 1 : 0;
     }
     return 0;
 }

@@ -244,14 +244,17 @@ __device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
 
 // The attempts of one Philox quad in one slot: positions 4q .. 4q+3 that lie in colour range
 // [cs, ce), with word k of rv for position 4q+k.  The first G edge records of every row are also
-// stored row-major in ell (zero records pad short rows: position 0, weight 0), so the quad's first
-// chunks are 4G contiguous records read with wide loads and no per-edge bounds test.  Same-colour
-// positions share no edge, so all four rows are gathered before any spin of the quad changes: the
-// records and the neighbour spins each wait on memory once per quad.  Rows longer than G finish
-// from the CSR in G-edge chunks.  Each field is summed from h in row order with the operations of
-// site_update_row; a padding record adds +0 or -0, which changes at most the sign of a zero field,
-// and a zero field is accepted with zero energy change either way, so the decisions are the same.
-// Positions of the quad outside [cs, ce) are gathered too (valid addresses) and then left alone.
+// stored row-major in ell, with the neighbour as its LDS offset (position * Rp) and zero records
+// padding short rows (offset 0, weight 0), so the quad's first chunks are 4G contiguous records
+// read with wide loads, no per-edge bounds test and no address multiply; h is padded to a multiple
+// of 4 positions and read as one 16-byte load.  Same-colour positions share no edge, so all four
+// rows are gathered before any spin of the quad changes: the records and the neighbour spins each
+// wait on memory once per quad.  Only when some row is longer than G (rest) are the row pointers
+// read and the longer rows finished from the CSR in G-edge chunks.  Each field is summed from h in
+// row order with the operations of site_update_row; a padding record adds +0 or -0, which changes
+// at most the sign of a zero field, and a zero field is accepted with zero energy change either
+// way, so the decisions are the same.  Positions of the quad outside [cs, ce) are gathered too
+// (valid addresses) and then left alone.
 // Loads through global-address-space pointers wait on their own counter, while flat loads (what a
 // pointer read from memory compiles to) also hold up the waits for LDS reads issued after them.
 #define PT_GLOBAL __attribute__((address_space(1)))
@@ -262,8 +265,8 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
                                                int8_t* sp, const int32_t* __restrict__ rp,
                                                const void* __restrict__ edge,
                                                const void* __restrict__ ell, const void* h,
-                                               const pt_u4& rv, const uint32_t* thr_row, int TQ,
-                                               float beta_f) {
+                                               bool rest, const pt_u4& rv,
+                                               const uint32_t* thr_row, int TQ, float beta_f) {
     using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
     constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
     constexpr int NW = 4 * G * EW;                // words of the quad's first chunks
@@ -279,24 +282,24 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
             ev[4 * i + 3] = x.w;
         }
     }
-    int rb[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) rb[k] = ((const PT_GLOBAL int32_t*)rp)[min(4 * q + k, n)];
-    LT l[4];
+    LT l[4] = {(LT)0, (LT)0, (LT)0, (LT)0};
+    if (h) {
+        const pt_v4u hv = ((const PT_GLOBAL pt_v4u*)h)[q];
+        if constexpr (WM == PT_WM_F32) {
+            l[0] = __uint_as_float(hv.x); l[1] = __uint_as_float(hv.y);
+            l[2] = __uint_as_float(hv.z); l[3] = __uint_as_float(hv.w);
+        } else {
+            l[0] = (int32_t)hv.x; l[1] = (int32_t)hv.y; l[2] = (int32_t)hv.z; l[3] = (int32_t)hv.w;
+        }
+    }
     int8_t si[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int pc = min(4 * q + k, n - 1);
-        l[k] = h ? ((const PT_GLOBAL LT*)h)[pc] : (LT)0;
-        si[k] = sp[(size_t)pc * Rp + a];
-    }
+    for (int k = 0; k < 4; ++k) si[k] = sp[(size_t)min(4 * q + k, n - 1) * Rp + a];
     {
         int8_t v[4 * G];
 #pragma unroll
-        for (int e = 0; e < 4 * G; ++e) {
-            const int c = WM == PT_WM_I8 ? (int)(ev[e] >> 8) : (int)ev[EW * e];
-            v[e] = sp[(size_t)c * Rp + a];
-        }
+        for (int e = 0; e < 4 * G; ++e)
+            v[e] = sp[(WM == PT_WM_I8 ? (ev[e] >> 8) : ev[EW * e]) + a];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -314,14 +317,19 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
     }
     bool on[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int p = 4 * q + k;
-        on[k] = p >= cs && p < ce;
-        if (on[k] && rb[k] + G < rb[k + 1]) {
-            if constexpr (WM == PT_WM_F32)
-                l[k] = field_float<G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
-            else
-                l[k] = field_int<WM, G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
+    for (int k = 0; k < 4; ++k) on[k] = 4 * q + k >= cs && 4 * q + k < ce;
+    if (rest) {
+        int rb[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rb[k] = ((const PT_GLOBAL int32_t*)rp)[min(4 * q + k, n)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (on[k] && rb[k] + G < rb[k + 1]) {
+                if constexpr (WM == PT_WM_F32)
+                    l[k] = field_float<G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
+                else
+                    l[k] = field_int<WM, G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
+            }
         }
     }
     int64_t d = 0;

@@ -28,8 +28,10 @@ struct PtInstance {
                               // {position, weight bits}.  One load per neighbour.
     const void* ell;          // LDS engine: the first gq edge records of each row, row-major,
                               // zero records padding short rows and the rows up to a multiple of
-                              // 4 positions (null on the other engines)
-    const void* h;            // n (position order) int32 / float, or null
+                              // 4 positions; the neighbour field holds position * Rp, the offset
+                              // of its spins in LDS (null on the other engines)
+    const void* h;            // n (position order) int32 / float, or null; zero-padded to a
+                              // multiple of 4 positions where ell is set
     const int32_t* color_ptr; // k + 1
     const int32_t* orig;      // n
     const double* beta;       // R, rung 0 coldest
@@ -48,6 +50,7 @@ struct PtInstance {
     int32_t ldim, lL;         // lattice dimension and side (0 when not a lattice)
     int32_t gq;               // LDS engine: edge records per row in ell (2, 3 or 4; longer rows
                               // finish from the CSR in chunks of the same size)
+    int32_t ell_rest;         // some row is longer than gq
 };
 
 #define PT_DACC_SPLIT 32

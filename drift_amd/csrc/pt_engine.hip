@@ -108,8 +108,8 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                                 const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t,
                                                            PT_TAG_METRO, I.k0, I.k1);
                                 dsum += quad_update<WM, decltype(gw)::value>(
-                                    qq, cs, ce, n, a, Rp, sp, I.row_ptr, I.edge, I.ell, I.h, rv,
-                                    thr_row, I.TQ, bf);
+                                    qq, cs, ce, n, a, Rp, sp, I.row_ptr, I.edge, I.ell, I.h,
+                                    I.ell_rest != 0, rv, thr_row, I.TQ, bf);
                             }
                         };
                         switch (I.gq) {
