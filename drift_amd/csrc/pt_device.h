@@ -292,9 +292,11 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
             l[0] = (int32_t)hv.x; l[1] = (int32_t)hv.y; l[2] = (int32_t)hv.z; l[3] = (int32_t)hv.w;
         }
     }
+    // positions past n (the last quad) read LDS words that follow the spins and are not stored
+    int8_t* own = sp + (uint32_t)(4 * q * Rp + a);
     int8_t si[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) si[k] = sp[(size_t)min(4 * q + k, n - 1) * Rp + a];
+    for (int k = 0; k < 4; ++k) si[k] = own[k * Rp];
     {
         int8_t v[4 * G];
 #pragma unroll
@@ -336,17 +338,16 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!on[k]) continue;
-        int8_t* own = sp + (size_t)(4 * q + k) * Rp + a;
         if constexpr (WM == PT_WM_F32) {
             const float qf = si[k] > 0 ? l[k] : -l[k];
             if (pt_accept_float(beta_f, qf, pt_word(rv, k))) {
-                *own = (int8_t)-si[k];
+                own[k * Rp] = (int8_t)-si[k];
                 d += pt_fix36(2.0f * qf);
             }
         } else {
             const int32_t qi = si[k] > 0 ? l[k] : -l[k];
             if (qi <= 0 || (qi <= TQ && pt_word(rv, k) < thr_row[qi - 1])) {
-                *own = (int8_t)-si[k];
+                own[k * Rp] = (int8_t)-si[k];
                 d += 2 * (int64_t)qi;
             }
         }
