@@ -440,11 +440,11 @@ struct Packer {
     size_t add(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
 };
 
-// The LDS engine's first-chunk records (PtInstance::ell): gq records per position, row-major,
-// zero-padded, positions padded to a multiple of 4, the neighbour stored as its LDS spin offset
-// position * Rp (below 2^24: the spins fit one workgroup's LDS).
-std::vector<uint32_t> build_ell(const HostInst& H) {
-    const int G = H.gq;
+// First-chunk records (PtInstance::ell): G records per position, row-major, zero-padded,
+// positions padded to a multiple of 4, the neighbour stored as position * mult: its LDS spin offset
+// on the LDS engine (mult = Rp; below 2^24 since the spins fit one workgroup's LDS), its spin word
+// on the word-per-position CSR engine (mult = 1; int8 records need n < 2^24, as the CSR does).
+std::vector<uint32_t> build_ell(const HostInst& H, int G, uint32_t mult) {
     const bool i8 = H.wm == PT_WM_I8;
     const int ew = i8 ? 1 : 2;
     const size_t n4 = ((size_t)H.n + 3) & ~(size_t)3;
@@ -455,10 +455,10 @@ std::vector<uint32_t> build_ell(const HostInst& H) {
             uint32_t* dst = &ell[((size_t)p * G + u) * ew];
             const uint32_t* src = &H.edge[(size_t)(t0 + u) * ew];
             if (i8) {
-                const uint32_t off = (src[0] >> 8) * (uint32_t)H.Rp;
+                const uint32_t off = (src[0] >> 8) * mult;
                 dst[0] = (off << 8) | (src[0] & 0xffu);
             } else {
-                dst[0] = src[0] * (uint32_t)H.Rp;
+                dst[0] = src[0] * mult;
                 dst[1] = src[1];
             }
         }
@@ -486,7 +486,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         Off& o = off[i];
         o.rp = in.add(H.row_ptr);
         o.edge = in.add(H.edge);
-        o.ell = with_ell ? in.add(build_ell(H)) : SIZE_MAX;
+        o.ell = with_ell ? in.add(build_ell(H, H.gq, (uint32_t)H.Rp)) : SIZE_MAX;
         o.cptr = in.add(H.color_ptr);
         o.orig = in.add(H.orig);
         o.h = !H.has_h ? SIZE_MAX
@@ -764,6 +764,14 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         PT_HIP(A.alloc(&spw, (size_t)H.n));
         G.spw = spw;
         PT_HIP(pt_launch_g_pack(I, G, false, sg.s));
+        // first-chunk records by spin word: 4 per row when that covers three rows in four, else 8
+        int short_rows = 0;
+        for (int p = 0; p < H.n; ++p) short_rows += H.row_ptr[p + 1] - H.row_ptr[p] <= 4;
+        I.gq = 4 * (int64_t)short_rows >= 3 * (int64_t)H.n ? 4 : 8;
+        I.ell_rest = H.maxdeg > I.gq ? 1 : 0;
+        uint32_t* ell = nullptr;
+        PT_HIP(A.upload(&ell, build_ell(H, I.gq, 1u)));
+        I.ell = ell;
     }
     // optional per-launch timing of the colour-step kernel (the dominant kernel)
     const bool time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;

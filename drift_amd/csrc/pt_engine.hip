@@ -286,8 +286,13 @@ __device__ __forceinline__ int8_t g_spin(const GState& G, int p, int Rp, int a) 
 // Colour step of the CSR engine with one 32-bit word per position (R <= 32, Rp = 32): the 32 lanes
 // of a position read each neighbour's word once (a broadcast), take their own bit, decide as
 // g_color_step does, and combine their flips with a ballot; lane 0 of the position stores the
-// word.  The whole spin array is n words, so the instance stays in each XCD's L2.
-template <int WM>
+// word.  The whole spin array is n words, so the instance stays in each XCD's L2.  The first GW
+// edge records of the row come from I.ell (row-major, zero-padded, neighbour as its word index),
+// so the neighbour words are one dependent load away; row pointers are read, alongside, only when
+// some row is longer than GW, and such rows finish from the CSR in GW-edge chunks.  A padding record
+// adds +-0 to the field, which changes at most the sign of a zero field (accepted with zero energy
+// change either way).
+template <int WM, int GW>
 __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int c, int j, int rl) {
     __shared__ long long red[1024];
     const int rnd = *G.round_base + rl, cur = rnd & 1;
@@ -295,6 +300,9 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
     for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
     __syncthreads();
+    using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
+    constexpr int NW = GW * EW;
     const int a = tid & 31;
     const int p = I.color_ptr[c] + blockIdx.x * (int)(blockDim.x >> 5) + (tid >> 5);
     const bool live = p < I.color_ptr[c + 1];
@@ -302,60 +310,74 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
     bool flip = false;
     int64_t d = 0;
     if (live) {
+        uint32_t ev[NW];
+        const pt_v4u* src = (const pt_v4u*)I.ell + (size_t)p * (NW / 4);
+#pragma unroll
+        for (int i = 0; i < NW / 4; ++i) {
+            const pt_v4u x = src[i];
+            ev[4 * i] = x.x;
+            ev[4 * i + 1] = x.y;
+            ev[4 * i + 2] = x.z;
+            ev[4 * i + 3] = x.w;
+        }
         own = G.spw[p];
-        const int t0 = I.row_ptr[p], n0 = I.row_ptr[p + 1] - t0;
-        const int8_t si = ((own >> a) & 1u) ? (int8_t)-1 : (int8_t)1;
-        if constexpr (WM == PT_WM_F32) {
-            float l = I.h ? ((const float*)I.h)[p] : 0.0f;
-            for (int k = 0; k < n0; k += 8) {
-                int cc[8];
-                float w[8];
-                uint32_t v[8];
+        int t0 = 0, t1 = 0;
+        if (I.ell_rest) {
+            t0 = I.row_ptr[p];
+            t1 = I.row_ptr[p + 1];
+        }
+        LT l = I.h ? ((const LT*)I.h)[p] : (LT)0;
+        uint32_t v[GW];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const bool ok = k + u < n0;
-                    cc[u] = edge_float(I.edge, ok ? t0 + k + u : 0, w[u]);
-                    cc[u] = ok ? cc[u] : 0;
-                }
+        for (int u = 0; u < GW; ++u) v[u] = G.spw[WM == PT_WM_I8 ? (ev[u] >> 8) : ev[EW * u]];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = G.spw[cc[u]];
+        for (int u = 0; u < GW; ++u) {
+            const bool neg = (v[u] >> a) & 1u;
+            if constexpr (WM == PT_WM_F32) {
+                const float w = __uint_as_float(ev[EW * u + 1]);
+                l = neg ? l - w : l + w;
+            } else {
+                const int32_t w = WM == PT_WM_I8 ? (int32_t)(int8_t)(ev[u] & 0xffu)
+                                                 : (int32_t)ev[EW * u + 1];
+                l += neg ? -w : w;
+            }
+        }
+        for (int k = t0 + GW; k < t1; k += GW) {
+            int cc[GW];
+            LT w[GW];
+            uint32_t vv[GW];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const float nl = ((v[u] >> a) & 1u) ? l - w[u] : l + w[u];
-                    l = k + u < n0 ? nl : l;
+            for (int u = 0; u < GW; ++u) {
+                const bool ok = k + u < t1;
+                if constexpr (WM == PT_WM_F32) cc[u] = edge_float(I.edge, ok ? k + u : 0, w[u]);
+                else cc[u] = edge_int<WM>(I.edge, ok ? k + u : 0, w[u]);
+                cc[u] = ok ? cc[u] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < GW; ++u) vv[u] = G.spw[cc[u]];
+#pragma unroll
+            for (int u = 0; u < GW; ++u) {
+                const bool neg = (vv[u] >> a) & 1u;
+                if constexpr (WM == PT_WM_F32) {
+                    const float nl = neg ? l - w[u] : l + w[u];
+                    l = k + u < t1 ? nl : l;
+                } else {
+                    l += k + u < t1 ? (neg ? -w[u] : w[u]) : 0;
                 }
             }
-            if (a < R) {
-                const int g = G.rung[cur * Rp + a];
-                const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+        }
+        const int8_t si = ((own >> a) & 1u) ? (int8_t)-1 : (int8_t)1;
+        if (a < R) {
+            const int g = G.rung[cur * Rp + a];
+            const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+            if constexpr (WM == PT_WM_F32) {
                 const float q = si > 0 ? l : -l;
                 if (pt_accept_float((float)I.beta[g], q, pt_word(rv, p & 3))) {
                     flip = true;
                     d = pt_fix36(2.0f * q);
                 }
-            }
-        } else {
-            int32_t l = I.h ? ((const int32_t*)I.h)[p] : 0;
-            for (int k = 0; k < n0; k += 8) {
-                int cc[8];
-                int32_t w[8];
-                uint32_t v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const bool ok = k + u < n0;
-                    cc[u] = edge_int<WM>(I.edge, ok ? t0 + k + u : 0, w[u]);
-                    cc[u] = ok ? cc[u] : 0;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = G.spw[cc[u]];
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    l += k + u < n0 ? (((v[u] >> a) & 1u) ? -w[u] : w[u]) : 0;
-            }
-            if (a < R) {
-                const int g = G.rung[cur * Rp + a];
+            } else {
                 const uint32_t* thr_row = I.thr + (size_t)g * I.TQ;
-                const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
                 const int32_t q = si > 0 ? l : -l;
                 if (q <= 0 || (q <= I.TQ && pt_word(rv, p & 3) < thr_row[q - 1])) {
                     flip = true;
@@ -711,9 +733,15 @@ hipError_t pt_launch_g_pack(const PtInstance& I, const GState& G, bool unpack, h
 hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j, int rl,
                              int block, int grid, hipStream_t s) {
     if (G.spw) {
-        if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8><<<grid, block, 0, s>>>(I, G, c, j, rl);
-        else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32><<<grid, block, 0, s>>>(I, G, c, j, rl);
-        else g_color_bits<PT_WM_F32><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        if (I.gq == 4) {
+            if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+            else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+            else g_color_bits<PT_WM_F32, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        } else {
+            if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+            else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+            else g_color_bits<PT_WM_F32, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        }
         return hipGetLastError();
     }
     if (I.Rp % 64 == 0) g_color_dispatch<true>(I, G, c, j, rl, block, grid, s);

@@ -104,8 +104,9 @@ def test_replica_counts(R):
 
 
 def _degree_profile_model(kind):
-    """Graphs whose row lengths pick each first-chunk width of the LDS engine's quad gather
-    (2, 3, 4) and send long rows through the chunked remainder."""
+    """Graphs whose row lengths pick each first-chunk width of the padded edge records (LDS
+    engine 2, 3, 4; word-per-position CSR engine 4, 8) and send long rows through the chunked
+    remainder."""
     rng = np.random.default_rng(11)
     if kind == "ring_int":            # every row 2 long, two isolated spins
         n = 50
@@ -134,11 +135,12 @@ def _degree_profile_model(kind):
     return IsingModel(J.astype(np.float32).astype(np.float64), np.zeros(n))
 
 
+@pytest.mark.parametrize("engine", ["lds", "global"])
 @pytest.mark.parametrize("kind", ["ring_int", "cubic_float", "er_int_field", "hub_float"])
-def test_lds_quad_chunk_widths(kind):
-    _announce(f"lds quad chunk {kind}")
+def test_first_chunk_widths(kind, engine):
+    _announce(f"first chunk {kind} {engine}")
     model = _degree_profile_model(kind)
-    compare(model, R=8, n_rounds=20, S=2, seed=21, engine="lds")
+    compare(model, R=8, n_rounds=20, S=2, seed=21, engine=engine)
 
 
 @pytest.mark.parametrize("block", [64, 128, 512, 1024])
