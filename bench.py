@@ -99,11 +99,17 @@ def engine_format(engine: str, model, enc, replicas: int = 0) -> str:
                    "exchanged as bits" if bits else
                    "as int8 spins and bond-sign bytes, halo planes exchanged as bytes")
                 + "; int8 spins [replica][site] in HBM at the ends of the run")
-    spins = ("spins as one 32-bit word per position (bit per replica)"
-             if engine == "global" and 0 < replicas <= 32 else "int8 spins [position][replica]")
-    return (f"{spins}, packed CSR edge records "
-            f"({'4' if enc.weight.dtype.itemsize == 1 else '8'} bytes: column and "
-            f"{enc.weight.dtype.name} weight)")
+    words = engine == "global" and 0 < replicas <= 32
+    spins = ("spins as one 32-bit word per position (bit per replica)" if words
+             else "int8 spins [position][replica]" + (" in LDS" if engine == "lds" else ""))
+    rec = (f"({'4' if enc.weight.dtype.itemsize == 1 else '8'} bytes: column and "
+           f"{enc.weight.dtype.name} weight)")
+    if words or engine == "lds":
+        return (f"{spins}; the first edge records of each row (4 or 8 on the word engine, 2-4 on "
+                f"the LDS engine) in a zero-padded row-major copy read with 16-byte loads, longer "
+                f"rows finished from packed CSR edge records {rec}; the algorithmic bytes count "
+                f"the unpadded records")
+    return f"{spins}, packed CSR edge records {rec}"
 
 
 def copy_calibration(nbytes: int) -> int:
@@ -184,6 +190,8 @@ def bench_c4(args) -> int:
         "ms_per_step": round(secs * 1e3 / args.steps, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload("c4")[3], "instances": count, "total_spins": int(total_n),
+                   "format": engine_format(res[0].engine, models[0],
+                                           gpu.encode_problem(models[0]), R),
                    "engine": res[0].engine, "wall_s": round(wall, 3),
                    "mean_best_E_per_spin": round(float((best / [m.n for m in models]).mean()), 5)},
         "roofline": roofline, "cpu_baseline": cpu,
