@@ -307,12 +307,14 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 const int e = k * G + u;
+                // the spin is +-1, so w * s is exact and the fused form rounds l +- w once,
+                // exactly as the select of l + w and l - w does
                 if constexpr (WM == PT_WM_F32) {
-                    const float w = __uint_as_float(ev[EW * e + 1]);
-                    l[k] = v[e] > 0 ? l[k] + w : l[k] - w;
+                    l[k] = __builtin_fmaf(__uint_as_float(ev[EW * e + 1]), (float)v[e], l[k]);
+                } else if constexpr (WM == PT_WM_I8) {
+                    l[k] += __mul24((int32_t)(int8_t)(ev[e] & 0xffu), (int32_t)v[e]);
                 } else {
-                    const int32_t w = WM == PT_WM_I8 ? (int32_t)(int8_t)(ev[e] & 0xffu)
-                                                     : (int32_t)ev[EW * e + 1];
+                    const int32_t w = (int32_t)ev[EW * e + 1];
                     l[k] += v[e] > 0 ? w : -w;
                 }
             }
