@@ -764,7 +764,9 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
         PT_HIP(A.alloc(&spw, (size_t)H.n));
         G.spw = spw;
         PT_HIP(pt_launch_g_pack(I, G, false, sg.s));
-        // first-chunk records by spin word: 4 per row when that covers three rows in four, else 8
+    }
+    if (!use_lattice) {
+        // first-chunk records by position: 4 per row when that covers three rows in four, else 8
         int short_rows = 0;
         for (int p = 0; p < H.n; ++p) short_rows += H.row_ptr[p + 1] - H.row_ptr[p] <= 4;
         I.gq = 4 * (int64_t)short_rows >= 3 * (int64_t)H.n ? 4 : 8;

@@ -204,44 +204,6 @@ __device__ __forceinline__ float field_float(int a, int Rp, const int8_t* sp, in
     return l;
 }
 
-// The row of p is edge[t0 .. t1).
-template <int WM, int G = 1, bool COH = false>
-__device__ __forceinline__ int64_t site_update_row(int p, int a, int Rp, int8_t* sp, int t0, int t1,
-                                                   const void* __restrict__ edge, const void* h,
-                                                   uint32_t u, const uint32_t* thr_row, int TQ,
-                                                   float beta_f) {
-    int8_t* own = sp + (size_t)p * Rp + a;
-    const int8_t si = ld_spin<COH>(own);
-    if constexpr (WM == PT_WM_F32) {
-        const float l = field_float<G, COH>(a, Rp, sp, t0, t1, edge, h ? ((const float*)h)[p] : 0.0f);
-        const float q = si > 0 ? l : -l;
-        if (pt_accept_float(beta_f, q, u)) {
-            st_spin<COH>(own, (int8_t)-si);
-            return pt_fix36(2.0f * q);
-        }
-        return 0;
-    } else {
-        const int32_t l = field_int<WM, G, COH>(a, Rp, sp, t0, t1, edge,
-                                                h ? ((const int32_t*)h)[p] : 0);
-        const int32_t q = si > 0 ? l : -l;
-        const bool acc = q <= 0 || (q <= TQ && u < thr_row[q - 1]);
-        if (acc) {
-            st_spin<COH>(own, (int8_t)-si);
-            return 2 * (int64_t)q;
-        }
-        return 0;
-    }
-}
-
-template <int WM, int G = 1>
-__device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
-                                               const int32_t* __restrict__ rp,
-                                               const void* __restrict__ edge, const void* h,
-                                               uint32_t u, const uint32_t* thr_row, int TQ,
-                                               float beta_f) {
-    return site_update_row<WM, G>(p, a, Rp, sp, rp[p], rp[p + 1], edge, h, u, thr_row, TQ, beta_f);
-}
-
 // The attempts of one Philox quad in one slot: positions 4q .. 4q+3 that lie in colour range
 // [cs, ce), with word k of rv for position 4q+k.  The first G edge records of every row are also
 // stored row-major in ell, with the neighbour as its LDS offset (position * Rp) and zero records
@@ -251,10 +213,10 @@ __device__ __forceinline__ int64_t site_update(int p, int a, int Rp, int8_t* sp,
 // rows are gathered before any spin of the quad changes: the records and the neighbour spins each
 // wait on memory once per quad.  Only when some row is longer than G (rest) are the row pointers
 // read and the longer rows finished from the CSR in G-edge chunks.  Each field is summed from h in
-// row order with the operations of site_update_row; a padding record adds +0 or -0, which changes
-// at most the sign of a zero field, and a zero field is accepted with zero energy change either
-// way, so the decisions are the same.  Positions of the quad outside [cs, ce) are gathered too
-// (valid addresses) and then left alone.
+// row order, one rounding per term (l +- w), as field_float / field_int sum it and as the oracle
+// does; a padding record adds +0 or -0, which changes at most the sign of a zero field, and a zero
+// field is accepted with zero energy change either way, so the decisions are the same.  Positions
+// of the quad outside [cs, ce) are gathered too (valid addresses) and then left alone.
 // Loads through global-address-space pointers wait on their own counter, while flat loads (what a
 // pointer read from memory compiles to) also hold up the waits for LDS reads issued after them.
 #define PT_GLOBAL __attribute__((address_space(1)))
@@ -355,6 +317,77 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
         }
     }
     return d;
+}
+
+// Site update of position p in slot a with spins in HBM [position][slot]: the first GW edge
+// records of the row from the padded row-major copy (neighbour as its position), the rest of a
+// longer row (when some row is, rest) from the CSR in GW-edge chunks.  The records do not depend
+// on the row pointers, so the chain is record -> neighbour spin.  Same decision and field order
+// as quad_update; a padding record adds +-0 (see there).
+template <int WM, int GW>
+__device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t* sp,
+                                                   const int32_t* __restrict__ rp,
+                                                   const void* __restrict__ edge,
+                                                   const void* __restrict__ ell, bool rest,
+                                                   const void* h, uint32_t u,
+                                                   const uint32_t* thr_row, int TQ, float beta_f) {
+    using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    constexpr int EW = WM == PT_WM_I8 ? 1 : 2;
+    constexpr int NW = GW * EW;
+    uint32_t ev[NW];
+    {
+        const PT_GLOBAL pt_v4u* src = (const PT_GLOBAL pt_v4u*)ell + (size_t)p * (NW / 4);
+#pragma unroll
+        for (int i = 0; i < NW / 4; ++i) {
+            const pt_v4u x = src[i];
+            ev[4 * i] = x.x;
+            ev[4 * i + 1] = x.y;
+            ev[4 * i + 2] = x.z;
+            ev[4 * i + 3] = x.w;
+        }
+    }
+    int t0 = 0, t1 = 0;
+    if (rest) {
+        t0 = rp[p];
+        t1 = rp[p + 1];
+    }
+    int8_t* own = sp + (size_t)p * Rp + a;
+    const int8_t si = *own;
+    LT l = h ? ((const LT*)h)[p] : (LT)0;
+    int8_t v[GW];
+#pragma unroll
+    for (int e = 0; e < GW; ++e)
+        v[e] = sp[(size_t)(WM == PT_WM_I8 ? (ev[e] >> 8) : ev[EW * e]) * Rp + a];
+#pragma unroll
+    for (int e = 0; e < GW; ++e) {
+        if constexpr (WM == PT_WM_F32) {
+            l = __builtin_fmaf(__uint_as_float(ev[EW * e + 1]), (float)v[e], l);
+        } else if constexpr (WM == PT_WM_I8) {
+            l += __mul24((int32_t)(int8_t)(ev[e] & 0xffu), (int32_t)v[e]);
+        } else {
+            const int32_t w = (int32_t)ev[EW * e + 1];
+            l += v[e] > 0 ? w : -w;
+        }
+    }
+    if (t0 + GW < t1) {
+        if constexpr (WM == PT_WM_F32) l = field_float<GW>(a, Rp, sp, t0 + GW, t1, edge, l);
+        else l = field_int<WM, GW>(a, Rp, sp, t0 + GW, t1, edge, l);
+    }
+    if constexpr (WM == PT_WM_F32) {
+        const float q = si > 0 ? l : -l;
+        if (pt_accept_float(beta_f, q, u)) {
+            *own = (int8_t)-si;
+            return pt_fix36(2.0f * q);
+        }
+        return 0;
+    } else {
+        const int32_t q = si > 0 ? l : -l;
+        if (q <= 0 || (q <= TQ && u < thr_row[q - 1])) {
+            *own = (int8_t)-si;
+            return 2 * (int64_t)q;
+        }
+        return 0;
+    }
 }
 
 // Energy contribution of position p in slot a at initialisation.  Integer modes return

@@ -29,8 +29,8 @@ struct PtInstance {
     const void* ell;          // the first gq edge records of each row, row-major, zero records
                               // padding short rows and the rows up to a multiple of 4 positions;
                               // the neighbour field holds position * Rp (its spins' LDS offset) on
-                              // the LDS engine, the position (its spin word) on the CSR engine
-                              // with one word per position; null on the other engines
+                              // the LDS engine, the position on the per-launch CSR engine; null on
+                              // the other engines
     const void* h;            // n (position order) int32 / float, or null; zero-padded to a
                               // multiple of 4 positions where ell is set
     const int32_t* color_ptr; // k + 1
@@ -49,9 +49,9 @@ struct PtInstance {
     int32_t n, k, R, Rp;      // Rp: replica slots padded (power of two <= 64, else multiple of 64)
     int32_t n_rounds, S, TQ, wm;
     int32_t ldim, lL;         // lattice dimension and side (0 when not a lattice)
-    int32_t gq;               // edge records per row in ell (LDS engine 2, 3 or 4; word-per-
-                              // position CSR engine 4 or 8); longer rows finish from the CSR in
-                              // chunks of the same size
+    int32_t gq;               // edge records per row in ell (LDS engine 2, 3 or 4; per-launch
+                              // CSR engine 4 or 8); longer rows finish from the CSR in chunks of
+                              // the same size
     int32_t ell_rest;         // some row is longer than gq
 };
 

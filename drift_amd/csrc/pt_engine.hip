@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void g_pack_bits(PtInstance I, GState G, int u
 // slots) independent lanes in flight to cover the dependent row-pointer -> edge -> neighbour-spin
 // loads.  Lane (r, a) updates rank r of colour c in slot a with word (p & 3) of the Philox block
 // (p >> 2, a, t): the same uniform the LDS engine draws for that attempt.
-template <int WM, bool UNI>
+template <int WM, bool UNI, int GW>
 __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int c, int j,
                                                      int rl) {
     __shared__ long long red[1024];
@@ -434,8 +434,8 @@ __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int
         const float bf = (float)I.beta[g];
         const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : I.thr + (size_t)g * I.TQ;
         const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
-        d = site_update<WM, 8>(p, a, Rp, G.sp, I.row_ptr, I.edge, I.h, pt_word(rv, p & 3), thr_row,
-                            I.TQ, bf);
+        d = site_update_ell<WM, GW>(p, a, Rp, G.sp, I.row_ptr, I.edge, I.ell, I.ell_rest != 0, I.h,
+                                    pt_word(rv, p & 3), thr_row, I.TQ, bf);
     }
     block_slot_reduce(d, a, R, Rp, red, G.dacc, cur);
 }
@@ -719,9 +719,15 @@ hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int
 template <bool UNI>
 static void g_color_dispatch(const PtInstance& I, const GState& G, int c, int j, int rl, int block,
                              int grid, hipStream_t s) {
-    if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
-    else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
-    else g_color_step<PT_WM_F32, UNI><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    if (I.gq == 4) {
+        if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else g_color_step<PT_WM_F32, UNI, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    } else {
+        if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        else g_color_step<PT_WM_F32, UNI, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+    }
 }
 
 hipError_t pt_launch_g_pack(const PtInstance& I, const GState& G, bool unpack, hipStream_t s) {
