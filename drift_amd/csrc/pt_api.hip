@@ -1255,6 +1255,12 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     PT_HIP(hipMemsetAsync(C.flags, 0, sizeof(uint32_t) * W, sg.s));
     PT_HIP(hipMemsetAsync(C.gen, 0, sizeof(uint32_t), sg.s));
     PT_HIP(hipMemsetAsync(C.abort, 0, sizeof(int32_t), sg.s));
+    if (upre) {   // arrival and release words that carry the energy changes (tags start at 0)
+        PT_HIP(A.alloc(&C.eflags, (size_t)W));
+        PT_HIP(A.alloc(&C.erel, 64));
+        PT_HIP(hipMemsetAsync(C.eflags, 0, sizeof(uint64_t) * W, sg.s));
+        PT_HIP(hipMemsetAsync(C.erel, 0, sizeof(uint64_t) * 64, sg.s));
+    }
     int dev = 0, khz = 0;
     PT_HIP(hipGetDevice(&dev));
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)

@@ -427,6 +427,23 @@ __device__ __forceinline__ int64_t ld_coh(const int64_t* p) {
     return __hip_atomic_load(const_cast<int64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint64_t ld_u64(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_u64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A 64-bit barrier word that carries a value: the barrier phase mod 2^16 over a 48-bit signed
+// value, stored and loaded as one word, so a tag that matches the awaited phase vouches for the
+// value beside it.  Equality is enough: a word moves to the next phase only after its reader has
+// passed the current one.
+__device__ __forceinline__ uint64_t tag_word(uint32_t phase, int64_t v) {
+    return ((uint64_t)(phase & 0xffffu) << 48) | ((uint64_t)v & 0xFFFFFFFFFFFFull);
+}
+__device__ __forceinline__ int64_t word_value(uint64_t w) { return (int64_t)(w << 16) >> 16; }
+
 // Grid barrier with a bounded wait.  Each wave waits for its own stores to complete; then every
 // workgroup stores the phase number into its own arrival flag, workgroup 0 polls all flags (one
 // thread per flag) and publishes the phase in the generation word, which the others poll.  No
@@ -444,6 +461,23 @@ __device__ __forceinline__ bool wait_phase(const CoopArgs& C, const uint32_t* p,
         __builtin_amdgcn_s_sleep(1);
     }
     return true;
+}
+
+// Wait (bounded, as wait_phase) until *p carries the tag of `phase`; w receives the word.
+__device__ __forceinline__ bool wait_tag(const CoopArgs& C, const uint64_t* p, uint32_t phase,
+                                         uint64_t& w) {
+    const unsigned long long t0 = (unsigned long long)wall_clock64();
+    const uint32_t tag = phase & 0xffffu;
+    for (;;) {
+        w = ld_u64(p);
+        if ((uint32_t)(w >> 48) == tag) return true;
+        if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
+            __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 // gsum/lsum (optional): an LDS partial sum the waves have added into; thread 0 adds it to the

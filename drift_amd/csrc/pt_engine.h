@@ -103,6 +103,11 @@ struct CoopArgs {
     uint32_t* flags;                // W arrival flags: the last barrier phase each workgroup
                                     // reached (start at 0)
     uint32_t* gen;                  // last phase every workgroup reached (starts at 0)
+    uint64_t* eflags;               // bit-sliced persistent engine with next-sweep uniforms: W
+                                    // arrival words, each the phase tag over the workgroup's
+                                    // energy change of the round (start at 0)
+    uint64_t* erel;                 // the same engine: 64 release words written by workgroup 0,
+                                    // word a the phase tag over slot a's energy change (start at 0)
     int32_t* abort;                 // set when a barrier wait exceeded timeout_ticks
     unsigned long long timeout_ticks;  // wall_clock64 ticks one barrier may wait
     int32_t thr_lds;                // thresholds staged in LDS

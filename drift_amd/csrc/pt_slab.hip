@@ -686,6 +686,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     __shared__ long long sh_dsum;
     __shared__ uint32_t sh_gen, sh_uready;
     __shared__ uint32_t sh_uex[PT_WAVE];
+    __shared__ long long sh_esum[PT_WAVE];   // workgroup 0: slot sums of the arrival words
     constexpr int D = 2 * DIM;
     int a, b;
     wg_slot_slab(blockIdx.x, S.NB, a, b);
@@ -737,6 +738,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
         sh_uready = 0;
         for (int k = 0; k < 5; ++k) bestL[k] = 0;
     }
+    if (tid < PT_WAVE) sh_esum[tid] = 0;
     // R <= 64: the round end is wave 0's (WaveLadder); the other waves meanwhile bring in the
     // halo planes of the round's last sweep
     const bool fast = R <= PT_WAVE;
@@ -761,6 +763,10 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     // on position, slot and sweep only) into LDS while wave 0 alone completes the grid barrier and
     // the round end; the colour passes then read them.
     const bool pre = S.upre != 0 && fast && nt > PT_WAVE;
+    // pre: the energy changes of a round reach wave 0 through the barrier itself.  Each workgroup's
+    // arrival word carries its change; workgroup 0 sums them by slot into the release words; lane a
+    // of every wave 0 keeps slot a's change (relv) for the round end.
+    int64_t relv = 0;
     uint4* U = (uint4*)(lw + ((((size_t)nslots + (size_t)D * njs) * WPP + 3) & ~(size_t)3));
     const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
     auto draw = [&](uint32_t tt, int first, int stride) {
@@ -785,8 +791,12 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
         if (fast) {
             if (tid < PT_WAVE) {
                 int64_t E = WL.Eb;
-                if (rnd > 0 && tid < R)
+                if (pre) {
+                    const int64_t er = __shfl(relv, WL.rep & (PT_WAVE - 1));
+                    if (rnd > 0 && tid < R) E += er;
+                } else if (rnd > 0 && tid < R) {
                     E += ld_coh(S.dacc + (size_t)((rnd - 1) % 3) * Rp + WL.rep);
+                }
                 // pre: wave 1 drew this exchange's uniforms during the barrier (a short wait)
                 if (pre && rnd > 0)
                     while (__hip_atomic_load(&sh_uready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < phase)
@@ -798,7 +808,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                     sh_imp = imp;
                     sh_slot = WL.best_slot;
                 }
-                if (owner && tid < R)   // buffer of round rnd + 1 (see below)
+                if (owner && !pre && tid < R)   // buffer of round rnd + 1 (see below)
                     __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + tid, (int64_t)0,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else if (defer && live && rnd > 0 && rnd < I.n_rounds) {
@@ -1001,28 +1011,51 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                 dsum = 0;
             }
             if (pre) {
-                // arrival: every wave's stores complete; thread 0 adds the energy sum and flags
+                // arrival: every wave's stores complete; thread 0 stores the arrival word with
+                // the workgroup's energy change of the round (zero on the other sweeps)
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __syncthreads();
                 ++phase;
-                if (tid == 0) {
-                    const long long v = sh_dsum;
-                    sh_dsum = 0;
-                    if (last && live && v != 0) {
-                        atomicAdd((unsigned long long*)(S.dacc + (size_t)(rnd % 3) * Rp + a),
-                                  (unsigned long long)v);
-                        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                    }
-                    if (blockIdx.x != 0) st_u32(C.flags + blockIdx.x, phase);
-                }
                 if (tid < PT_WAVE) {
-                    // wave 0 completes the barrier (the hub polls every flag, one lane per flag)
+                    long long v = 0;
+                    if (tid == 0) {
+                        v = last && live ? sh_dsum : 0;
+                        sh_dsum = 0;
+                        if (blockIdx.x != 0) st_u64(C.eflags + blockIdx.x, tag_word(phase, v));
+                    }
                     if (blockIdx.x == 0) {
-                        for (int x = 1 + tid; x < (int)gridDim.x; x += PT_WAVE)
-                            if (!wait_phase(C, C.flags + x, phase)) sh_abort = 1;
-                        if (tid == 0 && sh_abort == 0) st_u32(C.gen, phase);
-                    } else if (tid == 0) {
-                        if (!wait_phase(C, C.gen, phase)) sh_abort = 1;
+                        // the hub: lane x collects the words of workgroups x, x + 64, ... and adds
+                        // each change to its slot's sum; lane a then releases slot a's sum
+                        for (int x = tid; x < (int)gridDim.x; x += PT_WAVE) {
+                            int64_t e = v;
+                            if (x != 0) {
+                                uint64_t w;
+                                if (!wait_tag(C, C.eflags + x, phase, w)) {
+                                    sh_abort = 1;
+                                    break;
+                                }
+                                e = word_value(w);
+                            }
+                            int xa, xb;
+                            wg_slot_slab(x, S.NB, xa, xb);
+                            if (e != 0 && xa < PT_WAVE)
+                                atomicAdd((unsigned long long*)&sh_esum[xa], (unsigned long long)e);
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                        __builtin_amdgcn_wave_barrier();
+                        relv = sh_esum[tid];
+                        sh_esum[tid] = 0;
+                        if (sh_abort == 0) {
+                            st_u64(C.erel + tid, tag_word(phase, relv));
+                            if (tid == 0) st_u32(C.gen, phase);
+                        }
+                    } else {
+                        // the release is the one generation word; the sums follow in one load
+                        // (their tags confirm them)
+                        if (tid == 0 && !wait_phase(C, C.gen, phase)) sh_abort = 1;
+                        uint64_t w;
+                        if (!wait_tag(C, C.erel + tid, phase, w)) sh_abort = 1;
+                        relv = word_value(w);
                     }
                     if (tid == 0)
                         __hip_atomic_store(&sh_gen, phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
