@@ -1205,7 +1205,8 @@ int pslab_block(const HostInst& H, int block_req, size_t& lds, bool* upre = null
     // an explicit block size is taken or refused; otherwise the largest that fits
     const int first = block_req > 0 ? std::max(64, std::min(1024, (block_req / 64) * 64)) : 512;
     for (int block = first; block >= 64; block /= 2) {
-        if (lds1 && block > 64 && upre) {
+        // the uniform path's barrier words are polled by one wave, at most 32 per lane
+        if (lds1 && block > 64 && upre && W <= 32 * 64) {
             const int f = fits(block, lds1);
             if (f == 1) {
                 lds = lds1;

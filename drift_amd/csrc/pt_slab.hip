@@ -1024,22 +1024,50 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                         if (blockIdx.x != 0) st_u64(C.eflags + blockIdx.x, tag_word(phase, v));
                     }
                     if (blockIdx.x == 0) {
-                        // the hub: lane x collects the words of workgroups x, x + 64, ... and adds
-                        // each change to its slot's sum; lane a then releases slot a's sum
-                        for (int x = tid; x < (int)gridDim.x; x += PT_WAVE) {
-                            int64_t e = v;
-                            if (x != 0) {
-                                uint64_t w;
-                                if (!wait_tag(C, C.eflags + x, phase, w)) {
-                                    sh_abort = 1;
-                                    break;
+                        // the hub: lane x polls the words of workgroups x, x + 64, ... together
+                        // (every pass loads each word, the loads in flight at once), adds each
+                        // change to its slot's sum once its tag arrives; lane a then releases slot
+                        // a's sum.  Workgroup 0 (slot 0) adds its own change directly.
+                        const int W = (int)gridDim.x, nmax = (W + PT_WAVE - 1) / PT_WAVE;
+                        uint32_t pending = 0;
+                        for (int i = 0; i < nmax; ++i) {
+                            const int x = tid + i * PT_WAVE;
+                            if (x != 0 && x < W) pending |= 1u << i;
+                        }
+                        if (tid == 0 && v != 0)
+                            atomicAdd((unsigned long long*)&sh_esum[0], (unsigned long long)v);
+                        const unsigned long long t0 = (unsigned long long)wall_clock64();
+                        const uint32_t tag = phase & 0xffffu;
+                        while (pending) {
+                            for (int i0 = 0; i0 < nmax; i0 += 8) {   // W <= 2048: nmax <= 32
+                                uint64_t w[8];
+#pragma unroll
+                                for (int i = 0; i < 8; ++i)
+                                    if (i0 + i < nmax)
+                                        w[i] = ld_u64(C.eflags + min(tid + (i0 + i) * PT_WAVE, W - 1));
+#pragma unroll
+                                for (int i = 0; i < 8; ++i) {
+                                    const int k = i0 + i;
+                                    if (k < nmax && ((pending >> k) & 1u) &&
+                                        (uint32_t)(w[i] >> 48) == tag) {
+                                        pending &= ~(1u << k);
+                                        const int64_t e = word_value(w[i]);
+                                        int xa, xb;
+                                        wg_slot_slab(tid + k * PT_WAVE, S.NB, xa, xb);
+                                        if (e != 0 && xa < PT_WAVE)
+                                            atomicAdd((unsigned long long*)&sh_esum[xa],
+                                                      (unsigned long long)e);
+                                    }
                                 }
-                                e = word_value(w);
                             }
-                            int xa, xb;
-                            wg_slot_slab(x, S.NB, xa, xb);
-                            if (e != 0 && xa < PT_WAVE)
-                                atomicAdd((unsigned long long*)&sh_esum[xa], (unsigned long long)e);
+                            if (!pending) break;
+                            if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                                (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
+                                __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                sh_abort = 1;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
                         }
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                         __builtin_amdgcn_wave_barrier();
