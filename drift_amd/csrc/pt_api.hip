@@ -118,6 +118,8 @@ struct HostInst {
     std::vector<uint8_t> jbits;     // per site: bit d set when the bond in direction d is -1
     int gq = 4;                     // PtInstance::gq
     int maxdeg = 0;                 // longest row
+    int force_ebar = -1;            // bit-sliced persistent engine: SState::ebar, or -1 to choose
+                                    // by a timed trial (energy_path_choice)
 };
 
 // First-chunk width of the LDS engine's quad gather: the smallest of 2, 3, 4 that covers at least
@@ -1090,6 +1092,8 @@ int finish_slab(const HostInst& H, const PtInstance& I, const SState& S, drift_p
 int slab_setup(DevArena& A, const HostInst& H, const PtInstance& I, SState& S, hipStream_t st) {
     const int Rp = H.Rp, n = H.n;
     S.NB = slab_count(H);
+    S.upre = 0;
+    S.ebar = 0;
     PT_HIP(A.alloc(&S.sp, 2 * (size_t)Rp * n));
     PT_HIP(A.alloc(&S.Eb, 2 * (size_t)Rp));
     PT_HIP(A.alloc(&S.dacc, 3 * (size_t)Rp));
@@ -1225,11 +1229,61 @@ int pslab_block(const HostInst& H, int block_req, size_t& lds, bool* upre = null
     return 0;
 }
 
+int run_pslab(const HostInst& H, drift_pt_result* res, int block_req);
+
+// The bit-sliced engine's two ways of bringing a round's energy changes to the round end (atomic
+// adds before the arrival flags, or the changes inside the arrival and release words) cost
+// differently from one device to another: each was measured up to 10% faster than the other on
+// different MI355X boards running the same 3-D L=32, R=64 run.  The first run of a grid shape in
+// a process therefore times both on a copy of the instance (rounds N and 2N, the difference
+// cancelling the fixed costs) and keeps the faster.  Both make the same decisions.
+int energy_path_choice(const HostInst& H, int block_req, int block) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int>, int> cache;
+    const auto key = std::make_tuple(H.Rp, H.lL, H.ldim, block);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    HostInst T = H;
+    T.flags &= ~(DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE | DRIFT_PT_RECORD_FINAL |
+                 DRIFT_PT_TIME_KERNELS);
+    const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
+    const int rounds = (int)std::min(2000.0, std::max(64.0, 4.0e8 / per_round));
+    std::vector<int8_t> bs((size_t)std::max(H.n, 1));
+    drift_pt_result r{};
+    r.best_s = bs.data();
+    double secs[2] = {-1.0, -1.0};
+    for (int v = 0; v < 2; ++v) {
+        T.force_ebar = v;
+        T.n_rounds = 2;
+        clear_result(&r);
+        if (run_pslab(T, &r, block_req) != 0) continue;
+        T.n_rounds = rounds;
+        clear_result(&r);
+        if (run_pslab(T, &r, block_req) != 0) continue;
+        const double t1 = r.seconds;
+        T.n_rounds = 2 * rounds;
+        clear_result(&r);
+        if (run_pslab(T, &r, block_req) != 0) continue;
+        secs[v] = r.seconds - t1 > 0.0 ? r.seconds - t1 : 0.5 * r.seconds;
+    }
+    (void)hipGetLastError();
+    g_err.clear();
+    const int choice = secs[1] > 0.0 && (secs[0] <= 0.0 || secs[1] < secs[0]) ? 1 : 0;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = choice;
+    return choice;
+}
+
 int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     size_t lds = 0;
     bool upre = false;
     const int block = pslab_block(H, block_req, lds, &upre);
     if (!block) return fail(DRIFT_PT_EINVAL, "pslab engine: the slab grid cannot be co-resident");
+    // chosen before this run allocates anything
+    const int ebar = !upre ? 0 : H.force_ebar >= 0 ? H.force_ebar : energy_path_choice(H, block_req, block);
     DevArena A;
     PtInstance I;
     int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
@@ -1243,6 +1297,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     if (rc) return rc;
     S.hbits = nullptr;
     S.upre = upre ? 1 : 0;
+    S.ebar = ebar;
     if (pt_pslab_bits(H.lL)) {
         const size_t words = 2 * (size_t)H.Rp * (size_t)H.n / 32;
         PT_HIP(A.alloc(&S.hbits, words));
@@ -1256,7 +1311,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     PT_HIP(hipMemsetAsync(C.flags, 0, sizeof(uint32_t) * W, sg.s));
     PT_HIP(hipMemsetAsync(C.gen, 0, sizeof(uint32_t), sg.s));
     PT_HIP(hipMemsetAsync(C.abort, 0, sizeof(int32_t), sg.s));
-    if (upre) {   // arrival and release words that carry the energy changes (tags start at 0)
+    if (S.ebar) {   // arrival and release words that carry the energy changes (tags start at 0)
         PT_HIP(A.alloc(&C.eflags, (size_t)W));
         PT_HIP(A.alloc(&C.erel, 64));
         PT_HIP(hipMemsetAsync(C.eflags, 0, sizeof(uint64_t) * W, sg.s));

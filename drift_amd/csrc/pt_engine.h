@@ -93,6 +93,8 @@ struct SState {
     int32_t NB;        // slabs per replica (1: the whole lattice wraps inside one workgroup)
     int32_t upre;      // bit-sliced persistent engine: the next sweep's Metropolis uniforms are
                        // drawn into LDS while the grid barrier and the round end are in progress
+    int32_t ebar;      // with upre: the energy changes travel in the barrier words (CoopArgs
+                       // eflags, erel) instead of atomic adds into dacc
 };
 
 // Persistent CSR engine (one cooperative launch).  Workgroup w of W owns ranks

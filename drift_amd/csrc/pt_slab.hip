@@ -669,7 +669,8 @@ __device__ __forceinline__ void bits_row(uint32_t bits, int8_t* dst) {
 }
 
 // At most 512 threads: the next-sweep uniform path needs more than 128 registers per lane.
-template <int DIM, bool WRAP>
+// EBAR: the energy path of the barrier (SState::ebar), fixed per instantiation.
+template <int DIM, bool WRAP, bool EBAR>
 __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, CoopArgs C) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lw[];
     __shared__ int64_t Es[1024];
@@ -763,9 +764,11 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     // on position, slot and sweep only) into LDS while wave 0 alone completes the grid barrier and
     // the round end; the colour passes then read them.
     const bool pre = S.upre != 0 && fast && nt > PT_WAVE;
-    // pre: the energy changes of a round reach wave 0 through the barrier itself.  Each workgroup's
-    // arrival word carries its change; workgroup 0 sums them by slot into the release words; lane a
-    // of every wave 0 keeps slot a's change (relv) for the round end.
+    // ebar (with pre): the energy changes of a round reach wave 0 through the barrier itself.  Each
+    // workgroup's arrival word carries its change; workgroup 0 sums them by slot into the release
+    // words; lane a of every wave 0 keeps slot a's change (relv) for the round end.  Otherwise the
+    // changes go through atomic adds into S.dacc before the arrival flags.
+    const bool ebar = EBAR && pre;
     int64_t relv = 0;
     uint4* U = (uint4*)(lw + ((((size_t)nslots + (size_t)D * njs) * WPP + 3) & ~(size_t)3));
     const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
@@ -791,7 +794,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
         if (fast) {
             if (tid < PT_WAVE) {
                 int64_t E = WL.Eb;
-                if (pre) {
+                if (ebar) {
                     const int64_t er = __shfl(relv, WL.rep & (PT_WAVE - 1));
                     if (rnd > 0 && tid < R) E += er;
                 } else if (rnd > 0 && tid < R) {
@@ -808,7 +811,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                     sh_imp = imp;
                     sh_slot = WL.best_slot;
                 }
-                if (owner && !pre && tid < R)   // buffer of round rnd + 1 (see below)
+                if (owner && !ebar && tid < R)   // buffer of round rnd + 1 (see below)
                     __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + tid, (int64_t)0,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else if (defer && live && rnd > 0 && rnd < I.n_rounds) {
@@ -1011,12 +1014,33 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                 dsum = 0;
             }
             if (pre) {
-                // arrival: every wave's stores complete; thread 0 stores the arrival word with
-                // the workgroup's energy change of the round (zero on the other sweeps)
+                // arrival: every wave's stores complete, then wave 0 completes the barrier
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __syncthreads();
                 ++phase;
-                if (tid < PT_WAVE) {
+                if (tid < PT_WAVE && !ebar) {
+                    // thread 0 adds the energy change of the round to the slot's accumulator and
+                    // flags; the hub (one lane per flag) publishes the generation word
+                    if (tid == 0) {
+                        const long long v = sh_dsum;
+                        sh_dsum = 0;
+                        if (last && live && v != 0) {
+                            atomicAdd((unsigned long long*)(S.dacc + (size_t)(rnd % 3) * Rp + a),
+                                      (unsigned long long)v);
+                            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                        }
+                        if (blockIdx.x != 0) st_u32(C.flags + blockIdx.x, phase);
+                    }
+                    if (blockIdx.x == 0) {
+                        for (int x = 1 + tid; x < (int)gridDim.x; x += PT_WAVE)
+                            if (!wait_phase(C, C.flags + x, phase)) sh_abort = 1;
+                        if (tid == 0 && sh_abort == 0) st_u32(C.gen, phase);
+                    } else if (tid == 0) {
+                        if (!wait_phase(C, C.gen, phase)) sh_abort = 1;
+                    }
+                    if (tid == 0)
+                        __hip_atomic_store(&sh_gen, phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else if (tid < PT_WAVE) {
                     long long v = 0;
                     if (tid == 0) {
                         v = last && live ? sh_dsum : 0;
@@ -1162,14 +1186,15 @@ const void* pslab_fn() {
 }
 
 template <int DIM, bool WRAP>
-const void* pslab_bits_fn() {
-    return reinterpret_cast<const void*>(&pslab_bits_run<DIM, WRAP>);
+const void* pslab_bits_fn(bool ebar) {
+    return ebar ? reinterpret_cast<const void*>(&pslab_bits_run<DIM, WRAP, true>)
+                : reinterpret_cast<const void*>(&pslab_bits_run<DIM, WRAP, false>);
 }
 
-const void* pslab_pick(int dim, bool wrap, bool bits) {
+const void* pslab_pick(int dim, bool wrap, bool bits, bool ebar = false) {
     if (bits) {
-        if (dim == 3) return wrap ? pslab_bits_fn<3, true>() : pslab_bits_fn<3, false>();
-        return wrap ? pslab_bits_fn<2, true>() : pslab_bits_fn<2, false>();
+        if (dim == 3) return wrap ? pslab_bits_fn<3, true>(ebar) : pslab_bits_fn<3, false>(ebar);
+        return wrap ? pslab_bits_fn<2, true>(ebar) : pslab_bits_fn<2, false>(ebar);
     }
     if (dim == 3) return wrap ? pslab_fn<3, true>() : pslab_fn<3, false>();
     return wrap ? pslab_fn<2, true>() : pslab_fn<2, false>();
@@ -1237,11 +1262,20 @@ size_t pt_pslab_lds_bytes(int L, int dim, int NB, bool upre) {
     return ((words * 4 + 15) & ~(size_t)15) + slots * segs * 16;
 }
 
+// The smaller occupancy of the two energy-path variants, so either can be launched on a grid
+// this says is co-resident.
 hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu) {
-    const void* f = pslab_pick(dim, NB == 1, pt_pslab_bits(L));
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, block, lds);
+    *per_cu = 0;
+    for (int v = 0; v < 2; ++v) {
+        const void* f = pslab_pick(dim, NB == 1, pt_pslab_bits(L), v == 1);
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        int k = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, f, block, lds);
+        if (e != hipSuccess) return e;
+        *per_cu = v == 0 ? k : std::min(*per_cu, k);
+    }
+    return hipSuccess;
 }
 
 int pt_pslab_grid(const PtInstance& I, const SState& S) { return ((I.R + 7) / 8) * 8 * S.NB; }
@@ -1252,7 +1286,7 @@ hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs&
     SState st = S;
     CoopArgs c = C;
     void* args[] = {(void*)&i, (void*)&st, (void*)&c};
-    const void* f = pslab_pick(I.ldim, S.NB == 1, pt_pslab_bits(I.lL));
+    const void* f = pslab_pick(I.ldim, S.NB == 1, pt_pslab_bits(I.lL), S.ebar != 0);
     const dim3 grid(pt_pslab_grid(I, S));
     return cooperative ? hipLaunchCooperativeKernel(f, grid, dim3(block), args, (unsigned)lds, s)
                        : hipLaunchKernel(f, grid, dim3(block), args, lds, s);
