@@ -1239,8 +1239,8 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req);
 // cancelling the fixed costs) and keeps the faster.  Both make the same decisions.
 int energy_path_choice(const HostInst& H, int block_req, int block) {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int>, int> cache;
-    const auto key = std::make_tuple(H.Rp, H.lL, H.ldim, block);
+    static std::map<std::tuple<int, int, int, int, int>, int> cache;
+    const auto key = std::make_tuple(H.Rp, H.lL, H.ldim, block, H.S);   // S: barriers per round end
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);
