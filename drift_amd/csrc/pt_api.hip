@@ -1282,8 +1282,15 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     bool upre = false;
     const int block = pslab_block(H, block_req, lds, &upre);
     if (!block) return fail(DRIFT_PT_EINVAL, "pslab engine: the slab grid cannot be co-resident");
-    // chosen before this run allocates anything
-    const int ebar = !upre ? 0 : H.force_ebar >= 0 ? H.force_ebar : energy_path_choice(H, block_req, block);
+    // chosen before this run allocates anything; DRIFT_PT_PSLAB_ENERGY_PATH=atomic|words fixes it
+    int ebar = 0;
+    if (upre) {
+        const char* ep = std::getenv("DRIFT_PT_PSLAB_ENERGY_PATH");
+        if (H.force_ebar >= 0) ebar = H.force_ebar;
+        else if (ep && std::strcmp(ep, "atomic") == 0) ebar = 0;
+        else if (ep && std::strcmp(ep, "words") == 0) ebar = 1;
+        else ebar = energy_path_choice(H, block_req, block);
+    }
     DevArena A;
     PtInstance I;
     int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);

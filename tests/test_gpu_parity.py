@@ -353,6 +353,20 @@ def test_pslab_workgroup_sizes(block):
     assert g.engine == "pslab"
 
 
+@pytest.mark.parametrize("path", ["atomic", "words"])
+@pytest.mark.parametrize("case", [(3, 32, 16, 2), (3, 32, 64, 1), (2, 64, 12, 3)])
+def test_pslab_energy_paths(case, path, monkeypatch):
+    """Both ways the bit-sliced persistent engine brings a round's energy changes to the round end
+    (atomic adds before the arrival flags; the changes inside the arrival and release words) give
+    the oracle's chain.  The library picks one per board by timing; this fixes each in turn."""
+    dim, L, R, S = case
+    monkeypatch.setenv("DRIFT_PT_PSLAB_ENERGY_PATH", path)
+    _announce(f"pslab energy path {path} dim={dim} L={L} R={R} S={S}")
+    model = square_ferromagnet(L) if dim == 2 else cubic_pmj_glass(L, 5)
+    g = compare(model, R=R, n_rounds=30, S=S, seed=R + S, T_min=0.3, T_max=4.0, engine="pslab")[0]
+    assert g.engine == "pslab"
+
+
 @pytest.mark.parametrize("engine", ["pslab", "coop"])
 def test_persistent_engines_with_an_ordinary_dispatch(engine):
     """DRIFT_PT_PLAIN_LAUNCH starts the persistent engines without the cooperative-launch API;
