@@ -1236,7 +1236,8 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req);
 // differently from one device to another: each was measured up to 10% faster than the other on
 // different MI355X boards running the same 3-D L=32, R=64 run.  The first run of a grid shape in
 // a process therefore times both on a copy of the instance (rounds N and 2N, the difference
-// cancelling the fixed costs) and keeps the faster.  Both make the same decisions.
+// cancelling the fixed costs), twice each in the order 0 1 1 0 so that a clock still ramping up
+// favours neither, and keeps the faster minimum.  Both make the same decisions.
 int energy_path_choice(const HostInst& H, int block_req, int block) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int, int, int>, int> cache;
@@ -1250,12 +1251,13 @@ int energy_path_choice(const HostInst& H, int block_req, int block) {
     T.flags &= ~(DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE | DRIFT_PT_RECORD_FINAL |
                  DRIFT_PT_TIME_KERNELS);
     const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
-    const int rounds = (int)std::min(2000.0, std::max(64.0, 4.0e8 / per_round));
+    const int rounds = (int)std::min(4000.0, std::max(64.0, 2.0e9 / per_round));
     std::vector<int8_t> bs((size_t)std::max(H.n, 1));
     drift_pt_result r{};
     r.best_s = bs.data();
     double secs[2] = {-1.0, -1.0};
-    for (int v = 0; v < 2; ++v) {
+    const int order[4] = {0, 1, 1, 0};
+    for (int v : order) {
         T.force_ebar = v;
         T.n_rounds = 2;
         clear_result(&r);
@@ -1267,7 +1269,8 @@ int energy_path_choice(const HostInst& H, int block_req, int block) {
         T.n_rounds = 2 * rounds;
         clear_result(&r);
         if (run_pslab(T, &r, block_req) != 0) continue;
-        secs[v] = r.seconds - t1 > 0.0 ? r.seconds - t1 : 0.5 * r.seconds;
+        const double d = r.seconds - t1 > 0.0 ? r.seconds - t1 : 0.5 * r.seconds;
+        if (secs[v] < 0.0 || d < secs[v]) secs[v] = d;
     }
     (void)hipGetLastError();
     g_err.clear();
