@@ -542,6 +542,9 @@ def main() -> int:
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--engine", default="auto")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="with --engine auto: time every candidate engine first (the default takes "
+                         "the library's engine choice)")
     ap.add_argument("--kernel-timing-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -583,7 +586,7 @@ def main() -> int:
     enc = gpu.encode_problem(model)
     kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,
               block_threads=args.block, record_history=False,
-              autotune=args.engine == "auto" and args.block == 0,
+              autotune=args.autotune and args.engine == "auto" and args.block == 0,
               # the bench owns the device, so the persistent engines start with an ordinary
               # dispatch (rocprofv3 cannot trace cooperative launches)
               plain_launch=True)
