@@ -795,7 +795,8 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
                     PT_HIP(hipEventRecord(a, sg.s));
                 }
                 if (use_lattice) PT_HIP(pt_launch_g_lattice(I, G, c, j, rl, block, grid, sg.s));
-                else PT_HIP(pt_launch_g_color(I, G, c, j, rl, block, cgrid[c], sg.s));
+                else PT_HIP(pt_launch_g_color(I, G, H.color_ptr[c], H.color_ptr[c + 1], j, rl,
+                                              block, cgrid[c], sg.s));
                 if (tk) {
                     PT_HIP(hipEventRecord(b, sg.s));
                     ++timed;

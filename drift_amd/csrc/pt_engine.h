@@ -152,8 +152,9 @@ hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int 
 
 hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
                             hipStream_t s);
-// Colour step c of sweep j of round (*round_base + rl).
-hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j, int rl,
+// Colour step of sweep j of round (*round_base + rl) over colour-list positions [cs, ce) (one
+// colour class; passed by value so the first loads of the step do not wait on the colour list).
+hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int cs, int ce, int j, int rl,
                              int block, int grid, hipStream_t s);
 hipError_t pt_launch_g_lattice(const PtInstance& I, const GState& G, int c, int j, int rl,
                                int block, int grid, hipStream_t s);

@@ -293,7 +293,8 @@ __device__ __forceinline__ int8_t g_spin(const GState& G, int p, int Rp, int a) 
 // adds +-0 to the field, which changes at most the sign of a zero field (accepted with zero energy
 // change either way).
 template <int WM, int GW>
-__global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int c, int j, int rl) {
+__global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int cs, int ce, int j,
+                                                     int rl) {
     __shared__ long long red[1024];
     const int rnd = *G.round_base + rl, cur = rnd & 1;
     const uint32_t t = (uint32_t)(rnd * I.S + j);
@@ -304,8 +305,8 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
     constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
     constexpr int NW = GW * EW;
     const int a = tid & 31;
-    const int p = I.color_ptr[c] + blockIdx.x * (int)(blockDim.x >> 5) + (tid >> 5);
-    const bool live = p < I.color_ptr[c + 1];
+    const int p = cs + blockIdx.x * (int)(blockDim.x >> 5) + (tid >> 5);
+    const bool live = p < ce;
     uint32_t own = 0;
     bool flip = false;
     int64_t d = 0;
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(256) void g_pack_bits(PtInstance I, GState G, int u
 // loads.  Lane (r, a) updates rank r of colour c in slot a with word (p & 3) of the Philox block
 // (p >> 2, a, t): the same uniform the LDS engine draws for that attempt.
 template <int WM, bool UNI, int GW>
-__global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int c, int j,
+__global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int cs, int ce, int j,
                                                      int rl) {
     __shared__ long long red[1024];
     const int rnd = *G.round_base + rl, cur = rnd & 1;
@@ -427,9 +428,9 @@ __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int
     // so row pointers and edges go through the scalar unit
     const int a = tid % Rp;
     const int ro = UNI ? __builtin_amdgcn_readfirstlane(tid / Rp) : tid / Rp;
-    const int p = I.color_ptr[c] + blockIdx.x * (blockDim.x / Rp) + ro;
+    const int p = cs + blockIdx.x * (blockDim.x / Rp) + ro;
     int64_t d = 0;
-    if (a < R && p < I.color_ptr[c + 1]) {
+    if (a < R && p < ce) {
         const int g = G.rung[cur * Rp + a];
         const float bf = (float)I.beta[g];
         const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : I.thr + (size_t)g * I.TQ;
@@ -717,16 +718,16 @@ hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int
 }
 
 template <bool UNI>
-static void g_color_dispatch(const PtInstance& I, const GState& G, int c, int j, int rl, int block,
-                             int grid, hipStream_t s) {
+static void g_color_dispatch(const PtInstance& I, const GState& G, int cs, int ce, int j, int rl,
+                             int block, int grid, hipStream_t s) {
     if (I.gq == 4) {
-        if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
-        else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
-        else g_color_step<PT_WM_F32, UNI, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI, 4><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+        else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI, 4><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+        else g_color_step<PT_WM_F32, UNI, 4><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
     } else {
-        if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
-        else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
-        else g_color_step<PT_WM_F32, UNI, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+        if (I.wm == PT_WM_I8) g_color_step<PT_WM_I8, UNI, 8><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+        else if (I.wm == PT_WM_I32) g_color_step<PT_WM_I32, UNI, 8><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+        else g_color_step<PT_WM_F32, UNI, 8><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
     }
 }
 
@@ -736,22 +737,22 @@ hipError_t pt_launch_g_pack(const PtInstance& I, const GState& G, bool unpack, h
     return hipGetLastError();
 }
 
-hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int c, int j, int rl,
+hipError_t pt_launch_g_color(const PtInstance& I, const GState& G, int cs, int ce, int j, int rl,
                              int block, int grid, hipStream_t s) {
     if (G.spw) {
         if (I.gq == 4) {
-            if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
-            else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
-            else g_color_bits<PT_WM_F32, 4><<<grid, block, 0, s>>>(I, G, c, j, rl);
+            if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8, 4><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+            else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32, 4><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+            else g_color_bits<PT_WM_F32, 4><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
         } else {
-            if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
-            else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
-            else g_color_bits<PT_WM_F32, 8><<<grid, block, 0, s>>>(I, G, c, j, rl);
+            if (I.wm == PT_WM_I8) g_color_bits<PT_WM_I8, 8><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+            else if (I.wm == PT_WM_I32) g_color_bits<PT_WM_I32, 8><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
+            else g_color_bits<PT_WM_F32, 8><<<grid, block, 0, s>>>(I, G, cs, ce, j, rl);
         }
         return hipGetLastError();
     }
-    if (I.Rp % 64 == 0) g_color_dispatch<true>(I, G, c, j, rl, block, grid, s);
-    else g_color_dispatch<false>(I, G, c, j, rl, block, grid, s);
+    if (I.Rp % 64 == 0) g_color_dispatch<true>(I, G, cs, ce, j, rl, block, grid, s);
+    else g_color_dispatch<false>(I, G, cs, ce, j, rl, block, grid, s);
     return hipGetLastError();
 }
 
