@@ -55,7 +55,10 @@ enum drift_pt_engine {
                                    LDS, colour steps are separated by grid barriers */
     DRIFT_PT_ENGINE_PSLAB = 6   /* the slab engine's lattices in one cooperative launch: slabs stay
                                    in LDS for the whole run, halo planes move through HBM between
-                                   sweeps, one grid barrier per sweep */
+                                   sweeps, one grid barrier per sweep.  The first run of a lattice
+                                   shape in a process times the barrier's two energy paths and
+                                   keeps the faster (environment DRIFT_PT_PSLAB_ENERGY_PATH=atomic
+                                   or =words fixes it); results do not depend on the path */
 };
 
 enum drift_pt_flags {
