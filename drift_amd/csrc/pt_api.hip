@@ -1252,7 +1252,8 @@ int energy_path_choice(const HostInst& H, int block_req, int block) {
     T.flags &= ~(DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE | DRIFT_PT_RECORD_FINAL |
                  DRIFT_PT_TIME_KERNELS);
     const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
-    const int rounds = (int)std::min(4000.0, std::max(64.0, 2.0e9 / per_round));
+    // about 2e9 attempts per timed run, so large instances or many sweeps per round stay short
+    const int rounds = (int)std::min(4000.0, std::max(4.0, 2.0e9 / per_round));
     std::vector<int8_t> bs((size_t)std::max(H.n, 1));
     drift_pt_result r{};
     r.best_s = bs.data();
