@@ -659,7 +659,18 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     PT_HIP(A.upload(&d_insts, insts));
     PT_HIP(A.upload(&d_order, order));
     PT_HIP(A.alloc(&d_queue, 1));
-    const int block = choose_lds_block(maxRp, block_req);
+    // A few instances leave most of the chip idle, so their workgroups are sized to sweep the
+    // largest colour class in one pass (each pass is a dependent chain of loads); batches that
+    // fill the chip keep 256 threads.
+    int req = block_req;
+    if (req <= 0 && n_inst < 64) {
+        int quads = 1;
+        for (const HostInst& H : hs)
+            for (int c = 0; c < H.k; ++c)
+                quads = std::max(quads, (H.color_ptr[c + 1] - H.color_ptr[c] + 3) / 4 + 1);
+        req = std::max(256, std::min(1024, quads * maxRp));
+    }
+    const int block = choose_lds_block(maxRp, req);
     for (const HostInst& H : hs)
         if (block % H.Rp) return fail(DRIFT_PT_EINVAL, "replica counts in one batch need a common block size");
     const int grid = std::min(n_inst, 256 * 8);
