@@ -299,8 +299,6 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
     const int rnd = *G.round_base + rl, cur = rnd & 1;
     const uint32_t t = (uint32_t)(rnd * I.S + j);
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
-    for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
-    __syncthreads();
     using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
     constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
     constexpr int NW = GW * EW;
@@ -393,6 +391,9 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
         const uint32_t fm = (uint32_t)(m >> (tid & 32));
         if (fm) G.spw[p] = own ^ fm;
     }
+    // the slot sums' LDS is cleared here, not at entry, so no barrier precedes the loads
+    for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
+    __syncthreads();
     block_slot_reduce(d, a, R, Rp, red, G.dacc, cur);
 }
 
