@@ -5,8 +5,10 @@ libdriftpt.so) and on the oracle (oracle/pt_oracle.c) and compares spins, energi
 decisions, the rung permutation and the best state exactly.  Covered: integer couplings (int8,
 int32, half-integer QUBO scale, fields), float couplings, both engines, several tile sizes, replica
 counts that are not powers of two, R = 1 and 2, the 2-D ferromagnet of config C1 (L=4 and L=16),
-the first 10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), and a batch of uneven float instances
-in one persistent launch.  Reference GPU tests (tests/test_gpu.py:45-67) are reproduced last.
+the first 10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), a batch of uneven float instances
+in one persistent launch, every first-chunk width of the padded edge records (with rows longer
+than the chunk), and both barrier energy paths of the bit-sliced persistent slab engine.
+Reference GPU tests (tests/test_gpu.py:45-67) are reproduced last.
 """
 
 import os
