@@ -576,11 +576,19 @@ void clear_result(drift_pt_result* r) {
 }
 
 // Copy results of one instance back and undo the colour relabelling.
+// mirror (optional): a host copy of device memory starting at device address mbase that covers
+// this instance's outputs; reads come from it instead of one device copy each.
 int fetch_results(const HostInst& H, const PtInstance& I, const std::vector<int64_t>& out,
-                  const std::vector<int8_t>& final_pos_major, drift_pt_result* res) {
+                  const std::vector<int8_t>& final_pos_major, drift_pt_result* res,
+                  const uint8_t* mirror = nullptr, const uint8_t* mbase = nullptr) {
     const int n = H.n, R = H.R;
+    auto get = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (!mirror) return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+        std::memcpy(dst, mirror + ((const uint8_t*)src - mbase), bytes);
+        return hipSuccess;
+    };
     std::vector<int8_t> best((size_t)n);
-    PT_HIP(hipMemcpy(best.data(), I.best_s, (size_t)n, hipMemcpyDeviceToHost));
+    PT_HIP(get(best.data(), I.best_s, (size_t)n));
     for (int p = 0; p < n; ++p) res->best_s[H.orig[p]] = best[p];
     res->best_E_units = out[0];
     res->best_E = (double)out[0] * H.unit;
@@ -606,18 +614,16 @@ int fetch_results(const HostInst& H, const PtInstance& I, const std::vector<int6
                     res->final_spins[(size_t)a * n + H.orig[p]] = final_pos_major[(size_t)p * H.Rp + a];
         } else {
             std::vector<int8_t> fs((size_t)n * R);
-            PT_HIP(hipMemcpy(fs.data(), I.final_s, fs.size(), hipMemcpyDeviceToHost));
+            PT_HIP(get(fs.data(), I.final_s, fs.size()));
             for (int a = 0; a < R; ++a)
                 for (int p = 0; p < n; ++p)
                     res->final_spins[(size_t)a * n + H.orig[p]] = fs[(size_t)a * n + p];
         }
     }
     if (res->best_history && (H.flags & DRIFT_PT_RECORD_HISTORY) && H.n_rounds > 0)
-        PT_HIP(hipMemcpy(res->best_history, I.hist, sizeof(double) * (size_t)H.n_rounds,
-                         hipMemcpyDeviceToHost));
+        PT_HIP(get(res->best_history, I.hist, sizeof(double) * (size_t)H.n_rounds));
     if (res->rung_energy_trace && (H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
-        PT_HIP(hipMemcpy(res->rung_energy_trace, I.trace,
-                         sizeof(double) * (size_t)H.n_rounds * R, hipMemcpyDeviceToHost));
+        PT_HIP(get(res->rung_energy_trace, I.trace, sizeof(double) * (size_t)H.n_rounds * R));
     res->attempts = (double)R * (double)H.n_rounds * (double)H.S * (double)n;
     return 0;
 }
@@ -685,18 +691,39 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
     PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    // Every instance's outputs live in the one arena allocation: one copy brings them all back.
+    const uint8_t *lo = nullptr, *hi = nullptr;
+    auto span = [&](const void* p, size_t bytes) {
+        if (!p || !bytes) return;
+        const uint8_t* a = (const uint8_t*)p;
+        if (!lo || a < lo) lo = a;
+        if (!hi || a + bytes > hi) hi = a + bytes;
+    };
+    for (int i = 0; i < n_inst; ++i) {
+        const HostInst& H = hs[i];
+        const PtInstance& I = insts[i];
+        span(I.out, sizeof(int64_t) * (size_t)pt_out_len(H.R));
+        span(I.best_s, (size_t)H.n);
+        if (H.flags & DRIFT_PT_RECORD_FINAL) span(I.final_s, (size_t)H.n * H.R);
+        if (H.flags & DRIFT_PT_RECORD_HISTORY) span(I.hist, sizeof(double) * (size_t)H.n_rounds);
+        if (H.flags & DRIFT_PT_RECORD_TRACE)
+            span(I.trace, sizeof(double) * (size_t)H.n_rounds * H.R);
+    }
+    std::vector<uint8_t> mirror(lo ? (size_t)(hi - lo) : 0);
+    if (lo) PT_HIP(hipMemcpy(mirror.data(), lo, mirror.size(), hipMemcpyDeviceToHost));
+    double total = 0.0;
+    for (const HostInst& Hx : hs) total += (double)Hx.R * Hx.n_rounds * Hx.S * Hx.n;
     for (int i = 0; i < n_inst; ++i) {
         const HostInst& H = hs[i];
         std::vector<int64_t> out((size_t)pt_out_len(H.R));
-        PT_HIP(hipMemcpy(out.data(), insts[i].out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-        int rc = fetch_results(H, insts[i], out, {}, results[i]);
+        std::memcpy(out.data(), mirror.data() + ((const uint8_t*)insts[i].out - lo),
+                    out.size() * sizeof(int64_t));
+        int rc = fetch_results(H, insts[i], out, {}, results[i], mirror.data(), lo);
         if (rc) return rc;
         results[i]->seconds = ms * 1e-3;
         results[i]->engine_used = DRIFT_PT_ENGINE_LDS;
         results[i]->kernel_seconds = ms * 1e-3;      // one persistent launch does everything
         results[i]->kernel_launches = 1;
-        double total = 0.0;
-        for (const HostInst& Hx : hs) total += (double)Hx.R * Hx.n_rounds * Hx.S * Hx.n;
         results[i]->kernel_attempts = total;
     }
     return 0;
