@@ -64,6 +64,10 @@ def _csr_of(model):
 
 def _integer_encoding(weight: np.ndarray, h: np.ndarray):
     """Smallest k in {0, 1, 2} such that 2^k J and 2^k h are integers below 2^30, else None."""
+    w4, h4 = weight * 4.0, h * 4.0
+    if (np.all(np.isfinite(w4)) and np.all(np.isfinite(h4))
+            and not (np.array_equal(w4, np.round(w4)) and np.array_equal(h4, np.round(h4)))):
+        return None   # 4J or 4h has a fraction, so 2^k J or 2^k h has one for every k <= 2
     for k in range(3):
         sc = float(1 << k)
         ws, hs = weight * sc, h * sc
