@@ -17,6 +17,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -1641,11 +1642,31 @@ int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* 
         return fail(DRIFT_PT_EINVAL, "bad batch arguments");
     std::vector<HostInst> hs((size_t)n_instances);
     std::vector<drift_pt_result*> rs((size_t)n_instances);
-    for (int i = 0; i < n_instances; ++i) {
+    for (int i = 0; i < n_instances; ++i)
         if (!results[i].best_s) return fail(DRIFT_PT_EINVAL, "result.best_s is required");
-        clear_result(&results[i]);
-        int rc = prepare(&problems[i], &params[i], hs[i]);
-        if (rc) return fail(rc, "instance " + std::to_string(i) + ": " + g_err);
+    // instances are prepared independently (colouring, relabelling, thresholds), on several host
+    // threads for large batches; errors are reported for the first failing instance
+    std::vector<int> rcs((size_t)n_instances, 0);
+    std::vector<std::string> errs((size_t)n_instances);
+    auto work = [&](int first, int stride) {
+        for (int i = first; i < n_instances; i += stride) {
+            clear_result(&results[i]);
+            rcs[i] = prepare(&problems[i], &params[i], hs[i]);
+            if (rcs[i]) errs[i] = g_err;
+        }
+    };
+    const int nthr = n_instances >= 64
+                         ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()))
+                         : 1;
+    if (nthr > 1) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthr; ++t) pool.emplace_back(work, t, nthr);
+        for (std::thread& th : pool) th.join();
+    } else {
+        work(0, 1);
+    }
+    for (int i = 0; i < n_instances; ++i) {
+        if (rcs[i]) return fail(rcs[i], "instance " + std::to_string(i) + ": " + errs[i]);
         if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "all instances of a batch need one weight_type");
         rs[i] = &results[i];
     }
