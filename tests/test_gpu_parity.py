@@ -369,6 +369,48 @@ def test_pslab_energy_paths(case, path, monkeypatch):
     assert g.engine == "pslab"
 
 
+def _chain_fingerprint(r):
+    return (int(r.best_E_units), int(r.best_round), r.best_s.astype(np.int8).tobytes(),
+            r.final_spins.tobytes(), np.asarray(r.final_E).tobytes(), np.asarray(r.rung_of).tobytes(),
+            np.asarray(r.best_history).tobytes(), np.asarray(r.swap_accepted).tobytes())
+
+
+def test_pslab_energy_paths_agree_over_a_long_run(monkeypatch):
+    """The two barrier energy paths on the full C2 lattice (3-D +/-J, L=32, R=64) for 2*10^4
+    sweeps, far past what the oracle runs in a test: identical chains."""
+    _announce("pslab energy paths, C2 lattice, 20000 sweeps")
+    model = cubic_pmj_glass(32, 0)
+    beta = ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))
+    fp = {}
+    for path in ("atomic", "words"):
+        monkeypatch.setenv("DRIFT_PT_PSLAB_ENERGY_PATH", path)
+        r = gpu.parallel_tempering_gpu(model, n_replicas=64, betas=beta, n_rounds=20000,
+                                       sweeps_per_round=1, seed=5, engine="pslab",
+                                       record_history=True, return_final=True)
+        assert r.engine == "pslab"
+        fp[path] = _chain_fingerprint(r)
+    assert fp["atomic"] == fp["words"]
+
+
+def test_csr_engines_agree_over_many_rounds():
+    """The C3 instance (Erdos-Renyi N = 10^5, mean degree 6, int8 weights, R = 32) for 300 rounds on
+    the spin-word colour step, the int8-spin colour step and the persistent CSR engine: identical
+    chains."""
+    from drift_amd.builders.lattice import er_sparse_weighted
+
+    _announce("CSR engines, C3 instance, 300 rounds")
+    model = er_sparse_weighted(100000, 6.0, 0)
+    beta = ladder_betas(geometric_ladder(32, 0.2, 20.0))
+    fps = []
+    for engine, block in (("global", 256), ("global", 96), ("coop", 0)):
+        r = gpu.parallel_tempering_gpu(model, n_replicas=32, betas=beta, n_rounds=300,
+                                       sweeps_per_round=1, seed=9, engine=engine,
+                                       block_threads=block, record_history=True, return_final=True)
+        assert r.engine == engine
+        fps.append(_chain_fingerprint(r))
+    assert fps[0] == fps[1] == fps[2]
+
+
 @pytest.mark.parametrize("engine", ["pslab", "coop"])
 def test_persistent_engines_with_an_ordinary_dispatch(engine):
     """DRIFT_PT_PLAIN_LAUNCH starts the persistent engines without the cooperative-launch API;
