@@ -427,6 +427,25 @@ int check_device() {
     return 0;
 }
 
+// Bound of one grid-barrier wait of the persistent engines, in device wall-clock ticks: 20 s, or
+// DRIFT_PT_BARRIER_TIMEOUT_MS milliseconds when that is set (0 makes the first wait that has to
+// spin give up, so the abort path can be exercised from the host; every wave still exits).
+unsigned long long barrier_timeout_ticks() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+        (void)hipGetLastError();
+        khz = 100000;
+    }
+    unsigned long long ms = 20000ull;
+    if (const char* e = std::getenv("DRIFT_PT_BARRIER_TIMEOUT_MS")) {
+        char* end = nullptr;
+        const unsigned long long v = std::strtoull(e, &end, 10);
+        if (end != e && *end == 0) ms = std::min(v, 3600000ull);
+    }
+    return (unsigned long long)khz * ms;
+}
+
 // Device copies of one instance's arrays, and the PtInstance pointing at them.
 // Device copies of many instances' arrays in one allocation: every input array is packed into a
 // host staging blob at 256-byte aligned offsets and copied once; outputs get their own aligned
@@ -1048,11 +1067,7 @@ int run_coop(const HostInst& H, drift_pt_result* res, CoopPlan P) {
     const int reg = std::max(1, std::min(256, (H.n + 1023) / 1024));   // round-end grid
     PT_HIP(pt_launch_g_init(I, G, iblock, reg, sg.s));
     PT_HIP(hipMemsetAsync(G.dacc, 0, 3 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
-    int dev = 0, khz = 0;
-    PT_HIP(hipGetDevice(&dev));
-    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
-        khz = 100000;
-    P.C.timeout_ticks = (unsigned long long)khz * 1000ull * 20ull;   // 20 s per barrier
+    P.C.timeout_ticks = barrier_timeout_ticks();
     PT_HIP(hipEventRecord(sg.e0, sg.s));
     if (H.n_rounds > 0)
         PT_HIP(pt_launch_coop(I, G, P.C, P.block, P.grid, P.lds,
@@ -1316,6 +1331,7 @@ int energy_path_choice(const HostInst& H, int block_req, int block) {
     (void)hipGetLastError();
     g_err.clear();
     const int choice = secs[1] > 0.0 && (secs[0] <= 0.0 || secs[1] < secs[0]) ? 1 : 0;
+    if (secs[0] <= 0.0 && secs[1] <= 0.0) return choice;   // neither trial ran: nothing learned
     std::lock_guard<std::mutex> lk(mu);
     cache[key] = choice;
     return choice;
@@ -1368,11 +1384,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
         PT_HIP(hipMemsetAsync(C.eflags, 0, sizeof(uint64_t) * W, sg.s));
         PT_HIP(hipMemsetAsync(C.erel, 0, sizeof(uint64_t) * 64, sg.s));
     }
-    int dev = 0, khz = 0;
-    PT_HIP(hipGetDevice(&dev));
-    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
-        khz = 100000;
-    C.timeout_ticks = (unsigned long long)khz * 1000ull * 20ull;   // 20 s per barrier
+    C.timeout_ticks = barrier_timeout_ticks();
     PT_HIP(hipEventRecord(sg.e0, sg.s));
     PT_HIP(pt_launch_pslab(I, S, C, block, lds, !(H.flags & DRIFT_PT_PLAIN_LAUNCH), sg.s));
     PT_HIP(hipEventRecord(sg.e1, sg.s));

@@ -444,6 +444,12 @@ __device__ __forceinline__ uint64_t tag_word(uint32_t phase, int64_t v) {
 }
 __device__ __forceinline__ int64_t word_value(uint64_t w) { return (int64_t)(w << 16) >> 16; }
 
+// Barrier phases count up without bound and wrap at 2^32; a flag is never more than one phase
+// ahead of its reader, so "v has not reached phase" is the sign of the 32-bit difference.
+__device__ __forceinline__ bool phase_before(uint32_t v, uint32_t phase) {
+    return (int32_t)(v - phase) < 0;
+}
+
 // Grid barrier with a bounded wait.  Each wave waits for its own stores to complete; then every
 // workgroup stores the phase number into its own arrival flag, workgroup 0 polls all flags (one
 // thread per flag) and publishes the phase in the generation word, which the others poll.  No
@@ -452,7 +458,7 @@ __device__ __forceinline__ int64_t word_value(uint64_t w) { return (int64_t)(w <
 // abort flag; the caller then leaves the kernel, so every wave reaches the end.
 __device__ __forceinline__ bool wait_phase(const CoopArgs& C, const uint32_t* p, uint32_t phase) {
     const unsigned long long t0 = (unsigned long long)wall_clock64();
-    while (ld_u32(p) < phase) {
+    while (phase_before(ld_u32(p), phase)) {
         if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
             (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
             __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

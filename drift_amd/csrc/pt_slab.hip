@@ -802,7 +802,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                 }
                 // pre: wave 1 drew this exchange's uniforms during the barrier (a short wait)
                 if (pre && rnd > 0)
-                    while (__hip_atomic_load(&sh_uready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < phase)
+                    while (phase_before(__hip_atomic_load(&sh_uready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), phase))
                         __builtin_amdgcn_s_sleep(1);
                 const bool imp = wave_ladder_round(WL, I, rnd, E, tid, owner,
                                                    pre && rnd > 0 ? sh_uex : nullptr);
@@ -1124,7 +1124,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                             __hip_atomic_store(&sh_uready, phase, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     if (live && t + 1 < nsweeps) draw(t + 1, tid - PT_WAVE, nt - PT_WAVE);
-                    while (__hip_atomic_load(&sh_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < phase)
+                    while (phase_before(__hip_atomic_load(&sh_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), phase))
                         __builtin_amdgcn_s_sleep(1);
                 }
                 if (last) continue;   // the round end brings in the halo planes

@@ -433,6 +433,47 @@ def test_persistent_engines_with_an_ordinary_dispatch(engine):
     assert np.array_equal(g.final_spins, o["final_spins"])
 
 
+@pytest.mark.parametrize("engine", ["global", "lattice", "slab"])
+def test_direct_launches_equal_graph_replay(engine):
+    """The per-launch engines replay chunks of rounds from a captured hipGraph and launch the
+    remaining rounds directly; with DRIFT_PT_NO_GRAPH every round launches directly.  The round
+    counts make two full replays plus a remainder.  Both runs equal the oracle and each other."""
+    from drift_amd.builders.lattice import er_sparse_weighted
+
+    _announce(f"graph replay vs direct launches, {engine}")
+    model = er_sparse_weighted(3000, 6.0, 4) if engine == "global" else cubic_pmj_glass(8, 4)
+    S = 2 if engine == "slab" else 1
+    rounds = 1100 if engine == "slab" else (400 if engine == "global" else 700)
+    fps = []
+    for graph in (True, False):
+        g = compare(model, R=16, n_rounds=rounds, S=S, seed=13, T_min=0.3, T_max=4.0,
+                    engine=engine, graph=graph)[0]
+        assert g.engine == engine
+        fps.append(_chain_fingerprint(g))
+    assert fps[0] == fps[1]
+
+
+@pytest.mark.parametrize("case", [("pslab", "atomic"), ("pslab", "words"), ("coop", "atomic")])
+def test_bounded_barrier_gives_up_from_the_host(case, monkeypatch):
+    """A grid-barrier bound of 0 ms (DRIFT_PT_BARRIER_TIMEOUT_MS, set from the host) makes the
+    first barrier wait that has to spin give up: the abort flag reaches every workgroup, every
+    wave leaves the kernel, and the call fails with the engine's message instead of hanging.
+    The next call, with the default bound, runs normally and matches the oracle."""
+    from drift_amd.builders.lattice import er_sparse_weighted
+
+    engine, path = case
+    _announce(f"barrier give-up {engine} ({path})")
+    monkeypatch.setenv("DRIFT_PT_PSLAB_ENERGY_PATH", path)
+    model = cubic_pmj_glass(32, 5) if engine == "pslab" else er_sparse_weighted(5000, 6.0, 9)
+    monkeypatch.setenv("DRIFT_PT_BARRIER_TIMEOUT_MS", "0")
+    with pytest.raises(RuntimeError, match="exceeded its bound"):
+        gpu.parallel_tempering_gpu(model, n_replicas=16, n_rounds=40, sweeps_per_round=2, seed=6,
+                                   engine=engine)
+    monkeypatch.delenv("DRIFT_PT_BARRIER_TIMEOUT_MS")
+    g = compare(model, R=16, n_rounds=20, S=2, seed=6, T_min=0.3, T_max=4.0, engine=engine)[0]
+    assert g.engine == engine
+
+
 def test_reference_problem_file_runs_on_the_engine(tmp_path):
     """A problem.bin as drift/gpu.py writes it, solved by run_problem_bin: the result.bin reads back
     as the reference parses it, and equals a direct call with the file's parameters."""
