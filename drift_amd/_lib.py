@@ -32,6 +32,7 @@ RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GR
 AUTOTUNE = 64
 PLAIN_LAUNCH = 128
 TUNE_MAX = 16
+NOALLOC_BYTES = 64 << 20   # DRIFT_PT_NOALLOC_BYTES
 
 
 class Problem(ctypes.Structure):
@@ -116,6 +117,8 @@ SIGNATURES = {
         [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)], c_int),
     "drift_pt_run": ([POINTER(Problem), POINTER(Params), POINTER(Result)], c_int),
     "drift_pt_run_batch": ([POINTER(Problem), POINTER(Params), c_int32, POINTER(Result)], c_int),
+    "drift_pt_run_batch_noalloc": ([POINTER(Problem), POINTER(Params), c_int32, POINTER(Result)],
+                                   c_int),
     "drift_pt_autotune": ([POINTER(Problem), POINTER(Params), c_int32, POINTER(Tuning)], c_int),
     "drift_pt_autotune_clear": ([], None),
 }

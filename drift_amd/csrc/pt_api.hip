@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -45,16 +46,33 @@ namespace {
                         std::string(#call) + ": " + hipGetErrorString(e_));                   \
     } while (0)
 
-// Device allocations owned by one call; freed on every exit path.
+// Device memory of one call.  By default every allocation is a hipMalloc, freed on every exit
+// path.  With a fixed region (the allocation-free batch entry point) allocations are 256-byte
+// aligned slices of that region instead: nothing is allocated or freed, and a request past its end
+// fails with hipErrorOutOfMemory, leaving in `need` the bytes the call asked for up to then.
 struct DevArena {
     std::vector<void*> ptrs;
+    uint8_t* region = nullptr;
+    size_t cap = 0, used = 0, need = 0;
     ~DevArena() {
         for (void* p : ptrs) (void)hipFree(p);
     }
     template <class T>
     hipError_t alloc(T** out, size_t count) {
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (region) {
+            const size_t at = (used + 255) & ~(size_t)255;
+            need = at + bytes;
+            if (need > cap) {
+                *out = nullptr;
+                return hipErrorOutOfMemory;
+            }
+            used = need;
+            *out = (T*)(region + at);
+            return hipSuccess;
+        }
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+        hipError_t e = hipMalloc(&p, bytes);
         if (e == hipSuccess) ptrs.push_back(p);
         *out = (T*)p;
         return e;
@@ -657,9 +675,12 @@ int choose_lds_block(int Rp, int requested) {
     return std::min(b, 1024);
 }
 
+// fixed (optional): carve every device buffer from this arena's fixed region and run on the null
+// stream, timed on the host, so the call creates no device memory, stream or event.
 int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& results,
-            int block_req) {
-    DevArena A;
+            int block_req, DevArena* fixed = nullptr) {
+    DevArena own;
+    DevArena& A = fixed ? *fixed : own;
     const int n_inst = (int)hs.size();
     std::vector<PtInstance> insts((size_t)n_inst);
     size_t lds = 0;
@@ -701,16 +722,26 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
         if (block % H.Rp) return fail(DRIFT_PT_EINVAL, "replica counts in one batch need a common block size");
     const int grid = std::min(n_inst, 256 * 8);
     StreamGuard sg;
-    PT_HIP(hipStreamCreate(&sg.s));
-    PT_HIP(hipEventCreate(&sg.e0));
-    PT_HIP(hipEventCreate(&sg.e1));
-    PT_HIP(hipMemsetAsync(d_queue, 0, sizeof(int32_t), sg.s));
-    PT_HIP(hipEventRecord(sg.e0, sg.s));
-    PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, hs[0].wm, block, grid, lds, sg.s));
-    PT_HIP(hipEventRecord(sg.e1, sg.s));
-    PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
-    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    if (fixed) {
+        PT_HIP(hipMemsetAsync(d_queue, 0, sizeof(int32_t), 0));
+        PT_HIP(hipStreamSynchronize(0));
+        const auto t0 = std::chrono::steady_clock::now();
+        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, hs[0].wm, block, grid, lds, 0));
+        PT_HIP(hipStreamSynchronize(0));
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                 .count();
+    } else {
+        PT_HIP(hipStreamCreate(&sg.s));
+        PT_HIP(hipEventCreate(&sg.e0));
+        PT_HIP(hipEventCreate(&sg.e1));
+        PT_HIP(hipMemsetAsync(d_queue, 0, sizeof(int32_t), sg.s));
+        PT_HIP(hipEventRecord(sg.e0, sg.s));
+        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, hs[0].wm, block, grid, lds, sg.s));
+        PT_HIP(hipEventRecord(sg.e1, sg.s));
+        PT_HIP(hipStreamSynchronize(sg.s));
+        PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    }
     // Every instance's outputs live in the one arena allocation: one copy brings them all back.
     const uint8_t *lo = nullptr, *hi = nullptr;
     auto span = [&](const void* p, size_t bytes) {
@@ -1647,13 +1678,19 @@ void drift_pt_autotune_clear(void) {
     g_tune.clear();
 }
 
-int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
-                       int32_t n_instances, drift_pt_result* results) {
-    g_err.clear();
+}  // extern "C"
+
+namespace {
+
+// Preparation shared by the two batch entry points: every instance validated, coloured,
+// relabelled and given its thresholds; all must fit one workgroup's LDS.
+int prepare_batch(const drift_pt_problem* problems, const drift_pt_params* params,
+                  int32_t n_instances, drift_pt_result* results, std::vector<HostInst>& hs,
+                  std::vector<drift_pt_result*>& rs) {
     if (n_instances < 1 || !problems || !params || !results)
         return fail(DRIFT_PT_EINVAL, "bad batch arguments");
-    std::vector<HostInst> hs((size_t)n_instances);
-    std::vector<drift_pt_result*> rs((size_t)n_instances);
+    hs.assign((size_t)n_instances, HostInst());
+    rs.assign((size_t)n_instances, nullptr);
     for (int i = 0; i < n_instances; ++i)
         if (!results[i].best_s) return fail(DRIFT_PT_EINVAL, "result.best_s is required");
     // instances are prepared independently (colouring, relabelling, thresholds), on several host
@@ -1686,7 +1723,79 @@ int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* 
     if (rc) return rc;
     for (const HostInst& H : hs)
         if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "batched instances must fit one workgroup's LDS");
+    return 0;
+}
+
+// The allocation-free entry point's device region.  Every device byte drift_pt_run_batch_noalloc
+// uses -- the packed instance arrays, the instance descriptors, the work queue, the statistics
+// blocks and the other outputs -- is a slice of it.  The module holds one copy per device from
+// the time the library's code object is loaded on that device until the process ends.  Host code
+// reaches it only through the per-device address hipGetSymbolAddress returns (noalloc_region).
+static __device__ __align__(256) uint8_t scratch[DRIFT_PT_NOALLOC_BYTES];
+
+struct NoallocSlot {
+    std::mutex mu;            // one call at a time per device: calls share the region
+    uint8_t* ptr = nullptr;   // device address of this device's copy of scratch
+};
+NoallocSlot g_noalloc[64];
+
+// The slot of the current device, its region's address looked up on first use; null (with the
+// error set) when that fails.
+NoallocSlot* noalloc_region() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        fail(DRIFT_PT_EHIP, "allocation-free batch: no current device");
+        return nullptr;
+    }
+    NoallocSlot& S = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(S.mu);
+    if (!S.ptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fail(DRIFT_PT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        S.ptr = (uint8_t*)p;
+    }
+    return &S;
+}
+
+}  // namespace
+
+extern "C" {
+
+int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
+                       int32_t n_instances, drift_pt_result* results) {
+    g_err.clear();
+    std::vector<HostInst> hs;
+    std::vector<drift_pt_result*> rs;
+    int rc = prepare_batch(problems, params, n_instances, results, hs, rs);
+    if (rc) return rc;
     return run_lds(hs, rs, 0);
+}
+
+int drift_pt_run_batch_noalloc(const drift_pt_problem* problems, const drift_pt_params* params,
+                               int32_t n_instances, drift_pt_result* results) {
+    g_err.clear();
+    std::vector<HostInst> hs;
+    std::vector<drift_pt_result*> rs;
+    int rc = prepare_batch(problems, params, n_instances, results, hs, rs);
+    if (rc) return rc;
+    NoallocSlot* S = noalloc_region();
+    if (!S) return DRIFT_PT_EHIP;
+    std::lock_guard<std::mutex> lk(S->mu);
+    DevArena A;
+    A.region = S->ptr;
+    A.cap = DRIFT_PT_NOALLOC_BYTES;
+    rc = run_lds(hs, rs, 0, &A);
+    if (rc && A.need > A.cap)
+        return fail(DRIFT_PT_ERANGE,
+                    "allocation-free batch: the instances need at least " + std::to_string(A.need) +
+                        " bytes of device memory, more than the " +
+                        std::to_string((size_t)DRIFT_PT_NOALLOC_BYTES) + " bytes it holds");
+    return rc;
 }
 
 }  // extern "C"

@@ -291,11 +291,15 @@ def parallel_tempering_gpu_batch(
     record_trace: bool = False,
     force_float: bool = False,
     block_threads: int = 0,
+    no_alloc: bool = False,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
     result equals ``parallel_tempering_gpu`` on that instance alone.  ``block_threads`` sets the
-    workgroup size (0: the library's choice)."""
+    workgroup size (0: the library's choice).  ``no_alloc`` runs the batch through
+    drift_pt_run_batch_noalloc, which allocates no device memory (a fixed region of
+    ``_lib.NOALLOC_BYTES`` per device; larger batches raise RuntimeError); the results are the
+    same."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -314,7 +318,7 @@ def parallel_tempering_gpu_batch(
     P = (_lib.Problem * m)(*[p.struct for p in probs])
     Q = (_lib.Params * m)(*prms)
     Rs = (_lib.Result * m)(*[bf.struct for bf in bufs])
-    rc = lib.drift_pt_run_batch(P, Q, m, Rs)
+    rc = (lib.drift_pt_run_batch_noalloc if no_alloc else lib.drift_pt_run_batch)(P, Q, m, Rs)
     if rc != 0:
         raise RuntimeError(f"GPU engine failed (status {rc}): {_lib.last_error(lib)}")
     return [_to_result(Rs[i], bufs[i], ladders[i][0], n_rounds, n_replicas, record_history)

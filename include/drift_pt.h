@@ -178,6 +178,23 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
                        int32_t n_instances, drift_pt_result* results);
 
+/* Size of the device region drift_pt_run_batch_noalloc works in: 64 MiB per device, about 900
+ * instances of 1000 spins with 16 replicas (each needs about 70 KB: its packed CSR, padded
+ * first-chunk records, field, ladder, statistics block and best state). */
+#define DRIFT_PT_NOALLOC_BYTES (64u << 20)
+
+/* drift_pt_run_batch without any device allocation, for callers that cannot allocate (code
+ * between graph captures, tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * same results bit for bit.  Every device buffer of the call -- the packed instances, their
+ * descriptors, the work queue and the outputs -- is carved from a region of
+ * DRIFT_PT_NOALLOC_BYTES that the library holds on each device for the life of the process; the
+ * call creates no stream or event (it runs on the null stream, and result.seconds is the host's
+ * clock around the launch).  A batch needing more than the region fails with DRIFT_PT_ERANGE.
+ * Calls from several threads on one device are serialised, because they share the region; the
+ * region keeps the last call's contents.  No reference counterpart. */
+int drift_pt_run_batch_noalloc(const drift_pt_problem* problems, const drift_pt_params* params,
+                               int32_t n_instances, drift_pt_result* results);
+
 /* Autotuner.  Times every engine and block size that can run this instance (LDS, slab, lattice,
  * global, coop, pslab; 256 / 512 / 1024 threads) for trial_rounds rounds (0: about 2.5e8 attempts),
  * stores the fastest in a process-wide cache keyed by the instance's shape class (weight type,

@@ -36,6 +36,8 @@ def test_library_loads_and_exports_every_declared_symbol():
     version = int(re.search(r"#define DRIFT_PT_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
     assert lib.drift_pt_abi_version() == version == _lib.ABI_VERSION
     assert lib.drift_pt_device_available() in (0, 1)
+    size = re.search(r"#define DRIFT_PT_NOALLOC_BYTES \((\d+)u << (\d+)\)", open(HEADER).read())
+    assert int(size.group(1)) << int(size.group(2)) == _lib.NOALLOC_BYTES
 
 
 def test_ctypes_layouts_match_the_header():
