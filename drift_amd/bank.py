@@ -155,9 +155,11 @@ def v2_specs() -> list[dict]:
     return out
 
 
-# The recorded best-known run of every v2 row: PT with R replicas over the geometric ladder
-# T_MIN..T_MAX, one sweep and one exchange per round; the runs are deterministic (the Philox
-# stream contract), so a replay reproduces the recorded energy exactly.
+# The best-known runs of every v2 row: PT with R replicas over the geometric ladder T_MIN..T_MAX,
+# one sweep and one exchange per round, from each of the independent seeds V2_SEEDS; the row's
+# best-known energy is the lowest of them (tests/golden/make_bank_v2.py).  The runs are
+# deterministic under the Philox stream contract.
+V2_SEEDS = (101, 102, 103, 104)
 V2_RUN = {"ea3d": dict(R=64, T_min=1 / 3.0, T_max=5.0),
           "er-sparse": dict(R=32, T_min=0.2, T_max=20.0),
           "rrg3": dict(R=32, T_min=0.1, T_max=3.0)}
@@ -168,8 +170,9 @@ def v2_rounds(spec: dict) -> int:
     return int(min(200_000, 2e11 // (V2_RUN[spec["family"]]["R"] * spec["n"])))
 
 
-def v2_best_known_run(spec: dict, n_rounds: int, seed: int = 1):
-    """The recorded best-known run of a v2 row; returns the PtResult."""
+def v2_best_known_run(spec: dict, n_rounds: int, seed: int = V2_SEEDS[0]):
+    """One best-known run of a v2 row (the row's ladder, ``n_rounds`` rounds); returns the
+    PtResult."""
     from . import gpu
 
     cfg = V2_RUN[spec["family"]]

@@ -537,7 +537,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         o.jb = H.ldim ? in.add(H.jbits) : SIZE_MAX;
     }
     // outputs after the inputs
-    struct OutOff { size_t best, fin, hist, trace, out; };
+    struct OutOff { size_t best, fin, hist, trace, out, img, e0; };
     std::vector<OutOff> oo(m);
     size_t cursor = (in.blob.size() + 255) & ~(size_t)255;
     const size_t out_begin = cursor;
@@ -560,6 +560,8 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
                          ? take(sizeof(double) * (size_t)H.n_rounds) : SIZE_MAX;
         oo[i].trace = ((H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
                           ? take(sizeof(double) * (size_t)H.n_rounds * H.R) : SIZE_MAX;
+        oo[i].img = with_ell ? take(pt_lds_img_bytes(H.n, H.Rp)) : SIZE_MAX;
+        oo[i].e0 = with_ell ? take(sizeof(int64_t) * (size_t)H.Rp) : SIZE_MAX;
     }
     uint8_t* base;
     PT_HIP(A.alloc(&base, cursor));
@@ -584,6 +586,8 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.hist = oo[i].hist == SIZE_MAX ? nullptr : (double*)(base + oo[i].hist);
         I.trace = oo[i].trace == SIZE_MAX ? nullptr : (double*)(base + oo[i].trace);
         I.out = (int64_t*)(base + oo[i].out);
+        I.img = oo[i].img == SIZE_MAX ? nullptr : (int8_t*)(base + oo[i].img);
+        I.e0 = oo[i].e0 == SIZE_MAX ? nullptr : (int64_t*)(base + oo[i].e0);
         I.unit = H.unit;
         I.k0 = (uint32_t)H.seed;
         I.k1 = (uint32_t)(H.seed >> 32);
@@ -705,7 +709,7 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     int32_t *d_order, *d_queue;
     PT_HIP(A.upload(&d_insts, insts));
     PT_HIP(A.upload(&d_order, order));
-    PT_HIP(A.alloc(&d_queue, 1));
+    PT_HIP(A.alloc(&d_queue, 2));   // work queues of the init launch and of the timed launch
     // A few instances leave most of the chip idle, so their workgroups are sized to sweep the
     // largest colour class in one pass (each pass is a dependent chain of loads); batches that
     // fill the chip keep 256 threads.
@@ -723,11 +727,15 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     const int grid = std::min(n_inst, 256 * 8);
     StreamGuard sg;
     float ms = 0.0f;
+    // The initial spins and energies are drawn by a launch of their own, so the timed launch is
+    // the round loop alone (cuda/ising_pt.cu:213-222 times the rounds, not init_kernel).
+    const int wm = hs[0].wm;
     if (fixed) {
-        PT_HIP(hipMemsetAsync(d_queue, 0, sizeof(int32_t), 0));
+        PT_HIP(hipMemsetAsync(d_queue, 0, 2 * sizeof(int32_t), 0));
+        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, wm, block, grid, lds, true, 0));
         PT_HIP(hipStreamSynchronize(0));
         const auto t0 = std::chrono::steady_clock::now();
-        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, hs[0].wm, block, grid, lds, 0));
+        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue + 1, wm, block, grid, lds, false, 0));
         PT_HIP(hipStreamSynchronize(0));
         ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                  .count();
@@ -735,9 +743,10 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
         PT_HIP(hipStreamCreate(&sg.s));
         PT_HIP(hipEventCreate(&sg.e0));
         PT_HIP(hipEventCreate(&sg.e1));
-        PT_HIP(hipMemsetAsync(d_queue, 0, sizeof(int32_t), sg.s));
+        PT_HIP(hipMemsetAsync(d_queue, 0, 2 * sizeof(int32_t), sg.s));
+        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, wm, block, grid, lds, true, sg.s));
         PT_HIP(hipEventRecord(sg.e0, sg.s));
-        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue, hs[0].wm, block, grid, lds, sg.s));
+        PT_HIP(pt_launch_lds(d_insts, d_order, n_inst, d_queue + 1, wm, block, grid, lds, false, sg.s));
         PT_HIP(hipEventRecord(sg.e1, sg.s));
         PT_HIP(hipStreamSynchronize(sg.s));
         PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));

@@ -36,7 +36,9 @@ __device__ __forceinline__ void slice(int cs, int cz, int w, int W, int& lo, int
 }
 
 template <int WM>
-using field_t = std::conditional_t<WM == PT_WM_F32, float, int32_t>;
+using field_t = std::conditional_t<WM == PT_WM_F32, float, int32_t>;    // one coupling
+template <int WM>
+using acc_t = std::conditional_t<WM == PT_WM_F32, double, int32_t>;     // a local field
 
 template <int WM>
 __device__ __forceinline__ int edge_at(const uint32_t* e, int t, field_t<WM>& w) {
@@ -53,8 +55,8 @@ __device__ __forceinline__ int edge_at(const uint32_t* e, int t, field_t<WM>& w)
 
 template <int WM>
 __device__ __forceinline__ void fields2(int a, int Rp, const int8_t* sp, const uint32_t* edge,
-                                        int s0, int e0, int s1, int e1, field_t<WM>& l0,
-                                        field_t<WM>& l1) {
+                                        int s0, int e0, int s1, int e1, acc_t<WM>& l0,
+                                        acc_t<WM>& l1) {
     constexpr int K = PT_PAIR_CHUNK;
     int c0[K], c1[K];
     int8_t v0[K], v1[K];
@@ -74,32 +76,32 @@ __device__ __forceinline__ void fields2(int a, int Rp, const int8_t* sp, const u
     }
 #pragma unroll
     for (int u = 0; u < K; ++u) {
-        const field_t<WM> n0 = v0[u] > 0 ? l0 + w0[u] : l0 - w0[u];
-        const field_t<WM> n1 = v1[u] > 0 ? l1 + w1[u] : l1 - w1[u];
+        const acc_t<WM> n0 = v0[u] > 0 ? l0 + (acc_t<WM>)w0[u] : l0 - (acc_t<WM>)w0[u];
+        const acc_t<WM> n1 = v1[u] > 0 ? l1 + (acc_t<WM>)w1[u] : l1 - (acc_t<WM>)w1[u];
         l0 = s0 + u < e0 ? n0 : l0;
         l1 = s1 + u < e1 ? n1 : l1;
     }
     for (int t = s0 + K; t < e0; ++t) {
         field_t<WM> w;
         const int c = edge_at<WM>(edge, t, w);
-        l0 = ld_spin<true>(sp + (size_t)c * Rp + a) > 0 ? l0 + w : l0 - w;
+        l0 = ld_spin<true>(sp + (size_t)c * Rp + a) > 0 ? l0 + (acc_t<WM>)w : l0 - (acc_t<WM>)w;
     }
     for (int t = s1 + K; t < e1; ++t) {
         field_t<WM> w;
         const int c = edge_at<WM>(edge, t, w);
-        l1 = ld_spin<true>(sp + (size_t)c * Rp + a) > 0 ? l1 + w : l1 - w;
+        l1 = ld_spin<true>(sp + (size_t)c * Rp + a) > 0 ? l1 + (acc_t<WM>)w : l1 - (acc_t<WM>)w;
     }
 }
 
 // Metropolis decision for spin si with local field l: true when the flip is accepted (a flip
 // with zero energy change is accepted), with its energy change in units in d.
 template <int WM>
-__device__ __forceinline__ bool decide(int8_t si, field_t<WM> l, uint32_t u,
+__device__ __forceinline__ bool decide(int8_t si, acc_t<WM> l, uint32_t u,
                                        const uint32_t* thr_row, int TQ, float bf, int64_t& d) {
     if constexpr (WM == PT_WM_F32) {
-        const float q = si > 0 ? l : -l;
-        d = pt_fix36(2.0f * q);
-        return pt_accept_float(bf, q, u);
+        const double q = si > 0 ? l : -l;
+        d = pt_fix36d(2.0 * q);
+        return pt_accept_float(bf, (float)q, u);
     } else {
         const int32_t q = si > 0 ? l : -l;
         d = 2 * (int64_t)q;
@@ -108,8 +110,8 @@ __device__ __forceinline__ bool decide(int8_t si, field_t<WM> l, uint32_t u,
 }
 
 template <int WM>
-__device__ __forceinline__ field_t<WM> field_h(const void* h, int p) {
-    if constexpr (WM == PT_WM_F32) return h ? ((const float*)h)[p] : 0.0f;
+__device__ __forceinline__ acc_t<WM> field_h(const void* h, int p) {
+    if constexpr (WM == PT_WM_F32) return h ? (double)((const float*)h)[p] : 0.0;
     else return h ? ((const int32_t*)h)[p] : 0;
 }
 
@@ -203,8 +205,8 @@ __global__ __launch_bounds__(1024) void coop_run(PtInstance I, GState G, CoopArg
                         const int p0 = lo + i0, p1 = lo + i1;
                         const int s0 = rows[i0], e0 = rows[i0 + 1];
                         const int s1 = has1 ? rows[i1] : 0, e1 = has1 ? rows[i1 + 1] : 0;
-                        field_t<WM> l0 = field_h<WM>(I.h, p0);
-                        field_t<WM> l1 = has1 ? field_h<WM>(I.h, p1) : field_t<WM>(0);
+                        acc_t<WM> l0 = field_h<WM>(I.h, p0);
+                        acc_t<WM> l1 = has1 ? field_h<WM>(I.h, p1) : acc_t<WM>(0);
                         fields2<WM>(a, Rp, G.sp, ledge, s0, e0, s1, e1, l0, l1);
                         const pt_u4 r0 = pt_philox((uint32_t)(p0 >> 2), (uint32_t)a, t,
                                                    PT_TAG_METRO, I.k0, I.k1);

@@ -134,6 +134,9 @@ __device__ __forceinline__ int edge_float(const void* __restrict__ e, int t, flo
     return (int)v.x;
 }
 
+// Sign bit of a +-1 spin as a float's sign: XORed into a coupling's bits it gives J * s exactly.
+__device__ __forceinline__ uint32_t pt_sign_bit(int8_t s) { return (uint32_t)(int32_t)s & 0x80000000u; }
+
 // Spin access.  COH accesses are agent-scope relaxed atomics (loads and stores coherent across
 // the chip without cache maintenance): the coop engine's workgroups read spins other workgroups
 // wrote in the same launch.  Other engines use plain accesses.
@@ -181,8 +184,8 @@ __device__ __forceinline__ int32_t field_int(int a, int Rp, const int8_t* sp, in
 }
 
 template <int G, bool COH = false>
-__device__ __forceinline__ float field_float(int a, int Rp, const int8_t* sp, int t0, int t1,
-                                             const void* __restrict__ edge, float l) {
+__device__ __forceinline__ double field_float(int a, int Rp, const int8_t* sp, int t0, int t1,
+                                              const void* __restrict__ edge, double l) {
     for (int t = t0; t < t1; t += G) {
         int c[G];
         float w[G];
@@ -197,7 +200,7 @@ __device__ __forceinline__ float field_float(int a, int Rp, const int8_t* sp, in
         for (int u = 0; u < G; ++u) v[u] = ld_spin<COH>(sp + (size_t)c[u] * Rp + a);
 #pragma unroll
         for (int u = 0; u < G; ++u) {
-            const float nl = v[u] > 0 ? l + w[u] : l - w[u];
+            const double nl = l + (double)__uint_as_float(__float_as_uint(w[u]) ^ pt_sign_bit(v[u]));
             l = t + u < t1 ? nl : l;
         }
     }
@@ -213,8 +216,8 @@ __device__ __forceinline__ float field_float(int a, int Rp, const int8_t* sp, in
 // rows are gathered before any spin of the quad changes: the records and the neighbour spins each
 // wait on memory once per quad.  Only when some row is longer than G (rest) are the row pointers
 // read and the longer rows finished from the CSR in G-edge chunks.  Each field is summed from h in
-// row order, one rounding per term (l +- w), as field_float / field_int sum it and as the oracle
-// does; a padding record adds +0 or -0, which changes at most the sign of a zero field, and a zero
+// row order (float couplings in double), one rounding per term (l +- w), as field_float /
+// field_int sum it and as the oracle does; a padding record adds +0 or -0, which changes at most the sign of a zero field, and a zero
 // field is accepted with zero energy change either way, so the decisions are the same.  Positions
 // of the quad outside [cs, ce) are gathered too (valid addresses) and then left alone.
 // Loads through global-address-space pointers wait on their own counter, while flat loads (what a
@@ -229,7 +232,7 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
                                                const void* __restrict__ ell, const void* h,
                                                bool rest, const pt_u4& rv,
                                                const uint32_t* thr_row, int TQ, float beta_f) {
-    using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    using LT = typename std::conditional<WM == PT_WM_F32, double, int32_t>::type;
     constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
     constexpr int NW = 4 * G * EW;                // words of the quad's first chunks
     uint32_t ev[NW];
@@ -248,8 +251,8 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
     if (h) {
         const pt_v4u hv = ((const PT_GLOBAL pt_v4u*)h)[q];
         if constexpr (WM == PT_WM_F32) {
-            l[0] = __uint_as_float(hv.x); l[1] = __uint_as_float(hv.y);
-            l[2] = __uint_as_float(hv.z); l[3] = __uint_as_float(hv.w);
+            l[0] = (double)__uint_as_float(hv.x); l[1] = (double)__uint_as_float(hv.y);
+            l[2] = (double)__uint_as_float(hv.z); l[3] = (double)__uint_as_float(hv.w);
         } else {
             l[0] = (int32_t)hv.x; l[1] = (int32_t)hv.y; l[2] = (int32_t)hv.z; l[3] = (int32_t)hv.w;
         }
@@ -269,10 +272,10 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 const int e = k * G + u;
-                // the spin is +-1, so w * s is exact and the fused form rounds l +- w once,
-                // exactly as the select of l + w and l - w does
+                // the spin is +-1, so w * s is w with the spin's sign bit XORed in: exact, and
+                // l + w*s rounds once, as the select of l + w and l - w does
                 if constexpr (WM == PT_WM_F32) {
-                    l[k] = __builtin_fmaf(__uint_as_float(ev[EW * e + 1]), (float)v[e], l[k]);
+                    l[k] += (double)__uint_as_float(ev[EW * e + 1] ^ pt_sign_bit(v[e]));
                 } else if constexpr (WM == PT_WM_I8) {
                     l[k] += __mul24((int32_t)(int8_t)(ev[e] & 0xffu), (int32_t)v[e]);
                 } else {
@@ -303,10 +306,10 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
     for (int k = 0; k < 4; ++k) {
         if (!on[k]) continue;
         if constexpr (WM == PT_WM_F32) {
-            const float qf = si[k] > 0 ? l[k] : -l[k];
-            if (pt_accept_float(beta_f, qf, pt_word(rv, k))) {
+            const double qd = si[k] > 0 ? l[k] : -l[k];
+            if (pt_accept_float(beta_f, (float)qd, pt_word(rv, k))) {
                 own[k * Rp] = (int8_t)-si[k];
-                d += pt_fix36(2.0f * qf);
+                d += pt_fix36d(2.0 * qd);
             }
         } else {
             const int32_t qi = si[k] > 0 ? l[k] : -l[k];
@@ -331,7 +334,7 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
                                                    const void* __restrict__ ell, bool rest,
                                                    const void* h, uint32_t u,
                                                    const uint32_t* thr_row, int TQ, float beta_f) {
-    using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    using LT = typename std::conditional<WM == PT_WM_F32, double, int32_t>::type;
     constexpr int EW = WM == PT_WM_I8 ? 1 : 2;
     constexpr int NW = GW * EW;
     uint32_t ev[NW];
@@ -353,7 +356,11 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
     }
     int8_t* own = sp + (size_t)p * Rp + a;
     const int8_t si = *own;
-    LT l = h ? ((const LT*)h)[p] : (LT)0;
+    LT l = (LT)0;
+    if (h) {
+        if constexpr (WM == PT_WM_F32) l = (double)((const float*)h)[p];
+        else l = ((const int32_t*)h)[p];
+    }
     int8_t v[GW];
 #pragma unroll
     for (int e = 0; e < GW; ++e)
@@ -361,7 +368,7 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
 #pragma unroll
     for (int e = 0; e < GW; ++e) {
         if constexpr (WM == PT_WM_F32) {
-            l = __builtin_fmaf(__uint_as_float(ev[EW * e + 1]), (float)v[e], l);
+            l += (double)__uint_as_float(ev[EW * e + 1] ^ pt_sign_bit(v[e]));
         } else if constexpr (WM == PT_WM_I8) {
             l += __mul24((int32_t)(int8_t)(ev[e] & 0xffu), (int32_t)v[e]);
         } else {
@@ -374,10 +381,10 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
         else l = field_int<WM, GW>(a, Rp, sp, t0 + GW, t1, edge, l);
     }
     if constexpr (WM == PT_WM_F32) {
-        const float q = si > 0 ? l : -l;
-        if (pt_accept_float(beta_f, q, u)) {
+        const double q = si > 0 ? l : -l;
+        if (pt_accept_float(beta_f, (float)q, u)) {
             *own = (int8_t)-si;
-            return pt_fix36(2.0f * q);
+            return pt_fix36d(2.0 * q);
         }
         return 0;
     } else {
@@ -391,7 +398,8 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
 }
 
 // Energy contribution of position p in slot a at initialisation.  Integer modes return
-// s_p (sum_j J s_j + 2 h_p) (E = -total/2); float returns fix36(-s_p (l_J/2 + h_p)).
+// s_p (sum_j J s_j + 2 h_p) (E = -total/2); float returns fix36(-s_p (l_J/2 + h_p)), summed in
+// double.
 template <int WM, int G = 1>
 __device__ __forceinline__ int64_t site_energy(int p, int a, int Rp, const int8_t* sp,
                                                const int32_t* __restrict__ rp,
@@ -399,9 +407,9 @@ __device__ __forceinline__ int64_t site_energy(int p, int a, int Rp, const int8_
     const int t0 = rp[p], t1 = rp[p + 1];
     const int8_t si = sp[(size_t)p * Rp + a];
     if constexpr (WM == PT_WM_F32) {
-        const float lj = field_float<G>(a, Rp, sp, t0, t1, edge, 0.0f);
-        const float v = 0.5f * lj + (h ? ((const float*)h)[p] : 0.0f);
-        return pt_fix36(si > 0 ? -v : v);
+        const double lj = field_float<G>(a, Rp, sp, t0, t1, edge, 0.0);
+        const double v = 0.5 * lj + (h ? (double)((const float*)h)[p] : 0.0);
+        return pt_fix36d(si > 0 ? -v : v);
     } else {
         // |sum_j J s_j| < 2^31 is checked on the host (maximum field), so int32 is exact
         const int64_t lj = field_int<WM, G>(a, Rp, sp, t0, t1, edge, 0);

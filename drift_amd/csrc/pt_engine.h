@@ -42,6 +42,10 @@ struct PtInstance {
     double* hist;             // n_rounds or null
     double* trace;            // n_rounds * R energies by rung, or null
     int64_t* out;             // statistics block
+    int8_t* img;              // LDS engine: the initial spins as the LDS holds them ([position]
+                              // [slot], n * Rp bytes rounded up to 16), written by the init launch
+                              // and read by the launch that runs the rounds; null on other engines
+    int64_t* e0;              // LDS engine: the initial energy of every slot (Rp), likewise
     const uint8_t* jbits;     // lattice engine: per site, bit d set when the bond in direction
                               // d (+x,-x,+y,-y,+z,-z) is -1; null otherwise
     double unit;              // energy per unit
@@ -146,9 +150,14 @@ size_t pt_lds_bytes(int n, int Rp);
 int pt_fail(int code, const char* msg);
 void pt_clear_error();
 
+// One persistent launch of the LDS engine over the work queue *d_queue (start at 0).  init: each
+// workgroup draws its instances' initial spins and energies into PtInstance::img / e0; otherwise
+// it loads them from there and runs every round.
 hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int n_inst,
                          int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
-                         hipStream_t s);
+                         bool init, hipStream_t s);
+// Bytes of PtInstance::img for n positions and Rp slots.
+__host__ __device__ inline size_t pt_lds_img_bytes(int n, int Rp) { return ((size_t)n * Rp + 15) & ~(size_t)15; }
 
 hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
                             hipStream_t s);

@@ -37,7 +37,9 @@ struct LShared {
     int have, bround, brung, bslot, improved, inst;
 };
 
-template <int WM>
+// INIT: draw the instance's initial spins and energies and leave them in I.img / I.e0 (the launch
+// before the timed one).  Otherwise: load them and run every round.
+template <int WM, bool INIT>
 __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long long* scrE,
                              int* scrR) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -55,32 +57,36 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
     int32_t* trips = (int32_t*)(smem + off);  off += 4 * (size_t)Rp;
     float* beta_f = (float*)(smem + off);
 
-    // --- init: spins from the stream, bookkeeping, energies -------------------------------
-    for (int idx = tid; idx < n * Rp; idx += nt) {
-        const int p = idx / Rp, a = idx - p * Rp;
-        sp[idx] = a < R ? init_spin(I, p, a) : (int8_t)0;
-    }
-    for (int a = tid; a < Rp; a += nt) {
-        E[a] = 0; tried[a] = 0; accd[a] = 0; up[a] = 0; down[a] = 0; trips[a] = 0;
-        rung[a] = a;
-        rep[a] = a;
-        label[a] = (R > 1 && a == 0) ? 1 : ((R > 1 && a == R - 1) ? -1 : 0);
-        beta_f[a] = a < R ? (float)I.beta[a] : 0.0f;
-    }
-    if (tid == 0) { sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; }
-    __syncthreads();
-
     const int a = tid % Rp, qo = tid / Rp, qs = nt / Rp;
-    {
+    const int img16 = (int)(pt_lds_img_bytes(n, Rp) >> 4);   // the spins as 16-byte words
+    if constexpr (INIT) {
+        // --- initial spins from the stream and their energies, out to the image ---------------
+        for (int idx = tid; idx < n * Rp; idx += nt) {
+            const int p = idx / Rp, s = idx - p * Rp;
+            sp[idx] = s < R ? init_spin(I, p, s) : (int8_t)0;
+        }
+        for (int x = tid; x < Rp; x += nt) E[x] = 0;
+        __syncthreads();
         int64_t acc = 0;
         if (a < R)
             for (int p = qo; p < n; p += qs)
                 acc += site_energy<WM>(p, a, Rp, sp, I.row_ptr, I.edge, I.h);
         if (acc != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)acc);
+        __syncthreads();
+        for (int x = tid; x < img16; x += nt) ((uint4*)I.img)[x] = ((const uint4*)sp)[x];
+        for (int x = tid; x < Rp; x += nt) I.e0[x] = WM != PT_WM_F32 ? -(E[x] / 2) : E[x];
+        return;
     }
-    __syncthreads();
-    if (WM != PT_WM_F32)
-        for (int x = tid; x < R; x += nt) E[x] = -(E[x] / 2);
+    // --- the initial state from the image, bookkeeping ---------------------------------------
+    for (int x = tid; x < img16; x += nt) ((uint4*)sp)[x] = ((const uint4*)I.img)[x];
+    for (int x = tid; x < Rp; x += nt) {
+        E[x] = I.e0[x]; tried[x] = 0; accd[x] = 0; up[x] = 0; down[x] = 0; trips[x] = 0;
+        rung[x] = x;
+        rep[x] = x;
+        label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
+        beta_f[x] = x < R ? (float)I.beta[x] : 0.0f;
+    }
+    if (tid == 0) { sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; }
     __syncthreads();
 
     // R <= 64: the round end is wave 0's (WaveLadder), with one workgroup barrier per round.  The
@@ -222,7 +228,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
         }
 }
 
-template <int WM>
+template <int WM, bool INIT>
 __global__ __launch_bounds__(1024) void pt_lds_kernel(const PtInstance* __restrict__ insts,
                                                       const int32_t* __restrict__ order,
                                                       int n_inst, int32_t* queue) {
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(1024) void pt_lds_kernel(const PtInstance* __restri
         const int qi = sc.inst;
         if (qi >= n_inst) return;
         const PtInstance I = insts[order[qi]];
-        lds_instance<WM>(I, smem, &sc, scrE, scrR);
+        lds_instance<WM, INIT>(I, smem, &sc, scrE, scrR);
         __syncthreads();
     }
 }
@@ -299,7 +305,8 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
     const int rnd = *G.round_base + rl, cur = rnd & 1;
     const uint32_t t = (uint32_t)(rnd * I.S + j);
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
-    using LT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+    using LT = typename std::conditional<WM == PT_WM_F32, double, int32_t>::type;   // field
+    using WT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;    // weight
     constexpr int EW = WM == PT_WM_I8 ? 1 : 2;   // 32-bit words per edge record
     constexpr int NW = GW * EW;
     const int a = tid & 31;
@@ -325,7 +332,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
             t0 = I.row_ptr[p];
             t1 = I.row_ptr[p + 1];
         }
-        LT l = I.h ? ((const LT*)I.h)[p] : (LT)0;
+        LT l = I.h ? (LT)((const WT*)I.h)[p] : (LT)0;
         uint32_t v[GW];
 #pragma unroll
         for (int u = 0; u < GW; ++u) v[u] = G.spw[WM == PT_WM_I8 ? (ev[u] >> 8) : ev[EW * u]];
@@ -333,8 +340,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
         for (int u = 0; u < GW; ++u) {
             const bool neg = (v[u] >> a) & 1u;
             if constexpr (WM == PT_WM_F32) {
-                const float w = __uint_as_float(ev[EW * u + 1]);
-                l = neg ? l - w : l + w;
+                l += (double)__uint_as_float(ev[EW * u + 1] ^ ((uint32_t)neg << 31));
             } else {
                 const int32_t w = WM == PT_WM_I8 ? (int32_t)(int8_t)(ev[u] & 0xffu)
                                                  : (int32_t)ev[EW * u + 1];
@@ -343,7 +349,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
         }
         for (int k = t0 + GW; k < t1; k += GW) {
             int cc[GW];
-            LT w[GW];
+            WT w[GW];
             uint32_t vv[GW];
 #pragma unroll
             for (int u = 0; u < GW; ++u) {
@@ -358,7 +364,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
             for (int u = 0; u < GW; ++u) {
                 const bool neg = (vv[u] >> a) & 1u;
                 if constexpr (WM == PT_WM_F32) {
-                    const float nl = neg ? l - w[u] : l + w[u];
+                    const double nl = l + (double)__uint_as_float(__float_as_uint(w[u]) ^ ((uint32_t)neg << 31));
                     l = k + u < t1 ? nl : l;
                 } else {
                     l += k + u < t1 ? (neg ? -w[u] : w[u]) : 0;
@@ -370,10 +376,10 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
             const int g = G.rung[cur * Rp + a];
             const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
             if constexpr (WM == PT_WM_F32) {
-                const float q = si > 0 ? l : -l;
-                if (pt_accept_float((float)I.beta[g], q, pt_word(rv, p & 3))) {
+                const double q = si > 0 ? l : -l;
+                if (pt_accept_float((float)I.beta[g], (float)q, pt_word(rv, p & 3))) {
                     flip = true;
-                    d = pt_fix36(2.0f * q);
+                    d = pt_fix36d(2.0 * q);
                 }
             } else {
                 const uint32_t* thr_row = I.thr + (size_t)g * I.TQ;
@@ -680,22 +686,28 @@ size_t pt_lds_bytes(int n, int Rp) {
     return (((size_t)n * Rp + 15) & ~(size_t)15) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
 }
 
+template <int WM, bool INIT>
+static hipError_t lds_launch(const PtInstance* d_insts, const int32_t* d_order, int n_inst,
+                             int32_t* d_queue, int block, int grid, size_t lds_bytes,
+                             hipStream_t s) {
+    hipError_t e = hipFuncSetAttribute((const void*)pt_lds_kernel<WM, INIT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    pt_lds_kernel<WM, INIT><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
+    return hipGetLastError();
+}
+
 hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int n_inst,
                          int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
-                         hipStream_t s) {
-    const void* fn = wm == PT_WM_I8    ? (const void*)pt_lds_kernel<PT_WM_I8>
-                     : wm == PT_WM_I32 ? (const void*)pt_lds_kernel<PT_WM_I32>
-                                       : (const void*)pt_lds_kernel<PT_WM_F32>;
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds_bytes);
-    if (e != hipSuccess) return e;
+                         bool init, hipStream_t s) {
     if (wm == PT_WM_I8)
-        pt_lds_kernel<PT_WM_I8><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
-    else if (wm == PT_WM_I32)
-        pt_lds_kernel<PT_WM_I32><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
-    else
-        pt_lds_kernel<PT_WM_F32><<<grid, block, lds_bytes, s>>>(d_insts, d_order, n_inst, d_queue);
-    return hipGetLastError();
+        return init ? lds_launch<PT_WM_I8, true>(d_insts, d_order, n_inst, d_queue, block, grid, lds_bytes, s)
+                    : lds_launch<PT_WM_I8, false>(d_insts, d_order, n_inst, d_queue, block, grid, lds_bytes, s);
+    if (wm == PT_WM_I32)
+        return init ? lds_launch<PT_WM_I32, true>(d_insts, d_order, n_inst, d_queue, block, grid, lds_bytes, s)
+                    : lds_launch<PT_WM_I32, false>(d_insts, d_order, n_inst, d_queue, block, grid, lds_bytes, s);
+    return init ? lds_launch<PT_WM_F32, true>(d_insts, d_order, n_inst, d_queue, block, grid, lds_bytes, s)
+                : lds_launch<PT_WM_F32, false>(d_insts, d_order, n_inst, d_queue, block, grid, lds_bytes, s);
 }
 
 template <int WM>

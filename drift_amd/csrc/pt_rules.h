@@ -142,6 +142,15 @@ PT_HD bool pt_accept_exchange(double b_lo, double b_hi, double e_lo, double e_hi
 
 // Float energies are carried as int64 multiples of 2^-36 so that sums are associative (the
 // same in any reduction order) and therefore identical on every engine and on the CPU.
-PT_HD int64_t pt_fix36(float v) { return (int64_t)floor((double)v * 68719476736.0 + 0.5); }
+PT_HD int64_t pt_fix36d(double v) { return (int64_t)floor(v * 68719476736.0 + 0.5); }
+PT_HD int64_t pt_fix36(float v) { return pt_fix36d((double)v); }
+
+// Float couplings: the local field h_i + sum_j J_ij s_j of float32 values is summed in double, in
+// row order.  A double holds the sum of float32 values whose magnitudes lie within 2^29 of one
+// another exactly, so the field is (for such couplings) the exact real sum, and the energy change
+// of a flip, pt_fix36d(2 q) with q = s_i * field, is exact in 2^-36 units for any coupling that is
+// a multiple of 2^-36 (every float32 of magnitude at least 2^-13).  The tracked energies are
+// therefore the energies of the spins, not a sum of rounded float fields that drifts with the
+// number of accepted flips.  The Metropolis decision takes the field rounded once to float.
 
 #define PT_FIX36_UNIT 0x1p-36

@@ -119,8 +119,8 @@ static int exch_accept(double b_lo, double b_hi, double e_lo, double e_hi, uint3
     return (double)u < or_exp(d) * 4294967296.0;
 }
 
-static inline int64_t fix36(float v) {
-    return (int64_t)floor((double)v * 68719476736.0 + 0.5);
+static inline int64_t fix36d(double v) {
+    return (int64_t)floor(v * 68719476736.0 + 0.5);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -219,12 +219,12 @@ int32_t or_pt_run(const or_problem *P, const or_params *prm, or_result *res) {
         if (is_float) {
             int64_t acc = 0;
             for (int32_t i = 0; i < n; ++i) {
-                float lj = 0.0f;
+                double lj = 0.0;
                 for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
-                    lj = sa[ci[t]] > 0 ? lj + wf[t] : lj - wf[t];
-                float v = 0.5f * lj + (hf ? hf[i] : 0.0f);
-                float e = sa[i] > 0 ? -v : v;
-                acc += fix36(e);
+                    lj = sa[ci[t]] > 0 ? lj + (double)wf[t] : lj - (double)wf[t];
+                double v = 0.5 * lj + (hf ? (double)hf[i] : 0.0);
+                double e = sa[i] > 0 ? -v : v;
+                acc += fix36d(e);
             }
             E[a] = acc;
         } else {
@@ -296,13 +296,16 @@ int32_t or_pt_run(const or_problem *P, const or_params *prm, or_result *res) {
                         const int32_t i = P->color_spins[p];
                         const int8_t si = sa[i];
                         if (is_float) {
-                            float l = hf ? hf[i] : 0.0f;
+                            /* the field of float32 couplings summed in double (exact for
+                             * magnitudes within 2^29 of one another), the decision on its
+                             * float rounding, the energy change from the double */
+                            double l = hf ? (double)hf[i] : 0.0;
                             for (int32_t tt = rp[i]; tt < rp[i + 1]; ++tt)
-                                l = sa[ci[tt]] > 0 ? l + wf[tt] : l - wf[tt];
-                            float q = si > 0 ? l : -l;
-                            if (metro_float(beta_f, q, u)) {
+                                l = sa[ci[tt]] > 0 ? l + (double)wf[tt] : l - (double)wf[tt];
+                            double q = si > 0 ? l : -l;
+                            if (metro_float(beta_f, (float)q, u)) {
                                 sa[i] = (int8_t)-si;
-                                dE_acc += fix36(2.0f * q);
+                                dE_acc += fix36d(2.0 * q);
                             }
                         } else {
                             int64_t l = hi ? hi[i] : 0;

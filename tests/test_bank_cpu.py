@@ -77,3 +77,29 @@ def test_bank_v2_generators_reproduce_the_recorded_fingerprints():
     for s in specs:
         r = rec[s["id"]]
         assert (s["n"], s["nnz"], s["fingerprint"]) == (r["n"], r["nnz"], r["fingerprint"])
+
+
+def test_bank_v2_best_known_energies_are_energies_of_recorded_states():
+    """Each v2 best-known energy is the lowest of its independent runs and is the energy of the
+    configuration stored beside it (bank_v2_states.npz), recomputed here in fp64 from the model.
+    Integer rows agree exactly; float rows within 1e-10 relative (the engine tracks energies in
+    2^-36 units, exact for couplings and fields of magnitude 2^-13 or more; a site whose field has
+    smaller terms rounds by at most 2^-37 per flip, over 2e5 sweeps)."""
+    import numpy as np
+
+    from drift_amd.bank import V2_SEEDS, v2_model
+
+    rows = json.load(open(os.path.join(HERE, "golden/bank_v2.json")))["rows"]
+    states = np.load(os.path.join(HERE, "golden/bank_v2_states.npz"))
+    for r in rows:
+        runs = r["runs"]
+        assert sorted(x["seed"] for x in runs) == sorted(V2_SEEDS)
+        assert r["best_known_energy"] == min(x["best_E"] for x in runs)
+        assert r["seeds_reaching_best"] == sum(x["best_E"] == r["best_known_energy"] for x in runs)
+        model = v2_model(r["family"], r["size"], r["seed"])
+        s = 1.0 - 2.0 * np.unpackbits(states[r["id"]])[: model.n].astype(np.float64)
+        e = model.energy(s)
+        if r["family"] == "rrg3":
+            assert e == pytest.approx(r["best_known_energy"], rel=1e-10), r["id"]
+        else:
+            assert e == r["best_known_energy"], r["id"]

@@ -58,13 +58,31 @@ def test_planted_bipartite_max_cut_is_optimal(n):
 V2 = json.load(open(os.path.join(HERE, "golden/bank_v2.json")))["rows"]
 
 
-@pytest.mark.parametrize("row", [r for r in V2 if r["seed"] == 1], ids=lambda r: r["id"])
-def test_bank_v2_best_known_energy_replays_exactly(row):
-    """The recorded best-known run of a v2 row replays to the recorded energy and round: the runs
-    are deterministic under the stream contract, whichever engine the automatic choice takes."""
-    from drift_amd.bank import v2_best_known_run
+@pytest.mark.parametrize("row_id", ["v2.ea3d.L000008.s001", "v2.ea3d.L000032.s002",
+                                    "v2.er-sparse.n001000.s001", "v2.er-sparse.n010000.s002",
+                                    "v2.rrg3.n000512.s001", "v2.rrg3.n008192.s002"])
+def test_bank_v2_rows_match_the_oracle(row_id):
+    """Short runs of v2 rows on the row's own ladder, through the engine the automatic choice
+    takes at that size (LDS, persistent slabs, per-launch CSR), equal the CPU oracle bit for bit."""
+    import numpy as np
 
-    res = v2_best_known_run(row, row["run"]["n_rounds"], seed=row["run"]["seed"])
-    print(f"[bank-gpu] {row['id']}: {res.best_E} on {res.engine} in {res.seconds:.2f} s", flush=True)
-    assert res.best_E == row["best_known_energy"]
-    assert res.best_round == row["best_round"]
+    import oracle
+    from drift_amd.bank import V2_RUN, V2_SEEDS, v2_best_known_run, v2_model
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    row = next(r for r in V2 if r["id"] == row_id)
+    cfg = V2_RUN[row["family"]]
+    rounds = 12 if row["n"] > 20000 else 40
+    print(f"[bank-gpu] {row_id}: {rounds} rounds against the oracle", flush=True)
+    res = v2_best_known_run(row, rounds, seed=V2_SEEDS[0])
+    model = v2_model(row["family"], row["size"], row["seed"])
+    enc = gpu.encode_problem(model)
+    k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    o = oracle.pt_run_encoded(enc, (cptr, cs), seed=V2_SEEDS[0], R=cfg["R"], n_rounds=rounds,
+                              sweeps_per_round=1,
+                              beta=ladder_betas(geometric_ladder(cfg["R"], cfg["T_min"], cfg["T_max"])),
+                              n_threads=min(16, len(os.sched_getaffinity(0))))
+    assert res.best_E_units == o["best_units"]
+    assert (res.best_round, res.best_rung) == (o["best_round"], o["best_rung"])
+    assert np.array_equal(res.best_s.astype(np.int8), o["best_s"])
+    assert np.array_equal(res.rung_of, o["rung_of"])
