@@ -1,22 +1,41 @@
 """Throughput benchmark of the gfx950 parallel-tempering engine (one JSON line on stdout).
 
-Workload (default): 3-D Edwards-Anderson +/-J glass, L = 32 periodic (N = 32768), 64 replicas on a
-geometric beta ladder 0.2 .. 3.0, one sweep then one exchange attempt per step.  A step is one
-sweep of all 64 replicas (64 x 32768 spin-flip attempts) plus the device-side exchange.
+Headline (default): config C4 -- 4096 independent random 3-regular instances, N uniform over the
+even values of [64, 2048], float32 J ~ N(0,1) and h ~ N(0, 0.1^2), 16 replicas each on a geometric
+beta ladder 0.2 .. 5.0, all in one persistent batched launch.  A step is one sweep of every replica
+of every instance plus the device-side exchange (69.7 M spin-flip attempts).  The initial states are
+drawn by an untimed launch of their own; each timed run is the launch that runs --steps rounds, timed
+by device events with the inputs already in HBM.  The value is the median of --repeats timed runs.
 
-Three device runs: a warm-up (--warmup steps), the timed run (--steps steps; device events around
-the sweep + exchange loop, inputs already in HBM), and a short run that times every sweep-kernel
-launch with device events for the roofline figure.  The CPU baseline is the oracle
-(oracle/pt_oracle.c, the same chain) on a bounded sample of the same workload on this host's cores.
+Sub-results, each with its own roofline and CPU baseline, ride on the same line:
+  c2         3-D Edwards-Anderson +/-J glass, L = 32 (N = 32768), R = 64, beta 0.2 .. 3.0
+  c3         Erdos-Renyi N = 10^5, mean degree 6, weights in {+-1, +-2, +-3}, R = 32, beta 0.05 .. 5.0
+  published  the reference's own timed workload (experiments/phase14_gpu.py:74-81): max-cut of
+             random_graph(2048, p = min(0.1, 20/n), seed 0), R = 256, T 0.05 .. 5, rounds of four
+             sweeps; its only published rate is 1.29 attempts/ns on an RTX 5060 Ti
+             (docs/results/PHASE14-results.md:140-142), a different card: context, not a target
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+Rooflines count the bytes one attempt touches at the storage format the engine declares (no
+credit for on-chip reuse); the fixed per-attempt figure of the measurement plan (8.75 B for C2,
+50 B for C3, 45 B for C4) is shown beside it.  Kernels whose state lives in LDS are also put
+against this device's LDS read rate, measured by a probe kernel at the start of the run.  What
+limits each kernel is taken from its rocprofv3 counters (profiles/limiters.json).
+
+    python bench.py [--steps K] [--warmup W] [--repeats 5] [--workload all|c4|c2|c3|c1|published]
+                    [--no-cpu-baseline] [--plain-launch]
+
+--plain-launch starts the persistent engines (pslab, coop) with an ordinary dispatch, for profiler
+commands; timed runs use the cooperative launch.  The ladder workloads (--workload ladder,
+ladder-c5) measure time to target instead.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -26,105 +45,43 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+PROFILES = os.path.join(ROOT, "profiles")
+PUBLISHED_RTX5060TI = 1.29   # attempts/ns, docs/results/PHASE14-results.md:140-142
+
+# the fixed per-attempt byte figures of the measurement plan, per config
+SURVEY_BYTES = {"c2": 8.75, "c3": 50.0, "c4": 45.0}
 
 
 def workload(name: str):
+    """(model, R, betas, sweeps per round, description) of a single-instance workload."""
     from drift_amd.builders.lattice import cubic_pmj_glass, er_sparse_weighted, square_ferromagnet
+    from drift_amd.builders.qubo import maxcut_ising, random_graph
     from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
 
     if name == "c2":
-        model = cubic_pmj_glass(32, 0)
-        R, beta = 64, ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0))
-        desc = ("3-D Edwards-Anderson +/-J glass, L=32 periodic (N=32768), R=64, beta 0.2..3.0 "
-                "geometric, exchange every sweep")
-    elif name == "c1":
-        model = square_ferromagnet(16)
-        R, beta = 16, ladder_betas(geometric_ladder(16, 1.0, 10.0))
-        desc = "2-D ferromagnet, L=16 periodic (N=256), R=16, beta 0.1..1.0, exchange every sweep"
-    elif name == "c4":
-        return None, 16, None, ("4096 random 3-regular instances, N uniform over even values in "
-                                "[64, 2048], J ~ N(0,1) and h ~ N(0,0.1^2) in float32, R=16 each, "
-                                "beta 0.2..5.0 geometric, all in one persistent batched launch")
-    elif name == "c3":
-        model = er_sparse_weighted(100000, 6.0, 0)
-        R, beta = 32, ladder_betas(geometric_ladder(32, 0.2, 20.0))
-        desc = ("Erdos-Renyi N=100000 mean degree 6, weights in {+-1,+-2,+-3}, R=32, "
-                "beta 0.05..5.0, exchange every sweep")
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    return model, R, beta, desc
+        return (cubic_pmj_glass(32, 0), 64, ladder_betas(geometric_ladder(64, 1.0 / 3.0, 5.0)), 1,
+                "3-D Edwards-Anderson +/-J glass, L=32 periodic (N=32768), R=64, beta 0.2..3.0 "
+                "geometric, one sweep and one exchange per step")
+    if name == "c1":
+        return (square_ferromagnet(16), 16, ladder_betas(geometric_ladder(16, 1.0, 10.0)), 1,
+                "2-D ferromagnet, L=16 periodic (N=256), R=16, beta 0.1..1.0, one sweep and one "
+                "exchange per step")
+    if name == "c3":
+        return (er_sparse_weighted(100000, 6.0, 0), 32, ladder_betas(geometric_ladder(32, 0.2, 20.0)),
+                1, "Erdos-Renyi N=100000 mean degree 6, weights in {+-1,+-2,+-3}, R=32, "
+                   "beta 0.05..5.0, one sweep and one exchange per step")
+    if name == "published":
+        n = 2048
+        return (maxcut_ising(random_graph(n, p=min(0.1, 20.0 / n), seed=0)), 256,
+                ladder_betas(geometric_ladder(256, 0.05, 5.0)), 4,
+                "the reference's timed workload: max-cut of random_graph(2048, p=min(0.1, 20/n), "
+                "seed=0), R=256, T 0.05..5.0, four sweeps and one exchange per step")
+    raise SystemExit(f"unknown workload {name}")
 
 
-def lattice_side(n: int) -> int:
-    """Side L of an L^3 or L^2 lattice of n sites (0 if n is neither)."""
-    for d in (3, 2):
-        L = int(round(n ** (1.0 / d)))
-        if L ** d == n:
-            return L
-    return 0
-
-
-def algorithmic_bytes_per_attempt(model, enc, engine: str, replicas: int = 0) -> float:
-    """Bytes one attempt touches at the declared storage format, no credit for on-chip reuse.
-    Spins are int8 (b_s = 1), so own spin read + write and d neighbour reads are d + 2 bytes;
-    the CSR engine keeps up to 32 replicas as one bit each of a word per position (b_s = 1/8).
-    The CSR kernels add d edge records (int8 weight and 24-bit column packed in 4 bytes; int32 /
-    float weight and int32 column in 8) and two int32 row pointers (plus the field if any);
-    positions are relabelled by colour, so no colour list is read.  The lattice kernel computes
-    neighbours from coordinates and reads one byte holding the site's bond signs."""
-    d = model.nnz / model.n
-    b_s = 0.125 if engine == "global" and 0 < replicas <= 32 else 1.0
-    if engine in ("lattice", "slab", "pslab"):
-        return b_s * (d + 1) + b_s + 1.0
-    b_e = 4.0 if enc.weight.dtype.itemsize == 1 and model.n < (1 << 24) else 8.0
-    b_h = 0.0 if enc.h is None else enc.h.dtype.itemsize
-    return b_s * (d + 1) + b_s + d * b_e + 8.0 + b_h
-
-
-def engine_format(engine: str, model, enc, replicas: int = 0) -> str:
-    """Storage format the engine ran the workload in (what the algorithmic bytes count)."""
-    if engine == "lattice":
-        return ("int8 spins [position][replica], implicit periodic neighbours, one byte of bond "
-                "signs per site")
-    if engine == "slab":
-        return ("int8 spins [replica][site], slabs of planes staged in LDS each sweep, implicit "
-                "periodic neighbours, one byte of bond signs per site")
-    if engine == "pslab":
-        side = round(model.n ** (1 / 3))
-        side = side if side ** 3 == model.n else round(model.n ** 0.5)
-        bits = side >= 32 and side % 32 == 0
-        return ("one launch for the whole run (occupancy-sized grid, bounded grid barriers); "
-                "each workgroup keeps its slab resident in LDS "
-                + ("as 32-site spin words and bond bit-planes (bit-sliced field), halo planes "
-                   "exchanged as bits" if bits else
-                   "as int8 spins and bond-sign bytes, halo planes exchanged as bytes")
-                + "; int8 spins [replica][site] in HBM at the ends of the run")
-    words = engine == "global" and 0 < replicas <= 32
-    spins = ("spins as one 32-bit word per position (bit per replica)" if words
-             else "int8 spins [position][replica]" + (" in LDS" if engine == "lds" else ""))
-    rec = (f"({'4' if enc.weight.dtype.itemsize == 1 else '8'} bytes: column and "
-           f"{enc.weight.dtype.name} weight)")
-    if words or engine == "lds":
-        return (f"{spins}; the first edge records of each row (4 or 8 on the word engine, 2-4 on "
-                f"the LDS engine) in a zero-padded row-major copy read with 16-byte loads, longer "
-                f"rows finished from packed CSR edge records {rec}; the algorithmic bytes count "
-                f"the unpadded records")
-    return f"{spins}, packed CSR edge records {rec}"
-
-
-def copy_calibration(nbytes: int) -> int:
-    """One device-to-device copy of nbytes (torch's copy kernel) for scaling rocprof's
-    FETCH_SIZE / WRITE_SIZE counters against a transfer of known size."""
-    import torch
-
-    n = nbytes // 4
-    src = torch.arange(n, device="cuda", dtype=torch.float32)
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n * 4}), flush=True)
-    return 0
+C4_DESC = ("4096 random 3-regular instances, N uniform over even values in [64, 2048], J ~ N(0,1) "
+           "and h ~ N(0,0.1^2) in float32, R=16 each, beta 0.2..5.0 geometric, all in one "
+           "persistent batched launch; one sweep and one exchange of every instance per step")
 
 
 def c4_instances(count: int, seed: int = 0):
@@ -136,66 +93,382 @@ def c4_instances(count: int, seed: int = 0):
     return [random_regular_gaussian(int(n), seed=10_000 + i) for i, n in enumerate(sizes)]
 
 
-def bench_c4(args) -> int:
-    """Batched uneven instances: whole-job attempts/ns of one persistent launch."""
+def lattice_side(n: int) -> int:
+    """Side L of an L^3 or L^2 lattice of n sites (0 if n is neither)."""
+    for d in (3, 2):
+        L = int(round(n ** (1.0 / d)))
+        if L ** d == n:
+            return L
+    return 0
+
+
+def kernel_name(engine: str, model, R: int) -> str:
+    return {"global": "g_color_bits" if R <= 32 else "g_color_step",
+            "lattice": "g_lattice_step", "slab": "slab_sweep", "coop": "coop_run",
+            "pslab": "pslab_bits_run" if lattice_side(model.n) % 32 == 0 else "pslab_run",
+            }.get(engine, "pt_lds_kernel")
+
+
+def declared_bytes(model, enc, engine: str, R: int) -> tuple[float, str]:
+    """Bytes one attempt touches at the storage format the engine declares, no credit for on-chip
+    reuse: B = b_s (d + 1) + b_s + d (b_J + b_idx) + b_ptr + b_list + b_h, with b_s the bytes of a
+    spin, b_J of a coupling, b_idx of a column, b_ptr the row pointers (read only when some row is
+    longer than the padded first chunk of edge records), b_list 0 (positions are renumbered by
+    colour, so no colour list is read) and b_h the field.  Returns (B, format)."""
+    d = enc.col_idx.size / model.n
+    if engine == "pslab" and lattice_side(model.n) % 32 == 0:
+        # bit-sliced slabs: one bit per spin and per bond, neighbours implicit
+        return (d + 2) / 8.0 + d / 8.0, (
+            "one launch for the whole run; each workgroup keeps its slab of one replica in LDS as "
+            "32-site spin words and bond bit-planes (one bit per spin, one bit per bond), halo "
+            "planes exchanged through HBM as bits")
+    if engine in ("pslab", "slab", "lattice"):
+        fmt = {"pslab": "slabs resident in LDS as int8 spins, one byte of bond signs per site",
+               "slab": "int8 spins [replica][site], slabs staged in LDS each sweep, one byte of "
+                       "bond signs per site",
+               "lattice": "int8 spins [position][replica], implicit periodic neighbours, one byte "
+                          "of bond signs per site"}[engine]
+        return (d + 2) + 1.0, fmt
+    deg = np.diff(np.asarray(enc.row_ptr))
+    w_bytes = 4.0 if enc.weight.dtype.itemsize == 1 and model.n < (1 << 24) else 8.0
+    b_h = 0.0 if enc.h is None else float(enc.h.dtype.itemsize)
+    if engine == "lds":
+        # the first G edge records of every row (G = 2, 3 or 4: the smallest covering three rows
+        # in four), spins in LDS
+        chunk = next((g for g in (2, 3, 4) if np.mean(deg <= g) >= 0.75), 4)
+        b_s, fmt = 1.0, "int8 spins [position][replica] in LDS"
+    elif engine == "global":
+        chunk = 4 if np.mean(deg <= 4) >= 0.75 else 8
+        if R <= 32:
+            b_s, fmt = 0.125, "spins as one 32-bit word per position (one bit per replica)"
+        else:
+            b_s, fmt = 1.0, "int8 spins [position][replica] in HBM"
+    else:   # coop: rows and edge records of each workgroup's slice in its LDS
+        chunk = 0
+        b_s, fmt = 1.0, "int8 spins [position][replica] in HBM, each workgroup's rows in its LDS"
+    b_ptr = 8.0 if deg.max(initial=0) > chunk else 0.0
+    if chunk:
+        fmt += (f"; the first {chunk} edge records of each row ({int(w_bytes)} bytes: column and "
+                f"{enc.weight.dtype.name} weight) in a zero-padded row-major copy read with 16-byte "
+                f"loads" + ("; longer rows finished from packed CSR records" if b_ptr else ""))
+    if enc.weight.dtype.kind == "f":
+        fmt += "; float fields summed in double"
+    return b_s * (d + 1) + b_s + d * w_bytes + b_ptr + b_h, fmt
+
+
+def lds_bytes_per_attempt(kernel: str, model) -> float | None:
+    """LDS bytes one attempt reads and writes in the kernels whose state lives in LDS."""
+    d = model.to_csr()[1].size / model.n
+    if kernel == "pslab_bits_run":
+        # per 32-site row word (16 sites of the colour): the word, its two row neighbours, 2 (d-2)
+        # other neighbour words, d bond words, four 16-byte uniform blocks, one store
+        words = 3 + (d - 2) + d
+        return (4.0 * words + 64.0 + 4.0) / 16.0
+    if kernel == "pt_lds_kernel":
+        return 1.0 + d + 0.5   # own spin, d neighbour spins, a flip store at most every other attempt
+    return None
+
+
+def median(v):
+    return float(statistics.median(v))
+
+
+_probe_cache = {}
+
+
+def lds_probe() -> dict:
+    """This device's LDS read rate (16-byte and 1-byte reads), measured once per process."""
+    if _probe_cache:
+        return _probe_cache
+    lib = ctypes.CDLL(os.path.join(ROOT, "drift_amd", "libdriftprobe.so"))
+    lib.drift_probe_lds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    lib.drift_probe_lds.restype = ctypes.c_int
+    out = {}
+    for kind, name in ((0, "b128"), (1, "u8")):
+        sec, byt, ins = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        rc = lib.drift_probe_lds(kind, 4096, ctypes.byref(sec), ctypes.byref(byt), ctypes.byref(ins))
+        if rc != 0:
+            raise RuntimeError(f"LDS probe failed ({rc})")
+        out[name] = {"GBs": byt.value / sec.value / 1e9,
+                     "wave_reads_per_s": ins.value / sec.value, "seconds": sec.value}
+    _probe_cache.update(out)
+    return out
+
+
+def limiter(kernel: str):
+    """What limits a kernel, from its rocprofv3 counters (profiles/limiters.json), or None."""
+    path = os.path.join(PROFILES, "limiters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(kernel)
+
+
+def traffic_per_attempt(kernel: str):
+    """Counted HBM bytes per attempt of a kernel from its rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (profiles/limiters.json), with the profile it comes from."""
+    lim = limiter(kernel)
+    if not lim or "hbm_bytes_per_attempt" not in lim:
+        return None, None
+    return lim["hbm_bytes_per_attempt"], lim.get("source")
+
+
+CUS, CLOCK_HZ = 256, 2.4e9   # MI355X compute units and peak engine clock
+
+
+def roofline(kernel, B, B_survey, attempts_per_launch, launch_s, model=None):
+    """The contract's HBM roofline at the declared storage format, with the ceilings that can bind
+    a kernel whose state is on chip beside it: HBM traffic as counted by rocprofv3, VALU issue
+    (one wave instruction per CU per cycle) and LDS issue (the probe's rate of wave-level LDS
+    reads), each from the kernel's counter-measured instructions per attempt times the rate
+    measured here.  `binding` names the ceiling the kernel sits closest to."""
+    rate = attempts_per_launch / launch_s
+    achieved = B * rate / 1e9
+    line = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": kernel, "bytes_per_attempt": round(B, 4),
+            "attempts_per_launch": attempts_per_launch, "avg_launch_us": round(launch_s * 1e6, 3)}
+    if B_survey is not None:
+        line["bytes_per_attempt_plan"] = B_survey
+        line["frac_plan"] = round(B_survey * rate / 1e9 / HBM_PEAK_GBS, 4)
+    ceilings = {"hbm_algorithmic": line["frac"]}
+    tpa, src = traffic_per_attempt(kernel)
+    if tpa is not None:
+        line["traffic"] = round(tpa * attempts_per_launch)
+        line["traffic_source"] = src
+        ceilings["hbm_counted"] = round(tpa * rate / (HBM_PEAK_GBS * 1e9), 4)
+    lim = limiter(kernel)
+    if lim:
+        line["limiter"] = {k: v for k, v in lim.items() if k != "hbm_bytes_per_attempt"}
+        if "valu_insts_per_attempt" in lim:
+            ceilings["valu_issue"] = round(lim["valu_insts_per_attempt"] * rate / 64 / (CUS * CLOCK_HZ), 4)
+        if "lds_insts_per_attempt" in lim and lds_bytes_per_attempt(kernel, model) is not None:
+            ceilings["lds_issue"] = round(lim["lds_insts_per_attempt"] * rate / 64
+                                          / lds_probe()["u8"]["wave_reads_per_s"], 4)
+    lb = lds_bytes_per_attempt(kernel, model) if model is not None else None
+    if lb is not None:
+        peak = lds_probe()["b128"]["GBs"]
+        ach = lb * rate / 1e9
+        line["lds"] = {"bytes_per_attempt": round(lb, 3), "achieved_GBs": round(ach, 1),
+                       "peak_GBs_measured": round(peak, 1), "frac": round(ach / peak, 4)}
+    line["ceilings"] = ceilings
+    measured = {k: v for k, v in ceilings.items() if k != "hbm_algorithmic"}
+    top = max(measured, key=measured.get) if measured else None
+    if top is not None and measured[top] >= 0.6:
+        line["binding"] = top
+    elif lim:
+        line["binding"] = "latency (no throughput ceiling above 0.6; see limiter)"
+    if ceilings["hbm_algorithmic"] > 1.0:
+        line["note"] = ("frac above 1: the declared format counts bytes that stay on chip (spins in "
+                        "LDS, edge records re-read from L2 every sweep), so HBM cannot be the "
+                        "bound; the counted traffic and the issue ceilings show what is")
+    return line
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or '' where that is not readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline(model, enc, R, beta, S, budget_s: float):
+    """The oracle's chain (its -O3 build, OpenMP over replicas) on this host's cores over a bounded
+    number of rounds of the same workload, and on one thread over a fifth of the budget."""
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    kw = dict(seed=1, R=R, sweeps_per_round=S, beta=beta, fast=True)
+    probe = oracle.pt_run_encoded(enc, (cptr, cs), n_rounds=2, n_threads=cores, **kw)
+    per_round = max(probe["seconds"] / 2, 1e-6)
+    rounds = int(max(2, min(20000, budget_s / per_round)))
+    run = oracle.pt_run_encoded(enc, (cptr, cs), n_rounds=rounds, n_threads=cores, **kw)
+    attempts = R * model.n * S * rounds
+    rounds1 = int(max(1, min(rounds, budget_s / 5 / (per_round * min(cores, R)))))
+    one = oracle.pt_run_encoded(enc, (cptr, cs), n_rounds=rounds1, n_threads=1, **kw)
+    return {
+        "value": attempts / run["seconds"] / 1e9, "unit": "attempts/ns", "cores": min(cores, R),
+        "kind": "port",
+        "sample": f"same workload, {rounds} rounds of all {R} replicas ({attempts:.3g} attempts, "
+                  f"{run['seconds']:.1f} s, OpenMP over replicas, -O3 build of the oracle)",
+        "value_1_thread": R * model.n * S * rounds1 / one["seconds"] / 1e9,
+        "sample_1_thread": f"{rounds1} rounds on one thread ({one['seconds']:.1f} s)",
+        "cpu_model": cpu_model(), "host_cores": cores,
+    }
+
+
+def cpu_baseline_batch(models, R, beta, budget_s: float):
+    """The oracle's chain on a sample of the batch, instances one after another (OpenMP over the
+    replicas of each), sized to about budget_s seconds."""
+    import oracle
+    from drift_amd import gpu
+
+    cores = len(os.sched_getaffinity(0))
+    tot_s, tot_a, used = 0.0, 0, 0
+    rounds = 50
+    for m in models:
+        enc = gpu.encode_problem(m)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        o = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=rounds, sweeps_per_round=1,
+                                  beta=beta, n_threads=cores, fast=True)
+        tot_s += o["seconds"]
+        tot_a += R * m.n * rounds
+        used += 1
+        if tot_s >= budget_s:
+            break
+    return {"value": tot_a / tot_s / 1e9, "unit": "attempts/ns", "cores": min(cores, R),
+            "kind": "port",
+            "sample": f"first {used} instances of the batch, {rounds} sweeps each, one after another "
+                      f"(OpenMP over replicas, -O3 build of the oracle; {tot_a:.3g} attempts, "
+                      f"{tot_s:.1f} s)",
+            "cpu_model": cpu_model(), "host_cores": cores}
+
+
+def bench_c4(steps, warmup, repeats, cpu_budget, count=4096):
     from drift_amd import gpu
     from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
 
-    count = args.instances
-    print(f"[bench] building {count} instances", file=sys.stderr, flush=True)
+    log(f"c4: building {count} instances")
     models = c4_instances(count)
     R = 16
     kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, sweeps_per_round=1)
-    print(f"[bench] warm-up {args.warmup} steps", file=sys.stderr, flush=True)
-    gpu.parallel_tempering_gpu_batch(models, n_rounds=args.warmup, **kw)
-    print(f"[bench] timed {args.steps} steps", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    res = gpu.parallel_tempering_gpu_batch(models, n_rounds=args.steps, **kw)
-    wall = time.perf_counter() - t0
+    log(f"c4: warm-up {warmup} steps")
+    gpu.parallel_tempering_gpu_batch(models, n_rounds=max(warmup, 1), **kw)
     total_n = sum(m.n for m in models)
-    attempts = R * total_n * args.steps
-    secs = res[0].seconds
-    value = attempts / secs / 1e9
-    d = 3.0
-    B = (d + 1) + 1 + d * (4 + 4) + 8 + 4          # int8 spins, fp32 J + int32 column, row ptrs, fp32 h
-    roofline = {"bound": "hbm", "achieved": round(B * attempts / secs / 1e9, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(B * attempts / secs / 1e9 / HBM_PEAK_GBS, 4),
-                "traffic": None, "kernel": "pt_lds_kernel", "bytes_per_attempt": B,
-                "attempts_per_launch": attempts, "avg_launch_us": round(secs * 1e6, 1)}
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_c4_lds.json")
-    if os.path.exists(traffic_file):   # one launch runs the whole timed region
-        with open(traffic_file) as f:
-            roofline["traffic"] = round(json.load(f)["hbm_bytes_per_sweep"] * args.steps)
-    cpu = None
-    if not args.no_cpu_baseline:
-        import oracle
-        cores = len(os.sched_getaffinity(0))
-        sample = models[:64]
-        beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
-        tot_s, tot_a = 0.0, 0
-        for m in sample:
-            enc = gpu.encode_problem(m)
-            k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
-            o = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=50, sweeps_per_round=1,
-                                      beta=beta, n_threads=cores)
-            tot_s += o["seconds"]
-            tot_a += R * m.n * 50
-        cpu = {"value": tot_a / tot_s / 1e9, "unit": "attempts/ns", "cores": cores, "kind": "port",
-               "sample": f"first 64 instances of the batch, 50 sweeps each, one at a time "
-                         f"(OpenMP over replicas; {tot_a:.3g} attempts, {tot_s:.1f} s)"}
-    best = np.array([r.best_E for r in res])
-    line = {
-        "metric": "spin-flips/ns on 1 GPU", "value": round(value, 3), "unit": "attempts/ns",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(secs * 1e3 / args.steps, 5), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": workload("c4")[3], "instances": count, "total_spins": int(total_n),
-                   "format": engine_format(res[0].engine, models[0],
-                                           gpu.encode_problem(models[0]), R),
-                   "engine": res[0].engine, "wall_s": round(wall, 3),
-                   "mean_best_E_per_spin": round(float((best / [m.n for m in models]).mean()), 5)},
-        "roofline": roofline, "cpu_baseline": cpu,
+    attempts = R * total_n * steps
+    secs, walls = [], []
+    for r in range(repeats):
+        t0 = time.perf_counter()
+        res = gpu.parallel_tempering_gpu_batch(models, n_rounds=steps, **kw)
+        walls.append(time.perf_counter() - t0)
+        secs.append(res[0].seconds)
+        log(f"c4: timed run {r + 1}/{repeats}: {attempts / secs[-1] / 1e9:.1f} attempts/ns")
+    med = median(secs)
+    enc0 = gpu.encode_problem(models[0])
+    # every instance is 3-regular with a field: the per-attempt bytes of one apply to all
+    B, fmt = declared_bytes(models[0], enc0, "lds", R)
+    out = {
+        "value": round(attempts / med / 1e9, 3), "steps": steps, "warmup": warmup,
+        "ms_per_step": round(med * 1e3 / steps, 5), "dtype": "f32",
+        "config": {"workload": C4_DESC, "instances": count, "total_spins": int(total_n),
+                   "attempts_per_step": R * total_n, "format": fmt, "engine": res[0].engine,
+                   "wall_s_median": round(median(walls), 3)},
+        "timing": {"repeats": repeats, "runs_ms": [round(x * 1e3, 4) for x in secs],
+                   "median_ms": round(med * 1e3, 4), "window": "the launch that runs the rounds "
+                   "(initial states drawn by a launch of their own, untimed)"},
+        "roofline": roofline("pt_lds_kernel", B, SURVEY_BYTES["c4"], attempts, med, models[0]),
+        "cpu_baseline": None,
     }
+    if cpu_budget > 0:
+        log("c4: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline_batch(models, R, ladder_betas(geometric_ladder(R, 0.2, 5.0)),
+                                                 cpu_budget)
+    return out
+
+
+def bench_single(name, steps, warmup, repeats, cpu_budget, plain, engine="auto", block=0,
+                 autotune=False, kernel_timing_steps=200):
+    from drift_amd import gpu
+
+    model, R, beta, S, desc = workload(name)
+    enc = gpu.encode_problem(model)
+    kw = dict(n_replicas=R, betas=beta, sweeps_per_round=S, seed=1, engine=engine,
+              block_threads=block, record_history=False, plain_launch=plain,
+              autotune=autotune and engine == "auto" and block == 0)
+    log(f"{name}: warm-up {warmup} steps")
+    gpu.parallel_tempering_gpu(model, n_rounds=max(warmup, 1), **kw)
+    attempts = R * model.n * S * steps
+    secs, walls = [], []
+    for r in range(repeats):
+        t0 = time.perf_counter()
+        res = gpu.parallel_tempering_gpu(model, n_rounds=steps, **kw)
+        walls.append(time.perf_counter() - t0)
+        secs.append(res.seconds)
+        log(f"{name}: timed run {r + 1}/{repeats}: {attempts / secs[-1] / 1e9:.2f} attempts/ns "
+            f"({res.engine})")
+    med = median(secs)
+    kern = kernel_name(res.engine, model, R)
+    B, fmt = declared_bytes(model, enc, res.engine, R)
+    if res.engine in ("global", "lattice", "slab"):
+        # several launches per step: the dominant kernel's average launch in the timed region is
+        # the region's time per step times that kernel's share of device time (from a run that
+        # brackets every launch with events) over its launches per step
+        kt = gpu.parallel_tempering_gpu(model, n_rounds=kernel_timing_steps, time_kernels=True, **kw)
+        per_step = kt.kernel_launches / kernel_timing_steps
+        share = kt.kernel_seconds / (kt.kernel_seconds + (kt.other_kernel_seconds or 0.0))
+        launch_s = med / steps * share / per_step
+        launch_attempts = kt.kernel_attempts / kt.kernel_launches
+    else:
+        launch_s, launch_attempts = med, attempts   # one launch runs the whole timed region
+    out = {
+        "value": round(attempts / med / 1e9, 3), "steps": steps, "warmup": warmup,
+        "ms_per_step": round(med * 1e3 / steps, 5),
+        "dtype": "int32" if enc.is_integer else "f32",
+        "config": {"workload": desc, "n": model.n, "replicas": R, "sweeps_per_step": S,
+                   "attempts_per_step": R * model.n * S, "format": fmt, "engine": res.engine,
+                   "wall_s_median": round(median(walls), 3)},
+        "timing": {"repeats": repeats, "runs_ms": [round(x * 1e3, 4) for x in secs],
+                   "median_ms": round(med * 1e3, 4)},
+        "roofline": roofline(kern, B, SURVEY_BYTES.get(name), launch_attempts, launch_s, model),
+        "cpu_baseline": None,
+    }
+    if cpu_budget > 0:
+        log(f"{name}: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(model, enc, R, beta, S, cpu_budget)
+    return out
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def headline(sub: dict) -> dict:
+    return {
+        "metric": "spin-flips/ns on 1 GPU", "value": sub["value"], "unit": "attempts/ns",
+        "n_gpus": 1, "steps": sub["steps"], "warmup": sub["warmup"],
+        "ms_per_step": sub["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": sub["dtype"], "data": "synthetic", "config": sub["config"],
+        "roofline": sub["roofline"], "cpu_baseline": sub["cpu_baseline"], "timing": sub["timing"],
+    }
+
+
+SUB_DEFAULT_STEPS = {"c2": 10000, "c3": 1000, "c1": 20000, "published": 400}
+
+
+def bench_throughput(args) -> int:
+    cpu = 0.0 if args.no_cpu_baseline else args.cpu_budget
+    if args.workload == "c4":
+        print(json.dumps(headline(bench_c4(args.steps, args.warmup, args.repeats, cpu,
+                                           args.instances))), flush=True)
+        return 0
+    if args.workload in ("c2", "c3", "c1", "published"):
+        sub = bench_single(args.workload, args.steps, args.warmup, args.repeats, cpu,
+                           args.plain_launch, args.engine, args.block, args.autotune)
+        print(json.dumps(headline(sub)), flush=True)
+        return 0
+    # default: C4 headline with the other configs as sub-results
+    line = headline(bench_c4(args.steps, args.warmup, args.repeats, cpu, args.instances))
+    subs = {}
+    for name in ("c2", "c3", "published"):
+        steps = args.sub_steps or SUB_DEFAULT_STEPS[name]
+        sub = bench_single(name, steps, min(args.warmup, steps) or 1, args.repeats,
+                           cpu, args.plain_launch)
+        if name == "published":
+            sub["context"] = {"reference_rtx5060ti_attempts_per_ns": PUBLISHED_RTX5060TI,
+                              "source": "docs/results/PHASE14-results.md:140-142",
+                              "note": "a different card; context, not a target"}
+        subs[name] = sub
+    line["sub_results"] = subs
+    line["lds_probe"] = {k: {"GBs": round(v["GBs"], 1),
+                             "wave_reads_per_s": float(f"{v['wave_reads_per_s']:.4g}")}
+                         for k, v in lds_probe().items()}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -493,60 +766,23 @@ def bench_ladder_c5(args) -> int:
     return 0
 
 
-def cpu_baseline(model, enc, R, beta, budget_s: float):
-    import oracle
-
-    cores = len(os.sched_getaffinity(0))
-    k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
-    probe = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=2, sweeps_per_round=1,
-                                  beta=beta, n_threads=cores)
-    per_sweep = max(probe["seconds"] / 2, 1e-6)
-    sweeps = int(max(2, min(10000, budget_s / per_sweep)))
-    run = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=sweeps, sweeps_per_round=1,
-                                beta=beta, n_threads=cores)
-    attempts = R * model.n * sweeps
-    # the same chain on one thread, over a fifth of the budget
-    sweeps1 = int(max(1, min(sweeps, budget_s / 5 / (per_sweep * cores))))
-    one = oracle.pt_run_encoded(enc, (cptr, cs), seed=1, R=R, n_rounds=sweeps1, sweeps_per_round=1,
-                                beta=beta, n_threads=1)
-    return {
-        "value": attempts / run["seconds"] / 1e9,
-        "unit": "attempts/ns",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"same workload, {sweeps} sweeps+exchanges of all {R} replicas "
-                  f"({attempts:.3g} attempts, {run['seconds']:.1f} s, OpenMP over replicas)",
-        "value_1_thread": R * model.n * sweeps1 / one["seconds"] / 1e9,
-        "sample_1_thread": f"{sweeps1} sweeps+exchanges on one thread ({one['seconds']:.1f} s)",
-        "cpu_model": cpu_model(),
-    }
-
-
-def cpu_model() -> str:
-    """The host CPU's model name (/proc/cpuinfo), or '' where that is not readable."""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return ""
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100000)
-    ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
+    ap.add_argument("--workload", default="all",
+                    choices=("all", "c4", "c2", "c3", "c1", "published", "ladder", "ladder-c5"))
+    ap.add_argument("--sub-steps", type=int, default=0,
+                    help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--autotune", action="store_true",
-                    help="with --engine auto: time every candidate engine first (the default takes "
-                         "the library's engine choice)")
-    ap.add_argument("--kernel-timing-steps", type=int, default=200)
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+                    help="with --engine auto: time every candidate engine first")
+    ap.add_argument("--plain-launch", action="store_true",
+                    help="ordinary dispatch of the persistent engines (for profiler commands)")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
     ap.add_argument("--train-per-family", type=int, default=128)
@@ -566,8 +802,8 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this engine has no "
-              "multi-GPU path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this engine has no "
+              "multi-GPU path; the sharding design is described in DESIGN.md", file=sys.stderr)
         return 2
 
     from drift_amd import gpu
@@ -576,109 +812,25 @@ def main() -> int:
         print("bench.py: GPU engine not available (no gfx950 device or library not built)",
               file=sys.stderr)
         return 1
-    if args.workload == "c4":
-        return bench_c4(args)
     if args.workload == "ladder":
         return bench_ladder(args)
     if args.workload == "ladder-c5":
         return bench_ladder_c5(args)
-    model, R, beta, desc = workload(args.workload)
-    enc = gpu.encode_problem(model)
-    kw = dict(n_replicas=R, betas=beta, sweeps_per_round=1, seed=1, engine=args.engine,
-              block_threads=args.block, record_history=False,
-              autotune=args.autotune and args.engine == "auto" and args.block == 0,
-              # the bench owns the device, so the persistent engines start with an ordinary
-              # dispatch (rocprofv3 cannot trace cooperative launches)
-              plain_launch=True)
+    return bench_throughput(args)
 
-    print(f"[bench] warm-up {args.warmup} steps", file=sys.stderr, flush=True)
-    gpu.parallel_tempering_gpu(model, n_rounds=args.warmup, **kw)
-    print(f"[bench] timed {args.steps} steps", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    res = gpu.parallel_tempering_gpu(model, n_rounds=args.steps, **kw)
-    wall = time.perf_counter() - t0
-    attempts = R * model.n * args.steps
-    value = attempts / res.seconds / 1e9
-    print(f"[bench] {value:.2f} attempts/ns over {res.seconds:.3f} s (wall {wall:.3f} s), "
-          f"engine {res.engine}", file=sys.stderr, flush=True)
 
-    print("[bench] kernel timing run", file=sys.stderr, flush=True)
-    kt = gpu.parallel_tempering_gpu(model, n_rounds=args.kernel_timing_steps, time_kernels=True, **kw)
-    B = algorithmic_bytes_per_attempt(model, enc, kt.engine, R)
-    if kt.kernel_launches > 1:
-        # The dominant kernel's average launch in the timed region: the region's time per step
-        # times that kernel's share of device time, over its launches per step.  The share comes
-        # from the per-launch event timing run, where every launch (sweep kernels and round ends
-        # alike) is bracketed by an event pair that itself adds about 2 us.
-        launches_per_step = kt.kernel_launches / args.kernel_timing_steps
-        share = kt.kernel_seconds / (kt.kernel_seconds + (kt.other_kernel_seconds or 0.0))
-        per_launch_s = res.seconds / args.steps * share / launches_per_step
-        per_launch_attempts = kt.kernel_attempts / kt.kernel_launches
-        event_us = kt.kernel_seconds / kt.kernel_launches * 1e6
-    else:
-        # one launch runs the whole timed region
-        per_launch_s, per_launch_attempts, event_us = res.seconds, attempts, res.seconds * 1e6
-    achieved = B * per_launch_attempts / per_launch_s / 1e9
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "kernel": {"global": "g_color_bits" if R <= 32 else "g_color_step",
-                   "lattice": "g_lattice_step",
-                   "slab": "slab_sweep", "coop": "coop_run",
-                   # the persistent slab engine keeps slabs as bits when L is a multiple of 32
-                   "pslab": "pslab_bits_run" if lattice_side(model.n) % 32 == 0 else "pslab_run",
-                   }.get(kt.engine, "pt_lds_kernel"),
-        "bytes_per_attempt": round(B, 3),
-        "attempts_per_launch": per_launch_attempts,
-        "avg_launch_us": round(per_launch_s * 1e6, 3),
-        "avg_launch_us_event_pairs": round(event_us, 3),
-    }
-    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{kt.engine}.json")
-    if os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            tr = json.load(f)
-        if "hbm_bytes_per_launch" in tr:
-            roofline["traffic"] = tr["hbm_bytes_per_launch"]
-        elif "hbm_bytes_per_sweep" in tr and kt.kernel_launches <= 1:
-            # one launch runs the whole timed region
-            roofline["traffic"] = round(tr["hbm_bytes_per_sweep"] * args.steps)
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device copy of nbytes (torch's copy kernel) for scaling rocprof's
+    FETCH_SIZE / WRITE_SIZE counters against a transfer of known size."""
+    import torch
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(model, enc, R, beta, args.cpu_budget)
-
-    line = {
-        "metric": "spin-flips/ns on 1 GPU",
-        "value": round(value, 3),
-        "unit": "attempts/ns",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(res.seconds * 1e3 / args.steps, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32" if enc.is_integer else "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": desc,
-            "format": engine_format(res.engine, model, enc, R),
-            "engine": res.engine,
-            "n": model.n,
-            "replicas": R,
-            "sweeps_per_step": 1,
-            "attempts_per_step": R * model.n,
-            "wall_s": round(wall, 3),
-        },
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+    n = nbytes // 4
+    src = torch.arange(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4}), flush=True)
     return 0
 
 

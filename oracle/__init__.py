@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
+FAST_LIB_PATH = os.path.join(HERE, "liboracle_pt_fast.so")   # -O3 build for CPU-baseline timing
 
 W_I8, W_I32, W_F32 = 1, 2, 3
 
@@ -60,6 +61,7 @@ class OrResult(ctypes.Structure):
 
 
 _lib = None
+_fast = None
 
 
 def build() -> None:
@@ -67,30 +69,42 @@ def build() -> None:
     subprocess.run(["make", "-C", HERE, "-s"], check=True)
 
 
-def load():
-    global _lib
+def load(fast: bool = False):
+    """The checker build, or (fast) the -O3 build of the same source used to time the CPU
+    baseline; both run the same chain bit for bit."""
+    global _lib, _fast
+    if fast:
+        if _fast is None:
+            if not os.path.exists(FAST_LIB_PATH):
+                build()
+            _fast = _bind(ctypes.CDLL(FAST_LIB_PATH))
+        return _fast
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = ctypes.CDLL(LIB_PATH)
-        lib.or_philox.argtypes = [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]
-        lib.or_philox.restype = None
-        lib.or_exp.argtypes = [c_double]
-        lib.or_exp.restype = c_double
-        lib.or_expf.argtypes = [c_float]
-        lib.or_expf.restype = c_float
-        lib.or_thr_int.argtypes = [c_double, c_int32]
-        lib.or_thr_int.restype = c_uint32
-        lib.or_greedy_coloring.argtypes = [c_int32] + [POINTER(c_int32)] * 4
-        lib.or_greedy_coloring.restype = c_int32
-        lib.or_pt_run.argtypes = [POINTER(OrProblem), POINTER(OrParams), POINTER(OrResult)]
-        lib.or_pt_run.restype = c_int32
-        lib.or_exact.argtypes = [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_double),
-                                 POINTER(c_double), c_int32, POINTER(c_double), POINTER(c_double),
-                                 POINTER(c_int64), POINTER(c_double), POINTER(c_double)]
-        lib.or_exact.restype = c_int32
-        _lib = lib
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def _bind(lib):
+    """Argument and result types of the oracle's exports."""
+    lib.or_philox.argtypes = [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]
+    lib.or_philox.restype = None
+    lib.or_exp.argtypes = [c_double]
+    lib.or_exp.restype = c_double
+    lib.or_expf.argtypes = [c_float]
+    lib.or_expf.restype = c_float
+    lib.or_thr_int.argtypes = [c_double, c_int32]
+    lib.or_thr_int.restype = c_uint32
+    lib.or_greedy_coloring.argtypes = [c_int32] + [POINTER(c_int32)] * 4
+    lib.or_greedy_coloring.restype = c_int32
+    lib.or_pt_run.argtypes = [POINTER(OrProblem), POINTER(OrParams), POINTER(OrResult)]
+    lib.or_pt_run.restype = c_int32
+    lib.or_exact.argtypes = [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_double),
+                             POINTER(c_double), c_int32, POINTER(c_double), POINTER(c_double),
+                             POINTER(c_int64), POINTER(c_double), POINTER(c_double)]
+    lib.or_exact.restype = c_int32
+    return lib
 
 
 def _p(a, t):
@@ -128,8 +142,8 @@ def greedy_coloring(row_ptr, col_idx):
 
 
 def pt_run(n, row_ptr, col_idx, weight_type, weight, h, energy_scale, color_ptr, color_spins,
-           *, seed, R, n_rounds, sweeps_per_round, beta, n_threads=0, trace=False):
-    """Run the CPU chain; returns a dict of numpy arrays and scalars."""
+           *, seed, R, n_rounds, sweeps_per_round, beta, n_threads=0, trace=False, fast=False):
+    """Run the CPU chain; returns a dict of numpy arrays and scalars (fast: the -O3 build)."""
     rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
     ci = np.ascontiguousarray(col_idx, dtype=np.int32)
     cp = np.ascontiguousarray(color_ptr, dtype=np.int32)
@@ -164,7 +178,7 @@ def pt_run(n, row_ptr, col_idx, weight_type, weight, h, energy_scale, color_ptr,
         rung_of=_p(out["rung_of"], c_int32), final_units=_p(out["final_units"], c_int64),
         final_spins=_p(out["final_spins"], c_int8), best_history=_p(out["best_history"], c_double),
         rung_trace=_p(out["rung_trace"], c_double))
-    rc = load().or_pt_run(prob, prm, res)
+    rc = load(fast).or_pt_run(prob, prm, res)
     if rc != 0:
         raise RuntimeError(f"oracle rejected the problem ({rc})")
     out["swap_tried"] = out["swap_tried"][: max(R - 1, 0)]

@@ -14,7 +14,8 @@ sys.path.insert(0, ROOT)
 
 import bench  # noqa: E402
 from drift_amd import gpu  # noqa: E402
-from drift_amd.builders.lattice import cubic_pmj_glass, er_sparse_weighted  # noqa: E402
+from drift_amd.builders.lattice import (cubic_pmj_glass, er_sparse_weighted,  # noqa: E402
+                                        random_regular_gaussian)
 
 
 def test_several_gpus_are_refused_with_a_message():
@@ -26,23 +27,34 @@ def test_several_gpus_are_refused_with_a_message():
 
 
 def test_lattice_bytes_per_attempt():
-    # 3-D lattice, d = 6 int8 spins: 7 reads + 1 write + 1 sign byte = 9 B
-    model = cubic_pmj_glass(8, 0)
-    enc = gpu.encode_problem(model)
-    assert bench.algorithmic_bytes_per_attempt(model, enc, "pslab", 64) == pytest.approx(9.0)
-    assert bench.lattice_side(model.n) == 8
+    # 3-D lattice, d = 6.  Bit-sliced slabs (L a multiple of 32): one bit per spin and per bond,
+    # (7 + 1 + 6) / 8 = 1.75 B; byte slabs: 7 spin reads + 1 write + 1 sign byte = 9 B
+    big, small = cubic_pmj_glass(32, 0), cubic_pmj_glass(8, 0)
+    assert bench.declared_bytes(big, gpu.encode_problem(big), "pslab", 64)[0] == pytest.approx(1.75)
+    assert bench.declared_bytes(small, gpu.encode_problem(small), "pslab", 64)[0] == pytest.approx(9.0)
+    assert bench.lattice_side(small.n) == 8
 
 
 def test_csr_bytes_per_attempt_by_spin_format():
     model = er_sparse_weighted(4000, 6.0, 0)
     enc = gpu.encode_problem(model)
     d = model.nnz / model.n
-    # int8 weights pack with the column into 4-byte records; no field
+    # int8 weights pack with the column into 4-byte records; no field; Poisson rows longer than the
+    # first chunk, so the row pointers are read
     assert enc.weight.dtype.itemsize == 1 and enc.h is None
-    int8_spins = bench.algorithmic_bytes_per_attempt(model, enc, "global", 64)
-    words = bench.algorithmic_bytes_per_attempt(model, enc, "global", 32)
+    int8_spins, fmt8 = bench.declared_bytes(model, enc, "global", 64)
+    words, fmtw = bench.declared_bytes(model, enc, "global", 32)
     assert int8_spins == pytest.approx((d + 2) + 4 * d + 8)
     # one bit of a 32-bit word per spin when R <= 32
     assert words == pytest.approx((d + 2) / 8 + 4 * d + 8)
-    assert "one 32-bit word per position" in bench.engine_format("global", model, enc, 32)
-    assert "int8 spins [position][replica]" in bench.engine_format("global", model, enc, 64)
+    assert "one 32-bit word per position" in fmtw
+    assert "int8 spins [position][replica]" in fmt8
+
+
+def test_batched_float_bytes_per_attempt():
+    # 3-regular float instances on the LDS engine: 4 spin reads + 1 write, three 8-byte records
+    # (every row fits the first chunk, so no row pointers), a 4-byte field: 33 B
+    model = random_regular_gaussian(200, 3)
+    B, fmt = bench.declared_bytes(model, gpu.encode_problem(model), "lds", 16)
+    assert B == pytest.approx(33.0)
+    assert "in LDS" in fmt and "summed in double" in fmt

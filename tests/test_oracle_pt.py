@@ -146,3 +146,23 @@ def test_same_seed_same_chain_and_threads_do_not_matter():
     assert a["best_units"] == b["best_units"]
     c = run(model, R=8, T_min=0.2, T_max=3.0, n_rounds=40, S=2, seed=78)[1]
     assert not np.array_equal(a["final_spins"], c["final_spins"])
+
+
+def test_fast_build_runs_the_checkers_chain():
+    """The -O3 oracle build that times the CPU baseline gives the checker's chain bit for bit
+    (integer and float couplings)."""
+    from drift_amd import gpu
+    from drift_amd.builders.lattice import er_sparse_weighted, random_regular_gaussian
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    for model in (er_sparse_weighted(300, 5.0, 2), random_regular_gaussian(200, 4)):
+        enc = gpu.encode_problem(model)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        beta = ladder_betas(geometric_ladder(8, 0.2, 4.0))
+        runs = [oracle.pt_run_encoded(enc, (cptr, cs), seed=3, R=8, n_rounds=40,
+                                      sweeps_per_round=2, beta=beta, n_threads=2, fast=f)
+                for f in (False, True)]
+        assert runs[0]["best_units"] == runs[1]["best_units"]
+        assert np.array_equal(runs[0]["final_spins"], runs[1]["final_spins"])
+        assert np.array_equal(runs[0]["final_units"], runs[1]["final_units"])
+        assert np.array_equal(runs[0]["swap_acc"], runs[1]["swap_acc"])
