@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64,
 LIB_NAME = "libdriftpt.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # enum drift_pt_status
 OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
@@ -31,6 +31,7 @@ ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GRAPH = 1, 2, 4, 8, 16, 32
 AUTOTUNE = 64
 PLAIN_LAUNCH = 128
+STOP_AT_TARGET = 256
 TUNE_MAX = 16
 NOALLOC_BYTES = 64 << 20   # DRIFT_PT_NOALLOC_BYTES
 
@@ -63,6 +64,7 @@ class Params(ctypes.Structure):
         ("beta", POINTER(c_double)),
         ("flags", c_int32),
         ("block_threads", c_int32),
+        ("target_energy", c_double),
     ]
 
 
@@ -92,6 +94,7 @@ class Result(ctypes.Structure):
         ("kernel_launches", c_int64),
         ("other_kernel_seconds", c_double),
         ("other_kernel_launches", c_int64),
+        ("rounds_run", c_int32),
     ]
 
 

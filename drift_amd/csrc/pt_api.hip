@@ -10,11 +10,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -139,6 +141,8 @@ struct HostInst {
     int maxdeg = 0;                 // longest row
     int force_ebar = -1;            // bit-sliced persistent engine: SState::ebar, or -1 to choose
                                     // by a timed trial (energy_path_choice)
+    int has_target = 0;             // DRIFT_PT_STOP_AT_TARGET: stop at best <= target_units
+    int64_t target_units = 0;
 };
 
 // First-chunk width of the LDS engine's quad gather: the smallest of 2, 3, 4 that covers at least
@@ -328,6 +332,16 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
         if (!(P->energy_scale > 0.0) || !std::isfinite(P->energy_scale))
             return fail(DRIFT_PT_EINVAL, "energy_scale must be positive");
         H.unit = P->energy_scale;
+    }
+
+    if (prm->flags & DRIFT_PT_STOP_AT_TARGET) {
+        const double x = prm->target_energy / H.unit;
+        if (std::isnan(x)) return fail(DRIFT_PT_EINVAL, "target_energy is NaN");
+        H.has_target = 1;
+        // beyond the units' range a target is never reached (below) or reached at once (above)
+        H.target_units = x <= -9.0e18 ? INT64_MIN
+                       : x >= 9.0e18  ? INT64_MAX
+                                      : (int64_t)std::floor(x + 1e-9 * std::max(1.0, std::fabs(x)));
     }
 
     // ladder (drift/solvers/parallel_tempering.py:29-37, cuda/ising_pt.cu:174-178)
@@ -596,6 +610,8 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.ldim = H.ldim; I.lL = H.lL;
         I.gq = H.gq;
         I.ell_rest = H.maxdeg > H.gq ? 1 : 0;
+        I.has_target = H.has_target;
+        I.target_units = H.target_units;
     }
     return 0;
 }
@@ -614,7 +630,7 @@ void clear_result(drift_pt_result* r) {
     r->best_E = 0.0; r->best_E_units = 0; r->best_round = 0; r->best_rung = 0; r->best_slot = 0;
     r->engine_used = 0; r->seconds = 0.0; r->attempts = 0.0;
     r->kernel_seconds = 0.0; r->kernel_attempts = 0.0; r->kernel_launches = 0;
-    r->other_kernel_seconds = 0.0; r->other_kernel_launches = 0;
+    r->other_kernel_seconds = 0.0; r->other_kernel_launches = 0; r->rounds_run = 0;
 }
 
 // Copy results of one instance back and undo the colour relabelling.
@@ -637,6 +653,8 @@ int fetch_results(const HostInst& H, const PtInstance& I, const std::vector<int6
     res->best_round = (int32_t)out[1];
     res->best_rung = (int32_t)out[2];
     res->best_slot = (int32_t)out[3];
+    const int ran = (int)std::max<int64_t>(0, std::min<int64_t>(out[5], H.n_rounds));
+    res->rounds_run = ran;
     int64_t* o = const_cast<int64_t*>(out.data());
     for (int r = 0; r < R; ++r) {
         if (r + 1 < R) {
@@ -662,11 +680,17 @@ int fetch_results(const HostInst& H, const PtInstance& I, const std::vector<int6
                     res->final_spins[(size_t)a * n + H.orig[p]] = fs[(size_t)a * n + p];
         }
     }
-    if (res->best_history && (H.flags & DRIFT_PT_RECORD_HISTORY) && H.n_rounds > 0)
-        PT_HIP(get(res->best_history, I.hist, sizeof(double) * (size_t)H.n_rounds));
-    if (res->rung_energy_trace && (H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
-        PT_HIP(get(res->rung_energy_trace, I.trace, sizeof(double) * (size_t)H.n_rounds * R));
-    res->attempts = (double)R * (double)H.n_rounds * (double)H.S * (double)n;
+    // rounds after a reached target were not run: the history keeps the best, the trace is NaN
+    if (res->best_history && (H.flags & DRIFT_PT_RECORD_HISTORY) && H.n_rounds > 0) {
+        if (ran > 0) PT_HIP(get(res->best_history, I.hist, sizeof(double) * (size_t)ran));
+        for (int r = ran; r < H.n_rounds; ++r) res->best_history[r] = res->best_E;
+    }
+    if (res->rung_energy_trace && (H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0) {
+        if (ran > 0) PT_HIP(get(res->rung_energy_trace, I.trace, sizeof(double) * (size_t)ran * R));
+        for (size_t x = (size_t)ran * R; x < (size_t)H.n_rounds * R; ++x)
+            res->rung_energy_trace[x] = std::nan("");
+    }
+    res->attempts = (double)R * (double)ran * (double)H.S * (double)n;
     return 0;
 }
 
@@ -790,10 +814,11 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
 }
 
 // Results of the engines that keep their state in a GState: statistics block, best record, final
-// energies and rungs from the buffers of parity n_rounds & 1.
-int finish_global(const HostInst& H, const PtInstance& I, const GState& G, drift_pt_result* res) {
+// energies and rungs from the buffers of the parity of the rounds run.
+int finish_global(const HostInst& H, const PtInstance& I, const GState& G, int rounds_run,
+                  drift_pt_result* res) {
     const int Rp = H.Rp;
-    const int fin = H.n_rounds & 1;
+    const int fin = rounds_run & 1;
     std::vector<int64_t> out((size_t)pt_out_len(H.R));
     PT_HIP(hipMemcpy(out.data(), I.out, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     int64_t best[5];
@@ -803,6 +828,7 @@ int finish_global(const HostInst& H, const PtInstance& I, const GState& G, drift
     PT_HIP(hipMemcpy(E.data(), G.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
     PT_HIP(hipMemcpy(rung.data(), G.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
     for (int x = 0; x < 5; ++x) out[x] = best[x];
+    out[5] = rounds_run;
     for (int r = 0; r < H.R; ++r) {
         pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
         pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
@@ -815,84 +841,49 @@ int finish_global(const HostInst& H, const PtInstance& I, const GState& G, drift
     return fetch_results(H, I, out, fs, res);
 }
 
-int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_lattice) {
+// One run of the per-launch engines (global CSR or lattice), driven step by step so that several
+// can share the device: setup() uploads the instance, draws the initial state and captures a
+// chunk of rounds into a hipGraph; each step() enqueues one chunk (a graph replay, or direct
+// launches of up to a chunk of rounds) on the run's own stream; finish() waits and fetches the
+// results.  With a target, every step also copies the device's stop flag into pinned host memory,
+// and the flag of the step before is read once that step is done, so the device never waits on
+// the host; rounds enqueued after the target was reached return at entry.
+struct GlobalRun {
+    const HostInst& H;
+    drift_pt_result* res;
+    int block_req;
+    bool lattice;
     DevArena A;
-    PtInstance I;
-    int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
-    if (rc) return rc;
-    if (!use_lattice) I.jbits = nullptr;
-    const int Rp = H.Rp;
+    PtInstance I{};
     GState G{};
-    PT_HIP(A.alloc(&G.sp, (size_t)H.n * Rp));
-    PT_HIP(A.alloc(&G.Eb, 2 * (size_t)Rp));
-    PT_HIP(A.alloc(&G.dacc, 2 * (size_t)PT_DACC_SPLIT * Rp));
-    PT_HIP(A.alloc(&G.rung, 2 * (size_t)Rp));
-    PT_HIP(A.alloc(&G.rep, 2 * (size_t)Rp));
-    PT_HIP(A.alloc(&G.label, (size_t)Rp));
-    PT_HIP(A.alloc(&G.best, 10));
-    PT_HIP(A.alloc(&G.round_base, 1));
     StreamGuard sg;
-    PT_HIP(hipStreamCreate(&sg.s));
-    PT_HIP(hipEventCreate(&sg.e0));
-    PT_HIP(hipEventCreate(&sg.e1));
-    PT_HIP(hipMemsetAsync(G.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(G.dacc, 0, 2 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(G.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(G.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(G.label, 0, sizeof(int32_t) * Rp, sg.s));
-    PT_HIP(hipMemsetAsync(G.best, 0, 10 * sizeof(int64_t), sg.s));
-    PT_HIP(hipMemsetAsync(G.round_base, 0, sizeof(int32_t), sg.s));
-
-    int block = block_req > 0 ? block_req : 256;
-    block = std::max(block, Rp);
-    block = std::min(((block + Rp - 1) / Rp) * Rp, 1024);
-    int maxc = 0;
-    for (int c = 0; c < H.k; ++c) maxc = std::max(maxc, H.color_ptr[c + 1] - H.color_ptr[c]);
-    const int64_t threads = (int64_t)((maxc + 3) / 4 + 1) * Rp;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((threads + block - 1) / block, 2048));
-    // CSR colour steps run one attempt per lane: (block / Rp) positions per workgroup
-    // R <= 32 on the CSR engine: spins as one word per position (32 lanes per position)
-    const bool bits = !use_lattice && H.R <= 32 && block % 64 == 0;
-    const int ppb = bits ? block / 32 : block / Rp;
-    std::vector<int> cgrid((size_t)H.k);
-    for (int c = 0; c < H.k; ++c)
-        cgrid[c] = std::max(1, (H.color_ptr[c + 1] - H.color_ptr[c] + ppb - 1) / ppb);
-    const int reg = std::max(1, std::min(256, (H.n + 1023) / 1024));   // round-end grid
-    PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));   // byte spins, initial energies and state
-    uint32_t* spw = nullptr;
-    if (bits) {
-        PT_HIP(A.alloc(&spw, (size_t)H.n));
-        G.spw = spw;
-        PT_HIP(pt_launch_g_pack(I, G, false, sg.s));
-    }
-    if (!use_lattice) {
-        // first-chunk records by position: 4 per row when that covers three rows in four, else 8
-        int short_rows = 0;
-        for (int p = 0; p < H.n; ++p) short_rows += H.row_ptr[p + 1] - H.row_ptr[p] <= 4;
-        I.gq = 4 * (int64_t)short_rows >= 3 * (int64_t)H.n ? 4 : 8;
-        I.ell_rest = H.maxdeg > I.gq ? 1 : 0;
-        uint32_t* ell = nullptr;
-        PT_HIP(A.upload(&ell, build_ell(H, I.gq, 1u)));
-        I.ell = ell;
-    }
-    // optional per-launch timing of the colour-step kernel (the dominant kernel)
-    const bool time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;
-    EventPool pool, other;
-    const int64_t max_timed = 4096;
+    GraphGuard gg;
+    EventPool pool, other, polls;
+    int block = 0, grid = 0, reg = 0, chunk = 0, replays = 0, rest = 0, done_replays = 0,
+        done_rest = 0, steps = 0, rounds_run = -1;
+    bool bits = false, time_k = false;
+    std::vector<int> cgrid;
+    int32_t* host_stop = nullptr;   // pinned, one slot per step parity
+    std::vector<hipEvent_t> poll_ev;
     int64_t timed = 0, timed_other = 0;
     double timed_attempts = 0.0;
-    // one round: S sweeps of k colour steps, then the round-end kernel; rl is relative to the
-    // device-side round counter
-    auto launch_round = [&](int rl, bool timing) -> int {
+
+    GlobalRun(const HostInst& h, drift_pt_result* r, int b, bool lat)
+        : H(h), res(r), block_req(b), lattice(lat) {}
+    ~GlobalRun() {
+        if (host_stop) (void)hipHostFree(host_stop);
+    }
+
+    int launch_round(int rl, bool timing) {
         for (int j = 0; j < H.S; ++j)
             for (int c = 0; c < H.k; ++c) {
-                const bool tk = timing && timed < max_timed;
+                const bool tk = timing && timed < 4096;
                 hipEvent_t a = nullptr, b = nullptr;
                 if (tk) {
                     PT_HIP(pool.pair(&a, &b));
                     PT_HIP(hipEventRecord(a, sg.s));
                 }
-                if (use_lattice) PT_HIP(pt_launch_g_lattice(I, G, c, j, rl, block, grid, sg.s));
+                if (lattice) PT_HIP(pt_launch_g_lattice(I, G, c, j, rl, block, grid, sg.s));
                 else PT_HIP(pt_launch_g_color(I, G, H.color_ptr[c], H.color_ptr[c + 1], j, rl,
                                               block, cgrid[c], sg.s));
                 if (tk) {
@@ -901,7 +892,7 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
                     timed_attempts += (double)H.R * (H.color_ptr[c + 1] - H.color_ptr[c]);
                 }
             }
-        const bool tr = timing && timed_other < max_timed;
+        const bool tr = timing && timed_other < 4096;
         hipEvent_t a = nullptr, b = nullptr;
         if (tr) {
             PT_HIP(other.pair(&a, &b));
@@ -913,71 +904,238 @@ int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_
             ++timed_other;
         }
         return 0;
-    };
-    // Chunks of rounds are captured once into a hipGraph (rounds read their index from the
-    // device counter, which a tick kernel advances) and replayed; leftovers launch directly.
-    GraphGuard gg;
-    int chunk = 0;
-    const int per_round = H.S * H.k + 1;
-    if (!time_k && !(H.flags & DRIFT_PT_NO_GRAPH) && H.n_rounds >= 4) {
-        chunk = std::max(1, std::min(H.n_rounds, 1024 / per_round));
-        PT_HIP(hipStreamBeginCapture(sg.s, hipStreamCaptureModeThreadLocal));
-        for (int rl = 0; rl < chunk; ++rl) {
-            rc = launch_round(rl, false);
-            if (rc) {
-                hipGraph_t junk = nullptr;
-                (void)hipStreamEndCapture(sg.s, &junk);
-                if (junk) (void)hipGraphDestroy(junk);
-                return rc;
-            }
-        }
-        PT_HIP(pt_launch_g_tick(G, chunk, sg.s));
-        PT_HIP(hipStreamEndCapture(sg.s, &gg.graph));
-        PT_HIP(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0));
-    }
-    const int replays = chunk ? H.n_rounds / chunk : 0;
-    PT_HIP(hipEventRecord(sg.e0, sg.s));
-    for (int g = 0; g < replays; ++g) PT_HIP(hipGraphLaunch(gg.exec, sg.s));
-    for (int rl = 0; rl < H.n_rounds - replays * chunk; ++rl) {
-        rc = launch_round(rl, time_k);
-        if (rc) return rc;
-    }
-    PT_HIP(hipEventRecord(sg.e1, sg.s));
-    PT_HIP(hipStreamSynchronize(sg.s));
-    float ms = 0.0f;
-    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
-    double kernel_s = 0.0;
-    for (size_t x = 0; x < pool.ev.size(); x += 2) {
-        float km = 0.0f;
-        PT_HIP(hipEventElapsedTime(&km, pool.ev[x], pool.ev[x + 1]));
-        kernel_s += km * 1e-3;
-    }
-    double other_s = 0.0;
-    for (size_t x = 0; x < other.ev.size(); x += 2) {
-        float km = 0.0f;
-        PT_HIP(hipEventElapsedTime(&km, other.ev[x], other.ev[x + 1]));
-        other_s += km * 1e-3;
     }
 
-    if (bits) {   // final spins back to bytes where finish_global reads them
-        PT_HIP(pt_launch_g_pack(I, G, true, sg.s));
+    int setup() {
+        int rc = upload_instance(A, H, I, false, H.flags & DRIFT_PT_RECORD_HISTORY);
+        if (rc) return rc;
+        if (!lattice) I.jbits = nullptr;
+        const int Rp = H.Rp;
+        PT_HIP(A.alloc(&G.sp, (size_t)H.n * Rp));
+        PT_HIP(A.alloc(&G.Eb, 2 * (size_t)Rp));
+        PT_HIP(A.alloc(&G.dacc, 2 * (size_t)PT_DACC_SPLIT * Rp));
+        PT_HIP(A.alloc(&G.rung, 2 * (size_t)Rp));
+        PT_HIP(A.alloc(&G.rep, 2 * (size_t)Rp));
+        PT_HIP(A.alloc(&G.label, (size_t)Rp));
+        PT_HIP(A.alloc(&G.best, 10));
+        PT_HIP(A.alloc(&G.round_base, 1));
+        if (H.has_target) {
+            PT_HIP(A.alloc(&G.stop, 1));
+            PT_HIP(hipHostMalloc((void**)&host_stop, 2 * sizeof(int32_t), hipHostMallocDefault));
+            host_stop[0] = host_stop[1] = 0;
+        }
+        PT_HIP(hipStreamCreate(&sg.s));
+        PT_HIP(hipEventCreate(&sg.e0));
+        PT_HIP(hipEventCreate(&sg.e1));
+        PT_HIP(hipMemsetAsync(G.Eb, 0, 2 * sizeof(int64_t) * Rp, sg.s));
+        PT_HIP(hipMemsetAsync(G.dacc, 0, 2 * PT_DACC_SPLIT * sizeof(int64_t) * Rp, sg.s));
+        PT_HIP(hipMemsetAsync(G.rung, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+        PT_HIP(hipMemsetAsync(G.rep, 0, 2 * sizeof(int32_t) * Rp, sg.s));
+        PT_HIP(hipMemsetAsync(G.label, 0, sizeof(int32_t) * Rp, sg.s));
+        PT_HIP(hipMemsetAsync(G.best, 0, 10 * sizeof(int64_t), sg.s));
+        PT_HIP(hipMemsetAsync(G.round_base, 0, sizeof(int32_t), sg.s));
+        if (G.stop) PT_HIP(hipMemsetAsync(G.stop, 0, sizeof(int32_t), sg.s));
+
+        block = block_req > 0 ? block_req : 256;
+        block = std::max(block, Rp);
+        block = std::min(((block + Rp - 1) / Rp) * Rp, 1024);
+        int maxc = 0;
+        for (int c = 0; c < H.k; ++c) maxc = std::max(maxc, H.color_ptr[c + 1] - H.color_ptr[c]);
+        const int64_t threads = (int64_t)((maxc + 3) / 4 + 1) * Rp;
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>((threads + block - 1) / block, 2048));
+        // CSR colour steps run one attempt per lane: (block / Rp) positions per workgroup;
+        // R <= 32 on the CSR engine: spins as one word per position (32 lanes per position)
+        bits = !lattice && H.R <= 32 && block % 64 == 0;
+        const int ppb = bits ? block / 32 : block / Rp;
+        cgrid.assign((size_t)H.k, 1);
+        for (int c = 0; c < H.k; ++c)
+            cgrid[c] = std::max(1, (H.color_ptr[c + 1] - H.color_ptr[c] + ppb - 1) / ppb);
+        reg = std::max(1, std::min(256, (H.n + 1023) / 1024));   // round-end grid
+        PT_HIP(pt_launch_g_init(I, G, block, reg, sg.s));   // byte spins, initial energies and state
+        if (bits) {
+            uint32_t* spw = nullptr;
+            PT_HIP(A.alloc(&spw, (size_t)H.n));
+            G.spw = spw;
+            PT_HIP(pt_launch_g_pack(I, G, false, sg.s));
+        }
+        if (!lattice) {
+            // first-chunk records by position: 4 per row when that covers three rows in four, else 8
+            int short_rows = 0;
+            for (int p = 0; p < H.n; ++p) short_rows += H.row_ptr[p + 1] - H.row_ptr[p] <= 4;
+            I.gq = 4 * (int64_t)short_rows >= 3 * (int64_t)H.n ? 4 : 8;
+            I.ell_rest = H.maxdeg > I.gq ? 1 : 0;
+            uint32_t* ell = nullptr;
+            PT_HIP(A.upload(&ell, build_ell(H, I.gq, 1u)));
+            I.ell = ell;
+        }
+        time_k = (H.flags & DRIFT_PT_TIME_KERNELS) != 0;
+        // Chunks of rounds are captured once into a hipGraph (rounds read their index from the
+        // device counter, which a tick kernel advances) and replayed; leftovers launch directly.
+        // With a target the chunks are short, so little runs past the round that reached it.
+        const int per_round = H.S * H.k + 1;
+        const int launches = H.has_target ? 256 : 1024;
+        chunk = std::max(1, std::min(H.n_rounds, launches / per_round));
+        if (!time_k && !(H.flags & DRIFT_PT_NO_GRAPH) && H.n_rounds >= 4) {
+            PT_HIP(hipStreamBeginCapture(sg.s, hipStreamCaptureModeThreadLocal));
+            for (int rl = 0; rl < chunk; ++rl) {
+                const int rc2 = launch_round(rl, false);
+                if (rc2) {
+                    hipGraph_t junk = nullptr;
+                    (void)hipStreamEndCapture(sg.s, &junk);
+                    if (junk) (void)hipGraphDestroy(junk);
+                    return rc2;
+                }
+            }
+            PT_HIP(pt_launch_g_tick(G, chunk, sg.s));
+            PT_HIP(hipStreamEndCapture(sg.s, &gg.graph));
+            PT_HIP(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0));
+            replays = H.n_rounds / chunk;
+        }
+        rest = H.n_rounds - replays * chunk;
+        PT_HIP(hipEventRecord(sg.e0, sg.s));
+        return 0;
+    }
+
+    // The stop flag of step k - 1, once that step is done (with a target; else nothing).
+    int poll_previous() {
+        if (!H.has_target || steps < 2) return 0;
+        const int k = steps - 2;
+        PT_HIP(hipEventSynchronize(poll_ev[(size_t)k]));
+        const int32_t s = host_stop[k & 1];
+        if (s != 0) rounds_run = s - 1;
+        return 0;
+    }
+
+    // Enqueue the next chunk; *more is false once every round is enqueued or the target was seen.
+    int step(bool* more) {
+        *more = false;
+        if (rounds_run >= 0) return 0;
+        if (done_replays < replays) {
+            PT_HIP(hipGraphLaunch(gg.exec, sg.s));
+            ++done_replays;
+        } else if (done_rest < rest) {
+            const int n = std::min(chunk, rest - done_rest);
+            for (int x = 0; x < n; ++x) {
+                const int rc = launch_round(done_rest + x, time_k);
+                if (rc) return rc;
+            }
+            done_rest += n;
+        } else {
+            return 0;
+        }
+        if (H.has_target) {
+            hipEvent_t e = nullptr;
+            PT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            polls.ev.push_back(e);
+            poll_ev.push_back(e);
+            PT_HIP(hipMemcpyAsync(host_stop + (steps & 1), G.stop, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, sg.s));
+            PT_HIP(hipEventRecord(e, sg.s));
+        }
+        ++steps;
+        int rc = poll_previous();
+        if (rc) return rc;
+        *more = rounds_run < 0 && (done_replays < replays || done_rest < rest);
+        return 0;
+    }
+
+    int finish() {
+        PT_HIP(hipEventRecord(sg.e1, sg.s));
         PT_HIP(hipStreamSynchronize(sg.s));
+        if (rounds_run < 0) {
+            rounds_run = H.n_rounds;
+            if (H.has_target) {
+                int32_t s = 0;
+                PT_HIP(hipMemcpy(&s, G.stop, sizeof(s), hipMemcpyDeviceToHost));
+                if (s != 0) rounds_run = s - 1;
+            }
+        }
+        float ms = 0.0f;
+        PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+        double kernel_s = 0.0, other_s = 0.0;
+        for (size_t x = 0; x < pool.ev.size(); x += 2) {
+            float km = 0.0f;
+            PT_HIP(hipEventElapsedTime(&km, pool.ev[x], pool.ev[x + 1]));
+            kernel_s += km * 1e-3;
+        }
+        for (size_t x = 0; x < other.ev.size(); x += 2) {
+            float km = 0.0f;
+            PT_HIP(hipEventElapsedTime(&km, other.ev[x], other.ev[x + 1]));
+            other_s += km * 1e-3;
+        }
+        if (bits) {   // final spins back to bytes where finish_global reads them
+            PT_HIP(pt_launch_g_pack(I, G, true, sg.s));
+            PT_HIP(hipStreamSynchronize(sg.s));
+        }
+        const int rc = finish_global(H, I, G, rounds_run, res);
+        if (rc) return rc;
+        res->seconds = ms * 1e-3;
+        res->engine_used = lattice ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+        if (time_k) {
+            res->kernel_seconds = kernel_s;
+            res->kernel_attempts = timed_attempts;
+            res->kernel_launches = timed;
+            res->other_kernel_seconds = other_s;
+            res->other_kernel_launches = timed_other;
+        } else {
+            res->kernel_seconds = res->seconds;
+            res->kernel_attempts = res->attempts;
+            res->kernel_launches = 1;
+        }
+        return 0;
     }
-    rc = finish_global(H, I, G, res);
+};
+
+int run_global(const HostInst& H, drift_pt_result* res, int block_req, bool use_lattice) {
+    GlobalRun run(H, res, block_req, use_lattice);
+    int rc = run.setup();
     if (rc) return rc;
-    res->seconds = ms * 1e-3;
-    res->engine_used = use_lattice ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
-    if (time_k) {
-        res->kernel_seconds = kernel_s;
-        res->kernel_attempts = timed_attempts;
-        res->kernel_launches = timed;
-        res->other_kernel_seconds = other_s;
-        res->other_kernel_launches = timed_other;
-    } else {
-        res->kernel_seconds = res->seconds;
-        res->kernel_attempts = res->attempts;
-        res->kernel_launches = 1;
+    for (bool more = true; more;) {
+        rc = run.step(&more);
+        if (rc) return rc;
     }
+    return run.finish();
+}
+
+// Several per-launch runs sharing the device: up to `width` at a time, each on its own stream, one
+// chunk each in turn, so one run's launches fill the gaps between another's.  Every run makes the
+// decisions it makes alone.  seconds of each result is its own stream's time; wall_s (optional)
+// receives the host time of the whole set.
+int run_global_set(const std::vector<const HostInst*>& hs, const std::vector<drift_pt_result*>& rs,
+                   int width, double* wall_s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::unique_ptr<GlobalRun>> active;
+    size_t next = 0;
+    auto start_next = [&]() -> int {
+        const HostInst& H = *hs[next];
+        active.emplace_back(new GlobalRun(H, rs[next], 0, H.ldim != 0));
+        ++next;
+        return active.back()->setup();
+    };
+    while (next < hs.size() && (int)active.size() < width) {
+        const int rc = start_next();
+        if (rc) return rc;
+    }
+    while (!active.empty()) {
+        for (size_t i = 0; i < active.size();) {
+            bool more = false;
+            int rc = active[i]->step(&more);
+            if (rc) return rc;
+            if (more) {
+                ++i;
+                continue;
+            }
+            rc = active[i]->finish();
+            if (rc) return rc;
+            active.erase(active.begin() + (long)i);
+            if (next < hs.size()) {
+                rc = start_next();
+                if (rc) return rc;
+            }
+        }
+    }
+    if (wall_s)
+        *wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }
 
@@ -1119,7 +1277,7 @@ int run_coop(const HostInst& H, drift_pt_result* res, CoopPlan P) {
     int32_t aborted = 0;
     PT_HIP(hipMemcpy(&aborted, P.C.abort, sizeof(aborted), hipMemcpyDeviceToHost));
     if (aborted) return fail(DRIFT_PT_EHIP, "coop engine: a grid barrier wait exceeded its bound");
-    rc = finish_global(H, I, G, res);
+    rc = finish_global(H, I, G, H.n_rounds, res);
     if (rc) return rc;
     res->seconds = ms * 1e-3;
     res->engine_used = DRIFT_PT_ENGINE_COOP;
@@ -1166,6 +1324,7 @@ int finish_slab(const HostInst& H, const PtInstance& I, const SState& S, drift_p
     PT_HIP(hipMemcpy(E.data(), S.Eb + (size_t)fin * Rp, sizeof(int64_t) * Rp, hipMemcpyDeviceToHost));
     PT_HIP(hipMemcpy(rung.data(), S.rung + (size_t)fin * Rp, sizeof(int32_t) * Rp, hipMemcpyDeviceToHost));
     for (int x = 0; x < 5; ++x) out[x] = best[x];
+    out[5] = H.n_rounds;
     for (int r = 0; r < H.R; ++r) {
         pt_out_arr(out.data(), H.R, PT_OUT_E)[r] = E[r];
         pt_out_arr(out.data(), H.R, PT_OUT_RUNG)[r] = rung[r];
@@ -1451,6 +1610,10 @@ bool fits_lds(const HostInst& H) {
 
 // Run one prepared instance on a resolved engine (not AUTO).
 int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
+    if (H.has_target && engine != DRIFT_PT_ENGINE_LDS && engine != DRIFT_PT_ENGINE_GLOBAL &&
+        engine != DRIFT_PT_ENGINE_LATTICE)
+        return fail(DRIFT_PT_EINVAL, "DRIFT_PT_STOP_AT_TARGET needs the lds, global or lattice "
+                                     "engine (the persistent engines run every round in one launch)");
     switch (engine) {
     case DRIFT_PT_ENGINE_SLAB:
         if (!slab_fits(H))
@@ -1496,6 +1659,7 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
 int default_engine(const HostInst& H) {
     const int64_t per_sweep = (int64_t)H.R * H.n;
     if (fits_lds(H) && per_sweep <= 8192) return DRIFT_PT_ENGINE_LDS;
+    if (H.has_target) return H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
     if (H.ldim && pslab_fits(H)) {
         size_t lds = 0;
         if (pslab_block(H, 0, lds)) return DRIFT_PT_ENGINE_PSLAB;
@@ -1646,7 +1810,7 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     int engine = params->engine, block = params->block_threads;
     if (engine == DRIFT_PT_ENGINE_AUTO) {
         engine = default_engine(H);
-        if (params->flags & DRIFT_PT_AUTOTUNE) {
+        if ((params->flags & DRIFT_PT_AUTOTUNE) && !H.has_target) {
             TuneEntry choice;
             bool hit = false;
             {
@@ -1695,7 +1859,7 @@ namespace {
 // relabelled and given its thresholds; all must fit one workgroup's LDS.
 int prepare_batch(const drift_pt_problem* problems, const drift_pt_params* params,
                   int32_t n_instances, drift_pt_result* results, std::vector<HostInst>& hs,
-                  std::vector<drift_pt_result*>& rs) {
+                  std::vector<drift_pt_result*>& rs, bool require_lds) {
     if (n_instances < 1 || !problems || !params || !results)
         return fail(DRIFT_PT_EINVAL, "bad batch arguments");
     hs.assign((size_t)n_instances, HostInst());
@@ -1725,13 +1889,48 @@ int prepare_batch(const drift_pt_problem* problems, const drift_pt_params* param
     }
     for (int i = 0; i < n_instances; ++i) {
         if (rcs[i]) return fail(rcs[i], "instance " + std::to_string(i) + ": " + errs[i]);
-        if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "all instances of a batch need one weight_type");
         rs[i] = &results[i];
     }
     int rc = check_device();
     if (rc) return rc;
-    for (const HostInst& H : hs)
-        if (!fits_lds(H)) return fail(DRIFT_PT_EINVAL, "batched instances must fit one workgroup's LDS");
+    for (int i = 0; i < n_instances; ++i) {
+        HostInst& H = hs[i];
+        if (require_lds && !fits_lds(H))
+            return fail(DRIFT_PT_EINVAL, "batched instances must fit one workgroup's LDS");
+        if (!fits_lds(H)) detect_lattice(&problems[i], H);
+    }
+    return 0;
+}
+
+// The batch: instances that fit one workgroup's LDS in one persistent launch per weight type;
+// the others on the per-launch engines, four at a time on streams of their own.
+int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs) {
+    std::map<int, std::vector<size_t>> by_wm;
+    std::vector<const HostInst*> big;
+    std::vector<drift_pt_result*> big_rs;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        if (fits_lds(hs[i])) by_wm[hs[i].wm].push_back(i);
+        else {
+            big.push_back(&hs[i]);
+            big_rs.push_back(rs[i]);
+        }
+    }
+    for (auto& g : by_wm) {
+        std::vector<HostInst> part;
+        std::vector<drift_pt_result*> prs;
+        if (by_wm.size() == 1 && big.empty()) {
+            const int rc = run_lds(hs, rs, 0);
+            if (rc) return rc;
+            break;
+        }
+        for (size_t i : g.second) {
+            part.push_back(hs[i]);
+            prs.push_back(rs[i]);
+        }
+        const int rc = run_lds(part, prs, 0);
+        if (rc) return rc;
+    }
+    if (!big.empty()) return run_global_set(big, big_rs, 4, nullptr);
     return 0;
 }
 
@@ -1780,9 +1979,9 @@ int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* 
     g_err.clear();
     std::vector<HostInst> hs;
     std::vector<drift_pt_result*> rs;
-    int rc = prepare_batch(problems, params, n_instances, results, hs, rs);
+    int rc = prepare_batch(problems, params, n_instances, results, hs, rs, false);
     if (rc) return rc;
-    return run_lds(hs, rs, 0);
+    return run_batch(hs, rs);
 }
 
 int drift_pt_run_batch_noalloc(const drift_pt_problem* problems, const drift_pt_params* params,
@@ -1790,8 +1989,11 @@ int drift_pt_run_batch_noalloc(const drift_pt_problem* problems, const drift_pt_
     g_err.clear();
     std::vector<HostInst> hs;
     std::vector<drift_pt_result*> rs;
-    int rc = prepare_batch(problems, params, n_instances, results, hs, rs);
+    int rc = prepare_batch(problems, params, n_instances, results, hs, rs, true);
     if (rc) return rc;
+    for (const HostInst& H : hs)
+        if (H.wm != hs[0].wm)
+            return fail(DRIFT_PT_EINVAL, "allocation-free batch: all instances need one weight_type");
     NoallocSlot* S = noalloc_region();
     if (!S) return DRIFT_PT_EHIP;
     std::lock_guard<std::mutex> lk(S->mu);

@@ -9,9 +9,10 @@ enum : int { PT_WM_I8 = 1, PT_WM_I32 = 2, PT_WM_F32 = 3 };
 
 // Per-instance statistics block in device memory (int64 entries):
 //   [0] best units  [1] best round  [2] best rung  [3] best slot  [4] have-best flag
+//   [5] rounds run (written by the engines that stop at a target)
 //   then R-long arrays: E by slot, rung by slot, tried by rung, accepted by rung,
 //   round trips by slot, up-visits by rung, down-visits by rung.
-#define PT_OUT_HDR 5
+#define PT_OUT_HDR 6
 enum : int { PT_OUT_E = 0, PT_OUT_RUNG, PT_OUT_TRIED, PT_OUT_ACC, PT_OUT_TRIPS, PT_OUT_UP, PT_OUT_DOWN, PT_OUT_NARR };
 __host__ __device__ inline int64_t* pt_out_arr(int64_t* out, int R, int which) {
     return out + PT_OUT_HDR + (int64_t)which * R;
@@ -57,6 +58,8 @@ struct PtInstance {
                               // CSR engine 4 or 8); longer rows finish from the CSR in chunks of
                               // the same size
     int32_t ell_rest;         // some row is longer than gq
+    int32_t has_target;       // stop after the first round end whose best is <= target_units
+    int64_t target_units;
 };
 
 #define PT_DACC_SPLIT 32
@@ -76,6 +79,9 @@ struct GState {
                           // device by pt_launch_g_tick so a captured chunk of rounds can replay
     uint32_t* spw;        // R <= 32 on the CSR engine: one word per position, bit a = slot a's spin
                           // (1 = -1); the colour steps and round ends use it instead of sp
+    int32_t* stop;        // with PtInstance::has_target: 0 while running; r + 2 once the round end
+                          // of round r (r = -1: the initial states) reached the target, after
+                          // which every kernel of a later round returns at entry (starts at 0)
 };
 
 // Slab-engine state: a periodic lattice split into slabs of planes along its slowest axis; one

@@ -92,6 +92,10 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
     // R <= 64: the round end is wave 0's (WaveLadder), with one workgroup barrier per round.  The
     // ladder stays in the LDS arrays between round ends, so no registers are held over the sweeps.
     const bool fast = R <= PT_WAVE;
+    // the best record sits in LDS after every round end's barrier, so every thread takes the same
+    // decision to stop
+    int rounds_run = I.n_rounds;
+    auto reached = [&]() { return I.has_target && sc->have && sc->bestE <= I.target_units; };
 
     for (int rnd = -1; rnd < I.n_rounds; ++rnd) {
         if (rnd >= 0) {
@@ -176,6 +180,10 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                 for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
                 __syncthreads();   // read before the next round's sweeps change the spins
             }
+            if (reached()) {
+                rounds_run = rnd + 1;
+                break;
+            }
             continue;
         }
         // --- best tracking at round granularity -------------------------------------------
@@ -197,13 +205,23 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
             for (int r = tid; r < R; r += nt)
                 I.trace[(size_t)rnd * R + r] = (double)E[rep[r]] * I.unit;
         __syncthreads();
-        if (rnd < 0) continue;
+        if (rnd < 0) {
+            if (reached()) {
+                rounds_run = 0;
+                break;
+            }
+            continue;
+        }
 
         // --- replica exchange and round-trip labels ---------------------------------------
         exchange_pairs(I, rnd, E, rep, rung, tried, accd);
         __syncthreads();
         update_labels(R, rung, label, trips, nullptr, up, down);
         __syncthreads();
+        if (reached()) {
+            rounds_run = rnd + 1;
+            break;
+        }
     }
 
     // --- results ---------------------------------------------------------------------------
@@ -211,6 +229,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
     if (tid == 0) {
         out[0] = sc->bestE; out[1] = sc->bround; out[2] = sc->brung; out[3] = sc->bslot;
         out[4] = sc->have;
+        out[5] = rounds_run;
     }
     for (int x = tid; x < R; x += nt) {
         pt_out_arr(out, R, PT_OUT_E)[x] = E[x];
@@ -284,6 +303,14 @@ __global__ __launch_bounds__(1024) void g_init_energy(PtInstance I, GState G) {
     block_slot_reduce(acc, a, R, Rp, red, G.dacc, 1);   // parity 1 holds the init totals
 }
 
+// With a target: true once the round end of a round before rnd reached it; the launches of round
+// rnd then return at entry, every workgroup alike (the flag was written by an earlier launch).
+__device__ __forceinline__ bool g_stopped(const GState& G, int rnd) {
+    if (!G.stop) return false;
+    const int s = *G.stop;
+    return s != 0 && rnd >= s - 1;
+}
+
 // Spin of slot a at position p in either layout.
 __device__ __forceinline__ int8_t g_spin(const GState& G, int p, int Rp, int a) {
     return G.spw ? (((G.spw[p] >> a) & 1u) ? (int8_t)-1 : (int8_t)1) : G.sp[(size_t)p * Rp + a];
@@ -303,6 +330,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
                                                      int rl) {
     __shared__ long long red[1024];
     const int rnd = *G.round_base + rl, cur = rnd & 1;
+    if (g_stopped(G, rnd)) return;
     const uint32_t t = (uint32_t)(rnd * I.S + j);
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
     using LT = typename std::conditional<WM == PT_WM_F32, double, int32_t>::type;   // field
@@ -427,6 +455,7 @@ __global__ __launch_bounds__(1024) void g_color_step(PtInstance I, GState G, int
                                                      int rl) {
     __shared__ long long red[1024];
     const int rnd = *G.round_base + rl, cur = rnd & 1;
+    if (g_stopped(G, rnd)) return;
     const uint32_t t = (uint32_t)(rnd * I.S + j);
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
     for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
@@ -458,6 +487,7 @@ __global__ __launch_bounds__(1024) void g_lattice_step(PtInstance I, GState G, i
                                                        int rl) {
     __shared__ long long red[1024];
     const int rnd = *G.round_base + rl, cur = rnd & 1;
+    if (g_stopped(G, rnd)) return;
     const uint32_t t = (uint32_t)(rnd * I.S + j);
     const int Rp = I.Rp, R = I.R, tid = threadIdx.x;
     for (int x = tid; x < Rp; x += blockDim.x) red[x] = 0;
@@ -536,6 +566,7 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
     const int R = I.R, Rp = I.Rp, tid = threadIdx.x;
     const bool init = init_end != 0;
     const int rnd = init ? -1 : *G.round_base + rl;
+    if (!init && g_stopped(G, rnd)) return;
     const int cur = rnd & 1, nxt = cur ^ 1;
     if (R <= PT_WAVE) {
         // wave 0 evaluates the round end (WaveLadder) with the ladder loaded from this round's
@@ -595,6 +626,7 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
                     int64_t* bn = G.best + 5 * nxt;
                     bn[0] = W.bestE; bn[1] = W.best_round; bn[2] = W.best_rung; bn[3] = W.best_slot;
                     bn[4] = W.have;
+                    if (G.stop && I.has_target && W.have && W.bestE <= I.target_units) *G.stop = rnd + 2;
                 }
             }
         }
@@ -647,6 +679,7 @@ __global__ __launch_bounds__(1024) void g_round_end(PtInstance I, GState G, int 
         } else {
             for (int x = 0; x < 5; ++x) bn[x] = bc[x];
         }
+        if (G.stop && I.has_target && (improved ? mE : bc[0]) <= I.target_units) *G.stop = rnd + 2;
         if (!init && I.hist) I.hist[rnd] = (double)(improved ? mE : bc[0]) * I.unit;
     }
     if (!init && I.trace)

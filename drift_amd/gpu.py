@@ -166,11 +166,12 @@ class _Buffers:
 
 def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, engine,
             block_threads, final, history, trace=False, time_kernels=False, graph=True,
-            autotune=False, plain_launch=False):
+            autotune=False, plain_launch=False, target=None):
     flags = ((_lib.RECORD_FINAL if final else 0) | (_lib.RECORD_HISTORY if history else 0)
              | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0)
              | (0 if graph else _lib.NO_GRAPH) | (_lib.AUTOTUNE if autotune else 0)
-             | (_lib.PLAIN_LAUNCH if plain_launch else 0))
+             | (_lib.PLAIN_LAUNCH if plain_launch else 0)
+             | (_lib.STOP_AT_TARGET if target is not None else 0))
     if engine not in _lib.ENGINES:
         raise ValueError(f"engine must be one of {sorted(_lib.ENGINES)}")
     return _lib.Params(
@@ -184,6 +185,7 @@ def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, e
         beta=_lib.ptr(betas, _lib.c_double),
         flags=flags,
         block_threads=int(block_threads),
+        target_energy=0.0 if target is None else float(target),
     )
 
 
@@ -230,6 +232,7 @@ def _to_result(res: _lib.Result, buf: _Buffers, temps, n_rounds, R, history) -> 
         kernel_launches=int(res.kernel_launches),
         other_kernel_seconds=float(res.other_kernel_seconds),
         other_kernel_launches=int(res.other_kernel_launches),
+        rounds_run=int(res.rounds_run),
     )
 
 
@@ -254,6 +257,7 @@ def parallel_tempering_gpu(
     force_float: bool = False,
     autotune: bool = False,
     plain_launch: bool = False,
+    target_energy: float | None = None,
 ) -> PtResult:
     """Solve ``model`` by parallel tempering on the GPU; same signature and result as the CPU
     ``parallel_tempering`` (drift/gpu.py:43-114).  ``throughput`` is spin-flip attempts per second
@@ -261,13 +265,16 @@ def parallel_tempering_gpu(
     ``engine="auto"``) takes the engine and block size from the library's autotune cache, timing
     the candidates the first time this shape class is seen; results do not depend on it.
     ``plain_launch`` starts the persistent engines with an ordinary dispatch (for profilers that
-    cannot follow cooperative launches; needs the device to itself)."""
+    cannot follow cooperative launches; needs the device to itself).  ``target_energy`` stops the
+    run after the first round end whose best energy is at or below it (engines lds, global and
+    lattice); ``rounds_run`` then says how many rounds ran, and the result is that of a run of
+    that many rounds."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
     prm = _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, b, engine,
                   block_threads, return_final, record_history, record_trace, time_kernels,
-                  graph, autotune, plain_launch)
+                  graph, autotune, plain_launch, target_energy)
     buf = _Buffers(prob.n, n_replicas, n_rounds, return_final, record_history, record_trace)
     res = buf.struct
     rc = lib.drift_pt_run(prob.struct, prm, res)
@@ -292,6 +299,7 @@ def parallel_tempering_gpu_batch(
     force_float: bool = False,
     block_threads: int = 0,
     no_alloc: bool = False,
+    targets=None,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
@@ -299,7 +307,9 @@ def parallel_tempering_gpu_batch(
     workgroup size (0: the library's choice).  ``no_alloc`` runs the batch through
     drift_pt_run_batch_noalloc, which allocates no device memory (a fixed region of
     ``_lib.NOALLOC_BYTES`` per device; larger batches raise RuntimeError); the results are the
-    same."""
+    same.  Instances too large for one workgroup run on the per-launch engine, four at a time on
+    streams of their own.  ``targets`` (one energy or None per instance) stops each instance at
+    its target, as ``target_energy`` does for one."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -312,7 +322,8 @@ def parallel_tempering_gpu_batch(
         ladders.append((temps, b))
         probs.append(encode_problem(model, None, force_float))
         prms.append(_params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seeds[i], b,
-                            "lds", block_threads, return_final, record_history, record_trace))
+                            "lds", block_threads, return_final, record_history, record_trace,
+                            target=None if targets is None else targets[i]))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])

@@ -61,6 +61,7 @@ class PtResult:
     kernel_launches: int = None
     other_kernel_seconds: float = None
     other_kernel_launches: int = None
+    rounds_run: int = None               # rounds run (fewer than asked when a target was reached)
 
 
 def _neighbour_access(model):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_PT_ABI_VERSION 3
+#define DRIFT_PT_ABI_VERSION 4
 
 enum drift_pt_status {
     DRIFT_PT_OK = 0,
@@ -70,12 +70,18 @@ enum drift_pt_flags {
     DRIFT_PT_NO_GRAPH = 32,       /* launch every round directly instead of replaying a hipGraph */
     DRIFT_PT_AUTOTUNE = 64,       /* with DRIFT_PT_ENGINE_AUTO: take the engine and block size from
                                      the autotune cache, timing the candidates on a miss */
-    DRIFT_PT_PLAIN_LAUNCH = 128   /* persistent engines (coop, pslab): an ordinary dispatch of the
+    DRIFT_PT_PLAIN_LAUNCH = 128,  /* persistent engines (coop, pslab): an ordinary dispatch of the
                                      occupancy-sized grid instead of a cooperative launch.  All
                                      workgroups are resident when nothing else runs on the device;
                                      otherwise the bounded grid barrier fails the call after its
                                      time bound.  For profilers that cannot follow cooperative
-                                     launches. */
+                                     launches.  (Environment DRIFT_PT_BARRIER_TIMEOUT_MS sets the
+                                     bound of one barrier wait; default 20000.) */
+    DRIFT_PT_STOP_AT_TARGET = 256 /* stop after the first round end whose best energy is at or
+                                     below params.target_energy (within 1e-9 relative); the result
+                                     is then that of a run of result.rounds_run rounds.  Engines
+                                     LDS, GLOBAL and LATTICE (AUTO picks among them); the others
+                                     refuse it. */
 };
 
 /* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and
@@ -105,6 +111,7 @@ typedef struct drift_pt_params {
     const double* beta;         /* optional R inverse temperatures, rung 0 coldest */
     int32_t flags;              /* enum drift_pt_flags */
     int32_t block_threads;      /* 0 = chosen by the tuner */
+    double target_energy;       /* with DRIFT_PT_STOP_AT_TARGET: the energy to stop at */
 } drift_pt_params;
 
 /* Results, all arrays caller-allocated (NULL = not wanted, except best_s). */
@@ -135,6 +142,7 @@ typedef struct drift_pt_result {
     double other_kernel_seconds;    /* DRIFT_PT_TIME_KERNELS: summed device time of the timed
                                        non-sweep launches (round ends of the per-launch engines) */
     int64_t other_kernel_launches;  /* and their number (0 when the sweep kernel does everything) */
+    int32_t rounds_run;         /* rounds run: n_rounds, or fewer when the target was reached */
 } drift_pt_result;
 
 /* Timed candidates of one autotune call (drift_pt_autotune). */
@@ -172,9 +180,11 @@ int drift_pt_greedy_coloring(int32_t n, const int32_t* row_ptr, const int32_t* c
 int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
                  drift_pt_result* result);
 
-/* Many independent instances in one persistent launch; each result equals what drift_pt_run gives
- * for that instance and its params alone.  No reference counterpart: the reference solves one
- * instance per process (drift/gpu.py:97). */
+/* Many independent instances; each result equals what drift_pt_run gives for that instance and
+ * its params alone.  Instances whose replicas fit one workgroup's LDS run in one persistent launch
+ * (a work queue, largest first); the others run on the per-launch engine, several at a time on
+ * streams of their own so their launches overlap.  No reference counterpart: the reference solves
+ * one instance per process (drift/gpu.py:97). */
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
                        int32_t n_instances, drift_pt_result* results);
 
