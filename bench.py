@@ -25,8 +25,8 @@ limits each kernel is taken from its rocprofv3 counters (profiles/limiters.json)
                     [--no-cpu-baseline] [--plain-launch]
 
 --plain-launch starts the persistent engines (pslab, coop) with an ordinary dispatch, for profiler
-commands; timed runs use the cooperative launch.  The ladder workloads (--workload ladder,
-ladder-c5) measure time to target instead.
+commands; timed runs use the cooperative launch.  --workload ladder-c5 measures time to target
+with the learned ladder instead, in three stages (--stage train, bestknown, eval).
 """
 
 from __future__ import annotations
@@ -473,295 +473,227 @@ def bench_throughput(args) -> int:
     return 0
 
 
-LADDER_FAMILIES = {          # family: (T_min, T_max)
-    "ferro2d": (0.8, 8.0),
-    "ea3d": (0.3, 3.0),
-    "rrg": (0.1, 3.0),
-    "er": (0.2, 10.0),
-}
+def _write_json(path, obj):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(obj, f, indent=1)
 
 
-def ladder_instance(family: str, seed: int):
-    """One instance of the learned-ladder mix (N between 216 and 2048)."""
-    from drift_amd.builders.lattice import (cubic_pmj_glass, er_sparse_weighted,
-                                            random_regular_gaussian, square_ferromagnet)
+def c5_train(args) -> dict:
+    """Labels, features and the network.  Each training instance runs every ladder shape of
+    ladder.BUDGET_GAMMAS for --label-budget sweeps on the same streams; its label is the shape with
+    the lowest best energy (the earliest on ties).  The 16 features come from the instance and a
+    200-sweep run; the network is trained by the library's HIP kernels and saved to --net."""
+    from drift_amd import ladder
 
-    rng = np.random.default_rng(seed)
-    if family == "ferro2d":
-        return square_ferromagnet(int(2 * rng.integers(8, 23)))
-    if family == "ea3d":
-        return cubic_pmj_glass(int(rng.choice([6, 8, 10, 12])), seed)
-    if family == "rrg":
-        return random_regular_gaussian(int(2 * rng.integers(128, 1025)), seed)
-    return er_sparse_weighted(int(rng.integers(256, 2049)), float(rng.uniform(3.0, 12.0)), seed)
-
-
-def ladder_set(per_family: int, base_seed: int):
-    items = []
-    for fi, fam in enumerate(LADDER_FAMILIES):
-        for k in range(per_family):
-            items.append((fam, ladder_instance(fam, base_seed + 1000 * fi + k)))
-    return items
-
-
-def bench_ladder(args) -> int:
-    """Learned vs geometric ladder: feedback labels -> features -> network trained by the HIP
-    kernels -> held-out time-to-target, with batched vs one-instance-per-launch wall time."""
-    from drift_amd import gpu, ladder
-
-    R, budget = 32, args.steps
     t_all = time.perf_counter()
-    train = ladder_set(args.train_per_family, 1)
-    test = ladder_set(args.test_per_family, 777_777)
-    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)   # noqa: E731
-
-    def by_family(items):
-        groups = {}
-        for i, (fam, m) in enumerate(items):
-            groups.setdefault(fam, []).append(i)
-        return groups
-
-    # 1. feedback-optimised labels and features of the training set, one batch per family
-    X, Y = np.zeros((len(train), 16)), np.zeros((len(train), R))
-    t0 = time.perf_counter()
-    for fam, idx in by_family(train).items():
-        T_min, T_max = LADDER_FAMILIES[fam]
-        models = [train[i][1] for i in idx]
-        x, _ = ladder.feedback_optimise(models, T_min=T_min, T_max=T_max, R=R,
-                                        iterations=5, sweeps=args.label_sweeps, damping=0.5)
-        Y[idx] = x
-        X[idx] = ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R)
-        log(f"labels {fam}: {len(idx)} instances")
-    t_labels = time.perf_counter() - t0
-    # 2. train the network with its own kernels
+    items = ladder.training_instances(args.train_per_family)
+    feats, labels, picks, sizes = [], [], {}, []
+    for fam, (T_min, T_max) in ladder.MIX_FAMILIES.items():
+        models = [m for f, m in items if f == fam]
+        t0 = time.perf_counter()
+        for c0 in range(0, len(models), 256):
+            chunk = models[c0:c0 + 256]
+            x, pick = ladder.budget_labels(chunk, T_min=T_min, T_max=T_max, R=32,
+                                           budget=args.label_budget, seeds=range(c0, c0 + len(chunk)))
+            labels.append(x)
+            picks.setdefault(fam, []).extend(int(k) for k in pick)
+            feats.append(ladder.instance_features(chunk, T_min=T_min, T_max=T_max, R=32))
+            log(f"c5 train: {fam} labels {c0 + len(chunk)}/{len(models)} "
+                f"({time.perf_counter() - t0:.1f} s)")
+        sizes += [m.n for m in models]
+    X, Y = np.concatenate(feats), np.concatenate(labels)
     net = ladder.LadderNet(seed=11)
     t0 = time.perf_counter()
     losses = net.fit(X, Y, epochs=args.epochs, lr=3e-3)
-    t_train = time.perf_counter() - t0
-    log(f"trained: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
-    # 3. held-out: geometric vs learned ladders, same seeds, one batched launch per family
-    rows, wall_learn_batch, t_geo_batch, t_learn_batch = [], 0.0, 0.0, 0.0
-    for fam, idx in by_family(test).items():
-        T_min, T_max = LADDER_FAMILIES[fam]
-        models = [test[i][1] for i in idx]
-        xs = net.predict(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R))
-        seeds = [9000 + i for i in idx]
-        kw = dict(n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=budget, sweeps_per_round=1,
-                  seeds=seeds, record_history=True)
-        t0 = time.perf_counter()
-        geo = gpu.parallel_tempering_gpu_batch(models, **kw)
-        t1 = time.perf_counter()
-        lrn = gpu.parallel_tempering_gpu_batch(
-            models, betas=[ladder.betas_from_positions(x, T_min, T_max) for x in xs], **kw)
-        t2 = time.perf_counter()
-        t_geo_batch += t1 - t0
-        t_learn_batch += t2 - t1
-        fam_rows = []
-        for m, g, l in zip(models, geo, lrn):
-            target = min(g.best_E, l.best_E)
-            fam_rows.append(dict(family=fam, n=m.n, target=target, seed=seeds[len(fam_rows)],
-                                 tt_geo=ladder.sweeps_to_target(g.best_history, target),
-                                 tt_learn=ladder.sweeps_to_target(l.best_history, target),
-                                 rt_geo=ladder.round_trip_rate(g, budget),
-                                 rt_learn=ladder.round_trip_rate(l, budget), best_geo=g.best_E))
-        # the batched learned launch stopped once its slowest instance is at the target
-        slowest = max(r["tt_learn"] or budget for r in fam_rows)
-        wall_learn_batch += (t2 - t1) * min(slowest, budget) / budget
-        rows += fam_rows
-        log(f"held-out {fam}: {len(idx)} instances")
-    # 4. the reference's way: each instance alone in its own launch with the geometric ladder,
-    # stopped at its own time to target (instances that never reach it run the whole budget)
-    wall_geo_single, mismatch = 0.0, 0
-    for r, (fam, m) in zip(rows, test):
-        T_min, T_max = LADDER_FAMILIES[fam]
-        t0 = time.perf_counter()
-        one = gpu.parallel_tempering_gpu(m, n_replicas=R, T_min=T_min, T_max=T_max,
-                                         n_rounds=budget, sweeps_per_round=1, seed=r["seed"],
-                                         record_history=False)
-        wall = time.perf_counter() - t0
-        mismatch += int(one.best_E != r["best_geo"])       # same stream -> same answer
-        wall_geo_single += wall * min(r["tt_geo"] or budget, budget) / budget
-    log(f"single launches done, {mismatch} best-energy mismatches against the batched run")
+    net.save(args.net)
+    os.makedirs(args.out, exist_ok=True)
+    np.savez(os.path.join(args.out, "c5_train_data.npz"), features=X, labels=Y)
+    return {"stage": "train", "instances": int(X.shape[0]), "n_min": int(min(sizes)),
+            "n_max": int(max(sizes)), "label_budget_sweeps": args.label_budget,
+            "label_shapes": [float(g) for g in ladder.BUDGET_GAMMAS],
+            "chosen_shape": {f: {str(g): p.count(k) for k, g in enumerate(ladder.BUDGET_GAMMAS)}
+                             for f, p in picks.items()},
+            "train_loss": [float(losses[0]), float(losses[-1])], "epochs": args.epochs,
+            "train_s": round(time.perf_counter() - t0, 1), "net": os.path.basename(args.net),
+            "total_s": round(time.perf_counter() - t_all, 1)}
 
-    def med(v):
-        return float(np.median(v)) if len(v) else None
 
-    tt_g = np.array([r["tt_geo"] or 2 * budget for r in rows], dtype=float)
-    tt_l = np.array([r["tt_learn"] or 2 * budget for r in rows], dtype=float)
-    hit_g = int(sum(r["tt_geo"] is not None for r in rows))
-    hit_l = int(sum(r["tt_learn"] is not None for r in rows))
-    per_family = {}
-    for fam in LADDER_FAMILIES:
-        sel = [i for i, r in enumerate(rows) if r["family"] == fam]
-        per_family[fam] = {
-            "hit_geo": int(sum(rows[i]["tt_geo"] is not None for i in sel)),
-            "hit_learned": int(sum(rows[i]["tt_learn"] is not None for i in sel)),
-            "median_sweeps_geo": med(tt_g[sel]), "median_sweeps_learned": med(tt_l[sel]),
-            "round_trip_ratio": med([rows[i]["rt_learn"] / max(rows[i]["rt_geo"], 1e-12) for i in sel]),
-        }
-    line = {
+# best-known runs: four independent chains per held-out instance, two on the geometric ladder and
+# two on the power ladder of exponent 0.7 (a shape neither favours), each --cap x 10 sweeps
+BESTKNOWN_RUNS = ((101, 1.0), (102, 1.0), (103, 0.7), (104, 0.7))
+
+
+def c5_bestknown(args) -> dict:
+    """Best-known energies of the held-out instances, from independent runs ten times the cap."""
+    from drift_amd import ladder
+
+    t_all = time.perf_counter()
+    held = ladder.heldout_instances(args.test_per_family, args.test_n)
+    rows = {}
+    states = {}
+    rounds = 10 * args.cap
+    fix_path = os.path.join(args.out, "c5_heldout_bestknown.json")
+    st_path = os.path.join(args.out, "c5_heldout_states.npz")
+    families = args.families.split(",") if args.families else list(ladder.MIX_FAMILIES)
+    if args.families and os.path.exists(fix_path):
+        # a run over some families adds to the rows of the same held-out set recorded before
+        with open(fix_path) as f:
+            old = json.load(f)
+        if (old["per_family"], old["test_n"], old["cap_sweeps"]) == (args.test_per_family, args.test_n, args.cap):
+            rows = {r["index"]: r for r in old["rows"] if r["family"] not in families}
+            with np.load(st_path) as z:
+                states = {k: z[k] for k in z.files if int(k) in rows}
+    for fam, (T_min, T_max) in ladder.MIX_FAMILIES.items():
+        if fam not in families:
+            continue
+        idx = [i for i, (f, _) in enumerate(held) if f == fam]
+        models = [held[i][1] for i in idx]
+        for seed, gamma in BESTKNOWN_RUNS:
+            b = ladder.betas_from_positions(ladder.power_positions(gamma), T_min, T_max)
+            t0 = time.perf_counter()
+            res = ladder.run_instances(models, n_replicas=32, T_min=T_min, T_max=T_max,
+                                       n_rounds=rounds, seeds=[seed] * len(models),
+                                       betas=[b] * len(models))
+            for i, m, r in zip(idx, models, res):
+                row = rows.setdefault(i, {"index": i, "family": fam, "n": m.n, "runs": []})
+                row["runs"].append({"seed": seed, "ladder_power": gamma, "best_E": r.best_E,
+                                    "best_round": r.best_round, "engine": r.engine})
+                if "best_known_energy" not in row or r.best_E < row["best_known_energy"]:
+                    row["best_known_energy"] = r.best_E
+                    states[str(i)] = np.packbits(r.best_s < 0)
+            log(f"c5 best-known: {fam} seed {seed} ({time.perf_counter() - t0:.1f} s)")
+    out = [rows[i] for i in sorted(rows)]
+    for r in out:
+        r["runs_reaching_best"] = sum(x["best_E"] == r["best_known_energy"] for x in r["runs"])
+    fix = {"description": "best-known energies of the held-out instances of the learned-ladder "
+                          "evaluation: the lowest of four independent runs",
+           "source": f"gpu-pt, {rounds} rounds of one sweep each, R=32, seeds and ladder shapes "
+                     f"{list(BESTKNOWN_RUNS)}", "per_family": args.test_per_family,
+           "test_n": args.test_n, "cap_sweeps": args.cap, "rows": out}
+    os.makedirs(args.out, exist_ok=True)
+    _write_json(fix_path, fix)
+    np.savez_compressed(st_path, **states)
+    return {"stage": "bestknown", "instances": len(out), "rounds": rounds,
+            "total_s": round(time.perf_counter() - t_all, 1)}
+
+
+def _reach(hist, target, rel):
+    """First round (1-based) whose best energy is within rel of the target, or None."""
+    tol = rel * max(1.0, abs(target)) + 1e-9 * max(1.0, abs(target))
+    hit = np.nonzero(np.asarray(hist) <= target + tol)[0]
+    return int(hit[0]) + 1 if hit.size else None
+
+
+def c5_eval(args) -> dict:
+    """Time to the best-known energy on the held-out instances, every run stopping at its target
+    within --cap sweeps: (a) one call per instance on the geometric ladder, as the reference runs
+    it; (b) one batched call on the learned ladders; (c) one batched call on the geometric ladders
+    (the batching alone).  Wall time is the library calls' host time; the instances' encoding is
+    the same for all three and is not counted."""
+    from drift_amd import gpu, ladder
+
+    t_all = time.perf_counter()
+    with open(args.bestknown) as f:
+        bk = json.load(f)
+    held = ladder.heldout_instances(bk["per_family"], bk["test_n"])
+    target = {r["index"]: r["best_known_energy"] for r in bk["rows"]}
+    net = ladder.LadderNet(seed=11)
+    net.load(args.net)
+    cap = args.cap
+    rows = []
+    walls = {"geo_single": 0.0, "learned_batched": 0.0, "geo_batched": 0.0}
+    for fam, (T_min, T_max) in ladder.MIX_FAMILIES.items():
+        idx = [i for i, (f, _) in enumerate(held) if f == fam]
+        models = [held[i][1] for i in idx]
+        tg = [target[i] for i in idx]
+        seeds = [4242 + i for i in idx]
+        xs = net.predict_ladder(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=32))
+        learned = [ladder.betas_from_positions(x, T_min, T_max) for x in xs]
+        kw = dict(n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=cap, sweeps_per_round=1,
+                  record_history=True)
+        # (a) the reference's way: one instance per call, geometric ladder
+        single = []
+        for m, s, t in zip(models, seeds, tg):
+            t0 = time.perf_counter()
+            r = gpu.parallel_tempering_gpu(m, seed=s, target_energy=t, **kw)
+            walls["geo_single"] += time.perf_counter() - t0
+            single.append(r)
+        log(f"c5 eval: {fam} single geometric done")
+        t0 = time.perf_counter()
+        lrn = gpu.parallel_tempering_gpu_batch(models, seeds=seeds, betas=learned, targets=tg, **kw)
+        walls["learned_batched"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        geo = gpu.parallel_tempering_gpu_batch(models, seeds=seeds, targets=tg, **kw)
+        walls["geo_batched"] += time.perf_counter() - t0
+        log(f"c5 eval: {fam} batched runs done")
+        for i, m, t, a, b, c, x in zip(idx, models, tg, single, lrn, geo, xs):
+            rows.append({"index": i, "family": fam, "n": m.n, "target": t,
+                         "geo_single": {"rounds": a.rounds_run, "reached": a.best_E <= t + 1e-9 * abs(t),
+                                        "best_E": a.best_E, "device_s": a.seconds},
+                         "learned_batched": {"rounds": b.rounds_run, "reached": b.best_E <= t + 1e-9 * abs(t),
+                                             "best_E": b.best_E},
+                         "geo_batched": {"rounds": c.rounds_run, "reached": c.best_E <= t + 1e-9 * abs(t),
+                                         "best_E": c.best_E},
+                         "within_1e-4": {k: _reach(r.best_history[:max(r.rounds_run, 1)], t, 1e-4)
+                                         for k, r in (("geo", a), ("learned", b))},
+                         "within_1e-3": {k: _reach(r.best_history[:max(r.rounds_run, 1)], t, 1e-3)
+                                         for k, r in (("geo", a), ("learned", b))},
+                         # a target both can reach: the energy the geometric run ends at
+                         "to_geo_final": {k: _reach(r.best_history[:max(r.rounds_run, 1)], a.best_E, 0.0)
+                                          for k, r in (("geo", a), ("learned", b))},
+                         "learned_rung8_position": float(x[8]),
+                         "learned_power": float(np.log(max(x[8], 1e-12)) / np.log(8 / 31))})
+
+    def summary(sel):
+        n = len(sel)
+        out = {"instances": n}
+        for k in ("geo_single", "learned_batched", "geo_batched"):
+            out[f"reached_{k}"] = int(sum(r[k]["reached"] for r in sel))
+            out[f"median_rounds_{k}"] = float(np.median([r[k]["rounds"] for r in sel]))
+        for tol in ("within_1e-4", "within_1e-3"):
+            out[f"{tol}_geo"] = int(sum(r[tol]["geo"] is not None for r in sel))
+            out[f"{tol}_learned"] = int(sum(r[tol]["learned"] is not None for r in sel))
+        # sweeps to the geometric run's final energy (unreached: twice the cap)
+        tg = np.array([r["to_geo_final"]["geo"] or 2 * cap for r in sel], dtype=float)
+        tl = np.array([r["to_geo_final"]["learned"] or 2 * cap for r in sel], dtype=float)
+        out["to_geo_final_reached_learned"] = int(sum(r["to_geo_final"]["learned"] is not None for r in sel))
+        out["to_geo_final_median_sweeps_geo"] = float(np.median(tg))
+        out["to_geo_final_median_sweeps_learned"] = float(np.median(tl))
+        out["to_geo_final_sweeps_ratio_median"] = float(np.median(tg / tl))
+        out["learned_lower_final_energy"] = int(sum(r["learned_batched"]["best_E"] < r["geo_single"]["best_E"] for r in sel))
+        out["median_gap_to_best_known_geo"] = float(np.median([(r["geo_single"]["best_E"] - r["target"]) / abs(r["target"]) for r in sel]))
+        out["median_gap_to_best_known_learned"] = float(np.median([(r["learned_batched"]["best_E"] - r["target"]) / abs(r["target"]) for r in sel]))
+        return out
+
+    total = summary(rows)
+    speedup = walls["geo_single"] / max(walls["learned_batched"], 1e-9)
+    return {
         "metric": "time-to-target energy, learned vs geometric ladder",
-        "value": round(wall_geo_single / max(wall_learn_batch, 1e-9), 3),
-        "unit": "x less wall time (batched+learned vs one-launch-per-instance+geometric)",
-        "n_gpus": 1, "steps": budget, "warmup": 0, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32/f32", "data": "synthetic",
-        "config": {
-            "workload": "learned 32-rung ladders on held-out instances of four families "
-                        "(2-D ferromagnet, 3-D +/-J glass, 3-regular N(0,1), sparse ER integer), "
-                        "N 216..2048",
-            "train_instances": len(train), "test_instances": len(test),
-            "budget_sweeps": budget, "label_sweeps": args.label_sweeps, "epochs": args.epochs,
-            "target": "best energy found by either ladder within the budget",
-        },
-        "held_out": {
-            "reached_target_geo": hit_g, "reached_target_learned": hit_l,
-            "median_sweeps_to_target_geo": med(tt_g), "median_sweeps_to_target_learned": med(tt_l),
-            "sweeps_ratio_median": med(tt_g / tt_l),
-            "wall_s_geo_one_per_launch": round(wall_geo_single, 3),
-            "wall_s_learned_batched": round(wall_learn_batch, 3),
-            "batched_s_geo_full_budget": round(t_geo_batch, 3),
-            "batched_s_learned_full_budget": round(t_learn_batch, 3),
-            "single_vs_batched_best_mismatches": mismatch,
-            "per_family": per_family,
-        },
-        "timing_s": {"labels": round(t_labels, 1), "train": round(t_train, 1),
-                     "total": round(time.perf_counter() - t_all, 1)},
-        "train_loss": [float(losses[0]), float(losses[-1])],
+        "value": round(speedup, 3),
+        "unit": "x less wall time to the best-known energies (batched + learned vs one call per "
+                "instance + geometric, every run stopping at its target)",
+        "n_gpus": 1, "steps": cap, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32/f32", "data": "synthetic",
+        "config": {"workload": f"held-out instances of the learned-ladder mix at N ~ {bk['test_n']} "
+                               "(3-D +/-J glass at the nearest cube, 3-regular N(0,1), sparse ER "
+                               "mean degree 3..12), R=32",
+                   "per_family": bk["per_family"], "cap_sweeps": cap,
+                   "target": "best-known energy: lowest of four independent runs of ten times the "
+                             "cap (" + bk["source"] + ")", "net": os.path.basename(args.net)},
+        "wall_s": {k: round(v, 3) for k, v in walls.items()},
+        "speedup_ladder_only": round(walls["geo_batched"] / max(walls["learned_batched"], 1e-9), 3),
+        "speedup_batching_only": round(walls["geo_single"] / max(walls["geo_batched"], 1e-9), 3),
+        "held_out": total,
+        "per_family": {f: summary([r for r in rows if r["family"] == f]) for f in ladder.MIX_FAMILIES},
+        "total_s": round(time.perf_counter() - t_all, 1),
+        "rows": rows,
     }
-    print(json.dumps(line), flush=True)
-    return 0
-
-
-C5_FAMILIES = {            # family: (T_min, T_max)
-    "ea3d": (0.3, 3.0),
-    "rrg": (0.1, 3.0),
-    "er": (0.2, 10.0),
-}
-
-
-def c5_instance(family: str, seed: int, n_target: int = 0):
-    """One instance of the C5 mix: 3-D +/-J glasses, 3-regular N(0,1) graphs and sparse ER graphs
-    (mean degree 3..12); N uniform on a log scale over 256..16384, or about n_target when given
-    (the glass then takes the nearest cube)."""
-    from drift_amd.builders.lattice import (cubic_pmj_glass, er_sparse_weighted,
-                                            random_regular_gaussian)
-
-    rng = np.random.default_rng(seed)
-    n = n_target or int(round(np.exp(rng.uniform(np.log(256), np.log(16384)))))
-    if family == "ea3d":
-        return cubic_pmj_glass(max(6, int(round(n ** (1 / 3)))), seed)
-    if family == "rrg":
-        return random_regular_gaussian(2 * (n // 2), seed)
-    return er_sparse_weighted(n, float(rng.uniform(3.0, 12.0)), seed)
 
 
 def bench_ladder_c5(args) -> int:
-    """C5: learned 32-rung ladders at scale, in two stages (each fits one call of a few minutes).
-    --stage train: feedback-optimised labels for --train-per-family instances of each family (N 256..
-    16384), features, network trained by the HIP kernels, saved to --net.
-    --stage eval: the saved network's ladders against geometric ones on --test-per-family held-out
-    instances per family at N ~ --test-n, time to target within --steps sweeps."""
-    from drift_amd import ladder
-
-    R = 32
-    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)   # noqa: E731
-    t_all = time.perf_counter()
-    if args.stage == "train":
-        feats, labels, sizes, picks = [], [], [], {}
-        for fi, fam in enumerate(C5_FAMILIES):
-            T_min, T_max = C5_FAMILIES[fam]
-            models = [c5_instance(fam, 10_000 * fi + k) for k in range(args.train_per_family)]
-            t0 = time.perf_counter()
-            step = 64 if args.labels == "budget" else 128   # chunks, so progress shows as it goes
-            for c0 in range(0, len(models), step):
-                chunk = models[c0:c0 + step]
-                if args.labels == "budget":
-                    x, pick = ladder.budget_labels(chunk, T_min=T_min, T_max=T_max, R=R,
-                                                   budget=args.steps,
-                                                   seeds=range(c0, c0 + len(chunk)))
-                    picks.setdefault(fam, []).extend(int(k) for k in pick)
-                else:
-                    x, _ = ladder.feedback_optimise(chunk, T_min=T_min, T_max=T_max, R=R,
-                                                    iterations=5, sweeps=args.label_sweeps,
-                                                    damping=0.5, seeds=range(c0, c0 + len(chunk)))
-                labels.append(x)
-                feats.append(ladder.instance_features(chunk, T_min=T_min, T_max=T_max, R=R))
-                log(f"labels {fam}: {c0 + len(chunk)}/{len(models)}, "
-                    f"{time.perf_counter() - t0:.1f} s")
-            sizes += [m.n for m in models]
-            log(f"labels {fam}: {len(models)} instances, N {min(m.n for m in models)}.."
-                f"{max(m.n for m in models)}, {time.perf_counter() - t0:.1f} s")
-        X, Y = np.concatenate(feats), np.concatenate(labels)
-        net = ladder.LadderNet(seed=11)
-        t0 = time.perf_counter()
-        losses = net.fit(X, Y, epochs=args.epochs, lr=3e-3)
-        net.save(args.net)
-        line = {"stage": "train", "labels": args.labels, "instances": int(X.shape[0]),
-                "n_min": int(min(sizes)), "n_max": int(max(sizes)),
-                "label_sweeps": args.steps if args.labels == "budget" else args.label_sweeps,
-                "train_loss": [float(losses[0]), float(losses[-1])],
-                "chosen_gamma": {f: {str(g): p.count(k) for k, g in enumerate(ladder.BUDGET_GAMMAS)}
-                                 for f, p in picks.items()},
-                "train_s": round(time.perf_counter() - t0, 1),
-                "total_s": round(time.perf_counter() - t_all, 1), "net": args.net}
-        print(json.dumps(line), flush=True)
-        return 0
-
-    net = ladder.LadderNet(seed=11)
-    net.load(args.net)
-    budget, rows = args.steps, []
-    for fi, fam in enumerate(C5_FAMILIES):
-        T_min, T_max = C5_FAMILIES[fam]
-        models = [c5_instance(fam, 900_000 + 1000 * fi + k, n_target=args.test_n)
-                  for k in range(args.test_per_family)]
-        xs = net.predict(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=R))
-        log(f"{fam}: predicted rung 8 position {np.mean(xs[:, 8]):.3f} (geometric 0.258), "
-            f"rung 16 {np.mean(xs[:, 16]):.3f} (0.516)")
-        seeds = [4242 + k for k in range(len(models))]
-        kw = dict(n_replicas=R, T_min=T_min, T_max=T_max, n_rounds=budget, sweeps_per_round=1,
-                  seeds=seeds, record_history=True)
-        t0 = time.perf_counter()
-        geo = ladder.run_instances(models, **kw)
-        t1 = time.perf_counter()
-        lrn = ladder.run_instances(
-            models, betas=[ladder.betas_from_positions(x, T_min, T_max) for x in xs], **kw)
-        t2 = time.perf_counter()
-        for m, g, l in zip(models, geo, lrn):
-            target = min(g.best_E, l.best_E)
-            rows.append(dict(family=fam, n=m.n,
-                             tt_geo=ladder.sweeps_to_target(g.best_history, target),
-                             tt_learn=ladder.sweeps_to_target(l.best_history, target),
-                             rt_geo=ladder.round_trip_rate(g, budget),
-                             rt_learn=ladder.round_trip_rate(l, budget),
-                             gap=(g.best_E - l.best_E) / abs(target), engine=g.engine))
-        log(f"held-out {fam}: {len(models)} instances at N {models[0].n}, geometric {t1 - t0:.1f} s, "
-            f"learned {t2 - t1:.1f} s")
-
-    def summary(sel):
-        tg = np.array([r["tt_geo"] or 2 * budget for r in sel], dtype=float)
-        tl = np.array([r["tt_learn"] or 2 * budget for r in sel], dtype=float)
-        return {"instances": len(sel), "reached_geo": int((tg <= budget).sum()),
-                "reached_learned": int((tl <= budget).sum()),
-                "median_sweeps_geo": float(np.median(tg)), "median_sweeps_learned": float(np.median(tl)),
-                "learned_first": int((tl < tg).sum()), "geometric_first": int((tg < tl).sum()),
-                "median_rel_gap_geo_minus_learned": float(np.median([r["gap"] for r in sel])),
-                "round_trip_ratio_median": float(np.median(
-                    [r["rt_learn"] / max(r["rt_geo"], 1e-12) for r in sel]))}
-
-    line = {
-        "metric": "time-to-target energy, learned vs geometric ladder (held-out, large N)",
-        "value": summary(rows)["reached_learned"], "unit": "held-out instances at target",
-        "n_gpus": 1, "steps": budget, "warmup": 0, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32/f32", "data": "synthetic",
-        "config": {"workload": f"C5: learned 32-rung ladders, held-out N ~ {args.test_n} "
-                               "(3-D +/-J glass at the nearest cube, 3-regular N(0,1), sparse ER)",
-                   "test_per_family": args.test_per_family, "budget_sweeps": budget,
-                   "target": "best energy found by either ladder within the budget",
-                   "net": args.net},
-        "held_out": summary(rows),
-        "per_family": {f: summary([r for r in rows if r["family"] == f]) for f in C5_FAMILIES},
-        "total_s": round(time.perf_counter() - t_all, 1),
-    }
+    stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval}[args.stage]
+    line = stage(args)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -773,7 +705,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
-                    choices=("all", "c4", "c2", "c3", "c1", "published", "ladder", "ladder-c5"))
+                    choices=("all", "c4", "c2", "c3", "c1", "published", "ladder-c5"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
@@ -785,16 +717,25 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
-    ap.add_argument("--train-per-family", type=int, default=128)
-    ap.add_argument("--stage", default="train", choices=("train", "eval"),
-                    help="ladder-c5 workload: train the network or evaluate a saved one")
-    ap.add_argument("--net", default="ladder_c5_net.npz", help="ladder-c5 network file")
-    ap.add_argument("--labels", default="feedback", choices=("feedback", "budget"),
-                    help="ladder-c5 training labels: feedback-optimised ladders, or the best of "
-                         "the ladder shapes at a budget of --steps sweeps")
+    ap.add_argument("--train-per-family", type=int, default=667)
+    ap.add_argument("--stage", default="train", choices=("train", "bestknown", "eval"),
+                    help="ladder-c5 workload: train the network, record the held-out instances' "
+                         "best-known energies, or measure time to them")
+    ap.add_argument("--net", default=os.path.join(ROOT, "drift_amd", "ladder_net_c5.npz"),
+                    help="ladder-c5 network file")
+    ap.add_argument("--label-budget", type=int, default=5000,
+                    help="ladder-c5: sweeps of each labelling run")
+    ap.add_argument("--cap", type=int, default=5000,
+                    help="ladder-c5: sweeps a held-out run may take to reach its target (the "
+                         "best-known runs take ten times as many)")
+    ap.add_argument("--bestknown", default=os.path.join(ROOT, "tests", "golden",
+                                                        "c5_heldout_bestknown.json"))
+    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "c5"))
     ap.add_argument("--test-n", type=int, default=65536, help="ladder-c5 held-out size")
-    ap.add_argument("--test-per-family", type=int, default=16)
-    ap.add_argument("--label-sweeps", type=int, default=2000)
+    ap.add_argument("--test-per-family", type=int, default=67)
+    ap.add_argument("--families", default="",
+                    help="ladder-c5 bestknown: comma-separated families to run (default all); "
+                         "rows of the others already in --out are kept")
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (rocprof counter scaling)")
@@ -812,8 +753,6 @@ def main() -> int:
         print("bench.py: GPU engine not available (no gfx950 device or library not built)",
               file=sys.stderr)
         return 1
-    if args.workload == "ladder":
-        return bench_ladder(args)
     if args.workload == "ladder-c5":
         return bench_ladder_c5(args)
     return bench_throughput(args)

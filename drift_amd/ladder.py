@@ -85,36 +85,21 @@ def feedback_update(x, f, *, damping: float = 0.5, floor: float = 1e-4) -> np.nd
     return np.maximum.accumulate(out)
 
 
-# instances up to this many spin-replica bytes run in the batched launch (one workgroup's LDS
-# each); larger ones run one per launch on the automatic multi-CU engine
-BATCH_LIMIT = 120_000
-
-
 def run_instances(models, *, n_replicas: int, T_min: float, T_max: float, n_rounds: int,
                   sweeps_per_round: int = 1, seeds=None, betas=None, record_history: bool = False,
-                  record_trace: bool = False):
-    """PtResults of many instances in input order: those small enough in one batched launch, the
-    rest one launch each.  betas: None or one ladder per instance."""
+                  record_trace: bool = False, targets=None):
+    """PtResults of many instances in input order, in one batch call: those that fit one
+    workgroup's LDS in one persistent launch, the others on the per-launch engine, four at a time
+    on streams of their own.  betas: None or one ladder per instance; targets: None or one energy
+    (or None) per instance to stop at."""
     from . import gpu
 
     m = len(models)
     seeds = list(range(m)) if seeds is None else list(seeds)
-    small = [i for i, x in enumerate(models) if x.n * n_replicas <= BATCH_LIMIT]
-    out = [None] * m
-    kw = dict(n_replicas=n_replicas, T_min=T_min, T_max=T_max, n_rounds=n_rounds,
-              sweeps_per_round=sweeps_per_round, record_history=record_history,
-              record_trace=record_trace)
-    if small:
-        res = gpu.parallel_tempering_gpu_batch(
-            [models[i] for i in small], seeds=[seeds[i] for i in small],
-            betas=None if betas is None else [betas[i] for i in small], **kw)
-        for i, r in zip(small, res):
-            out[i] = r
-    for i in range(m):
-        if out[i] is None:
-            out[i] = gpu.parallel_tempering_gpu(models[i], seed=seeds[i],
-                                                betas=None if betas is None else betas[i], **kw)
-    return out
+    return gpu.parallel_tempering_gpu_batch(
+        list(models), n_replicas=n_replicas, T_min=T_min, T_max=T_max, n_rounds=n_rounds,
+        sweeps_per_round=sweeps_per_round, seeds=seeds, betas=betas,
+        record_history=record_history, record_trace=record_trace, targets=targets)
 
 
 def feedback_optimise(models, *, T_min: float, T_max: float, R: int = N_RUNGS,
@@ -148,6 +133,17 @@ def power_positions(gamma: float, R: int = N_RUNGS) -> np.ndarray:
 BUDGET_GAMMAS = (0.5, 0.7, 1.0, 1.4, 2.0)
 
 
+def nearest_power_positions(x, gammas=(min(BUDGET_GAMMAS), max(BUDGET_GAMMAS))) -> np.ndarray:
+    """The power ladder power_positions(gamma) closest to positions x in least squares, gamma
+    restricted to [gammas[0], gammas[1]] (the span of the shapes a network was trained on), so a
+    prediction outside the labelled family falls back to its nearest member."""
+    x = np.asarray(x, dtype=np.float64)
+    R = x.size
+    grid = np.linspace(gammas[0], gammas[1], 301)
+    err = [np.sum((power_positions(g, R) - x) ** 2) for g in grid]
+    return power_positions(float(grid[int(np.argmin(err))]), R)
+
+
 def budget_labels(models, *, T_min: float, T_max: float, R: int = N_RUNGS, budget: int = 20000,
                   gammas=BUDGET_GAMMAS, seeds=None):
     """Time-to-target labels: every instance runs each ladder shape power_positions(gamma) for
@@ -173,6 +169,44 @@ def budget_labels(models, *, T_min: float, T_max: float, R: int = N_RUNGS, budge
 def round_trip_rate(res, sweeps: int) -> float:
     """Round trips per replica per sweep of a result."""
     return float(np.sum(res.round_trips)) / (len(res.round_trips) * max(sweeps, 1))
+
+
+# ------------------------------------------------------------------------------------------------
+# the instance mix the network is trained and judged on
+# ------------------------------------------------------------------------------------------------
+# family: (T_min, T_max) of its ladder
+MIX_FAMILIES = {"ea3d": (0.3, 3.0), "rrg": (0.1, 3.0), "er": (0.2, 10.0)}
+
+
+def mix_instance(family: str, seed: int, n_target: int = 0):
+    """One instance of the mix: a 3-D +/-J glass (the nearest cube), a random 3-regular graph with
+    N(0,1) couplings and N(0, 0.1^2) fields, or a sparse Erdos-Renyi graph with mean degree drawn
+    from [3, 12] and integer weights in {+-1, +-2, +-3}.  N is log-uniform over 256..16384, or
+    about n_target when that is given."""
+    from .builders.lattice import cubic_pmj_glass, er_sparse_weighted, random_regular_gaussian
+
+    rng = np.random.default_rng(seed)
+    n = n_target or int(round(np.exp(rng.uniform(np.log(256), np.log(16384)))))
+    if family == "ea3d":
+        return cubic_pmj_glass(max(6, int(round(n ** (1 / 3)))), seed)
+    if family == "rrg":
+        return random_regular_gaussian(2 * (n // 2), seed)
+    if family == "er":
+        return er_sparse_weighted(n, float(rng.uniform(3.0, 12.0)), seed)
+    raise ValueError(f"unknown family {family!r}; expected one of {sorted(MIX_FAMILIES)}")
+
+
+def heldout_instances(per_family: int, n_target: int = 65536):
+    """The held-out instances: per_family of each family at about n_target spins, seeds apart from
+    every training seed.  Returns [(family, model)]."""
+    return [(fam, mix_instance(fam, 900_000 + 1000 * fi + k, n_target))
+            for fi, fam in enumerate(MIX_FAMILIES) for k in range(per_family)]
+
+
+def training_instances(per_family: int):
+    """The training instances: per_family of each family, N log-uniform over 256..16384."""
+    return [(fam, mix_instance(fam, 10_000 * fi + k))
+            for fi, fam in enumerate(MIX_FAMILIES) for k in range(per_family)]
 
 
 # ------------------------------------------------------------------------------------------------
@@ -355,8 +389,14 @@ class LadderNet:
         np.savez(path, params=self.params, mu=self.mu, sd=self.sd)
 
     def load(self, path: str):
-        d = np.load(path)
-        self.params, self.mu, self.sd = d["params"], d["mu"], d["sd"]
+        with np.load(path, allow_pickle=False) as d:   # plain arrays only
+            self.params, self.mu, self.sd = d["params"], d["mu"], d["sd"]
+
+    def predict_ladder(self, F) -> np.ndarray:
+        """Rung positions for raw features F, each projected onto the nearest power ladder of the
+        shapes the network was trained on (nearest_power_positions): a prediction for an
+        instance unlike the training ones cannot leave the labelled family."""
+        return np.array([nearest_power_positions(x) for x in self.predict(F)])
 
 
 # ------------------------------------------------------------------------------------------------

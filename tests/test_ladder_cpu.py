@@ -116,3 +116,45 @@ def test_power_shapes_and_budget_label_choice(monkeypatch):
     # (round 5); instance 2: all equal, the first shape wins
     assert [ladder.BUDGET_GAMMAS[k] for k in pick] == [0.7, 1.0, 0.5]
     assert np.allclose(x[1], ladder.power_positions(1.0))
+
+
+def test_heldout_best_known_energies_are_energies_of_recorded_states():
+    """The held-out instances' best-known energies (tests/golden/c5_heldout_bestknown.json) are the
+    lowest of their four recorded runs and the energies of the states stored beside them,
+    recomputed in fp64 from the regenerated instances."""
+    import json
+    import os
+
+    from drift_amd import ladder
+
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    bk = json.load(open(os.path.join(here, "c5_heldout_bestknown.json")))
+    states = np.load(os.path.join(here, "c5_heldout_states.npz"))
+    held = ladder.heldout_instances(bk["per_family"], bk["test_n"])
+    assert len(bk["rows"]) == len(held) == 3 * bk["per_family"]
+    for r in bk["rows"][::7]:   # every seventh row: about 30 instances of N ~ 65536
+        fam, model = held[r["index"]]
+        assert fam == r["family"] and model.n == r["n"]
+        assert len(r["runs"]) == 4
+        assert r["best_known_energy"] == min(x["best_E"] for x in r["runs"])
+        s = 1.0 - 2.0 * np.unpackbits(states[str(r["index"])])[: model.n].astype(np.float64)
+        e = model.energy(s)
+        assert e == pytest.approx(r["best_known_energy"], rel=1e-10), r["index"]
+
+
+def test_committed_network_is_weights_only_and_predicts_monotone_ladders():
+    """drift_amd/ladder_net_c5.npz holds plain arrays (39,328 parameters and the feature scaler)
+    and no pickled objects."""
+    import os
+
+    from drift_amd import ladder
+
+    path = os.path.join(os.path.dirname(ladder.__file__), "ladder_net_c5.npz")
+    with np.load(path, allow_pickle=False) as d:
+        assert d["params"].shape == (ladder.N_PARAMS,) and d["params"].dtype == np.float32
+        for k in ("mu", "sd"):
+            assert d[k].shape == (ladder.N_FEATURES,)
+        assert sorted(d.files) == ["mu", "params", "sd"]
+    # the projection onto the labelled family keeps the ends and is monotone
+    x = ladder.nearest_power_positions(np.linspace(0, 1, 32) ** 0.3)
+    assert x[0] == 0.0 and x[-1] == 1.0 and np.all(np.diff(x) > 0)

@@ -137,3 +137,57 @@ def test_learned_ladder_reaches_target_more_often_on_held_out_glasses():
     assert hit_l >= hit_g + 6
     assert int((tl < tg).sum()) >= 10
     assert np.median(tl) < np.median(tg)
+
+
+def _net_c5():
+    import os
+
+    net = ladder.LadderNet(seed=0)
+    net.load(os.path.join(os.path.dirname(ladder.__file__), "ladder_net_c5.npz"))
+    return net
+
+
+def test_committed_network_on_held_out_instances_at_full_size():
+    """The committed network (drift_amd/ladder_net_c5.npz) on one held-out instance of each family
+    at N ~ 65536: ladders monotone with the family's ends fixed, a batched run equal to the same
+    instance run alone, and reported energies equal to the energies of the reported states."""
+    _announce("committed network, held-out N ~ 65536, one per family")
+    net = _net_c5()
+    held = ladder.heldout_instances(67, 65536)
+    for k in (0, 67, 134):
+        fam, model = held[k]
+        T_min, T_max = ladder.MIX_FAMILIES[fam]
+        x = net.predict_ladder(ladder.instance_features([model], T_min=T_min, T_max=T_max))[0]
+        b = ladder.betas_from_positions(x, T_min, T_max)
+        assert np.all(np.diff(b) < 0)
+        assert b[0] == pytest.approx(1 / T_min) and b[-1] == pytest.approx(1 / T_max)
+        kw = dict(n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=150, sweeps_per_round=1,
+                  return_final=True)
+        batched = gpu.parallel_tempering_gpu_batch([model, model], seeds=[7, 8], betas=[b, b], **kw)
+        alone = gpu.parallel_tempering_gpu(model, seed=7, betas=b, **kw)
+        assert alone.best_E_units == batched[0].best_E_units
+        assert np.array_equal(alone.final_spins, batched[0].final_spins)
+        assert alone.best_E == pytest.approx(model.energy(alone.best_s), rel=1e-10)
+        e64 = model.energy_batch(alone.final_spins.astype(np.float64))
+        assert np.allclose(alone.final_E * alone.best_E / alone.best_E_units, e64, rtol=1e-10)
+        print(f"[ladder-gpu] {fam} N={model.n}: learned power "
+              f"{np.log(x[8]) / np.log(8 / 31):.2f}, best {alone.best_E:.2f}", flush=True)
+
+
+def test_committed_network_gains_on_held_out_random_graphs():
+    """The committed network's ladders against the geometric ones on six held-out 3-regular
+    graphs at N ~ 65536, 3000 sweeps on the same streams: the learned ladders end lower."""
+    _announce("committed network vs geometric, six held-out 3-regular graphs")
+    net = _net_c5()
+    held = [m for f, m in ladder.heldout_instances(6, 65536) if f == "rrg"]
+    T_min, T_max = ladder.MIX_FAMILIES["rrg"]
+    xs = net.predict_ladder(ladder.instance_features(held, T_min=T_min, T_max=T_max))
+    kw = dict(n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=3000, sweeps_per_round=1,
+              seeds=range(60, 66))
+    geo = gpu.parallel_tempering_gpu_batch(held, **kw)
+    lrn = gpu.parallel_tempering_gpu_batch(
+        held, betas=[ladder.betas_from_positions(x, T_min, T_max) for x in xs], **kw)
+    lower = sum(l.best_E < g.best_E for l, g in zip(lrn, geo))
+    gaps = [(g.best_E - l.best_E) / abs(g.best_E) for l, g in zip(lrn, geo)]
+    print(f"[ladder-gpu] learned lower on {lower}/6; relative gains {np.round(gaps, 5)}", flush=True)
+    assert lower >= 5
