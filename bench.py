@@ -691,6 +691,47 @@ def c5_eval(args) -> dict:
     }
 
 
+def bench_c3_ttt(args) -> dict:
+    """C3's time-to-target report: the best energy a plain run of 10 x --steps sweeps finds (seed 1,
+    geometric ladder) is the target; runs of at most --steps sweeps from another seed, on the
+    geometric ladder and on the committed network's ladder, stop when they reach it.  Wall time is
+    the library call's host time; device time is the sweep loop's."""
+    from drift_amd import gpu, ladder
+
+    model, R, beta, S, desc = workload("c3")
+    T_min, T_max = 1.0 / beta[0], 1.0 / beta[-1]
+    cap = args.steps
+    log(f"c3-ttt: plain run of {10 * cap} sweeps for the target")
+    t0 = time.perf_counter()
+    long = gpu.parallel_tempering_gpu(model, n_replicas=R, betas=beta, n_rounds=10 * cap,
+                                      sweeps_per_round=1, seed=1, record_history=False)
+    long_wall = time.perf_counter() - t0
+    target = long.best_E
+    net = ladder.LadderNet(seed=0)
+    net.load(os.path.join(ROOT, "drift_amd", "ladder_net_c5.npz"))
+    x = net.predict_ladder(ladder.instance_features([model], T_min=T_min, T_max=T_max, R=R))[0]
+    runs = {}
+    for name, b in (("geometric", beta), ("learned", ladder.betas_from_positions(x, T_min, T_max))):
+        log(f"c3-ttt: {name} ladder, at most {cap} sweeps")
+        t0 = time.perf_counter()
+        r = gpu.parallel_tempering_gpu(model, n_replicas=R, betas=b, n_rounds=cap, sweeps_per_round=1,
+                                       seed=2, target_energy=target, record_history=True)
+        wall = time.perf_counter() - t0
+        runs[name] = {"reached": bool(r.best_E <= target), "rounds_run": r.rounds_run,
+                      "wall_s": round(wall, 3), "device_s": round(r.seconds, 3),
+                      "best_E": r.best_E, "gap_to_target": (r.best_E - target) / abs(target),
+                      "within_1e-4_after": _reach(r.best_history[:max(r.rounds_run, 1)], target, 1e-4),
+                      "within_1e-3_after": _reach(r.best_history[:max(r.rounds_run, 1)], target, 1e-3),
+                      "engine": r.engine}
+    return {"metric": "time to the best energy of a 10x longer plain run (C3)",
+            "value": runs["geometric"]["wall_s"], "unit": "s (geometric ladder; the cap's time if "
+            "not reached)", "higher_is_better": False, "n_gpus": 1, "steps": cap,
+            "config": {"workload": desc, "target_run_sweeps": 10 * cap, "target_run_seed": 1,
+                       "timed_seed": 2, "learned_power": float(np.log(x[8]) / np.log(8 / 31))},
+            "target": target, "target_found_after_round": long.best_round,
+            "target_run_wall_s": round(long_wall, 2), "runs": runs}
+
+
 def bench_ladder_c5(args) -> int:
     stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval}[args.stage]
     line = stage(args)
@@ -705,7 +746,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
-                    choices=("all", "c4", "c2", "c3", "c1", "published", "ladder-c5"))
+                    choices=("all", "c4", "c2", "c3", "c1", "published", "c3-ttt", "ladder-c5"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
@@ -755,6 +796,9 @@ def main() -> int:
         return 1
     if args.workload == "ladder-c5":
         return bench_ladder_c5(args)
+    if args.workload == "c3-ttt":
+        print(json.dumps(bench_c3_ttt(args)), flush=True)
+        return 0
     return bench_throughput(args)
 
 
