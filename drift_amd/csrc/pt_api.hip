@@ -324,8 +324,9 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     H.gq = quad_chunk(H.row_ptr, n);
     for (int p = 0; p < n; ++p) H.maxdeg = std::max(H.maxdeg, H.row_ptr[p + 1] - H.row_ptr[p]);
     if (isf) {
-        if (maxfield >= 32768.0 || total_abs >= 1.0e8)
-            return fail(DRIFT_PT_ERANGE, "float couplings too large for the 2^-36 fixed-point energy");
+        if (maxfield >= 16384.0 || total_abs >= 1.0e8)
+            return fail(DRIFT_PT_ERANGE, "float couplings too large for the 2^-36 fixed-point energy "
+                                         "(local fields up to 2^14, energies up to 1e8)");
         H.unit = PT_FIX36_UNIT;
     } else {
         if (maxfield >= (double)(1 << 30)) return fail(DRIFT_PT_ERANGE, "local field exceeds 2^30");

@@ -142,8 +142,18 @@ PT_HD bool pt_accept_exchange(double b_lo, double b_hi, double e_lo, double e_hi
 
 // Float energies are carried as int64 multiples of 2^-36 so that sums are associative (the
 // same in any reduction order) and therefore identical on every engine and on the CPU.
-PT_HD int64_t pt_fix36d(double v) { return (int64_t)floor(v * 68719476736.0 + 0.5); }
-PT_HD int64_t pt_fix36(float v) { return pt_fix36d((double)v); }
+// v in 2^-36 units, rounded to the nearest integer (ties to even): adding 1.5 * 2^52 to the exact
+// v * 2^36 leaves that integer in the low mantissa bits, read back by subtracting the constant's
+// bits.  Exact for |v * 2^36| < 2^51, which the host's bound on float local fields (|field| <
+// 2^14, so |2 q| < 2^15) ensures.
+PT_HD int64_t pt_fix36d(double v) {
+    union {
+        double d;
+        int64_t i;
+    } c;
+    c.d = v * 68719476736.0 + 6755399441055744.0;
+    return c.i - (int64_t)0x4338000000000000ll;
+}
 
 // Float couplings: the local field h_i + sum_j J_ij s_j of float32 values is summed in double, in
 // row order.  A double holds the sum of float32 values whose magnitudes lie within 2^29 of one

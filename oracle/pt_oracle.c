@@ -119,8 +119,12 @@ static int exch_accept(double b_lo, double b_hi, double e_lo, double e_hi, uint3
     return (double)u < or_exp(d) * 4294967296.0;
 }
 
+/* v in 2^-36 units, rounded to nearest with ties to even (exact for |v 2^36| < 2^51): the exact
+ * product plus 1.5 * 2^52 keeps the integer in the low mantissa bits. */
 static inline int64_t fix36d(double v) {
-    return (int64_t)floor(v * 68719476736.0 + 0.5);
+    union { double d; int64_t i; } c;
+    c.d = v * 68719476736.0 + 6755399441055744.0;
+    return c.i - (int64_t)0x4338000000000000ll;
 }
 
 /* ------------------------------------------------------------------------------------------
