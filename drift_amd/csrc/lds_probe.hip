@@ -1,14 +1,22 @@
-// lds_probe.hip -- the LDS read rate of this device, measured, for the LDS roofline of the engines
-// whose state lives in LDS (the batched LDS engine, the persistent slab engines).  Built into
-// libdriftprobe.so for the benchmark; not part of the engine's C ABI (include/drift_pt.h).
+// lds_probe.hip -- device probes, built into libdriftprobe.so; not part of the engine's C ABI
+// (include/drift_pt.h).  The LDS read rate of this device, for the LDS roofline of the engines whose
+// state lives in LDS (the batched LDS engine, the persistent slab engines), and the largest gap
+// between the hardware exponential and the deterministic one over every float argument the float
+// Metropolis decision can see, which the engine's fast decision relies on.
 //
 //   int drift_probe_lds(int kind, int iters, double* seconds, double* bytes, int64_t* instrs)
 //     kind 0: 16-byte reads (ds_read_b128), kind 1: 1-byte reads (ds_read_u8).  Eight 256-thread
 //     workgroups per CU each make `iters` reads per lane; seconds is the device time of the timed
 //     launch (after one warm-up launch), bytes the bytes read, instrs the wave-level LDS read
 //     instructions.  Returns 0, or a negative HIP error code.
+//   int drift_probe_exp_margin(double* max_rel, float* worst_x)
+//     max over every float x in (-87, 0] of |p - pa| / pa, p = pt_expf_det(x), pa = pt_exp_hw(x),
+//     and the x where it occurs.  Returns 0, or a negative HIP error code.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
+#include "pt_device.h"
 
 namespace {
 
@@ -41,7 +49,47 @@ __global__ __launch_bounds__(256) void lds_read_u8(uint32_t* sink, int iters, ui
     if (acc == never) sink[blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
+// Float x = -(bits i) for i in [0, 0x42AE0000): -0 up to the float just above -87.  Each lane keeps
+// the largest (relative gap bits, i) pair of its strided share; the gap is non-negative, so its
+// float bits order like the gap and the packed pair orders by gap, then by i.
+constexpr uint32_t kExpArgs = 0x42AE0000u;
+
+__global__ __launch_bounds__(256) void exp_margin(unsigned long long* best) {
+    unsigned long long mine = 0;
+    const uint32_t stride = gridDim.x * 256u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < kExpArgs; i += stride) {
+        const float x = __uint_as_float(0x80000000u | i);
+        const double p = (double)pt_expf_det(x);
+        const double pa = (double)pt_dev::pt_exp_hw(x);
+        const float rel = (float)(fabs(p - pa) / pa);
+        const unsigned long long key = ((unsigned long long)__float_as_uint(rel) << 32) | i;
+        mine = key > mine ? key : mine;
+    }
+    atomicMax(best, mine);
+}
+
 }  // namespace
+
+extern "C" int drift_probe_exp_margin(double* max_rel, float* worst_x) {
+    unsigned long long* best = nullptr;
+    unsigned long long host = 0;
+    hipError_t e = hipMalloc(&best, sizeof(*best));
+    if (e == hipSuccess) e = hipMemset(best, 0, sizeof(*best));
+    if (e == hipSuccess) {
+        exp_margin<<<4096, 256>>>(best);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&host, best, sizeof(host), hipMemcpyDeviceToHost);
+    if (best) (void)hipFree(best);
+    if (e != hipSuccess) return -(int)e;
+    const uint32_t relbits = (uint32_t)(host >> 32), i = (uint32_t)host;
+    float rel;
+    memcpy(&rel, &relbits, sizeof(rel));
+    const uint32_t xbits = 0x80000000u | i;
+    memcpy(worst_x, &xbits, sizeof(*worst_x));
+    *max_rel = rel;
+    return 0;
+}
 
 extern "C" int drift_probe_lds(int kind, int iters, double* seconds, double* bytes,
                                int64_t* instrs) {

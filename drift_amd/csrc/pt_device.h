@@ -137,6 +137,29 @@ __device__ __forceinline__ int edge_float(const void* __restrict__ e, int t, flo
 // Sign bit of a +-1 spin as a float's sign: XORed into a coupling's bits it gives J * s exactly.
 __device__ __forceinline__ uint32_t pt_sign_bit(int8_t s) { return (uint32_t)(int32_t)s & 0x80000000u; }
 
+// exp(x) from the hardware base-2 exponential (v_exp_f32), for x <= 0.
+__device__ __forceinline__ float pt_exp_hw(float x) {
+    return __builtin_amdgcn_exp2f(x * 0x1.715476p+0f);
+}
+
+// pt_accept_float with the same result, decided from the hardware exponential where that cannot
+// differ: with pa = pt_exp_hw(x) and p = pt_expf_det(x), |p - pa| <= 2^-13 pa for every float
+// x in (-87, 0] (drift_probe_exp_margin checks this on the device for all of them).  Then
+//   u < floor(p 2^32)  whenever  (float)u < pa 2^32 (1 - 2^-12) - 512,  and
+//   u >= floor(p 2^32) whenever  (float)u > pa 2^32 (1 + 2^-12) + 512,
+// the 512 covering the rounding of (float)u (at most 128), the floor (1) and the float rounding
+// of the bounds themselves (at most 128 each).  For x <= -87, p = 0 and the first test fails
+// while the second holds for u > 512.  Only u within the band between the two bounds (about
+// 2^-11 of them) takes the exact polynomial.
+__device__ __forceinline__ bool pt_accept_float_dev(float beta, float q, uint32_t u) {
+    if (q <= 0.0f) return true;
+    const float ta = pt_exp_hw(-(beta * (2.0f * q))) * 4294967296.0f;
+    const float uf = (float)u;
+    if (uf < ta * 0x1.ffep-1f - 512.0f) return true;
+    if (uf > ta * 0x1.001p+0f + 512.0f) return false;
+    return pt_accept_float(beta, q, u);
+}
+
 // Spin access.  COH accesses are agent-scope relaxed atomics (loads and stores coherent across
 // the chip without cache maintenance): the coop engine's workgroups read spins other workgroups
 // wrote in the same launch.  Other engines use plain accesses.
@@ -307,7 +330,7 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
         if (!on[k]) continue;
         if constexpr (WM == PT_WM_F32) {
             const double qd = si[k] > 0 ? l[k] : -l[k];
-            if (pt_accept_float(beta_f, (float)qd, pt_word(rv, k))) {
+            if (pt_accept_float_dev(beta_f, (float)qd, pt_word(rv, k))) {
                 own[k * Rp] = (int8_t)-si[k];
                 d += pt_fix36d(2.0 * qd);
             }
@@ -382,7 +405,7 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
     }
     if constexpr (WM == PT_WM_F32) {
         const double q = si > 0 ? l : -l;
-        if (pt_accept_float(beta_f, (float)q, u)) {
+        if (pt_accept_float_dev(beta_f, (float)q, u)) {
             *own = (int8_t)-si;
             return pt_fix36d(2.0 * q);
         }

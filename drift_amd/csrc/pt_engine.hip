@@ -405,7 +405,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
             const pt_u4 rv = pt_philox((uint32_t)(p >> 2), (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
             if constexpr (WM == PT_WM_F32) {
                 const double q = si > 0 ? l : -l;
-                if (pt_accept_float((float)I.beta[g], (float)q, pt_word(rv, p & 3))) {
+                if (pt_accept_float_dev((float)I.beta[g], (float)q, pt_word(rv, p & 3))) {
                     flip = true;
                     d = pt_fix36d(2.0 * q);
                 }
