@@ -9,11 +9,13 @@
 //     workgroups per CU each make `iters` reads per lane; seconds is the device time of the timed
 //     launch (after one warm-up launch), bytes the bytes read, instrs the wave-level LDS read
 //     instructions.  Returns 0, or a negative HIP error code.
-//   int drift_probe_exp_margin(double* max_rel, float* worst_x)
+//   int drift_probe_exp_margin(double* max_rel, float* worst_x, double* min_nonneg)
 //     max over every float x in (-87, 0] of |p - pa| / pa, p = pt_expf_det(x), pa = pt_exp_hw(x),
-//     and the x where it occurs.  Returns 0, or a negative HIP error code.
+//     and the x where it occurs; min_nonneg is the least pt_exp_hw(x) over every float x >= 0
+//     (+inf included).  Returns 0, or a negative HIP error code.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <math.h>
 #include <string.h>
 
 #include "pt_device.h"
@@ -68,20 +70,44 @@ __global__ __launch_bounds__(256) void exp_margin(unsigned long long* best) {
     atomicMax(best, mine);
 }
 
+// Least pt_exp_hw(x) over x = +(bits i), i in [0, 0x7F800000]: +0 up to +inf.  The values are
+// non-negative (or NaN, kept as the largest bits), so their float bits order like the values.
+__global__ __launch_bounds__(256) void exp_nonneg_min(unsigned int* least) {
+    unsigned int mine = 0xFFFFFFFFu;
+    const uint32_t stride = gridDim.x * 256u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i <= 0x7F800000u; i += stride) {
+        const float pa = pt_dev::pt_exp_hw(__uint_as_float(i));
+        const unsigned int b = pa == pa ? __float_as_uint(pa) : 0xFFFFFFFFu;
+        mine = b < mine ? b : mine;
+    }
+    atomicMin(least, mine);
+}
+
 }  // namespace
 
-extern "C" int drift_probe_exp_margin(double* max_rel, float* worst_x) {
+extern "C" int drift_probe_exp_margin(double* max_rel, float* worst_x, double* min_nonneg) {
     unsigned long long* best = nullptr;
     unsigned long long host = 0;
-    hipError_t e = hipMalloc(&best, sizeof(*best));
+    unsigned int least_host = 0;
+    hipError_t e = hipMalloc(&best, sizeof(*best) + sizeof(unsigned int));
+    unsigned int* least = (unsigned int*)(best + 1);
     if (e == hipSuccess) e = hipMemset(best, 0, sizeof(*best));
+    if (e == hipSuccess) e = hipMemset(least, 0xFF, sizeof(unsigned int));
     if (e == hipSuccess) {
         exp_margin<<<4096, 256>>>(best);
         e = hipGetLastError();
     }
+    if (e == hipSuccess) {
+        exp_nonneg_min<<<4096, 256>>>(least);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMemcpy(&host, best, sizeof(host), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&least_host, least, sizeof(least_host), hipMemcpyDeviceToHost);
     if (best) (void)hipFree(best);
     if (e != hipSuccess) return -(int)e;
+    float least_f;
+    memcpy(&least_f, &least_host, sizeof(least_f));
+    *min_nonneg = least_host == 0xFFFFFFFFu ? NAN : (double)least_f;
     const uint32_t relbits = (uint32_t)(host >> 32), i = (uint32_t)host;
     float rel;
     memcpy(&rel, &relbits, sizeof(rel));

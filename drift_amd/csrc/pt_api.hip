@@ -363,6 +363,11 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
             H.beta[r] = 1.0 / std::max(T, 1e-12);
         }
     }
+    // the float decision computes -(beta (2 q)) as (-2 beta) q, the same float when 2 beta is finite
+    if (isf)
+        for (double b : H.beta)
+            if (b >= 0x1p100)
+                return fail(DRIFT_PT_ERANGE, "beta entries must be below 2^100 with float couplings");
 
     // integer thresholds [rung][q-1]; moves with q > TQ are rejected at every rung
     if (!isf) {

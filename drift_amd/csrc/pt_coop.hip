@@ -100,7 +100,7 @@ __device__ __forceinline__ bool decide(int8_t si, acc_t<WM> l, uint32_t u,
                                        const uint32_t* thr_row, int TQ, float bf, int64_t& d) {
     if constexpr (WM == PT_WM_F32) {
         const double q = si > 0 ? l : -l;
-        d = pt_fix36d(2.0 * q);
+        d = pt_fix37d(q);
         return pt_accept_float_dev(bf, (float)q, u);
     } else {
         const int32_t q = si > 0 ? l : -l;

@@ -143,21 +143,35 @@ __device__ __forceinline__ float pt_exp_hw(float x) {
 }
 
 // pt_accept_float with the same result, decided from the hardware exponential where that cannot
-// differ: with pa = pt_exp_hw(x) and p = pt_expf_det(x), |p - pa| <= 2^-13 pa for every float
-// x in (-87, 0] (drift_probe_exp_margin checks this on the device for all of them).  Then
-//   u < floor(p 2^32)  whenever  (float)u < pa 2^32 (1 - 2^-12) - 512,  and
-//   u >= floor(p 2^32) whenever  (float)u > pa 2^32 (1 + 2^-12) + 512,
-// the 512 covering the rounding of (float)u (at most 128), the floor (1) and the float rounding
-// of the bounds themselves (at most 128 each).  For x <= -87, p = 0 and the first test fails
-// while the second holds for u > 512.  Only u within the band between the two bounds (about
-// 2^-11 of them) takes the exact polynomial.
+// differ.  x = (-2 beta) q is pt_accept_float's -(beta (2 q)) (one rounding of the same product;
+// the host keeps beta below 2^100 so 2 beta is exact).  With pa = pt_exp_hw(x) and
+// p = pt_expf_det(x), |p - pa| <= 2^-13 pa for every float x in (-87, 0]
+// (drift_probe_exp_margin checks this on the device for all of them).  Then for q > 0
+//   u < floor(p 2^32)  whenever  (float)u < fma(pa, 2^32 (1 - 2^-12), -512),  and
+//   u >= floor(p 2^32) whenever  (float)u > fma(pa, 2^32 (1 + 2^-12), 512):
+// the relative margin 2^-12 exceeds the 2^-13 gap plus the bound's rounding (2^-24), and 512
+// exceeds the rounding of (float)u (at most 128) plus the floor (1) plus 128.  For x <= -87,
+// p = 0 and the first test fails while the second holds for u > 512; p >= 1 (threshold 2^32 - 1)
+// only where pa > 1 - 2^-13, so the second test never holds there.  For q <= 0 (always accepted)
+// pa >= 1 - 2^-13 as well, or +inf, so the second test fails and the first holds outside the band.
+// Only u in the band between the bounds (about 2^-11 of them) takes the exact polynomial.
 __device__ __forceinline__ bool pt_accept_float_dev(float beta, float q, uint32_t u) {
-    if (q <= 0.0f) return true;
-    const float ta = pt_exp_hw(-(beta * (2.0f * q))) * 4294967296.0f;
+    const float pa = pt_exp_hw((-2.0f * beta) * q);
     const float uf = (float)u;
-    if (uf < ta * 0x1.ffep-1f - 512.0f) return true;
-    if (uf > ta * 0x1.001p+0f + 512.0f) return false;
-    return pt_accept_float(beta, q, u);
+    const bool acc = uf < __builtin_fmaf(pa, 0x1.ffep+31f, -512.0f);
+    const bool rej = uf > __builtin_fmaf(pa, 0x1.001p+32f, 512.0f);
+    if (__builtin_expect(acc == rej, 0)) return pt_accept_float(beta, q, u);
+    return acc;
+}
+
+// pt_fix36d(2 v): 2^37 v is exact, so one fused multiply-add gives the same rounded sum.
+__device__ __forceinline__ int64_t pt_fix37d(double v) {
+    union {
+        double d;
+        int64_t i;
+    } c;
+    c.d = __builtin_fma(v, 137438953472.0, 6755399441055744.0);
+    return c.i - (int64_t)0x4338000000000000ll;
 }
 
 // Spin access.  COH accesses are agent-scope relaxed atomics (loads and stores coherent across
@@ -332,7 +346,7 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
             const double qd = si[k] > 0 ? l[k] : -l[k];
             if (pt_accept_float_dev(beta_f, (float)qd, pt_word(rv, k))) {
                 own[k * Rp] = (int8_t)-si[k];
-                d += pt_fix36d(2.0 * qd);
+                d += pt_fix37d(qd);
             }
         } else {
             const int32_t qi = si[k] > 0 ? l[k] : -l[k];
@@ -407,7 +421,7 @@ __device__ __forceinline__ int64_t site_update_ell(int p, int a, int Rp, int8_t*
         const double q = si > 0 ? l : -l;
         if (pt_accept_float_dev(beta_f, (float)q, u)) {
             *own = (int8_t)-si;
-            return pt_fix36d(2.0 * q);
+            return pt_fix37d(q);
         }
         return 0;
     } else {

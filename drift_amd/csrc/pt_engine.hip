@@ -407,7 +407,7 @@ __global__ __launch_bounds__(1024) void g_color_bits(PtInstance I, GState G, int
                 const double q = si > 0 ? l : -l;
                 if (pt_accept_float_dev((float)I.beta[g], (float)q, pt_word(rv, p & 3))) {
                     flip = true;
-                    d = pt_fix36d(2.0 * q);
+                    d = pt_fix37d(q);
                 }
             } else {
                 const uint32_t* thr_row = I.thr + (size_t)g * I.TQ;

@@ -285,6 +285,13 @@ def test_engine_errors_are_reported():
                                    coloring=(np.array([0, 4]), np.arange(4)))
     with pytest.raises(RuntimeError, match="n_replicas"):
         gpu.parallel_tempering_gpu(model, n_replicas=2000, n_rounds=1)
+    # float couplings take beta below 2^100 (integer ones any finite beta)
+    Jf = J.copy()
+    Jf[0, 1] = Jf[1, 0] = 0.5 + 2.0 ** -20
+    with pytest.raises(RuntimeError, match="2\\^100"):
+        gpu.parallel_tempering_gpu(IsingModel(Jf, np.zeros(4)), n_replicas=2, n_rounds=1,
+                                   betas=np.array([1.0, 2.0 ** 100]), force_float=True)
+    gpu.parallel_tempering_gpu(model, n_replicas=2, n_rounds=1, betas=np.array([1.0, 2.0 ** 100]))
 
 
 def test_autotune_times_candidates_and_keeps_results():
