@@ -1542,14 +1542,57 @@ int energy_path_choice(const HostInst& H, int block_req, int block) {
     return choice;
 }
 
+// Bit slabs run on the neighbour-synchronised kernel unless the run records the best energy of
+// every round or the rung trace (both need the whole ladder each round) or
+// DRIFT_PT_PSLAB_SYNC=grid asks for the grid-barrier kernel.
+bool pslab_use_nbr(const HostInst& H) {
+    if (!pt_pslab_bits(H.lL)) return false;
+    if (H.flags & (DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE)) return false;
+    const char* e = std::getenv("DRIFT_PT_PSLAB_SYNC");
+    return !(e && std::strcmp(e, "grid") == 0);
+}
+
+// Colour-list position of lattice site `site` (the parity colouring in index order).
+int slab_pos_of(int site, const HostInst& H) {
+    const int L = H.lL;
+    const int x = site % L, y = (site / L) % L, z = H.ldim == 3 ? site / (L * L) : 0;
+    const int par = (H.ldim == 3 ? x + y + z : x + (site / L)) & 1;
+    return par * (H.n / 2) + (site >> 1);
+}
+
+// The neighbour-synchronised kernel leaves one best record per slot: the run's best is their
+// lexicographic minimum over (energy, round, rung), its state that slot's bits.
+int nbr_best(const HostInst& H, const PtInstance& I, const SState& S, const NbrArgs& N) {
+    std::vector<int64_t> sb(4 * (size_t)H.R);
+    PT_HIP(hipMemcpy(sb.data(), N.sbest, sizeof(int64_t) * sb.size(), hipMemcpyDeviceToHost));
+    int w = 0;
+    for (int x = 1; x < H.R; ++x) {
+        const int64_t* c = &sb[4 * (size_t)x];
+        const int64_t* m = &sb[4 * (size_t)w];
+        if (c[0] < m[0] || (c[0] == m[0] && (c[1] < m[1] || (c[1] == m[1] && c[2] < m[2])))) w = x;
+    }
+    const int64_t rec[5] = {sb[4 * (size_t)w], sb[4 * (size_t)w + 1], sb[4 * (size_t)w + 2], w, 1};
+    PT_HIP(hipMemcpy(S.best + 5 * (H.n_rounds & 1), rec, sizeof(rec), hipMemcpyHostToDevice));
+    const size_t words = (size_t)H.n / 32;
+    std::vector<uint32_t> bits(words);
+    PT_HIP(hipMemcpy(bits.data(), N.bestbits + (size_t)w * words, sizeof(uint32_t) * words,
+                     hipMemcpyDeviceToHost));
+    std::vector<int8_t> st((size_t)H.n);
+    for (int site = 0; site < H.n; ++site)
+        st[(size_t)slab_pos_of(site, H)] = ((bits[(size_t)site >> 5] >> (site & 31)) & 1u) ? -1 : 1;
+    PT_HIP(hipMemcpy(I.best_s, st.data(), st.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
 int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     size_t lds = 0;
     bool upre = false;
     const int block = pslab_block(H, block_req, lds, &upre);
     if (!block) return fail(DRIFT_PT_EINVAL, "pslab engine: the slab grid cannot be co-resident");
+    const bool nbr = pslab_use_nbr(H);
     // chosen before this run allocates anything; DRIFT_PT_PSLAB_ENERGY_PATH=atomic|words fixes it
     int ebar = 0;
-    if (upre) {
+    if (upre && !nbr) {
         const char* ep = std::getenv("DRIFT_PT_PSLAB_ENERGY_PATH");
         if (H.force_ebar >= 0) ebar = H.force_ebar;
         else if (ep && std::strcmp(ep, "atomic") == 0) ebar = 0;
@@ -1590,15 +1633,33 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
         PT_HIP(hipMemsetAsync(C.erel, 0, sizeof(uint64_t) * 64, sg.s));
     }
     C.timeout_ticks = barrier_timeout_ticks();
+    NbrArgs N{};
+    if (nbr) {
+        const size_t nbox = (size_t)NBR_ROUNDS * H.Rp * S.NB;
+        PT_HIP(A.alloc(&N.box, nbox));
+        PT_HIP(A.alloc(&N.hflag, (size_t)H.Rp * S.NB));
+        PT_HIP(A.alloc(&N.sbest, 4 * (size_t)H.Rp));
+        PT_HIP(A.alloc(&N.bestbits, (size_t)H.Rp * H.n / 32));
+        PT_HIP(hipMemsetAsync(N.box, 0xFF, sizeof(uint64_t) * nbox, sg.s));
+        PT_HIP(hipMemsetAsync(N.hflag, 0, sizeof(uint32_t) * H.Rp * S.NB, sg.s));
+        // rung and pair counters are added into
+        PT_HIP(hipMemsetAsync(I.out, 0, sizeof(int64_t) * pt_out_len(H.R), sg.s));
+    }
     PT_HIP(hipEventRecord(sg.e0, sg.s));
-    PT_HIP(pt_launch_pslab(I, S, C, block, lds, !(H.flags & DRIFT_PT_PLAIN_LAUNCH), sg.s));
+    if (nbr)
+        PT_HIP(pt_launch_pslab_nbr(I, S, C, N, block, lds, !(H.flags & DRIFT_PT_PLAIN_LAUNCH), sg.s));
+    else
+        PT_HIP(pt_launch_pslab(I, S, C, block, lds, !(H.flags & DRIFT_PT_PLAIN_LAUNCH), sg.s));
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
     PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
     int32_t aborted = 0;
     PT_HIP(hipMemcpy(&aborted, C.abort, sizeof(aborted), hipMemcpyDeviceToHost));
-    if (aborted) return fail(DRIFT_PT_EHIP, "pslab engine: a grid barrier wait exceeded its bound");
+    if (aborted)
+        return fail(DRIFT_PT_EHIP, nbr ? "pslab engine: a neighbour wait exceeded its bound"
+                                       : "pslab engine: a grid barrier wait exceeded its bound");
+    if (nbr && (rc = nbr_best(H, I, S, N)) != 0) return rc;
     rc = finish_slab(H, I, S, res);
     if (rc) return rc;
     res->seconds = ms * 1e-3;

@@ -668,6 +668,229 @@ __device__ __forceinline__ void bits_row(uint32_t bits, int8_t* dst) {
     }
 }
 
+// Geometry and LDS layout of one workgroup's bit slab.  Slot s of the slab holds plane
+// (WRAP ? s : z0 - 2 + s) mod L.  A sweep is two passes: colour 0 over the slab and its two halo
+// planes (slots 1 .. T + 2), then colour 1 over the owned planes (slots 2 .. T + 1).
+struct BitSlab {
+    int L, P, Lb, WPR, WPP, wppb;          // side, sites per plane, log2 L, words per row and plane,
+                                           // log2 words per plane
+    int z0, T, nslots, jlo, njs, so;       // first owned plane, owned planes, staged spin slots,
+                                           // first bond slot, bond slots, slot of plane z0
+    int segb, rb, wb, pb, psb, jstride;    // log2 of: 8-site segments per row, rows per plane, words
+                                           // per row, sites per plane, segments per slot; words per
+                                           // bond direction
+    int ns0, ns1, uoff1, nU;               // slots of the two passes; uniforms of a sweep
+    uint32_t* sb;                          // [slot][row][word] spin bits (1 = spin -1)
+    uint32_t* jb;                          // [dir][bond slot][row][word] bond bits (1 = bond -1)
+    uint4* U;                              // next-sweep uniforms: one Philox block per segment
+};
+
+template <int DIM, bool WRAP>
+__device__ __forceinline__ BitSlab make_bit_slab(const Geo& g, int NB, int b, uint32_t* lw) {
+    BitSlab B;
+    B.L = g.L;
+    B.P = g.P;
+    B.Lb = 31 - __builtin_clz(g.L);
+    B.WPR = g.L >> 5;
+    B.WPP = (DIM == 3 ? g.L : 1) * B.WPR;
+    B.wppb = (DIM == 3 ? B.Lb : 0) + B.Lb - 5;
+    B.z0 = b * g.L / NB;
+    B.T = (b + 1) * g.L / NB - B.z0;
+    B.nslots = WRAP ? g.L : B.T + 4;
+    B.jlo = WRAP ? 0 : 1;
+    B.njs = WRAP ? g.L : B.T + 2;
+    B.so = WRAP ? 0 : 2;
+    B.segb = B.Lb - 3;
+    B.rb = DIM == 3 ? B.Lb : 0;
+    B.wb = B.Lb - 5;
+    B.pb = DIM == 3 ? 2 * B.Lb : B.Lb;
+    B.psb = B.rb + B.segb;
+    B.jstride = B.njs << B.wppb;
+    B.ns0 = WRAP ? g.L : B.T + 2;
+    B.ns1 = WRAP ? g.L : B.T;
+    B.uoff1 = B.ns0 << B.psb;
+    B.nU = B.uoff1 + (B.ns1 << B.psb);
+    B.sb = lw;
+    B.jb = lw + (size_t)B.nslots * B.WPP;
+    B.U = (uint4*)(lw + ((((size_t)B.nslots + (size_t)2 * DIM * B.njs) * B.WPP + 3) & ~(size_t)3));
+    return B;
+}
+
+// Pass c, slot i of the pass -> slab slot.
+template <bool WRAP>
+__device__ __forceinline__ int bit_pass_slot(int c, int i) {
+    return WRAP ? i : (c == 0 ? 1 + i : 2 + i);
+}
+
+// Stage slot a's slab (spin buffer 0, written by the init kernels) and its bond planes as bits.
+template <int DIM, bool WRAP>
+__device__ void bit_stage(const BitSlab& B, const PtInstance& I, const SState& S, const Geo& g,
+                          int a) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int8_t* in = S.sp + (size_t)a * g.N;
+    for (int idx = tid; idx < B.nslots * B.WPP; idx += nt) {
+        const int sl = idx >> B.wppb, rw = idx & (B.WPP - 1);
+        const int plane = WRAP ? sl : (B.z0 - 2 + sl) & (B.L - 1);
+        B.sb[idx] = row_bits(in + ((size_t)plane * B.P) + ((size_t)rw << 5));
+    }
+    for (int idx = tid; idx < 2 * DIM * B.njs * B.WPP; idx += nt) {
+        const int d = idx / (B.njs * B.WPP), r2 = idx - d * B.njs * B.WPP;
+        const int sl = B.jlo + (r2 >> B.wppb), rw = r2 & (B.WPP - 1);
+        const int plane = WRAP ? sl : (B.z0 - 2 + sl) & (B.L - 1);
+        B.jb[idx] = bond_bits(I.jbits + ((size_t)plane * B.P) + ((size_t)rw << 5), d);
+    }
+}
+
+// The Philox blocks of sweep tt (they depend on position, slot and sweep only) into U, entries
+// first, first + stride, ...
+template <int DIM, bool WRAP>
+__device__ void bit_draw(const BitSlab& B, const PtInstance& I, const Geo& g, int a, uint32_t tt,
+                         int first, int stride) {
+    for (int idx = first; idx < B.nU; idx += stride) {
+        const int c = idx < B.uoff1 ? 0 : 1;
+        const int w = idx - (c == 0 ? 0 : B.uoff1);
+        const int i = w >> B.psb, rem = w & ((1 << B.psb) - 1);
+        const int s = bit_pass_slot<WRAP>(c, i);
+        const int r = rem >> B.segb, k = rem & ((1 << B.segb) - 1);
+        const int plane = WRAP ? s : (B.z0 - 2 + s) & (B.L - 1);
+        const uint32_t q = ((uint32_t)(c * g.half) >> 2) +
+                           ((uint32_t)((plane << B.pb) + (r << B.Lb) + (k << 3)) >> 3);
+        const pt_u4 v = pt_philox(q, (uint32_t)a, tt, PT_TAG_METRO, I.k0, I.k1);
+        B.U[idx] = make_uint4(v.x, v.y, v.z, v.w);
+    }
+}
+
+// Both colour passes of sweep t (every thread of the workgroup calls it; each pass ends with a
+// workgroup barrier).  Returns the energy change of the owned sites.
+//
+// One lane per 32-site row word: the neighbour words, bond planes and unsatisfied-bond counts of
+// the word serve its 16 sites of the colour, decided against four Philox blocks (one per 8-site
+// segment, the blocks every engine draws; from U when pre).  The lane owns the word in this pass;
+// lanes reading it as a neighbour use only the other colour's bits, which do not change, so a
+// plain store suffices.  t2, t4, t6: thresholds of q = 2, 4, 6 at the slot's rung.
+template <int DIM, bool WRAP>
+__device__ __forceinline__ int64_t bit_sweep(const BitSlab& B, const PtInstance& I, const Geo& g,
+                                             int a, uint32_t t, bool pre, uint32_t t2, uint32_t t4,
+                                             uint32_t t6) {
+    constexpr int D = 2 * DIM;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int L = B.L, Lb = B.Lb, rb = B.rb, wb = B.wb, WPR = B.WPR, T = B.T;
+    uint32_t* sb = B.sb;
+    const uint32_t* jb = B.jb;
+    const int wpb = rb + wb;                      // log2 words per slot
+    int64_t dsum = 0;
+    for (int c = 0; c < 2; ++c) {
+        const int ns = c == 0 ? B.ns0 : B.ns1;
+        const int uo = c == 0 ? 0 : B.uoff1;
+        for (int w = tid; w < (ns << wpb); w += nt) {
+            const int i = w >> wpb, rem = w & ((1 << wpb) - 1);
+            const int s = bit_pass_slot<WRAP>(c, i);
+            const int r = rem >> wb, wi = rem & (WPR - 1);
+            const int plane = WRAP ? s : (B.z0 - 2 + s) & (L - 1);
+            const bool owned = WRAP || (s >= 2 && s < T + 2);
+            const int su = WRAP ? ((s + 1) & (L - 1)) : s + 1;
+            const int sd = WRAP ? ((s - 1) & (L - 1)) : s - 1;
+            // word index of (slot, row) + wi, by shifts: rows = 2^rb, words/row = 2^wb
+            const int wrow = (((s << rb) + r) << wb) + wi;
+            const uint32_t W = sb[wrow];
+            const uint32_t Wn = sb[wrow - wi + ((wi + 1) & (WPR - 1))];
+            const uint32_t Wp = sb[wrow - wi + ((wi - 1) & (WPR - 1))];
+            uint32_t A[6];
+            A[0] = (W >> 1) | (Wn << 31);               // +x
+            A[1] = (W << 1) | (Wp >> 31);               // -x
+            if (DIM == 3) {
+                A[2] = sb[(((s << rb) + ((r + 1) & (L - 1))) << wb) + wi];
+                A[3] = sb[(((s << rb) + ((r - 1) & (L - 1))) << wb) + wi];
+                A[4] = sb[(((su << rb) + r) << wb) + wi];
+                A[5] = sb[(((sd << rb) + r) << wb) + wi];
+            } else {
+                A[2] = sb[(su << wb) + wi];
+                A[3] = sb[(sd << wb) + wi];
+            }
+            const int jrow = ((((s - B.jlo) << rb) + r) << wb) + wi;
+            // Y[d] = 1 where the bond to neighbour d is unsatisfied (s_i J s_j = -1):
+            // neighbour bit ^ bond bit ^ own bit
+            uint32_t Y[6];
+#pragma unroll
+            for (int d = 0; d < D; ++d) Y[d] = A[d] ^ jb[d * B.jstride + jrow] ^ W;
+            // unsatisfied-bond count m of every site: bits c0, c1, c2
+            uint32_t c0, c1, c2;
+            if (DIM == 3) {
+                uint32_t s1, k1, s2, k2;
+                fadd(Y[0], Y[1], Y[2], s1, k1);
+                fadd(Y[3], Y[4], Y[5], s2, k2);
+                c0 = s1 ^ s2;
+                fadd(k1, k2, s1 & s2, c1, c2);
+            } else {
+                uint32_t s1, k1;
+                fadd(Y[0], Y[1], Y[2], s1, k1);
+                c0 = s1 ^ Y[3];
+                const uint32_t k0 = s1 & Y[3];
+                c1 = k1 ^ k0;
+                c2 = k1 & k0;
+            }
+            const int par = DIM == 3 ? ((c + r + plane) & 1) : ((c + plane) & 1);
+            // q = D - 2m <= 0 always flips: m >= 3 in 3-D, m >= 2 in 2-D.  Otherwise the
+            // threshold of q: 3-D m = 2, 1, 0 -> t2, t4, t6; 2-D m = 1, 0 -> t2, t4.
+            const uint32_t always = DIM == 3 ? (c2 | (c1 & c0)) : (c2 | c1);
+            uint32_t flip = 0;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {          // the word's four segments
+                pt_u4 rv;
+                if (pre) {
+                    const uint4 v = B.U[uo + (w << 2) + kk];
+                    rv = pt_u4{v.x, v.y, v.z, v.w};
+                } else {
+                    const uint32_t q = ((uint32_t)(c * g.half) >> 2) +
+                                       ((uint32_t)((plane << B.pb) + (r << Lb) + (wi << 5) +
+                                                   (kk << 3)) >> 3);
+                    rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
+                }
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) {
+                    const uint32_t bit = 1u << (8 * kk + par + 2 * ii);
+                    const uint32_t th = DIM == 3 ? ((c1 & bit) ? t2 : ((c0 & bit) ? t4 : t6))
+                                                 : ((c0 & bit) ? t2 : t4);
+                    flip |= (always & bit) | (pt_word(rv, ii) < th ? bit : 0u);
+                }
+            }
+            sb[wrow] = W ^ flip;
+            // energy change of the flips: sum of 2q = 2(D - 2m) over the flipped sites (in
+            // signed arithmetic: __popc returns unsigned)
+            if (owned) {
+                const int nf = (int)__popc(flip);
+                const int sm = (int)__popc(flip & c0) + 2 * (int)__popc(flip & c1) +
+                               4 * (int)__popc(flip & c2);
+                dsum += 2 * (D * nf - 2 * sm);
+            }
+        }
+        __syncthreads();
+    }
+    return dsum;
+}
+
+// The first and last min(2, T) owned planes -- the neighbours' halos -- into nxt ([plane][word]).
+__device__ __forceinline__ void bit_publish(const BitSlab& B, uint32_t* nxt) {
+    const int nb = B.T < 2 ? B.T : 2;
+    for (int idx = threadIdx.x; idx < 2 * nb * B.WPP; idx += blockDim.x) {
+        const int kk = idx >> B.wppb, w = idx & (B.WPP - 1);
+        const int o = kk < nb ? kk : B.T - 2 * nb + kk;
+        st_u32(nxt + (size_t)(B.z0 + o) * B.WPP + w, B.sb[(size_t)(B.so + o) * B.WPP + w]);
+    }
+}
+
+// The four halo slots (0, 1, T + 2, T + 3) from src ([plane][word]), entries first, first +
+// stride, ...
+__device__ __forceinline__ void bit_load_halos(const BitSlab& B, const uint32_t* src, int first,
+                                               int stride) {
+    for (int idx = first; idx < 4 * B.WPP; idx += stride) {
+        const int kk = idx >> B.wppb, w = idx & (B.WPP - 1);
+        const int sl = kk < 2 ? kk : B.T + kk;
+        const int plane = (B.z0 - 2 + sl) & (B.L - 1);
+        B.sb[(size_t)sl * B.WPP + w] = ld_u32(src + (size_t)plane * B.WPP + w);
+    }
+}
+
 // At most 512 threads: the next-sweep uniform path needs more than 128 registers per lane.
 // EBAR: the energy path of the barrier (SState::ebar), fixed per instantiation.
 template <int DIM, bool WRAP, bool EBAR>
@@ -688,42 +911,20 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     __shared__ uint32_t sh_gen, sh_uready;
     __shared__ uint32_t sh_uex[PT_WAVE];
     __shared__ long long sh_esum[PT_WAVE];   // workgroup 0: slot sums of the arrival words
-    constexpr int D = 2 * DIM;
     int a, b;
     wg_slot_slab(blockIdx.x, S.NB, a, b);
     const bool live = a < I.R, owner = blockIdx.x == 0;
     const Geo g = make_geo<DIM>(I);
+    const BitSlab B = make_bit_slab<DIM, WRAP>(g, S.NB, b, lw);
     const int L = g.L, P = g.P, tid = threadIdx.x, nt = blockDim.x;
-    const int Lb = 31 - __builtin_clz(L);
-    const int WPR = L >> 5;                               // words per row
-    const int rows = DIM == 3 ? L : 1;                    // rows per plane
-    const int WPP = rows * WPR;                           // words per plane
-    const int wppb = (DIM == 3 ? Lb : 0) + Lb - 5;        // log2 WPP
+    const int WPP = B.WPP, wppb = B.wppb, T = B.T, z0 = B.z0, z1 = z0 + T, so = B.so;
     const int R = I.R, Rp = I.Rp;
-    const int z0 = b * L / S.NB, z1 = (b + 1) * L / S.NB, T = z1 - z0;
-    const int nslots = WRAP ? L : T + 4;
-    const int jlo = WRAP ? 0 : 1, njs = WRAP ? L : T + 2;
-    const int so = WRAP ? 0 : 2;
-    uint32_t* sb = lw;                                    // [slot][row][word]
-    uint32_t* jb = lw + (size_t)nslots * WPP;             // [dir][jslot][row][word]
+    uint32_t* sb = B.sb;
     uint32_t* hb = S.hbits;                               // [parity][slot][plane][word]
     const size_t hrep = (size_t)L * WPP;                  // words per replica and parity
 
     // ---- stage the slab (bits) and the bond planes ----------------------------------------------
-    if (live) {
-        const int8_t* in = S.sp + (size_t)a * g.N;        // spin buffer 0, written by the init kernels
-        for (int idx = tid; idx < nslots * WPP; idx += nt) {
-            const int sl = idx >> wppb, rw = idx & (WPP - 1);
-            const int plane = WRAP ? sl : (z0 - 2 + sl) & (L - 1);
-            sb[idx] = row_bits(in + ((size_t)plane * P) + ((size_t)rw << 5));
-        }
-        for (int idx = tid; idx < D * njs * WPP; idx += nt) {
-            const int d = idx / (njs * WPP), r2 = idx - d * njs * WPP;
-            const int sl = jlo + (r2 >> wppb), rw = r2 & (WPP - 1);
-            const int plane = WRAP ? sl : (z0 - 2 + sl) & (L - 1);
-            jb[idx] = bond_bits(I.jbits + ((size_t)plane * P) + ((size_t)rw << 5), d);
-        }
-    }
+    if (live) bit_stage<DIM, WRAP>(B, I, S, g, a);
     for (int x = tid; x < R; x += nt) {
         betaL[x] = I.beta[x];
         labelL[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
@@ -748,21 +949,9 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     if (fast && tid < PT_WAVE) wave_ladder_init(WL, I, S.einit, tid);
 
     uint32_t phase = 0;
-    const int segb = Lb - 3;                              // log2 segments of 8 sites per row
-    const int rb = DIM == 3 ? Lb : 0;                     // log2 rows per plane
-    const int wb = Lb - 5;                                // log2 words per row
-    const int pb = DIM == 3 ? 2 * Lb : Lb;                // log2 sites per plane
-    const int psb = rb + segb;                            // log2 segments per slot
-    const int jstride = njs << wppb;                      // words per bond direction
-    // A sweep is two passes: colour 0 over the slab and its two halo planes (slots 1 .. T + 2), then
-    // colour 1 over the owned planes (slots 2 .. T + 1).  Pass c, slot i -> slab slot.
-    auto pass_slot = [&](int c, int i) -> int { return WRAP ? i : (c == 0 ? 1 + i : 2 + i); };
-    const int ns0 = WRAP ? L : T + 2, ns1 = WRAP ? L : T;
-    const int uoff1 = ns0 << psb;
-    const int nU = uoff1 + (ns1 << psb);                  // 8-site segments a sweep updates
-    // Next-sweep uniforms: with S.upre, waves 1.. draw the Philox blocks of sweep t + 1 (they depend
-    // on position, slot and sweep only) into LDS while wave 0 alone completes the grid barrier and
-    // the round end; the colour passes then read them.
+    // Next-sweep uniforms: with S.upre, waves 1.. draw the Philox blocks of sweep t + 1 into LDS
+    // while wave 0 alone completes the grid barrier and the round end; the colour passes then read
+    // them.
     const bool pre = S.upre != 0 && fast && nt > PT_WAVE;
     // ebar (with pre): the energy changes of a round reach wave 0 through the barrier itself.  Each
     // workgroup's arrival word carries its change; workgroup 0 sums them by slot into the release
@@ -770,24 +959,8 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     // changes go through atomic adds into S.dacc before the arrival flags.
     const bool ebar = EBAR && pre;
     int64_t relv = 0;
-    uint4* U = (uint4*)(lw + ((((size_t)nslots + (size_t)D * njs) * WPP + 3) & ~(size_t)3));
     const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
-    auto draw = [&](uint32_t tt, int first, int stride) {
-        for (int idx = first; idx < nU; idx += stride) {
-            const int c = idx < uoff1 ? 0 : 1;
-            const int w = idx - (c == 0 ? 0 : uoff1);
-            const int i = w >> psb, rem = w & ((1 << psb) - 1);
-            const int s = pass_slot(c, i);
-            const int r = rem >> segb, k = rem & ((1 << segb) - 1);
-            const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
-            const int col = c;
-            const uint32_t q = ((uint32_t)(col * g.half) >> 2) +
-                               ((uint32_t)((plane << pb) + (r << Lb) + (k << 3)) >> 3);
-            const pt_u4 v = pt_philox(q, (uint32_t)a, tt, PT_TAG_METRO, I.k0, I.k1);
-            U[idx] = make_uint4(v.x, v.y, v.z, v.w);
-        }
-    };
-    if (pre && live && nsweeps > 0) draw(0u, tid, nt);
+    if (pre && live && nsweeps > 0) bit_draw<DIM, WRAP>(B, I, g, a, 0u, tid, nt);
     __syncthreads();
     for (int rnd = 0;; ++rnd) {
         // ---- end of round rnd - 1 (rnd == 0: the initial state), identical in every workgroup
@@ -815,13 +988,8 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                     __hip_atomic_store(S.dacc + (size_t)((rnd + 1) % 3) * Rp + tid, (int64_t)0,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else if (defer && live && rnd > 0 && rnd < I.n_rounds) {
-                const uint32_t* prv = hb + ((size_t)(((uint32_t)rnd * (uint32_t)I.S) & 1) * Rp + a) * hrep;
-                for (int idx = tid - PT_WAVE; idx < 4 * WPP; idx += nt - PT_WAVE) {
-                    const int kk = idx >> wppb, w = idx & (WPP - 1);
-                    const int sl = kk < 2 ? kk : T + kk;
-                    const int plane = (z0 - 2 + sl) & (L - 1);
-                    sb[(size_t)sl * WPP + w] = ld_u32(prv + (size_t)plane * WPP + w);
-                }
+                bit_load_halos(B, hb + ((size_t)(((uint32_t)rnd * (uint32_t)I.S) & 1) * Rp + a) * hrep,
+                               tid - PT_WAVE, nt - PT_WAVE);
             }
             __syncthreads();
             if (sh_abort) return;
@@ -899,111 +1067,8 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
         for (int j = 0; j < I.S; ++j) {
             const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
             uint32_t* nxt = hb + ((size_t)((t + 1) & 1) * Rp + a) * hrep;
-            // the first and last min(2, T) owned planes: the neighbours' halos
-            auto publish = [&]() {
-                const int nb = T < 2 ? T : 2;
-                for (int idx = tid; idx < 2 * nb * WPP; idx += nt) {
-                    const int kk = idx >> wppb, w = idx & (WPP - 1);
-                    const int o = kk < nb ? kk : T - 2 * nb + kk;
-                    st_u32(nxt + (size_t)(z0 + o) * WPP + w, sb[(size_t)(so + o) * WPP + w]);
-                }
-            };
-            // One lane per 32-site row word: the neighbour words, bond planes and unsatisfied-bond
-            // counts of the word serve its 16 sites of the colour, decided against four Philox
-            // blocks (one per 8-site segment, the blocks every engine draws).  The lane owns the
-            // word in this pass; lanes reading it as a neighbour use only the other colour's bits,
-            // which do not change, so a plain store suffices.
-            if (live) {
-                const int wpb = rb + wb;                      // log2 words per slot
-                for (int c = 0; c < 2; ++c) {
-                    const int ns = c == 0 ? ns0 : ns1;
-                    const int uo = c == 0 ? 0 : uoff1;
-                    for (int w = tid; w < (ns << wpb); w += nt) {
-                        const int i = w >> wpb, rem = w & ((1 << wpb) - 1);
-                        const int s = pass_slot(c, i);
-                        const int r = rem >> wb, wi = rem & (WPR - 1);
-                        const int plane = WRAP ? s : (z0 - 2 + s) & (L - 1);
-                        const bool owned = WRAP || (s >= 2 && s < T + 2);
-                        const int su = WRAP ? ((s + 1) & (L - 1)) : s + 1;
-                        const int sd = WRAP ? ((s - 1) & (L - 1)) : s - 1;
-                        // word index of (slot, row) + wi, by shifts: rows = 2^rb, words/row = 2^wb
-                        const int wrow = (((s << rb) + r) << wb) + wi;
-                        const uint32_t W = sb[wrow];
-                        const uint32_t Wn = sb[wrow - wi + ((wi + 1) & (WPR - 1))];
-                        const uint32_t Wp = sb[wrow - wi + ((wi - 1) & (WPR - 1))];
-                        uint32_t A[6];
-                        A[0] = (W >> 1) | (Wn << 31);               // +x
-                        A[1] = (W << 1) | (Wp >> 31);               // -x
-                        if (DIM == 3) {
-                            A[2] = sb[(((s << rb) + ((r + 1) & (L - 1))) << wb) + wi];
-                            A[3] = sb[(((s << rb) + ((r - 1) & (L - 1))) << wb) + wi];
-                            A[4] = sb[(((su << rb) + r) << wb) + wi];
-                            A[5] = sb[(((sd << rb) + r) << wb) + wi];
-                        } else {
-                            A[2] = sb[(su << wb) + wi];
-                            A[3] = sb[(sd << wb) + wi];
-                        }
-                        const int jrow = ((((s - jlo) << rb) + r) << wb) + wi;
-                        // Z[d] = 1 where the bond to neighbour d is unsatisfied (s_i J s_j = -1):
-                        // neighbour bit ^ bond bit ^ own bit
-                        uint32_t Y[6];
-#pragma unroll
-                        for (int d = 0; d < D; ++d) Y[d] = A[d] ^ jb[d * jstride + jrow] ^ W;
-                        // unsatisfied-bond count m of every site: bits c0, c1, c2
-                        uint32_t c0, c1, c2;
-                        if (DIM == 3) {
-                            uint32_t s1, k1, s2, k2;
-                            fadd(Y[0], Y[1], Y[2], s1, k1);
-                            fadd(Y[3], Y[4], Y[5], s2, k2);
-                            c0 = s1 ^ s2;
-                            fadd(k1, k2, s1 & s2, c1, c2);
-                        } else {
-                            uint32_t s1, k1;
-                            fadd(Y[0], Y[1], Y[2], s1, k1);
-                            c0 = s1 ^ Y[3];
-                            const uint32_t k0 = s1 & Y[3];
-                            c1 = k1 ^ k0;
-                            c2 = k1 & k0;
-                        }
-                        const int par = DIM == 3 ? ((c + r + plane) & 1) : ((c + plane) & 1);
-                        // q = D - 2m <= 0 always flips: m >= 3 in 3-D, m >= 2 in 2-D.  Otherwise
-                        // the threshold of q: 3-D m = 2, 1, 0 -> t2, t4, t6; 2-D m = 1, 0 -> t2, t4.
-                        const uint32_t always = DIM == 3 ? (c2 | (c1 & c0)) : (c2 | c1);
-                        uint32_t flip = 0;
-#pragma unroll
-                        for (int kk = 0; kk < 4; ++kk) {          // the word's four segments
-                            pt_u4 rv;
-                            if (pre) {
-                                const uint4 v = U[uo + (w << 2) + kk];
-                                rv = pt_u4{v.x, v.y, v.z, v.w};
-                            } else {
-                                const uint32_t q = ((uint32_t)(c * g.half) >> 2) +
-                                                   ((uint32_t)((plane << pb) + (r << Lb) + (wi << 5) +
-                                                               (kk << 3)) >> 3);
-                                rv = pt_philox(q, (uint32_t)a, t, PT_TAG_METRO, I.k0, I.k1);
-                            }
-#pragma unroll
-                            for (int ii = 0; ii < 4; ++ii) {
-                                const uint32_t bit = 1u << (8 * kk + par + 2 * ii);
-                                const uint32_t th = DIM == 3 ? ((c1 & bit) ? t2 : ((c0 & bit) ? t4 : t6))
-                                                             : ((c0 & bit) ? t2 : t4);
-                                flip |= (always & bit) | (pt_word(rv, ii) < th ? bit : 0u);
-                            }
-                        }
-                        sb[wrow] = W ^ flip;
-                        // energy change of the flips: sum of 2q = 2(D - 2m) over the flipped sites
-                        // (in signed arithmetic: __popc returns unsigned)
-                        if (owned) {
-                            const int nf = (int)__popc(flip);
-                            const int sm = (int)__popc(flip & c0) + 2 * (int)__popc(flip & c1) +
-                                           4 * (int)__popc(flip & c2);
-                            dsum += 2 * (D * nf - 2 * sm);
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            if (live && !WRAP) publish();
+            if (live) dsum += bit_sweep<DIM, WRAP>(B, I, g, a, t, pre, t2, t4, t6);
+            if (live && !WRAP) bit_publish(B, nxt);
             const bool last = j == I.S - 1;
             if (last) {   // the slot's energy change: waves into LDS, thread 0 to HBM on arrival
                 long long v = (long long)dsum;
@@ -1123,18 +1188,12 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                         if (tid == PT_WAVE)
                             __hip_atomic_store(&sh_uready, phase, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    if (live && t + 1 < nsweeps) draw(t + 1, tid - PT_WAVE, nt - PT_WAVE);
+                    if (live && t + 1 < nsweeps) bit_draw<DIM, WRAP>(B, I, g, a, t + 1, tid - PT_WAVE, nt - PT_WAVE);
                     while (phase_before(__hip_atomic_load(&sh_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), phase))
                         __builtin_amdgcn_s_sleep(1);
                 }
                 if (last) continue;   // the round end brings in the halo planes
-                if (tid >= PT_WAVE && live && !WRAP)
-                    for (int idx = tid - PT_WAVE; idx < 4 * WPP; idx += nt - PT_WAVE) {
-                        const int kk = idx >> wppb, w = idx & (WPP - 1);
-                        const int sl = kk < 2 ? kk : T + kk;
-                        const int plane = (z0 - 2 + sl) & (L - 1);
-                        sb[(size_t)sl * WPP + w] = ld_u32(nxt + (size_t)plane * WPP + w);
-                    }
+                if (tid >= PT_WAVE && live && !WRAP) bit_load_halos(B, nxt, tid - PT_WAVE, nt - PT_WAVE);
                 __syncthreads();
                 if (sh_abort) return;
                 continue;
@@ -1143,14 +1202,7 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
                            last && live ? &sh_dsum : nullptr))
                 return;
             if (last && defer) continue;   // halo planes come in during the round end
-            if (live && !WRAP) {
-                for (int idx = tid; idx < 4 * WPP; idx += nt) {
-                    const int kk = idx >> wppb, w = idx & (WPP - 1);
-                    const int sl = kk < 2 ? kk : T + kk;
-                    const int plane = (z0 - 2 + sl) & (L - 1);
-                    sb[(size_t)sl * WPP + w] = ld_u32(nxt + (size_t)plane * WPP + w);
-                }
-            }
+            if (live && !WRAP) bit_load_halos(B, nxt, tid, nt);
             __syncthreads();
         }
     }
@@ -1180,6 +1232,214 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Bit-sliced persistent slab engine with neighbour-scoped synchronisation.  The same sweeps as
+// pslab_bits_run, but no workgroup waits for the whole grid:
+//  * halos: before sweep t a workgroup waits only for the workgroups owning its four halo planes
+//    (the adjacent slabs of its own slot) to have published sweep t - 1 (NbrArgs::hflag);
+//  * round end: each workgroup posts its slab's energy change, tagged with the round, into the
+//    mailbox of its slot's rung (NbrArgs::box; slab 0 adds the slot's energy of the last round
+//    end, so the words of a rung sum to that slot's new energy).  Exchange partners read the two
+//    rungs' words -- nothing else -- and take the same decision (the pair's uniform, betas and
+//    energies); a slot carries its own rung, label and trip count.
+//  * best: each slot keeps its own best (energy, then first round, at the rung it held) and its
+//    state as bits (NbrArgs::sbest, bestbits); the host's lexicographic minimum over the slots is
+//    the first round whose lowest rung reached the lowest energy, which the per-round scan of the
+//    reference finds (cuda/ising_pt.cu:126-137 tracks per replica, 224-229 takes the minimum).
+// Counters of rungs and pairs are added atomically (one writer per rung and round, but rounds of
+// different slots overlap).  A workgroup is at most a few rounds ahead of any rung it reads (a rung
+// pairs with each neighbour every second round), within the NBR_ROUNDS buffered rounds; a word
+// whose tag is not the round awaited is not taken.  Best energy per round and the rung trace need
+// the whole ladder every round: runs that record them use pslab_bits_run.
+// ---------------------------------------------------------------------------------------------
+template <int DIM, bool WRAP>
+__global__ __launch_bounds__(512) void pslab_bits_nbr(PtInstance I, SState S, CoopArgs C,
+                                                     NbrArgs N) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lw[];
+    __shared__ uint32_t thr3[3 * 1024];
+    __shared__ double betaL[1024];
+    __shared__ int sh_abort, sh_imp, sh_rung;
+    __shared__ long long sh_dsum;
+    int a, b;
+    wg_slot_slab(blockIdx.x, S.NB, a, b);
+    if (a >= I.R) return;                                 // padding slots take no part
+    const Geo g = make_geo<DIM>(I);
+    const BitSlab B = make_bit_slab<DIM, WRAP>(g, S.NB, b, lw);
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (PT_WAVE - 1);
+    const int R = I.R, Rp = I.Rp, NB = S.NB;
+    uint32_t* hb = S.hbits;                               // [parity][slot][plane][word]
+    const size_t hrep = (size_t)g.L * B.WPP;              // words per replica and parity
+
+    bit_stage<DIM, WRAP>(B, I, S, g, a);
+    for (int x = tid; x < R; x += nt) {
+        betaL[x] = I.beta[x];
+        const uint32_t* tr = I.thr + (size_t)x * I.TQ;
+        thr3[3 * x] = I.TQ >= 2 ? tr[1] : 0u;
+        thr3[3 * x + 1] = I.TQ >= 4 ? tr[3] : 0u;
+        thr3[3 * x + 2] = (DIM == 3 && I.TQ >= 6) ? tr[5] : 0u;
+    }
+    if (tid == 0) {
+        sh_abort = 0;
+        sh_imp = 0;
+        sh_rung = a;
+        sh_dsum = 0;
+    }
+    const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
+    // waves 1.. draw the next sweep's uniforms while wave 0 waits for the mailboxes
+    const bool pre = S.upre != 0 && nt > PT_WAVE;
+    if (pre && nsweeps > 0) bit_draw<DIM, WRAP>(B, I, g, a, 0u, tid, nt);
+    // the slot's ladder state, the same in every workgroup of the slot (wave 0's copy is used)
+    int64_t E = -(S.einit[a] / 2);
+    int gr = a, trips = 0;
+    int lab = (R > 1 && a == 0) ? 1 : ((R > 1 && a == R - 1) ? -1 : 0);
+    int64_t bestE = E;
+    int best_round = 0, best_rung = a;
+    uint32_t* mybest = N.bestbits + (size_t)a * hrep;
+    // halo slot k (0, 1, T + 2, T + 3): plane and the slab that owns it
+    int howner = 0;
+    if (!WRAP && lane < 4) {
+        const int sl = lane < 2 ? lane : B.T + lane;
+        const int plane = (B.z0 - 2 + sl) & (g.L - 1);
+        howner = ((plane + 1) * NB - 1) / g.L;
+    }
+    const int hw0 = nt > PT_WAVE ? PT_WAVE : 0;           // the lanes that wait for halos
+    __syncthreads();
+    // the initial state is every slot's first best (round 0)
+    for (int idx = tid; idx < B.T * B.WPP; idx += nt)
+        mybest[(size_t)B.z0 * B.WPP + idx] = B.sb[(size_t)B.so * B.WPP + idx];
+
+    for (int rnd = 0; rnd < I.n_rounds; ++rnd) {
+        const int gcur = sh_rung;
+        const uint32_t t2 = thr3[3 * gcur], t4 = thr3[3 * gcur + 1], t6 = thr3[3 * gcur + 2];
+        int64_t dsum = 0;
+        for (int j = 0; j < I.S; ++j) {
+            const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
+            const bool last = j == I.S - 1, more = t + 1 < nsweeps;
+            uint32_t* nxt = hb + ((size_t)((t + 1) & 1) * Rp + a) * hrep;
+            dsum += bit_sweep<DIM, WRAP>(B, I, g, a, t, pre, t2, t4, t6);
+            if (!WRAP && more) bit_publish(B, nxt);
+            if (last) {
+                long long v = (long long)dsum;
+#pragma unroll
+                for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
+                if (lane == 0 && v != 0) atomicAdd((unsigned long long*)&sh_dsum, (unsigned long long)v);
+                dsum = 0;
+            }
+            // every wave's halo stores complete before the flag that releases them
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (!WRAP && more && tid == 0) st_u32(N.hflag + (size_t)a * NB + b, t + 1);
+            if (tid < PT_WAVE && last) {
+                // ---- round end: this slot and its exchange partner, from two rungs' mailboxes
+                uint64_t* box = N.box + (size_t)(rnd & (NBR_ROUNDS - 1)) * Rp * NB;
+                if (lane == 0)
+                    st_u64(box + (size_t)gr * NB + b,
+                           tag_word((uint32_t)rnd, (b == 0 ? E : 0) + (int64_t)sh_dsum));
+                const int par = rnd & 1;
+                const bool lo = R > 1 && gr >= par && ((gr - par) & 1) == 0 && gr + 1 < R;
+                const bool hi = R > 1 && gr >= par + 1 && ((gr - 1 - par) & 1) == 0;
+                const int pr = lo ? gr + 1 : (hi ? gr - 1 : -1);
+                int64_t own = 0, oth = 0;
+                bool bad = false;
+                for (int x = lane; x < 2 * NB; x += PT_WAVE) {
+                    const bool mine = x < NB;
+                    if (!mine && pr < 0) continue;
+                    uint64_t w = 0;
+                    if (!wait_tag(C, box + (size_t)(mine ? gr : pr) * NB + (mine ? x : x - NB),
+                                  (uint32_t)rnd, w)) {
+                        bad = true;
+                        break;
+                    }
+                    if (mine) own += word_value(w);
+                    else oth += word_value(w);
+                }
+#pragma unroll
+                for (int m = PT_WAVE / 2; m > 0; m >>= 1) {
+                    own += __shfl_xor(own, m);
+                    oth += __shfl_xor(oth, m);
+                }
+                if (__any(bad)) {
+                    if (lane == 0) sh_abort = 1;
+                } else {
+                    const int64_t En = own;
+                    const bool imp = En < bestE;
+                    if (imp) {
+                        bestE = En;
+                        best_round = rnd + 1;
+                        best_rung = gr;
+                    }
+                    if (R > 1) {
+                        int64_t* o = I.out;
+                        if (pr >= 0) {
+                            const int rlo = lo ? gr : pr;
+                            const pt_u4 rv = pt_philox((uint32_t)(rlo >> 2), 0u, (uint32_t)rnd,
+                                                       PT_TAG_EXCH, I.k0, I.k1);
+                            const int64_t Elo = lo ? En : oth, Ehi = lo ? oth : En;
+                            const bool ok = pt_accept_exchange(betaL[rlo], betaL[rlo + 1],
+                                                               (double)Elo * I.unit,
+                                                               (double)Ehi * I.unit,
+                                                               pt_word(rv, rlo & 3));
+                            if (lo && b == 0 && lane == 0) {
+                                bump(pt_out_arr(o, R, PT_OUT_TRIED) + gr);
+                                if (ok) bump(pt_out_arr(o, R, PT_OUT_ACC) + gr);
+                            }
+                            if (ok) gr = pr;
+                        }
+                        if (gr == 0) {
+                            if (lab == -1) ++trips;
+                            lab = 1;
+                        } else if (gr == R - 1) {
+                            lab = -1;
+                        }
+                        if (b == 0 && lane == 0) {
+                            if (lab == 1) bump(pt_out_arr(o, R, PT_OUT_UP) + gr);
+                            else if (lab == -1) bump(pt_out_arr(o, R, PT_OUT_DOWN) + gr);
+                        }
+                    }
+                    E = En;
+                    if (lane == 0) {
+                        sh_rung = gr;
+                        sh_imp = imp;
+                        sh_dsum = 0;
+                    }
+                }
+            } else if (tid >= PT_WAVE && pre && more) {
+                bit_draw<DIM, WRAP>(B, I, g, a, t + 1, tid - PT_WAVE, nt - PT_WAVE);
+            }
+            // the halo planes of sweep t + 1: their owners' flags reach t + 1
+            if (!WRAP && more && tid >= hw0 && tid < hw0 + 4)
+                if (!wait_phase(C, N.hflag + (size_t)a * NB + howner, t + 1)) sh_abort = 1;
+            __syncthreads();
+            if (sh_abort) return;
+            if (!WRAP && more) bit_load_halos(B, nxt, tid, nt);
+            if (last && sh_imp)   // the owned planes after this round's sweeps: the slot's new best
+                for (int idx = tid; idx < B.T * B.WPP; idx += nt)
+                    mybest[(size_t)B.z0 * B.WPP + idx] = B.sb[(size_t)B.so * B.WPP + idx];
+            __syncthreads();
+        }
+    }
+
+    // ---- final state and the slot's ladder bookkeeping --------------------------------------------
+    int8_t* fb = S.sp + ((size_t)(((int64_t)I.n_rounds * I.S) & 1) * Rp + a) * g.N;
+    for (int idx = tid; idx < B.T * B.WPP; idx += nt) {
+        const int o = idx >> B.wppb, rw = idx & (B.WPP - 1);
+        bits_row(B.sb[(size_t)(B.so + o) * B.WPP + rw],
+                 fb + (size_t)(B.z0 + o) * g.P + ((size_t)rw << 5));
+    }
+    if (b == 0 && tid == 0) {
+        const int fin = I.n_rounds & 1;
+        S.Eb[(size_t)fin * Rp + a] = E;
+        S.rung[(size_t)fin * Rp + a] = gr;
+        S.rep[(size_t)fin * Rp + gr] = a;
+        S.label[a] = lab;
+        pt_out_arr(I.out, R, PT_OUT_TRIPS)[a] = trips;
+        N.sbest[4 * (size_t)a] = bestE;
+        N.sbest[4 * (size_t)a + 1] = best_round;
+        N.sbest[4 * (size_t)a + 2] = best_rung;
+        N.sbest[4 * (size_t)a + 3] = 1;
+    }
+}
+
 template <int DIM, bool WRAP>
 const void* pslab_fn() {
     return reinterpret_cast<const void*>(&pslab_run<DIM, WRAP>);
@@ -1189,6 +1449,16 @@ template <int DIM, bool WRAP>
 const void* pslab_bits_fn(bool ebar) {
     return ebar ? reinterpret_cast<const void*>(&pslab_bits_run<DIM, WRAP, true>)
                 : reinterpret_cast<const void*>(&pslab_bits_run<DIM, WRAP, false>);
+}
+
+template <int DIM, bool WRAP>
+const void* pslab_nbr_fn() {
+    return reinterpret_cast<const void*>(&pslab_bits_nbr<DIM, WRAP>);
+}
+
+const void* pslab_nbr_pick(int dim, bool wrap) {
+    if (dim == 3) return wrap ? pslab_nbr_fn<3, true>() : pslab_nbr_fn<3, false>();
+    return wrap ? pslab_nbr_fn<2, true>() : pslab_nbr_fn<2, false>();
 }
 
 const void* pslab_pick(int dim, bool wrap, bool bits, bool ebar = false) {
@@ -1262,12 +1532,14 @@ size_t pt_pslab_lds_bytes(int L, int dim, int NB, bool upre) {
     return ((words * 4 + 15) & ~(size_t)15) + slots * segs * 16;
 }
 
-// The smaller occupancy of the two energy-path variants, so either can be launched on a grid
-// this says is co-resident.
+// The smallest occupancy of the kernels a run may take (the two energy-path variants and, for bit
+// slabs, the neighbour-synchronised kernel), so any of them can be launched on a grid this says is
+// co-resident.
 hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu) {
     *per_cu = 0;
-    for (int v = 0; v < 2; ++v) {
-        const void* f = pslab_pick(dim, NB == 1, pt_pslab_bits(L), v == 1);
+    const int nv = pt_pslab_bits(L) ? 3 : 2;
+    for (int v = 0; v < nv; ++v) {
+        const void* f = v == 2 ? pslab_nbr_pick(dim, NB == 1) : pslab_pick(dim, NB == 1, pt_pslab_bits(L), v == 1);
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         int k = 0;
@@ -1287,6 +1559,22 @@ hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs&
     CoopArgs c = C;
     void* args[] = {(void*)&i, (void*)&st, (void*)&c};
     const void* f = pslab_pick(I.ldim, S.NB == 1, pt_pslab_bits(I.lL), S.ebar != 0);
+    const dim3 grid(pt_pslab_grid(I, S));
+    return cooperative ? hipLaunchCooperativeKernel(f, grid, dim3(block), args, (unsigned)lds, s)
+                       : hipLaunchKernel(f, grid, dim3(block), args, lds, s);
+}
+
+hipError_t pt_launch_pslab_nbr(const PtInstance& I, const SState& S, const CoopArgs& C,
+                               const NbrArgs& N, int block, size_t lds, bool cooperative,
+                               hipStream_t s) {
+    PtInstance i = I;
+    SState st = S;
+    CoopArgs c = C;
+    NbrArgs nb = N;
+    void* args[] = {(void*)&i, (void*)&st, (void*)&c, (void*)&nb};
+    const void* f = pslab_nbr_pick(I.ldim, S.NB == 1);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
     const dim3 grid(pt_pslab_grid(I, S));
     return cooperative ? hipLaunchCooperativeKernel(f, grid, dim3(block), args, (unsigned)lds, s)
                        : hipLaunchKernel(f, grid, dim3(block), args, lds, s);
