@@ -1637,11 +1637,11 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     if (nbr) {
         const size_t nbox = (size_t)NBR_ROUNDS * H.Rp * S.NB;
         PT_HIP(A.alloc(&N.box, nbox));
-        PT_HIP(A.alloc(&N.hflag, (size_t)H.Rp * S.NB));
+        PT_HIP(A.alloc(&N.halo, 2 * (size_t)H.Rp * H.n / 32));
         PT_HIP(A.alloc(&N.sbest, 4 * (size_t)H.Rp));
         PT_HIP(A.alloc(&N.bestbits, (size_t)H.Rp * H.n / 32));
         PT_HIP(hipMemsetAsync(N.box, 0xFF, sizeof(uint64_t) * nbox, sg.s));
-        PT_HIP(hipMemsetAsync(N.hflag, 0, sizeof(uint32_t) * H.Rp * S.NB, sg.s));
+        PT_HIP(hipMemsetAsync(N.halo, 0xFF, sizeof(uint64_t) * 2 * H.Rp * H.n / 32, sg.s));
         // rung and pair counters are added into
         PT_HIP(hipMemsetAsync(I.out, 0, sizeof(int64_t) * pt_out_len(H.R), sg.s));
     }

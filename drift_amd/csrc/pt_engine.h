@@ -148,12 +148,14 @@ bool pt_pslab_bits(int L);
 size_t pt_pslab_lds_bytes(int L, int dim, int NB, bool upre = false);
 hipError_t pt_pslab_occupancy(int dim, int NB, int L, int block, size_t lds, int* per_cu);
 // Neighbour-synchronised bit-sliced persistent engine (no grid barrier): round-tagged energy
-// mailboxes by rung, NBR_ROUNDS rounds deep, halo flags by workgroup, per-slot best records.
+// mailboxes by rung, NBR_ROUNDS rounds deep, sweep-tagged halo words, per-slot best records.
 enum : int { NBR_ROUNDS = 8 };
 struct NbrArgs {
     uint64_t* box;        // [NBR_ROUNDS][Rp][NB]: round tag over a slab's energy word (all bits
                           // set at the start: a tag no round of the first NBR_ROUNDS carries)
-    uint32_t* hflag;      // [Rp][NB]: sweeps whose halo planes each workgroup has published (0)
+    uint64_t* halo;       // [sweep parity][Rp][plane][word]: sweep count over the spin bits of
+                          // each plane a workgroup hands to its neighbours (all bits set at the
+                          // start)
     int64_t* sbest;       // [Rp][4]: each slot's best energy, its round, the rung it held, 1
     uint32_t* bestbits;   // [Rp][plane][word]: each slot's best state as spin bits (1 = -1)
 };

@@ -891,6 +891,42 @@ __device__ __forceinline__ void bit_load_halos(const BitSlab& B, const uint32_t*
     }
 }
 
+// bit_publish with each word tagged by the sweep count tag (high half): the reader polls the data
+// itself, so no flag has to follow the stores.
+__device__ __forceinline__ void bit_publish_tagged(const BitSlab& B, uint64_t* dst, uint32_t tag) {
+    const int nb = B.T < 2 ? B.T : 2;
+    for (int idx = threadIdx.x; idx < 2 * nb * B.WPP; idx += blockDim.x) {
+        const int kk = idx >> B.wppb, w = idx & (B.WPP - 1);
+        const int o = kk < nb ? kk : B.T - 2 * nb + kk;
+        st_u64(dst + (size_t)(B.z0 + o) * B.WPP + w,
+               ((uint64_t)tag << 32) | B.sb[(size_t)(B.so + o) * B.WPP + w]);
+    }
+}
+
+// The four halo slots from the tagged words of src once each carries tag (one wave: entries
+// lane, lane + 64, ...).  False when a wait exceeded its bound.
+__device__ __forceinline__ bool bit_wait_halos(const BitSlab& B, const CoopArgs& C,
+                                               const uint64_t* src, uint32_t tag, int lane) {
+    for (int idx = lane; idx < 4 * B.WPP; idx += PT_WAVE) {
+        const int kk = idx >> B.wppb, w = idx & (B.WPP - 1);
+        const int sl = kk < 2 ? kk : B.T + kk;
+        const int plane = (B.z0 - 2 + sl) & (B.L - 1);
+        const uint64_t* p = src + (size_t)plane * B.WPP + w;
+        const unsigned long long t0 = (unsigned long long)wall_clock64();
+        uint64_t v;
+        while ((uint32_t)((v = ld_u64(p)) >> 32) != tag) {
+            if (__hip_atomic_load(C.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                (unsigned long long)wall_clock64() - t0 > C.timeout_ticks) {
+                __hip_atomic_store(C.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        B.sb[(size_t)sl * B.WPP + w] = (uint32_t)v;
+    }
+    return true;
+}
+
 // At most 512 threads: the next-sweep uniform path needs more than 128 registers per lane.
 // EBAR: the energy path of the barrier (SState::ebar), fixed per instantiation.
 template <int DIM, bool WRAP, bool EBAR>
@@ -1235,8 +1271,10 @@ __global__ __launch_bounds__(512) void pslab_bits_run(PtInstance I, SState S, Co
 // ---------------------------------------------------------------------------------------------
 // Bit-sliced persistent slab engine with neighbour-scoped synchronisation.  The same sweeps as
 // pslab_bits_run, but no workgroup waits for the whole grid:
-//  * halos: before sweep t a workgroup waits only for the workgroups owning its four halo planes
-//    (the adjacent slabs of its own slot) to have published sweep t - 1 (NbrArgs::hflag);
+//  * halos: a workgroup publishes the planes its neighbours need as words tagged with the sweep
+//    count (NbrArgs::halo) and, before sweep t, polls the words of its four halo planes -- the
+//    adjacent slabs of its own slot -- until they carry sweep t; one wave does this while wave 0
+//    takes the round end and the others draw the next sweep's uniforms;
 //  * round end: each workgroup posts its slab's energy change, tagged with the round, into the
 //    mailbox of its slot's rung (NbrArgs::box; slab 0 adds the slot's energy of the last round
 //    end, so the words of a rung sum to that slot's new energy).  Exchange partners read the two
@@ -1267,8 +1305,12 @@ __global__ __launch_bounds__(512) void pslab_bits_nbr(PtInstance I, SState S, Co
     const BitSlab B = make_bit_slab<DIM, WRAP>(g, S.NB, b, lw);
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (PT_WAVE - 1);
     const int R = I.R, Rp = I.Rp, NB = S.NB;
-    uint32_t* hb = S.hbits;                               // [parity][slot][plane][word]
-    const size_t hrep = (size_t)g.L * B.WPP;              // words per replica and parity
+    const size_t hrep = (size_t)g.L * B.WPP;              // words per replica
+    // this slot's tagged plane words, two buffers by sweep parity: a neighbour can finish sweep
+    // t + 1 (it needs only this workgroup's planes of sweep t) before this workgroup has read its
+    // planes of sweep t, but not sweep t + 2
+    uint64_t* halo = N.halo + (size_t)a * hrep;
+    const size_t hpar = (size_t)Rp * hrep;
 
     bit_stage<DIM, WRAP>(B, I, S, g, a);
     for (int x = tid; x < R; x += nt) {
@@ -1285,8 +1327,11 @@ __global__ __launch_bounds__(512) void pslab_bits_nbr(PtInstance I, SState S, Co
         sh_dsum = 0;
     }
     const uint32_t nsweeps = (uint32_t)I.n_rounds * (uint32_t)I.S;
-    // waves 1.. draw the next sweep's uniforms while wave 0 waits for the mailboxes
+    // wave 0 takes the round end, wave 1 (wave 0 when alone) the halo planes, the waves after
+    // (wave 1 too when there are two) draw the next sweep's uniforms
     const bool pre = S.upre != 0 && nt > PT_WAVE;
+    const int hwave = nt > PT_WAVE ? 1 : 0;
+    const int dfirst = nt > 2 * PT_WAVE ? 2 * PT_WAVE : PT_WAVE;
     if (pre && nsweeps > 0) bit_draw<DIM, WRAP>(B, I, g, a, 0u, tid, nt);
     // the slot's ladder state, the same in every workgroup of the slot (wave 0's copy is used)
     int64_t E = -(S.einit[a] / 2);
@@ -1295,14 +1340,6 @@ __global__ __launch_bounds__(512) void pslab_bits_nbr(PtInstance I, SState S, Co
     int64_t bestE = E;
     int best_round = 0, best_rung = a;
     uint32_t* mybest = N.bestbits + (size_t)a * hrep;
-    // halo slot k (0, 1, T + 2, T + 3): plane and the slab that owns it
-    int howner = 0;
-    if (!WRAP && lane < 4) {
-        const int sl = lane < 2 ? lane : B.T + lane;
-        const int plane = (B.z0 - 2 + sl) & (g.L - 1);
-        howner = ((plane + 1) * NB - 1) / g.L;
-    }
-    const int hw0 = nt > PT_WAVE ? PT_WAVE : 0;           // the lanes that wait for halos
     __syncthreads();
     // the initial state is every slot's first best (round 0)
     for (int idx = tid; idx < B.T * B.WPP; idx += nt)
@@ -1315,20 +1352,16 @@ __global__ __launch_bounds__(512) void pslab_bits_nbr(PtInstance I, SState S, Co
         for (int j = 0; j < I.S; ++j) {
             const uint32_t t = (uint32_t)rnd * (uint32_t)I.S + (uint32_t)j;
             const bool last = j == I.S - 1, more = t + 1 < nsweeps;
-            uint32_t* nxt = hb + ((size_t)((t + 1) & 1) * Rp + a) * hrep;
             dsum += bit_sweep<DIM, WRAP>(B, I, g, a, t, pre, t2, t4, t6);
-            if (!WRAP && more) bit_publish(B, nxt);
+            if (!WRAP && more) bit_publish_tagged(B, halo + ((t + 1) & 1) * hpar, t + 1);
             if (last) {
                 long long v = (long long)dsum;
 #pragma unroll
                 for (int m = PT_WAVE / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
                 if (lane == 0 && v != 0) atomicAdd((unsigned long long*)&sh_dsum, (unsigned long long)v);
                 dsum = 0;
+                __syncthreads();
             }
-            // every wave's halo stores complete before the flag that releases them
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
-            if (!WRAP && more && tid == 0) st_u32(N.hflag + (size_t)a * NB + b, t + 1);
             if (tid < PT_WAVE && last) {
                 // ---- round end: this slot and its exchange partner, from two rungs' mailboxes
                 uint64_t* box = N.box + (size_t)(rnd & (NBR_ROUNDS - 1)) * Rp * NB;
@@ -1403,19 +1436,19 @@ __global__ __launch_bounds__(512) void pslab_bits_nbr(PtInstance I, SState S, Co
                         sh_dsum = 0;
                     }
                 }
-            } else if (tid >= PT_WAVE && pre && more) {
-                bit_draw<DIM, WRAP>(B, I, g, a, t + 1, tid - PT_WAVE, nt - PT_WAVE);
             }
-            // the halo planes of sweep t + 1: their owners' flags reach t + 1
-            if (!WRAP && more && tid >= hw0 && tid < hw0 + 4)
-                if (!wait_phase(C, N.hflag + (size_t)a * NB + howner, t + 1)) sh_abort = 1;
+            // the halo planes of sweep t + 1, as their owners publish them
+            if (!WRAP && more && tid / PT_WAVE == hwave)
+                if (!bit_wait_halos(B, C, halo + ((t + 1) & 1) * hpar, t + 1, lane)) sh_abort = 1;
+            if (pre && more && tid >= dfirst)
+                bit_draw<DIM, WRAP>(B, I, g, a, t + 1, tid - dfirst, nt - dfirst);
             __syncthreads();
             if (sh_abort) return;
-            if (!WRAP && more) bit_load_halos(B, nxt, tid, nt);
-            if (last && sh_imp)   // the owned planes after this round's sweeps: the slot's new best
+            if (last && sh_imp) {   // the owned planes after this round's sweeps: the slot's new best
                 for (int idx = tid; idx < B.T * B.WPP; idx += nt)
                     mybest[(size_t)B.z0 * B.WPP + idx] = B.sb[(size_t)B.so * B.WPP + idx];
-            __syncthreads();
+                __syncthreads();
+            }
         }
     }
 
