@@ -105,7 +105,9 @@ def lattice_side(n: int) -> int:
 def kernel_name(engine: str, model, R: int) -> str:
     return {"global": "g_color_bits" if R <= 32 else "g_color_step",
             "lattice": "g_lattice_step", "slab": "slab_sweep", "coop": "coop_run",
-            "pslab": "pslab_bits_run" if lattice_side(model.n) % 32 == 0 else "pslab_run",
+            # bit slabs (side a multiple of 32) run without a grid barrier when no per-round history
+            # or trace is recorded, as here
+            "pslab": "pslab_bits_nbr" if lattice_side(model.n) % 32 == 0 else "pslab_run",
             }.get(engine, "pt_lds_kernel")
 
 
@@ -159,7 +161,7 @@ def declared_bytes(model, enc, engine: str, R: int) -> tuple[float, str]:
 def lds_bytes_per_attempt(kernel: str, model) -> float | None:
     """LDS bytes one attempt reads and writes in the kernels whose state lives in LDS."""
     d = model.to_csr()[1].size / model.n
-    if kernel == "pslab_bits_run":
+    if kernel in ("pslab_bits_run", "pslab_bits_nbr"):
         # per 32-site row word (16 sites of the colour): the word, its two row neighbours, 2 (d-2)
         # other neighbour words, d bond words, four 16-byte uniform blocks, one store
         words = 3 + (d - 2) + d
