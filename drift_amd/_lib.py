@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64,
 LIB_NAME = "libdriftpt.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # enum drift_pt_status
 OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
@@ -23,16 +23,17 @@ W_I8, W_I32, W_F32 = 1, 2, 3
 # enum drift_pt_engine
 ENGINE_AUTO, ENGINE_LDS, ENGINE_GLOBAL, ENGINE_LATTICE, ENGINE_SLAB, ENGINE_COOP = 0, 1, 2, 3, 4, 5
 ENGINE_PSLAB = 6
+ENGINE_SPLIT = 7
 ENGINES = {"auto": ENGINE_AUTO, "lds": ENGINE_LDS, "global": ENGINE_GLOBAL,
            "lattice": ENGINE_LATTICE, "slab": ENGINE_SLAB, "coop": ENGINE_COOP,
-           "pslab": ENGINE_PSLAB}
+           "pslab": ENGINE_PSLAB, "split": ENGINE_SPLIT}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 # enum drift_pt_flags
 RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GRAPH = 1, 2, 4, 8, 16, 32
 AUTOTUNE = 64
 PLAIN_LAUNCH = 128
 STOP_AT_TARGET = 256
-TUNE_MAX = 16
+TUNE_MAX = 48
 NOALLOC_BYTES = 64 << 20   # DRIFT_PT_NOALLOC_BYTES
 
 
@@ -65,6 +66,7 @@ class Params(ctypes.Structure):
         ("flags", c_int32),
         ("block_threads", c_int32),
         ("target_energy", c_double),
+        ("slots_per_group", c_int32),
     ]
 
 
@@ -95,6 +97,8 @@ class Result(ctypes.Structure):
         ("other_kernel_seconds", c_double),
         ("other_kernel_launches", c_int64),
         ("rounds_run", c_int32),
+        ("block_threads_used", c_int32),
+        ("slots_per_group_used", c_int32),
     ]
 
 
@@ -104,10 +108,12 @@ class Tuning(ctypes.Structure):
     _fields_ = [
         ("engine", c_int32),
         ("block_threads", c_int32),
+        ("slots_per_group", c_int32),
         ("trial_rounds", c_int32),
         ("n_candidates", c_int32),
         ("cand_engine", c_int32 * TUNE_MAX),
         ("cand_block", c_int32 * TUNE_MAX),
+        ("cand_group", c_int32 * TUNE_MAX),
         ("cand_seconds", c_double * TUNE_MAX),
     ]
 

@@ -143,6 +143,8 @@ struct HostInst {
                                     // by a timed trial (energy_path_choice)
     int has_target = 0;             // DRIFT_PT_STOP_AT_TARGET: stop at best <= target_units
     int64_t target_units = 0;
+    int rg_req = 0;                 // drift_pt_params::slots_per_group (0: the library chooses)
+    int block_req = 0;              // drift_pt_params::block_threads (0: the library chooses)
 };
 
 // First-chunk width of the LDS engine's quad gather: the smallest of 2, 3, 4 that covers at least
@@ -207,6 +209,9 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
         return fail(DRIFT_PT_EINVAL, "n_rounds and sweeps_per_round must be >= 0");
     if ((int64_t)prm->n_rounds * std::max(prm->sweeps_per_round, 1) >= ((int64_t)1 << 32))
         return fail(DRIFT_PT_ERANGE, "more than 2^32 sweeps");
+    if (prm->slots_per_group < 0 || prm->slots_per_group > 64 ||
+        (prm->slots_per_group & (prm->slots_per_group - 1)) != 0)
+        return fail(DRIFT_PT_EINVAL, "slots_per_group must be 0 or a power of two up to 64");
     const int32_t* rp = P->row_ptr;
     const int32_t* ci = P->col_idx;
     if (rp[0] != 0 || rp[n] != nnz) return fail(DRIFT_PT_EINVAL, "row_ptr must run 0..nnz");
@@ -282,6 +287,8 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     H.n = n; H.nnz = nnz; H.k = k; H.R = R; H.Rp = padded_slots(R);
     H.wm = P->weight_type; H.S = prm->sweeps_per_round; H.n_rounds = prm->n_rounds;
     H.flags = prm->flags; H.seed = prm->seed; H.has_h = P->h != nullptr;
+    H.rg_req = prm->slots_per_group;
+    H.block_req = prm->block_threads;
     H.color_ptr = cptr;
     H.orig = cspins;
     H.row_ptr.assign((size_t)n + 1, 0);
@@ -535,9 +542,14 @@ std::vector<T> pad4(const std::vector<T>& v) {
     return out;
 }
 
+// rgs (split engine): slots per workgroup of each instance; the first-chunk records then hold
+// neighbour offsets for groups of that many slots, and img holds one image per group, and each
+// instance gets its energy mailboxes (all bits set).
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
-                bool force_final, bool with_ell) {
+                bool force_final, bool with_ell, const std::vector<int>* rgs = nullptr) {
     const size_t m = hs.size();
+    auto rg_of = [&](size_t i) { return rgs ? (*rgs)[i] : hs[i].Rp; };
+    auto ng_of = [&](size_t i) { return rgs ? (hs[i].R + (*rgs)[i] - 1) / (*rgs)[i] : 1; };
     struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb; };
     std::vector<Off> off(m);
     Packer in;
@@ -546,7 +558,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         Off& o = off[i];
         o.rp = in.add(H.row_ptr);
         o.edge = in.add(H.edge);
-        o.ell = with_ell ? in.add(build_ell(H, H.gq, (uint32_t)H.Rp)) : SIZE_MAX;
+        o.ell = with_ell ? in.add(build_ell(H, H.gq, (uint32_t)rg_of(i))) : SIZE_MAX;
         o.cptr = in.add(H.color_ptr);
         o.orig = in.add(H.orig);
         o.h = !H.has_h ? SIZE_MAX
@@ -557,7 +569,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         o.jb = H.ldim ? in.add(H.jbits) : SIZE_MAX;
     }
     // outputs after the inputs
-    struct OutOff { size_t best, fin, hist, trace, out, img, e0; };
+    struct OutOff { size_t best, fin, hist, trace, out, img, e0, box; };
     std::vector<OutOff> oo(m);
     size_t cursor = (in.blob.size() + 255) & ~(size_t)255;
     const size_t out_begin = cursor;
@@ -571,6 +583,9 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         oo[i].out = take(sizeof(int64_t) * (size_t)pt_out_len(H.R));
     }
     const size_t out_end = cursor;   // statistics blocks are the region that must start at zero
+    for (size_t i = 0; i < m; ++i)
+        oo[i].box = rgs ? take(sizeof(uint64_t) * 4 * (size_t)hs[i].Rp) : SIZE_MAX;
+    const size_t box_end = cursor;   // mailboxes start with every bit set
     for (size_t i = 0; i < m; ++i) {
         const HostInst& H = hs[i];
         oo[i].best = take((size_t)H.n);
@@ -580,13 +595,14 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
                          ? take(sizeof(double) * (size_t)H.n_rounds) : SIZE_MAX;
         oo[i].trace = ((H.flags & DRIFT_PT_RECORD_TRACE) && H.n_rounds > 0)
                           ? take(sizeof(double) * (size_t)H.n_rounds * H.R) : SIZE_MAX;
-        oo[i].img = with_ell ? take(pt_lds_img_bytes(H.n, H.Rp)) : SIZE_MAX;
+        oo[i].img = with_ell ? take(pt_lds_img_bytes(H.n, rg_of(i)) * (size_t)ng_of(i)) : SIZE_MAX;
         oo[i].e0 = with_ell ? take(sizeof(int64_t) * (size_t)H.Rp) : SIZE_MAX;
     }
     uint8_t* base;
     PT_HIP(A.alloc(&base, cursor));
     PT_HIP(hipMemcpy(base, in.blob.data(), in.blob.size(), hipMemcpyHostToDevice));
     PT_HIP(hipMemset(base + out_begin, 0, out_end - out_begin));
+    if (box_end > out_end) PT_HIP(hipMemset(base + out_end, 0xFF, box_end - out_end));
     insts.assign(m, PtInstance{});
     for (size_t i = 0; i < m; ++i) {
         const HostInst& H = hs[i];
@@ -618,6 +634,9 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.ell_rest = H.maxdeg > H.gq ? 1 : 0;
         I.has_target = H.has_target;
         I.target_units = H.target_units;
+        I.box = oo[i].box == SIZE_MAX ? nullptr : (uint64_t*)(base + oo[i].box);
+        I.rg = rg_of(i);
+        I.ng = ng_of(i);
     }
     return 0;
 }
@@ -637,6 +656,7 @@ void clear_result(drift_pt_result* r) {
     r->engine_used = 0; r->seconds = 0.0; r->attempts = 0.0;
     r->kernel_seconds = 0.0; r->kernel_attempts = 0.0; r->kernel_launches = 0;
     r->other_kernel_seconds = 0.0; r->other_kernel_launches = 0; r->rounds_run = 0;
+    r->block_threads_used = 0; r->slots_per_group_used = 0;
 }
 
 // Copy results of one instance back and undo the colour relabelling.
@@ -812,7 +832,186 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
         if (rc) return rc;
         results[i]->seconds = ms * 1e-3;
         results[i]->engine_used = DRIFT_PT_ENGINE_LDS;
+        results[i]->block_threads_used = block;
+        results[i]->slots_per_group_used = H.Rp;
         results[i]->kernel_seconds = ms * 1e-3;      // one persistent launch does everything
+        results[i]->kernel_launches = 1;
+        results[i]->kernel_attempts = total;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// split engine: an instance's replica slots in groups, one workgroup's LDS each
+// ------------------------------------------------------------------------------------------
+int device_cus() {
+    static int cached = -1;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cached < 0) {
+        hipDeviceProp_t prop;
+        int dev = 0;
+        cached = 256;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cached = std::max(1, prop.multiProcessorCount);
+    }
+    return cached;
+}
+
+// Largest power-of-two number of slots per workgroup (at most 64 and Rp) whose spins and ladder
+// arrays fit one workgroup's LDS; 0 when not even one slot fits.
+int split_rg_max(const HostInst& H) {
+    for (int rg = std::min(64, H.Rp); rg >= 1; rg >>= 1)
+        if (pt_split_lds_bytes(H.n, rg, H.Rp) + 1024 <= (size_t)device_lds_limit()) return rg;
+    return 0;
+}
+
+bool split_fits(const HostInst& H) { return split_rg_max(H) >= 1; }
+
+// Slots per workgroup for a set of instances run together: the requested size where one is
+// given, else the largest common cap c (a power of two) under which the groups number at least
+// two per compute unit, each instance taking min(c, its largest fitting size); c = 1 when no cap
+// reaches that.  Larger groups share each position's edge records among more lanes; more groups
+// keep more compute units busy.
+std::vector<int> split_rgs(const std::vector<HostInst>& hs) {
+    std::vector<int> out(hs.size(), 1);
+    const int64_t want = 2 * (int64_t)device_cus();
+    int cap = 1;
+    for (int c = 64; c >= 1; c >>= 1) {
+        int64_t groups = 0;
+        for (const HostInst& H : hs) {
+            const int rg = H.rg_req > 0 ? H.rg_req : std::min(c, split_rg_max(H));
+            groups += (H.R + rg - 1) / rg;
+        }
+        if (groups >= want) {
+            cap = c;
+            break;
+        }
+    }
+    for (size_t i = 0; i < hs.size(); ++i)
+        out[i] = hs[i].rg_req > 0 ? hs[i].rg_req : std::max(1, std::min(cap, split_rg_max(hs[i])));
+    return out;
+}
+
+// Workgroup size of the split engine: the requested one, else by the LDS a workgroup takes, so
+// the workgroups a compute unit holds fill its 32 wave slots (1024 threads when at most two fit,
+// down to 256 when eight or more do).  On 64 3-regular instances of N = 65536 with R = 32,
+// 1024-thread workgroups ran at 442 attempts/ns against 351 at 256 threads.
+int split_block(size_t lds, int block_req) {
+    if (block_req > 0) return std::min(1024, std::max(64, (block_req + 63) / 64 * 64));
+    const int per_cu = std::max(1, (int)((size_t)device_lds_limit() / (lds + 1024)));
+    int b = 2048 / per_cu;
+    b = std::max(256, std::min(1024, b / 64 * 64));
+    return b;
+}
+
+// Instances on the split engine, all of one weight type: one launch draws every group's initial
+// state, then one persistent launch (timed) runs every round of every instance.
+int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& results,
+              int block_req) {
+    DevArena A;
+    const int n_inst = (int)hs.size();
+    const std::vector<int> rgs = split_rgs(hs);
+    size_t lds = 0;
+    int max_ng = 1;
+    bool plain = false;
+    for (int i = 0; i < n_inst; ++i) {
+        const HostInst& H = hs[i];
+        const int rg = rgs[i];
+        if (rg < 1 || rg > H.Rp || rg > split_rg_max(H))
+            return fail(DRIFT_PT_EINVAL, "split engine: " + std::to_string(rg) +
+                                             " slots per workgroup do not fit one workgroup's LDS "
+                                             "(or exceed the padded replica count)");
+        lds = std::max(lds, pt_split_lds_bytes(H.n, rg, H.Rp));
+        max_ng = std::max(max_ng, (H.R + rg - 1) / rg);
+        if (H.flags & DRIFT_PT_PLAIN_LAUNCH) plain = true;
+        if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "split engine: one weight type per launch");
+    }
+    const int block = split_block(lds, block_req);
+    const int wm = hs[0].wm;
+    int per_cu = 0;
+    PT_HIP(pt_split_occupancy(wm, block, lds, &per_cu));
+    const int capacity = per_cu * device_cus();
+    if (capacity < max_ng)
+        return fail(DRIFT_PT_EINVAL, "split engine: the " + std::to_string(max_ng) +
+                                         " groups of an instance cannot all be resident (room for " +
+                                         std::to_string(capacity) + " workgroups of " +
+                                         std::to_string(block) + " threads)");
+    std::vector<PtInstance> insts((size_t)n_inst);
+    int rc = upload_many(A, hs, insts, false, true, &rgs);
+    if (rc) return rc;
+    // largest work first; the groups of one instance consecutive
+    std::vector<int32_t> order((size_t)n_inst);
+    std::iota(order.begin(), order.end(), 0);
+    auto work = [&](int i) {
+        const HostInst& H = hs[i];
+        return (double)H.n_rounds * std::max(H.S, 1) * (H.n + H.nnz) * rgs[i];
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::vector<int32_t> items;
+    for (int i : order)
+        for (int g = 0; g < insts[i].ng; ++g) items.push_back((int32_t)((i << 10) | g));
+    const int n_items = (int)items.size();
+    PtInstance* d_insts;
+    int32_t *d_items, *d_queue, *d_abort;
+    PT_HIP(A.upload(&d_insts, insts));
+    PT_HIP(A.upload(&d_items, items));
+    PT_HIP(A.alloc(&d_queue, 2));
+    PT_HIP(A.alloc(&d_abort, 1));
+    SplitArgs SA{d_abort, barrier_timeout_ticks()};
+    const int grid = std::min(n_items, capacity);
+    StreamGuard sg;
+    PT_HIP(hipStreamCreate(&sg.s));
+    PT_HIP(hipEventCreate(&sg.e0));
+    PT_HIP(hipEventCreate(&sg.e1));
+    PT_HIP(hipMemsetAsync(d_queue, 0, 2 * sizeof(int32_t), sg.s));
+    PT_HIP(hipMemsetAsync(d_abort, 0, sizeof(int32_t), sg.s));
+    PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue, wm, block, grid, lds, true, SA,
+                           false, sg.s));
+    PT_HIP(hipEventRecord(sg.e0, sg.s));
+    PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue + 1, wm, block, grid, lds, false, SA,
+                           !plain, sg.s));
+    PT_HIP(hipEventRecord(sg.e1, sg.s));
+    PT_HIP(hipStreamSynchronize(sg.s));
+    float ms = 0.0f;
+    PT_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
+    int32_t aborted = 0;
+    PT_HIP(hipMemcpy(&aborted, d_abort, sizeof(aborted), hipMemcpyDeviceToHost));
+    if (aborted) return fail(DRIFT_PT_EHIP, "split engine: a mailbox wait exceeded its bound");
+    // every instance's outputs in one copy, as on the LDS engine
+    const uint8_t *lo = nullptr, *hi = nullptr;
+    auto span = [&](const void* p, size_t bytes) {
+        if (!p || !bytes) return;
+        const uint8_t* a = (const uint8_t*)p;
+        if (!lo || a < lo) lo = a;
+        if (!hi || a + bytes > hi) hi = a + bytes;
+    };
+    for (int i = 0; i < n_inst; ++i) {
+        const HostInst& H = hs[i];
+        const PtInstance& I = insts[i];
+        span(I.out, sizeof(int64_t) * (size_t)pt_out_len(H.R));
+        span(I.best_s, (size_t)H.n);
+        if (H.flags & DRIFT_PT_RECORD_FINAL) span(I.final_s, (size_t)H.n * H.R);
+        if (H.flags & DRIFT_PT_RECORD_HISTORY) span(I.hist, sizeof(double) * (size_t)H.n_rounds);
+        if (H.flags & DRIFT_PT_RECORD_TRACE)
+            span(I.trace, sizeof(double) * (size_t)H.n_rounds * H.R);
+    }
+    std::vector<uint8_t> mirror(lo ? (size_t)(hi - lo) : 0);
+    if (lo) PT_HIP(hipMemcpy(mirror.data(), lo, mirror.size(), hipMemcpyDeviceToHost));
+    double total = 0.0;
+    for (const HostInst& Hx : hs) total += (double)Hx.R * Hx.n_rounds * Hx.S * Hx.n;
+    for (int i = 0; i < n_inst; ++i) {
+        const HostInst& H = hs[i];
+        std::vector<int64_t> out((size_t)pt_out_len(H.R));
+        std::memcpy(out.data(), mirror.data() + ((const uint8_t*)insts[i].out - lo),
+                    out.size() * sizeof(int64_t));
+        rc = fetch_results(H, insts[i], out, {}, results[i], mirror.data(), lo);
+        if (rc) return rc;
+        results[i]->seconds = ms * 1e-3;
+        results[i]->engine_used = DRIFT_PT_ENGINE_SPLIT;
+        results[i]->block_threads_used = block;
+        results[i]->slots_per_group_used = rgs[i];
+        results[i]->kernel_seconds = ms * 1e-3;
         results[i]->kernel_launches = 1;
         results[i]->kernel_attempts = total;
     }
@@ -1077,6 +1276,7 @@ struct GlobalRun {
         if (rc) return rc;
         res->seconds = ms * 1e-3;
         res->engine_used = lattice ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+        res->block_threads_used = block;
         if (time_k) {
             res->kernel_seconds = kernel_s;
             res->kernel_attempts = timed_attempts;
@@ -1287,6 +1487,7 @@ int run_coop(const HostInst& H, drift_pt_result* res, CoopPlan P) {
     if (rc) return rc;
     res->seconds = ms * 1e-3;
     res->engine_used = DRIFT_PT_ENGINE_COOP;
+    res->block_threads_used = P.block;
     res->kernel_seconds = res->seconds;
     res->kernel_attempts = res->attempts;
     res->kernel_launches = 1;
@@ -1437,6 +1638,7 @@ int run_slab(const HostInst& H, drift_pt_result* res, int block_req) {
     if (rc) return rc;
     res->seconds = ms * 1e-3;
     res->engine_used = DRIFT_PT_ENGINE_SLAB;
+    res->block_threads_used = block;
     res->kernel_seconds = res->seconds;   // one sweep kernel per sweep carries all the work
     res->kernel_attempts = res->attempts;
     res->kernel_launches = (int64_t)H.n_rounds * per_round + 1;
@@ -1664,6 +1866,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
     if (rc) return rc;
     res->seconds = ms * 1e-3;
     res->engine_used = DRIFT_PT_ENGINE_PSLAB;
+    res->block_threads_used = block;
     res->kernel_seconds = res->seconds;
     res->kernel_attempts = res->attempts;
     res->kernel_launches = 1;
@@ -1678,9 +1881,10 @@ bool fits_lds(const HostInst& H) {
 // Run one prepared instance on a resolved engine (not AUTO).
 int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
     if (H.has_target && engine != DRIFT_PT_ENGINE_LDS && engine != DRIFT_PT_ENGINE_GLOBAL &&
-        engine != DRIFT_PT_ENGINE_LATTICE)
-        return fail(DRIFT_PT_EINVAL, "DRIFT_PT_STOP_AT_TARGET needs the lds, global or lattice "
-                                     "engine (the persistent engines run every round in one launch)");
+        engine != DRIFT_PT_ENGINE_LATTICE && engine != DRIFT_PT_ENGINE_SPLIT)
+        return fail(DRIFT_PT_EINVAL, "DRIFT_PT_STOP_AT_TARGET needs the lds, split, global or "
+                                     "lattice engine (the other persistent engines run every round "
+                                     "in one launch)");
     switch (engine) {
     case DRIFT_PT_ENGINE_SLAB:
         if (!slab_fits(H))
@@ -1712,6 +1916,13 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
     }
     case DRIFT_PT_ENGINE_GLOBAL:
         return run_global(H, res, block, false);
+    case DRIFT_PT_ENGINE_SPLIT: {
+        if (!split_fits(H))
+            return fail(DRIFT_PT_EINVAL, "split engine: one replica's spins do not fit one "
+                                         "workgroup's LDS");
+        std::vector<HostInst> hs{H};
+        return run_split(hs, {res}, block);
+    }
     default:
         return fail(DRIFT_PT_EINVAL, "unknown engine");
     }
@@ -1743,7 +1954,7 @@ int default_engine(const HostInst& H) {
 using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
 
 struct TuneEntry {
-    int engine = 0, block = 0;
+    int engine = 0, block = 0, rg = 0;
 };
 
 std::mutex g_tune_mu;
@@ -1756,19 +1967,40 @@ TuneKey tune_key(const HostInst& H) {
     return TuneKey(H.wm, H.Rp, H.ldim, H.lL, nb, std::min(half_degree, 256), H.S, H.has_h ? 1 : 0);
 }
 
+// Timed trials of every engine and launch shape that can run H.  The first candidate is what
+// default_engine takes with its own launch shape (block 0); the choice stays there unless another
+// candidate is more than 3% faster.  Every candidate is timed in two passes, the second in the
+// opposite order, and keeps the faster of its two times, so drift of the device's clock over the
+// trials favours no candidate.
 int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry& choice) {
-    std::vector<std::pair<int, int>> cands;
+    struct Cand {
+        int engine, block, rg;
+        double secs;
+    };
+    std::vector<Cand> cands;
+    cands.push_back({default_engine(H), 0, 0, -1.0});
     const int blocks[3] = {256, 512, 1024};
     for (int b : blocks) {
-        if (fits_lds(H)) cands.emplace_back(DRIFT_PT_ENGINE_LDS, b);
-        if (H.ldim && slab_fits(H)) cands.emplace_back(DRIFT_PT_ENGINE_SLAB, b);
+        if (fits_lds(H)) cands.push_back({DRIFT_PT_ENGINE_LDS, b, 0, -1.0});
+        if (H.ldim && slab_fits(H)) cands.push_back({DRIFT_PT_ENGINE_SLAB, b, 0, -1.0});
         size_t plds = 0;
         if (H.ldim && pslab_fits(H) && pslab_block(H, b, plds) == b)
-            cands.emplace_back(DRIFT_PT_ENGINE_PSLAB, b);
-        if (H.ldim) cands.emplace_back(DRIFT_PT_ENGINE_LATTICE, b);
-        cands.emplace_back(DRIFT_PT_ENGINE_GLOBAL, b);
+            cands.push_back({DRIFT_PT_ENGINE_PSLAB, b, 0, -1.0});
+        if (H.ldim) cands.push_back({DRIFT_PT_ENGINE_LATTICE, b, 0, -1.0});
+        cands.push_back({DRIFT_PT_ENGINE_GLOBAL, b, 0, -1.0});
         CoopPlan P;
-        if (coop_plan(H, b, P)) cands.emplace_back(DRIFT_PT_ENGINE_COOP, b);
+        if (coop_plan(H, b, P)) cands.push_back({DRIFT_PT_ENGINE_COOP, b, 0, -1.0});
+        // the split engine, every group size whose groups can all be resident
+        for (int rg = split_rg_max(H); rg >= 1; rg >>= 1) {
+            const size_t lds = pt_split_lds_bytes(H.n, rg, H.Rp);
+            int per_cu = 0;
+            if (pt_split_occupancy(H.wm, b, lds, &per_cu) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            if ((int64_t)per_cu * device_cus() >= (H.R + rg - 1) / rg)
+                cands.push_back({DRIFT_PT_ENGINE_SPLIT, b, rg, -1.0});
+        }
     }
     const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
     const int rounds = trial_rounds > 0 ? trial_rounds
@@ -1779,50 +2011,56 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
     std::vector<int8_t> bs((size_t)std::max(H.n, 1));
     drift_pt_result r{};
     r.best_s = bs.data();
-    double best = -1.0;
-    if (table) {
-        std::memset(table, 0, sizeof(*table));
-        table->trial_rounds = rounds;
-    }
-    int slot = 0;
-    // Each candidate runs `rounds` and then 2 * `rounds` rounds; the difference is its time for
+    // Each trial runs `rounds` and then 2 * `rounds` rounds; the difference is the time of
     // `rounds` rounds with the fixed costs (staging, ramp, launch) cancelled, which is what a long
     // run pays.
-    for (const auto& c : cands) {
+    auto trial = [&](Cand& c) {
         double secs = -1.0;
+        T.rg_req = c.rg;
         T.n_rounds = 2;                                      // warm-up: code objects, first touch
         clear_result(&r);
-        if (run_engine(T, c.first, c.second, &r) == 0) {
+        if (run_engine(T, c.engine, c.block, &r) == 0) {
             T.n_rounds = rounds;
             clear_result(&r);
-            if (run_engine(T, c.first, c.second, &r) == 0) {
+            if (run_engine(T, c.engine, c.block, &r) == 0) {
                 const double t1 = r.seconds;
                 T.n_rounds = 2 * rounds;
                 clear_result(&r);
-                if (run_engine(T, c.first, c.second, &r) == 0)
+                if (run_engine(T, c.engine, c.block, &r) == 0)
                     secs = r.seconds - t1 > 0.0 ? r.seconds - t1 : 0.5 * r.seconds;
             }
         }
         (void)hipGetLastError();
-        if (secs > 0.0 && (best < 0.0 || secs < best)) {
-            best = secs;
-            choice.engine = c.first;
-            choice.block = c.second;
-        }
-        if (table && slot < DRIFT_PT_TUNE_MAX) {
-            table->cand_engine[slot] = c.first;
-            table->cand_block[slot] = c.second;
-            table->cand_seconds[slot] = secs;
+        if (secs > 0.0 && (c.secs < 0.0 || secs < c.secs)) c.secs = secs;
+    };
+    for (Cand& c : cands) trial(c);
+    for (size_t i = cands.size(); i-- > 0;) trial(cands[i]);
+    g_err.clear();
+    int best = -1;
+    for (size_t i = 0; i < cands.size(); ++i)
+        if (cands[i].secs > 0.0 && (best < 0 || cands[i].secs < cands[best].secs)) best = (int)i;
+    if (best < 0) return fail(DRIFT_PT_EHIP, "autotune: no candidate engine ran");
+    if (cands[0].secs > 0.0 && cands[best].secs > cands[0].secs / 1.03) best = 0;
+    choice.engine = cands[best].engine;
+    choice.block = cands[best].block;
+    choice.rg = cands[best].rg;
+    if (table) {
+        std::memset(table, 0, sizeof(*table));
+        table->trial_rounds = rounds;
+        int slot = 0;
+        for (const Cand& c : cands) {
+            if (slot >= DRIFT_PT_TUNE_MAX) break;
+            table->cand_engine[slot] = c.engine;
+            table->cand_block[slot] = c.block;
+            table->cand_group[slot] = c.rg;
+            table->cand_seconds[slot] = c.secs;
             ++slot;
         }
-    }
-    g_err.clear();
-    if (table) {
         table->n_candidates = slot;
         table->engine = choice.engine;
         table->block_threads = choice.block;
+        table->slots_per_group = choice.rg;
     }
-    if (best < 0.0) return fail(DRIFT_PT_EHIP, "autotune: no candidate engine ran");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[tune_key(H)] = choice;
     return 0;
@@ -1894,6 +2132,7 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
             }
             engine = choice.engine;
             block = choice.block;
+            H.rg_req = choice.rg;
         }
     }
     return run_engine(H, engine, block, result);
@@ -1970,35 +2209,79 @@ int prepare_batch(const drift_pt_problem* problems, const drift_pt_params* param
 }
 
 // The batch: instances that fit one workgroup's LDS in one persistent launch per weight type;
-// the others on the per-launch engines, four at a time on streams of their own.
+// larger ones whose single replica fits on the split engine, in one persistent launch per weight
+// type, when their groups number at least one per compute unit; the rest on the per-launch
+// engines, four at a time on streams of their own.  A batch shares one workgroup size per launch:
+// the largest requested (0: the library's choice).
 int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs) {
-    std::map<int, std::vector<size_t>> by_wm;
-    std::vector<const HostInst*> big;
-    std::vector<drift_pt_result*> big_rs;
+    std::map<int, std::vector<size_t>> by_wm, split_wm;
+    std::vector<size_t> big;
     for (size_t i = 0; i < hs.size(); ++i) {
         if (fits_lds(hs[i])) by_wm[hs[i].wm].push_back(i);
-        else {
-            big.push_back(&hs[i]);
-            big_rs.push_back(rs[i]);
-        }
+        else big.push_back(i);
     }
+    auto part_of = [&](const std::vector<size_t>& idx, std::vector<HostInst>& part,
+                       std::vector<drift_pt_result*>& prs, int& block) {
+        block = 0;
+        for (size_t i : idx) {
+            part.push_back(hs[i]);
+            prs.push_back(rs[i]);
+            block = std::max(block, hs[i].block_req);
+        }
+    };
     for (auto& g : by_wm) {
         std::vector<HostInst> part;
         std::vector<drift_pt_result*> prs;
+        int block = 0;
         if (by_wm.size() == 1 && big.empty()) {
-            const int rc = run_lds(hs, rs, 0);
+            for (const HostInst& H : hs) block = std::max(block, H.block_req);
+            const int rc = run_lds(hs, rs, block);
             if (rc) return rc;
             break;
         }
-        for (size_t i : g.second) {
-            part.push_back(hs[i]);
-            prs.push_back(rs[i]);
-        }
-        const int rc = run_lds(part, prs, 0);
+        part_of(g.second, part, prs, block);
+        const int rc = run_lds(part, prs, block);
         if (rc) return rc;
     }
-    if (!big.empty()) return run_global_set(big, big_rs, 4, nullptr);
-    return 0;
+    if (big.empty()) return 0;
+    // the split engine when its groups fill the chip; the fits are checked per instance
+    std::vector<HostInst> cand;
+    std::vector<size_t> cand_idx, rest;
+    for (size_t i : big) {
+        if (split_fits(hs[i])) {
+            cand.push_back(hs[i]);
+            cand_idx.push_back(i);
+        } else {
+            rest.push_back(i);
+        }
+    }
+    if (!cand.empty()) {
+        const std::vector<int> rgs = split_rgs(cand);
+        int64_t groups = 0;
+        for (size_t j = 0; j < cand.size(); ++j) groups += (cand[j].R + rgs[j] - 1) / rgs[j];
+        if (groups >= device_cus()) {
+            for (size_t j = 0; j < cand.size(); ++j) split_wm[cand[j].wm].push_back(cand_idx[j]);
+        } else {
+            rest.insert(rest.end(), cand_idx.begin(), cand_idx.end());
+        }
+    }
+    for (auto& g : split_wm) {
+        std::vector<HostInst> part;
+        std::vector<drift_pt_result*> prs;
+        int block = 0;
+        part_of(g.second, part, prs, block);
+        const int rc = run_split(part, prs, block);
+        if (rc) return rc;
+    }
+    if (rest.empty()) return 0;
+    std::sort(rest.begin(), rest.end());
+    std::vector<const HostInst*> gh;
+    std::vector<drift_pt_result*> grs;
+    for (size_t i : rest) {
+        gh.push_back(&hs[i]);
+        grs.push_back(rs[i]);
+    }
+    return run_global_set(gh, grs, 4, nullptr);
 }
 
 // The allocation-free entry point's device region.  Every device byte drift_pt_run_batch_noalloc

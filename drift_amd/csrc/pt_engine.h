@@ -60,6 +60,20 @@ struct PtInstance {
     int32_t ell_rest;         // some row is longer than gq
     int32_t has_target;       // stop after the first round end whose best is <= target_units
     int64_t target_units;
+    // Split engine: the slots are held by ng workgroups of rg consecutive slots each (the last
+    // group may hold padding slots); img holds ng images of n * rg bytes (rounded up to 16), each
+    // group's spins [position][slot].  box: [round parity][Rp slots][2] energy mailboxes, the low
+    // and the high 32 bits of a slot's energy after a round, each word under the round's tag
+    // (pt_split_word); all bits set at the start.  Other engines: rg = Rp, ng = 1, box null.
+    uint64_t* box;
+    int32_t rg, ng;
+};
+
+// Split engine kernel arguments: the abort flag (set when a mailbox wait exceeds timeout_ticks;
+// every workgroup then leaves) and the bound of one wait in device wall-clock ticks.
+struct SplitArgs {
+    int32_t* abort;
+    unsigned long long timeout_ticks;
 };
 
 #define PT_DACC_SPLIT 32
@@ -179,6 +193,20 @@ hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int 
                          bool init, hipStream_t s);
 // Bytes of PtInstance::img for n positions and Rp slots.
 __host__ __device__ inline size_t pt_lds_img_bytes(int n, int Rp) { return ((size_t)n * Rp + 15) & ~(size_t)15; }
+
+// Split engine: dynamic LDS of one workgroup holding rg of an instance's Rp slots (its spins,
+// and the ladder arrays of all Rp slots).
+size_t pt_split_lds_bytes(int n, int rg, int Rp);
+// One persistent launch of the split engine over work items items[0 .. n_items) (instance index
+// << 10 | group), taken from the queue *d_queue (start at 0), groups of one instance consecutive.
+// init: each item draws its group's initial spins and energies (no waiting); otherwise the groups
+// of an instance run every round together, so every group of the largest instance must be
+// resident at once: grid <= the co-resident capacity (pt_split_occupancy), cooperative unless
+// plain.
+hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, int n_items,
+                           int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
+                           bool init, const SplitArgs& A, bool cooperative, hipStream_t s);
+hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu);
 
 hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
                             hipStream_t s);

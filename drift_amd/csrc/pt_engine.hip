@@ -34,17 +34,75 @@ using namespace pt_dev;
 // ------------------------------------------------------------------------------------------
 struct LShared {
     long long bestE;
-    int have, bround, brung, bslot, improved, inst;
+    int have, bround, brung, bslot, improved, inst, abort;
 };
 
-// INIT: draw the instance's initial spins and energies and leave them in I.img / I.e0 (the launch
-// before the timed one).  Otherwise: load them and run every round.
-template <int WM, bool INIT>
-__device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long long* scrE,
-                             int* scrR) {
+// Split engine mailbox word: a round tag (16 bits) over 32 bits of an energy.  A slot's energy
+// travels as two such words (low and high half), each checked against the awaited tag, so a
+// reader never combines halves of different rounds.  Buffers alternate by round parity: a
+// workgroup writes round r + 2 only after reading every slot's round r + 1 words, each written
+// after its writer had read round r, so equal tags two rounds apart never meet.
+__device__ __forceinline__ uint64_t pt_split_word(uint32_t tag, uint32_t half) {
+    return ((uint64_t)(tag & 0xffffu) << 48) | (uint64_t)half;
+}
+
+// Split engine, after round rnd's sweeps: publish this group's slots' energies and gather every
+// slot's into E (wave 0 polls; one lane per slot).  False when a wait exceeded its bound or
+// another workgroup gave up (the abort flag is then set, and every workgroup leaves).
+__device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, int a0, int rg,
+                             int64_t* E, LShared* sc) {
+    const int tid = threadIdx.x, R = I.R;
+    uint64_t* box = I.box + (size_t)(rnd & 1) * 2 * I.Rp;
+    const uint32_t tag = (uint32_t)(rnd + 1) & 0xffffu;
+    if (tid < rg && a0 + tid < R) {
+        const uint64_t e = (uint64_t)E[a0 + tid];
+        st_u64(box + 2 * (a0 + tid), pt_split_word(tag, (uint32_t)e));
+        st_u64(box + 2 * (a0 + tid) + 1, pt_split_word(tag, (uint32_t)(e >> 32)));
+    }
+    if (tid < PT_WAVE) {
+        bool ok = true;
+        const unsigned long long t0 = (unsigned long long)wall_clock64();
+        for (int x = tid; x < R && ok; x += PT_WAVE) {
+            if (x >= a0 && x < a0 + rg) continue;
+            for (;;) {
+                const uint64_t lo = ld_u64(box + 2 * x), hi = ld_u64(box + 2 * x + 1);
+                if ((uint32_t)(lo >> 48) == tag && (uint32_t)(hi >> 48) == tag) {
+                    E[x] = (int64_t)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull));
+                    break;
+                }
+                if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                    (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
+                    __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (!ok) sc->abort = 1;
+    }
+    __syncthreads();
+    return sc->abort == 0;
+}
+
+// One instance (SPLIT = false: all its slots; SPLIT: the rg slots of group grp, the other groups
+// of the instance running at the same time in other workgroups).
+// INIT: draw the initial spins and energies and leave them in I.img / I.e0 (the launch before the
+// timed one).  Otherwise: load them and run every round.
+// Split groups run the same chain: the sweeps of a slot depend only on its own spins and on the
+// rung it holds; after a round's sweeps every group reads every slot's energy from the mailboxes,
+// and every group then takes the same round-end decisions (best record, exchanges, labels) from
+// the same inputs.  The group holding the best slot copies its state; group 0 writes the history,
+// the trace and the statistics.
+template <int WM, bool INIT, bool SPLIT>
+__device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* sc,
+                             long long* scrE, int* scrR, const SplitArgs& SA) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = I.n, R = I.R, Rp = I.Rp, k = I.k;
-    size_t off = ((size_t)n * Rp + 15) & ~(size_t)15;
+    const int rg = SPLIT ? I.rg : Rp;          // slots this workgroup holds
+    const int a0 = SPLIT ? grp * rg : 0;       // the first of them
+    const bool owner = !SPLIT || grp == 0;     // writes history, trace and statistics
+    size_t off = ((size_t)n * rg + 15) & ~(size_t)15;
     int8_t* sp = (int8_t*)smem;
     int64_t* E = (int64_t*)(smem + off);      off += 8 * (size_t)Rp;
     int64_t* tried = (int64_t*)(smem + off);  off += 8 * (size_t)Rp;
@@ -57,36 +115,41 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
     int32_t* trips = (int32_t*)(smem + off);  off += 4 * (size_t)Rp;
     float* beta_f = (float*)(smem + off);
 
-    const int a = tid % Rp, qo = tid / Rp, qs = nt / Rp;
-    const int img16 = (int)(pt_lds_img_bytes(n, Rp) >> 4);   // the spins as 16-byte words
+    const int al = tid % rg, qo = tid / rg, qs = nt / rg;
+    const int a = a0 + al;                     // the slot this thread sweeps
+    const int img16 = (int)(pt_lds_img_bytes(n, rg) >> 4);   // the spins as 16-byte words
+    int8_t* img = SPLIT ? I.img + (size_t)grp * pt_lds_img_bytes(n, rg) : I.img;
     if constexpr (INIT) {
         // --- initial spins from the stream and their energies, out to the image ---------------
-        for (int idx = tid; idx < n * Rp; idx += nt) {
-            const int p = idx / Rp, s = idx - p * Rp;
-            sp[idx] = s < R ? init_spin(I, p, s) : (int8_t)0;
+        for (int idx = tid; idx < n * rg; idx += nt) {
+            const int p = idx / rg, s = idx - p * rg;
+            sp[idx] = a0 + s < R ? init_spin(I, p, a0 + s) : (int8_t)0;
         }
-        for (int x = tid; x < Rp; x += nt) E[x] = 0;
+        for (int x = tid; x < rg; x += nt) E[a0 + x] = 0;
         __syncthreads();
         int64_t acc = 0;
         if (a < R)
             for (int p = qo; p < n; p += qs)
-                acc += site_energy<WM>(p, a, Rp, sp, I.row_ptr, I.edge, I.h);
+                acc += site_energy<WM>(p, al, rg, sp, I.row_ptr, I.edge, I.h);
         if (acc != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)acc);
         __syncthreads();
-        for (int x = tid; x < img16; x += nt) ((uint4*)I.img)[x] = ((const uint4*)sp)[x];
-        for (int x = tid; x < Rp; x += nt) I.e0[x] = WM != PT_WM_F32 ? -(E[x] / 2) : E[x];
+        for (int x = tid; x < img16; x += nt) ((uint4*)img)[x] = ((const uint4*)sp)[x];
+        for (int x = tid; x < rg; x += nt)
+            I.e0[a0 + x] = WM != PT_WM_F32 ? -(E[a0 + x] / 2) : E[a0 + x];
         return;
     }
     // --- the initial state from the image, bookkeeping ---------------------------------------
-    for (int x = tid; x < img16; x += nt) ((uint4*)sp)[x] = ((const uint4*)I.img)[x];
+    for (int x = tid; x < img16; x += nt) ((uint4*)sp)[x] = ((const uint4*)img)[x];
     for (int x = tid; x < Rp; x += nt) {
-        E[x] = I.e0[x]; tried[x] = 0; accd[x] = 0; up[x] = 0; down[x] = 0; trips[x] = 0;
+        E[x] = x < R ? I.e0[x] : 0; tried[x] = 0; accd[x] = 0; up[x] = 0; down[x] = 0; trips[x] = 0;
         rung[x] = x;
         rep[x] = x;
         label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
         beta_f[x] = x < R ? (float)I.beta[x] : 0.0f;
     }
-    if (tid == 0) { sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; }
+    if (tid == 0) {
+        sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; sc->abort = 0;
+    }
     __syncthreads();
 
     // R <= 64: the round end is wave 0's (WaveLadder), with one workgroup barrier per round.  The
@@ -96,6 +159,16 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
     // decision to stop
     int rounds_run = I.n_rounds;
     auto reached = [&]() { return I.has_target && sc->have && sc->bestE <= I.target_units; };
+    // copy of the best slot's state: the whole instance, or (split) only by the group holding it,
+    // with coherent stores so that a later group's copy is the one memory keeps
+    auto copy_best = [&](int slot) {
+        if constexpr (SPLIT) {
+            if (slot >= a0 && slot < a0 + rg)
+                for (int p = tid; p < n; p += nt) st_spin<true>(I.best_s + p, sp[(size_t)p * rg + slot - a0]);
+        } else {
+            for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
+        }
+    };
 
     for (int rnd = -1; rnd < I.n_rounds; ++rnd) {
         if (rnd >= 0) {
@@ -118,7 +191,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                                 const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t,
                                                            PT_TAG_METRO, I.k0, I.k1);
                                 dsum += quad_update<WM, decltype(gw)::value>(
-                                    qq, cs, ce, n, a, Rp, sp, I.row_ptr, I.edge, I.ell, I.h,
+                                    qq, cs, ce, n, al, rg, sp, I.row_ptr, I.edge, I.ell, I.h,
                                     I.ell_rest != 0, rv, thr_row, I.TQ, bf);
                             }
                         };
@@ -132,7 +205,13 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                 }
             }
             if (dsum != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)dsum);
+            // split: the best-state copy of the last round end is complete before the energies
+            // that let other groups pass this round go out
+            if constexpr (SPLIT) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
+            if constexpr (SPLIT) {
+                if (!split_gather(I, SA, rnd, a0, rg, E, sc)) return;
+            }
         }
 
         if (fast) {
@@ -154,7 +233,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
                 W.best_rung = sc->brung;
                 W.best_slot = sc->bslot;
                 const int64_t Ecur = r < R ? E[W.rep] : 0;
-                const bool imp = wave_ladder_round(W, I, rnd + 1, Ecur, r, true);
+                const bool imp = wave_ladder_round(W, I, rnd + 1, Ecur, r, owner);
                 if (r < R) {
                     rep[r] = W.rep;
                     rung[W.rep] = r;
@@ -176,8 +255,7 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
             }
             __syncthreads();
             if (sc->improved) {
-                const int slot = sc->bslot;
-                for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
+                copy_best(sc->bslot);
                 __syncthreads();   // read before the next round's sweeps change the spins
             }
             if (reached()) {
@@ -197,11 +275,10 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
             if (improved) {
                 sc->bestE = mE; sc->bround = rnd + 1; sc->brung = mR; sc->bslot = slot; sc->have = 1;
             }
-            if (rnd >= 0 && I.hist) I.hist[rnd] = (double)sc->bestE * I.unit;
+            if (rnd >= 0 && I.hist && owner) I.hist[rnd] = (double)sc->bestE * I.unit;
         }
-        if (improved)
-            for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
-        if (rnd >= 0 && I.trace)
+        if (improved) copy_best(slot);
+        if (rnd >= 0 && I.trace && owner)
             for (int r = tid; r < R; r += nt)
                 I.trace[(size_t)rnd * R + r] = (double)E[rep[r]] * I.unit;
         __syncthreads();
@@ -226,24 +303,26 @@ __device__ void lds_instance(const PtInstance& I, char* smem, LShared* sc, long 
 
     // --- results ---------------------------------------------------------------------------
     int64_t* out = I.out;
-    if (tid == 0) {
-        out[0] = sc->bestE; out[1] = sc->bround; out[2] = sc->brung; out[3] = sc->bslot;
-        out[4] = sc->have;
-        out[5] = rounds_run;
-    }
-    for (int x = tid; x < R; x += nt) {
-        pt_out_arr(out, R, PT_OUT_E)[x] = E[x];
-        pt_out_arr(out, R, PT_OUT_RUNG)[x] = rung[x];
-        pt_out_arr(out, R, PT_OUT_TRIED)[x] = tried[x];
-        pt_out_arr(out, R, PT_OUT_ACC)[x] = accd[x];
-        pt_out_arr(out, R, PT_OUT_TRIPS)[x] = trips[x];
-        pt_out_arr(out, R, PT_OUT_UP)[x] = up[x];
-        pt_out_arr(out, R, PT_OUT_DOWN)[x] = down[x];
+    if (owner) {
+        if (tid == 0) {
+            out[0] = sc->bestE; out[1] = sc->bround; out[2] = sc->brung; out[3] = sc->bslot;
+            out[4] = sc->have;
+            out[5] = rounds_run;
+        }
+        for (int x = tid; x < R; x += nt) {
+            pt_out_arr(out, R, PT_OUT_E)[x] = E[x];
+            pt_out_arr(out, R, PT_OUT_RUNG)[x] = rung[x];
+            pt_out_arr(out, R, PT_OUT_TRIED)[x] = tried[x];
+            pt_out_arr(out, R, PT_OUT_ACC)[x] = accd[x];
+            pt_out_arr(out, R, PT_OUT_TRIPS)[x] = trips[x];
+            pt_out_arr(out, R, PT_OUT_UP)[x] = up[x];
+            pt_out_arr(out, R, PT_OUT_DOWN)[x] = down[x];
+        }
     }
     if (I.final_s)
-        for (int idx = tid; idx < n * R; idx += nt) {
+        for (int idx = tid; idx < n * rg; idx += nt) {
             const int s = idx / n, p = idx - s * n;
-            I.final_s[idx] = sp[(size_t)p * Rp + s];
+            if (a0 + s < R) I.final_s[(size_t)(a0 + s) * n + p] = sp[(size_t)p * rg + s];
         }
 }
 
@@ -255,13 +334,41 @@ __global__ __launch_bounds__(1024) void pt_lds_kernel(const PtInstance* __restri
     __shared__ LShared sc;
     __shared__ long long scrE[16];
     __shared__ int scrR[16];
+    const SplitArgs none{nullptr, 0ull};
     for (;;) {
         if (threadIdx.x == 0) sc.inst = atomicAdd(queue, 1);
         __syncthreads();
         const int qi = sc.inst;
         if (qi >= n_inst) return;
         const PtInstance I = insts[order[qi]];
-        lds_instance<WM, INIT>(I, smem, &sc, scrE, scrR);
+        lds_instance<WM, INIT, false>(I, 0, smem, &sc, scrE, scrR, none);
+        __syncthreads();
+    }
+}
+
+// Split engine: work items are (instance, group), groups of one instance consecutive in the
+// queue, so a workgroup waiting in a mailbox waits only on items already taken by resident
+// workgroups or on the next ones taken (the grid holds every group of the largest instance).
+template <int WM, bool INIT>
+__global__ __launch_bounds__(1024) void pt_split_kernel(const PtInstance* __restrict__ insts,
+                                                        const int32_t* __restrict__ items,
+                                                        int n_items, int32_t* queue, SplitArgs SA) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ LShared sc;
+    __shared__ long long scrE[16];
+    __shared__ int scrR[16];
+    for (;;) {
+        if (threadIdx.x == 0)
+            sc.inst = (!INIT && __hip_atomic_load(SA.abort, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT) != 0)
+                          ? n_items
+                          : atomicAdd(queue, 1);
+        __syncthreads();
+        const int qi = sc.inst;
+        if (qi >= n_items) return;
+        const int item = items[qi];
+        const PtInstance I = insts[item >> 10];
+        lds_instance<WM, INIT, true>(I, item & 1023, smem, &sc, scrE, scrR, SA);
         __syncthreads();
     }
 }
@@ -717,6 +824,49 @@ inline int block_for_slots(int Rp) { return Rp <= 256 ? 256 : ((Rp + 63) / 64) *
 
 size_t pt_lds_bytes(int n, int Rp) {
     return (((size_t)n * Rp + 15) & ~(size_t)15) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+}
+
+size_t pt_split_lds_bytes(int n, int rg, int Rp) {
+    return pt_lds_img_bytes(n, rg) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+}
+
+template <int WM, bool INIT>
+static hipError_t split_launch(const PtInstance* d_insts, const int32_t* d_items, int n_items,
+                               int32_t* d_queue, int block, int grid, size_t lds_bytes,
+                               const SplitArgs& A, bool cooperative, hipStream_t s) {
+    const void* fn = (const void*)pt_split_kernel<WM, INIT>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    if (cooperative) {
+        SplitArgs a = A;
+        void* args[] = {(void*)&d_insts, (void*)&d_items, (void*)&n_items, (void*)&d_queue, (void*)&a};
+        return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, (unsigned)lds_bytes, s);
+    }
+    pt_split_kernel<WM, INIT><<<grid, block, lds_bytes, s>>>(d_insts, d_items, n_items, d_queue, A);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, int n_items,
+                           int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
+                           bool init, const SplitArgs& A, bool cooperative, hipStream_t s) {
+    if (init) cooperative = false;   // no group waits on another while drawing initial states
+    if (wm == PT_WM_I8)
+        return init ? split_launch<PT_WM_I8, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                    : split_launch<PT_WM_I8, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+    if (wm == PT_WM_I32)
+        return init ? split_launch<PT_WM_I32, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                    : split_launch<PT_WM_I32, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+    return init ? split_launch<PT_WM_F32, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                : split_launch<PT_WM_F32, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+}
+
+hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu) {
+    const void* fn = wm == PT_WM_I8    ? (const void*)pt_split_kernel<PT_WM_I8, false>
+                     : wm == PT_WM_I32 ? (const void*)pt_split_kernel<PT_WM_I32, false>
+                                       : (const void*)pt_split_kernel<PT_WM_F32, false>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, block, lds_bytes);
 }
 
 template <int WM, bool INIT>

@@ -166,7 +166,7 @@ class _Buffers:
 
 def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, engine,
             block_threads, final, history, trace=False, time_kernels=False, graph=True,
-            autotune=False, plain_launch=False, target=None):
+            autotune=False, plain_launch=False, target=None, slots_per_group=0):
     flags = ((_lib.RECORD_FINAL if final else 0) | (_lib.RECORD_HISTORY if history else 0)
              | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0)
              | (0 if graph else _lib.NO_GRAPH) | (_lib.AUTOTUNE if autotune else 0)
@@ -186,6 +186,7 @@ def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, e
         flags=flags,
         block_threads=int(block_threads),
         target_energy=0.0 if target is None else float(target),
+        slots_per_group=int(slots_per_group),
     )
 
 
@@ -233,6 +234,8 @@ def _to_result(res: _lib.Result, buf: _Buffers, temps, n_rounds, R, history) -> 
         other_kernel_seconds=float(res.other_kernel_seconds),
         other_kernel_launches=int(res.other_kernel_launches),
         rounds_run=int(res.rounds_run),
+        block_threads=int(res.block_threads_used),
+        slots_per_group=int(res.slots_per_group_used),
     )
 
 
@@ -258,6 +261,7 @@ def parallel_tempering_gpu(
     autotune: bool = False,
     plain_launch: bool = False,
     target_energy: float | None = None,
+    slots_per_group: int = 0,
 ) -> PtResult:
     """Solve ``model`` by parallel tempering on the GPU; same signature and result as the CPU
     ``parallel_tempering`` (drift/gpu.py:43-114).  ``throughput`` is spin-flip attempts per second
@@ -268,13 +272,14 @@ def parallel_tempering_gpu(
     cannot follow cooperative launches; needs the device to itself).  ``target_energy`` stops the
     run after the first round end whose best energy is at or below it (engines lds, global and
     lattice); ``rounds_run`` then says how many rounds ran, and the result is that of a run of
-    that many rounds."""
+    that many rounds.  ``slots_per_group`` sets the split engine's replica slots per workgroup
+    (a power of two up to 64; 0: the library's choice)."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
     prm = _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, b, engine,
                   block_threads, return_final, record_history, record_trace, time_kernels,
-                  graph, autotune, plain_launch, target_energy)
+                  graph, autotune, plain_launch, target_energy, slots_per_group)
     buf = _Buffers(prob.n, n_replicas, n_rounds, return_final, record_history, record_trace)
     res = buf.struct
     rc = lib.drift_pt_run(prob.struct, prm, res)
@@ -300,6 +305,7 @@ def parallel_tempering_gpu_batch(
     block_threads: int = 0,
     no_alloc: bool = False,
     targets=None,
+    slots_per_group: int = 0,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
@@ -307,9 +313,11 @@ def parallel_tempering_gpu_batch(
     workgroup size (0: the library's choice).  ``no_alloc`` runs the batch through
     drift_pt_run_batch_noalloc, which allocates no device memory (a fixed region of
     ``_lib.NOALLOC_BYTES`` per device; larger batches raise RuntimeError); the results are the
-    same.  Instances too large for one workgroup run on the per-launch engine, four at a time on
-    streams of their own.  ``targets`` (one energy or None per instance) stops each instance at
-    its target, as ``target_energy`` does for one."""
+    same.  Instances too large for one workgroup run on the split engine in a second persistent
+    launch when their replica groups fill the chip (``slots_per_group`` sets the group size, 0:
+    the library's choice), else on the per-launch engine, four at a time on streams of their own.
+    ``targets`` (one energy or None per instance) stops each instance at its target, as
+    ``target_energy`` does for one."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -323,7 +331,8 @@ def parallel_tempering_gpu_batch(
         probs.append(encode_problem(model, None, force_float))
         prms.append(_params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seeds[i], b,
                             "lds", block_threads, return_final, record_history, record_trace,
-                            target=None if targets is None else targets[i]))
+                            target=None if targets is None else targets[i],
+                            slots_per_group=slots_per_group))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])
@@ -339,9 +348,10 @@ def parallel_tempering_gpu_batch(
 def autotune(model, *, n_replicas: int = 16, T_min: float = 0.1, T_max: float = 5.0,
              sweeps_per_round: int = 5, seed: int = 0, betas=None, coloring=None,
              trial_rounds: int = 0, force_float: bool = False) -> dict:
-    """Time every engine and block size that can run ``model`` (drift_pt_autotune), cache the
-    fastest for its shape class, and return {"engine", "block_threads", "trial_rounds",
-    "candidates": [(engine, block, seconds), ...]} (seconds < 0: the candidate could not run)."""
+    """Time every engine and launch shape that can run ``model`` (drift_pt_autotune), cache the
+    choice for its shape class, and return {"engine", "block_threads", "slots_per_group",
+    "trial_rounds", "candidates": [(engine, block, slots_per_group, seconds), ...]} (block 0: the
+    engine's own choice; seconds < 0: the candidate could not run)."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
@@ -352,10 +362,11 @@ def autotune(model, *, n_replicas: int = 16, T_min: float = 0.1, T_max: float = 
     if rc != 0:
         raise RuntimeError(f"GPU engine failed (status {rc}): {_lib.last_error(lib)}")
     cands = [(_lib.ENGINE_NAMES.get(int(tab.cand_engine[i]), str(tab.cand_engine[i])),
-              int(tab.cand_block[i]), float(tab.cand_seconds[i])) for i in range(tab.n_candidates)]
+              int(tab.cand_block[i]), int(tab.cand_group[i]), float(tab.cand_seconds[i]))
+             for i in range(tab.n_candidates)]
     return {"engine": _lib.ENGINE_NAMES.get(int(tab.engine), str(tab.engine)),
-            "block_threads": int(tab.block_threads), "trial_rounds": int(tab.trial_rounds),
-            "candidates": cands}
+            "block_threads": int(tab.block_threads), "slots_per_group": int(tab.slots_per_group),
+            "trial_rounds": int(tab.trial_rounds), "candidates": cands}
 
 
 def autotune_clear() -> None:

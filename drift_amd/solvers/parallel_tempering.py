@@ -62,6 +62,8 @@ class PtResult:
     other_kernel_seconds: float = None
     other_kernel_launches: int = None
     rounds_run: int = None               # rounds run (fewer than asked when a target was reached)
+    block_threads: int = None            # workgroup size of the sweep kernel
+    slots_per_group: int = None          # replica slots per workgroup (split and LDS engines)
 
 
 def _neighbour_access(model):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_PT_ABI_VERSION 4
+#define DRIFT_PT_ABI_VERSION 5
 
 enum drift_pt_status {
     DRIFT_PT_OK = 0,
@@ -53,12 +53,19 @@ enum drift_pt_engine {
     DRIFT_PT_ENGINE_COOP = 5,   /* CSR instances too large for one workgroup: one cooperative launch
                                    runs every round; each workgroup keeps its share of the rows in
                                    LDS, colour steps are separated by grid barriers */
-    DRIFT_PT_ENGINE_PSLAB = 6   /* the slab engine's lattices in one cooperative launch: slabs stay
+    DRIFT_PT_ENGINE_PSLAB = 6,  /* the slab engine's lattices in one cooperative launch: slabs stay
                                    in LDS for the whole run, halo planes move through HBM between
                                    sweeps, one grid barrier per sweep.  The first run of a lattice
                                    shape in a process times the barrier's two energy paths and
                                    keeps the faster (environment DRIFT_PT_PSLAB_ENERGY_PATH=atomic
                                    or =words fixes it); results do not depend on the path */
+    DRIFT_PT_ENGINE_SPLIT = 7   /* instances whose replicas do not fit one workgroup's LDS (and
+                                   batches of them): the replica slots are split into groups of
+                                   params.slots_per_group, each group's spins in one workgroup's
+                                   LDS for the whole run; after each round's sweeps the groups
+                                   trade slot energies through round-tagged mailboxes in HBM and
+                                   all take the same exchange decisions.  One persistent launch;
+                                   every group of an instance is resident at once */
 };
 
 enum drift_pt_flags {
@@ -80,8 +87,8 @@ enum drift_pt_flags {
     DRIFT_PT_STOP_AT_TARGET = 256 /* stop after the first round end whose best energy is at or
                                      below params.target_energy (within 1e-9 relative); the result
                                      is then that of a run of result.rounds_run rounds.  Engines
-                                     LDS, GLOBAL and LATTICE (AUTO picks among them); the others
-                                     refuse it. */
+                                     LDS, SPLIT, GLOBAL and LATTICE (AUTO picks among them); the
+                                     others refuse it. */
 };
 
 /* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and
@@ -112,6 +119,8 @@ typedef struct drift_pt_params {
     int32_t flags;              /* enum drift_pt_flags */
     int32_t block_threads;      /* 0 = chosen by the tuner */
     double target_energy;       /* with DRIFT_PT_STOP_AT_TARGET: the energy to stop at */
+    int32_t slots_per_group;    /* SPLIT engine: replica slots per workgroup, a power of two up
+                                   to 64; 0 = chosen by the library (or the tuner) */
 } drift_pt_params;
 
 /* Results, all arrays caller-allocated (NULL = not wanted, except best_s). */
@@ -143,17 +152,22 @@ typedef struct drift_pt_result {
                                        non-sweep launches (round ends of the per-launch engines) */
     int64_t other_kernel_launches;  /* and their number (0 when the sweep kernel does everything) */
     int32_t rounds_run;         /* rounds run: n_rounds, or fewer when the target was reached */
+    int32_t block_threads_used; /* workgroup size of the sweep kernel */
+    int32_t slots_per_group_used; /* replica slots per workgroup (the SPLIT engine's groups; the
+                                     padded slot count on the LDS engine; 0 on the others) */
 } drift_pt_result;
 
 /* Timed candidates of one autotune call (drift_pt_autotune). */
-#define DRIFT_PT_TUNE_MAX 16
+#define DRIFT_PT_TUNE_MAX 48
 typedef struct drift_pt_tuning {
-    int32_t engine;                          /* the fastest candidate */
+    int32_t engine;                          /* the chosen candidate */
     int32_t block_threads;
+    int32_t slots_per_group;                 /* SPLIT: replica slots per workgroup, else 0 */
     int32_t trial_rounds;                    /* rounds each timed trial ran */
     int32_t n_candidates;
     int32_t cand_engine[DRIFT_PT_TUNE_MAX];
     int32_t cand_block[DRIFT_PT_TUNE_MAX];
+    int32_t cand_group[DRIFT_PT_TUNE_MAX];
     double cand_seconds[DRIFT_PT_TUNE_MAX];  /* device time of trial_rounds rounds (a run of
                                                 2 * trial_rounds minus a run of trial_rounds, so
                                                 fixed costs cancel); < 0 if it could not run */
@@ -205,11 +219,15 @@ int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* 
 int drift_pt_run_batch_noalloc(const drift_pt_problem* problems, const drift_pt_params* params,
                                int32_t n_instances, drift_pt_result* results);
 
-/* Autotuner.  Times every engine and block size that can run this instance (LDS, slab, lattice,
- * global, coop, pslab; 256 / 512 / 1024 threads) for trial_rounds rounds (0: about 2.5e8 attempts),
- * stores the fastest in a process-wide cache keyed by the instance's shape class (weight type,
- * replica slots, lattice shape, size and degree buckets, sweeps per round), and reports every
- * candidate's time.  All candidates run the same Markov chain, so the choice changes speed only.
+/* Autotuner.  Times every engine and launch shape that can run this instance (LDS, split, slab,
+ * lattice, global, coop, pslab; 256 / 512 / 1024 threads; for the split engine every power-of-two
+ * number of replica slots per workgroup whose spins fit LDS) for trial_rounds rounds (0: about
+ * 2.5e8 attempts), in two passes of opposite order with the faster of each candidate's two trials
+ * kept, stores the choice in a process-wide cache keyed by the instance's shape class (weight
+ * type, replica slots, lattice shape, size and degree buckets, sweeps per round), and reports
+ * every candidate's time.  The engine the library takes without tuning stays the choice unless
+ * another candidate is more than 3% faster, so trial noise does not move it.  All candidates run
+ * the same Markov chain, so the choice changes speed only.
  * No reference counterpart: the reference has one kernel with a fixed launch shape
  * (cuda/ising_pt.cu:196-211). */
 int drift_pt_autotune(const drift_pt_problem* problem, const drift_pt_params* params,
