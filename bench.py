@@ -22,10 +22,13 @@ against this device's LDS read rate, measured by a probe kernel at the start of 
 limits each kernel is taken from its rocprofv3 counters (profiles/limiters.json).
 
     python bench.py [--steps K] [--warmup W] [--repeats 5] [--workload all|c4|c2|c3|c1|published]
-                    [--no-cpu-baseline] [--plain-launch]
+                    [--no-cpu-baseline] [--plain-launch] [--no-autotune]
 
---plain-launch starts the persistent engines (pslab, coop) with an ordinary dispatch, for profiler
-commands; timed runs use the cooperative launch.  --workload ladder-c5 measures time to target
+Before the timed runs, the library's autotuners time the candidate engines and launch shapes of
+each workload (the batch: its launch shapes on the batch itself) and keep the default unless a
+candidate is more than 3% faster; --no-autotune skips that.  --plain-launch starts the persistent
+engines (pslab, coop, split) with an ordinary dispatch, for profiler commands; timed runs use the
+cooperative launch.  --workload ladder-c5 measures time to target
 with the learned ladder instead, in three stages (--stage train, bestknown, eval).
 """
 
@@ -49,7 +52,7 @@ PROFILES = os.path.join(ROOT, "profiles")
 PUBLISHED_RTX5060TI = 1.29   # attempts/ns, docs/results/PHASE14-results.md:140-142
 
 # the fixed per-attempt byte figures of the measurement plan, per config
-SURVEY_BYTES = {"c2": 8.75, "c3": 50.0, "c4": 45.0}
+PLAN_BYTES = {"c2": 8.75, "c3": 50.0, "c4": 45.0}
 
 
 def workload(name: str):
@@ -333,15 +336,16 @@ def cpu_baseline_batch(models, R, beta, budget_s: float):
             "cpu_model": cpu_model(), "host_cores": cores}
 
 
-def bench_c4(steps, warmup, repeats, cpu_budget, count=4096):
+def bench_c4(steps, warmup, repeats, cpu_budget, count=4096, autotune=False):
     from drift_amd import gpu
     from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
 
     log(f"c4: building {count} instances")
     models = c4_instances(count)
     R = 16
-    kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, sweeps_per_round=1)
-    log(f"c4: warm-up {warmup} steps")
+    kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, sweeps_per_round=1, autotune=autotune)
+    log(f"c4: warm-up {warmup} steps" + (" (the batch autotuner times launch shapes first)"
+                                          if autotune else ""))
     gpu.parallel_tempering_gpu_batch(models, n_rounds=max(warmup, 1), **kw)
     total_n = sum(m.n for m in models)
     attempts = R * total_n * steps
@@ -361,11 +365,12 @@ def bench_c4(steps, warmup, repeats, cpu_budget, count=4096):
         "ms_per_step": round(med * 1e3 / steps, 5), "dtype": "f32",
         "config": {"workload": C4_DESC, "instances": count, "total_spins": int(total_n),
                    "attempts_per_step": R * total_n, "format": fmt, "engine": res[0].engine,
+                   "block_threads": res[0].block_threads, "autotune": bool(autotune),
                    "wall_s_median": round(median(walls), 3)},
         "timing": {"repeats": repeats, "runs_ms": [round(x * 1e3, 4) for x in secs],
                    "median_ms": round(med * 1e3, 4), "window": "the launch that runs the rounds "
                    "(initial states drawn by a launch of their own, untimed)"},
-        "roofline": roofline("pt_lds_kernel", B, SURVEY_BYTES["c4"], attempts, med, models[0]),
+        "roofline": roofline("pt_lds_kernel", B, PLAN_BYTES["c4"], attempts, med, models[0]),
         "cpu_baseline": None,
     }
     if cpu_budget > 0:
@@ -415,10 +420,11 @@ def bench_single(name, steps, warmup, repeats, cpu_budget, plain, engine="auto",
         "dtype": "int32" if enc.is_integer else "f32",
         "config": {"workload": desc, "n": model.n, "replicas": R, "sweeps_per_step": S,
                    "attempts_per_step": R * model.n * S, "format": fmt, "engine": res.engine,
+                   "block_threads": res.block_threads, "autotune": bool(kw["autotune"]),
                    "wall_s_median": round(median(walls), 3)},
         "timing": {"repeats": repeats, "runs_ms": [round(x * 1e3, 4) for x in secs],
                    "median_ms": round(med * 1e3, 4)},
-        "roofline": roofline(kern, B, SURVEY_BYTES.get(name), launch_attempts, launch_s, model),
+        "roofline": roofline(kern, B, PLAN_BYTES.get(name), launch_attempts, launch_s, model),
         "cpu_baseline": None,
     }
     if cpu_budget > 0:
@@ -448,7 +454,7 @@ def bench_throughput(args) -> int:
     cpu = 0.0 if args.no_cpu_baseline else args.cpu_budget
     if args.workload == "c4":
         print(json.dumps(headline(bench_c4(args.steps, args.warmup, args.repeats, cpu,
-                                           args.instances))), flush=True)
+                                           args.instances, args.autotune))), flush=True)
         return 0
     if args.workload in ("c2", "c3", "c1", "published"):
         sub = bench_single(args.workload, args.steps, args.warmup, args.repeats, cpu,
@@ -456,12 +462,13 @@ def bench_throughput(args) -> int:
         print(json.dumps(headline(sub)), flush=True)
         return 0
     # default: C4 headline with the other configs as sub-results
-    line = headline(bench_c4(args.steps, args.warmup, args.repeats, cpu, args.instances))
+    line = headline(bench_c4(args.steps, args.warmup, args.repeats, cpu, args.instances,
+                             args.autotune))
     subs = {}
     for name in ("c2", "c3", "published"):
         steps = args.sub_steps or SUB_DEFAULT_STEPS[name]
         sub = bench_single(name, steps, min(args.warmup, steps) or 1, args.repeats,
-                           cpu, args.plain_launch)
+                           cpu, args.plain_launch, autotune=args.autotune)
         if name == "published":
             sub["context"] = {"reference_rtx5060ti_attempts_per_ns": PUBLISHED_RTX5060TI,
                               "source": "docs/results/PHASE14-results.md:140-142",
@@ -753,8 +760,10 @@ def main() -> int:
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--autotune", action="store_true",
-                    help="with --engine auto: time every candidate engine first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="take the library's default engine and launch shape instead of timing "
+                         "every candidate first (the tuning runs before the timed runs, and the "
+                         "default stays unless a candidate is more than 3%% faster)")
     ap.add_argument("--plain-launch", action="store_true",
                     help="ordinary dispatch of the persistent engines (for profiler commands)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
@@ -783,11 +792,14 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (rocprof counter scaling)")
     args = ap.parse_args()
+    args.autotune = not args.no_autotune
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this engine has no "
-              "multi-GPU path; the sharding design is described in DESIGN.md", file=sys.stderr)
+              "multi-GPU path (batched instances would shard across GPUs with no communication, "
+              "one instance's ladder by contiguous rung blocks); one GPU is measured",
+              file=sys.stderr)
         return 2
 
     from drift_amd import gpu

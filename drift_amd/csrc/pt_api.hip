@@ -1960,6 +1960,16 @@ struct TuneEntry {
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneEntry> g_tune;
 
+// launch shapes chosen by the batch autotuner (tune_batch), by batch shape class
+using BatchKey = std::tuple<int, int, int, int, int, int, int, int>;
+
+struct BatchShape {
+    int block = 0, rg_cap = 0;   // rg_cap: split engine, slots per group min(rg_cap, largest fit)
+};
+
+std::mutex g_btune_mu;
+std::map<BatchKey, BatchShape> g_btune;
+
 TuneKey tune_key(const HostInst& H) {
     int nb = 0;
     while ((1 << nb) < H.n) ++nb;                               // ceil(log2 n)
@@ -2153,8 +2163,12 @@ int drift_pt_autotune(const drift_pt_problem* problem, const drift_pt_params* pa
 }
 
 void drift_pt_autotune_clear(void) {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune.clear();
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune.clear();
+    }
+    std::lock_guard<std::mutex> lk(g_btune_mu);
+    g_btune.clear();
 }
 
 }  // extern "C"
@@ -2208,6 +2222,125 @@ int prepare_batch(const drift_pt_problem* problems, const drift_pt_params* param
     return 0;
 }
 
+// ------------------------------------------------------------------------------------------
+// batch autotuner: the launch shape of each persistent launch of a batch, timed on the batch
+// ------------------------------------------------------------------------------------------
+
+// Shape class of a batch: engine (1 LDS, 7 split), weight type, largest padded slot count,
+// ceil(log2) of the largest n and of the instance count, mean degree, sweeps per round, field.
+BatchKey batch_key(int engine, const std::vector<HostInst>& hs) {
+    int maxRp = 0, maxn = 1, S = 0, hh = 0;
+    double deg = 0.0, n_all = 0.0;
+    for (const HostInst& H : hs) {
+        maxRp = std::max(maxRp, H.Rp);
+        maxn = std::max(maxn, H.n);
+        S = std::max(S, H.S);
+        hh |= H.has_h ? 1 : 0;
+        deg += H.nnz;
+        n_all += H.n;
+    }
+    auto clog2 = [](int64_t v) {
+        int b = 0;
+        while (((int64_t)1 << b) < v) ++b;
+        return b;
+    };
+    return BatchKey(engine, hs.empty() ? 0 : hs[0].wm, maxRp, clog2(maxn), clog2((int64_t)hs.size()),
+                    (int)std::lround(n_all > 0 ? deg / n_all : 0.0), S, hh);
+}
+
+int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& results,
+              int block_req);
+
+// Timed trials of launch shapes for one persistent launch of a batch (the LDS engine: block
+// sizes; the split engine: group-size caps x block sizes).  Every trial runs the whole batch for
+// T and then 2T rounds (no records, no target) and takes the difference, so fixed costs cancel;
+// each candidate is timed in two passes of opposite order and keeps its faster time.  The first
+// candidate is the library's default shape, which stays unless another is more than 3% faster.
+int tune_batch(int engine, const std::vector<HostInst>& hs, BatchShape& out) {
+    const BatchKey key = batch_key(engine, hs);
+    {
+        std::lock_guard<std::mutex> lk(g_btune_mu);
+        auto it = g_btune.find(key);
+        if (it != g_btune.end()) {
+            out = it->second;
+            return 0;
+        }
+    }
+    struct Cand {
+        BatchShape shape;
+        double secs;
+    };
+    std::vector<Cand> cands;
+    cands.push_back({BatchShape{}, -1.0});
+    int maxRp = 1;
+    for (const HostInst& H : hs) maxRp = std::max(maxRp, H.Rp);
+    const int blocks[3] = {256, 512, 1024};
+    if (engine == DRIFT_PT_ENGINE_LDS) {
+        for (int b : blocks) {
+            bool ok = b >= maxRp;
+            for (const HostInst& H : hs) ok = ok && b % H.Rp == 0;
+            if (ok) cands.push_back({BatchShape{b, 0}, -1.0});
+        }
+    } else {
+        int top = 1;
+        for (const HostInst& H : hs) top = std::max(top, split_rg_max(H));
+        for (int c = top; c >= 1; c >>= 1)
+            for (int b : blocks) cands.push_back({BatchShape{b, c}, -1.0});
+    }
+    int64_t per_round = 0;
+    int max_rounds = 0;
+    for (const HostInst& H : hs) {
+        per_round += (int64_t)H.R * H.n * std::max(H.S, 1);
+        max_rounds = std::max(max_rounds, H.n_rounds);
+    }
+    // about 5e8 attempts per trial run, at least two rounds
+    const int T = (int)std::max<int64_t>(2, std::min<int64_t>(64, (int64_t)5e8 / std::max<int64_t>(per_round, 1)));
+    std::vector<HostInst> trial = hs;
+    std::vector<std::vector<int8_t>> bs(hs.size());
+    std::vector<drift_pt_result> res(hs.size());
+    std::vector<drift_pt_result*> rp(hs.size());
+    for (size_t i = 0; i < hs.size(); ++i) {
+        trial[i].flags &= ~(DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE | DRIFT_PT_RECORD_FINAL |
+                            DRIFT_PT_TIME_KERNELS);
+        trial[i].has_target = 0;
+        bs[i].resize((size_t)std::max(hs[i].n, 1));
+        std::memset(&res[i], 0, sizeof(res[i]));
+        res[i].best_s = bs[i].data();
+        rp[i] = &res[i];
+    }
+    auto timed = [&](const BatchShape& sh, int rounds, double& secs) {
+        for (HostInst& H : trial) {
+            H.n_rounds = rounds;
+            if (engine == DRIFT_PT_ENGINE_SPLIT)
+                H.rg_req = sh.rg_cap > 0 ? std::min(sh.rg_cap, split_rg_max(H)) : 0;
+        }
+        const int rc = engine == DRIFT_PT_ENGINE_LDS ? run_lds(trial, rp, sh.block)
+                                                     : run_split(trial, rp, sh.block);
+        (void)hipGetLastError();
+        secs = rc == 0 ? res[0].seconds : -1.0;
+        return rc == 0;
+    };
+    auto one = [&](Cand& c) {
+        double t1 = 0.0, t2 = 0.0;
+        if (timed(c.shape, T, t1) && timed(c.shape, 2 * T, t2)) {
+            const double d = t2 - t1 > 0.0 ? t2 - t1 : 0.5 * t2;
+            if (c.secs < 0.0 || d < c.secs) c.secs = d;
+        }
+    };
+    for (Cand& c : cands) one(c);
+    for (size_t i = cands.size(); i-- > 0;) one(cands[i]);
+    g_err.clear();
+    int best = -1;
+    for (size_t i = 0; i < cands.size(); ++i)
+        if (cands[i].secs > 0.0 && (best < 0 || cands[i].secs < cands[best].secs)) best = (int)i;
+    if (best < 0) return fail(DRIFT_PT_EHIP, "batch autotune: no launch shape ran");
+    if (cands[0].secs > 0.0 && cands[best].secs > cands[0].secs / 1.03) best = 0;
+    out = cands[best].shape;
+    std::lock_guard<std::mutex> lk(g_btune_mu);
+    g_btune[key] = out;
+    return 0;
+}
+
 // The batch: instances that fit one workgroup's LDS in one persistent launch per weight type;
 // larger ones whose single replica fits on the split engine, in one persistent launch per weight
 // type, when their groups number at least one per compute unit; the rest on the per-launch
@@ -2229,17 +2362,25 @@ int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs
             block = std::max(block, hs[i].block_req);
         }
     };
+    bool tune = false;
+    for (const HostInst& H : hs) tune = tune || (H.flags & DRIFT_PT_AUTOTUNE);
     for (auto& g : by_wm) {
         std::vector<HostInst> part;
         std::vector<drift_pt_result*> prs;
         int block = 0;
-        if (by_wm.size() == 1 && big.empty()) {
+        if (by_wm.size() == 1 && big.empty() && !tune) {
             for (const HostInst& H : hs) block = std::max(block, H.block_req);
             const int rc = run_lds(hs, rs, block);
             if (rc) return rc;
             break;
         }
         part_of(g.second, part, prs, block);
+        if (tune && block == 0) {
+            BatchShape sh;
+            const int rc = tune_batch(DRIFT_PT_ENGINE_LDS, part, sh);
+            if (rc) return rc;
+            block = sh.block;
+        }
         const int rc = run_lds(part, prs, block);
         if (rc) return rc;
     }
@@ -2270,6 +2411,16 @@ int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs
         std::vector<drift_pt_result*> prs;
         int block = 0;
         part_of(g.second, part, prs, block);
+        bool given = block > 0;
+        for (const HostInst& H : part) given = given || H.rg_req > 0;
+        if (tune && !given) {
+            BatchShape sh;
+            const int rc = tune_batch(DRIFT_PT_ENGINE_SPLIT, part, sh);
+            if (rc) return rc;
+            block = sh.block;
+            if (sh.rg_cap > 0)
+                for (HostInst& H : part) H.rg_req = std::min(sh.rg_cap, split_rg_max(H));
+        }
         const int rc = run_split(part, prs, block);
         if (rc) return rc;
     }

@@ -306,6 +306,7 @@ def parallel_tempering_gpu_batch(
     no_alloc: bool = False,
     targets=None,
     slots_per_group: int = 0,
+    autotune: bool = False,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
@@ -317,7 +318,9 @@ def parallel_tempering_gpu_batch(
     launch when their replica groups fill the chip (``slots_per_group`` sets the group size, 0:
     the library's choice), else on the per-launch engine, four at a time on streams of their own.
     ``targets`` (one energy or None per instance) stops each instance at its target, as
-    ``target_energy`` does for one."""
+    ``target_energy`` does for one.  ``autotune`` times the launch shapes of each persistent
+    launch on the batch itself (block size; for the split engine also slots per group), keeps the
+    choice for the batch's shape class, and runs with it; results do not depend on it."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -332,7 +335,7 @@ def parallel_tempering_gpu_batch(
         prms.append(_params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seeds[i], b,
                             "lds", block_threads, return_final, record_history, record_trace,
                             target=None if targets is None else targets[i],
-                            slots_per_group=slots_per_group))
+                            slots_per_group=slots_per_group, autotune=autotune))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])

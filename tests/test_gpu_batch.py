@@ -120,3 +120,41 @@ def test_full_c4_batch_properties():
         assert np.array_equal(alone.best_s, res[i].best_s)
         assert np.array_equal(alone.final_spins, res[i].final_spins)
         assert np.array_equal(alone.rung_of, res[i].rung_of)
+
+
+def test_batch_autotune_keeps_results():
+    """The batch autotuner times the LDS launch's block sizes on the batch itself and runs with
+    its choice; the results equal the untuned batch's bit for bit, and the oracle's."""
+    count, R, n_rounds = 80, 16, 12
+    sizes = _uneven_sizes(count, seed=5, near_top=(2048, 2040))
+    _announce(f"batch autotune, {count} instances")
+    gpu.autotune_clear()
+    models = [random_regular_gaussian(n, seed=30_000 + i) for i, n in enumerate(sizes)]
+    kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, n_rounds=n_rounds, sweeps_per_round=1,
+              return_final=True, record_history=True)
+    plain = gpu.parallel_tempering_gpu_batch(models, **kw)
+    tuned = gpu.parallel_tempering_gpu_batch(models, autotune=True, **kw)
+    assert tuned[0].block_threads in (256, 512, 1024)
+    for a, b in zip(plain, tuned):
+        assert a.best_E_units == b.best_E_units and np.array_equal(a.best_s, b.best_s)
+        assert np.array_equal(a.final_spins, b.final_spins) and np.array_equal(a.rung_of, b.rung_of)
+    beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+    for i in (0, count - 1):
+        _oracle_equal(models[i], tuned[i], seed=i, R=R, n_rounds=n_rounds, S=1, beta=beta)
+
+
+def test_batch_autotune_split_launch_keeps_results():
+    """Large instances go to the split engine; the tuner picks a group size and block size on
+    the batch and the results equal those of the library's default launch shape."""
+    count, R, n_rounds = 12, 32, 6
+    _announce(f"batch autotune on the split engine, {count} instances")
+    gpu.autotune_clear()
+    models = [random_regular_gaussian(12000 + 64 * i, seed=40_000 + i) for i in range(count)]
+    kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, n_rounds=n_rounds, sweeps_per_round=2,
+              return_final=True)
+    plain = gpu.parallel_tempering_gpu_batch(models, **kw)
+    tuned = gpu.parallel_tempering_gpu_batch(models, autotune=True, **kw)
+    assert plain[0].engine == tuned[0].engine == "split"
+    for a, b in zip(plain, tuned):
+        assert a.best_E_units == b.best_E_units and np.array_equal(a.best_s, b.best_s)
+        assert np.array_equal(a.final_spins, b.final_spins) and np.array_equal(a.rung_of, b.rung_of)
