@@ -688,8 +688,9 @@ def c5_eval(args) -> dict:
                                "(3-D +/-J glass at the nearest cube, 3-regular N(0,1), sparse ER "
                                "mean degree 3..12), R=32",
                    "per_family": bk["per_family"], "cap_sweeps": cap,
-                   "target": "best-known energy: lowest of four independent runs of ten times the "
-                             "cap (" + bk["source"] + ")", "net": os.path.basename(args.net)},
+                   "target": f"best-known energy: lowest of four independent runs of "
+                             f"{10 * bk['cap_sweeps']} sweeps (" + bk["source"] + ")",
+                   "net": os.path.basename(args.net)},
         "wall_s": {k: round(v, 3) for k, v in walls.items()},
         "speedup_ladder_only": round(walls["geo_batched"] / max(walls["learned_batched"], 1e-9), 3),
         "speedup_batching_only": round(walls["geo_single"] / max(walls["geo_batched"], 1e-9), 3),

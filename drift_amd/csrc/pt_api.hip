@@ -147,19 +147,21 @@ struct HostInst {
     int block_req = 0;              // drift_pt_params::block_threads (0: the library chooses)
 };
 
-// First-chunk width of the LDS engine's quad gather: the smallest of 2, 3, 4 that covers at least
-// three rows in four, else 4.  A wider chunk costs issue slots on every shorter row; a row longer
-// than the chunk costs one more wait on memory.  Wider chunks would hold more than 128 registers
-// per lane (four rows of edge records and spins) and spill.
-int quad_chunk(const std::vector<int32_t>& row_ptr, int n) {
-    int cnt[5] = {0};
-    for (int p = 0; p < n; ++p) cnt[std::min(row_ptr[p + 1] - row_ptr[p], 4)]++;
+// First-chunk width of the LDS and split engines' quad gather: the smallest of 2, 3, 4 (and 6 for
+// int8 couplings, whose records are one word) that covers at least three rows in four, else the
+// widest.  A wider chunk costs issue slots on every shorter row; a row longer than the chunk costs
+// one more wait on memory.  Wider chunks would hold more than 128 registers per lane (four rows
+// of edge records and spins) and spill: width 8 on int8 records spilled 40 bytes per lane.
+int quad_chunk(const std::vector<int32_t>& row_ptr, int n, int wm) {
+    const int top = wm == PT_WM_I8 ? 6 : 4;
+    int cnt[7] = {0};
+    for (int p = 0; p < n; ++p) cnt[std::min(row_ptr[p + 1] - row_ptr[p], top)]++;
     int covered = 0;
-    for (int d = 0; d <= 4; ++d) {
+    for (int d = 0; d <= top; ++d) {
         covered += cnt[d];
-        if (d >= 2 && 4 * (int64_t)covered >= 3 * (int64_t)n) return d;
+        if (d >= 2 && d != 5 && 4 * (int64_t)covered >= 3 * (int64_t)n) return d;
     }
-    return 4;
+    return top;
 }
 
 int32_t greedy_color(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* cptr,
@@ -328,7 +330,7 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
         total_abs += field;
     }
     if (!std::isfinite(maxfield)) return fail(DRIFT_PT_ERANGE, "non-finite coupling or field");
-    H.gq = quad_chunk(H.row_ptr, n);
+    H.gq = quad_chunk(H.row_ptr, n, H.wm);
     for (int p = 0; p < n; ++p) H.maxdeg = std::max(H.maxdeg, H.row_ptr[p + 1] - H.row_ptr[p]);
     if (isf) {
         if (maxfield >= 16384.0 || total_abs >= 1.0e8)

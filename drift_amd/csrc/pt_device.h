@@ -325,16 +325,53 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
 #pragma unroll
     for (int k = 0; k < 4; ++k) on[k] = 4 * q + k >= cs && 4 * q + k < ce;
     if (rest) {
+        // the rows' remainders in chunks, the four rows' chunks together: each pass issues every
+        // row's next records at once and waits on memory once, then on LDS once, so a quad pays
+        // one round trip per pass instead of one per row and chunk.  Each row still sums its
+        // terms in row order, one rounding each, as field_float / field_int do.
         int rb[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) rb[k] = ((const PT_GLOBAL int32_t*)rp)[min(4 * q + k, n)];
+        int t[4], e[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (on[k] && rb[k] + G < rb[k + 1]) {
-                if constexpr (WM == PT_WM_F32)
-                    l[k] = field_float<G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
-                else
-                    l[k] = field_int<WM, G>(a, Rp, sp, rb[k] + G, rb[k + 1], edge, l[k]);
+            t[k] = rb[k] + G;
+            e[k] = on[k] ? rb[k + 1] : 0;
+        }
+        // RC records per row and pass keep the pass within the register budget (4 RC records,
+        // their positions, weights and spins live at once)
+        constexpr int RC = WM == PT_WM_F32 ? 2 : 3;
+        while ((t[0] < e[0]) | (t[1] < e[1]) | (t[2] < e[2]) | (t[3] < e[3])) {
+            int c[4 * RC];
+            int8_t v[4 * RC];
+            using WT = typename std::conditional<WM == PT_WM_F32, float, int32_t>::type;
+            WT w[4 * RC];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int u = 0; u < RC; ++u) {
+                    const bool ok = t[k] + u < e[k];
+                    if constexpr (WM == PT_WM_F32) c[k * RC + u] = edge_float(edge, ok ? t[k] + u : 0, w[k * RC + u]);
+                    else c[k * RC + u] = edge_int<WM>(edge, ok ? t[k] + u : 0, w[k * RC + u]);
+                    c[k * RC + u] = ok ? c[k * RC + u] : 0;
+                }
+#pragma unroll
+            for (int x = 0; x < 4 * RC; ++x) v[x] = sp[(size_t)c[x] * Rp + a];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                for (int u = 0; u < RC; ++u) {
+                    const bool ok = t[k] + u < e[k];
+                    if constexpr (WM == PT_WM_F32) {
+                        const double nl = l[k] + (double)__uint_as_float(
+                            __float_as_uint(w[k * RC + u]) ^ pt_sign_bit(v[k * RC + u]));
+                        l[k] = ok ? nl : l[k];
+                    } else {
+                        const int32_t term = v[k * RC + u] > 0 ? w[k * RC + u] : -w[k * RC + u];
+                        l[k] += ok ? term : 0;
+                    }
+                }
+                t[k] += RC;
             }
         }
     }

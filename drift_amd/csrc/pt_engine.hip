@@ -198,6 +198,9 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
                         switch (I.gq) {
                             case 2: quads(std::integral_constant<int, 2>{}); break;
                             case 3: quads(std::integral_constant<int, 3>{}); break;
+                            case 6:
+                                if constexpr (WM == PT_WM_I8) quads(std::integral_constant<int, 6>{});
+                                break;
                             default: quads(std::integral_constant<int, 4>{}); break;
                         }
                     }

@@ -106,10 +106,20 @@ def test_replica_counts(R):
 
 
 def _degree_profile_model(kind):
-    """Graphs whose row lengths pick each first-chunk width of the padded edge records (LDS
-    engine 2, 3, 4; word-per-position CSR engine 4, 8) and send long rows through the chunked
-    remainder."""
+    """Graphs whose row lengths pick each first-chunk width of the padded edge records (LDS and
+    split engines 2, 3, 4, and 6 for int8 couplings; word-per-position CSR engine 4, 8) and send
+    long rows through the chunked remainder."""
     rng = np.random.default_rng(11)
+    if kind == "lattice6_int":        # every row 6 long: width 6 (int8), no remainder
+        return cubic_pmj_glass(4, 3)
+    if kind == "er_int_dense":        # mean degree ~10: width 6 plus remainder chunks
+        n = 90
+        J = np.zeros((n, n))
+        iu = np.triu_indices(n, 1)
+        on = rng.random(len(iu[0])) < 0.11
+        J[iu[0][on], iu[1][on]] = rng.choice([-2.0, -1.0, 1.0, 2.0], size=int(on.sum()))
+        J = J + J.T
+        return IsingModel(J, np.zeros(n))
     if kind == "ring_int":            # every row 2 long, two isolated spins
         n = 50
         J = np.zeros((n, n))
@@ -137,8 +147,9 @@ def _degree_profile_model(kind):
     return IsingModel(J.astype(np.float32).astype(np.float64), np.zeros(n))
 
 
-@pytest.mark.parametrize("engine", ["lds", "global"])
-@pytest.mark.parametrize("kind", ["ring_int", "cubic_float", "er_int_field", "hub_float"])
+@pytest.mark.parametrize("engine", ["lds", "global", "split"])
+@pytest.mark.parametrize("kind", ["ring_int", "cubic_float", "er_int_field", "hub_float",
+                                  "lattice6_int", "er_int_dense"])
 def test_first_chunk_widths(kind, engine):
     _announce(f"first chunk {kind} {engine}")
     model = _degree_profile_model(kind)
