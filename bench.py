@@ -137,11 +137,15 @@ def declared_bytes(model, enc, engine: str, R: int) -> tuple[float, str]:
     deg = np.diff(np.asarray(enc.row_ptr))
     w_bytes = 4.0 if enc.weight.dtype.itemsize == 1 and model.n < (1 << 24) else 8.0
     b_h = 0.0 if enc.h is None else float(enc.h.dtype.itemsize)
-    if engine == "lds":
-        # the first G edge records of every row (G = 2, 3 or 4: the smallest covering three rows
-        # in four), spins in LDS
-        chunk = next((g for g in (2, 3, 4) if np.mean(deg <= g) >= 0.75), 4)
-        b_s, fmt = 1.0, "int8 spins [position][replica] in LDS"
+    if engine in ("lds", "split"):
+        # the first G edge records of every row (G = 2, 3, 4, or 6 for int8 couplings: the
+        # smallest covering three rows in four, else the widest), spins in LDS
+        widths = (2, 3, 4, 6) if w_bytes == 4.0 else (2, 3, 4)
+        chunk = next((g for g in widths if np.mean(deg <= g) >= 0.75), widths[-1])
+        b_s, fmt = 1.0, ("int8 spins [position][replica] in LDS" if engine == "lds" else
+                         "int8 spins [position][slot] of each workgroup's group of replica slots "
+                         "in its LDS; slot energies traded through round-tagged mailboxes in HBM "
+                         "after every round")
     elif engine == "global":
         chunk = 4 if np.mean(deg <= 4) >= 0.75 else 8
         if R <= 32:
@@ -169,7 +173,7 @@ def lds_bytes_per_attempt(kernel: str, model) -> float | None:
         # other neighbour words, d bond words, four 16-byte uniform blocks, one store
         words = 3 + (d - 2) + d
         return (4.0 * words + 64.0 + 4.0) / 16.0
-    if kernel == "pt_lds_kernel":
+    if kernel in ("pt_lds_kernel", "pt_split_kernel"):
         return 1.0 + d + 0.5   # own spin, d neighbour spins, a flip store at most every other attempt
     return None
 
@@ -380,6 +384,59 @@ def bench_c4(steps, warmup, repeats, cpu_budget, count=4096, autotune=False):
     return out
 
 
+C5_BATCH_DESC = ("64 random 3-regular instances of N = 65536 (the held-out size of the learned-ladder "
+                 "evaluation), J ~ N(0,1) and h ~ N(0,0.1^2) in float32, R=32 each, beta 0.2..5.0 "
+                 "geometric; too large for one workgroup, so the batch runs on the split engine in "
+                 "one persistent launch; one sweep and one exchange of every instance per step")
+
+
+def bench_c5_batch(steps, warmup, repeats, cpu_budget, count=64, autotune=False, plain=False):
+    """Batched instances too large for one workgroup's LDS (the split engine)."""
+    from drift_amd import gpu
+    from drift_amd.builders.lattice import random_regular_gaussian
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    log(f"c5-batch: building {count} instances of N = 65536")
+    models = [random_regular_gaussian(65536, seed=60_000 + i) for i in range(count)]
+    R = 32
+    kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, sweeps_per_round=1, autotune=autotune,
+              plain_launch=plain)
+    log(f"c5-batch: warm-up {warmup} steps")
+    gpu.parallel_tempering_gpu_batch(models, n_rounds=max(warmup, 1), **kw)
+    total_n = sum(m.n for m in models)
+    attempts = R * total_n * steps
+    secs, walls = [], []
+    for r in range(repeats):
+        t0 = time.perf_counter()
+        res = gpu.parallel_tempering_gpu_batch(models, n_rounds=steps, **kw)
+        walls.append(time.perf_counter() - t0)
+        secs.append(res[0].seconds)
+        log(f"c5-batch: timed run {r + 1}/{repeats}: {attempts / secs[-1] / 1e9:.1f} attempts/ns "
+            f"({res[0].engine}, {res[0].slots_per_group} slots per workgroup)")
+    med = median(secs)
+    enc0 = gpu.encode_problem(models[0])
+    B, fmt = declared_bytes(models[0], enc0, res[0].engine, R)
+    kern = "pt_split_kernel" if res[0].engine == "split" else kernel_name(res[0].engine, models[0], R)
+    out = {
+        "value": round(attempts / med / 1e9, 3), "steps": steps, "warmup": warmup,
+        "ms_per_step": round(med * 1e3 / steps, 5), "dtype": "f32",
+        "config": {"workload": C5_BATCH_DESC, "instances": count, "total_spins": int(total_n),
+                   "attempts_per_step": R * total_n, "format": fmt, "engine": res[0].engine,
+                   "slots_per_group": res[0].slots_per_group, "block_threads": res[0].block_threads,
+                   "autotune": bool(autotune), "wall_s_median": round(median(walls), 3)},
+        "timing": {"repeats": repeats, "runs_ms": [round(x * 1e3, 4) for x in secs],
+                   "median_ms": round(med * 1e3, 4), "window": "the launch that runs the rounds "
+                   "(initial states drawn by a launch of their own, untimed)"},
+        "roofline": roofline(kern, B, PLAN_BYTES["c4"], attempts, med, models[0]),
+        "cpu_baseline": None,
+    }
+    if cpu_budget > 0:
+        log("c5-batch: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline_batch(models, R, ladder_betas(geometric_ladder(R, 0.2, 5.0)),
+                                                 cpu_budget)
+    return out
+
+
 def bench_single(name, steps, warmup, repeats, cpu_budget, plain, engine="auto", block=0,
                  autotune=False, kernel_timing_steps=200):
     from drift_amd import gpu
@@ -447,7 +504,7 @@ def headline(sub: dict) -> dict:
     }
 
 
-SUB_DEFAULT_STEPS = {"c2": 10000, "c3": 1000, "c1": 20000, "published": 400}
+SUB_DEFAULT_STEPS = {"c2": 10000, "c3": 1000, "c1": 20000, "published": 400, "c5_batch": 200}
 
 
 def bench_throughput(args) -> int:
@@ -455,6 +512,11 @@ def bench_throughput(args) -> int:
     if args.workload == "c4":
         print(json.dumps(headline(bench_c4(args.steps, args.warmup, args.repeats, cpu,
                                            args.instances, args.autotune))), flush=True)
+        return 0
+    if args.workload == "c5-batch":
+        print(json.dumps(headline(bench_c5_batch(args.steps, args.warmup, args.repeats, cpu,
+                                                 autotune=args.autotune,
+                                                 plain=args.plain_launch))), flush=True)
         return 0
     if args.workload in ("c2", "c3", "c1", "published"):
         sub = bench_single(args.workload, args.steps, args.warmup, args.repeats, cpu,
@@ -474,6 +536,9 @@ def bench_throughput(args) -> int:
                               "source": "docs/results/PHASE14-results.md:140-142",
                               "note": "a different card; context, not a target"}
         subs[name] = sub
+    subs["c5_batch"] = bench_c5_batch(args.sub_steps or SUB_DEFAULT_STEPS["c5_batch"],
+                                      min(args.warmup, 20) or 1, args.repeats, cpu,
+                                      autotune=args.autotune, plain=args.plain_launch)
     line["sub_results"] = subs
     line["lds_probe"] = {k: {"GBs": round(v["GBs"], 1),
                              "wave_reads_per_s": float(f"{v['wave_reads_per_s']:.4g}")}
@@ -756,7 +821,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
-                    choices=("all", "c4", "c2", "c3", "c1", "published", "c3-ttt", "ladder-c5"))
+                    choices=("all", "c4", "c2", "c3", "c1", "published", "c5-batch", "c3-ttt",
+                             "ladder-c5"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")

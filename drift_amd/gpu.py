@@ -307,6 +307,7 @@ def parallel_tempering_gpu_batch(
     targets=None,
     slots_per_group: int = 0,
     autotune: bool = False,
+    plain_launch: bool = False,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
@@ -320,7 +321,9 @@ def parallel_tempering_gpu_batch(
     ``targets`` (one energy or None per instance) stops each instance at its target, as
     ``target_energy`` does for one.  ``autotune`` times the launch shapes of each persistent
     launch on the batch itself (block size; for the split engine also slots per group), keeps the
-    choice for the batch's shape class, and runs with it; results do not depend on it."""
+    choice for the batch's shape class, and runs with it; results do not depend on it.
+    ``plain_launch`` starts the split engine's persistent launch with an ordinary dispatch (for
+    profilers that cannot follow cooperative launches; needs the device to itself)."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -335,7 +338,8 @@ def parallel_tempering_gpu_batch(
         prms.append(_params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seeds[i], b,
                             "lds", block_threads, return_final, record_history, record_trace,
                             target=None if targets is None else targets[i],
-                            slots_per_group=slots_per_group, autotune=autotune))
+                            slots_per_group=slots_per_group, autotune=autotune,
+                            plain_launch=plain_launch))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])
