@@ -168,3 +168,24 @@ def test_mailbox_wait_gives_up_from_the_host(monkeypatch):
     monkeypatch.delenv("DRIFT_PT_BARRIER_TIMEOUT_MS")
     g, o, _ = _run_pair(model, R=32, n_rounds=10, S=2, seed=6, rg=1, trace=False)
     _check(g, o, model)
+
+
+def test_batch_at_the_held_out_size_equals_oracle():
+    """Eight random 3-regular instances of N = 65536 (the size of the learned-ladder evaluation's
+    held-out instances), R = 32: 256 groups of one slot, one persistent launch, every instance
+    equal to the oracle bit for bit over 6 rounds of 2 sweeps."""
+    count, R, n_rounds, S = 8, 32, 6, 2
+    _announce(f"batch of {count} instances of N = 65536 on the split engine")
+    models = [random_regular_gaussian(65536, seed=80_000 + i) for i in range(count)]
+    seeds = [17 + i for i in range(count)]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.2, T_max=5.0,
+                                           n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                           return_final=True, record_history=True)
+    beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+    for model, seed, r in zip(models, seeds, res):
+        assert r.engine == "split" and r.slots_per_group == 1
+        enc = gpu.encode_problem(model)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=R, n_rounds=n_rounds,
+                                  sweeps_per_round=S, beta=beta, n_threads=THREADS)
+        _check(r, o, model)
