@@ -33,7 +33,8 @@ RECORD_HISTORY, RECORD_FINAL, SKIP_VALIDATION, RECORD_TRACE, TIME_KERNELS, NO_GR
 AUTOTUNE = 64
 PLAIN_LAUNCH = 128
 STOP_AT_TARGET = 256
-TUNE_MAX = 48
+COUPLINGS_LDS = 512
+TUNE_MAX = 64
 NOALLOC_BYTES = 64 << 20   # DRIFT_PT_NOALLOC_BYTES
 
 
@@ -99,6 +100,7 @@ class Result(ctypes.Structure):
         ("rounds_run", c_int32),
         ("block_threads_used", c_int32),
         ("slots_per_group_used", c_int32),
+        ("couplings_in_lds_used", c_int32),
     ]
 
 
@@ -109,11 +111,13 @@ class Tuning(ctypes.Structure):
         ("engine", c_int32),
         ("block_threads", c_int32),
         ("slots_per_group", c_int32),
+        ("couplings_in_lds", c_int32),
         ("trial_rounds", c_int32),
         ("n_candidates", c_int32),
         ("cand_engine", c_int32 * TUNE_MAX),
         ("cand_block", c_int32 * TUNE_MAX),
         ("cand_group", c_int32 * TUNE_MAX),
+        ("cand_lds_couplings", c_int32 * TUNE_MAX),
         ("cand_seconds", c_double * TUNE_MAX),
     ]
 

@@ -144,6 +144,7 @@ struct HostInst {
     int has_target = 0;             // DRIFT_PT_STOP_AT_TARGET: stop at best <= target_units
     int64_t target_units = 0;
     int rg_req = 0;                 // drift_pt_params::slots_per_group (0: the library chooses)
+    bool cl_ok = false;             // couplings fit 16-bit LDS records (int8, |w| <= 7, n <= 4096)
     int block_req = 0;              // drift_pt_params::block_threads (0: the library chooses)
 };
 
@@ -331,6 +332,13 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     }
     if (!std::isfinite(maxfield)) return fail(DRIFT_PT_ERANGE, "non-finite coupling or field");
     H.gq = quad_chunk(H.row_ptr, n, H.wm);
+    if (H.wm == PT_WM_I8 && n <= 4096) {
+        H.cl_ok = true;
+        for (size_t t = 0; t < H.edge.size() && H.cl_ok; ++t) {
+            const int w = (int)(int8_t)(H.edge[t] & 0xffu);
+            H.cl_ok = w >= -7 && w <= 7;
+        }
+    }
     for (int p = 0; p < n; ++p) H.maxdeg = std::max(H.maxdeg, H.row_ptr[p + 1] - H.row_ptr[p]);
     if (isf) {
         if (maxfield >= 16384.0 || total_abs >= 1.0e8)
@@ -547,12 +555,16 @@ std::vector<T> pad4(const std::vector<T>& v) {
 // rgs (split engine): slots per workgroup of each instance; the first-chunk records then hold
 // neighbour offsets for groups of that many slots, and img holds one image per group, and each
 // instance gets its energy mailboxes (all bits set).
+int split_thr_words(const HostInst& H, int rg);
+
+// cl (split engine): also each instance's CSR as 16-bit records (position << 4 | weight & 15).
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
-                bool force_final, bool with_ell, const std::vector<int>* rgs = nullptr) {
+                bool force_final, bool with_ell, const std::vector<int>* rgs = nullptr,
+                bool cl = false) {
     const size_t m = hs.size();
     auto rg_of = [&](size_t i) { return rgs ? (*rgs)[i] : hs[i].Rp; };
     auto ng_of = [&](size_t i) { return rgs ? (hs[i].R + (*rgs)[i] - 1) / (*rgs)[i] : 1; };
-    struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb; };
+    struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb, crec; };
     std::vector<Off> off(m);
     Packer in;
     for (size_t i = 0; i < m; ++i) {
@@ -569,6 +581,13 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         o.beta = in.add(H.beta);
         o.thr = H.wm != PT_WM_F32 ? in.add(H.thr) : SIZE_MAX;
         o.jb = H.ldim ? in.add(H.jbits) : SIZE_MAX;
+        o.crec = SIZE_MAX;
+        if (cl) {
+            std::vector<uint16_t> rec(H.edge.size());
+            for (size_t t = 0; t < rec.size(); ++t)
+                rec[t] = (uint16_t)(((H.edge[t] >> 8) << 4) | (H.edge[t] & 0xfu));
+            o.crec = in.add(rec);
+        }
     }
     // outputs after the inputs
     struct OutOff { size_t best, fin, hist, trace, out, img, e0, box; };
@@ -586,7 +605,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
     }
     const size_t out_end = cursor;   // statistics blocks are the region that must start at zero
     for (size_t i = 0; i < m; ++i)
-        oo[i].box = rgs ? take(sizeof(uint64_t) * 4 * (size_t)hs[i].Rp) : SIZE_MAX;
+        oo[i].box = rgs ? take(sizeof(uint64_t) * (4 * (size_t)hs[i].Rp + 2)) : SIZE_MAX;
     const size_t box_end = cursor;   // mailboxes start with every bit set
     for (size_t i = 0; i < m; ++i) {
         const HostInst& H = hs[i];
@@ -639,6 +658,9 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.box = oo[i].box == SIZE_MAX ? nullptr : (uint64_t*)(base + oo[i].box);
         I.rg = rg_of(i);
         I.ng = ng_of(i);
+        I.crec = o.crec == SIZE_MAX ? nullptr : (const uint16_t*)(base + o.crec);
+        I.nnz = H.nnz;
+        I.thr_lds = rgs && split_thr_words(H, rg_of(i)) > 0 ? 1 : 0;
     }
     return 0;
 }
@@ -658,7 +680,7 @@ void clear_result(drift_pt_result* r) {
     r->engine_used = 0; r->seconds = 0.0; r->attempts = 0.0;
     r->kernel_seconds = 0.0; r->kernel_attempts = 0.0; r->kernel_launches = 0;
     r->other_kernel_seconds = 0.0; r->other_kernel_launches = 0; r->rounds_run = 0;
-    r->block_threads_used = 0; r->slots_per_group_used = 0;
+    r->block_threads_used = 0; r->slots_per_group_used = 0; r->couplings_in_lds_used = 0;
 }
 
 // Copy results of one instance back and undo the colour relabelling.
@@ -744,7 +766,7 @@ int run_lds(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>
     int rc = upload_many(A, hs, insts, false, true);
     if (rc) return rc;
     for (int i = 0; i < n_inst; ++i) {
-        lds = std::max(lds, pt_lds_bytes(hs[i].n, hs[i].Rp));
+        lds = std::max(lds, pt_lds_bytes(hs[i].n, hs[i].Rp, hs[i].k));
         maxRp = std::max(maxRp, hs[i].Rp);
     }
     if (lds + 1024 > (size_t)device_lds_limit())
@@ -862,27 +884,45 @@ int device_cus() {
 
 // Largest power-of-two number of slots per workgroup (at most 64 and Rp) whose spins and ladder
 // arrays fit one workgroup's LDS; 0 when not even one slot fits.
-int split_rg_max(const HostInst& H) {
+// Threshold words a split workgroup of rg slots stages in LDS each round (integer couplings,
+// when they fit 16 KB), or 0.
+int split_thr_words(const HostInst& H, int rg) {
+    return H.wm != PT_WM_F32 && (int64_t)rg * H.TQ * 4 <= 16384 ? rg * H.TQ : 0;
+}
+
+size_t split_lds(const HostInst& H, int rg, bool cl) {
+    return pt_split_lds_bytes(H.n, rg, H.Rp, H.k, split_thr_words(H, rg), cl, H.nnz, H.has_h);
+}
+
+int split_rg_max(const HostInst& H, bool cl = false) {
+    if (cl && !H.cl_ok) return 0;
     for (int rg = std::min(64, H.Rp); rg >= 1; rg >>= 1)
-        if (pt_split_lds_bytes(H.n, rg, H.Rp) + 1024 <= (size_t)device_lds_limit()) return rg;
+        if (split_lds(H, rg, cl) + 1024 <= (size_t)device_lds_limit()) return rg;
     return 0;
 }
 
 bool split_fits(const HostInst& H) { return split_rg_max(H) >= 1; }
+
+// Couplings in LDS for a launch: asked for (DRIFT_PT_COUPLINGS_LDS) by every instance.
+bool wants_cl(const std::vector<HostInst>& hs) {
+    for (const HostInst& H : hs)
+        if (!(H.flags & DRIFT_PT_COUPLINGS_LDS)) return false;
+    return !hs.empty();
+}
 
 // Slots per workgroup for a set of instances run together: the requested size where one is
 // given, else the largest common cap c (a power of two) under which the groups number at least
 // two per compute unit, each instance taking min(c, its largest fitting size); c = 1 when no cap
 // reaches that.  Larger groups share each position's edge records among more lanes; more groups
 // keep more compute units busy.
-std::vector<int> split_rgs(const std::vector<HostInst>& hs) {
+std::vector<int> split_rgs(const std::vector<HostInst>& hs, bool cl = false) {
     std::vector<int> out(hs.size(), 1);
     const int64_t want = 2 * (int64_t)device_cus();
     int cap = 1;
     for (int c = 64; c >= 1; c >>= 1) {
         int64_t groups = 0;
         for (const HostInst& H : hs) {
-            const int rg = H.rg_req > 0 ? H.rg_req : std::min(c, split_rg_max(H));
+            const int rg = H.rg_req > 0 ? H.rg_req : std::max(1, std::min(c, split_rg_max(H, cl)));
             groups += (H.R + rg - 1) / rg;
         }
         if (groups >= want) {
@@ -891,7 +931,7 @@ std::vector<int> split_rgs(const std::vector<HostInst>& hs) {
         }
     }
     for (size_t i = 0; i < hs.size(); ++i)
-        out[i] = hs[i].rg_req > 0 ? hs[i].rg_req : std::max(1, std::min(cap, split_rg_max(hs[i])));
+        out[i] = hs[i].rg_req > 0 ? hs[i].rg_req : std::max(1, std::min(cap, split_rg_max(hs[i], cl)));
     return out;
 }
 
@@ -913,18 +953,24 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
               int block_req) {
     DevArena A;
     const int n_inst = (int)hs.size();
-    const std::vector<int> rgs = split_rgs(hs);
+    if (n_inst >= (1 << 21)) return fail(DRIFT_PT_EINVAL, "split engine: at most 2^21 instances per batch");
+    const bool cl = wants_cl(hs);
+    for (const HostInst& H : hs)
+        if (cl && !H.cl_ok)
+            return fail(DRIFT_PT_EINVAL, "couplings in LDS need int8 couplings with |w| <= 7 and "
+                                         "n <= 4096");
+    const std::vector<int> rgs = split_rgs(hs, cl);
     size_t lds = 0;
     int max_ng = 1;
     bool plain = false;
     for (int i = 0; i < n_inst; ++i) {
         const HostInst& H = hs[i];
         const int rg = rgs[i];
-        if (rg < 1 || rg > H.Rp || rg > split_rg_max(H))
+        if (rg < 1 || rg > H.Rp || rg > split_rg_max(H, cl))
             return fail(DRIFT_PT_EINVAL, "split engine: " + std::to_string(rg) +
                                              " slots per workgroup do not fit one workgroup's LDS "
                                              "(or exceed the padded replica count)");
-        lds = std::max(lds, pt_split_lds_bytes(H.n, rg, H.Rp));
+        lds = std::max(lds, split_lds(H, rg, cl));
         max_ng = std::max(max_ng, (H.R + rg - 1) / rg);
         if (H.flags & DRIFT_PT_PLAIN_LAUNCH) plain = true;
         if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "split engine: one weight type per launch");
@@ -932,7 +978,7 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
     const int block = split_block(lds, block_req);
     const int wm = hs[0].wm;
     int per_cu = 0;
-    PT_HIP(pt_split_occupancy(wm, block, lds, &per_cu));
+    PT_HIP(pt_split_occupancy(wm, block, lds, &per_cu, cl));
     const int capacity = per_cu * device_cus();
     if (capacity < max_ng)
         return fail(DRIFT_PT_EINVAL, "split engine: the " + std::to_string(max_ng) +
@@ -940,7 +986,7 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
                                          std::to_string(capacity) + " workgroups of " +
                                          std::to_string(block) + " threads)");
     std::vector<PtInstance> insts((size_t)n_inst);
-    int rc = upload_many(A, hs, insts, false, true, &rgs);
+    int rc = upload_many(A, hs, insts, false, true, &rgs, cl);
     if (rc) return rc;
     // largest work first; the groups of one instance consecutive
     std::vector<int32_t> order((size_t)n_inst);
@@ -969,10 +1015,10 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
     PT_HIP(hipMemsetAsync(d_queue, 0, 2 * sizeof(int32_t), sg.s));
     PT_HIP(hipMemsetAsync(d_abort, 0, sizeof(int32_t), sg.s));
     PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue, wm, block, grid, lds, true, SA,
-                           false, sg.s));
+                           false, sg.s, cl));
     PT_HIP(hipEventRecord(sg.e0, sg.s));
     PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue + 1, wm, block, grid, lds, false, SA,
-                           !plain, sg.s));
+                           !plain, sg.s, cl));
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
@@ -1013,6 +1059,7 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
         results[i]->engine_used = DRIFT_PT_ENGINE_SPLIT;
         results[i]->block_threads_used = block;
         results[i]->slots_per_group_used = rgs[i];
+        results[i]->couplings_in_lds_used = cl ? 1 : 0;
         results[i]->kernel_seconds = ms * 1e-3;
         results[i]->kernel_launches = 1;
         results[i]->kernel_attempts = total;
@@ -1877,7 +1924,7 @@ int run_pslab(const HostInst& H, drift_pt_result* res, int block_req) {
 
 // LDS engine fit: the dynamic area plus ~1 KB of the kernel's static shared variables.
 bool fits_lds(const HostInst& H) {
-    return pt_lds_bytes(H.n, H.Rp) + 1024 <= (size_t)device_lds_limit();
+    return pt_lds_bytes(H.n, H.Rp, H.k) + 1024 <= (size_t)device_lds_limit();
 }
 
 // Run one prepared instance on a resolved engine (not AUTO).
@@ -1956,7 +2003,7 @@ int default_engine(const HostInst& H) {
 using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
 
 struct TuneEntry {
-    int engine = 0, block = 0, rg = 0;
+    int engine = 0, block = 0, rg = 0, cl = 0;
 };
 
 std::mutex g_tune_mu;
@@ -1967,6 +2014,7 @@ using BatchKey = std::tuple<int, int, int, int, int, int, int, int>;
 
 struct BatchShape {
     int block = 0, rg_cap = 0;   // rg_cap: split engine, slots per group min(rg_cap, largest fit)
+    int cl = 0;                  // split engine: couplings in LDS
 };
 
 std::mutex g_btune_mu;
@@ -1986,33 +2034,35 @@ TuneKey tune_key(const HostInst& H) {
 // trials favours no candidate.
 int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry& choice) {
     struct Cand {
-        int engine, block, rg;
+        int engine, block, rg, cl;
         double secs;
     };
     std::vector<Cand> cands;
-    cands.push_back({default_engine(H), 0, 0, -1.0});
+    cands.push_back({default_engine(H), 0, 0, 0, -1.0});
     const int blocks[3] = {256, 512, 1024};
     for (int b : blocks) {
-        if (fits_lds(H)) cands.push_back({DRIFT_PT_ENGINE_LDS, b, 0, -1.0});
-        if (H.ldim && slab_fits(H)) cands.push_back({DRIFT_PT_ENGINE_SLAB, b, 0, -1.0});
+        if (fits_lds(H)) cands.push_back({DRIFT_PT_ENGINE_LDS, b, 0, 0, -1.0});
+        if (H.ldim && slab_fits(H)) cands.push_back({DRIFT_PT_ENGINE_SLAB, b, 0, 0, -1.0});
         size_t plds = 0;
         if (H.ldim && pslab_fits(H) && pslab_block(H, b, plds) == b)
-            cands.push_back({DRIFT_PT_ENGINE_PSLAB, b, 0, -1.0});
-        if (H.ldim) cands.push_back({DRIFT_PT_ENGINE_LATTICE, b, 0, -1.0});
-        cands.push_back({DRIFT_PT_ENGINE_GLOBAL, b, 0, -1.0});
+            cands.push_back({DRIFT_PT_ENGINE_PSLAB, b, 0, 0, -1.0});
+        if (H.ldim) cands.push_back({DRIFT_PT_ENGINE_LATTICE, b, 0, 0, -1.0});
+        cands.push_back({DRIFT_PT_ENGINE_GLOBAL, b, 0, 0, -1.0});
         CoopPlan P;
-        if (coop_plan(H, b, P)) cands.push_back({DRIFT_PT_ENGINE_COOP, b, 0, -1.0});
-        // the split engine, every group size whose groups can all be resident
-        for (int rg = split_rg_max(H); rg >= 1; rg >>= 1) {
-            const size_t lds = pt_split_lds_bytes(H.n, rg, H.Rp);
-            int per_cu = 0;
-            if (pt_split_occupancy(H.wm, b, lds, &per_cu) != hipSuccess) {
-                (void)hipGetLastError();
-                continue;
+        if (coop_plan(H, b, P)) cands.push_back({DRIFT_PT_ENGINE_COOP, b, 0, 0, -1.0});
+        // the split engine, every group size whose groups can all be resident, with the
+        // couplings in HBM and (where they qualify) in LDS
+        for (int cl = 0; cl <= (H.cl_ok ? 1 : 0); ++cl)
+            for (int rg = split_rg_max(H, cl); rg >= 1; rg >>= 1) {
+                const size_t lds = split_lds(H, rg, cl);
+                int per_cu = 0;
+                if (pt_split_occupancy(H.wm, b, lds, &per_cu, cl) != hipSuccess) {
+                    (void)hipGetLastError();
+                    continue;
+                }
+                if ((int64_t)per_cu * device_cus() >= (H.R + rg - 1) / rg)
+                    cands.push_back({DRIFT_PT_ENGINE_SPLIT, b, rg, cl, -1.0});
             }
-            if ((int64_t)per_cu * device_cus() >= (H.R + rg - 1) / rg)
-                cands.push_back({DRIFT_PT_ENGINE_SPLIT, b, rg, -1.0});
-        }
     }
     const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
     const int rounds = trial_rounds > 0 ? trial_rounds
@@ -2029,6 +2079,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
     auto trial = [&](Cand& c) {
         double secs = -1.0;
         T.rg_req = c.rg;
+        T.flags = c.cl ? (T.flags | DRIFT_PT_COUPLINGS_LDS) : (T.flags & ~DRIFT_PT_COUPLINGS_LDS);
         T.n_rounds = 2;                                      // warm-up: code objects, first touch
         clear_result(&r);
         if (run_engine(T, c.engine, c.block, &r) == 0) {
@@ -2056,6 +2107,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
     choice.engine = cands[best].engine;
     choice.block = cands[best].block;
     choice.rg = cands[best].rg;
+    choice.cl = cands[best].cl;
     if (table) {
         std::memset(table, 0, sizeof(*table));
         table->trial_rounds = rounds;
@@ -2065,6 +2117,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
             table->cand_engine[slot] = c.engine;
             table->cand_block[slot] = c.block;
             table->cand_group[slot] = c.rg;
+            table->cand_lds_couplings[slot] = c.cl;
             table->cand_seconds[slot] = c.secs;
             ++slot;
         }
@@ -2072,6 +2125,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
         table->engine = choice.engine;
         table->block_threads = choice.block;
         table->slots_per_group = choice.rg;
+        table->couplings_in_lds = choice.cl;
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[tune_key(H)] = choice;
@@ -2145,6 +2199,7 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
             engine = choice.engine;
             block = choice.block;
             H.rg_req = choice.rg;
+            if (choice.cl) H.flags |= DRIFT_PT_COUPLINGS_LDS;
         }
     }
     return run_engine(H, engine, block, result);
@@ -2284,10 +2339,14 @@ int tune_batch(int engine, const std::vector<HostInst>& hs, BatchShape& out) {
             if (ok) cands.push_back({BatchShape{b, 0}, -1.0});
         }
     } else {
-        int top = 1;
-        for (const HostInst& H : hs) top = std::max(top, split_rg_max(H));
-        for (int c = top; c >= 1; c >>= 1)
-            for (int b : blocks) cands.push_back({BatchShape{b, c}, -1.0});
+        bool cl_ok = true;
+        for (const HostInst& H : hs) cl_ok = cl_ok && H.cl_ok;
+        for (int cl = 0; cl <= (cl_ok ? 1 : 0); ++cl) {
+            int top = 1;
+            for (const HostInst& H : hs) top = std::max(top, split_rg_max(H, cl));
+            for (int c = top; c >= 1; c >>= 1)
+                for (int b : blocks) cands.push_back({BatchShape{b, c, cl}, -1.0});
+        }
     }
     int64_t per_round = 0;
     int max_rounds = 0;
@@ -2313,8 +2372,10 @@ int tune_batch(int engine, const std::vector<HostInst>& hs, BatchShape& out) {
     auto timed = [&](const BatchShape& sh, int rounds, double& secs) {
         for (HostInst& H : trial) {
             H.n_rounds = rounds;
-            if (engine == DRIFT_PT_ENGINE_SPLIT)
-                H.rg_req = sh.rg_cap > 0 ? std::min(sh.rg_cap, split_rg_max(H)) : 0;
+            if (engine == DRIFT_PT_ENGINE_SPLIT) {
+                H.rg_req = sh.rg_cap > 0 ? std::max(1, std::min(sh.rg_cap, split_rg_max(H, sh.cl))) : 0;
+                H.flags = sh.cl ? (H.flags | DRIFT_PT_COUPLINGS_LDS) : (H.flags & ~DRIFT_PT_COUPLINGS_LDS);
+            }
         }
         const int rc = engine == DRIFT_PT_ENGINE_LDS ? run_lds(trial, rp, sh.block)
                                                      : run_split(trial, rp, sh.block);
@@ -2420,8 +2481,10 @@ int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs
             const int rc = tune_batch(DRIFT_PT_ENGINE_SPLIT, part, sh);
             if (rc) return rc;
             block = sh.block;
-            if (sh.rg_cap > 0)
-                for (HostInst& H : part) H.rg_req = std::min(sh.rg_cap, split_rg_max(H));
+            for (HostInst& H : part) {
+                if (sh.rg_cap > 0) H.rg_req = std::max(1, std::min(sh.rg_cap, split_rg_max(H, sh.cl)));
+                if (sh.cl) H.flags |= DRIFT_PT_COUPLINGS_LDS;
+            }
         }
         const int rc = run_split(part, prs, block);
         if (rc) return rc;

@@ -339,8 +339,8 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
             e[k] = on[k] ? rb[k + 1] : 0;
         }
         // RC records per row and pass keep the pass within the register budget (4 RC records,
-        // their positions, weights and spins live at once)
-        constexpr int RC = WM == PT_WM_F32 ? 2 : 3;
+        // their positions, weights and spins live at once; 3 spilled on the split kernel)
+        constexpr int RC = 2;
         while ((t[0] < e[0]) | (t[1] < e[1]) | (t[2] < e[2]) | (t[3] < e[3])) {
             int c[4 * RC];
             int8_t v[4 * RC];
@@ -355,8 +355,12 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
                     else c[k * RC + u] = edge_int<WM>(edge, ok ? t[k] + u : 0, w[k * RC + u]);
                     c[k * RC + u] = ok ? c[k * RC + u] : 0;
                 }
+            // all records, then all spins: without the barriers the scheduler pairs each record
+            // with its spin and waits in between
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int x = 0; x < 4 * RC; ++x) v[x] = sp[(size_t)c[x] * Rp + a];
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
 #pragma unroll
@@ -391,6 +395,69 @@ __device__ __forceinline__ int64_t quad_update(int q, int cs, int ce, int n, int
                 own[k * Rp] = (int8_t)-si[k];
                 d += 2 * (int64_t)qi;
             }
+        }
+    }
+    return d;
+}
+
+// The attempts of one Philox quad in one slot with the instance's couplings in LDS (split engine,
+// int8 couplings with |w| <= 7 and n <= 4096): row pointers rpl (n + 1), 16-bit records recl
+// (position << 4 | weight & 15, in row order), field hl (n, or null).  Every gather is an LDS
+// read: the four rows' records are taken four per row and pass, all rows together, then their
+// spins.  The decisions are quad_update's on the integer path (an integer field is exact in any
+// order of summation).
+__device__ __forceinline__ int64_t quad_update_cl(int q, int cs, int ce, int a, int Rp, int8_t* sp,
+                                                  const int32_t* rpl, const uint16_t* recl,
+                                                  const int32_t* hl, const pt_u4& rv,
+                                                  const uint32_t* thr_row, int TQ) {
+    int8_t* own = sp + (uint32_t)(4 * q * Rp + a);
+    int8_t si[4];
+    int32_t l[4];
+    int t[4], e[4];
+    bool on[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = 4 * q + k;
+        on[k] = p >= cs && p < ce;
+        si[k] = own[k * Rp];
+        t[k] = on[k] ? rpl[p] : 0;
+        e[k] = on[k] ? rpl[p + 1] : 0;
+        l[k] = (on[k] && hl) ? hl[p] : 0;
+    }
+    while ((t[0] < e[0]) | (t[1] < e[1]) | (t[2] < e[2]) | (t[3] < e[3])) {
+        uint32_t r[16];
+        int8_t v[16];
+        // past a row's end a lane reads record 0 (it exists whenever some row is not done) and
+        // drops its term
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r[4 * k + u] = recl[t[k] + u < e[k] ? t[k] + u : 0];
+        // the scheduler otherwise pairs each record load with its spin load and waits between
+        // them (16 LDS round trips per pass instead of 2)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int x = 0; x < 16; ++x) v[x] = sp[(r[x] >> 4) * (uint32_t)Rp + a];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int32_t w = (int32_t)(r[4 * k + u] << 28) >> 28;
+                const int32_t term = v[4 * k + u] > 0 ? w : -w;
+                l[k] += t[k] + u < e[k] ? term : 0;
+            }
+            t[k] += 4;
+        }
+    }
+    int64_t d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!on[k]) continue;
+        const int32_t qi = si[k] > 0 ? l[k] : -l[k];
+        if (qi <= 0 || (qi <= TQ && pt_word(rv, k) < thr_row[qi - 1])) {
+            own[k * Rp] = (int8_t)-si[k];
+            d += 2 * (int64_t)qi;
         }
     }
     return d;

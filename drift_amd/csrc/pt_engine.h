@@ -64,9 +64,16 @@ struct PtInstance {
     // group may hold padding slots); img holds ng images of n * rg bytes (rounded up to 16), each
     // group's spins [position][slot].  box: [round parity][Rp slots][2] energy mailboxes, the low
     // and the high 32 bits of a slot's energy after a round, each word under the round's tag
-    // (pt_split_word); all bits set at the start.  Other engines: rg = Rp, ng = 1, box null.
+    // (pt_split_word), then two round words (by parity) that group 0 tags once it has read every
+    // slot's words; all bits set at the start.  Other engines: rg = Rp, ng = 1, box null.
     uint64_t* box;
     int32_t rg, ng;
+    // Couplings in LDS (split engine, int8 couplings with |w| <= 7, n <= 4096): the CSR as 16-bit
+    // records, position << 4 | (weight & 15), in row order; null otherwise.
+    const uint16_t* crec;
+    int32_t nnz;
+    int32_t thr_lds;          // split engine, integer couplings: each round's threshold rows of the
+                              // group's slots are copied into LDS (rg * TQ words fit 16 KB)
 };
 
 // Split engine kernel arguments: the abort flag (set when a mailbox wait exceeds timeout_ticks;
@@ -179,7 +186,9 @@ hipError_t pt_launch_pslab_nbr(const PtInstance& I, const SState& S, const CoopA
 hipError_t pt_launch_pslab(const PtInstance& I, const SState& S, const CoopArgs& C, int block,
                            size_t lds, bool cooperative, hipStream_t s);
 
-size_t pt_lds_bytes(int n, int Rp);
+// LDS engine: dynamic LDS of one workgroup (spins of all Rp slots, ladder arrays, the k + 1
+// colour-class bounds).
+size_t pt_lds_bytes(int n, int Rp, int k);
 
 // Error reporting shared by every entry point of the library (drift_pt_last_error).
 int pt_fail(int code, const char* msg);
@@ -195,18 +204,23 @@ hipError_t pt_launch_lds(const PtInstance* d_insts, const int32_t* d_order, int 
 __host__ __device__ inline size_t pt_lds_img_bytes(int n, int Rp) { return ((size_t)n * Rp + 15) & ~(size_t)15; }
 
 // Split engine: dynamic LDS of one workgroup holding rg of an instance's Rp slots (its spins,
-// and the ladder arrays of all Rp slots).
-size_t pt_split_lds_bytes(int n, int rg, int Rp);
+// and the ladder arrays of all Rp slots) and, with couplings in LDS, the row pointers, the 16-bit
+// records and the field.
+// thr_words: the group's threshold rows staged in LDS (rg * TQ, or 0).
+size_t pt_split_lds_bytes(int n, int rg, int Rp, int k, int thr_words, bool cl = false, int nnz = 0,
+                          bool has_h = false);
 // One persistent launch of the split engine over work items items[0 .. n_items) (instance index
 // << 10 | group), taken from the queue *d_queue (start at 0), groups of one instance consecutive.
 // init: each item draws its group's initial spins and energies (no waiting); otherwise the groups
 // of an instance run every round together, so every group of the largest instance must be
 // resident at once: grid <= the co-resident capacity (pt_split_occupancy), cooperative unless
 // plain.
+// cl: every instance's couplings staged in LDS (PtInstance::crec; int8 couplings only).
 hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, int n_items,
                            int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
-                           bool init, const SplitArgs& A, bool cooperative, hipStream_t s);
-hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu);
+                           bool init, const SplitArgs& A, bool cooperative, hipStream_t s,
+                           bool cl = false);
+hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu, bool cl = false);
 
 hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
                             hipStream_t s);

@@ -47,13 +47,22 @@ __device__ __forceinline__ uint64_t pt_split_word(uint32_t tag, uint32_t half) {
 }
 
 // Split engine, after round rnd's sweeps: publish this group's slots' energies and gather every
-// slot's into E (wave 0 polls; one lane per slot).  False when a wait exceeded its bound or
+// slot's into E (wave 0 reads; one lane per slot).  False when a wait exceeded its bound or
 // another workgroup gave up (the abort flag is then set, and every workgroup leaves).
-__device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, int a0, int rg,
-                             int64_t* E, LShared* sc) {
+// Instances of many groups and slots (ng * R >= PT_SPLIT_HOP2) gather in two hops: group 0 alone
+// polls every slot's words, then stores the round's tag into the round word (box[4 Rp + parity]);
+// the other groups wait on that one word and then read each slot's words once.  Every group
+// polling every slot would put ng * 2R coherent loads per round on the same few cache lines
+// (131 000 for R = 256 in 256 groups).  The tags are checked in both cases, so a group never takes
+// another round's energy whatever order the stores become visible in.
+#define PT_SPLIT_HOP2 1024
+__device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, int grp, int a0,
+                             int rg, int64_t* E, LShared* sc) {
     const int tid = threadIdx.x, R = I.R;
     uint64_t* box = I.box + (size_t)(rnd & 1) * 2 * I.Rp;
+    uint64_t* round_word = I.box + (size_t)4 * I.Rp + (rnd & 1);
     const uint32_t tag = (uint32_t)(rnd + 1) & 0xffffu;
+    const bool hop2 = I.ng * R >= PT_SPLIT_HOP2;
     if (tid < rg && a0 + tid < R) {
         const uint64_t e = (uint64_t)E[a0 + tid];
         st_u64(box + 2 * (a0 + tid), pt_split_word(tag, (uint32_t)e));
@@ -62,6 +71,24 @@ __device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, i
     if (tid < PT_WAVE) {
         bool ok = true;
         const unsigned long long t0 = (unsigned long long)wall_clock64();
+        auto give_up = [&]() {
+            if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
+                __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return true;
+            }
+            return false;
+        };
+        if (hop2 && grp != 0) {
+            // the round word first: group 0 stores it once it has read every slot's words
+            while ((uint32_t)(ld_u64(round_word) >> 48) != tag) {
+                if (give_up()) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
         for (int x = tid; x < R && ok; x += PT_WAVE) {
             if (x >= a0 && x < a0 + rg) continue;
             for (;;) {
@@ -70,15 +97,14 @@ __device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, i
                     E[x] = (int64_t)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull));
                     break;
                 }
-                if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                    (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
-                    __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (give_up()) {
                     ok = false;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
+        if (hop2 && grp == 0 && ok && tid == 0) st_u64(round_word, pt_split_word(tag, 0u));
         if (!ok) sc->abort = 1;
     }
     __syncthreads();
@@ -94,7 +120,9 @@ __device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, i
 // and every group then takes the same round-end decisions (best record, exchanges, labels) from
 // the same inputs.  The group holding the best slot copies its state; group 0 writes the history,
 // the trace and the statistics.
-template <int WM, bool INIT, bool SPLIT>
+// CL (split engine, int8 couplings): the instance's row pointers, 16-bit records and field are
+// copied into LDS after the ladder arrays, and the sweeps gather from there (quad_update_cl).
+template <int WM, bool INIT, bool SPLIT, bool CL = false>
 __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* sc,
                              long long* scrE, int* scrR, const SplitArgs& SA) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -113,7 +141,15 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
     int32_t* rep = (int32_t*)(smem + off);    off += 4 * (size_t)Rp;
     int32_t* label = (int32_t*)(smem + off);  off += 4 * (size_t)Rp;
     int32_t* trips = (int32_t*)(smem + off);  off += 4 * (size_t)Rp;
-    float* beta_f = (float*)(smem + off);
+    float* beta_f = (float*)(smem + off);     off += 4 * (size_t)Rp;
+    off = (off + 15) & ~(size_t)15;
+    int32_t* cptr = (int32_t*)(smem + off);   off += ((size_t)4 * (k + 1) + 15) & ~(size_t)15;
+    uint32_t* thr_l = (uint32_t*)(smem + off);
+    const bool thr_in_lds = SPLIT && WM != PT_WM_F32 && I.thr_lds != 0;
+    if (thr_in_lds) off += ((size_t)4 * rg * I.TQ + 15) & ~(size_t)15;
+    int32_t* rpl = (int32_t*)(smem + off);    off += ((size_t)4 * (n + 1) + 15) & ~(size_t)15;
+    uint16_t* recl = (uint16_t*)(smem + off); off += ((size_t)2 * I.nnz + 15) & ~(size_t)15;
+    int32_t* hl = I.h ? (int32_t*)(smem + off) : nullptr;
 
     const int al = tid % rg, qo = tid / rg, qs = nt / rg;
     const int a = a0 + al;                     // the slot this thread sweeps
@@ -150,6 +186,13 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
     if (tid == 0) {
         sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; sc->abort = 0;
     }
+    for (int x = tid; x <= k; x += nt) cptr[x] = I.color_ptr[x];
+    if constexpr (CL) {
+        for (int x = tid; x <= n; x += nt) rpl[x] = I.row_ptr[x];
+        for (int x = tid; x < I.nnz; x += nt) recl[x] = I.crec[x];
+        if (hl)
+            for (int x = tid; x < n; x += nt) hl[x] = ((const int32_t*)I.h)[x];
+    }
     __syncthreads();
 
     // R <= 64: the round end is wave 0's (WaveLadder), with one workgroup barrier per round.  The
@@ -180,11 +223,32 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
             const int g = live ? rung[a] : 0;
             const float bf = beta_f[g];
             const uint32_t* thr_row = WM == PT_WM_F32 ? nullptr : I.thr + (size_t)g * I.TQ;
+            if (thr_in_lds) {
+                // the threshold rows of this round's rungs of the group's slots, one copy per
+                // round instead of a dependent load from HBM in every colour step
+                const int TQ = I.TQ;
+                for (int idx = tid; idx < rg * TQ; idx += nt) {
+                    const int s = idx / TQ, q = idx - s * TQ;
+                    thr_l[idx] = a0 + s < R ? I.thr[(size_t)rung[a0 + s] * TQ + q] : 0u;
+                }
+                __syncthreads();
+                thr_row = thr_l + (size_t)al * TQ;
+            }
             for (int j = 0; j < I.S; ++j) {
                 const uint32_t t = (uint32_t)(rnd * I.S + j);
                 for (int c = 0; c < k; ++c) {
-                    const int cs = I.color_ptr[c], ce = I.color_ptr[c + 1];
-                    if (live && ce > cs) {
+                    const int cs = cptr[c], ce = cptr[c + 1];
+                    if constexpr (CL) {
+                        if (live && ce > cs) {
+                            const int q1 = (ce - 1) >> 2;
+                            for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
+                                const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t,
+                                                           PT_TAG_METRO, I.k0, I.k1);
+                                dsum += quad_update_cl(qq, cs, ce, al, rg, sp, rpl, recl, hl, rv,
+                                                       thr_row, I.TQ);
+                            }
+                        }
+                    } else if (live && ce > cs) {
                         const int q1 = (ce - 1) >> 2;
                         auto quads = [&](auto gw) {
                             for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
@@ -213,7 +277,7 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
             if constexpr (SPLIT) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             if constexpr (SPLIT) {
-                if (!split_gather(I, SA, rnd, a0, rg, E, sc)) return;
+                if (!split_gather(I, SA, rnd, grp, a0, rg, E, sc)) return;
             }
         }
 
@@ -352,7 +416,7 @@ __global__ __launch_bounds__(1024) void pt_lds_kernel(const PtInstance* __restri
 // Split engine: work items are (instance, group), groups of one instance consecutive in the
 // queue, so a workgroup waiting in a mailbox waits only on items already taken by resident
 // workgroups or on the next ones taken (the grid holds every group of the largest instance).
-template <int WM, bool INIT>
+template <int WM, bool INIT, bool CL>
 __global__ __launch_bounds__(1024) void pt_split_kernel(const PtInstance* __restrict__ insts,
                                                         const int32_t* __restrict__ items,
                                                         int n_items, int32_t* queue, SplitArgs SA) {
@@ -371,7 +435,7 @@ __global__ __launch_bounds__(1024) void pt_split_kernel(const PtInstance* __rest
         if (qi >= n_items) return;
         const int item = items[qi];
         const PtInstance I = insts[item >> 10];
-        lds_instance<WM, INIT, true>(I, item & 1023, smem, &sc, scrE, scrR, SA);
+        lds_instance<WM, INIT, true, CL>(I, item & 1023, smem, &sc, scrE, scrR, SA);
         __syncthreads();
     }
 }
@@ -825,19 +889,30 @@ inline int block_for_slots(int Rp) { return Rp <= 256 ? 256 : ((Rp + 63) / 64) *
 
 }  // namespace
 
-size_t pt_lds_bytes(int n, int Rp) {
-    return (((size_t)n * Rp + 15) & ~(size_t)15) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+size_t pt_lds_bytes(int n, int Rp, int k) {
+    const size_t b = (((size_t)n * Rp + 15) & ~(size_t)15) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+    return ((b + 15) & ~(size_t)15) + (((size_t)4 * (k + 1) + 15) & ~(size_t)15);
 }
 
-size_t pt_split_lds_bytes(int n, int rg, int Rp) {
-    return pt_lds_img_bytes(n, rg) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+size_t pt_split_lds_bytes(int n, int rg, int Rp, int k, int thr_words, bool cl, int nnz,
+                          bool has_h) {
+    size_t b = pt_lds_img_bytes(n, rg) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
+    b = (b + 15) & ~(size_t)15;
+    b += ((size_t)4 * (k + 1) + 15) & ~(size_t)15;
+    b += ((size_t)4 * thr_words + 15) & ~(size_t)15;
+    if (cl) {
+        b += ((size_t)4 * (n + 1) + 15) & ~(size_t)15;
+        b += ((size_t)2 * nnz + 15) & ~(size_t)15;
+        if (has_h) b += (size_t)4 * n;
+    }
+    return b;
 }
 
-template <int WM, bool INIT>
+template <int WM, bool INIT, bool CL>
 static hipError_t split_launch(const PtInstance* d_insts, const int32_t* d_items, int n_items,
                                int32_t* d_queue, int block, int grid, size_t lds_bytes,
                                const SplitArgs& A, bool cooperative, hipStream_t s) {
-    const void* fn = (const void*)pt_split_kernel<WM, INIT>;
+    const void* fn = (const void*)pt_split_kernel<WM, INIT, CL>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     if (cooperative) {
@@ -845,28 +920,35 @@ static hipError_t split_launch(const PtInstance* d_insts, const int32_t* d_items
         void* args[] = {(void*)&d_insts, (void*)&d_items, (void*)&n_items, (void*)&d_queue, (void*)&a};
         return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, (unsigned)lds_bytes, s);
     }
-    pt_split_kernel<WM, INIT><<<grid, block, lds_bytes, s>>>(d_insts, d_items, n_items, d_queue, A);
+    pt_split_kernel<WM, INIT, CL><<<grid, block, lds_bytes, s>>>(d_insts, d_items, n_items, d_queue, A);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, int n_items,
                            int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
-                           bool init, const SplitArgs& A, bool cooperative, hipStream_t s) {
+                           bool init, const SplitArgs& A, bool cooperative, hipStream_t s,
+                           bool cl) {
     if (init) cooperative = false;   // no group waits on another while drawing initial states
+    if (cl) {   // couplings in LDS: int8 couplings only; the init launch draws from the CSR in HBM
+        if (wm != PT_WM_I8) return hipErrorInvalidValue;
+        return init ? split_launch<PT_WM_I8, true, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                    : split_launch<PT_WM_I8, false, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+    }
     if (wm == PT_WM_I8)
-        return init ? split_launch<PT_WM_I8, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
-                    : split_launch<PT_WM_I8, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+        return init ? split_launch<PT_WM_I8, true, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                    : split_launch<PT_WM_I8, false, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
     if (wm == PT_WM_I32)
-        return init ? split_launch<PT_WM_I32, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
-                    : split_launch<PT_WM_I32, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
-    return init ? split_launch<PT_WM_F32, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
-                : split_launch<PT_WM_F32, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+        return init ? split_launch<PT_WM_I32, true, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                    : split_launch<PT_WM_I32, false, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+    return init ? split_launch<PT_WM_F32, true, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
+                : split_launch<PT_WM_F32, false, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
 }
 
-hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu) {
-    const void* fn = wm == PT_WM_I8    ? (const void*)pt_split_kernel<PT_WM_I8, false>
-                     : wm == PT_WM_I32 ? (const void*)pt_split_kernel<PT_WM_I32, false>
-                                       : (const void*)pt_split_kernel<PT_WM_F32, false>;
+hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu, bool cl) {
+    const void* fn = cl                ? (const void*)pt_split_kernel<PT_WM_I8, false, true>
+                     : wm == PT_WM_I8  ? (const void*)pt_split_kernel<PT_WM_I8, false, false>
+                     : wm == PT_WM_I32 ? (const void*)pt_split_kernel<PT_WM_I32, false, false>
+                                       : (const void*)pt_split_kernel<PT_WM_F32, false, false>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, block, lds_bytes);

@@ -64,6 +64,7 @@ class PtResult:
     rounds_run: int = None               # rounds run (fewer than asked when a target was reached)
     block_threads: int = None            # workgroup size of the sweep kernel
     slots_per_group: int = None          # replica slots per workgroup (split and LDS engines)
+    couplings_in_lds: bool = None        # split engine: couplings staged in LDS
 
 
 def _neighbour_access(model):

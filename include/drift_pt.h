@@ -84,11 +84,16 @@ enum drift_pt_flags {
                                      time bound.  For profilers that cannot follow cooperative
                                      launches.  (Environment DRIFT_PT_BARRIER_TIMEOUT_MS sets the
                                      bound of one barrier wait; default 20000.) */
-    DRIFT_PT_STOP_AT_TARGET = 256 /* stop after the first round end whose best energy is at or
+    DRIFT_PT_STOP_AT_TARGET = 256,/* stop after the first round end whose best energy is at or
                                      below params.target_energy (within 1e-9 relative); the result
                                      is then that of a run of result.rounds_run rounds.  Engines
                                      LDS, SPLIT, GLOBAL and LATTICE (AUTO picks among them); the
                                      others refuse it. */
+    DRIFT_PT_COUPLINGS_LDS = 512  /* SPLIT engine: each workgroup also stages its instance's
+                                     couplings in LDS (row pointers, 16-bit records, field), so
+                                     every neighbour gather is an LDS read.  Needs int8 couplings
+                                     with |w| <= 7 and n <= 4096; in a batch, every instance of
+                                     the split launch must ask for it */
 };
 
 /* One Ising instance.  Replaces the problem.bin layout of cuda/ising_pt.cu:40-45 and
@@ -155,19 +160,22 @@ typedef struct drift_pt_result {
     int32_t block_threads_used; /* workgroup size of the sweep kernel */
     int32_t slots_per_group_used; /* replica slots per workgroup (the SPLIT engine's groups; the
                                      padded slot count on the LDS engine; 0 on the others) */
+    int32_t couplings_in_lds_used; /* 1 when the couplings were staged in LDS */
 } drift_pt_result;
 
 /* Timed candidates of one autotune call (drift_pt_autotune). */
-#define DRIFT_PT_TUNE_MAX 48
+#define DRIFT_PT_TUNE_MAX 64
 typedef struct drift_pt_tuning {
     int32_t engine;                          /* the chosen candidate */
     int32_t block_threads;
     int32_t slots_per_group;                 /* SPLIT: replica slots per workgroup, else 0 */
+    int32_t couplings_in_lds;                /* SPLIT: 1 with the couplings staged in LDS */
     int32_t trial_rounds;                    /* rounds each timed trial ran */
     int32_t n_candidates;
     int32_t cand_engine[DRIFT_PT_TUNE_MAX];
     int32_t cand_block[DRIFT_PT_TUNE_MAX];
     int32_t cand_group[DRIFT_PT_TUNE_MAX];
+    int32_t cand_lds_couplings[DRIFT_PT_TUNE_MAX];
     double cand_seconds[DRIFT_PT_TUNE_MAX];  /* device time of trial_rounds rounds (a run of
                                                 2 * trial_rounds minus a run of trial_rounds, so
                                                 fixed costs cancel); < 0 if it could not run */
@@ -221,7 +229,8 @@ int drift_pt_run_batch_noalloc(const drift_pt_problem* problems, const drift_pt_
 
 /* Autotuner.  Times every engine and launch shape that can run this instance (LDS, split, slab,
  * lattice, global, coop, pslab; 256 / 512 / 1024 threads; for the split engine every power-of-two
- * number of replica slots per workgroup whose spins fit LDS) for trial_rounds rounds (0: about
+ * number of replica slots per workgroup whose spins fit LDS, with the couplings in HBM and, where
+ * they qualify, in LDS) for trial_rounds rounds (0: about
  * 2.5e8 attempts), in two passes of opposite order with the faster of each candidate's two trials
  * kept, stores the choice in a process-wide cache keyed by the instance's shape class (weight
  * type, replica slots, lattice shape, size and degree buckets, sweeps per round), and reports

@@ -58,14 +58,16 @@ def _check(g, o, model):
 
 
 def _run_pair(model, *, R, n_rounds, S, seed, rg, T_min=0.2, T_max=5.0, block=0, target=None,
-              trace=True):
+              trace=True, cl=False):
     enc = gpu.encode_problem(model)
     k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
     beta = ladder_betas(geometric_ladder(R, T_min, T_max))
     g = gpu.parallel_tempering_gpu(
         model, n_replicas=R, n_rounds=n_rounds, sweeps_per_round=S, seed=seed, betas=beta,
         engine="split", slots_per_group=rg, block_threads=block, coloring=(cptr, cspins),
-        record_history=True, return_final=True, record_trace=trace, target_energy=target)
+        record_history=True, return_final=True, record_trace=trace, target_energy=target,
+        couplings_in_lds=cl)
+    assert g.couplings_in_lds == cl
     ran = g.rounds_run if target is not None else n_rounds
     o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=seed, R=R, n_rounds=ran,
                               sweeps_per_round=S, beta=beta, n_threads=THREADS, trace=trace)
@@ -189,3 +191,47 @@ def test_batch_at_the_held_out_size_equals_oracle():
         o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=R, n_rounds=n_rounds,
                                   sweeps_per_round=S, beta=beta, n_threads=THREADS)
         _check(r, o, model)
+
+
+def _er_int_field(n, degree, seed):
+    """Erdos-Renyi graph with integer weights in +-1..+-3 and an integer field in -2..2."""
+    from drift_amd.ising import SparseIsingModel
+
+    base = er_sparse_weighted(n, degree, seed)
+    rng = np.random.default_rng(seed + 1)
+    rp, ci, w = base.to_csr()
+    return SparseIsingModel(rp, ci, w, rng.integers(-2, 3, size=n).astype(float))
+
+
+@pytest.mark.parametrize("case", [("er", 100, 8), ("er", 16, 16), ("er_field", 32, 2),
+                                  ("published", 256, 1), ("published", 256, 4)])
+def test_couplings_in_lds_equal_oracle(case):
+    """The split engine with every instance's couplings staged in LDS as 16-bit records (int8
+    couplings, |w| <= 7, n <= 4096): ER graphs with weights +-1..+-3 (R = 100 with padding slots,
+    one group of 16), an integer field, and the reference's published workload graph (degree 20,
+    R = 256) -- all equal to the oracle bit for bit."""
+    kind, R, rg = case
+    _announce(f"couplings in LDS: {kind}, R={R}, {rg} slots per group")
+    if kind == "er":
+        model = er_sparse_weighted(3000, 6.0, 21)
+    elif kind == "er_field":
+        model = _er_int_field(2500, 9.0, 22)
+    else:
+        model = maxcut_ising(random_graph(2048, p=20 / 2048, seed=0))
+    g, o, enc = _run_pair(model, R=R, n_rounds=10, S=3, seed=5, rg=rg, T_min=0.05, T_max=5.0,
+                          cl=True)
+    assert enc.weight_type == 1
+    _check(g, o, model)
+    assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
+
+
+def test_couplings_in_lds_refuse_what_does_not_fit_sixteen_bits():
+    """Float couplings, or n above 4096, cannot be staged as 16-bit records: the call fails
+    with the engine's message."""
+    _announce("couplings in LDS refusal")
+    with pytest.raises(RuntimeError, match="couplings in LDS need"):
+        gpu.parallel_tempering_gpu(random_regular_gaussian(600, seed=1), n_replicas=16,
+                                   n_rounds=2, engine="split", couplings_in_lds=True)
+    with pytest.raises(RuntimeError, match="couplings in LDS need"):
+        gpu.parallel_tempering_gpu(er_sparse_weighted(5000, 4.0, 2), n_replicas=16,
+                                   n_rounds=2, engine="split", couplings_in_lds=True)
