@@ -693,10 +693,13 @@ def c5_eval(args) -> dict:
             r = gpu.parallel_tempering_gpu(m, seed=s, target_energy=t, **kw)
             walls["geo_single"] += time.perf_counter() - t0
             single.append(r)
-        log(f"c5 eval: {fam} single geometric done")
+            if len(single) % 8 == 0:
+                log(f"c5 eval: {fam} single geometric {len(single)}/{len(models)}")
+        log(f"c5 eval: {fam} single geometric done; batched learned")
         t0 = time.perf_counter()
         lrn = gpu.parallel_tempering_gpu_batch(models, seeds=seeds, betas=learned, targets=tg, **kw)
         walls["learned_batched"] += time.perf_counter() - t0
+        log(f"c5 eval: {fam} batched learned done; batched geometric")
         t0 = time.perf_counter()
         geo = gpu.parallel_tempering_gpu_batch(models, seeds=seeds, targets=tg, **kw)
         walls["geo_batched"] += time.perf_counter() - t0
@@ -724,6 +727,9 @@ def c5_eval(args) -> dict:
         out = {"instances": n}
         for k in ("geo_single", "learned_batched", "geo_batched"):
             out[f"reached_{k}"] = int(sum(r[k]["reached"] for r in sel))
+            # a run that stopped at its target within half the cap: what a run capped there reaches
+            out[f"reached_within_half_cap_{k}"] = int(sum(r[k]["reached"] and r[k]["rounds"] <= cap // 2
+                                                          for r in sel))
             out[f"median_rounds_{k}"] = float(np.median([r[k]["rounds"] for r in sel]))
         for tol in ("within_1e-4", "within_1e-3"):
             out[f"{tol}_geo"] = int(sum(r[tol]["geo"] is not None for r in sel))
