@@ -58,3 +58,23 @@ def test_batched_float_bytes_per_attempt():
     B, fmt = bench.declared_bytes(model, gpu.encode_problem(model), "lds", 16)
     assert B == pytest.approx(33.0)
     assert "in LDS" in fmt and "summed in double" in fmt
+
+
+def test_split_engine_bytes_per_attempt_and_chunk_width():
+    # the split engine counts bytes as the LDS engine does: 3-regular float rows, 33 B
+    model = random_regular_gaussian(300, 4)
+    B, fmt = bench.declared_bytes(model, gpu.encode_problem(model), "split", 32)
+    assert B == pytest.approx(33.0)
+    assert "mailboxes" in fmt
+    # int8 couplings on a 3-D lattice read as CSR: every row of 6 fits a first chunk of width 6,
+    # so no row pointers are read: 7 spin reads + 1 write + six 4-byte records
+    lat = cubic_pmj_glass(8, 1)
+    enc = gpu.encode_problem(lat)
+    B6, fmt6 = bench.declared_bytes(lat, enc, "split", 32)
+    assert B6 == pytest.approx(8.0 + 6 * 4.0)
+    assert "first 6 edge records" in fmt6
+
+
+def test_c5_batch_is_a_sub_result_with_its_own_steps():
+    assert bench.SUB_DEFAULT_STEPS["c5_batch"] > 0
+    assert "65536" in bench.C5_BATCH_DESC and "split engine" in bench.C5_BATCH_DESC
