@@ -988,36 +988,57 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
     std::vector<PtInstance> insts((size_t)n_inst);
     int rc = upload_many(A, hs, insts, false, true, &rgs, cl);
     if (rc) return rc;
-    // largest work first; the groups of one instance consecutive
+    // Largest work first, the groups of one instance consecutive.  With eight queues (one per
+    // XCD: a workgroup takes from queue blockIdx.x mod 8 first) each instance goes, largest
+    // first, to the queue with the least work so far, so its groups share an L2.  A workgroup
+    // waits only on groups of an instance its queue has started; at most nq (max groups - 1)
+    // workgroups can hold such groups at once, so more resident workgroups than that guarantee
+    // progress; otherwise one queue.
     std::vector<int32_t> order((size_t)n_inst);
     std::iota(order.begin(), order.end(), 0);
     auto work = [&](int i) {
         const HostInst& H = hs[i];
-        return (double)H.n_rounds * std::max(H.S, 1) * (H.n + H.nnz) * rgs[i];
+        return (double)H.n_rounds * std::max(H.S, 1) * (H.n + H.nnz) * H.R;
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
-    std::vector<int32_t> items;
-    for (int i : order)
-        for (int g = 0; g < insts[i].ng; ++g) items.push_back((int32_t)((i << 10) | g));
+    const int n_groups_all = [&] {
+        int t = 0;
+        for (const PtInstance& I : insts) t += I.ng;
+        return t;
+    }();
+    const int grid = std::min(n_groups_all, capacity);
+    const int nq = (n_inst >= 8 && grid > 8 * (max_ng - 1)) ? 8 : 1;
+    std::vector<std::vector<int32_t>> qitems((size_t)nq);
+    std::vector<double> qwork((size_t)nq, 0.0);
+    for (int i : order) {
+        const int q = (int)(std::min_element(qwork.begin(), qwork.end()) - qwork.begin());
+        qwork[q] += work(i);
+        for (int g = 0; g < insts[i].ng; ++g) qitems[q].push_back((int32_t)((i << 10) | g));
+    }
+    std::vector<int32_t> items, qoff(1, 0);
+    for (const auto& v : qitems) {
+        items.insert(items.end(), v.begin(), v.end());
+        qoff.push_back((int32_t)items.size());
+    }
     const int n_items = (int)items.size();
     PtInstance* d_insts;
-    int32_t *d_items, *d_queue, *d_abort;
+    int32_t *d_items, *d_queue, *d_abort, *d_qoff;
     PT_HIP(A.upload(&d_insts, insts));
     PT_HIP(A.upload(&d_items, items));
-    PT_HIP(A.alloc(&d_queue, 2));
+    PT_HIP(A.upload(&d_qoff, qoff));
+    PT_HIP(A.alloc(&d_queue, 2 * (size_t)nq));   // the init launch's counters, then the timed one's
     PT_HIP(A.alloc(&d_abort, 1));
-    SplitArgs SA{d_abort, barrier_timeout_ticks()};
-    const int grid = std::min(n_items, capacity);
+    SplitArgs SA{d_abort, barrier_timeout_ticks(), d_qoff, nq};
     StreamGuard sg;
     PT_HIP(hipStreamCreate(&sg.s));
     PT_HIP(hipEventCreate(&sg.e0));
     PT_HIP(hipEventCreate(&sg.e1));
-    PT_HIP(hipMemsetAsync(d_queue, 0, 2 * sizeof(int32_t), sg.s));
+    PT_HIP(hipMemsetAsync(d_queue, 0, 2 * (size_t)nq * sizeof(int32_t), sg.s));
     PT_HIP(hipMemsetAsync(d_abort, 0, sizeof(int32_t), sg.s));
     PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue, wm, block, grid, lds, true, SA,
                            false, sg.s, cl));
     PT_HIP(hipEventRecord(sg.e0, sg.s));
-    PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue + 1, wm, block, grid, lds, false, SA,
+    PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue + nq, wm, block, grid, lds, false, SA,
                            !plain, sg.s, cl));
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));

@@ -77,10 +77,16 @@ struct PtInstance {
 };
 
 // Split engine kernel arguments: the abort flag (set when a mailbox wait exceeds timeout_ticks;
-// every workgroup then leaves) and the bound of one wait in device wall-clock ticks.
+// every workgroup then leaves), the bound of one wait in device wall-clock ticks, and the work
+// queues: nq queues of items, queue q holding items[qoff[q] .. qoff[q+1]) with its own counter.
+// A workgroup takes from queue blockIdx.x mod nq first (workgroups are dispatched to the eight
+// XCDs round robin, so with nq = 8 an instance's groups share one XCD's L2) and then from the
+// others.
 struct SplitArgs {
     int32_t* abort;
     unsigned long long timeout_ticks;
+    const int32_t* qoff;
+    int32_t nq;
 };
 
 #define PT_DACC_SPLIT 32

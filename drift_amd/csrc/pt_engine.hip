@@ -425,11 +425,24 @@ __global__ __launch_bounds__(1024) void pt_split_kernel(const PtInstance* __rest
     __shared__ long long scrE[16];
     __shared__ int scrR[16];
     for (;;) {
-        if (threadIdx.x == 0)
-            sc.inst = (!INIT && __hip_atomic_load(SA.abort, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT) != 0)
-                          ? n_items
-                          : atomicAdd(queue, 1);
+        if (threadIdx.x == 0) {
+            int got = n_items;
+            if (INIT || __hip_atomic_load(SA.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                const int home = (int)(blockIdx.x % (unsigned)SA.nq);
+                for (int d = 0; d < SA.nq; ++d) {
+                    const int q = home + d < SA.nq ? home + d : home + d - SA.nq;
+                    const int len = SA.qoff[q + 1] - SA.qoff[q];
+                    if (__hip_atomic_load(queue + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len)
+                        continue;
+                    const int i = atomicAdd(queue + q, 1);
+                    if (i < len) {
+                        got = SA.qoff[q] + i;
+                        break;
+                    }
+                }
+            }
+            sc.inst = got;
+        }
         __syncthreads();
         const int qi = sc.inst;
         if (qi >= n_items) return;
