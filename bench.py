@@ -14,6 +14,11 @@ Sub-results, each with its own roofline and CPU baseline, ride on the same line:
              random_graph(2048, p = min(0.1, 20/n), seed 0), R = 256, T 0.05 .. 5, rounds of four
              sweeps; its only published rate is 1.29 attempts/ns on an RTX 5060 Ti
              (docs/results/PHASE14-results.md:140-142), a different card: context, not a target
+  c5_batch   64 random 3-regular float instances of N = 65536, R = 32: the split engine's batch
+  time_to_target  the metric's second part on a sample of the learned-ladder evaluation's held-out
+             set (8 instances per family, N ~ 65536, cap 10^5 sweeps): wall time of one call per
+             instance on the geometric ladder over one batched call on the learned ladders, and
+             how many reach their best-known energies (--no-time-to-target leaves it out)
 
 Rooflines count the bytes one attempt touches at the storage format the engine declares (no
 credit for on-chip reuse); the fixed per-attempt figure of the measurement plan (8.75 B for C2,
@@ -539,6 +544,8 @@ def bench_throughput(args) -> int:
     subs["c5_batch"] = bench_c5_batch(args.sub_steps or SUB_DEFAULT_STEPS["c5_batch"],
                                       min(args.warmup, 20) or 1, args.repeats, cpu,
                                       autotune=args.autotune, plain=args.plain_launch)
+    if not args.no_time_to_target:
+        subs["time_to_target"] = bench_ttt_sample()
     line["sub_results"] = subs
     line["lds_probe"] = {k: {"GBs": round(v["GBs"], 1),
                              "wave_reads_per_s": float(f"{v['wave_reads_per_s']:.4g}")}
@@ -772,6 +779,69 @@ def c5_eval(args) -> dict:
     }
 
 
+def bench_ttt_sample(per_family: int = 8, cap: int = 100_000) -> dict:
+    """Time to target on a sample of the held-out set, as a sub-result of the default line:
+    per_family held-out instances of each family (N ~ 65536) spread evenly over its indices, every
+    run stopping at the instance's committed best-known energy or after cap sweeps; (a) one call per instance on the
+    geometric ladder, (b) one batched call on the committed network's learned ladders.  Same
+    seeds and targets as the full evaluation (--workload ladder-c5 --stage eval), so each run
+    equals that evaluation's run of the instance."""
+    from drift_amd import gpu, ladder
+
+    with open(os.path.join(ROOT, "tests", "golden", "c5_heldout_bestknown.json")) as f:
+        bk = json.load(f)
+    pf, n_target = bk["per_family"], bk["test_n"]
+    rows = {r["index"]: r for r in bk["rows"]}
+    net = ladder.LadderNet(seed=11)
+    net.load(os.path.join(ROOT, "drift_amd", "ladder_net_c5.npz"))
+    models, idx, targets, learned, ranges = [], [], [], [], []
+    ks = [(j * pf) // per_family for j in range(per_family)]
+    for fi, (fam, (T_min, T_max)) in enumerate(ladder.MIX_FAMILIES.items()):
+        fam_models = [ladder.mix_instance(fam, 900_000 + 1000 * fi + k, n_target) for k in ks]
+        xs = net.predict_ladder(ladder.instance_features(fam_models, T_min=T_min, T_max=T_max, R=32))
+        for k, m, x in zip(ks, fam_models, xs):
+            i = fi * pf + k
+            if rows[i]["n"] != m.n:
+                raise RuntimeError(f"held-out instance {i} does not match its best-known row")
+            models.append(m)
+            idx.append(i)
+            targets.append(rows[i]["best_known_energy"])
+            learned.append(ladder.betas_from_positions(x, T_min, T_max))
+            ranges.append((fam, T_min, T_max))
+    log(f"time to target: {len(models)} held-out instances, cap {cap} sweeps; one call each")
+    t_single, single = 0.0, []
+    for m, i, t, (fam, T_min, T_max) in zip(models, idx, targets, ranges):
+        t0 = time.perf_counter()
+        single.append(gpu.parallel_tempering_gpu(m, n_replicas=32, T_min=T_min, T_max=T_max,
+                                                 n_rounds=cap, sweeps_per_round=1, seed=4242 + i,
+                                                 target_energy=t, record_history=False))
+        t_single += time.perf_counter() - t0
+        log(f"time to target: instance {i} ({fam}) {'reached' if single[-1].best_E <= t else 'not reached'} "
+            f"after {single[-1].rounds_run} sweeps")
+    log("time to target: one batched call on the learned ladders")
+    t0 = time.perf_counter()
+    batched = gpu.parallel_tempering_gpu_batch(models, n_replicas=32, n_rounds=cap, sweeps_per_round=1,
+                                               seeds=[4242 + i for i in idx], betas=learned,
+                                               targets=targets)
+    t_batched = time.perf_counter() - t0
+    reached = lambda res: int(sum(r.best_E <= t + 1e-9 * abs(t) for r, t in zip(res, targets)))
+    return {"metric": "time-to-target energy, learned vs geometric ladder (sample of the held-out set)",
+            "value": round(t_single / max(t_batched, 1e-9), 3),
+            "unit": "x less wall time (batched + learned vs one call per instance + geometric)",
+            "higher_is_better": True, "steps": cap,
+            "config": {"workload": f"{per_family} held-out instances of each family of the "
+                                   f"learned-ladder evaluation (indices {ks} of {pf}; N ~ "
+                                   f"{n_target}), R=32, best-known targets from "
+                                   f"tests/golden/c5_heldout_bestknown.json",
+                       "cap_sweeps": cap, "instances": len(models),
+                       "batched_engine": batched[0].engine},
+            "reached": {"geometric_one_call_each": reached(single),
+                        "learned_batched": reached(batched), "of": len(models)},
+            "wall_s": {"geometric_one_call_each": round(t_single, 3),
+                       "learned_batched": round(t_batched, 3)},
+            "full_evaluation": "profiles/bench_ladder_c5_eval.json (201 instances)"}
+
+
 def bench_c3_ttt(args) -> dict:
     """C3's time-to-target report: the best energy a plain run of 10 x --steps sweeps finds (seed 1,
     geometric ladder) is the target; runs of at most --steps sweeps from another seed, on the
@@ -828,7 +898,7 @@ def main() -> int:
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
                     choices=("all", "c4", "c2", "c3", "c1", "published", "c5-batch", "c3-ttt",
-                             "ladder-c5"))
+                             "ladder-c5", "ttt-sample"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
@@ -841,6 +911,8 @@ def main() -> int:
                     help="ordinary dispatch of the persistent engines (for profiler commands)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-target", action="store_true",
+                    help="leave the time-to-target sample out of the default line")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
     ap.add_argument("--train-per-family", type=int, default=667)
     ap.add_argument("--stage", default="train", choices=("train", "bestknown", "eval"),
@@ -885,6 +957,9 @@ def main() -> int:
         return bench_ladder_c5(args)
     if args.workload == "c3-ttt":
         print(json.dumps(bench_c3_ttt(args)), flush=True)
+        return 0
+    if args.workload == "ttt-sample":
+        print(json.dumps(bench_ttt_sample()), flush=True)
         return 0
     return bench_throughput(args)
 
