@@ -204,9 +204,13 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
 
 /* Many independent instances; each result equals what drift_pt_run gives for that instance and
  * its params alone.  Instances whose replicas fit one workgroup's LDS run in one persistent launch
- * (a work queue, largest first); the others run on the per-launch engine, several at a time on
- * streams of their own so their launches overlap.  No reference counterpart: the reference solves
- * one instance per process (drift/gpu.py:97). */
+ * (a work queue, largest first); larger ones whose single replica fits run in a second persistent
+ * launch on the SPLIT engine when their replica groups number at least one per compute unit; the
+ * rest run on the per-launch engine, several at a time on streams of their own so their launches
+ * overlap.  With DRIFT_PT_AUTOTUNE on any instance, each persistent launch's shape (block size;
+ * for the split launch also slots per group and couplings in LDS) is timed on the batch first and
+ * cached per batch shape class; params.block_threads / slots_per_group fix it instead.  No
+ * reference counterpart: the reference solves one instance per process (drift/gpu.py:97). */
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
                        int32_t n_instances, drift_pt_result* results);
 
