@@ -2021,7 +2021,7 @@ int default_engine(const HostInst& H) {
 // ------------------------------------------------------------------------------------------
 // autotuner: timed trials per shape class, cached for the process
 // ------------------------------------------------------------------------------------------
-using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
 
 struct TuneEntry {
     int engine = 0, block = 0, rg = 0, cl = 0;
@@ -2031,7 +2031,7 @@ std::mutex g_tune_mu;
 std::map<TuneKey, TuneEntry> g_tune;
 
 // launch shapes chosen by the batch autotuner (tune_batch), by batch shape class
-using BatchKey = std::tuple<int, int, int, int, int, int, int, int>;
+using BatchKey = std::tuple<int, int, int, int, int, int, int, int, int>;
 
 struct BatchShape {
     int block = 0, rg_cap = 0;   // rg_cap: split engine, slots per group min(rg_cap, largest fit)
@@ -2045,7 +2045,9 @@ TuneKey tune_key(const HostInst& H) {
     int nb = 0;
     while ((1 << nb) < H.n) ++nb;                               // ceil(log2 n)
     const int half_degree = H.n ? (int)std::lround(2.0 * H.nnz / H.n) : 0;
-    return TuneKey(H.wm, H.Rp, H.ldim, H.lL, nb, std::min(half_degree, 256), H.S, H.has_h ? 1 : 0);
+    // whether the couplings fit 16-bit LDS records decides which candidates exist
+    return TuneKey(H.wm, H.Rp, H.ldim, H.lL, nb, std::min(half_degree, 256), H.S, H.has_h ? 1 : 0,
+                   H.cl_ok ? 1 : 0);
 }
 
 // Timed trials of every engine and launch shape that can run H.  The first candidate is what
@@ -2305,15 +2307,17 @@ int prepare_batch(const drift_pt_problem* problems, const drift_pt_params* param
 // ------------------------------------------------------------------------------------------
 
 // Shape class of a batch: engine (1 LDS, 7 split), weight type, largest padded slot count,
-// ceil(log2) of the largest n and of the instance count, mean degree, sweeps per round, field.
+// ceil(log2) of the largest n and of the instance count, mean degree, sweeps per round, field,
+// and whether every instance's couplings fit 16-bit LDS records.
 BatchKey batch_key(int engine, const std::vector<HostInst>& hs) {
-    int maxRp = 0, maxn = 1, S = 0, hh = 0;
+    int maxRp = 0, maxn = 1, S = 0, hh = 0, cl = 1;
     double deg = 0.0, n_all = 0.0;
     for (const HostInst& H : hs) {
         maxRp = std::max(maxRp, H.Rp);
         maxn = std::max(maxn, H.n);
         S = std::max(S, H.S);
         hh |= H.has_h ? 1 : 0;
+        cl &= H.cl_ok ? 1 : 0;
         deg += H.nnz;
         n_all += H.n;
     }
@@ -2323,7 +2327,7 @@ BatchKey batch_key(int engine, const std::vector<HostInst>& hs) {
         return b;
     };
     return BatchKey(engine, hs.empty() ? 0 : hs[0].wm, maxRp, clog2(maxn), clog2((int64_t)hs.size()),
-                    (int)std::lround(n_all > 0 ? deg / n_all : 0.0), S, hh);
+                    (int)std::lround(n_all > 0 ? deg / n_all : 0.0), S, hh, cl);
 }
 
 int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& results,
