@@ -235,3 +235,49 @@ def test_couplings_in_lds_refuse_what_does_not_fit_sixteen_bits():
     with pytest.raises(RuntimeError, match="couplings in LDS need"):
         gpu.parallel_tempering_gpu(er_sparse_weighted(5000, 4.0, 2), n_replicas=16,
                                    n_rounds=2, engine="split", couplings_in_lds=True)
+
+
+def test_mixed_batch_with_targets_equals_oracle_runs():
+    """One batch holding small instances (the LDS launch) and instances too large for one
+    workgroup (the split launch), each told to stop at an energy its chain reaches part-way:
+    every result equals the oracle's run of the rounds it ran."""
+    R, n_rounds, S = 32, 40, 1
+    _announce("mixed batch with targets")
+    small = [random_regular_gaussian(400 + 40 * i, seed=90_000 + i) for i in range(6)]
+    large = [random_regular_gaussian(12000 + 64 * i, seed=91_000 + i) for i in range(10)]
+    models = small + large
+    seeds = [7 + i for i in range(len(models))]
+    beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+    # targets from a first batched run: the best energy after a third of the rounds
+    first = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.2, T_max=5.0,
+                                             n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                             record_history=True)
+    targets = [float(r.best_history[n_rounds // 3]) for r in first]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.2, T_max=5.0,
+                                           n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                           return_final=True, record_history=True, targets=targets)
+    assert {r.engine for r in res[:6]} == {"lds"} and {r.engine for r in res[6:]} == {"split"}
+    for model, seed, t, r in zip(models, seeds, targets, res):
+        assert r.rounds_run <= n_rounds // 3 + 1 and r.best_E <= t + 1e-9 * max(1.0, abs(t))
+        enc = gpu.encode_problem(model)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=R, n_rounds=r.rounds_run,
+                                  sweeps_per_round=S, beta=beta, n_threads=THREADS)
+        _check(r, o, model)
+
+
+def test_batch_autotune_with_couplings_in_lds_candidates():
+    """Large int8 instances that qualify for couplings in LDS: the batch tuner times those shapes
+    too, and the tuned batch equals the untuned one bit for bit."""
+    count, R = 12, 64   # 64 slots of N ~ 3500 exceed one workgroup; N <= 4096 fits 16-bit records
+    _announce("batch autotune with couplings-in-LDS candidates")
+    gpu.autotune_clear()
+    models = [er_sparse_weighted(3500 + 20 * i, 7.0, 95_000 + i) for i in range(count)]
+    kw = dict(n_replicas=R, T_min=0.1, T_max=4.0, n_rounds=8, sweeps_per_round=2,
+              return_final=True, slots_per_group=0)
+    plain = gpu.parallel_tempering_gpu_batch(models, **kw)
+    tuned = gpu.parallel_tempering_gpu_batch(models, autotune=True, **kw)
+    assert plain[0].engine == tuned[0].engine == "split"
+    for a, b in zip(plain, tuned):
+        assert a.best_E_units == b.best_E_units and np.array_equal(a.final_spins, b.final_spins)
+        assert np.array_equal(a.rung_of, b.rung_of)
