@@ -671,22 +671,27 @@ def c5_eval(args) -> dict:
     within --cap sweeps: (a) one call per instance on the geometric ladder, as the reference runs
     it; (b) one batched call on the learned ladders; (c) one batched call on the geometric ladders
     (the batching alone).  Wall time is the library calls' host time; the instances' encoding is
-    the same for all three and is not counted."""
+    the same for all three and is not counted.  --families runs some families only and writes
+    their rows to --out; --stage eval-merge then puts the parts together."""
     from drift_amd import gpu, ladder
 
     t_all = time.perf_counter()
     with open(args.bestknown) as f:
         bk = json.load(f)
-    held = ladder.heldout_instances(bk["per_family"], bk["test_n"])
     target = {r["index"]: r["best_known_energy"] for r in bk["rows"]}
     net = ladder.LadderNet(seed=11)
     net.load(args.net)
     cap = args.cap
+    families = args.families.split(",") if args.families else list(ladder.MIX_FAMILIES)
     rows = []
     walls = {"geo_single": 0.0, "learned_batched": 0.0, "geo_batched": 0.0}
-    for fam, (T_min, T_max) in ladder.MIX_FAMILIES.items():
-        idx = [i for i, (f, _) in enumerate(held) if f == fam]
-        models = [held[i][1] for i in idx]
+    for fi, (fam, (T_min, T_max)) in enumerate(ladder.MIX_FAMILIES.items()):
+        if fam not in families:
+            continue
+        # the held-out instances of this family (ladder.heldout_instances, one family at a time)
+        idx = [fi * bk["per_family"] + k for k in range(bk["per_family"])]
+        models = [ladder.mix_instance(fam, 900_000 + 1000 * fi + k, bk["test_n"])
+                  for k in range(bk["per_family"])]
         tg = [target[i] for i in idx]
         seeds = [4242 + i for i in idx]
         xs = net.predict_ladder(ladder.instance_features(models, T_min=T_min, T_max=T_max, R=32))
@@ -729,14 +734,55 @@ def c5_eval(args) -> dict:
                          "learned_rung8_position": float(x[8]),
                          "learned_power": float(np.log(max(x[8], 1e-12)) / np.log(8 / 31))})
 
+    part = {"cap": cap, "families": families, "walls": walls, "rows": rows,
+            "net": os.path.basename(args.net), "total_s": round(time.perf_counter() - t_all, 1)}
+    if args.families:
+        os.makedirs(args.out, exist_ok=True)
+        path = os.path.join(args.out, f"c5_eval_part_{'_'.join(families)}_cap{cap}.json")
+        _write_json(path, part)
+        log(f"c5 eval: rows of {families} written to {path}")
+    return c5_eval_summary(part["rows"], part["walls"], cap, bk, part["net"], part["total_s"])
+
+
+def c5_eval_merge(args) -> dict:
+    """The evaluation's summary over parts written by --stage eval --families (--parts)."""
+    with open(args.bestknown) as f:
+        bk = json.load(f)
+    rows, walls, caps, nets, total = [], {}, set(), set(), 0.0
+    for path in args.parts.split(","):
+        with open(path) as f:
+            part = json.load(f)
+        caps.add(part["cap"])
+        nets.add(part["net"])
+        rows += part["rows"]
+        total += part["total_s"]
+        for k, v in part["walls"].items():
+            walls[k] = walls.get(k, 0.0) + v
+    if len(caps) != 1 or len(nets) != 1:
+        raise SystemExit("eval-merge: the parts were run with different caps or networks")
+    seen = [r["index"] for r in rows]
+    if len(seen) != len(set(seen)):
+        raise SystemExit("eval-merge: an instance appears in two parts")
+    rows.sort(key=lambda r: r["index"])
+    line = c5_eval_summary(rows, walls, caps.pop(), bk, nets.pop(), round(total, 1))
+    line["parts"] = [os.path.basename(p) for p in args.parts.split(",")]
+    return line
+
+
+def c5_eval_summary(rows, walls, cap, bk, net_name, total_s) -> dict:
+    from drift_amd import ladder
+
     def summary(sel):
         n = len(sel)
         out = {"instances": n}
         for k in ("geo_single", "learned_batched", "geo_batched"):
             out[f"reached_{k}"] = int(sum(r[k]["reached"] for r in sel))
-            # a run that stopped at its target within half the cap: what a run capped there reaches
+            # a run that stopped at its target within half (a quarter of) the cap: what a run
+            # capped there reaches
             out[f"reached_within_half_cap_{k}"] = int(sum(r[k]["reached"] and r[k]["rounds"] <= cap // 2
                                                           for r in sel))
+            out[f"reached_within_quarter_cap_{k}"] = int(sum(r[k]["reached"] and r[k]["rounds"] <= cap // 4
+                                                             for r in sel))
             out[f"median_rounds_{k}"] = float(np.median([r[k]["rounds"] for r in sel]))
         for tol in ("within_1e-4", "within_1e-3"):
             out[f"{tol}_geo"] = int(sum(r[tol]["geo"] is not None for r in sel))
@@ -768,13 +814,14 @@ def c5_eval(args) -> dict:
                    "per_family": bk["per_family"], "cap_sweeps": cap,
                    "target": f"best-known energy: lowest of four independent runs of "
                              f"{10 * bk['cap_sweeps']} sweeps (" + bk["source"] + ")",
-                   "net": os.path.basename(args.net)},
+                   "net": net_name},
         "wall_s": {k: round(v, 3) for k, v in walls.items()},
         "speedup_ladder_only": round(walls["geo_batched"] / max(walls["learned_batched"], 1e-9), 3),
         "speedup_batching_only": round(walls["geo_single"] / max(walls["geo_batched"], 1e-9), 3),
         "held_out": total,
-        "per_family": {f: summary([r for r in rows if r["family"] == f]) for f in ladder.MIX_FAMILIES},
-        "total_s": round(time.perf_counter() - t_all, 1),
+        "per_family": {f: summary([r for r in rows if r["family"] == f]) for f in ladder.MIX_FAMILIES
+                       if any(r["family"] == f for r in rows)},
+        "total_s": total_s,
         "rows": rows,
     }
 
@@ -884,7 +931,8 @@ def bench_c3_ttt(args) -> dict:
 
 
 def bench_ladder_c5(args) -> int:
-    stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval}[args.stage]
+    stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval,
+             "eval-merge": c5_eval_merge}[args.stage]
     line = stage(args)
     print(json.dumps(line), flush=True)
     return 0
@@ -915,7 +963,7 @@ def main() -> int:
                     help="leave the time-to-target sample out of the default line")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
     ap.add_argument("--train-per-family", type=int, default=667)
-    ap.add_argument("--stage", default="train", choices=("train", "bestknown", "eval"),
+    ap.add_argument("--stage", default="train", choices=("train", "bestknown", "eval", "eval-merge"),
                     help="ladder-c5 workload: train the network, record the held-out instances' "
                          "best-known energies, or measure time to them")
     ap.add_argument("--net", default=os.path.join(ROOT, "drift_amd", "ladder_net_c5.npz"),
@@ -931,8 +979,11 @@ def main() -> int:
     ap.add_argument("--test-n", type=int, default=65536, help="ladder-c5 held-out size")
     ap.add_argument("--test-per-family", type=int, default=67)
     ap.add_argument("--families", default="",
-                    help="ladder-c5 bestknown: comma-separated families to run (default all); "
-                         "rows of the others already in --out are kept")
+                    help="ladder-c5 bestknown / eval: comma-separated families to run (default "
+                         "all); bestknown keeps the other families' rows already in --out, eval "
+                         "writes the families' rows to --out for --stage eval-merge")
+    ap.add_argument("--parts", default="",
+                    help="ladder-c5 eval-merge: comma-separated part files of --stage eval")
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (rocprof counter scaling)")
@@ -946,6 +997,10 @@ def main() -> int:
               "one instance's ladder by contiguous rung blocks); one GPU is measured",
               file=sys.stderr)
         return 2
+
+    if args.workload == "ladder-c5" and args.stage == "eval-merge":   # host-only
+        print(json.dumps(c5_eval_merge(args)), flush=True)
+        return 0
 
     from drift_amd import gpu
 

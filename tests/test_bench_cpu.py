@@ -78,3 +78,47 @@ def test_split_engine_bytes_per_attempt_and_chunk_width():
 def test_c5_batch_is_a_sub_result_with_its_own_steps():
     assert bench.SUB_DEFAULT_STEPS["c5_batch"] > 0
     assert "65536" in bench.C5_BATCH_DESC and "split engine" in bench.C5_BATCH_DESC
+
+
+def test_time_to_target_parts_merge(tmp_path):
+    """--stage eval-merge puts per-family parts of the time-to-target evaluation together: the
+    summary counts every part's rows, adds their wall times, and refuses mismatched caps."""
+    import json
+
+    def row(i, fam, reached_geo, reached_lrn, rounds):
+        res = lambda ok: {"rounds": rounds, "reached": ok, "best_E": -10.0 if ok else -9.0}
+        return {"index": i, "family": fam, "n": 64, "target": -10.0,
+                "geo_single": res(reached_geo), "learned_batched": res(reached_lrn),
+                "geo_batched": res(reached_geo),
+                "within_1e-4": {"geo": 5 if reached_geo else None, "learned": 5},
+                "within_1e-3": {"geo": 5, "learned": 5},
+                "to_geo_final": {"geo": 5, "learned": 3},
+                "learned_rung8_position": 0.3, "learned_power": 0.5}
+    parts = []
+    for name, fam, rows, walls in (
+            ("a", "ea3d", [row(0, "ea3d", True, True, 10), row(1, "ea3d", False, True, 40)],
+             {"geo_single": 8.0, "learned_batched": 2.0, "geo_batched": 3.0}),
+            ("b", "er", [row(134, "er", False, False, 100)],
+             {"geo_single": 4.0, "learned_batched": 1.0, "geo_batched": 2.0})):
+        path = tmp_path / f"{name}.json"
+        path.write_text(json.dumps({"cap": 100, "families": [fam], "walls": walls, "rows": rows,
+                                    "net": "net.npz", "total_s": 1.0}))
+        parts.append(str(path))
+
+    class Args:
+        bestknown = os.path.join(ROOT, "tests", "golden", "c5_heldout_bestknown.json")
+    Args.parts = ",".join(parts)
+    line = bench.c5_eval_merge(Args)
+    assert line["held_out"]["instances"] == 3
+    assert line["held_out"]["reached_geo_single"] == 1 and line["held_out"]["reached_learned_batched"] == 2
+    assert line["held_out"]["reached_within_half_cap_learned_batched"] == 2   # rounds 10 and 40 <= 50
+    assert line["held_out"]["reached_within_quarter_cap_learned_batched"] == 1
+    assert line["wall_s"]["geo_single"] == pytest.approx(12.0)
+    assert line["value"] == pytest.approx(12.0 / 3.0)
+    assert set(line["per_family"]) == {"ea3d", "er"}
+    bad = tmp_path / "c.json"
+    bad.write_text(json.dumps({"cap": 50, "families": ["rrg"], "walls": {}, "rows": [],
+                               "net": "net.npz", "total_s": 0.0}))
+    Args.parts = parts[0] + "," + str(bad)
+    with pytest.raises(SystemExit):
+        bench.c5_eval_merge(Args)
