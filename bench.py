@@ -669,9 +669,10 @@ def _reach(hist, target, rel):
 def c5_eval(args) -> dict:
     """Time to the best-known energy on the held-out instances, every run stopping at its target
     within --cap sweeps: (a) one call per instance on the geometric ladder, as the reference runs
-    it; (b) one batched call on the learned ladders; (c) one batched call on the geometric ladders
-    (the batching alone).  Wall time is the library calls' host time; the instances' encoding is
-    the same for all three and is not counted.  --families runs some families only and writes
+    it, each on the engine and launch shape the autotuner picks for its shape class (the tuning
+    runs untimed before the family's timed calls); (b) one batched call on the learned ladders;
+    (c) one batched call on the geometric ladders (the batching alone).  Wall time is the library
+    calls' host time; the instances' encoding is the same for all three and is not counted.  --families runs some families only and writes
     their rows to --out; --stage eval-merge then puts the parts together."""
     from drift_amd import gpu, ladder
 
@@ -698,11 +699,13 @@ def c5_eval(args) -> dict:
         learned = [ladder.betas_from_positions(x, T_min, T_max) for x in xs]
         kw = dict(n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=cap, sweeps_per_round=1,
                   record_history=True)
-        # (a) the reference's way: one instance per call, geometric ladder
+        # (a) the reference's way: one instance per call, geometric ladder, on the tuned engine
+        _tune_singles(models, seeds, tg, kw)
+        log(f"c5 eval: {fam} engines tuned; single geometric")
         single = []
         for m, s, t in zip(models, seeds, tg):
             t0 = time.perf_counter()
-            r = gpu.parallel_tempering_gpu(m, seed=s, target_energy=t, **kw)
+            r = gpu.parallel_tempering_gpu(m, seed=s, target_energy=t, autotune=True, **kw)
             walls["geo_single"] += time.perf_counter() - t0
             single.append(r)
             if len(single) % 8 == 0:
@@ -719,7 +722,9 @@ def c5_eval(args) -> dict:
         for i, m, t, a, b, c, x in zip(idx, models, tg, single, lrn, geo, xs):
             rows.append({"index": i, "family": fam, "n": m.n, "target": t,
                          "geo_single": {"rounds": a.rounds_run, "reached": a.best_E <= t + 1e-9 * abs(t),
-                                        "best_E": a.best_E, "device_s": a.seconds},
+                                        "best_E": a.best_E, "device_s": a.seconds,
+                                        "engine": a.engine, "slots_per_group": a.slots_per_group,
+                                        "parts_per_group": a.parts_per_group},
                          "learned_batched": {"rounds": b.rounds_run, "reached": b.best_E <= t + 1e-9 * abs(t),
                                              "best_E": b.best_E},
                          "geo_batched": {"rounds": c.rounds_run, "reached": c.best_E <= t + 1e-9 * abs(t),
@@ -826,13 +831,25 @@ def c5_eval_summary(rows, walls, cap, bk, net_name, total_s) -> dict:
     }
 
 
+def _tune_singles(models, seeds, targets, kw) -> None:
+    """Fill the autotune cache for every shape class among models (runs that stop at a target),
+    untimed: a two-round call of each instance tunes its class on a miss and hits the cache
+    afterwards."""
+    from drift_amd import gpu
+
+    warm = dict(kw, n_rounds=2, record_history=False)
+    for m, s, t in zip(models, seeds, targets):
+        gpu.parallel_tempering_gpu(m, seed=s, target_energy=t, autotune=True, **warm)
+
+
 def bench_ttt_sample(per_family: int = 8, cap: int = 100_000) -> dict:
     """Time to target on a sample of the held-out set, as a sub-result of the default line:
     per_family held-out instances of each family (N ~ 65536) spread evenly over its indices, every
     run stopping at the instance's committed best-known energy or after cap sweeps; (a) one call per instance on the
-    geometric ladder, (b) one batched call on the committed network's learned ladders.  Same
-    seeds and targets as the full evaluation (--workload ladder-c5 --stage eval), so each run
-    equals that evaluation's run of the instance."""
+    geometric ladder on the autotuned engine (tuned untimed first), (b) one batched call on the
+    committed network's learned ladders.  Same seeds and targets as the full evaluation
+    (--workload ladder-c5 --stage eval), so each run equals that evaluation's run of the
+    instance."""
     from drift_amd import gpu, ladder
 
     with open(os.path.join(ROOT, "tests", "golden", "c5_heldout_bestknown.json")) as f:
@@ -855,13 +872,21 @@ def bench_ttt_sample(per_family: int = 8, cap: int = 100_000) -> dict:
             targets.append(rows[i]["best_known_energy"])
             learned.append(ladder.betas_from_positions(x, T_min, T_max))
             ranges.append((fam, T_min, T_max))
-    log(f"time to target: {len(models)} held-out instances, cap {cap} sweeps; one call each")
-    t_single, single = 0.0, []
+    log(f"time to target: {len(models)} held-out instances, cap {cap} sweeps; tuning the engines")
+    for m, i, t, (fam, T_min, T_max) in zip(models, idx, targets, ranges):
+        _tune_singles([m], [4242 + i], [t], dict(n_replicas=32, T_min=T_min, T_max=T_max,
+                                                 sweeps_per_round=1))
+    log("time to target: one call each")
+    t_single, single, engines = 0.0, [], {}
     for m, i, t, (fam, T_min, T_max) in zip(models, idx, targets, ranges):
         t0 = time.perf_counter()
         single.append(gpu.parallel_tempering_gpu(m, n_replicas=32, T_min=T_min, T_max=T_max,
                                                  n_rounds=cap, sweeps_per_round=1, seed=4242 + i,
-                                                 target_energy=t, record_history=False))
+                                                 target_energy=t, record_history=False,
+                                                 autotune=True))
+        r = single[-1]
+        engines.setdefault(fam, set()).add(
+            f"{r.engine}" + (f" {r.slots_per_group}x{r.parts_per_group}" if r.engine == "split" else ""))
         t_single += time.perf_counter() - t0
         log(f"time to target: instance {i} ({fam}) {'reached' if single[-1].best_E <= t else 'not reached'} "
             f"after {single[-1].rounds_run} sweeps")
@@ -881,7 +906,8 @@ def bench_ttt_sample(per_family: int = 8, cap: int = 100_000) -> dict:
                                    f"{n_target}), R=32, best-known targets from "
                                    f"tests/golden/c5_heldout_bestknown.json",
                        "cap_sweeps": cap, "instances": len(models),
-                       "batched_engine": batched[0].engine},
+                       "batched_engine": batched[0].engine,
+                       "single_engines": {f: sorted(v) for f, v in engines.items()}},
             "reached": {"geometric_one_call_each": reached(single),
                         "learned_batched": reached(batched), "of": len(models)},
             "wall_s": {"geometric_one_call_each": round(t_single, 3),

@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64,
 LIB_NAME = "libdriftpt.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # enum drift_pt_status
 OK, EINVAL, ECOLOR, ENOMEM, EHIP, ENODEV, ERANGE = 0, -1, -2, -3, -4, -5, -6
@@ -68,6 +68,7 @@ class Params(ctypes.Structure):
         ("block_threads", c_int32),
         ("target_energy", c_double),
         ("slots_per_group", c_int32),
+        ("parts_per_group", c_int32),
     ]
 
 
@@ -101,6 +102,7 @@ class Result(ctypes.Structure):
         ("block_threads_used", c_int32),
         ("slots_per_group_used", c_int32),
         ("couplings_in_lds_used", c_int32),
+        ("parts_per_group_used", c_int32),
     ]
 
 
@@ -112,12 +114,14 @@ class Tuning(ctypes.Structure):
         ("block_threads", c_int32),
         ("slots_per_group", c_int32),
         ("couplings_in_lds", c_int32),
+        ("parts_per_group", c_int32),
         ("trial_rounds", c_int32),
         ("n_candidates", c_int32),
         ("cand_engine", c_int32 * TUNE_MAX),
         ("cand_block", c_int32 * TUNE_MAX),
         ("cand_group", c_int32 * TUNE_MAX),
         ("cand_lds_couplings", c_int32 * TUNE_MAX),
+        ("cand_parts", c_int32 * TUNE_MAX),
         ("cand_seconds", c_double * TUNE_MAX),
     ]
 

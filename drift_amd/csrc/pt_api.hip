@@ -144,6 +144,7 @@ struct HostInst {
     int has_target = 0;             // DRIFT_PT_STOP_AT_TARGET: stop at best <= target_units
     int64_t target_units = 0;
     int rg_req = 0;                 // drift_pt_params::slots_per_group (0: the library chooses)
+    int np_req = 0;                 // drift_pt_params::parts_per_group (0: the library chooses)
     bool cl_ok = false;             // couplings fit 16-bit LDS records (int8, |w| <= 7, n <= 4096)
     int block_req = 0;              // drift_pt_params::block_threads (0: the library chooses)
 };
@@ -215,6 +216,8 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     if (prm->slots_per_group < 0 || prm->slots_per_group > 64 ||
         (prm->slots_per_group & (prm->slots_per_group - 1)) != 0)
         return fail(DRIFT_PT_EINVAL, "slots_per_group must be 0 or a power of two up to 64");
+    if (prm->parts_per_group < 0 || prm->parts_per_group > 64)
+        return fail(DRIFT_PT_EINVAL, "parts_per_group must be in [0, 64]");
     const int32_t* rp = P->row_ptr;
     const int32_t* ci = P->col_idx;
     if (rp[0] != 0 || rp[n] != nnz) return fail(DRIFT_PT_EINVAL, "row_ptr must run 0..nnz");
@@ -291,6 +294,7 @@ int prepare(const drift_pt_problem* P, const drift_pt_params* prm, HostInst& H) 
     H.wm = P->weight_type; H.S = prm->sweeps_per_round; H.n_rounds = prm->n_rounds;
     H.flags = prm->flags; H.seed = prm->seed; H.has_h = P->h != nullptr;
     H.rg_req = prm->slots_per_group;
+    H.np_req = prm->parts_per_group;
     H.block_req = prm->block_threads;
     H.color_ptr = cptr;
     H.orig = cspins;
@@ -558,12 +562,16 @@ std::vector<T> pad4(const std::vector<T>& v) {
 int split_thr_words(const HostInst& H, int rg);
 
 // cl (split engine): also each instance's CSR as 16-bit records (position << 4 | weight & 15).
+// nps (split engine): parts per group of each instance; those above 1 get their spin words and
+// flags (zero) and part mailboxes (all bits set).
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
                 bool force_final, bool with_ell, const std::vector<int>* rgs = nullptr,
-                bool cl = false) {
+                bool cl = false, const std::vector<int>* nps = nullptr) {
     const size_t m = hs.size();
     auto rg_of = [&](size_t i) { return rgs ? (*rgs)[i] : hs[i].Rp; };
     auto ng_of = [&](size_t i) { return rgs ? (hs[i].R + (*rgs)[i] - 1) / (*rgs)[i] : 1; };
+    auto np_of = [&](size_t i) { return nps ? (*nps)[i] : 1; };
+    auto nw_of = [&](size_t i) { return (hs[i].n + 31) / 32; };
     struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb, crec; };
     std::vector<Off> off(m);
     Packer in;
@@ -590,7 +598,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         }
     }
     // outputs after the inputs
-    struct OutOff { size_t best, fin, hist, trace, out, img, e0, box; };
+    struct OutOff { size_t best, fin, hist, trace, out, img, e0, box, pbox, pbits, pflag; };
     std::vector<OutOff> oo(m);
     size_t cursor = (in.blob.size() + 255) & ~(size_t)255;
     const size_t out_begin = cursor;
@@ -604,9 +612,20 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         oo[i].out = take(sizeof(int64_t) * (size_t)pt_out_len(H.R));
     }
     const size_t out_end = cursor;   // statistics blocks are the region that must start at zero
-    for (size_t i = 0; i < m; ++i)
+    for (size_t i = 0; i < m; ++i) {
         oo[i].box = rgs ? take(sizeof(uint64_t) * (4 * (size_t)hs[i].Rp + 2)) : SIZE_MAX;
+        oo[i].pbox = np_of(i) > 1 ? take(sizeof(uint64_t) * 4 * (size_t)np_of(i) * hs[i].Rp)
+                                  : SIZE_MAX;
+    }
     const size_t box_end = cursor;   // mailboxes start with every bit set
+    for (size_t i = 0; i < m; ++i) {
+        oo[i].pbits = np_of(i) > 1 ? take(sizeof(uint64_t) * 2 * (size_t)hs[i].Rp * nw_of(i))
+                                   : SIZE_MAX;
+        oo[i].pflag = np_of(i) > 1 ? take(sizeof(uint64_t) * PT_PFLAG_STRIDE * (size_t)ng_of(i) *
+                                          np_of(i))
+                                   : SIZE_MAX;
+    }
+    const size_t bits_end = cursor;  // part spin words start at step 0
     for (size_t i = 0; i < m; ++i) {
         const HostInst& H = hs[i];
         oo[i].best = take((size_t)H.n);
@@ -624,6 +643,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
     PT_HIP(hipMemcpy(base, in.blob.data(), in.blob.size(), hipMemcpyHostToDevice));
     PT_HIP(hipMemset(base + out_begin, 0, out_end - out_begin));
     if (box_end > out_end) PT_HIP(hipMemset(base + out_end, 0xFF, box_end - out_end));
+    if (bits_end > box_end) PT_HIP(hipMemset(base + box_end, 0, bits_end - box_end));
     insts.assign(m, PtInstance{});
     for (size_t i = 0; i < m; ++i) {
         const HostInst& H = hs[i];
@@ -661,6 +681,11 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.crec = o.crec == SIZE_MAX ? nullptr : (const uint16_t*)(base + o.crec);
         I.nnz = H.nnz;
         I.thr_lds = rgs && split_thr_words(H, rg_of(i)) > 0 ? 1 : 0;
+        I.pbox = oo[i].pbox == SIZE_MAX ? nullptr : (uint64_t*)(base + oo[i].pbox);
+        I.pbits = oo[i].pbits == SIZE_MAX ? nullptr : (uint64_t*)(base + oo[i].pbits);
+        I.pflag = oo[i].pflag == SIZE_MAX ? nullptr : (uint64_t*)(base + oo[i].pflag);
+        I.np = np_of(i);
+        I.nw = nw_of(i);
     }
     return 0;
 }
@@ -681,6 +706,7 @@ void clear_result(drift_pt_result* r) {
     r->kernel_seconds = 0.0; r->kernel_attempts = 0.0; r->kernel_launches = 0;
     r->other_kernel_seconds = 0.0; r->other_kernel_launches = 0; r->rounds_run = 0;
     r->block_threads_used = 0; r->slots_per_group_used = 0; r->couplings_in_lds_used = 0;
+    r->parts_per_group_used = 0;
 }
 
 // Copy results of one instance back and undo the colour relabelling.
@@ -890,8 +916,28 @@ int split_thr_words(const HostInst& H, int rg) {
     return H.wm != PT_WM_F32 && (int64_t)rg * H.TQ * 4 <= 16384 ? rg * H.TQ : 0;
 }
 
-size_t split_lds(const HostInst& H, int rg, bool cl) {
-    return pt_split_lds_bytes(H.n, rg, H.Rp, H.k, split_thr_words(H, rg), cl, H.nnz, H.has_h);
+size_t split_lds(const HostInst& H, int rg, bool cl, bool parts = false) {
+    return pt_split_lds_bytes(H.n, rg, H.Rp, H.k, split_thr_words(H, rg), cl, H.nnz, H.has_h,
+                              parts);
+}
+
+// 32-position words of the largest colour class.  Parts per group are at most this, so every part
+// owns a word of that class, which the split kernel's spin-word buffer reuse relies on.
+int class_words_max(const HostInst& H) {
+    int best = 1;
+    for (int c = 0; c < H.k; ++c) {
+        const int cs = H.color_ptr[c], ce = H.color_ptr[c + 1];
+        if (ce > cs) best = std::max(best, ((ce - 1) >> 5) + 1 - (cs >> 5));
+    }
+    return best;
+}
+
+// Parts per group of each instance: the requested count where one is given, else 1.
+std::vector<int> split_nps(const std::vector<HostInst>& hs) {
+    std::vector<int> out(hs.size(), 1);
+    for (size_t i = 0; i < hs.size(); ++i)
+        if (hs[i].np_req > 0) out[i] = hs[i].np_req;
+    return out;
 }
 
 int split_rg_max(const HostInst& H, bool cl = false) {
@@ -960,8 +1006,13 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
             return fail(DRIFT_PT_EINVAL, "couplings in LDS need int8 couplings with |w| <= 7 and "
                                          "n <= 4096");
     const std::vector<int> rgs = split_rgs(hs, cl);
+    const std::vector<int> nps = split_nps(hs);
+    const bool parts = std::any_of(nps.begin(), nps.end(), [](int v) { return v > 1; });
+    if (parts && cl)
+        return fail(DRIFT_PT_EINVAL, "split engine: couplings in LDS do not combine with parts "
+                                     "per group");
     size_t lds = 0;
-    int max_ng = 1;
+    int max_ng = 1;   // workgroups of the largest instance (groups x parts)
     bool plain = false;
     for (int i = 0; i < n_inst; ++i) {
         const HostInst& H = hs[i];
@@ -970,15 +1021,31 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
             return fail(DRIFT_PT_EINVAL, "split engine: " + std::to_string(rg) +
                                              " slots per workgroup do not fit one workgroup's LDS "
                                              "(or exceed the padded replica count)");
-        lds = std::max(lds, split_lds(H, rg, cl));
-        max_ng = std::max(max_ng, (H.R + rg - 1) / rg);
+        const int ng = (H.R + rg - 1) / rg, np = nps[i];
+        if (np > 1) {
+            if (np > class_words_max(H))
+                return fail(DRIFT_PT_EINVAL, "split engine: " + std::to_string(np) +
+                                                 " parts per group exceed the " +
+                                                 std::to_string(class_words_max(H)) +
+                                                 " 32-position words of the largest colour class");
+            if ((int64_t)ng * np > 1024)
+                return fail(DRIFT_PT_EINVAL, "split engine: more than 1024 workgroups (groups x "
+                                             "parts) for one instance");
+            if ((int64_t)H.n_rounds * std::max(H.S, 1) * std::max(H.k, 1) >= ((int64_t)1 << 31))
+                return fail(DRIFT_PT_ERANGE, "split engine with parts: more than 2^31 colour steps");
+            if (split_lds(H, rg, cl, true) > (size_t)device_lds_limit())
+                return fail(DRIFT_PT_EINVAL, "split engine: a part's state does not fit one "
+                                             "workgroup's LDS");
+        }
+        lds = std::max(lds, split_lds(H, rg, cl, parts));
+        max_ng = std::max(max_ng, ng * np);
         if (H.flags & DRIFT_PT_PLAIN_LAUNCH) plain = true;
         if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "split engine: one weight type per launch");
     }
     const int block = split_block(lds, block_req);
     const int wm = hs[0].wm;
     int per_cu = 0;
-    PT_HIP(pt_split_occupancy(wm, block, lds, &per_cu, cl));
+    PT_HIP(pt_split_occupancy(wm, block, lds, &per_cu, cl, parts));
     const int capacity = per_cu * device_cus();
     if (capacity < max_ng)
         return fail(DRIFT_PT_EINVAL, "split engine: the " + std::to_string(max_ng) +
@@ -986,7 +1053,7 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
                                          std::to_string(capacity) + " workgroups of " +
                                          std::to_string(block) + " threads)");
     std::vector<PtInstance> insts((size_t)n_inst);
-    int rc = upload_many(A, hs, insts, false, true, &rgs, cl);
+    int rc = upload_many(A, hs, insts, false, true, &rgs, cl, &nps);
     if (rc) return rc;
     // Largest work first, the groups of one instance consecutive.  With eight queues (one per
     // XCD: a workgroup takes from queue blockIdx.x mod 8 first) each instance goes, largest
@@ -1001,45 +1068,53 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
         return (double)H.n_rounds * std::max(H.S, 1) * (H.n + H.nnz) * H.R;
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
-    const int n_groups_all = [&] {
+    // the init launch takes groups, the timed one every (group, part) workgroup
+    const int n_work_all = [&] {
         int t = 0;
-        for (const PtInstance& I : insts) t += I.ng;
+        for (const PtInstance& I : insts) t += I.ng * I.np;
         return t;
     }();
-    const int grid = std::min(n_groups_all, capacity);
+    const int grid = std::min(n_work_all, capacity);
     const int nq = (n_inst >= 8 && grid > 8 * (max_ng - 1)) ? 8 : 1;
-    std::vector<std::vector<int32_t>> qitems((size_t)nq);
+    std::vector<std::vector<int32_t>> qitems((size_t)nq), qinit((size_t)nq);
     std::vector<double> qwork((size_t)nq, 0.0);
     for (int i : order) {
         const int q = (int)(std::min_element(qwork.begin(), qwork.end()) - qwork.begin());
         qwork[q] += work(i);
-        for (int g = 0; g < insts[i].ng; ++g) qitems[q].push_back((int32_t)((i << 10) | g));
+        for (int g = 0; g < insts[i].ng; ++g) qinit[q].push_back((int32_t)((i << 10) | g));
+        for (int w = 0; w < insts[i].ng * insts[i].np; ++w)
+            qitems[q].push_back((int32_t)((i << 10) | w));
     }
-    std::vector<int32_t> items, qoff(1, 0);
-    for (const auto& v : qitems) {
-        items.insert(items.end(), v.begin(), v.end());
+    std::vector<int32_t> items, qoff(1, 0), items0, qoff0(1, 0);
+    for (int q = 0; q < nq; ++q) {
+        items.insert(items.end(), qitems[q].begin(), qitems[q].end());
         qoff.push_back((int32_t)items.size());
+        items0.insert(items0.end(), qinit[q].begin(), qinit[q].end());
+        qoff0.push_back((int32_t)items0.size());
     }
-    const int n_items = (int)items.size();
+    const int n_items = (int)items.size(), n_items0 = (int)items0.size();
     PtInstance* d_insts;
-    int32_t *d_items, *d_queue, *d_abort, *d_qoff;
+    int32_t *d_items, *d_items0, *d_queue, *d_abort, *d_qoff, *d_qoff0;
     PT_HIP(A.upload(&d_insts, insts));
     PT_HIP(A.upload(&d_items, items));
     PT_HIP(A.upload(&d_qoff, qoff));
+    PT_HIP(A.upload(&d_items0, items0));
+    PT_HIP(A.upload(&d_qoff0, qoff0));
     PT_HIP(A.alloc(&d_queue, 2 * (size_t)nq));   // the init launch's counters, then the timed one's
     PT_HIP(A.alloc(&d_abort, 1));
     SplitArgs SA{d_abort, barrier_timeout_ticks(), d_qoff, nq};
+    const SplitArgs SA0{d_abort, barrier_timeout_ticks(), d_qoff0, nq};
     StreamGuard sg;
     PT_HIP(hipStreamCreate(&sg.s));
     PT_HIP(hipEventCreate(&sg.e0));
     PT_HIP(hipEventCreate(&sg.e1));
     PT_HIP(hipMemsetAsync(d_queue, 0, 2 * (size_t)nq * sizeof(int32_t), sg.s));
     PT_HIP(hipMemsetAsync(d_abort, 0, sizeof(int32_t), sg.s));
-    PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue, wm, block, grid, lds, true, SA,
+    PT_HIP(pt_launch_split(d_insts, d_items0, n_items0, d_queue, wm, block, grid, lds, true, SA0,
                            false, sg.s, cl));
     PT_HIP(hipEventRecord(sg.e0, sg.s));
     PT_HIP(pt_launch_split(d_insts, d_items, n_items, d_queue + nq, wm, block, grid, lds, false, SA,
-                           !plain, sg.s, cl));
+                           !plain, sg.s, cl, parts));
     PT_HIP(hipEventRecord(sg.e1, sg.s));
     PT_HIP(hipStreamSynchronize(sg.s));
     float ms = 0.0f;
@@ -1081,6 +1156,7 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
         results[i]->block_threads_used = block;
         results[i]->slots_per_group_used = rgs[i];
         results[i]->couplings_in_lds_used = cl ? 1 : 0;
+        results[i]->parts_per_group_used = nps[i];
         results[i]->kernel_seconds = ms * 1e-3;
         results[i]->kernel_launches = 1;
         results[i]->kernel_attempts = total;
@@ -2021,10 +2097,10 @@ int default_engine(const HostInst& H) {
 // ------------------------------------------------------------------------------------------
 // autotuner: timed trials per shape class, cached for the process
 // ------------------------------------------------------------------------------------------
-using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
 
 struct TuneEntry {
-    int engine = 0, block = 0, rg = 0, cl = 0;
+    int engine = 0, block = 0, rg = 0, cl = 0, np = 0;
 };
 
 std::mutex g_tune_mu;
@@ -2045,9 +2121,10 @@ TuneKey tune_key(const HostInst& H) {
     int nb = 0;
     while ((1 << nb) < H.n) ++nb;                               // ceil(log2 n)
     const int half_degree = H.n ? (int)std::lround(2.0 * H.nnz / H.n) : 0;
-    // whether the couplings fit 16-bit LDS records decides which candidates exist
+    // whether the couplings fit 16-bit LDS records decides which candidates exist, and so does
+    // a target (only the engines that can stop at one are candidates)
     return TuneKey(H.wm, H.Rp, H.ldim, H.lL, nb, std::min(half_degree, 256), H.S, H.has_h ? 1 : 0,
-                   H.cl_ok ? 1 : 0);
+                   H.cl_ok ? 1 : 0, H.has_target ? 1 : 0);
 }
 
 // Timed trials of every engine and launch shape that can run H.  The first candidate is what
@@ -2057,22 +2134,22 @@ TuneKey tune_key(const HostInst& H) {
 // trials favours no candidate.
 int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry& choice) {
     struct Cand {
-        int engine, block, rg, cl;
+        int engine, block, rg, cl, np;
         double secs;
     };
     std::vector<Cand> cands;
-    cands.push_back({default_engine(H), 0, 0, 0, -1.0});
+    cands.push_back({default_engine(H), 0, 0, 0, 0, -1.0});
     const int blocks[3] = {256, 512, 1024};
     for (int b : blocks) {
-        if (fits_lds(H)) cands.push_back({DRIFT_PT_ENGINE_LDS, b, 0, 0, -1.0});
-        if (H.ldim && slab_fits(H)) cands.push_back({DRIFT_PT_ENGINE_SLAB, b, 0, 0, -1.0});
+        if (fits_lds(H)) cands.push_back({DRIFT_PT_ENGINE_LDS, b, 0, 0, 0, -1.0});
+        if (H.ldim && slab_fits(H)) cands.push_back({DRIFT_PT_ENGINE_SLAB, b, 0, 0, 0, -1.0});
         size_t plds = 0;
         if (H.ldim && pslab_fits(H) && pslab_block(H, b, plds) == b)
-            cands.push_back({DRIFT_PT_ENGINE_PSLAB, b, 0, 0, -1.0});
-        if (H.ldim) cands.push_back({DRIFT_PT_ENGINE_LATTICE, b, 0, 0, -1.0});
-        cands.push_back({DRIFT_PT_ENGINE_GLOBAL, b, 0, 0, -1.0});
+            cands.push_back({DRIFT_PT_ENGINE_PSLAB, b, 0, 0, 0, -1.0});
+        if (H.ldim) cands.push_back({DRIFT_PT_ENGINE_LATTICE, b, 0, 0, 0, -1.0});
+        cands.push_back({DRIFT_PT_ENGINE_GLOBAL, b, 0, 0, 0, -1.0});
         CoopPlan P;
-        if (coop_plan(H, b, P)) cands.push_back({DRIFT_PT_ENGINE_COOP, b, 0, 0, -1.0});
+        if (coop_plan(H, b, P)) cands.push_back({DRIFT_PT_ENGINE_COOP, b, 0, 0, 0, -1.0});
         // the split engine, every group size whose groups can all be resident, with the
         // couplings in HBM and (where they qualify) in LDS
         for (int cl = 0; cl <= (H.cl_ok ? 1 : 0); ++cl)
@@ -2084,15 +2161,42 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
                     continue;
                 }
                 if ((int64_t)per_cu * device_cus() >= (H.R + rg - 1) / rg)
-                    cands.push_back({DRIFT_PT_ENGINE_SPLIT, b, rg, cl, -1.0});
+                    cands.push_back({DRIFT_PT_ENGINE_SPLIT, b, rg, cl, 1, -1.0});
             }
+        // one-slot groups split into parts (the packed spin-word trade), when the groups alone
+        // leave compute units idle: every power of two of parts whose workgroups can all be
+        // resident, at 1024 threads (fewer threads ran slower at every count measured)
+        if (b == 1024 && split_rg_max(H) >= 1 && H.R < device_cus()) {
+            const size_t lds = split_lds(H, 1, false, true);
+            int per_cu = 0;
+            if (lds <= (size_t)device_lds_limit() &&
+                pt_split_occupancy(H.wm, b, lds, &per_cu, false, true) == hipSuccess) {
+                const int64_t cap = (int64_t)per_cu * device_cus();
+                for (int np = 2; np <= 64 && np <= class_words_max(H) && (int64_t)H.R * np <= cap &&
+                                 (int64_t)H.R * np <= 1024;
+                     np *= 2)
+                    cands.push_back({DRIFT_PT_ENGINE_SPLIT, b, 1, 0, np, -1.0});
+            } else {
+                (void)hipGetLastError();
+            }
+        }
+    }
+    if (H.has_target) {
+        // a run that stops at a target: only the engines that can
+        std::vector<Cand> keep;
+        for (const Cand& c : cands)
+            if (c.engine == DRIFT_PT_ENGINE_LDS || c.engine == DRIFT_PT_ENGINE_GLOBAL ||
+                c.engine == DRIFT_PT_ENGINE_LATTICE || c.engine == DRIFT_PT_ENGINE_SPLIT)
+                keep.push_back(c);
+        cands.swap(keep);
     }
     const double per_round = std::max(1.0, (double)H.R * H.n * std::max(H.S, 1));
     const int rounds = trial_rounds > 0 ? trial_rounds
                                         : (int)std::min(2000.0, std::max(4.0, 2.5e8 / per_round));
     HostInst T = H;
     T.flags &= ~(DRIFT_PT_RECORD_HISTORY | DRIFT_PT_RECORD_TRACE | DRIFT_PT_RECORD_FINAL |
-                 DRIFT_PT_TIME_KERNELS);
+                 DRIFT_PT_TIME_KERNELS | DRIFT_PT_STOP_AT_TARGET);
+    T.has_target = 0;   // trials run every round
     std::vector<int8_t> bs((size_t)std::max(H.n, 1));
     drift_pt_result r{};
     r.best_s = bs.data();
@@ -2102,6 +2206,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
     auto trial = [&](Cand& c) {
         double secs = -1.0;
         T.rg_req = c.rg;
+        T.np_req = c.np;
         T.flags = c.cl ? (T.flags | DRIFT_PT_COUPLINGS_LDS) : (T.flags & ~DRIFT_PT_COUPLINGS_LDS);
         T.n_rounds = 2;                                      // warm-up: code objects, first touch
         clear_result(&r);
@@ -2131,6 +2236,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
     choice.block = cands[best].block;
     choice.rg = cands[best].rg;
     choice.cl = cands[best].cl;
+    choice.np = cands[best].np;
     if (table) {
         std::memset(table, 0, sizeof(*table));
         table->trial_rounds = rounds;
@@ -2141,6 +2247,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
             table->cand_block[slot] = c.block;
             table->cand_group[slot] = c.rg;
             table->cand_lds_couplings[slot] = c.cl;
+            table->cand_parts[slot] = c.np;
             table->cand_seconds[slot] = c.secs;
             ++slot;
         }
@@ -2149,6 +2256,7 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
         table->block_threads = choice.block;
         table->slots_per_group = choice.rg;
         table->couplings_in_lds = choice.cl;
+        table->parts_per_group = choice.np;
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[tune_key(H)] = choice;
@@ -2204,7 +2312,7 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     int engine = params->engine, block = params->block_threads;
     if (engine == DRIFT_PT_ENGINE_AUTO) {
         engine = default_engine(H);
-        if ((params->flags & DRIFT_PT_AUTOTUNE) && !H.has_target) {
+        if (params->flags & DRIFT_PT_AUTOTUNE) {
             TuneEntry choice;
             bool hit = false;
             {
@@ -2222,6 +2330,7 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
             engine = choice.engine;
             block = choice.block;
             H.rg_req = choice.rg;
+            H.np_req = choice.np;
             if (choice.cl) H.flags |= DRIFT_PT_COUPLINGS_LDS;
         }
     }

@@ -74,7 +74,21 @@ struct PtInstance {
     int32_t nnz;
     int32_t thr_lds;          // split engine, integer couplings: each round's threshold rows of the
                               // group's slots are copied into LDS (rg * TQ words fit 16 KB)
+    // Split engine with np > 1 parts per group: workgroup (group g, part q) holds the group's
+    // whole state but sweeps only part q's share of each colour class -- the class's 32-position
+    // words (word w = positions 32 w .. 32 w + 31) split into np runs of equal length, run q its
+    // own.  pbits: [sweep parity][Rp slots][nw words] spin words, the step count (sweep t, colour
+    // c: t k + c + 1) over 32 spin bits (1 = -1), all zero at the start.  pbox: [round parity]
+    // [np parts][Rp slots][2] each part's energy change since the start, as box's words.  np = 1:
+    // both null.
+    // pflag: [ng groups][np parts] flags, PT_PFLAG_STRIDE words apart, each the last step whose
+    // words the part has published (zero at the start).
+    uint64_t* pbits;
+    uint64_t* pbox;
+    uint64_t* pflag;
+    int32_t np, nw;
 };
+#define PT_PFLAG_STRIDE 8
 
 // Split engine kernel arguments: the abort flag (set when a mailbox wait exceeds timeout_ticks;
 // every workgroup then leaves), the bound of one wait in device wall-clock ticks, and the work
@@ -213,20 +227,25 @@ __host__ __device__ inline size_t pt_lds_img_bytes(int n, int Rp) { return ((siz
 // and the ladder arrays of all Rp slots) and, with couplings in LDS, the row pointers, the 16-bit
 // records and the field.
 // thr_words: the group's threshold rows staged in LDS (rg * TQ, or 0).
+// parts: the extra arrays of a workgroup of a group split into parts (initial energies of every
+// slot, this part's energy change of each of its slots).
 size_t pt_split_lds_bytes(int n, int rg, int Rp, int k, int thr_words, bool cl = false, int nnz = 0,
-                          bool has_h = false);
+                          bool has_h = false, bool parts = false);
 // One persistent launch of the split engine over work items items[0 .. n_items) (instance index
-// << 10 | group), taken from the queue *d_queue (start at 0), groups of one instance consecutive.
+// << 10 | group; with parts, instance << 10 | group * np + part; the init launch always takes
+// groups), taken from the queue *d_queue (start at 0), the items of one instance consecutive.
 // init: each item draws its group's initial spins and energies (no waiting); otherwise the groups
 // of an instance run every round together, so every group of the largest instance must be
 // resident at once: grid <= the co-resident capacity (pt_split_occupancy), cooperative unless
 // plain.
 // cl: every instance's couplings staged in LDS (PtInstance::crec; int8 couplings only).
+// parts: the kernel of groups split into parts (PtInstance::np, any np >= 1; not with cl).
 hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, int n_items,
                            int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
                            bool init, const SplitArgs& A, bool cooperative, hipStream_t s,
-                           bool cl = false);
-hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu, bool cl = false);
+                           bool cl = false, bool parts = false);
+hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu, bool cl = false,
+                              bool parts = false);
 
 hipError_t pt_launch_g_init(const PtInstance& I, const GState& G, int block, int round_end_grid,
                             hipStream_t s);

@@ -111,6 +111,235 @@ __device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, i
     return sc->abort == 0;
 }
 
+// Split engine with parts: colour class [cs, ce) covers the 32-position words w0 .. w0 + nwc - 1
+// (word w: positions 32 w .. 32 w + 31); part q of np owns the run [ow0, ow1) of them, the runs
+// as equal as integer division makes them, so every part owns a word of any class of at least
+// np words.  Each word of the class has exactly one owner, and holds at least one position of
+// the class.
+struct PartRun {
+    int w0, nwc, ow0, ow1;
+};
+__device__ __forceinline__ PartRun part_run(int cs, int ce, int np, int q) {
+    PartRun r;
+    r.w0 = cs >> 5;
+    r.nwc = ((ce - 1) >> 5) + 1 - r.w0;
+    r.ow0 = r.w0 + (int)(((int64_t)q * r.nwc) / np);
+    r.ow1 = r.w0 + (int)(((int64_t)(q + 1) * r.nwc) / np);
+    return r;
+}
+
+// Sign bits of four int8 spins held in a word (byte b: position 4 i + b) as bits 0..3, and back
+// (bit set: spin -1, else +1).
+__device__ __forceinline__ uint32_t sign_nibble(uint32_t x) {
+    return ((x >> 7) & 1u) | ((x >> 14) & 2u) | ((x >> 21) & 4u) | ((x >> 28) & 8u);
+}
+__device__ __forceinline__ uint32_t nibble_spins(uint32_t b) {
+    const uint32_t e = (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
+    return 0x01010101u | (e * 0xFEu);
+}
+
+// Split engine with parts: a colour class of at most two positions per thread of the workgroup
+// is swept whole by every part of the group -- the same positions, streams and neighbours, so the
+// same result in every part -- instead of being traded: one pass over it costs less than a trade.
+// Part 0 alone counts its energy change.
+__device__ __forceinline__ bool part_replicated(int cs, int ce) { return ce - cs <= 2 * (int)blockDim.x; }
+
+// Split engine with parts, after colour step (sweep t, colour class [cs, ce), traded): publish
+// this part's words of the class for every live slot of the group under the step tag stag, raise
+// this part's flag to stag, wait until the group's other parts have raised theirs, then read the
+// class's other words into the LDS spins (only the positions of the class are written).  The
+// flags only spare the readers from polling every word: each word carries its tag, and a word
+// read before its store is visible is read again until it carries stag or a later step.  A later
+// step is accepted because a word holding the end of one class and the start of the next is
+// written again by the next step's owner, with the first class's bits unchanged.  The words
+// alternate between two buffers by sweep parity: a part writes sweep t + 2's words only after
+// reading every part's words of the largest class of sweep t + 1 (never replicated unless every
+// class is, and every part owns a word of it: the host keeps np at most its word count), each
+// written after its writer had read all of sweep t, so no word is overwritten while a reader
+// still needs it.  False when a wait exceeded its bound or another workgroup gave up (the abort
+// flag is then set, and every workgroup leaves).
+__device__ bool part_trade(const PtInstance& I, const SplitArgs& A, int cs, int ce, int np,
+                           int part, int grp, int a0, int rg, int8_t* sp, uint32_t t,
+                           uint32_t stag, LShared* sc) {
+    const int tid = threadIdx.x, nt = blockDim.x, n = I.n, NW = I.nw;
+    const int rl = I.R - a0 < rg ? I.R - a0 : rg;   // live slots of the group
+    const PartRun pr = part_run(cs, ce, np, part);
+    const int own = pr.ow1 - pr.ow0;
+    uint64_t* mir = I.pbits + ((size_t)(t & 1u) * I.Rp + a0) * NW;   // [slot][word]
+    uint64_t* flags = I.pflag + (size_t)grp * np * PT_PFLAG_STRIDE;
+    for (int idx = tid; idx < own * rl; idx += nt) {
+        const int s = idx % rl, w = pr.ow0 + idx / rl, pb = w << 5;
+        uint32_t bits = 0;
+        if (rg == 1 && pb + 32 <= n) {
+            const uint4 x0 = ((const uint4*)(sp + pb))[0], x1 = ((const uint4*)(sp + pb))[1];
+            bits = sign_nibble(x0.x) | (sign_nibble(x0.y) << 4) | (sign_nibble(x0.z) << 8) |
+                   (sign_nibble(x0.w) << 12) | (sign_nibble(x1.x) << 16) |
+                   (sign_nibble(x1.y) << 20) | (sign_nibble(x1.z) << 24) |
+                   (sign_nibble(x1.w) << 28);
+        } else {
+            for (int b = 0; b < 32 && pb + b < n; ++b)
+                bits |= (uint32_t)(sp[(size_t)(pb + b) * rg + s] < 0) << b;
+        }
+        st_u64(mir + (size_t)s * NW + w, ((uint64_t)stag << 32) | bits);
+    }
+    __syncthreads();
+    if (tid == 0) st_u64(flags + (size_t)part * PT_PFLAG_STRIDE, (uint64_t)stag);
+    auto give_up = [&](unsigned long long t0) {
+        if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
+            __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+        return false;
+    };
+    if (tid < np && tid != part) {
+        const uint64_t* f = flags + (size_t)tid * PT_PFLAG_STRIDE;
+        const unsigned long long t0 = (unsigned long long)wall_clock64();
+        while ((int32_t)((uint32_t)ld_u64(f) - stag) < 0) {
+            if (give_up(t0)) {
+                sc->abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    if (sc->abort) return false;
+    bool ok = true;
+    const int other = pr.nwc - own;
+    for (int idx = tid; idx < other * rl; idx += nt) {
+        const int s = idx % rl;
+        int w = pr.w0 + idx / rl;
+        if (w >= pr.ow0) w += own;
+        const uint64_t* src = mir + (size_t)s * NW + w;
+        const unsigned long long t0 = (unsigned long long)wall_clock64();
+        uint64_t v = ld_u64(src);
+        while ((int32_t)((uint32_t)(v >> 32) - stag) < 0) {
+            if (give_up(t0)) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            v = ld_u64(src);
+        }
+        if (!ok) break;
+        const uint32_t bits = (uint32_t)v;
+        const int pb = w << 5;
+        const int lo = cs > pb ? cs : pb, hi = ce < pb + 32 ? ce : pb + 32;
+        if (rg == 1 && lo == pb && hi == pb + 32) {
+            uint4 x0, x1;
+            x0.x = nibble_spins(bits & 15u);         x0.y = nibble_spins((bits >> 4) & 15u);
+            x0.z = nibble_spins((bits >> 8) & 15u);  x0.w = nibble_spins((bits >> 12) & 15u);
+            x1.x = nibble_spins((bits >> 16) & 15u); x1.y = nibble_spins((bits >> 20) & 15u);
+            x1.z = nibble_spins((bits >> 24) & 15u); x1.w = nibble_spins(bits >> 28);
+            ((uint4*)(sp + pb))[0] = x0;
+            ((uint4*)(sp + pb))[1] = x1;
+        } else {
+            for (int p = lo; p < hi; ++p)
+                sp[(size_t)p * rg + s] = ((bits >> (p & 31)) & 1u) ? (int8_t)-1 : (int8_t)1;
+        }
+    }
+    if (!ok) sc->abort = 1;
+    __syncthreads();
+    return sc->abort == 0;
+}
+
+// Split engine with parts, after round rnd's sweeps: a slot's energy is its initial energy plus
+// the changes every part of its group made.  Each part publishes its slots' changes since the
+// start (two round-tagged words each, as split_gather's, in pbox by round parity); every
+// workgroup sums them (ng np R < PT_SPLIT_HOP2), or workgroup (group 0, part 0) alone does and
+// publishes the sums in box, then tags the round word, and the others read those.  Buffer reuse
+// two rounds apart is safe by split_gather's argument.  False when a wait exceeded its bound.
+__device__ bool part_gather(const PtInstance& I, const SplitArgs& A, int rnd, int grp, int part,
+                            int a0, int rg, int64_t* E, const int64_t* e0l, const int64_t* dpart,
+                            LShared* sc) {
+    const int tid = threadIdx.x, nt = blockDim.x, R = I.R, Rp = I.Rp, np = I.np;
+    const uint32_t tag = (uint32_t)(rnd + 1) & 0xffffu;
+    uint64_t* pb = I.pbox + (size_t)(rnd & 1) * 2 * np * Rp;
+    uint64_t* box = I.box + (size_t)(rnd & 1) * 2 * Rp;
+    uint64_t* round_word = I.box + (size_t)4 * Rp + (rnd & 1);
+    const bool hop2 = I.ng * np * R >= PT_SPLIT_HOP2;
+    const bool collect = !hop2 || (grp == 0 && part == 0);
+    const int rl = R - a0 < rg ? R - a0 : rg;
+    if (tid < rl) {
+        const uint64_t e = (uint64_t)dpart[tid];
+        uint64_t* d = pb + 2 * ((size_t)part * Rp + a0 + tid);
+        st_u64(d, pt_split_word(tag, (uint32_t)e));
+        st_u64(d + 1, pt_split_word(tag, (uint32_t)(e >> 32)));
+    }
+    if (collect)
+        for (int x = tid; x < R; x += nt) E[x] = e0l[x];
+    __syncthreads();
+    bool ok = true;
+    auto give_up = [&](unsigned long long t0) {
+        if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
+            __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+        return false;
+    };
+    // both words of a pair under tag, combined
+    auto take = [&](const uint64_t* w2, int64_t& v) {
+        const unsigned long long t0 = (unsigned long long)wall_clock64();
+        for (;;) {
+            const uint64_t lo = ld_u64(w2), hi = ld_u64(w2 + 1);
+            if ((uint32_t)(lo >> 48) == tag && (uint32_t)(hi >> 48) == tag) {
+                v = (int64_t)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull));
+                return true;
+            }
+            if (give_up(t0)) return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    };
+    if (collect) {
+        for (int idx = tid; idx < R * np && ok; idx += nt) {
+            const int x = idx % R, q = idx / R;
+            int64_t v;
+            if (q == part && x >= a0 && x < a0 + rl) v = dpart[x - a0];
+            else if (!take(pb + 2 * ((size_t)q * Rp + x), v)) {
+                ok = false;
+                break;
+            }
+            if (v != 0) atomicAdd((unsigned long long*)&E[x], (unsigned long long)v);
+        }
+        if (!ok) sc->abort = 1;
+        __syncthreads();
+        if (hop2 && sc->abort == 0) {
+            for (int x = tid; x < R; x += nt) {
+                const uint64_t e = (uint64_t)E[x];
+                st_u64(box + 2 * x, pt_split_word(tag, (uint32_t)e));
+                st_u64(box + 2 * x + 1, pt_split_word(tag, (uint32_t)(e >> 32)));
+            }
+            __syncthreads();
+            if (tid == 0) st_u64(round_word, pt_split_word(tag, 0u));
+        }
+    } else {
+        if (tid == 0) {
+            const unsigned long long t0 = (unsigned long long)wall_clock64();
+            while ((uint32_t)(ld_u64(round_word) >> 48) != tag) {
+                if (give_up(t0)) {
+                    sc->abort = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        if (sc->abort == 0)
+            for (int x = tid; x < R; x += nt) {
+                int64_t v;
+                if (!take(box + 2 * x, v)) {
+                    sc->abort = 1;
+                    break;
+                }
+                E[x] = v;
+            }
+    }
+    __syncthreads();
+    return sc->abort == 0;
+}
+
 // One instance (SPLIT = false: all its slots; SPLIT: the rg slots of group grp, the other groups
 // of the instance running at the same time in other workgroups).
 // INIT: draw the initial spins and energies and leave them in I.img / I.e0 (the launch before the
@@ -122,14 +351,25 @@ __device__ bool split_gather(const PtInstance& I, const SplitArgs& A, int rnd, i
 // the trace and the statistics.
 // CL (split engine, int8 couplings): the instance's row pointers, 16-bit records and field are
 // copied into LDS after the ladder arrays, and the sweeps gather from there (quad_update_cl).
-template <int WM, bool INIT, bool SPLIT, bool CL = false>
-__device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* sc,
+// PARTS (split engine): work item wi is group wi / np, part wi % np; the workgroup holds the
+// group's whole state and sweeps its part's words of each colour class (part_run), trading the
+// class's new spins with the group's other parts after every colour step (part_trade) and the
+// parts' energy changes after every round (part_gather).  Every part then takes the same
+// round-end decisions; the parts of the group holding a new best slot each copy their share of
+// its positions.
+template <int WM, bool INIT, bool SPLIT, bool CL = false, bool PARTS = false>
+__device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* sc,
                              long long* scrE, int* scrR, const SplitArgs& SA) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = I.n, R = I.R, Rp = I.Rp, k = I.k;
+    const int np = PARTS ? I.np : 1;
+    const int part = PARTS ? wi % np : 0;
+    const int grp = PARTS ? wi / np : wi;
     const int rg = SPLIT ? I.rg : Rp;          // slots this workgroup holds
     const int a0 = SPLIT ? grp * rg : 0;       // the first of them
-    const bool owner = !SPLIT || grp == 0;     // writes history, trace and statistics
+    const bool owner = !SPLIT || (grp == 0 && part == 0);   // writes history, trace, statistics
+    // positions this part copies when its group holds a new best slot, and writes at the end
+    const int pp0 = (int)(((int64_t)part * n) / np), pp1 = (int)(((int64_t)(part + 1) * n) / np);
     size_t off = ((size_t)n * rg + 15) & ~(size_t)15;
     int8_t* sp = (int8_t*)smem;
     int64_t* E = (int64_t*)(smem + off);      off += 8 * (size_t)Rp;
@@ -147,6 +387,9 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
     uint32_t* thr_l = (uint32_t*)(smem + off);
     const bool thr_in_lds = SPLIT && WM != PT_WM_F32 && I.thr_lds != 0;
     if (thr_in_lds) off += ((size_t)4 * rg * I.TQ + 15) & ~(size_t)15;
+    int64_t* e0l = (int64_t*)(smem + off);    // PARTS: initial energy of every slot
+    int64_t* dpart = e0l + Rp;                // PARTS: this part's change of each of its slots
+    if constexpr (PARTS) off += ((size_t)8 * (Rp + rg) + 15) & ~(size_t)15;
     int32_t* rpl = (int32_t*)(smem + off);    off += ((size_t)4 * (n + 1) + 15) & ~(size_t)15;
     uint16_t* recl = (uint16_t*)(smem + off); off += ((size_t)2 * I.nnz + 15) & ~(size_t)15;
     int32_t* hl = I.h ? (int32_t*)(smem + off) : nullptr;
@@ -178,6 +421,7 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
     for (int x = tid; x < img16; x += nt) ((uint4*)sp)[x] = ((const uint4*)img)[x];
     for (int x = tid; x < Rp; x += nt) {
         E[x] = x < R ? I.e0[x] : 0; tried[x] = 0; accd[x] = 0; up[x] = 0; down[x] = 0; trips[x] = 0;
+        if constexpr (PARTS) e0l[x] = E[x];
         rung[x] = x;
         rep[x] = x;
         label[x] = (R > 1 && x == 0) ? 1 : ((R > 1 && x == R - 1) ? -1 : 0);
@@ -187,6 +431,8 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
         sc->have = 0; sc->bestE = 0; sc->bround = 0; sc->brung = 0; sc->bslot = 0; sc->abort = 0;
     }
     for (int x = tid; x <= k; x += nt) cptr[x] = I.color_ptr[x];
+    if constexpr (PARTS)
+        for (int x = tid; x < rg; x += nt) dpart[x] = 0;
     if constexpr (CL) {
         for (int x = tid; x <= n; x += nt) rpl[x] = I.row_ptr[x];
         for (int x = tid; x < I.nnz; x += nt) recl[x] = I.crec[x];
@@ -207,7 +453,8 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
     auto copy_best = [&](int slot) {
         if constexpr (SPLIT) {
             if (slot >= a0 && slot < a0 + rg)
-                for (int p = tid; p < n; p += nt) st_spin<true>(I.best_s + p, sp[(size_t)p * rg + slot - a0]);
+                for (int p = pp0 + tid; p < pp1; p += nt)
+                    st_spin<true>(I.best_s + p, sp[(size_t)p * rg + slot - a0]);
         } else {
             for (int p = tid; p < n; p += nt) I.best_s[p] = sp[(size_t)p * Rp + slot];
         }
@@ -249,14 +496,26 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
                             }
                         }
                     } else if (live && ce > cs) {
-                        const int q1 = (ce - 1) >> 2;
+                        int qb = cs >> 2, q1 = (ce - 1) >> 2;
+                        bool count = true;   // this workgroup counts the class's energy change
+                        if constexpr (PARTS) {
+                            if (np > 1 && part_replicated(cs, ce)) {
+                                count = part == 0;
+                            } else {
+                                // this part's quads: those of its words (eight quads each)
+                                const PartRun pr = part_run(cs, ce, np, part);
+                                qb = qb > pr.ow0 * 8 ? qb : pr.ow0 * 8;
+                                q1 = q1 < pr.ow1 * 8 - 1 ? q1 : pr.ow1 * 8 - 1;
+                            }
+                        }
                         auto quads = [&](auto gw) {
-                            for (int qq = (cs >> 2) + qo; qq <= q1; qq += qs) {
+                            for (int qq = qb + qo; qq <= q1; qq += qs) {
                                 const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t,
                                                            PT_TAG_METRO, I.k0, I.k1);
-                                dsum += quad_update<WM, decltype(gw)::value>(
+                                const int64_t d = quad_update<WM, decltype(gw)::value>(
                                     qq, cs, ce, n, al, rg, sp, I.row_ptr, I.edge, I.ell, I.h,
                                     I.ell_rest != 0, rv, thr_row, I.TQ, bf);
+                                if (!PARTS || count) dsum += d;
                             }
                         };
                         switch (I.gq) {
@@ -269,14 +528,23 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
                         }
                     }
                     __syncthreads();
+                    if constexpr (PARTS) {
+                        if (np > 1 && ce > cs && !part_replicated(cs, ce) &&
+                            !part_trade(I, SA, cs, ce, np, part, grp, a0, rg, sp, t,
+                                        t * (uint32_t)k + (uint32_t)c + 1u, sc))
+                            return;
+                    }
                 }
             }
-            if (dsum != 0) atomicAdd((unsigned long long*)&E[a], (unsigned long long)dsum);
+            if (dsum != 0)
+                atomicAdd((unsigned long long*)(PARTS ? &dpart[al] : &E[a]), (unsigned long long)dsum);
             // split: the best-state copy of the last round end is complete before the energies
             // that let other groups pass this round go out
             if constexpr (SPLIT) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
-            if constexpr (SPLIT) {
+            if constexpr (PARTS) {
+                if (!part_gather(I, SA, rnd, grp, part, a0, rg, E, e0l, dpart, sc)) return;
+            } else if constexpr (SPLIT) {
                 if (!split_gather(I, SA, rnd, grp, a0, rg, E, sc)) return;
             }
         }
@@ -386,11 +654,13 @@ __device__ void lds_instance(const PtInstance& I, int grp, char* smem, LShared* 
             pt_out_arr(out, R, PT_OUT_DOWN)[x] = down[x];
         }
     }
-    if (I.final_s)
-        for (int idx = tid; idx < n * rg; idx += nt) {
-            const int s = idx / n, p = idx - s * n;
+    if (I.final_s) {
+        const int len = pp1 - pp0;
+        for (int idx = tid; idx < len * rg; idx += nt) {
+            const int s = idx / len, p = pp0 + idx - s * len;
             if (a0 + s < R) I.final_s[(size_t)(a0 + s) * n + p] = sp[(size_t)p * rg + s];
         }
+    }
 }
 
 template <int WM, bool INIT>
@@ -416,7 +686,7 @@ __global__ __launch_bounds__(1024) void pt_lds_kernel(const PtInstance* __restri
 // Split engine: work items are (instance, group), groups of one instance consecutive in the
 // queue, so a workgroup waiting in a mailbox waits only on items already taken by resident
 // workgroups or on the next ones taken (the grid holds every group of the largest instance).
-template <int WM, bool INIT, bool CL>
+template <int WM, bool INIT, bool CL, bool PARTS = false>
 __global__ __launch_bounds__(1024) void pt_split_kernel(const PtInstance* __restrict__ insts,
                                                         const int32_t* __restrict__ items,
                                                         int n_items, int32_t* queue, SplitArgs SA) {
@@ -448,7 +718,7 @@ __global__ __launch_bounds__(1024) void pt_split_kernel(const PtInstance* __rest
         if (qi >= n_items) return;
         const int item = items[qi];
         const PtInstance I = insts[item >> 10];
-        lds_instance<WM, INIT, true, CL>(I, item & 1023, smem, &sc, scrE, scrR, SA);
+        lds_instance<WM, INIT, true, CL, PARTS>(I, item & 1023, smem, &sc, scrE, scrR, SA);
         __syncthreads();
     }
 }
@@ -908,11 +1178,12 @@ size_t pt_lds_bytes(int n, int Rp, int k) {
 }
 
 size_t pt_split_lds_bytes(int n, int rg, int Rp, int k, int thr_words, bool cl, int nnz,
-                          bool has_h) {
+                          bool has_h, bool parts) {
     size_t b = pt_lds_img_bytes(n, rg) + (size_t)Rp * (5 * 8 + 4 * 4 + 4);
     b = (b + 15) & ~(size_t)15;
     b += ((size_t)4 * (k + 1) + 15) & ~(size_t)15;
     b += ((size_t)4 * thr_words + 15) & ~(size_t)15;
+    if (parts) b += ((size_t)8 * (Rp + rg) + 15) & ~(size_t)15;
     if (cl) {
         b += ((size_t)4 * (n + 1) + 15) & ~(size_t)15;
         b += ((size_t)2 * nnz + 15) & ~(size_t)15;
@@ -921,11 +1192,11 @@ size_t pt_split_lds_bytes(int n, int rg, int Rp, int k, int thr_words, bool cl, 
     return b;
 }
 
-template <int WM, bool INIT, bool CL>
+template <int WM, bool INIT, bool CL, bool PARTS = false>
 static hipError_t split_launch(const PtInstance* d_insts, const int32_t* d_items, int n_items,
                                int32_t* d_queue, int block, int grid, size_t lds_bytes,
                                const SplitArgs& A, bool cooperative, hipStream_t s) {
-    const void* fn = (const void*)pt_split_kernel<WM, INIT, CL>;
+    const void* fn = (const void*)pt_split_kernel<WM, INIT, CL, PARTS>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     if (cooperative) {
@@ -933,15 +1204,23 @@ static hipError_t split_launch(const PtInstance* d_insts, const int32_t* d_items
         void* args[] = {(void*)&d_insts, (void*)&d_items, (void*)&n_items, (void*)&d_queue, (void*)&a};
         return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, (unsigned)lds_bytes, s);
     }
-    pt_split_kernel<WM, INIT, CL><<<grid, block, lds_bytes, s>>>(d_insts, d_items, n_items, d_queue, A);
+    pt_split_kernel<WM, INIT, CL, PARTS><<<grid, block, lds_bytes, s>>>(d_insts, d_items, n_items, d_queue, A);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, int n_items,
                            int32_t* d_queue, int wm, int block, int grid, size_t lds_bytes,
                            bool init, const SplitArgs& A, bool cooperative, hipStream_t s,
-                           bool cl) {
+                           bool cl, bool parts) {
     if (init) cooperative = false;   // no group waits on another while drawing initial states
+    if (parts && !init) {   // the init launch draws by group, as without parts
+        if (cl) return hipErrorInvalidValue;
+        if (wm == PT_WM_I8)
+            return split_launch<PT_WM_I8, false, false, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+        if (wm == PT_WM_I32)
+            return split_launch<PT_WM_I32, false, false, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+        return split_launch<PT_WM_F32, false, false, true>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
+    }
     if (cl) {   // couplings in LDS: int8 couplings only; the init launch draws from the CSR in HBM
         if (wm != PT_WM_I8) return hipErrorInvalidValue;
         return init ? split_launch<PT_WM_I8, true, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s)
@@ -957,8 +1236,12 @@ hipError_t pt_launch_split(const PtInstance* d_insts, const int32_t* d_items, in
                 : split_launch<PT_WM_F32, false, false>(d_insts, d_items, n_items, d_queue, block, grid, lds_bytes, A, cooperative, s);
 }
 
-hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu, bool cl) {
-    const void* fn = cl                ? (const void*)pt_split_kernel<PT_WM_I8, false, true>
+hipError_t pt_split_occupancy(int wm, int block, size_t lds_bytes, int* per_cu, bool cl,
+                              bool parts) {
+    const void* fn = parts             ? (wm == PT_WM_I8    ? (const void*)pt_split_kernel<PT_WM_I8, false, false, true>
+                                          : wm == PT_WM_I32 ? (const void*)pt_split_kernel<PT_WM_I32, false, false, true>
+                                                            : (const void*)pt_split_kernel<PT_WM_F32, false, false, true>)
+                     : cl              ? (const void*)pt_split_kernel<PT_WM_I8, false, true>
                      : wm == PT_WM_I8  ? (const void*)pt_split_kernel<PT_WM_I8, false, false>
                      : wm == PT_WM_I32 ? (const void*)pt_split_kernel<PT_WM_I32, false, false>
                                        : (const void*)pt_split_kernel<PT_WM_F32, false, false>;

@@ -167,7 +167,7 @@ class _Buffers:
 def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, engine,
             block_threads, final, history, trace=False, time_kernels=False, graph=True,
             autotune=False, plain_launch=False, target=None, slots_per_group=0,
-            couplings_in_lds=False):
+            couplings_in_lds=False, parts_per_group=0):
     flags = ((_lib.RECORD_FINAL if final else 0) | (_lib.RECORD_HISTORY if history else 0)
              | (_lib.RECORD_TRACE if trace else 0) | (_lib.TIME_KERNELS if time_kernels else 0)
              | (0 if graph else _lib.NO_GRAPH) | (_lib.AUTOTUNE if autotune else 0)
@@ -189,6 +189,7 @@ def _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, betas, e
         block_threads=int(block_threads),
         target_energy=0.0 if target is None else float(target),
         slots_per_group=int(slots_per_group),
+        parts_per_group=int(parts_per_group),
     )
 
 
@@ -239,6 +240,7 @@ def _to_result(res: _lib.Result, buf: _Buffers, temps, n_rounds, R, history) -> 
         block_threads=int(res.block_threads_used),
         slots_per_group=int(res.slots_per_group_used),
         couplings_in_lds=bool(res.couplings_in_lds_used),
+        parts_per_group=int(res.parts_per_group_used),
     )
 
 
@@ -266,6 +268,7 @@ def parallel_tempering_gpu(
     target_energy: float | None = None,
     slots_per_group: int = 0,
     couplings_in_lds: bool = False,
+    parts_per_group: int = 0,
 ) -> PtResult:
     """Solve ``model`` by parallel tempering on the GPU; same signature and result as the CPU
     ``parallel_tempering`` (drift/gpu.py:43-114).  ``throughput`` is spin-flip attempts per second
@@ -278,14 +281,15 @@ def parallel_tempering_gpu(
     lattice); ``rounds_run`` then says how many rounds ran, and the result is that of a run of
     that many rounds.  ``slots_per_group`` sets the split engine's replica slots per workgroup
     (a power of two up to 64; 0: the library's choice); ``couplings_in_lds`` has it stage the
-    couplings in LDS too (int8 couplings with |w| <= 7, n <= 4096)."""
+    couplings in LDS too (int8 couplings with |w| <= 7, n <= 4096); ``parts_per_group`` splits
+    each group's positions over that many workgroups (1 to 64; 0: the library's choice)."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
     prm = _params(n_replicas, T_min, T_max, n_rounds, sweeps_per_round, seed, b, engine,
                   block_threads, return_final, record_history, record_trace, time_kernels,
                   graph, autotune, plain_launch, target_energy, slots_per_group,
-                  couplings_in_lds)
+                  couplings_in_lds, parts_per_group)
     buf = _Buffers(prob.n, n_replicas, n_rounds, return_final, record_history, record_trace)
     res = buf.struct
     rc = lib.drift_pt_run(prob.struct, prm, res)
@@ -315,6 +319,7 @@ def parallel_tempering_gpu_batch(
     autotune: bool = False,
     plain_launch: bool = False,
     couplings_in_lds: bool = False,
+    parts_per_group: int = 0,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
@@ -331,7 +336,8 @@ def parallel_tempering_gpu_batch(
     choice for the batch's shape class, and runs with it; results do not depend on it.
     ``plain_launch`` starts the split engine's persistent launch with an ordinary dispatch (for
     profilers that cannot follow cooperative launches; needs the device to itself);
-    ``couplings_in_lds`` has the split launch stage every instance's couplings in LDS."""
+    ``couplings_in_lds`` has the split launch stage every instance's couplings in LDS;
+    ``parts_per_group`` splits each of its replica groups' positions over that many workgroups."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
@@ -347,7 +353,8 @@ def parallel_tempering_gpu_batch(
                             "lds", block_threads, return_final, record_history, record_trace,
                             target=None if targets is None else targets[i],
                             slots_per_group=slots_per_group, autotune=autotune,
-                            plain_launch=plain_launch, couplings_in_lds=couplings_in_lds))
+                            plain_launch=plain_launch, couplings_in_lds=couplings_in_lds,
+                            parts_per_group=parts_per_group))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])
@@ -365,9 +372,9 @@ def autotune(model, *, n_replicas: int = 16, T_min: float = 0.1, T_max: float = 
              trial_rounds: int = 0, force_float: bool = False) -> dict:
     """Time every engine and launch shape that can run ``model`` (drift_pt_autotune), cache the
     choice for its shape class, and return {"engine", "block_threads", "slots_per_group",
-    "couplings_in_lds", "trial_rounds", "candidates": [(engine, block, slots_per_group,
-    couplings_in_lds, seconds), ...]} (block 0: the engine's own choice; seconds < 0: the
-    candidate could not run)."""
+    "couplings_in_lds", "parts_per_group", "trial_rounds", "candidates": [(engine, block,
+    slots_per_group, couplings_in_lds, parts_per_group, seconds), ...]} (block 0: the engine's
+    own choice; seconds < 0: the candidate could not run)."""
     lib = _engine_lib()
     temps, b = _ladder(n_replicas, T_min, T_max, betas)
     prob = encode_problem(model, coloring, force_float)
@@ -379,11 +386,12 @@ def autotune(model, *, n_replicas: int = 16, T_min: float = 0.1, T_max: float = 
         raise RuntimeError(f"GPU engine failed (status {rc}): {_lib.last_error(lib)}")
     cands = [(_lib.ENGINE_NAMES.get(int(tab.cand_engine[i]), str(tab.cand_engine[i])),
               int(tab.cand_block[i]), int(tab.cand_group[i]), bool(tab.cand_lds_couplings[i]),
-              float(tab.cand_seconds[i]))
+              int(tab.cand_parts[i]), float(tab.cand_seconds[i]))
              for i in range(tab.n_candidates)]
     return {"engine": _lib.ENGINE_NAMES.get(int(tab.engine), str(tab.engine)),
             "block_threads": int(tab.block_threads), "slots_per_group": int(tab.slots_per_group),
-            "couplings_in_lds": bool(tab.couplings_in_lds), "trial_rounds": int(tab.trial_rounds),
+            "couplings_in_lds": bool(tab.couplings_in_lds),
+            "parts_per_group": int(tab.parts_per_group), "trial_rounds": int(tab.trial_rounds),
             "candidates": cands}
 
 

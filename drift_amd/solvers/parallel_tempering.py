@@ -65,6 +65,7 @@ class PtResult:
     block_threads: int = None            # workgroup size of the sweep kernel
     slots_per_group: int = None          # replica slots per workgroup (split and LDS engines)
     couplings_in_lds: bool = None        # split engine: couplings staged in LDS
+    parts_per_group: int = None          # split engine: workgroups per replica group
 
 
 def _neighbour_access(model):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_PT_ABI_VERSION 5
+#define DRIFT_PT_ABI_VERSION 6
 
 enum drift_pt_status {
     DRIFT_PT_OK = 0,
@@ -65,7 +65,12 @@ enum drift_pt_engine {
                                    LDS for the whole run; after each round's sweeps the groups
                                    trade slot energies through round-tagged mailboxes in HBM and
                                    all take the same exchange decisions.  One persistent launch;
-                                   every group of an instance is resident at once */
+                                   every group of an instance is resident at once.  With
+                                   params.parts_per_group = P > 1 each group's positions are
+                                   split into P parts, one workgroup each (every one holding the
+                                   group's whole state): after each colour step the parts trade
+                                   that class's new spins as step-tagged bit words, so one large
+                                   instance with few replicas fills the chip */
 };
 
 enum drift_pt_flags {
@@ -75,8 +80,10 @@ enum drift_pt_flags {
     DRIFT_PT_RECORD_TRACE = 8,    /* fill result.rung_energy_trace */
     DRIFT_PT_TIME_KERNELS = 16,   /* time each sweep-kernel launch (up to 4096) with device events */
     DRIFT_PT_NO_GRAPH = 32,       /* launch every round directly instead of replaying a hipGraph */
-    DRIFT_PT_AUTOTUNE = 64,       /* with DRIFT_PT_ENGINE_AUTO: take the engine and block size from
-                                     the autotune cache, timing the candidates on a miss */
+    DRIFT_PT_AUTOTUNE = 64,       /* with DRIFT_PT_ENGINE_AUTO: take the engine and launch shape
+                                     from the autotune cache, timing the candidates on a miss
+                                     (with DRIFT_PT_STOP_AT_TARGET: among the engines that can
+                                     stop at a target) */
     DRIFT_PT_PLAIN_LAUNCH = 128,  /* persistent engines (coop, pslab): an ordinary dispatch of the
                                      occupancy-sized grid instead of a cooperative launch.  All
                                      workgroups are resident when nothing else runs on the device;
@@ -126,6 +133,9 @@ typedef struct drift_pt_params {
     double target_energy;       /* with DRIFT_PT_STOP_AT_TARGET: the energy to stop at */
     int32_t slots_per_group;    /* SPLIT engine: replica slots per workgroup, a power of two up
                                    to 64; 0 = chosen by the library (or the tuner) */
+    int32_t parts_per_group;    /* SPLIT engine: workgroups sharing each group's positions, 1 to
+                                   64 (groups x parts <= 1024 per instance); 0 = chosen by the
+                                   library (or the tuner) */
 } drift_pt_params;
 
 /* Results, all arrays caller-allocated (NULL = not wanted, except best_s). */
@@ -161,6 +171,8 @@ typedef struct drift_pt_result {
     int32_t slots_per_group_used; /* replica slots per workgroup (the SPLIT engine's groups; the
                                      padded slot count on the LDS engine; 0 on the others) */
     int32_t couplings_in_lds_used; /* 1 when the couplings were staged in LDS */
+    int32_t parts_per_group_used;  /* SPLIT: workgroups per replica group (1 = one each); 0 on
+                                      the other engines */
 } drift_pt_result;
 
 /* Timed candidates of one autotune call (drift_pt_autotune). */
@@ -170,12 +182,14 @@ typedef struct drift_pt_tuning {
     int32_t block_threads;
     int32_t slots_per_group;                 /* SPLIT: replica slots per workgroup, else 0 */
     int32_t couplings_in_lds;                /* SPLIT: 1 with the couplings staged in LDS */
+    int32_t parts_per_group;                 /* SPLIT: workgroups per replica group, else 0 */
     int32_t trial_rounds;                    /* rounds each timed trial ran */
     int32_t n_candidates;
     int32_t cand_engine[DRIFT_PT_TUNE_MAX];
     int32_t cand_block[DRIFT_PT_TUNE_MAX];
     int32_t cand_group[DRIFT_PT_TUNE_MAX];
     int32_t cand_lds_couplings[DRIFT_PT_TUNE_MAX];
+    int32_t cand_parts[DRIFT_PT_TUNE_MAX];
     double cand_seconds[DRIFT_PT_TUNE_MAX];  /* device time of trial_rounds rounds (a run of
                                                 2 * trial_rounds minus a run of trial_rounds, so
                                                 fixed costs cancel); < 0 if it could not run */

@@ -314,17 +314,18 @@ def test_autotune_times_candidates_and_keeps_results():
     gpu.autotune_clear()
     model = er_sparse_weighted(6000, 6.0, 5)
     tab = gpu.autotune(model, n_replicas=32, sweeps_per_round=1, trial_rounds=20)
-    ran = [c for c in tab["candidates"] if c[4] > 0]
+    ran = [c for c in tab["candidates"] if c[-1] > 0]
     engines = {c[0] for c in ran}
     assert {"global", "coop", "split"} <= engines and len(ran) >= 6
     # the default (first candidate) stays unless another is more than 3% faster
-    best = min(ran, key=lambda c: c[4])
+    best = min(ran, key=lambda c: c[-1])
     default = tab["candidates"][0]
-    chosen = (tab["engine"], tab["block_threads"], tab["slots_per_group"], tab["couplings_in_lds"])
-    if default[4] > 0 and best[4] > default[4] / 1.03:
-        assert chosen == default[:4]
+    chosen = (tab["engine"], tab["block_threads"], tab["slots_per_group"], tab["couplings_in_lds"],
+              tab["parts_per_group"])
+    if default[-1] > 0 and best[-1] > default[-1] / 1.03:
+        assert chosen == default[:5]
     else:
-        assert chosen == best[:4]
+        assert chosen == best[:5]
     enc = gpu.encode_problem(model)
     k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
     beta = ladder_betas(geometric_ladder(32, 0.1, 5.0))
