@@ -35,6 +35,8 @@ candidate is more than 3% faster; --no-autotune skips that.  --plain-launch star
 engines (pslab, coop, split) with an ordinary dispatch, for profiler commands; timed runs use the
 cooperative launch.  --workload ladder-c5 measures time to target
 with the learned ladder instead, in three stages (--stage train, bestknown, eval).
+--workload single-heldout times one held-out instance per family (N ~ 65536) on each engine that
+can run it alone, the split engine with its groups in parts among them.
 """
 
 from __future__ import annotations
@@ -956,6 +958,58 @@ def bench_c3_ttt(args) -> dict:
             "target_run_wall_s": round(long_wall, 2), "runs": runs}
 
 
+def bench_single_heldout(args) -> dict:
+    """One instance at a time: the first held-out instance of each family of the learned-ladder
+    mix (N ~ 65536, R = 32, geometric ladder, one sweep per round, --steps rounds) on the default
+    engine, the per-launch engine, one-slot split groups in 4 and 8 parts and (unless
+    --no-autotune) the autotuner's choice; --engine E keeps only the configurations that name E
+    (a profiler command's way to leave out the graph-replayed engines).  Each value is the median of --repeats timed runs after
+    one untimed run, in spin-flip attempts per ns; every configuration must end at the same best
+    energy (one chain)."""
+    from drift_amd import gpu, ladder
+
+    fams = args.families.split(",") if args.families else list(ladder.MIX_FAMILIES)
+    steps = args.steps
+    configs = [("default", dict(engine="auto")), ("global", dict(engine="global")),
+               ("split 1x4", dict(engine="split", slots_per_group=1, parts_per_group=4,
+                                  block_threads=1024)),
+               ("split 1x8", dict(engine="split", slots_per_group=1, parts_per_group=8,
+                                  block_threads=1024))]
+    if args.autotune:
+        configs.append(("autotuned", dict(engine="auto", autotune=True)))
+    if args.engine != "auto":   # --engine E: only the configurations that name E
+        configs = [c for c in configs if c[1]["engine"] == args.engine]
+    out = {}
+    for fi, (fam, (T_min, T_max)) in enumerate(ladder.MIX_FAMILIES.items()):
+        if fam not in fams:
+            continue
+        model = ladder.mix_instance(fam, 900_000 + 1000 * fi, args.test_n)
+        kw = dict(n_replicas=32, T_min=T_min, T_max=T_max, n_rounds=steps, sweeps_per_round=1,
+                  seed=1, record_history=False, plain_launch=args.plain_launch)
+        rows, best = {}, set()
+        for name, extra in configs:
+            log(f"single held-out: {fam} (N = {model.n}) on {name}")
+            gpu.parallel_tempering_gpu(model, **dict(kw, **extra))
+            runs = [gpu.parallel_tempering_gpu(model, **dict(kw, **extra))
+                    for _ in range(max(args.repeats, 1))]
+            secs = float(np.median([r.seconds for r in runs]))
+            r = runs[0]
+            best.add(r.best_E_units)
+            rows[name] = {"attempts_per_ns": round(32 * steps * model.n / secs / 1e9, 2),
+                          "engine": r.engine, "block_threads": r.block_threads,
+                          "slots_per_group": r.slots_per_group,
+                          "parts_per_group": r.parts_per_group}
+        if len(best) != 1:
+            raise RuntimeError(f"single held-out: engines disagree on {fam}")
+        out[fam] = {"n": model.n, "engines": rows}
+    return {"metric": "single-instance throughput by engine (held-out instances, N ~ 65536)",
+            "unit": "attempts/ns", "higher_is_better": True, "n_gpus": 1, "steps": steps,
+            "repeats": args.repeats, "data": "synthetic",
+            "config": {"workload": "first held-out instance of each family of the learned-ladder "
+                                   "mix, R=32, geometric ladder, one sweep per round"},
+            "families": out}
+
+
 def bench_ladder_c5(args) -> int:
     stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval,
              "eval-merge": c5_eval_merge}[args.stage]
@@ -972,7 +1026,7 @@ def main() -> int:
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
                     choices=("all", "c4", "c2", "c3", "c1", "published", "c5-batch", "c3-ttt",
-                             "ladder-c5", "ttt-sample"))
+                             "ladder-c5", "ttt-sample", "single-heldout"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
@@ -1041,6 +1095,9 @@ def main() -> int:
         return 0
     if args.workload == "ttt-sample":
         print(json.dumps(bench_ttt_sample()), flush=True)
+        return 0
+    if args.workload == "single-heldout":
+        print(json.dumps(bench_single_heldout(args)), flush=True)
         return 0
     return bench_throughput(args)
 
