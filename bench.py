@@ -962,8 +962,9 @@ def bench_single_heldout(args) -> dict:
     """One instance at a time: the first held-out instance of each family of the learned-ladder
     mix (N ~ 65536, R = 32, geometric ladder, one sweep per round, --steps rounds) on the default
     engine, the per-launch engine, one-slot split groups in 4 and 8 parts and (unless
-    --no-autotune) the autotuner's choice; --engine E keeps only the configurations that name E
-    (a profiler command's way to leave out the graph-replayed engines).  Each value is the median of --repeats timed runs after
+    --no-autotune) the autotuner's choice; --engine E keeps only the configurations named E
+    (e.g. "split 1x8") or on engine E (a profiler command's way to leave out the graph-replayed
+    engines).  Each value is the median of --repeats timed runs after
     one untimed run, in spin-flip attempts per ns; every configuration must end at the same best
     energy (one chain)."""
     from drift_amd import gpu, ladder
@@ -977,8 +978,8 @@ def bench_single_heldout(args) -> dict:
                                   block_threads=1024))]
     if args.autotune:
         configs.append(("autotuned", dict(engine="auto", autotune=True)))
-    if args.engine != "auto":   # --engine E: only the configurations that name E
-        configs = [c for c in configs if c[1]["engine"] == args.engine]
+    if args.engine != "auto":   # --engine E: only the configurations named E or on engine E
+        configs = [c for c in configs if args.engine in (c[0], c[1]["engine"])]
     out = {}
     for fi, (fam, (T_min, T_max)) in enumerate(ladder.MIX_FAMILIES.items()):
         if fam not in fams:

@@ -614,7 +614,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
     const size_t out_end = cursor;   // statistics blocks are the region that must start at zero
     for (size_t i = 0; i < m; ++i) {
         oo[i].box = rgs ? take(sizeof(uint64_t) * (4 * (size_t)hs[i].Rp + 2)) : SIZE_MAX;
-        oo[i].pbox = np_of(i) > 1 ? take(sizeof(uint64_t) * 4 * (size_t)np_of(i) * hs[i].Rp)
+        oo[i].pbox = np_of(i) > 1 ? take(sizeof(uint64_t) * 8 * (size_t)np_of(i) * hs[i].Rp)
                                   : SIZE_MAX;
     }
     const size_t box_end = cursor;   // mailboxes start with every bit set

@@ -76,11 +76,13 @@ struct PtInstance {
                               // group's slots are copied into LDS (rg * TQ words fit 16 KB)
     // Split engine with np > 1 parts per group: workgroup (group g, part q) holds the group's
     // whole state but sweeps only part q's share of each colour class -- the class's 32-position
-    // words (word w = positions 32 w .. 32 w + 31) split into np runs of equal length, run q its
-    // own.  pbits: [sweep parity][Rp slots][nw words] spin words, the step count (sweep t, colour
-    // c: t k + c + 1) over 32 spin bits (1 = -1), all zero at the start.  pbox: [round parity]
-    // [np parts][Rp slots][2] each part's energy change since the start, as box's words.  np = 1:
-    // both null.
+    // words (word w = positions 32 w .. 32 w + 31) split into np runs, as equal as integer
+    // division makes them, run q its own.  pbits: [sweep parity][Rp slots][nw words] spin words,
+    // the step count (sweep t, colour c: t k + c + 1) over 32 spin bits (1 = -1), all zero at
+    // the start.  pbox: two halves of
+    // [round parity][np parts][Rp slots][2] words in box's format: each part's energy change on
+    // traded classes since the start, then each part's copy of its group's slot energies (all
+    // bits set at the start).  np = 1: null.
     // pflag: [ng groups][np parts] flags, PT_PFLAG_STRIDE words apart, each the last step whose
     // words the part has published (zero at the start).
     uint64_t* pbits;

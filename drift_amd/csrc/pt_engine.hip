@@ -141,7 +141,7 @@ __device__ __forceinline__ uint32_t nibble_spins(uint32_t b) {
 // Split engine with parts: a colour class of at most two positions per thread of the workgroup
 // is swept whole by every part of the group -- the same positions, streams and neighbours, so the
 // same result in every part -- instead of being traded: one pass over it costs less than a trade.
-// Part 0 alone counts its energy change.
+// Every part keeps the class's energy change apart (drep), and the group's total counts it once.
 __device__ __forceinline__ bool part_replicated(int cs, int ce) { return ce - cs <= 2 * (int)blockDim.x; }
 
 // Split engine with parts, after colour step (sweep t, colour class [cs, ce), traded): publish
@@ -156,11 +156,15 @@ __device__ __forceinline__ bool part_replicated(int cs, int ce) { return ce - cs
 // reading every part's words of the largest class of sweep t + 1 (never replicated unless every
 // class is, and every part owns a word of it: the host keeps np at most its word count), each
 // written after its writer had read all of sweep t, so no word is overwritten while a reader
-// still needs it.  False when a wait exceeded its bound or another workgroup gave up (the abort
-// flag is then set, and every workgroup leaves).
+// still needs it.  ride (the round's last traded step): each part also publishes its slots'
+// energy changes on traded classes since the start (dpart, two round-tagged words each in the
+// first half of pbox, by round parity), and after the flags every part sums the group's into
+// gpart, so the round end needs no hop inside the group.  False when a wait exceeded its bound
+// or another workgroup gave up (the abort flag is then set, and every workgroup leaves).
 __device__ bool part_trade(const PtInstance& I, const SplitArgs& A, int cs, int ce, int np,
                            int part, int grp, int a0, int rg, int8_t* sp, uint32_t t,
-                           uint32_t stag, LShared* sc) {
+                           uint32_t stag, LShared* sc, bool ride, int rnd, const int64_t* dpart,
+                           int64_t* gpart) {
     const int tid = threadIdx.x, nt = blockDim.x, n = I.n, NW = I.nw;
     const int rl = I.R - a0 < rg ? I.R - a0 : rg;   // live slots of the group
     const PartRun pr = part_run(cs, ce, np, part);
@@ -181,6 +185,15 @@ __device__ bool part_trade(const PtInstance& I, const SplitArgs& A, int cs, int 
                 bits |= (uint32_t)(sp[(size_t)(pb + b) * rg + s] < 0) << b;
         }
         st_u64(mir + (size_t)s * NW + w, ((uint64_t)stag << 32) | bits);
+    }
+    const uint32_t rtag = (uint32_t)(rnd + 1) & 0xffffu;
+    uint64_t* pb = I.pbox + (size_t)(rnd & 1) * 2 * np * I.Rp;   // [part][slot][2]
+    if (ride && tid < rl) {
+        const uint64_t e = (uint64_t)dpart[tid];
+        uint64_t* d = pb + 2 * ((size_t)part * I.Rp + a0 + tid);
+        st_u64(d, pt_split_word(rtag, (uint32_t)e));
+        st_u64(d + 1, pt_split_word(rtag, (uint32_t)(e >> 32)));
+        gpart[tid] = 0;
     }
     __syncthreads();
     if (tid == 0) st_u64(flags + (size_t)part * PT_PFLAG_STRIDE, (uint64_t)stag);
@@ -206,6 +219,28 @@ __device__ bool part_trade(const PtInstance& I, const SplitArgs& A, int cs, int 
     __syncthreads();
     if (sc->abort) return false;
     bool ok = true;
+    if (ride)
+        for (int idx = tid; idx < np * rl && ok; idx += nt) {
+            const int q = idx / rl, x = idx - q * rl;
+            int64_t v = dpart[x];
+            if (q != part) {
+                const uint64_t* w2 = pb + 2 * ((size_t)q * I.Rp + a0 + x);
+                const unsigned long long t0 = (unsigned long long)wall_clock64();
+                for (;;) {
+                    const uint64_t lo = ld_u64(w2), hi = ld_u64(w2 + 1);
+                    if ((uint32_t)(lo >> 48) == rtag && (uint32_t)(hi >> 48) == rtag) {
+                        v = (int64_t)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull));
+                        break;
+                    }
+                    if (give_up(t0)) {
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (ok && v != 0) atomicAdd((unsigned long long*)&gpart[x], (unsigned long long)v);
+        }
     const int other = pr.nwc - own;
     for (int idx = tid; idx < other * rl; idx += nt) {
         const int s = idx % rl;
@@ -245,97 +280,50 @@ __device__ bool part_trade(const PtInstance& I, const SplitArgs& A, int cs, int 
 }
 
 // Split engine with parts, after round rnd's sweeps: a slot's energy is its initial energy plus
-// the changes every part of its group made.  Each part publishes its slots' changes since the
-// start (two round-tagged words each, as split_gather's, in pbox by round parity); every
-// workgroup sums them (ng np R < PT_SPLIT_HOP2), or workgroup (group 0, part 0) alone does and
-// publishes the sums in box, then tags the round word, and the others read those.  Buffer reuse
-// two rounds apart is safe by split_gather's argument.  False when a wait exceeded its bound.
-__device__ bool part_gather(const PtInstance& I, const SplitArgs& A, int rnd, int grp, int part,
-                            int a0, int rg, int64_t* E, const int64_t* e0l, const int64_t* dpart,
+// its group's changes -- on traded classes the sum over the group's parts (gpart, gathered at
+// the round's last traded step; a group in one part, or trading nothing, has only its own,
+// dpart) and on the classes every part sweeps whole (drep, the same in every part).  Every part
+// of a group so holds its slots' energies; each publishes them (two round-tagged words per slot,
+// in the second half of pbox by round parity, under its part index), and every workgroup reads
+// the other groups' slots from the part of its own index.  A part writes round r + 2's words
+// only after reading every slot's words of round r + 1 from the parts of its index, each written
+// after its writer had read round r, so two buffers suffice.  False when a wait exceeded its
+// bound.
+__device__ bool part_gather(const PtInstance& I, const SplitArgs& A, int rnd, int part, int a0,
+                            int rg, int64_t* E, const int64_t* e0l, const int64_t* dpart,
+                            const int64_t* gpart, const int64_t* drep, bool traded,
                             LShared* sc) {
     const int tid = threadIdx.x, nt = blockDim.x, R = I.R, Rp = I.Rp, np = I.np;
     const uint32_t tag = (uint32_t)(rnd + 1) & 0xffffu;
-    uint64_t* pb = I.pbox + (size_t)(rnd & 1) * 2 * np * Rp;
-    uint64_t* box = I.box + (size_t)(rnd & 1) * 2 * Rp;
-    uint64_t* round_word = I.box + (size_t)4 * Rp + (rnd & 1);
-    const bool hop2 = I.ng * np * R >= PT_SPLIT_HOP2;
-    const bool collect = !hop2 || (grp == 0 && part == 0);
+    uint64_t* gb = I.pbox + (size_t)4 * np * Rp + ((size_t)(rnd & 1) * np + part) * 2 * Rp;
     const int rl = R - a0 < rg ? R - a0 : rg;
     if (tid < rl) {
-        const uint64_t e = (uint64_t)dpart[tid];
-        uint64_t* d = pb + 2 * ((size_t)part * Rp + a0 + tid);
-        st_u64(d, pt_split_word(tag, (uint32_t)e));
-        st_u64(d + 1, pt_split_word(tag, (uint32_t)(e >> 32)));
+        const int64_t e = e0l[a0 + tid] + (np > 1 && traded ? gpart[tid] : dpart[tid]) + drep[tid];
+        E[a0 + tid] = e;
+        st_u64(gb + 2 * (a0 + tid), pt_split_word(tag, (uint32_t)(uint64_t)e));
+        st_u64(gb + 2 * (a0 + tid) + 1, pt_split_word(tag, (uint32_t)((uint64_t)e >> 32)));
     }
-    if (collect)
-        for (int x = tid; x < R; x += nt) E[x] = e0l[x];
-    __syncthreads();
     bool ok = true;
-    auto give_up = [&](unsigned long long t0) {
-        if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-            (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
-            __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return true;
-        }
-        return false;
-    };
-    // both words of a pair under tag, combined
-    auto take = [&](const uint64_t* w2, int64_t& v) {
+    for (int x = tid; x < R && ok; x += nt) {
+        if (x >= a0 && x < a0 + rl) continue;
+        const uint64_t* w2 = gb + 2 * x;
         const unsigned long long t0 = (unsigned long long)wall_clock64();
         for (;;) {
             const uint64_t lo = ld_u64(w2), hi = ld_u64(w2 + 1);
             if ((uint32_t)(lo >> 48) == tag && (uint32_t)(hi >> 48) == tag) {
-                v = (int64_t)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull));
-                return true;
+                E[x] = (int64_t)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull));
+                break;
             }
-            if (give_up(t0)) return false;
-            __builtin_amdgcn_s_sleep(1);
-        }
-    };
-    if (collect) {
-        for (int idx = tid; idx < R * np && ok; idx += nt) {
-            const int x = idx % R, q = idx / R;
-            int64_t v;
-            if (q == part && x >= a0 && x < a0 + rl) v = dpart[x - a0];
-            else if (!take(pb + 2 * ((size_t)q * Rp + x), v)) {
+            if (__hip_atomic_load(A.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                (unsigned long long)wall_clock64() - t0 > A.timeout_ticks) {
+                __hip_atomic_store(A.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = false;
                 break;
             }
-            if (v != 0) atomicAdd((unsigned long long*)&E[x], (unsigned long long)v);
+            __builtin_amdgcn_s_sleep(1);
         }
-        if (!ok) sc->abort = 1;
-        __syncthreads();
-        if (hop2 && sc->abort == 0) {
-            for (int x = tid; x < R; x += nt) {
-                const uint64_t e = (uint64_t)E[x];
-                st_u64(box + 2 * x, pt_split_word(tag, (uint32_t)e));
-                st_u64(box + 2 * x + 1, pt_split_word(tag, (uint32_t)(e >> 32)));
-            }
-            __syncthreads();
-            if (tid == 0) st_u64(round_word, pt_split_word(tag, 0u));
-        }
-    } else {
-        if (tid == 0) {
-            const unsigned long long t0 = (unsigned long long)wall_clock64();
-            while ((uint32_t)(ld_u64(round_word) >> 48) != tag) {
-                if (give_up(t0)) {
-                    sc->abort = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        __syncthreads();
-        if (sc->abort == 0)
-            for (int x = tid; x < R; x += nt) {
-                int64_t v;
-                if (!take(box + 2 * x, v)) {
-                    sc->abort = 1;
-                    break;
-                }
-                E[x] = v;
-            }
     }
+    if (!ok) sc->abort = 1;
     __syncthreads();
     return sc->abort == 0;
 }
@@ -353,8 +341,9 @@ __device__ bool part_gather(const PtInstance& I, const SplitArgs& A, int rnd, in
 // copied into LDS after the ladder arrays, and the sweeps gather from there (quad_update_cl).
 // PARTS (split engine): work item wi is group wi / np, part wi % np; the workgroup holds the
 // group's whole state and sweeps its part's words of each colour class (part_run), trading the
-// class's new spins with the group's other parts after every colour step (part_trade) and the
-// parts' energy changes after every round (part_gather).  Every part then takes the same
+// class's new spins with the group's other parts after every colour step (part_trade, which at
+// the round's last traded step also sums the group's energy changes) and the slots' energies
+// after every round (part_gather).  Every part then takes the same
 // round-end decisions; the parts of the group holding a new best slot each copy their share of
 // its positions.
 template <int WM, bool INIT, bool SPLIT, bool CL = false, bool PARTS = false>
@@ -388,8 +377,11 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
     const bool thr_in_lds = SPLIT && WM != PT_WM_F32 && I.thr_lds != 0;
     if (thr_in_lds) off += ((size_t)4 * rg * I.TQ + 15) & ~(size_t)15;
     int64_t* e0l = (int64_t*)(smem + off);    // PARTS: initial energy of every slot
-    int64_t* dpart = e0l + Rp;                // PARTS: this part's change of each of its slots
-    if constexpr (PARTS) off += ((size_t)8 * (Rp + rg) + 15) & ~(size_t)15;
+    int64_t* dpart = e0l + Rp;                // PARTS: this part's change of each of its slots on
+                                              // traded classes, since the start
+    int64_t* gpart = dpart + rg;              // PARTS: the group's sum of dpart
+    int64_t* drep = gpart + rg;               // PARTS: the change on classes every part sweeps
+    if constexpr (PARTS) off += ((size_t)8 * (Rp + 3 * rg) + 15) & ~(size_t)15;
     int32_t* rpl = (int32_t*)(smem + off);    off += ((size_t)4 * (n + 1) + 15) & ~(size_t)15;
     uint16_t* recl = (uint16_t*)(smem + off); off += ((size_t)2 * I.nnz + 15) & ~(size_t)15;
     int32_t* hl = I.h ? (int32_t*)(smem + off) : nullptr;
@@ -432,7 +424,7 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
     }
     for (int x = tid; x <= k; x += nt) cptr[x] = I.color_ptr[x];
     if constexpr (PARTS)
-        for (int x = tid; x < rg; x += nt) dpart[x] = 0;
+        for (int x = tid; x < rg; x += nt) dpart[x] = gpart[x] = drep[x] = 0;
     if constexpr (CL) {
         for (int x = tid; x <= n; x += nt) rpl[x] = I.row_ptr[x];
         for (int x = tid; x < I.nnz; x += nt) recl[x] = I.crec[x];
@@ -440,6 +432,14 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
             for (int x = tid; x < n; x += nt) hl[x] = ((const int32_t*)I.h)[x];
     }
     __syncthreads();
+
+    // PARTS: the last colour class the parts trade (-1: none), whose step in a round's last sweep
+    // carries the parts' energy changes
+    int last_tc = -1;
+    if constexpr (PARTS)
+        if (np > 1)
+            for (int c = 0; c < k; ++c)
+                if (cptr[c + 1] > cptr[c] && !part_replicated(cptr[c], cptr[c + 1])) last_tc = c;
 
     // R <= 64: the round end is wave 0's (WaveLadder), with one workgroup barrier per round.  The
     // ladder stays in the LDS arrays between round ends, so no registers are held over the sweeps.
@@ -497,10 +497,10 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
                         }
                     } else if (live && ce > cs) {
                         int qb = cs >> 2, q1 = (ce - 1) >> 2;
-                        bool count = true;   // this workgroup counts the class's energy change
+                        bool whole = false;   // every part sweeps the class
                         if constexpr (PARTS) {
                             if (np > 1 && part_replicated(cs, ce)) {
-                                count = part == 0;
+                                whole = true;
                             } else {
                                 // this part's quads: those of its words (eight quads each)
                                 const PartRun pr = part_run(cs, ce, np, part);
@@ -515,7 +515,13 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
                                 const int64_t d = quad_update<WM, decltype(gw)::value>(
                                     qq, cs, ce, n, al, rg, sp, I.row_ptr, I.edge, I.ell, I.h,
                                     I.ell_rest != 0, rv, thr_row, I.TQ, bf);
-                                if (!PARTS || count) dsum += d;
+                                // classes every part sweeps: straight into drep (they are small)
+                                if (PARTS && whole) {
+                                    if (d != 0)
+                                        atomicAdd((unsigned long long*)&drep[al], (unsigned long long)d);
+                                } else {
+                                    dsum += d;
+                                }
                             }
                         };
                         switch (I.gq) {
@@ -527,11 +533,19 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
                             default: quads(std::integral_constant<int, 4>{}); break;
                         }
                     }
+                    const bool ride = PARTS && c == last_tc && j == I.S - 1;
+                    if (ride) {
+                        // the traded classes' changes so far go out with this step's words
+                        if (dsum != 0)
+                            atomicAdd((unsigned long long*)&dpart[al], (unsigned long long)dsum);
+                        dsum = 0;
+                    }
                     __syncthreads();
                     if constexpr (PARTS) {
                         if (np > 1 && ce > cs && !part_replicated(cs, ce) &&
                             !part_trade(I, SA, cs, ce, np, part, grp, a0, rg, sp, t,
-                                        t * (uint32_t)k + (uint32_t)c + 1u, sc))
+                                        t * (uint32_t)k + (uint32_t)c + 1u, sc, ride, rnd, dpart,
+                                        gpart))
                             return;
                     }
                 }
@@ -543,7 +557,9 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
             if constexpr (SPLIT) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             if constexpr (PARTS) {
-                if (!part_gather(I, SA, rnd, grp, part, a0, rg, E, e0l, dpart, sc)) return;
+                if (!part_gather(I, SA, rnd, part, a0, rg, E, e0l, dpart, gpart, drep,
+                                 last_tc >= 0, sc))
+                    return;
             } else if constexpr (SPLIT) {
                 if (!split_gather(I, SA, rnd, grp, a0, rg, E, sc)) return;
             }
@@ -1183,7 +1199,7 @@ size_t pt_split_lds_bytes(int n, int rg, int Rp, int k, int thr_words, bool cl, 
     b = (b + 15) & ~(size_t)15;
     b += ((size_t)4 * (k + 1) + 15) & ~(size_t)15;
     b += ((size_t)4 * thr_words + 15) & ~(size_t)15;
-    if (parts) b += ((size_t)8 * (Rp + rg) + 15) & ~(size_t)15;
+    if (parts) b += ((size_t)8 * (Rp + 3 * rg) + 15) & ~(size_t)15;
     if (cl) {
         b += ((size_t)4 * (n + 1) + 15) & ~(size_t)15;
         b += ((size_t)2 * nnz + 15) & ~(size_t)15;
