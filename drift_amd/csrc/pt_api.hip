@@ -2582,7 +2582,8 @@ int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs
         if (rc) return rc;
     }
     if (big.empty()) return 0;
-    // the split engine when its groups fill the chip; the fits are checked per instance
+    // the split engine when its workgroups (groups, times the parts asked for) fill the chip; the
+    // fits are checked per instance
     std::vector<HostInst> cand;
     std::vector<size_t> cand_idx, rest;
     for (size_t i : big) {
@@ -2596,7 +2597,8 @@ int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs
     if (!cand.empty()) {
         const std::vector<int> rgs = split_rgs(cand);
         int64_t groups = 0;
-        for (size_t j = 0; j < cand.size(); ++j) groups += (cand[j].R + rgs[j] - 1) / rgs[j];
+        for (size_t j = 0; j < cand.size(); ++j)
+            groups += (int64_t)((cand[j].R + rgs[j] - 1) / rgs[j]) * std::max(1, cand[j].np_req);
         if (groups >= device_cus()) {
             for (size_t j = 0; j < cand.size(); ++j) split_wm[cand[j].wm].push_back(cand_idx[j]);
         } else {
@@ -2609,7 +2611,7 @@ int run_batch(std::vector<HostInst>& hs, const std::vector<drift_pt_result*>& rs
         int block = 0;
         part_of(g.second, part, prs, block);
         bool given = block > 0;
-        for (const HostInst& H : part) given = given || H.rg_req > 0;
+        for (const HostInst& H : part) given = given || H.rg_req > 0 || H.np_req > 0;
         if (tune && !given) {
             BatchShape sh;
             const int rc = tune_batch(DRIFT_PT_ENGINE_SPLIT, part, sh);

@@ -13,8 +13,8 @@ traded, and one case at 128 threads trades the three largest classes of seven.  
 that do and do not divide a class's words; groups of one slot (the packed spin-word path) and of
 several (the strided path); R above 64 (the workgroup round end); padding slots in the last
 group; one replica; the two-hop energy gather (groups x parts x R >= 1024); a stop at a target;
-the Erdos-Renyi workload of N = 10^5 and R = 32 on 256 workgroups; the refusals; and a wait
-forced to give up from the host.  The reference runs each replica in one warp with no exchange
+the Erdos-Renyi workload of N = 10^5 and R = 32 on 256 workgroups; a batch whose groups fill
+the chip only in parts; the refusals; and a wait forced to give up from the host.  The reference runs each replica in one warp with no exchange
 between blocks (cuda/ising_pt.cu:86-146); this engine has no counterpart there.
 """
 
@@ -133,6 +133,29 @@ def test_erdos_renyi_1e5_on_256_workgroups():
                         T_max=5.0, block=1024)
     _check(g, o, model)
     assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
+
+
+def test_batch_with_parts_equals_oracle():
+    """A batch of three instances too large for one workgroup (ER, N = 6000, R = 32) whose 32
+    one-slot groups would leave the chip idle; with 4 parts per group their 384 workgroups fill it,
+    so the batch takes one persistent split launch, and every instance equals the oracle."""
+    _announce("parts: batch of three ER N=6000")
+    R, n_rounds, S = 32, 6, 2
+    models = [er_sparse_weighted(6000, 6.0, 30 + i) for i in range(3)]
+    seeds = [9 + i for i in range(3)]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.1, T_max=4.0,
+                                           n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                           return_final=True, record_history=True,
+                                           slots_per_group=1, parts_per_group=4,
+                                           block_threads=256)
+    beta = ladder_betas(geometric_ladder(R, 0.1, 4.0))
+    for model, seed, r in zip(models, seeds, res):
+        assert r.engine == "split" and r.parts_per_group == 4 and r.slots_per_group == 1
+        enc = gpu.encode_problem(model)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=R, n_rounds=n_rounds,
+                                  sweeps_per_round=S, beta=beta, n_threads=THREADS)
+        _check(r, o, model)
 
 
 def test_parts_refusals():
