@@ -36,7 +36,8 @@ engines (pslab, coop, split) with an ordinary dispatch, for profiler commands; t
 cooperative launch.  --workload ladder-c5 measures time to target
 with the learned ladder instead, in three stages (--stage train, bestknown, eval).
 --workload single-heldout times one held-out instance per family (N ~ 65536) on each engine that
-can run it alone, the split engine with its groups in parts among them.
+can run it alone, the split engine with its groups in parts among them; --workload parts-sweep
+does the same across sizes and colour-class counts (the evidence for the default engine's rule).
 """
 
 from __future__ import annotations
@@ -1011,6 +1012,58 @@ def bench_single_heldout(args) -> dict:
             "families": out}
 
 
+def bench_parts_sweep(args) -> dict:
+    """The single-instance engine choice across shapes: 3-regular N(0,1) graphs (N 8192 ..
+    131072, four colour classes), 3-D +/-J glasses (L 20, 24, 40, 48; two classes) and sparse
+    Erdos-Renyi graphs (five to seven classes), at R = 32 and 16, geometric ladder 0.2 .. 5.0,
+    one sweep per round, --steps rounds: the default engine, the per-launch engine and one-slot
+    split groups in 8 and 16 parts (1024 threads), each the median of --repeats timed runs after an
+    untimed one, in attempts/ns.  Every configuration of a shape must end at the same best
+    energy."""
+    from drift_amd import gpu
+    from drift_amd.builders.lattice import cubic_pmj_glass, er_sparse_weighted, random_regular_gaussian
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    shapes = [("3-regular", n, lambda n=n: random_regular_gaussian(n, seed=3))
+              for n in (8192, 16384, 32768, 65536, 131072)]
+    shapes += [("3-D glass", L ** 3, lambda L=L: cubic_pmj_glass(L, 1)) for L in (20, 24, 40, 48)]
+    shapes += [("ER degree 3", 65536, lambda: er_sparse_weighted(65536, 3.0, 2)),
+               ("ER degree 6", 65536, lambda: er_sparse_weighted(65536, 6.0, 2)),
+               ("ER degree 3", 16384, lambda: er_sparse_weighted(16384, 3.0, 2))]
+    configs = [("default", dict(engine="auto")), ("global", dict(engine="global")),
+               ("split 1x8", dict(engine="split", slots_per_group=1, parts_per_group=8,
+                                  block_threads=1024)),
+               ("split 1x16", dict(engine="split", slots_per_group=1, parts_per_group=16,
+                                   block_threads=1024))]
+    rows = []
+    for R in (32, 16):
+        beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+        for fam, n, make in shapes:
+            model = make()
+            row = {"family": fam, "n": n, "R": R}
+            best = set()
+            for name, kw in configs:
+                if "parts_per_group" in kw and R * kw["parts_per_group"] > 256:
+                    continue   # more workgroups than compute units
+                run = dict(n_replicas=R, n_rounds=args.steps, sweeps_per_round=1, seed=1,
+                           betas=beta, record_history=False, **kw)
+                gpu.parallel_tempering_gpu(model, **dict(run, n_rounds=2))
+                res = [gpu.parallel_tempering_gpu(model, **run) for _ in range(max(args.repeats, 1))]
+                secs = float(np.median([r.seconds for r in res]))
+                best.add(res[0].best_E_units)
+                row[name] = round(R * args.steps * n / secs / 1e9, 2)
+                if name == "default":
+                    row["default_engine"] = res[0].engine + (
+                        f" 1x{res[0].parts_per_group}" if res[0].engine == "split" else "")
+            if len(best) != 1:
+                raise RuntimeError(f"parts sweep: engines disagree on {fam} n={n} R={R}")
+            log(f"parts sweep: {row}")
+            rows.append(row)
+    return {"metric": "single-instance throughput by engine across shapes", "unit": "attempts/ns",
+            "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "repeats": args.repeats,
+            "data": "synthetic", "rows": rows}
+
+
 def bench_ladder_c5(args) -> int:
     stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval,
              "eval-merge": c5_eval_merge}[args.stage]
@@ -1027,7 +1080,7 @@ def main() -> int:
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
                     choices=("all", "c4", "c2", "c3", "c1", "published", "c5-batch", "c3-ttt",
-                             "ladder-c5", "ttt-sample", "single-heldout"))
+                             "ladder-c5", "ttt-sample", "single-heldout", "parts-sweep"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
@@ -1099,6 +1152,9 @@ def main() -> int:
         return 0
     if args.workload == "single-heldout":
         print(json.dumps(bench_single_heldout(args)), flush=True)
+        return 0
+    if args.workload == "parts-sweep":
+        print(json.dumps(bench_parts_sweep(args)), flush=True)
         return 0
     return bench_throughput(args)
 

@@ -1042,7 +1042,8 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
         if (H.flags & DRIFT_PT_PLAIN_LAUNCH) plain = true;
         if (hs[i].wm != hs[0].wm) return fail(DRIFT_PT_EINVAL, "split engine: one weight type per launch");
     }
-    const int block = split_block(lds, block_req);
+    // groups in parts: 1024 threads unless asked (fewer ran slower at every parts count measured)
+    const int block = parts && block_req <= 0 ? 1024 : split_block(lds, block_req);
     const int wm = hs[0].wm;
     int per_cu = 0;
     PT_HIP(pt_split_occupancy(wm, block, lds, &per_cu, cl, parts));
@@ -2080,18 +2081,56 @@ int run_engine(const HostInst& H, int engine, int block, drift_pt_result* res) {
 // engine; other instances to the coop engine (one launch, a grid barrier per colour) up to 2^17
 // attempts per sweep and to the per-launch global engine above (ER N=10^4, R=32: 6.7 s vs 8.1 s
 // for 2e5 rounds; C3: 56 vs 49 attempts/ns).
+// Parts per one-slot group for one instance on its own, by default: the largest power of two p
+// (at most 64 and the largest colour class's word count) whose R p workgroups of 1024 threads are
+// all resident, when the instance has n >= 8000 positions, at most four colour classes and fewer
+// replicas than compute units; else 0.  Each traded class costs a hop per sweep: on 3-regular
+// graphs (four classes) and 3-D glasses (two) of 8000 to 131 072 spins at R = 16 and 32 the
+// groups in parts ran 6-57% faster than the engine otherwise chosen, on Erdos-Renyi graphs (five
+// to seven classes) 9-22% slower.
+int parts_default(const HostInst& H) {
+    if (H.n < 8000 || H.k > 4 || H.R >= device_cus() || split_rg_max(H) < 1) return 0;
+    const size_t lds = split_lds(H, 1, false, true);
+    int per_cu = 0;
+    if (lds > (size_t)device_lds_limit() ||
+        pt_split_occupancy(H.wm, 1024, lds, &per_cu, false, true) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const int64_t cap = std::min<int64_t>((int64_t)per_cu * device_cus(), 1024);
+    int np = 1;
+    while (np < 64 && (int64_t)H.R * np * 2 <= cap && np * 2 <= class_words_max(H)) np *= 2;
+    return np >= 2 ? np : 0;
+}
+
 int default_engine(const HostInst& H) {
     const int64_t per_sweep = (int64_t)H.R * H.n;
     if (fits_lds(H) && per_sweep <= 8192) return DRIFT_PT_ENGINE_LDS;
-    if (H.has_target) return H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+    // (not when the caller fixed the split engine's launch shape: then AUTO keeps its other rules)
+    const bool parts = H.rg_req == 0 && H.np_req == 0 && parts_default(H) >= 2;
+    if (H.has_target)
+        return parts ? DRIFT_PT_ENGINE_SPLIT
+                     : (H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL);
     if (H.ldim && pslab_fits(H)) {
         size_t lds = 0;
         if (pslab_block(H, 0, lds)) return DRIFT_PT_ENGINE_PSLAB;
     }
     if (H.ldim && slab_fits(H)) return DRIFT_PT_ENGINE_SLAB;
+    if (parts) return DRIFT_PT_ENGINE_SPLIT;
     CoopPlan P;
     if (per_sweep <= (1 << 17) && coop_plan(H, 0, P)) return DRIFT_PT_ENGINE_COOP;
     return H.ldim ? DRIFT_PT_ENGINE_LATTICE : DRIFT_PT_ENGINE_GLOBAL;
+}
+
+// The launch shape default_engine means when it picks the split engine for one instance: one
+// slot per group in parts_default parts (unless the caller fixed either).
+void default_split_shape(HostInst& H) {
+    if (H.rg_req > 0 || H.np_req > 0) return;
+    const int np = parts_default(H);
+    if (np >= 2) {
+        H.rg_req = 1;
+        H.np_req = np;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2138,7 +2177,14 @@ int tune(const HostInst& H, int trial_rounds, drift_pt_tuning* table, TuneEntry&
         double secs;
     };
     std::vector<Cand> cands;
-    cands.push_back({default_engine(H), 0, 0, 0, 0, -1.0});
+    {
+        // the default, with the launch shape it means (a split default: one-slot groups in parts)
+        HostInst D = H;
+        const int e = default_engine(D);
+        if (e == DRIFT_PT_ENGINE_SPLIT) default_split_shape(D);
+        cands.push_back({e, 0, e == DRIFT_PT_ENGINE_SPLIT ? D.rg_req : 0, 0,
+                         e == DRIFT_PT_ENGINE_SPLIT ? D.np_req : 0, -1.0});
+    }
     const int blocks[3] = {256, 512, 1024};
     for (int b : blocks) {
         if (fits_lds(H)) cands.push_back({DRIFT_PT_ENGINE_LDS, b, 0, 0, 0, -1.0});
@@ -2312,6 +2358,8 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
     int engine = params->engine, block = params->block_threads;
     if (engine == DRIFT_PT_ENGINE_AUTO) {
         engine = default_engine(H);
+        if (!(params->flags & DRIFT_PT_AUTOTUNE) && engine == DRIFT_PT_ENGINE_SPLIT)
+            default_split_shape(H);
         if (params->flags & DRIFT_PT_AUTOTUNE) {
             TuneEntry choice;
             bool hit = false;

@@ -14,7 +14,8 @@ that do and do not divide a class's words; groups of one slot (the packed spin-w
 several (the strided path); R above 64 (the workgroup round end); padding slots in the last
 group; one replica; the two-hop energy gather (groups x parts x R >= 1024); a stop at a target;
 the Erdos-Renyi workload of N = 10^5 and R = 32 on 256 workgroups; a batch whose groups fill
-the chip only in parts; the refusals; and a wait forced to give up from the host.  The reference runs each replica in one warp with no exchange
+the chip only in parts; the default engine's choice of parts; the refusals; and a wait forced to
+give up from the host.  The reference runs each replica in one warp with no exchange
 between blocks (cuda/ising_pt.cu:86-146); this engine has no counterpart there.
 """
 
@@ -156,6 +157,29 @@ def test_batch_with_parts_equals_oracle():
         o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=R, n_rounds=n_rounds,
                                   sweeps_per_round=S, beta=beta, n_threads=THREADS)
         _check(r, o, model)
+
+
+def test_default_engine_takes_parts_for_few_colour_classes():
+    """Without tuning, one instance of n >= 8000 with at most four colour classes and fewer
+    replicas than compute units runs in one-slot groups in parts (a 3-regular graph of N = 8192 at
+    R = 16: 16 parts, 1024 threads), and equals the oracle; an Erdos-Renyi graph (five or more
+    classes) keeps the per-launch engine."""
+    _announce("default engine: parts for few colour classes")
+    model = random_regular_gaussian(8192, seed=8)
+    enc = gpu.encode_problem(model)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    assert k <= 4
+    beta = ladder_betas(geometric_ladder(16, 0.2, 5.0))
+    g = gpu.parallel_tempering_gpu(model, n_replicas=16, n_rounds=6, sweeps_per_round=1, seed=21,
+                                   betas=beta, coloring=(cptr, cspins), record_history=True,
+                                   return_final=True)
+    assert g.engine == "split" and g.slots_per_group == 1 and g.parts_per_group == 16
+    assert g.block_threads == 1024
+    o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=21, R=16, n_rounds=6, sweeps_per_round=1,
+                              beta=beta, n_threads=THREADS)
+    _check(g, o, model)
+    er = er_sparse_weighted(16384, 3.0, 2)
+    assert gpu.parallel_tempering_gpu(er, n_replicas=16, n_rounds=2).engine == "global"
 
 
 def test_parts_refusals():
