@@ -562,8 +562,9 @@ std::vector<T> pad4(const std::vector<T>& v) {
 int split_thr_words(const HostInst& H, int rg);
 
 // cl (split engine): also each instance's CSR as 16-bit records (position << 4 | weight & 15).
-// nps (split engine): parts per group of each instance; those above 1 get their spin words and
-// flags (zero) and part mailboxes (all bits set).
+// nps (split engine, a launch of the kernel with parts): parts per group of each instance; every
+// instance gets its spin words and flags (zero) and part mailboxes (all bits set), those of one
+// part too, whose energies go through the part mailboxes in such a launch.
 int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInstance>& insts,
                 bool force_final, bool with_ell, const std::vector<int>* rgs = nullptr,
                 bool cl = false, const std::vector<int>* nps = nullptr) {
@@ -614,16 +615,15 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
     const size_t out_end = cursor;   // statistics blocks are the region that must start at zero
     for (size_t i = 0; i < m; ++i) {
         oo[i].box = rgs ? take(sizeof(uint64_t) * (4 * (size_t)hs[i].Rp + 2)) : SIZE_MAX;
-        oo[i].pbox = np_of(i) > 1 ? take(sizeof(uint64_t) * 8 * (size_t)np_of(i) * hs[i].Rp)
-                                  : SIZE_MAX;
+        oo[i].pbox = nps ? take(sizeof(uint64_t) * 8 * (size_t)np_of(i) * hs[i].Rp) : SIZE_MAX;
     }
     const size_t box_end = cursor;   // mailboxes start with every bit set
     for (size_t i = 0; i < m; ++i) {
-        oo[i].pbits = np_of(i) > 1 ? take(sizeof(uint64_t) * 2 * (size_t)hs[i].Rp * nw_of(i))
-                                   : SIZE_MAX;
-        oo[i].pflag = np_of(i) > 1 ? take(sizeof(uint64_t) * PT_PFLAG_STRIDE * (size_t)ng_of(i) *
-                                          np_of(i))
-                                   : SIZE_MAX;
+        oo[i].pbits = nps && np_of(i) > 1
+                          ? take(sizeof(uint64_t) * 2 * (size_t)hs[i].Rp * nw_of(i)) : SIZE_MAX;
+        oo[i].pflag = nps && np_of(i) > 1
+                          ? take(sizeof(uint64_t) * PT_PFLAG_STRIDE * (size_t)ng_of(i) * np_of(i))
+                          : SIZE_MAX;
     }
     const size_t bits_end = cursor;  // part spin words start at step 0
     for (size_t i = 0; i < m; ++i) {
@@ -1054,7 +1054,7 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
                                          std::to_string(capacity) + " workgroups of " +
                                          std::to_string(block) + " threads)");
     std::vector<PtInstance> insts((size_t)n_inst);
-    int rc = upload_many(A, hs, insts, false, true, &rgs, cl, &nps);
+    int rc = upload_many(A, hs, insts, false, true, &rgs, cl, parts ? &nps : nullptr);
     if (rc) return rc;
     // Largest work first, the groups of one instance consecutive.  With eight queues (one per
     // XCD: a workgroup takes from queue blockIdx.x mod 8 first) each instance goes, largest

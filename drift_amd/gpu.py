@@ -319,7 +319,7 @@ def parallel_tempering_gpu_batch(
     autotune: bool = False,
     plain_launch: bool = False,
     couplings_in_lds: bool = False,
-    parts_per_group: int = 0,
+    parts_per_group=0,
 ) -> list[PtResult]:
     """Many independent instances in one persistent launch.  ``seeds`` (default 0, 1, 2, ...) and
     ``betas`` (one ladder per instance, or None for the geometric ladder) are per instance; each
@@ -337,13 +337,18 @@ def parallel_tempering_gpu_batch(
     ``plain_launch`` starts the split engine's persistent launch with an ordinary dispatch (for
     profilers that cannot follow cooperative launches; needs the device to itself);
     ``couplings_in_lds`` has the split launch stage every instance's couplings in LDS;
-    ``parts_per_group`` splits each of its replica groups' positions over that many workgroups."""
+    ``parts_per_group`` (one count, or one per instance) splits replica groups' positions over that
+    many workgroups."""
     lib = _engine_lib()
     models = list(models)
     m = len(models)
     if m == 0:
         return []
     seeds = list(range(m)) if seeds is None else [int(s) for s in seeds]
+    parts = ([int(parts_per_group)] * m if np.isscalar(parts_per_group)
+             else [int(p) for p in parts_per_group])
+    if len(parts) != m:
+        raise ValueError(f"parts_per_group must be one count or {m} counts")
     probs, prms, bufs, ladders = [], [], [], []
     for i, model in enumerate(models):
         temps, b = _ladder(n_replicas, T_min, T_max, None if betas is None else betas[i])
@@ -354,7 +359,7 @@ def parallel_tempering_gpu_batch(
                             target=None if targets is None else targets[i],
                             slots_per_group=slots_per_group, autotune=autotune,
                             plain_launch=plain_launch, couplings_in_lds=couplings_in_lds,
-                            parts_per_group=parts_per_group))
+                            parts_per_group=parts[i]))
         bufs.append(_Buffers(probs[-1].n, n_replicas, n_rounds, return_final, record_history,
                              record_trace))
     P = (_lib.Problem * m)(*[p.struct for p in probs])

@@ -159,6 +159,30 @@ def test_batch_with_parts_equals_oracle():
         _check(r, o, model)
 
 
+def test_batch_mixing_one_part_and_four_equals_oracle():
+    """A batch whose instances ask for different parts counts (1, 4 and 4: 288 workgroups, so the
+    batch takes the split launch) runs in one launch of the kernel with parts; the instance in one
+    part gathers its energies through the part mailboxes too.  Every instance equals the
+    oracle."""
+    _announce("parts: batch mixing 1, 4 and 4 parts")
+    R, n_rounds, S = 32, 5, 2
+    models = [er_sparse_weighted(6000, 6.0, 40 + i) for i in range(3)]
+    seeds = [19 + i for i in range(3)]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.1, T_max=4.0,
+                                           n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                           return_final=True, record_history=True,
+                                           slots_per_group=1, parts_per_group=[1, 4, 4],
+                                           block_threads=256)
+    beta = ladder_betas(geometric_ladder(R, 0.1, 4.0))
+    for model, seed, r, parts in zip(models, seeds, res, (1, 4, 4)):
+        assert r.engine == "split" and r.parts_per_group == parts
+        enc = gpu.encode_problem(model)
+        k, cptr, cs = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+        o = oracle.pt_run_encoded(enc, (cptr, cs), seed=seed, R=R, n_rounds=n_rounds,
+                                  sweeps_per_round=S, beta=beta, n_threads=THREADS)
+        _check(r, o, model)
+
+
 def test_default_engine_takes_parts_for_few_colour_classes():
     """Without tuning, one instance of n >= 8000 with at most four colour classes and fewer
     replicas than compute units runs in one-slot groups in parts (a 3-regular graph of N = 8192 at
