@@ -1033,10 +1033,10 @@ int run_split(const std::vector<HostInst>& hs, const std::vector<drift_pt_result
                                              "parts) for one instance");
             if ((int64_t)H.n_rounds * std::max(H.S, 1) * std::max(H.k, 1) >= ((int64_t)1 << 31))
                 return fail(DRIFT_PT_ERANGE, "split engine with parts: more than 2^31 colour steps");
-            if (split_lds(H, rg, cl, true) > (size_t)device_lds_limit())
-                return fail(DRIFT_PT_EINVAL, "split engine: a part's state does not fit one "
-                                             "workgroup's LDS");
         }
+        if (parts && split_lds(H, rg, cl, true) > (size_t)device_lds_limit())
+            return fail(DRIFT_PT_EINVAL, "split engine: a group's state with the arrays of parts "
+                                         "does not fit one workgroup's LDS");
         lds = std::max(lds, split_lds(H, rg, cl, parts));
         max_ng = std::max(max_ng, ng * np);
         if (H.flags & DRIFT_PT_PLAIN_LAUNCH) plain = true;
