@@ -1109,7 +1109,9 @@ def main() -> int:
                          "best-known runs take ten times as many)")
     ap.add_argument("--bestknown", default=os.path.join(ROOT, "tests", "golden",
                                                         "c5_heldout_bestknown.json"))
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "c5"))
+    ap.add_argument("--out", default=os.path.join(ROOT, "c5_out"),
+                    help="ladder-c5: directory of the stage files (labels, best-known rows, "
+                         "evaluation parts)")
     ap.add_argument("--test-n", type=int, default=65536, help="ladder-c5 held-out size")
     ap.add_argument("--test-per-family", type=int, default=67)
     ap.add_argument("--families", default="",

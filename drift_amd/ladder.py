@@ -172,7 +172,7 @@ def round_trip_rate(res, sweeps: int) -> float:
 
 
 # ------------------------------------------------------------------------------------------------
-# the instance mix the network is trained and judged on
+# the instance mix the network is trained and evaluated on
 # ------------------------------------------------------------------------------------------------
 # family: (T_min, T_max) of its ladder
 MIX_FAMILIES = {"ea3d": (0.3, 3.0), "rrg": (0.1, 3.0), "er": (0.2, 10.0)}
