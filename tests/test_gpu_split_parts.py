@@ -9,7 +9,8 @@ history and trace).
 
 Colour classes of at most two positions per thread are swept whole by every part instead of
 traded; the small instances here run 64-thread workgroups so that most of their classes are
-traded, and one case at 128 threads trades the three largest classes of seven.  Cases: float and integer couplings; parts counts
+traded, and one case at 128 threads trades the three largest classes of seven.  Cases: float,
+int8 and int32 couplings; parts counts
 that do and do not divide a class's words; groups of one slot (the packed spin-word path) and of
 several (the strided path); R above 64 (the workgroup round end); padding slots in the last
 group; one replica; the two-hop energy gather (groups x parts x R >= 1024); a stop at a target;
@@ -106,6 +107,23 @@ def test_int_parts_equal_oracle(R, rg, parts, block):
     g, o, enc = _run_pair(model, R=R, n_rounds=10, S=2, seed=3, rg=rg, parts=parts, T_min=0.1,
                           T_max=4.0, block=block)
     assert enc.weight_type == 1
+    _check(g, o, model)
+    assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
+
+
+def test_int32_couplings_in_parts_equal_oracle():
+    """Integer couplings too wide for int8 (ER weights x 97 with a field, so int32 records and
+    thresholds), one-slot groups in 4 parts at 128 threads: the int32 kernel with parts."""
+    from drift_amd.ising import SparseIsingModel
+
+    _announce("parts: int32 couplings")
+    base = er_sparse_weighted(3000, 5.0, 7)
+    rng = np.random.default_rng(5)
+    model = SparseIsingModel(base.row_ptr, base.col_idx, base.weight * 97.0,
+                             rng.integers(-60, 61, size=base.n).astype(np.float64))
+    g, o, enc = _run_pair(model, R=16, n_rounds=8, S=2, seed=13, rg=1, parts=4, T_min=20.0,
+                          T_max=600.0, block=128)
+    assert enc.weight_type == 2
     _check(g, o, model)
     assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
 
