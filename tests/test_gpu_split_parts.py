@@ -15,8 +15,8 @@ that do and do not divide a class's words; groups of one slot (the packed spin-w
 several (the strided path); R above 64 (the workgroup round end); padding slots in the last
 group; one replica; the two-hop energy gather (groups x parts x R >= 1024); a stop at a target;
 the Erdos-Renyi workload of N = 10^5 and R = 32 on 256 workgroups; a batch whose groups fill
-the chip only in parts; the default engine's choice of parts; the refusals; and a wait forced to
-give up from the host.  The reference runs each replica in one warp with no exchange
+the chip only in parts; the default engine's choice of parts; 70 000 rounds past the round tags'
+wrap, against the per-launch engine; the refusals; and a wait forced to give up from the host.  The reference runs each replica in one warp with no exchange
 between blocks (cuda/ising_pt.cu:86-146); this engine has no counterpart there.
 """
 
@@ -222,6 +222,32 @@ def test_default_engine_takes_parts_for_few_colour_classes():
     _check(g, o, model)
     er = er_sparse_weighted(16384, 3.0, 2)
     assert gpu.parallel_tempering_gpu(er, n_replicas=16, n_rounds=2).engine == "global"
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 1024), (4, 1, 256)])
+def test_split_runs_past_the_round_tag_wrap(shape):
+    """70 000 rounds, past the 2^16 wrap of the mailboxes' round tags, of a 3-regular N = 8192,
+    R = 16 instance: one-slot groups in 16 parts (the default's shape) and groups of 4 slots
+    without parts, each against the per-launch engine on the same chain -- best record, final
+    spins and energies, rung permutation, swap and round-trip counters equal."""
+    rg, parts, block = shape
+    _announce(f"split {rg} slots x {parts} parts: 70000 rounds")
+    model = random_regular_gaussian(8192, seed=8)
+    beta = ladder_betas(geometric_ladder(16, 0.2, 5.0))
+    kw = dict(n_replicas=16, n_rounds=70_000, sweeps_per_round=1, seed=3, betas=beta,
+              record_history=False, return_final=True)
+    g = gpu.parallel_tempering_gpu(model, engine="split", slots_per_group=rg,
+                                   parts_per_group=parts, block_threads=block, **kw)
+    ref = gpu.parallel_tempering_gpu(model, engine="global", **kw)
+    assert g.engine == "split" and ref.engine == "global"
+    assert (g.best_E_units, g.best_round, g.best_rung) == (ref.best_E_units, ref.best_round,
+                                                           ref.best_rung)
+    assert np.array_equal(g.best_s, ref.best_s)
+    assert np.array_equal(g.final_spins, ref.final_spins)
+    assert np.array_equal(g.final_E, ref.final_E)
+    assert np.array_equal(g.rung_of, ref.rung_of)
+    assert np.array_equal(g.swap_accepted, ref.swap_accepted)
+    assert np.array_equal(g.round_trips, ref.round_trips)
 
 
 def test_parts_refusals():
