@@ -43,7 +43,12 @@ enum drift_pt_weight_type {
 };
 
 enum drift_pt_engine {
-    DRIFT_PT_ENGINE_AUTO = 0,
+    DRIFT_PT_ENGINE_AUTO = 0,   /* the library's choice: LDS for small instances; lattices on the
+                                   slab engines where they fit; one instance of n >= 8000 with at
+                                   most four colour classes and fewer replicas than compute units
+                                   on SPLIT with one-slot groups in parts; otherwise COOP for
+                                   R n <= 2^17, else LATTICE or GLOBAL (DRIFT_PT_AUTOTUNE times
+                                   the candidates instead) */
     DRIFT_PT_ENGINE_LDS = 1,    /* whole instance (all replicas) in one workgroup's LDS, persistent */
     DRIFT_PT_ENGINE_GLOBAL = 2, /* replica-interleaved spins in HBM, one launch per colour step */
     DRIFT_PT_ENGINE_LATTICE = 3,/* the global engine's periodic-lattice kernel: implicit neighbours,
@@ -219,12 +224,13 @@ int drift_pt_run(const drift_pt_problem* problem, const drift_pt_params* params,
 /* Many independent instances; each result equals what drift_pt_run gives for that instance and
  * its params alone.  Instances whose replicas fit one workgroup's LDS run in one persistent launch
  * (a work queue, largest first); larger ones whose single replica fits run in a second persistent
- * launch on the SPLIT engine when their replica groups number at least one per compute unit; the
- * rest run on the per-launch engine, several at a time on streams of their own so their launches
- * overlap.  With DRIFT_PT_AUTOTUNE on any instance, each persistent launch's shape (block size;
- * for the split launch also slots per group and couplings in LDS) is timed on the batch first and
- * cached per batch shape class; params.block_threads / slots_per_group fix it instead.  No
- * reference counterpart: the reference solves one instance per process (drift/gpu.py:97). */
+ * launch on the SPLIT engine when their workgroups (replica groups, times params.parts_per_group
+ * where given) number at least one per compute unit; the rest run on the per-launch engine,
+ * several at a time on streams of their own so their launches overlap.  With DRIFT_PT_AUTOTUNE on
+ * any instance, each persistent launch's shape (block size; for the split launch also slots per
+ * group and couplings in LDS) is timed on the batch first and cached per batch shape class;
+ * params.block_threads / slots_per_group / parts_per_group fix it instead.  No reference
+ * counterpart: the reference solves one instance per process (drift/gpu.py:97). */
 int drift_pt_run_batch(const drift_pt_problem* problems, const drift_pt_params* params,
                        int32_t n_instances, drift_pt_result* results);
 
