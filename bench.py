@@ -17,8 +17,9 @@ Sub-results, each with its own roofline and CPU baseline, ride on the same line:
   c5_batch   64 random 3-regular float instances of N = 65536, R = 32: the split engine's batch
   time_to_target  the metric's second part on a sample of the learned-ladder evaluation's held-out
              set (8 instances per family, N ~ 65536, cap 10^5 sweeps): wall time of one call per
-             instance on the geometric ladder over one batched call on the learned ladders, and
-             how many reach their best-known energies (--no-time-to-target leaves it out)
+             instance on the geometric ladder (each on its autotuned engine, tuned untimed first)
+             over one batched call on the learned ladders, and how many reach their best-known
+             energies (--no-time-to-target leaves it out)
 
 Rooflines count the bytes one attempt touches at the storage format the engine declares (no
 credit for on-chip reuse); the fixed per-attempt figure of the measurement plan (8.75 B for C2,
