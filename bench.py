@@ -15,6 +15,8 @@ Sub-results, each with its own roofline and CPU baseline, ride on the same line:
              sweeps; its only published rate is 1.29 attempts/ns on an RTX 5060 Ti
              (docs/results/PHASE14-results.md:140-142), a different card: context, not a target
   c5_batch   64 random 3-regular float instances of N = 65536, R = 32: the split engine's batch
+  single3    one random 3-regular float instance of N = 65536, R = 32, T 0.1 .. 3.0: the default
+             engine for it, one-slot split groups in 8 parts
   time_to_target  the metric's second part on a sample of the learned-ladder evaluation's held-out
              set (8 instances per family, N ~ 65536, cap 10^5 sweeps): wall time of one call per
              instance on the geometric ladder (each on its autotuned engine, tuned untimed first)
@@ -82,6 +84,15 @@ def workload(name: str):
         return (er_sparse_weighted(100000, 6.0, 0), 32, ladder_betas(geometric_ladder(32, 0.2, 20.0)),
                 1, "Erdos-Renyi N=100000 mean degree 6, weights in {+-1,+-2,+-3}, R=32, "
                    "beta 0.05..5.0, one sweep and one exchange per step")
+    if name == "single3":
+        from drift_amd import ladder
+
+        T_min, T_max = ladder.MIX_FAMILIES["rrg"]
+        return (ladder.mix_instance("rrg", 901_000, 65536), 32,
+                ladder_betas(geometric_ladder(32, T_min, T_max)), 1,
+                "one random 3-regular instance, N=65536, J ~ N(0,1) and h ~ N(0,0.1^2) in "
+                "float32 (the first held-out 3-regular instance of the learned-ladder mix), R=32, "
+                "T 0.1..3.0 geometric, one sweep and one exchange per step")
     if name == "published":
         n = 2048
         return (maxcut_ising(random_graph(n, p=min(0.1, 20.0 / n), seed=0)), 256,
@@ -114,7 +125,9 @@ def lattice_side(n: int) -> int:
     return 0
 
 
-def kernel_name(engine: str, model, R: int) -> str:
+def kernel_name(engine: str, model, R: int, parts: int = 0) -> str:
+    if engine == "split" and parts > 1:
+        return "pt_split_kernel_parts"
     return {"global": "g_color_bits" if R <= 32 else "g_color_step",
             "lattice": "g_lattice_step", "slab": "slab_sweep", "coop": "coop_run",
             # bit slabs (side a multiple of 32) run without a grid barrier when no per-round history
@@ -467,8 +480,11 @@ def bench_single(name, steps, warmup, repeats, cpu_budget, plain, engine="auto",
         log(f"{name}: timed run {r + 1}/{repeats}: {attempts / secs[-1] / 1e9:.2f} attempts/ns "
             f"({res.engine})")
     med = median(secs)
-    kern = kernel_name(res.engine, model, R)
+    kern = kernel_name(res.engine, model, R, res.parts_per_group or 0)
     B, fmt = declared_bytes(model, enc, res.engine, R)
+    if res.engine == "split" and (res.parts_per_group or 0) > 1:
+        fmt += (f"; each group's positions split over {res.parts_per_group} workgroups, which trade "
+                "every colour class's new spins as step-tagged words of 32 spin bits in HBM")
     if res.engine in ("global", "lattice", "slab"):
         # several launches per step: the dominant kernel's average launch in the timed region is
         # the region's time per step times that kernel's share of device time (from a run that
@@ -487,6 +503,7 @@ def bench_single(name, steps, warmup, repeats, cpu_budget, plain, engine="auto",
         "config": {"workload": desc, "n": model.n, "replicas": R, "sweeps_per_step": S,
                    "attempts_per_step": R * model.n * S, "format": fmt, "engine": res.engine,
                    "block_threads": res.block_threads, "autotune": bool(kw["autotune"]),
+                   "slots_per_group": res.slots_per_group, "parts_per_group": res.parts_per_group,
                    "wall_s_median": round(median(walls), 3)},
         "timing": {"repeats": repeats, "runs_ms": [round(x * 1e3, 4) for x in secs],
                    "median_ms": round(med * 1e3, 4)},
@@ -513,7 +530,8 @@ def headline(sub: dict) -> dict:
     }
 
 
-SUB_DEFAULT_STEPS = {"c2": 10000, "c3": 1000, "c1": 20000, "published": 400, "c5_batch": 200}
+SUB_DEFAULT_STEPS = {"c2": 10000, "c3": 1000, "c1": 20000, "published": 400, "c5_batch": 200,
+                     "single3": 500}
 
 
 def bench_throughput(args) -> int:
@@ -527,7 +545,7 @@ def bench_throughput(args) -> int:
                                                  autotune=args.autotune,
                                                  plain=args.plain_launch))), flush=True)
         return 0
-    if args.workload in ("c2", "c3", "c1", "published"):
+    if args.workload in ("c2", "c3", "c1", "published", "single3"):
         sub = bench_single(args.workload, args.steps, args.warmup, args.repeats, cpu,
                            args.plain_launch, args.engine, args.block, args.autotune)
         print(json.dumps(headline(sub)), flush=True)
@@ -536,7 +554,7 @@ def bench_throughput(args) -> int:
     line = headline(bench_c4(args.steps, args.warmup, args.repeats, cpu, args.instances,
                              args.autotune))
     subs = {}
-    for name in ("c2", "c3", "published"):
+    for name in ("c2", "c3", "published", "single3"):
         steps = args.sub_steps or SUB_DEFAULT_STEPS[name]
         sub = bench_single(name, steps, min(args.warmup, steps) or 1, args.repeats,
                            cpu, args.plain_launch, autotune=args.autotune)
@@ -1081,7 +1099,8 @@ def main() -> int:
     ap.add_argument("--repeats", type=int, default=5, help="timed runs; the value is their median")
     ap.add_argument("--workload", default="all",
                     choices=("all", "c4", "c2", "c3", "c1", "published", "c5-batch", "c3-ttt",
-                             "ladder-c5", "ttt-sample", "single-heldout", "parts-sweep"))
+                             "ladder-c5", "ttt-sample", "single-heldout", "parts-sweep",
+                             "single3"))
     ap.add_argument("--sub-steps", type=int, default=0,
                     help="steps of the sub-result workloads (0: their defaults)")
     ap.add_argument("--engine", default="auto")
