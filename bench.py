@@ -52,6 +52,7 @@ import os
 import statistics
 import sys
 import time
+import types
 
 import numpy as np
 
@@ -771,6 +772,96 @@ def c5_eval(args) -> dict:
     return c5_eval_summary(part["rows"], part["walls"], cap, bk, part["net"], part["total_s"])
 
 
+def c5_eval_batched(args) -> dict:
+    """The evaluation's batched legs over every family at once: (b) one batched call on the
+    learned ladders and (c) one on the geometric ladders for all held-out instances of the families
+    in --parts (part files of --stage eval, whose one-call-per-instance leg (a) and its wall time
+    are taken over as they are).  The library runs each call's instances in persistent launches
+    by weight type, so one family's slow instances overlap another family's work instead of
+    finishing alone.  Writes the complete rows to --out and returns the summary."""
+    from drift_amd import gpu, ladder
+    from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
+
+    t_all = time.perf_counter()
+    with open(args.bestknown) as f:
+        bk = json.load(f)
+    single, walls_a, caps, nets = {}, 0.0, set(), set()
+    for path in args.parts.split(","):
+        with open(path) as f:
+            part = json.load(f)
+        caps.add(part["cap"])
+        nets.add(part["net"])
+        walls_a += part["walls"]["geo_single"]
+        for r in part["rows"]:
+            single[r["index"]] = r
+    if len(caps) != 1 or len(nets) != 1:
+        raise SystemExit("eval-batched: the parts were run with different caps or networks")
+    cap = caps.pop()
+    net = ladder.LadderNet(seed=11)
+    net.load(args.net)
+    models, idx, targets, learned, geometric, seeds, xs_all = [], [], [], [], [], [], []
+    for fi, (fam, (T_min, T_max)) in enumerate(ladder.MIX_FAMILIES.items()):
+        ks = [k for k in range(bk["per_family"]) if fi * bk["per_family"] + k in single]
+        if not ks:
+            continue
+        fam_models = [ladder.mix_instance(fam, 900_000 + 1000 * fi + k, bk["test_n"]) for k in ks]
+        xs = net.predict_ladder(ladder.instance_features(fam_models, T_min=T_min, T_max=T_max, R=32))
+        geo_b = ladder_betas(geometric_ladder(32, T_min, T_max))
+        for k, m, x in zip(ks, fam_models, xs):
+            i = fi * bk["per_family"] + k
+            models.append(m)
+            idx.append(i)
+            targets.append(single[i]["target"])
+            learned.append(ladder.betas_from_positions(x, T_min, T_max))
+            geometric.append(geo_b)
+            seeds.append(4242 + i)
+            xs_all.append(x)
+    log(f"c5 eval-batched: {len(models)} instances in one batched call per ladder")
+    kw = dict(n_replicas=32, n_rounds=cap, sweeps_per_round=1, record_history=True,
+              slots_per_group=args.slots_per_group)
+    t0 = time.perf_counter()
+    lrn = gpu.parallel_tempering_gpu_batch(models, seeds=seeds, betas=learned, targets=targets, **kw)
+    wall_b = time.perf_counter() - t0
+    log(f"c5 eval-batched: learned done in {wall_b:.1f} s; geometric")
+    if args.learned_only:   # leg (c) as the part files have it
+        geo = [types.SimpleNamespace(rounds_run=single[i]["geo_batched"]["rounds"],
+                                     best_E=single[i]["geo_batched"]["best_E"]) for i in idx]
+        wall_c = sum(json.load(open(p))["walls"]["geo_batched"] for p in args.parts.split(","))
+    else:
+        t0 = time.perf_counter()
+        geo = gpu.parallel_tempering_gpu_batch(models, seeds=seeds, betas=geometric,
+                                               targets=targets, **kw)
+        wall_c = time.perf_counter() - t0
+        log(f"c5 eval-batched: geometric done in {wall_c:.1f} s")
+    rows = []
+    for i, t, b, c, x in zip(idx, targets, lrn, geo, xs_all):
+        a = single[i]
+        hb = b.best_history[:max(b.rounds_run, 1)]
+        rows.append({"index": i, "family": a["family"], "n": a["n"], "target": t,
+                     "geo_single": a["geo_single"],
+                     "learned_batched": {"rounds": b.rounds_run, "reached": b.best_E <= t + 1e-9 * abs(t),
+                                         "best_E": b.best_E},
+                     "geo_batched": {"rounds": c.rounds_run, "reached": c.best_E <= t + 1e-9 * abs(t),
+                                     "best_E": c.best_E},
+                     "within_1e-4": {"geo": a["within_1e-4"]["geo"], "learned": _reach(hb, t, 1e-4)},
+                     "within_1e-3": {"geo": a["within_1e-3"]["geo"], "learned": _reach(hb, t, 1e-3)},
+                     "to_geo_final": {"geo": a["to_geo_final"]["geo"],
+                                      "learned": _reach(hb, a["geo_single"]["best_E"], 0.0)},
+                     "learned_rung8_position": float(x[8]),
+                     "learned_power": float(np.log(max(x[8], 1e-12)) / np.log(8 / 31))})
+    walls = {"geo_single": walls_a, "learned_batched": wall_b, "geo_batched": wall_c}
+    part = {"cap": cap, "families": sorted({r["family"] for r in rows}), "walls": walls,
+            "rows": rows, "net": nets.pop(), "total_s": round(time.perf_counter() - t_all, 1),
+            "batched": "one call over every family"}
+    os.makedirs(args.out, exist_ok=True)
+    tag = "_".join(part["families"]) + (f"_rg{args.slots_per_group}" if args.slots_per_group else "")
+    part["slots_per_group"] = args.slots_per_group
+    path = os.path.join(args.out, f"c5_eval_part_batched_{tag}_cap{cap}.json")
+    _write_json(path, part)
+    log(f"c5 eval-batched: rows written to {path}")
+    return c5_eval_summary(rows, walls, cap, bk, part["net"], part["total_s"])
+
+
 def c5_eval_merge(args) -> dict:
     """The evaluation's summary over parts written by --stage eval --families (--parts)."""
     with open(args.bestknown) as f:
@@ -1085,7 +1176,7 @@ def bench_parts_sweep(args) -> dict:
 
 def bench_ladder_c5(args) -> int:
     stage = {"train": c5_train, "bestknown": c5_bestknown, "eval": c5_eval,
-             "eval-merge": c5_eval_merge}[args.stage]
+             "eval-merge": c5_eval_merge, "eval-batched": c5_eval_batched}[args.stage]
     line = stage(args)
     print(json.dumps(line), flush=True)
     return 0
@@ -1117,7 +1208,8 @@ def main() -> int:
                     help="leave the time-to-target sample out of the default line")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
     ap.add_argument("--train-per-family", type=int, default=667)
-    ap.add_argument("--stage", default="train", choices=("train", "bestknown", "eval", "eval-merge"),
+    ap.add_argument("--stage", default="train",
+                    choices=("train", "bestknown", "eval", "eval-merge", "eval-batched"),
                     help="ladder-c5 workload: train the network, record the held-out instances' "
                          "best-known energies, or measure time to them")
     ap.add_argument("--net", default=os.path.join(ROOT, "drift_amd", "ladder_net_c5.npz"),
@@ -1139,7 +1231,14 @@ def main() -> int:
                          "all); bestknown keeps the other families' rows already in --out, eval "
                          "writes the families' rows to --out for --stage eval-merge")
     ap.add_argument("--parts", default="",
-                    help="ladder-c5 eval-merge: comma-separated part files of --stage eval")
+                    help="ladder-c5 eval-merge / eval-batched: comma-separated part files of "
+                         "--stage eval")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="ladder-c5 eval-batched: rerun the learned batched leg only, taking the "
+                         "geometric batched leg from the part files")
+    ap.add_argument("--slots-per-group", type=int, default=0,
+                    help="ladder-c5 eval-batched: the split engine's slots per group in the "
+                         "batched calls (0: the library's choice)")
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (rocprof counter scaling)")

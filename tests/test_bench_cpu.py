@@ -122,3 +122,75 @@ def test_time_to_target_parts_merge(tmp_path):
     Args.parts = parts[0] + "," + str(bad)
     with pytest.raises(SystemExit):
         bench.c5_eval_merge(Args)
+
+
+def test_time_to_target_batched_stage(tmp_path, monkeypatch):
+    """--stage eval-batched reruns the batched legs over every family in one call each: leg (a)
+    and its wall time come over from the part files unchanged, the batched legs' rows come from
+    the calls, the learned run's sweeps to the geometric run's final energy use leg (a)'s energy,
+    and the complete rows go to --out (the calls are replaced by stand-ins here)."""
+    import json
+    import types
+
+    import numpy as np
+
+    from drift_amd import gpu, ladder
+
+    def row(i, fam, best):
+        res = {"rounds": 50, "reached": False, "best_E": best}
+        return {"index": i, "family": fam, "n": 0, "target": -10.0, "geo_single": res,
+                "learned_batched": res, "geo_batched": res,
+                "within_1e-4": {"geo": None, "learned": None}, "within_1e-3": {"geo": 7, "learned": 7},
+                "to_geo_final": {"geo": 50, "learned": 40},
+                "learned_rung8_position": 0.3, "learned_power": 0.5}
+    parts = []
+    for name, fam, rows, wall in (("a", "ea3d", [row(0, "ea3d", -9.5)], 8.0),
+                                  ("b", "er", [row(134, "er", -9.0), row(135, "er", -9.9)], 4.0)):
+        path = tmp_path / f"{name}.json"
+        path.write_text(json.dumps({"cap": 100, "families": [fam], "rows": rows, "net": "net.npz",
+                                    "walls": {"geo_single": wall, "learned_batched": 1.0,
+                                              "geo_batched": 1.0}, "total_s": 1.0}))
+        parts.append(str(path))
+    calls = []
+
+    def fake_batch(models, *, seeds, betas, targets, n_rounds, **kw):
+        calls.append((len(models), list(seeds)))
+        out = []
+        for t, s in zip(targets, seeds):
+            hist = np.linspace(-9.0, -10.0, 20) if s % 2 == 0 else np.full(20, -9.8)
+            out.append(types.SimpleNamespace(best_E=float(hist[-1]), rounds_run=20,
+                                             best_history=hist))
+        return out
+    monkeypatch.setattr(gpu, "parallel_tempering_gpu_batch", fake_batch)
+    monkeypatch.setattr(ladder, "mix_instance", lambda fam, seed, n: types.SimpleNamespace(n=64))
+    monkeypatch.setattr(ladder, "instance_features", lambda models, **kw: np.zeros((len(models), 16)))
+
+    class FakeNet:          # the network runs on the device; a fixed ladder stands in
+        def __init__(self, seed=0):
+            pass
+
+        def load(self, path):
+            pass
+
+        def predict_ladder(self, F):
+            return np.array([ladder.power_positions(0.7, 32)] * len(F))
+    monkeypatch.setattr(ladder, "LadderNet", FakeNet)
+
+    class Args:
+        bestknown = os.path.join(ROOT, "tests", "golden", "c5_heldout_bestknown.json")
+        net = "net.npz"
+        out = str(tmp_path / "out")
+        slots_per_group = 0
+        learned_only = False
+    Args.parts = ",".join(parts)
+    line = bench.c5_eval_batched(Args)
+    assert [c[0] for c in calls] == [3, 3]                 # one call per ladder, all families
+    assert calls[0][1] == [4242, 4242 + 134, 4242 + 135]
+    assert line["wall_s"]["geo_single"] == pytest.approx(12.0)
+    assert line["held_out"]["instances"] == 3
+    assert line["held_out"]["reached_learned_batched"] == 2   # seeds 4242 and 4376 end at -10
+    written = json.loads((tmp_path / "out" / "c5_eval_part_batched_ea3d_er_cap100.json").read_text())
+    r = {x["index"]: x for x in written["rows"]}
+    assert r[0]["geo_single"]["best_E"] == -9.5               # leg (a) as it was
+    assert r[0]["to_geo_final"]["learned"] is not None        # reaches -9.5 on the way to -10
+    assert r[135]["to_geo_final"]["learned"] is None          # -9.8 never reaches -9.9
