@@ -411,16 +411,26 @@ C5_BATCH_DESC = ("64 random 3-regular instances of N = 65536 (the held-out size 
                  "evaluation), J ~ N(0,1) and h ~ N(0,0.1^2) in float32, R=32 each, beta 0.2..5.0 "
                  "geometric; too large for one workgroup, so the batch runs on the split engine in "
                  "one persistent launch; one sweep and one exchange of every instance per step")
+C5_BATCH_ER_DESC = ("the learned-ladder evaluation's held-out sparse Erdos-Renyi instances of N=65536 "
+                    "(mean degree 3..12, weights in {+-1,+-2,+-3}), R=32, beta 0.2..5.0 geometric, "
+                    "all in one persistent split-engine launch; one sweep and one exchange of every "
+                    "instance per step")
 
 
-def bench_c5_batch(steps, warmup, repeats, cpu_budget, count=64, autotune=False, plain=False):
-    """Batched instances too large for one workgroup's LDS (the split engine)."""
-    from drift_amd import gpu
+def bench_c5_batch(steps, warmup, repeats, cpu_budget, count=64, autotune=False, plain=False,
+                   family="rrg"):
+    """Batched instances too large for one workgroup's LDS (the split engine): random 3-regular
+    float graphs, or (family "er") the learned-ladder evaluation's held-out sparse Erdos-Renyi
+    graphs (int8 weights, mean degree 3 to 12)."""
+    from drift_amd import gpu, ladder
     from drift_amd.builders.lattice import random_regular_gaussian
     from drift_amd.solvers.parallel_tempering import geometric_ladder, ladder_betas
 
-    log(f"c5-batch: building {count} instances of N = 65536")
-    models = [random_regular_gaussian(65536, seed=60_000 + i) for i in range(count)]
+    log(f"c5-batch: building {count} {family} instances of N = 65536")
+    if family == "er":
+        models = [ladder.mix_instance("er", 902_000 + k, 65536) for k in range(count)]
+    else:
+        models = [random_regular_gaussian(65536, seed=60_000 + i) for i in range(count)]
     R = 32
     kw = dict(n_replicas=R, T_min=0.2, T_max=5.0, sweeps_per_round=1, autotune=autotune,
               plain_launch=plain)
@@ -439,11 +449,13 @@ def bench_c5_batch(steps, warmup, repeats, cpu_budget, count=64, autotune=False,
     med = median(secs)
     enc0 = gpu.encode_problem(models[0])
     B, fmt = declared_bytes(models[0], enc0, res[0].engine, R)
-    kern = "pt_split_kernel" if res[0].engine == "split" else kernel_name(res[0].engine, models[0], R)
+    kern = (("pt_split_kernel_er_batch" if family == "er" else "pt_split_kernel")
+            if res[0].engine == "split" else kernel_name(res[0].engine, models[0], R))
     out = {
         "value": round(attempts / med / 1e9, 3), "steps": steps, "warmup": warmup,
         "ms_per_step": round(med * 1e3 / steps, 5), "dtype": "f32",
-        "config": {"workload": C5_BATCH_DESC, "instances": count, "total_spins": int(total_n),
+        "config": {"workload": C5_BATCH_DESC if family != "er" else C5_BATCH_ER_DESC,
+                   "instances": count, "total_spins": int(total_n),
                    "attempts_per_step": R * total_n, "format": fmt, "engine": res[0].engine,
                    "slots_per_group": res[0].slots_per_group, "block_threads": res[0].block_threads,
                    "autotune": bool(autotune), "wall_s_median": round(median(walls), 3)},
@@ -543,8 +555,10 @@ def bench_throughput(args) -> int:
         return 0
     if args.workload == "c5-batch":
         print(json.dumps(headline(bench_c5_batch(args.steps, args.warmup, args.repeats, cpu,
+                                                 count=args.batch_count,
                                                  autotune=args.autotune,
-                                                 plain=args.plain_launch))), flush=True)
+                                                 plain=args.plain_launch,
+                                                 family=args.batch_family))), flush=True)
         return 0
     if args.workload in ("c2", "c3", "c1", "published", "single3"):
         sub = bench_single(args.workload, args.steps, args.warmup, args.repeats, cpu,
@@ -1207,6 +1221,9 @@ def main() -> int:
     ap.add_argument("--no-time-to-target", action="store_true",
                     help="leave the time-to-target sample out of the default line")
     ap.add_argument("--instances", type=int, default=4096, help="batch size of the c4 workload")
+    ap.add_argument("--batch-family", default="rrg", choices=("rrg", "er"),
+                    help="c5-batch workload: 3-regular float graphs or the held-out sparse ER graphs")
+    ap.add_argument("--batch-count", type=int, default=64, help="c5-batch workload: instances")
     ap.add_argument("--train-per-family", type=int, default=667)
     ap.add_argument("--stage", default="train",
                     choices=("train", "bestknown", "eval", "eval-merge", "eval-batched"),
