@@ -525,6 +525,32 @@ struct Packer {
 // positions padded to a multiple of 4, the neighbour stored as position * mult: its LDS spin offset
 // on the LDS engine (mult = Rp; below 2^24 since the spins fit one workgroup's LDS), its spin word
 // on the word-per-position CSR engine (mult = 1; int8 records need n < 2^24, as the CSR does).
+// Each colour class's quads (a quad is 4 consecutive positions, Philox-keyed by its index; the
+// quads of class [cs, ce) are cs/4 .. (ce-1)/4, the boundary quads in both their classes) ordered
+// by the longest of their rows in the class, longest first (ties by index), the classes in order.
+// Any thread may take any quad of a class, so the order changes who sweeps a quad, not what it
+// draws or decides.
+std::vector<int32_t> build_qorder(const HostInst& H) {
+    std::vector<int32_t> out;
+    out.reserve((size_t)H.n / 4 + (size_t)H.k + 1);
+    for (int c = 0; c < H.k; ++c) {
+        const int cs = H.color_ptr[c], ce = H.color_ptr[c + 1];
+        if (ce <= cs) continue;
+        const int q0 = cs >> 2, q1 = (ce - 1) >> 2;
+        std::vector<std::pair<int, int32_t>> key;
+        key.reserve((size_t)(q1 - q0 + 1));
+        for (int q = q0; q <= q1; ++q) {
+            int m = 0;
+            for (int p = std::max(4 * q, cs); p < std::min(4 * q + 4, ce); ++p)
+                m = std::max(m, H.row_ptr[p + 1] - H.row_ptr[p]);
+            key.push_back({-m, q});
+        }
+        std::sort(key.begin(), key.end());
+        for (const auto& kv : key) out.push_back(kv.second);
+    }
+    return out;
+}
+
 std::vector<uint32_t> build_ell(const HostInst& H, int G, uint32_t mult) {
     const bool i8 = H.wm == PT_WM_I8;
     const int ew = i8 ? 1 : 2;
@@ -573,7 +599,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
     auto ng_of = [&](size_t i) { return rgs ? (hs[i].R + (*rgs)[i] - 1) / (*rgs)[i] : 1; };
     auto np_of = [&](size_t i) { return nps ? (*nps)[i] : 1; };
     auto nw_of = [&](size_t i) { return (hs[i].n + 31) / 32; };
-    struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb, crec; };
+    struct Off { size_t rp, edge, ell, cptr, orig, h, beta, thr, jb, crec, qord; };
     std::vector<Off> off(m);
     Packer in;
     for (size_t i = 0; i < m; ++i) {
@@ -582,6 +608,8 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         o.rp = in.add(H.row_ptr);
         o.edge = in.add(H.edge);
         o.ell = with_ell ? in.add(build_ell(H, H.gq, (uint32_t)rg_of(i))) : SIZE_MAX;
+        o.qord = with_ell && H.wm == PT_WM_I8 && H.maxdeg > H.gq ? in.add(build_qorder(H))
+                                                                   : SIZE_MAX;
         o.cptr = in.add(H.color_ptr);
         o.orig = in.add(H.orig);
         o.h = !H.has_h ? SIZE_MAX
@@ -652,6 +680,7 @@ int upload_many(DevArena& A, const std::vector<HostInst>& hs, std::vector<PtInst
         I.row_ptr = (const int32_t*)(base + o.rp);
         I.edge = base + o.edge;
         I.ell = o.ell == SIZE_MAX ? nullptr : base + o.ell;
+        I.qorder = o.qord == SIZE_MAX ? nullptr : (const int32_t*)(base + o.qord);
         I.color_ptr = (const int32_t*)(base + o.cptr);
         I.orig = (const int32_t*)(base + o.orig);
         I.h = o.h == SIZE_MAX ? nullptr : base + o.h;

@@ -74,6 +74,11 @@ struct PtInstance {
     int32_t nnz;
     int32_t thr_lds;          // split engine, integer couplings: each round's threshold rows of the
                               // group's slots are copied into LDS (rg * TQ words fit 16 KB)
+    // LDS and split engines, int8 couplings with rows longer than the first chunk (ell_rest): each
+    // colour class's quads (cs/4 .. (ce-1)/4) listed longest row first, the classes one after
+    // another, so the lanes of a wave take quads of like length and the remainder's passes, which
+    // run until the wave's longest row ends, waste few slots; null otherwise (natural order)
+    const int32_t* qorder;
     // Split engine with np > 1 parts per group: workgroup (group g, part q) holds the group's
     // whole state but sweeps only part q's share of each colour class -- the class's 32-position
     // words (word w = positions 32 w .. 32 w + 31) split into np runs, as equal as integer

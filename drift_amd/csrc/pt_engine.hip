@@ -483,8 +483,10 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
             }
             for (int j = 0; j < I.S; ++j) {
                 const uint32_t t = (uint32_t)(rnd * I.S + j);
+                int qbase = 0;   // the class's first entry in I.qorder
                 for (int c = 0; c < k; ++c) {
                     const int cs = cptr[c], ce = cptr[c + 1];
+                    const int nqc = ce > cs ? ((ce - 1) >> 2) - (cs >> 2) + 1 : 0;
                     if constexpr (CL) {
                         if (live && ce > cs) {
                             const int q1 = (ce - 1) >> 2;
@@ -508,8 +510,13 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
                                 q1 = q1 < pr.ow1 * 8 - 1 ? q1 : pr.ow1 * 8 - 1;
                             }
                         }
+                        // int8 couplings: the class's quads longest row first where the instance
+                        // has an order (not with parts, whose runs are ranges of words)
+                        const int32_t* qord = nullptr;
+                        if constexpr (WM == PT_WM_I8 && !PARTS)
+                            if (I.qorder) qord = I.qorder + qbase;
                         auto quads = [&](auto gw) {
-                            for (int qq = qb + qo; qq <= q1; qq += qs) {
+                            auto one = [&](int qq) {
                                 const pt_u4 rv = pt_philox((uint32_t)qq, (uint32_t)a, t,
                                                            PT_TAG_METRO, I.k0, I.k1);
                                 const int64_t d = quad_update<WM, decltype(gw)::value>(
@@ -522,7 +529,14 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
                                 } else {
                                     dsum += d;
                                 }
+                            };
+                            if constexpr (WM == PT_WM_I8 && !PARTS) {
+                                if (qord) {
+                                    for (int i = qo; i <= q1 - qb; i += qs) one(qord[i]);
+                                    return;
+                                }
                             }
+                            for (int qq = qb + qo; qq <= q1; qq += qs) one(qq);
                         };
                         switch (I.gq) {
                             case 2: quads(std::integral_constant<int, 2>{}); break;
@@ -533,6 +547,7 @@ __device__ void lds_instance(const PtInstance& I, int wi, char* smem, LShared* s
                             default: quads(std::integral_constant<int, 4>{}); break;
                         }
                     }
+                    qbase += nqc;
                     const bool ride = PARTS && c == last_tc && j == I.S - 1;
                     if (ride) {
                         // the traded classes' changes so far go out with this step's words
