@@ -249,7 +249,7 @@ def traffic_per_attempt(kernel: str):
 CUS, CLOCK_HZ = 256, 2.4e9   # MI355X compute units and peak engine clock
 
 
-def roofline(kernel, B, B_survey, attempts_per_launch, launch_s, model=None):
+def roofline(kernel, B, B_plan, attempts_per_launch, launch_s, model=None):
     """The contract's HBM roofline at the declared storage format, with the ceilings that can bind
     a kernel whose state is on chip beside it: HBM traffic as counted by rocprofv3, VALU issue
     (one wave instruction per CU per cycle) and LDS issue (the probe's rate of wave-level LDS
@@ -261,9 +261,9 @@ def roofline(kernel, B, B_survey, attempts_per_launch, launch_s, model=None):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": kernel, "bytes_per_attempt": round(B, 4),
             "attempts_per_launch": attempts_per_launch, "avg_launch_us": round(launch_s * 1e6, 3)}
-    if B_survey is not None:
-        line["bytes_per_attempt_plan"] = B_survey
-        line["frac_plan"] = round(B_survey * rate / 1e9 / HBM_PEAK_GBS, 4)
+    if B_plan is not None:
+        line["bytes_per_attempt_plan"] = B_plan
+        line["frac_plan"] = round(B_plan * rate / 1e9 / HBM_PEAK_GBS, 4)
     ceilings = {"hbm_algorithmic": line["frac"]}
     tpa, src = traffic_per_attempt(kernel)
     if tpa is not None:
