@@ -85,6 +85,35 @@ def test_uneven_batch_up_to_2048_equals_oracle(case):
         _oracle_equal(model, r, seed=seed, R=R, n_rounds=n_rounds, S=S, beta=beta)
 
 
+def test_degenerate_instances_in_a_batch_equal_oracle():
+    """One spin, spins with no couplings at all (nnz = 0; integer and float fields), an all-zero
+    model and one coupled pair beside isolated spins, batched with two ordinary instances (one
+    launch per weight type), each equal to the oracle and at its closed-form ground energy."""
+    from drift_amd.ising import IsingModel
+
+    _announce("degenerate instances in a batch")
+    rng = np.random.default_rng(8)
+    pair = np.zeros((5, 5))
+    pair[1, 3] = pair[3, 1] = -2.0
+    odd = [IsingModel(np.zeros((1, 1)), np.array([2.0])),
+           IsingModel(np.zeros((7, 7)), rng.normal(size=7)),
+           IsingModel(np.zeros((9, 9)), rng.integers(-3, 4, size=9).astype(float)),
+           IsingModel(np.zeros((5, 5)), np.zeros(5)),
+           IsingModel(pair, np.zeros(5))]
+    models = odd + [random_regular_gaussian(64, seed=3), random_regular_gaussian(130, seed=4)]
+    R, n_rounds, S = 8, 15, 2
+    seeds = [40 + i for i in range(len(models))]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.2, T_max=5.0,
+                                           n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                           return_final=True, record_history=True)
+    beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+    for model, seed, r in zip(models, seeds, res):
+        _oracle_equal(model, r, seed=seed, R=R, n_rounds=n_rounds, S=S, beta=beta)
+    for model, r in zip(odd, res):
+        assert r.best_E == pytest.approx(-np.abs(model.h).sum() - np.abs(model.J).sum() / 2,
+                                         rel=1e-6, abs=1e-9)
+
+
 def test_full_c4_batch_properties():
     """The C4 batch itself: 4096 random 3-regular instances, N uniform over even values in
     [64, 2048], J ~ N(0,1) and h ~ N(0, 0.1^2) in float32, R = 16, beta 0.2 .. 5.0, one

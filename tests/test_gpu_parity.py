@@ -4,10 +4,11 @@ Every case runs the same instance, colouring, ladder and seed on the GPU (drift_
 libdriftpt.so) and on the oracle (oracle/pt_oracle.c) and compares spins, energies, swap
 decisions, the rung permutation and the best state exactly.  Covered: integer couplings (int8,
 int32, half-integer QUBO scale, fields), float couplings, both engines, several tile sizes, replica
-counts that are not powers of two, R = 1 and 2, the 2-D ferromagnet of config C1 (L=4 and L=16),
-the first 10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), a batch of uneven float instances
-in one persistent launch, every first-chunk width of the padded edge records (with rows longer
-than the chunk), and both barrier energy paths of the bit-sliced persistent slab engine.
+counts that are not powers of two, R = 1 and 2, degenerate instances (one spin, no couplings at
+all, an all-zero model, one pair beside isolated spins), the 2-D ferromagnet of config C1 (L=4
+and L=16), the first 10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), a batch of uneven float
+instances in one persistent launch, every first-chunk width of the padded edge records (with rows
+longer than the chunk), and both barrier energy paths of the bit-sliced persistent slab engine.
 Reference GPU tests (tests/test_gpu.py:45-67) are reproduced last.
 """
 
@@ -103,6 +104,38 @@ def test_replica_counts(R):
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="lds")
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="global")
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="coop")
+
+
+def _degenerate_model(kind):
+    """Instances at the edge of what the boundary takes (n = 0 is refused, tests/test_lib_abi.py):
+    one spin, spins with no couplings at all (nnz = 0: every row empty, the first chunk all
+    padding), an all-zero model (every move has zero energy change) and one coupled pair beside
+    isolated spins."""
+    rng = np.random.default_rng(5)
+    if kind == "one_spin_field":
+        return IsingModel(np.zeros((1, 1)), np.array([2.0]))
+    if kind == "no_couplings_float":
+        return IsingModel(np.zeros((7, 7)), rng.normal(size=7))
+    if kind == "no_couplings_int":
+        return IsingModel(np.zeros((9, 9)), rng.integers(-3, 4, size=9).astype(float))
+    if kind == "all_zero":
+        return IsingModel(np.zeros((5, 5)), np.zeros(5))
+    J = np.zeros((5, 5))   # one_pair
+    J[1, 3] = J[3, 1] = -2.0
+    return IsingModel(J, np.zeros(5))
+
+
+@pytest.mark.parametrize("engine", ["lds", "global", "coop", "split"])
+@pytest.mark.parametrize("kind", ["one_spin_field", "no_couplings_float", "no_couplings_int",
+                                  "all_zero", "one_pair"])
+@pytest.mark.parametrize("R", [3, 40])
+def test_degenerate_instances(kind, engine, R):
+    _announce(f"degenerate {kind} {engine} R={R}")
+    model = _degenerate_model(kind)
+    g, _, _ = compare(model, R=R, n_rounds=12, S=2, seed=31, engine=engine)
+    # closed form: each uncoupled spin follows its field, the pair takes its coupling
+    assert g.best_E == pytest.approx(-np.abs(model.h).sum() - np.abs(model.J).sum() / 2,
+                                     rel=1e-6, abs=1e-9)
 
 
 def _degree_profile_model(kind):

@@ -146,11 +146,24 @@ def test_validation_happens_before_the_device():
     assert rc == _lib.EINVAL and "symmetric" in _lib.last_error(lib)
 
 
+def test_an_empty_problem_is_refused_before_the_device():
+    lib = _lib.load()
+    enc = gpu.encode_problem(IsingModel(np.zeros((3, 3)), np.array([1.0, -2.0, 0.0])))
+    enc.struct.n = 0
+    prm = gpu._params(2, 0.1, 5.0, 1, 1, 0, np.ones(2), "auto", 0, False, False)
+    buf = gpu._Buffers(3, 2, 1, False, False)
+    assert lib.drift_pt_run(enc.struct, prm, buf.struct) == _lib.EINVAL
+    assert "n must be >= 1" in _lib.last_error(lib)
+
+
 @pytest.mark.skipif(gpu.gpu_available(), reason="a gfx950 device is present")
 def test_valid_problem_without_device_reports_not_built():
     J = np.zeros((4, 4))
     J[0, 1] = J[1, 0] = 1.0
     rc, msg = _call_run(IsingModel(J, np.zeros(4)))
+    assert rc == _lib.ENODEV and "not built" in msg
+    # no couplings at all (fields only) passes validation too
+    rc, msg = _call_run(IsingModel(np.zeros((3, 3)), np.array([1.0, -2.0, 0.0])))
     assert rc == _lib.ENODEV and "not built" in msg
 
 
