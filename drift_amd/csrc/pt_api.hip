@@ -985,19 +985,32 @@ bool wants_cl(const std::vector<HostInst>& hs) {
     return !hs.empty();
 }
 
+// Fewest slots per group (a power of two) that keep an instance's groups within one per compute
+// unit: every group of an instance is resident at once, and one workgroup per compute unit
+// always is.
+int split_rg_min(const HostInst& H) {
+    int rg = 1;
+    while (rg < 64 && (H.R + rg - 1) / rg > device_cus()) rg <<= 1;
+    return rg;
+}
+
 // Slots per workgroup for a set of instances run together: the requested size where one is
 // given, else the largest common cap c (a power of two) under which the groups number at least
-// two per compute unit, each instance taking min(c, its largest fitting size); c = 1 when no cap
-// reaches that.  Larger groups share each position's edge records among more lanes; more groups
-// keep more compute units busy.
+// two per compute unit, each instance taking min(max(c, split_rg_min), its largest fitting size);
+// c = 1 when no cap reaches that.  Larger groups share each position's edge records among more
+// lanes; more groups keep more compute units busy.
 std::vector<int> split_rgs(const std::vector<HostInst>& hs, bool cl = false) {
     std::vector<int> out(hs.size(), 1);
     const int64_t want = 2 * (int64_t)device_cus();
+    auto pick = [&](const HostInst& H, int c) {
+        return H.rg_req > 0 ? H.rg_req
+                            : std::max(1, std::min(std::max(c, split_rg_min(H)), split_rg_max(H, cl)));
+    };
     int cap = 1;
     for (int c = 64; c >= 1; c >>= 1) {
         int64_t groups = 0;
         for (const HostInst& H : hs) {
-            const int rg = H.rg_req > 0 ? H.rg_req : std::max(1, std::min(c, split_rg_max(H, cl)));
+            const int rg = pick(H, c);
             groups += (H.R + rg - 1) / rg;
         }
         if (groups >= want) {
@@ -1005,8 +1018,7 @@ std::vector<int> split_rgs(const std::vector<HostInst>& hs, bool cl = false) {
             break;
         }
     }
-    for (size_t i = 0; i < hs.size(); ++i)
-        out[i] = hs[i].rg_req > 0 ? hs[i].rg_req : std::max(1, std::min(cap, split_rg_max(hs[i], cl)));
+    for (size_t i = 0; i < hs.size(); ++i) out[i] = pick(hs[i], cap);
     return out;
 }
 

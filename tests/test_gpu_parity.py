@@ -106,6 +106,18 @@ def test_replica_counts(R):
     compare(model, R=R, n_rounds=30, S=2, seed=R + 100, T_min=0.2, T_max=3.0, engine="coop")
 
 
+@pytest.mark.parametrize("engine", ["lds", "global", "coop", "split"])
+def test_maximum_replica_count(engine):
+    """R = 1024, the most the boundary takes (2000 is refused, test_engine_errors_are_reported):
+    1024-thread workgroups on the LDS, CSR and coop engines, 16 groups of 64 slots on the split
+    engine."""
+    _announce(f"R=1024 {engine}")
+    model = maxcut_ising(random_graph(24, p=0.3, seed=6))
+    g, _, _ = compare(model, R=1024, n_rounds=6, S=1, seed=77, T_min=0.1, T_max=5.0,
+                      engine=engine)
+    assert g.engine == engine
+
+
 def _degenerate_model(kind):
     """Instances at the edge of what the boundary takes (n = 0 is refused, tests/test_lib_abi.py):
     one spin, spins with no couplings at all (nnz = 0: every row empty, the first chunk all
