@@ -114,6 +114,24 @@ def test_degenerate_instances_in_a_batch_equal_oracle():
                                          rel=1e-6, abs=1e-9)
 
 
+def test_maximum_replica_count_batch_equals_oracle():
+    """R = 1024 in a batch: two instances too large for one workgroup's LDS at that R go to the
+    split engine (each instance's groups no more than the compute units, so they are resident
+    together), one small instance to the LDS engine; each equals the oracle."""
+    _announce("batch at R = 1024")
+    models = [random_regular_gaussian(300, seed=31), random_regular_gaussian(250, seed=32),
+              random_regular_gaussian(64, seed=33)]
+    R, n_rounds, S = 1024, 6, 1
+    seeds = [90, 91, 92]
+    res = gpu.parallel_tempering_gpu_batch(models, n_replicas=R, T_min=0.2, T_max=5.0,
+                                           n_rounds=n_rounds, sweeps_per_round=S, seeds=seeds,
+                                           return_final=True, record_history=True)
+    assert [r.engine for r in res] == ["split", "split", "lds"]
+    beta = ladder_betas(geometric_ladder(R, 0.2, 5.0))
+    for model, seed, r in zip(models, seeds, res):
+        _oracle_equal(model, r, seed=seed, R=R, n_rounds=n_rounds, S=S, beta=beta)
+
+
 def test_full_c4_batch_properties():
     """The C4 batch itself: 4096 random 3-regular instances, N uniform over even values in
     [64, 2048], J ~ N(0,1) and h ~ N(0, 0.1^2) in float32, R = 16, beta 0.2 .. 5.0, one
