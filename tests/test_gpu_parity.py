@@ -262,6 +262,37 @@ def test_zero_sweeps_per_round_only_exchanges(engine):
     assert g.best_round == 0
 
 
+@pytest.mark.parametrize("engine", ["lds", "global", "lattice", "slab", "coop", "pslab", "split"])
+def test_zero_rounds_report_the_initial_states(engine):
+    """n_rounds = 0: the best of the initial states (round 0), with an empty history and trace,
+    as the oracle's (the reference's round loop runs zero times, cuda/ising_pt.cu:213-222)."""
+    _announce(f"n_rounds=0 {engine}")
+    model = cubic_pmj_glass(8, 2)
+    g = compare(model, R=8, n_rounds=0, S=2, seed=3, T_min=0.5, T_max=2.0, engine=engine)[0]
+    assert g.best_round == 0 and g.best_history.size == 0 and g.engine == engine
+
+
+def test_zero_rounds_on_the_neighbour_synchronised_slab():
+    """The same on the bit-sliced slab kernel that runs without per-round records (L = 32)."""
+    _announce("n_rounds=0 pslab, neighbour-synchronised")
+    model = cubic_pmj_glass(32, 4)
+    R = 8
+    enc = gpu.encode_problem(model)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    beta = ladder_betas(geometric_ladder(R, 0.5, 2.0))
+    g = gpu.parallel_tempering_gpu(model, n_replicas=R, n_rounds=0, sweeps_per_round=2, seed=3,
+                                   betas=beta, engine="pslab", coloring=(cptr, cspins),
+                                   record_history=False, return_final=True)
+    o = oracle.pt_run_encoded(enc, (cptr, cspins), seed=3, R=R, n_rounds=0, sweeps_per_round=2,
+                              beta=beta, n_threads=THREADS)
+    assert g.best_E_units == o["best_units"] and g.best_round == o["best_round"] == 0
+    assert g.best_rung == o["best_rung"]
+    assert np.array_equal(g.best_s.astype(np.int8), o["best_s"])
+    assert np.array_equal(g.final_spins, o["final_spins"])
+    assert np.array_equal(g.final_E, o["final_units"])
+    assert np.array_equal(g.rung_of, o["rung_of"])
+
+
 def test_lattice_engine_rejects_non_lattices():
     _announce("lattice rejects a random graph")
     with pytest.raises(RuntimeError, match="lattice"):
