@@ -156,6 +156,37 @@ def test_an_empty_problem_is_refused_before_the_device():
     assert "n must be >= 1" in _lib.last_error(lib)
 
 
+def test_ladder_and_range_refusals_before_the_device():
+    lib = _lib.load()
+    J = np.zeros((4, 4))
+    J[0, 1] = J[1, 0] = 1.0
+    J[2, 3] = J[3, 2] = -1.0
+    buf = gpu._Buffers(4, 4, 1, False, False)
+
+    def run(model, beta=None, T=(0.1, 5.0), target=None):
+        prm = gpu._params(4, T[0], T[1], 1, 1, 0, beta, "auto", 0, False, False, target=target)
+        rc = lib.drift_pt_run(gpu.encode_problem(model).struct, prm, buf.struct)
+        return rc, _lib.last_error(lib)
+
+    model = IsingModel(J, np.zeros(4))
+    for bad in ([1.0, 2.0, 0.0, 3.0], [1.0, -2.0, 2.0, 3.0], [1.0, np.inf, 2.0, 3.0],
+                [np.nan, 1.0, 2.0, 3.0]):
+        rc, msg = run(model, beta=np.array(bad))
+        assert rc == _lib.EINVAL and "positive and finite" in msg
+    rc, msg = run(model, T=(0.0, 5.0))
+    assert rc == _lib.EINVAL and "T_min and T_max" in msg
+    rc, msg = run(model, target=float("nan"))
+    assert rc == _lib.EINVAL and "NaN" in msg
+    # float couplings: beta below 2^100
+    rc, msg = run(IsingModel(J * 0.5 + J * 2.0 ** -20, np.zeros(4)),
+                  beta=np.array([1.0, 2.0, 3.0, 2.0 ** 100]))
+    assert rc == _lib.ERANGE and "2^100" in msg
+    # integer couplings of 2^23 with beta 1e-12: every field up to 2^23 has a nonzero threshold,
+    # 4 rungs x 2^23 entries, more than the 2^24 the table takes
+    rc, msg = run(IsingModel(J * 2.0 ** 23, np.zeros(4)), beta=np.full(4, 1e-12))
+    assert rc == _lib.ERANGE and "threshold table" in msg
+
+
 @pytest.mark.skipif(gpu.gpu_available(), reason="a gfx950 device is present")
 def test_valid_problem_without_device_reports_not_built():
     J = np.zeros((4, 4))
