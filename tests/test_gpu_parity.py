@@ -5,10 +5,11 @@ libdriftpt.so) and on the oracle (oracle/pt_oracle.c) and compares spins, energi
 decisions, the rung permutation and the best state exactly.  Covered: integer couplings (int8,
 int32, half-integer QUBO scale, fields), float couplings, both engines, several tile sizes, replica
 counts that are not powers of two, R = 1 and 2, degenerate instances (one spin, no couplings at
-all, an all-zero model, one pair beside isolated spins), the 2-D ferromagnet of config C1 (L=4
-and L=16), the first 10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), a batch of uneven float
-instances in one persistent launch, every first-chunk width of the padded edge records (with rows
-longer than the chunk), and both barrier energy paths of the bit-sliced persistent slab engine.
+all, an all-zero model, one pair beside isolated spins), R = 1024, zero rounds, caller colourings
+(empty classes, one spin per class), the 2-D ferromagnet of config C1 (L=4 and L=16), the first
+10^3 sweeps of config C2 (3-D +/-J, L=32, R=64), a batch of uneven float instances in one
+persistent launch, every first-chunk width of the padded edge records (with rows longer than the
+chunk), and both barrier energy paths of the bit-sliced persistent slab engine.
 Reference GPU tests (tests/test_gpu.py:45-67) are reproduced last.
 """
 
@@ -40,9 +41,12 @@ def _announce(name):
 
 
 def compare(model, *, R, n_rounds, S, seed, T_min=0.1, T_max=5.0, beta=None, engine="auto",
-            block=0, force_float=False, graph=True):
+            block=0, force_float=False, graph=True, coloring=None):
     enc = gpu.encode_problem(model, force_float=force_float)
-    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    if coloring is None:
+        k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    else:
+        cptr, cspins = coloring
     if beta is None:
         beta = ladder_betas(geometric_ladder(R, T_min, T_max))
     g = gpu.parallel_tempering_gpu(
@@ -148,6 +152,28 @@ def test_degenerate_instances(kind, engine, R):
     # closed form: each uncoupled spin follows its field, the pair takes its coupling
     assert g.best_E == pytest.approx(-np.abs(model.h).sum() - np.abs(model.J).sum() / 2,
                                      rel=1e-6, abs=1e-9)
+
+
+def _colouring(model, kind):
+    """Colourings a caller may pass besides the greedy one: with empty classes (at the start, in
+    the middle and at the end) or one spin per class."""
+    enc = gpu.encode_problem(model)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    cptr = np.asarray(cptr, dtype=np.int64)
+    if kind == "empty_classes":
+        mid = cptr[: k // 2 + 1]
+        return (np.concatenate([[0], mid, [mid[-1]], cptr[k // 2 + 1:], [cptr[-1]]]),
+                np.asarray(cspins))
+    return np.arange(enc.n + 1), np.random.default_rng(3).permutation(enc.n)   # one per class
+
+
+@pytest.mark.parametrize("engine", ["lds", "global", "coop", "split"])
+@pytest.mark.parametrize("kind", ["empty_classes", "one_per_class"])
+def test_caller_colourings(kind, engine):
+    _announce(f"colouring {kind} {engine}")
+    model = maxcut_ising(random_graph(30, p=0.3, seed=12))
+    compare(model, R=12, n_rounds=15, S=2, seed=41, engine=engine,
+            coloring=_colouring(model, kind))
 
 
 def _degree_profile_model(kind):
