@@ -10,14 +10,15 @@ history and trace).
 Colour classes of at most two positions per thread are swept whole by every part instead of
 traded; the small instances here run 64-thread workgroups so that most of their classes are
 traded, and one case at 128 threads trades the three largest classes of seven.  Cases: float,
-int8 and int32 couplings; parts counts
-that do and do not divide a class's words; groups of one slot (the packed spin-word path) and of
-several (the strided path); R above 64 (the workgroup round end); padding slots in the last
-group; one replica; the two-hop energy gather (groups x parts x R >= 1024); a stop at a target;
-the Erdos-Renyi workload of N = 10^5 and R = 32 on 256 workgroups; a batch whose groups fill
-the chip only in parts; the default engine's choice of parts; 70 000 rounds past the round tags'
-wrap, against the per-launch engine; the refusals; and a wait forced to give up from the host.  The reference runs each replica in one warp with no exchange
-between blocks (cuda/ising_pt.cu:86-146); this engine has no counterpart there.
+int8 and int32 couplings; parts counts that do and do not divide a class's words; groups of one
+slot (the packed spin-word path) and of several (the strided path); R above 64 (the workgroup
+round end); padding slots in the last group; one replica; groups x parts x R of 1024; a stop at
+a target; the Erdos-Renyi workload of N = 10^5 and R = 32 on 256 workgroups; a batch whose
+groups fill the chip only in parts; a caller colouring with empty classes; the default engine's
+choice of parts; 70 000 rounds past the round tags' wrap, against the per-launch engine; the
+refusals; and a wait forced to give up from the host.  The reference runs each replica in one
+warp with no exchange between blocks (cuda/ising_pt.cu:86-146); this engine has no counterpart
+there.
 """
 
 import os
@@ -67,9 +68,12 @@ def _check(g, o, model):
 
 
 def _run_pair(model, *, R, n_rounds, S, seed, rg, parts, T_min=0.2, T_max=5.0, target=None,
-              trace=True, block=64):
+              trace=True, block=64, coloring=None):
     enc = gpu.encode_problem(model)
-    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    if coloring is None:
+        k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    else:
+        cptr, cspins = coloring
     beta = ladder_betas(geometric_ladder(R, T_min, T_max))
     g = gpu.parallel_tempering_gpu(
         model, n_replicas=R, n_rounds=n_rounds, sweeps_per_round=S, seed=seed, betas=beta,
@@ -95,13 +99,30 @@ def test_float_parts_equal_oracle(rg, parts):
     assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
 
 
+def test_caller_colouring_with_empty_classes_in_parts():
+    """Empty colour classes first, in the middle and last: parts skip them in the sweep, the
+    trade and the round's energy ride (which goes with the last traded class)."""
+    _announce("float 3-regular N=600 R=16, 4 slots x 4 parts, empty classes")
+    model = random_regular_gaussian(600, seed=41)
+    enc = gpu.encode_problem(model)
+    k, cptr, cspins = oracle.greedy_coloring(enc.row_ptr, enc.col_idx)
+    cptr = [int(x) for x in cptr]
+    m = k // 2
+    cptr = [0] + cptr[: m + 1] + [cptr[m]] + cptr[m + 1:] + [cptr[-1]]
+    g, o, _ = _run_pair(model, R=16, n_rounds=20, S=2, seed=7, rg=4, parts=4,
+                        coloring=(np.array(cptr), cspins))
+    _check(g, o, model)
+    assert np.array_equal(g.rung_energy_trace, o["rung_trace"])
+
+
 @pytest.mark.parametrize("R,rg,parts,block", [(100, 8, 2, 64), (33, 16, 3, 64), (1, 1, 4, 64),
                                               (32, 1, 8, 64), (32, 1, 8, 128)])
 def test_int_parts_equal_oracle(R, rg, parts, block):
     """Integer couplings (ER, weights +-1..+-3, N = 1500): R = 100 takes the workgroup round end
     and leaves padding slots in the last group, as R = 33 does; one replica has no exchange;
-    32 groups of one slot in 8 parts gather energies in two hops; at 128 threads the three
-    largest classes are traded and the others swept by every part."""
+    32 one-slot groups in 8 parts (256 workgroups) each read the other 31 slots' energies from
+    the parts of their own index; at 128 threads the three largest classes are traded and the
+    others swept by every part."""
     _announce(f"int ER N=1500 R={R}, {rg} slots x {parts} parts, {block} threads")
     model = er_sparse_weighted(1500, 6.0, 12)
     g, o, enc = _run_pair(model, R=R, n_rounds=10, S=2, seed=3, rg=rg, parts=parts, T_min=0.1,
