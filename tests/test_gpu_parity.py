@@ -154,6 +154,16 @@ def test_degenerate_instances(kind, engine, R):
                                      rel=1e-6, abs=1e-9)
 
 
+@pytest.mark.parametrize("seed", [2 ** 32, 2 ** 63, 2 ** 64 - 1])
+def test_seeds_over_the_whole_64_bit_range(seed):
+    """The seed keys Philox by its two 32-bit halves; seeds with the high half set or every bit
+    set give the oracle's streams."""
+    _announce(f"seed {seed}")
+    model = maxcut_ising(random_graph(20, p=0.3, seed=2))
+    for engine in ("lds", "global"):
+        compare(model, R=8, n_rounds=10, S=2, seed=seed, engine=engine)
+
+
 def _colouring(model, kind):
     """Colourings a caller may pass besides the greedy one: with empty classes (at the start, in
     the middle and at the end) or one spin per class."""
