@@ -164,6 +164,29 @@ def test_seeds_over_the_whole_64_bit_range(seed):
         compare(model, R=8, n_rounds=10, S=2, seed=seed, engine=engine)
 
 
+def test_int8_couplings_past_the_24_bit_position_limit():
+    """A ring of 2^24 + 2 spins with +-1 couplings: int8 couplings pack a 24-bit position beside
+    the weight, so an instance this large takes the two-word integer records instead; the run
+    still equals the oracle's, which reads the CSR as given."""
+    from drift_amd.ising import SparseIsingModel
+
+    n = 2 ** 24 + 2
+    _announce(f"ring of {n} spins, int8 couplings")
+    i = np.arange(n, dtype=np.int64)
+    col = np.empty(2 * n, dtype=np.int64)
+    col[0::2] = (i - 1) % n
+    col[1::2] = (i + 1) % n
+    w_edge = np.where(np.random.default_rng(9).random(n) < 0.5, -1.0, 1.0)   # bond (i, i + 1)
+    w = np.empty(2 * n)
+    w[1::2] = w_edge
+    w[0::2] = np.roll(w_edge, 1)
+    model = SparseIsingModel(np.arange(0, 2 * n + 1, 2, dtype=np.int64), col, w, np.zeros(n),
+                             check=False)
+    g, _, enc = compare(model, R=2, n_rounds=2, S=1, seed=4, T_min=0.5, T_max=2.0,
+                        engine="global")
+    assert enc.weight_type == 1
+
+
 def _colouring(model, kind):
     """Colourings a caller may pass besides the greedy one: with empty classes (at the start, in
     the middle and at the end) or one spin per class."""
