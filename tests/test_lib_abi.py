@@ -156,6 +156,13 @@ def test_an_empty_problem_is_refused_before_the_device():
     assert "n must be >= 1" in _lib.last_error(lib)
 
 
+def test_a_batch_of_no_instances_is_refused_before_the_device():
+    lib = _lib.load()
+    for entry in (lib.drift_pt_run_batch, lib.drift_pt_run_batch_noalloc):
+        assert entry(None, None, 0, None) == _lib.EINVAL
+        assert "bad batch arguments" in _lib.last_error(lib)
+
+
 def test_ladder_and_range_refusals_before_the_device():
     lib = _lib.load()
     J = np.zeros((4, 4))
