@@ -87,6 +87,28 @@ def test_stop_at_target_equals_a_run_of_that_many_rounds(case):
     assert g.rounds_run == 0 and g.best_round == 0
 
 
+def test_autotune_with_a_target_keeps_to_engines_that_stop():
+    """A tuned run with a target: the tuner times only the engines that can stop at a target
+    (lds, split, global, lattice), under a shape class apart from the untargeted one, and the
+    result is the oracle's run of the rounds it took."""
+    gpu.autotune_clear()
+    model = cubic_pmj_glass(16, 2)
+    R, S = 16, 1
+    beta = ladder_betas(geometric_ladder(R, 0.2, 4.0))
+    kw = dict(n_replicas=R, betas=beta, sweeps_per_round=S, seed=13, return_final=True,
+              record_history=True, autotune=True)
+    free = gpu.parallel_tempering_gpu(model, n_rounds=300, **kw)
+    hist = free.best_history
+    target = float(hist[len(hist) // 3])
+    stop = int(np.nonzero(hist <= target)[0][0]) + 1
+    _announce(f"tuned with a target: untargeted choice {free.engine}, stop after round {stop}")
+    g = gpu.parallel_tempering_gpu(model, n_rounds=300, target_energy=target, **kw)
+    assert g.engine in ("lds", "split", "global", "lattice") and g.rounds_run == stop
+    _equal(g, _oracle(model, seed=13, R=R, n_rounds=stop, S=S, beta=beta))
+    # the untargeted choice is still the cached one
+    assert gpu.parallel_tempering_gpu(model, n_rounds=20, **kw).engine == free.engine
+
+
 def test_persistent_engines_refuse_a_target():
     _announce("pslab and coop refuse a target")
     for engine, model in (("pslab", cubic_pmj_glass(32, 1)), ("coop", er_sparse_weighted(3000, 6.0, 1))):
