@@ -569,7 +569,21 @@ def bench_throughput(args) -> int:
     line = headline(bench_c4(args.steps, args.warmup, args.repeats, cpu, args.instances,
                              args.autotune))
     subs = {}
-    for name in ("c2", "c3", "published", "single3"):
+    failed = []
+
+    def sub_result(name, run):
+        # a sub-result that raises is recorded and ends the GPU work; the headline still prints
+        if failed:
+            subs[name] = {"skipped": f"not run after the failure of {failed[0]}"}
+            return
+        try:
+            subs[name] = run()
+        except Exception as e:
+            failed.append(name)
+            subs[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"sub-result {name} failed: {type(e).__name__}: {e}")
+
+    def single(name):
         steps = args.sub_steps or SUB_DEFAULT_STEPS[name]
         sub = bench_single(name, steps, min(args.warmup, steps) or 1, args.repeats,
                            cpu, args.plain_launch, autotune=args.autotune)
@@ -577,16 +591,21 @@ def bench_throughput(args) -> int:
             sub["context"] = {"reference_rtx5060ti_attempts_per_ns": PUBLISHED_RTX5060TI,
                               "source": "docs/results/PHASE14-results.md:140-142",
                               "note": "a different card; context, not a target"}
-        subs[name] = sub
-    subs["c5_batch"] = bench_c5_batch(args.sub_steps or SUB_DEFAULT_STEPS["c5_batch"],
-                                      min(args.warmup, 20) or 1, args.repeats, cpu,
-                                      autotune=args.autotune, plain=args.plain_launch)
+        return sub
+
+    for name in ("c2", "c3", "published", "single3"):
+        sub_result(name, lambda name=name: single(name))
+    sub_result("c5_batch", lambda: bench_c5_batch(args.sub_steps or SUB_DEFAULT_STEPS["c5_batch"],
+                                                  min(args.warmup, 20) or 1, args.repeats, cpu,
+                                                  autotune=args.autotune,
+                                                  plain=args.plain_launch))
     if not args.no_time_to_target:
-        subs["time_to_target"] = bench_ttt_sample()
+        sub_result("time_to_target", bench_ttt_sample)
     line["sub_results"] = subs
-    line["lds_probe"] = {k: {"GBs": round(v["GBs"], 1),
-                             "wave_reads_per_s": float(f"{v['wave_reads_per_s']:.4g}")}
-                         for k, v in lds_probe().items()}
+    if not failed:
+        line["lds_probe"] = {k: {"GBs": round(v["GBs"], 1),
+                                 "wave_reads_per_s": float(f"{v['wave_reads_per_s']:.4g}")}
+                             for k, v in lds_probe().items()}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -80,6 +80,41 @@ def test_c5_batch_is_a_sub_result_with_its_own_steps():
     assert "65536" in bench.C5_BATCH_DESC and "split engine" in bench.C5_BATCH_DESC
 
 
+def test_a_failing_sub_result_keeps_the_headline(monkeypatch, capsys):
+    """The default line: a sub-result that raises is recorded with its error, the later ones are
+    not started, and the C4 headline still prints."""
+    import argparse
+    import json
+
+    fake = {"value": 500.0, "steps": 20, "warmup": 5, "ms_per_step": 0.1, "dtype": "f32",
+            "config": {"workload": "stand-in"}, "roofline": {}, "cpu_baseline": {}, "timing": {}}
+    calls = []
+
+    def single(name, *a, **kw):
+        calls.append(name)
+        if name == "c3":
+            raise RuntimeError("GPU engine failed (status -1): stand-in")
+        return dict(fake, value=1.0)
+
+    def must_not_run(*a, **kw):
+        raise AssertionError("started after a failed sub-result")
+
+    monkeypatch.setattr(bench, "bench_c4", lambda *a, **kw: dict(fake))
+    monkeypatch.setattr(bench, "bench_single", single)
+    monkeypatch.setattr(bench, "bench_c5_batch", must_not_run)
+    monkeypatch.setattr(bench, "bench_ttt_sample", must_not_run)
+    monkeypatch.setattr(bench, "lds_probe", must_not_run)
+    args = argparse.Namespace(no_cpu_baseline=True, cpu_budget=0.0, workload="all", steps=20,
+                              warmup=5, repeats=1, instances=8, autotune=False, sub_steps=0,
+                              plain_launch=False, no_time_to_target=False)
+    assert bench.bench_throughput(args) == 0
+    line = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    assert line["value"] == 500.0 and calls == ["c2", "c3"]
+    subs = line["sub_results"]
+    assert subs["c2"]["value"] == 1.0 and "stand-in" in subs["c3"]["error"]
+    assert all("skipped" in subs[k] for k in ("published", "single3", "c5_batch", "time_to_target"))
+
+
 def test_time_to_target_parts_merge(tmp_path):
     """--stage eval-merge puts per-family parts of the time-to-target evaluation together: the
     summary counts every part's rows, adds their wall times, and refuses mismatched caps."""
